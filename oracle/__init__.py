@@ -1,0 +1,192 @@
+"""ctypes bindings for the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+The oracle is a scalar CPU restatement of the reference FAZ hot path
+(interpolation predictor, linear quantizer, Huffman coder, engine buffer and
+stream framing).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+CPU-baseline leg import it, always as the checker, never as the thing shipped.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+
+class InterpParams(ctypes.Structure):
+    """Mirror of oracle_interp_params (faz_oracle.h)."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("eb", ctypes.c_double),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("max_step", ctypes.c_size_t),
+        ("block_size", ctypes.c_int),
+        ("radius", ctypes.c_int),
+        ("levelwise", ctypes.c_int),
+        ("ops", ctypes.c_uint8 * 32),
+        ("dirs", ctypes.c_uint8 * 32),
+    ]
+
+
+def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0,
+                block_size=32, radius=32768, ops=(), dirs=()):
+    p = InterpParams()
+    p.N = len(dims)
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    p.interp_algo = int(algo)
+    p.direction = int(direction)
+    p.alpha = float(alpha)
+    p.beta = float(beta)
+    p.max_step = int(max_step)
+    p.block_size = int(block_size)
+    p.radius = int(radius)
+    p.levelwise = len(ops)
+    for i, (o, d) in enumerate(zip(ops, dirs)):
+        p.ops[i] = int(o)
+        p.dirs[i] = int(d)
+    return p
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.oracle_interp_compress.argtypes = [
+            P(InterpParams), ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_void_p, P(ctypes.c_size_t), ctypes.c_void_p, P(ctypes.c_size_t),
+            P(ctypes.c_double), ctypes.c_void_p, P(ctypes.c_int)]
+        L.oracle_interp_raw.argtypes = [P(InterpParams), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_size_t, P(ctypes.c_size_t)]
+        L.oracle_interp_decompress_raw.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                                   ctypes.c_void_p, ctypes.c_size_t]
+        L.oracle_huffman_encode.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                            ctypes.c_size_t, P(ctypes.c_size_t)]
+        L.oracle_huffman_decode.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                            ctypes.c_size_t, P(ctypes.c_size_t)]
+        L.oracle_compress_stream.argtypes = [P(InterpParams), ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                             P(ctypes.c_size_t)]
+        L.oracle_decompress_stream.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                               ctypes.c_size_t]
+        L.oracle_range.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_float),
+                                   P(ctypes.c_float)]
+        L.oracle_interp_seq_index.argtypes = [P(InterpParams), ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def interp_compress(data, params, tuning=0, start=0, end=0):
+    """Returns dict(codes, unpred, recon, loss, level_counts, levels)."""
+    recon = np.ascontiguousarray(data, dtype=np.float32).ravel().copy()
+    n = recon.size
+    codes = np.zeros(n + 1, np.int32)
+    unpred = np.zeros(n + 1, np.float32)
+    nc = ctypes.c_size_t()
+    nu = ctypes.c_size_t()
+    loss = ctypes.c_double()
+    lc = np.zeros(64, np.uint64)
+    nl = ctypes.c_int()
+    rc = lib().oracle_interp_compress(ctypes.byref(params), _ptr(recon), tuning, start, end,
+                                      _ptr(codes), ctypes.byref(nc), _ptr(unpred), ctypes.byref(nu),
+                                      ctypes.byref(loss), _ptr(lc), ctypes.byref(nl))
+    assert rc == 0, rc
+    return dict(codes=codes[:nc.value].copy(), unpred=unpred[:nu.value].copy(), recon=recon,
+                loss=loss.value, level_counts=lc[:max(nl.value, 0)].copy(), levels=nl.value)
+
+
+def interp_raw(data, params):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().oracle_interp_raw(ctypes.byref(params), _ptr(data), _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def interp_decompress_raw(raw, dims):
+    buf = np.frombuffer(raw, np.uint8).copy()
+    num = int(np.prod(dims))
+    out = np.zeros(num, np.float32)
+    rc = lib().oracle_interp_decompress_raw(_ptr(buf), buf.size, len(dims), _ptr(out), num)
+    assert rc == 0, rc
+    return out
+
+
+def huffman_encode(codes):
+    codes = np.ascontiguousarray(codes, dtype=np.int32)
+    cap = 64 * 1024 + codes.size * 16 + 16 * 200000
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().oracle_huffman_encode(_ptr(codes), codes.size, _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def huffman_decode(buf, num):
+    b = np.frombuffer(buf, np.uint8).copy()
+    out = np.zeros(num, np.int32)
+    used = ctypes.c_size_t()
+    rc = lib().oracle_huffman_decode(_ptr(b), b.size, _ptr(out), num, ctypes.byref(used))
+    assert rc == 0, rc
+    return out, used.value
+
+
+def compress_stream(data, params, eb, rel=True):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().oracle_compress_stream(ctypes.byref(params), 1 if rel else 0, float(eb), _ptr(data),
+                                      _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def decompress_stream(stream, num):
+    b = np.frombuffer(stream, np.uint8).copy()
+    out = np.zeros(num, np.float32)
+    rc = lib().oracle_decompress_stream(_ptr(b), b.size, _ptr(out), num)
+    assert rc == 0, rc
+    return out
+
+
+def data_range(data):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    mn = ctypes.c_float()
+    mx = ctypes.c_float()
+    lib().oracle_range(_ptr(data), data.size, ctypes.byref(mn), ctypes.byref(mx))
+    return mn.value, mx.value
+
+
+def seq_index(params):
+    num = 1
+    for i in range(params.N):
+        num *= params.dims[i]
+    seq = np.zeros(num, np.int64)
+    lib().oracle_interp_seq_index(ctypes.byref(params), _ptr(seq))
+    return seq

@@ -1,0 +1,1038 @@
+// CPU oracle for the FAZ compression hot path — TEST INFRASTRUCTURE ONLY.
+//
+// A scalar restatement of the reference algorithm (JLiu-1/FAZ, namespace QoZ),
+// following its operation order exactly so that results are bit-identical.
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it,
+// and only as the checker; the product in faz_amd/ never links or calls it.
+//
+// Built with -ffp-contract=off: the reference is built for x86-64 without FMA,
+// so every float/double operation below rounds individually.
+//
+// Parity pins (tests/test_oracle_golden.py, tests/test_oracle_math.py):
+//   quantizer, interpolators, block/level/pass order, anchors, tuning modes,
+//   Huffman (ska::unordered_map order, heap, codes, tree bytes, bitstream),
+//   engine buffer, Config trailer and stream: golden vectors written by the
+//   reference engine compiled from its own sources (oracle/_ref/ref_harness)
+//   plus closed forms and invariants.
+#include "faz_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+extern "C" {
+// libzstd (system shared library); prototypes as published in zstd.h
+size_t ZSTD_compress(void *dst, size_t dstCapacity, const void *src, size_t srcSize, int compressionLevel);
+size_t ZSTD_decompress(void *dst, size_t dstCapacity, const void *src, size_t compressedSize);
+size_t ZSTD_compressBound(size_t srcSize);
+unsigned ZSTD_isError(size_t code);
+}
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// LinearQuantizer (include/QoZ/quantizer/IntegerQuantizer.hpp:14-257)
+// ---------------------------------------------------------------------------
+
+// (int) of a non-negative double as x86-64 cvttsd2si does it: values outside
+// the int range (and NaN) give INT_MIN.
+static inline int32_t x86_trunc_i32(double v) {
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return INT32_MIN;
+    return (int32_t) v;
+}
+
+struct Quantizer {
+    double eb = 1, recip = 1;
+    int radius = 32768;
+    std::vector<float> unpred;
+    size_t index = 0;  // decompression cursor
+
+    void set_eb(double e) {  // Q:28-31
+        eb = e;
+        recip = 1.0 / e;
+    }
+
+    // quantize_and_overwrite(T&, T, bool) with trimToZero == 0 (Q:69-108)
+    int quantize_and_overwrite(float &data, float pred, bool save_unpred = true) {
+        float diff = data - pred;
+        uint32_t qi = (uint32_t) x86_trunc_i32(std::fabs(diff) * recip) + 1u;
+        if ((int32_t) qi < radius * 2) {
+            int32_t q = ((int32_t) qi) >> 1;  // arithmetic shift
+            int32_t half = q;
+            uint32_t qs = (uint32_t) q << 1;
+            int32_t shifted;
+            if (diff < 0) {
+                qs = 0u - qs;
+                shifted = radius - half;
+            } else {
+                shifted = radius + half;
+            }
+            float dec = (float) ((double) pred + (double) (int32_t) qs * eb);
+            if (std::fabs(dec - data) > eb) {
+                if (save_unpred) unpred.push_back(data);
+                return 0;
+            }
+            data = dec;
+            return shifted;
+        }
+        if (save_unpred) unpred.push_back(data);
+        return 0;
+    }
+
+    // recover (Q:160-180)
+    float recover(float pred, int code) {
+        if (code) return (float) ((double) pred + (double) (2 * (code - radius)) * eb);
+        return unpred[index++];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Interpolators (include/QoZ/utils/Interpolators.hpp:9-36), evaluated in T=float
+// left to right; linear1 uses double constants.
+// ---------------------------------------------------------------------------
+static inline float interp_linear(float a, float b) { return (a + b) / 2; }
+static inline float interp_linear1(float a, float b) { return (float) (-0.5 * a + 1.5 * b); }
+static inline float interp_quad_1(float a, float b, float c) { return (3 * a + 6 * b - c) / 8; }
+static inline float interp_quad_2(float a, float b, float c) { return (-a + 6 * b + 3 * c) / 8; }
+static inline float interp_quad_3(float a, float b, float c) { return (3 * a - 10 * b + 15 * c) / 8; }
+static inline float interp_cubic(float a, float b, float c, float d) { return (-a + 9 * b + 9 * c - d) / 16; }
+
+// ---------------------------------------------------------------------------
+// SZInterpolationCompressor (include/QoZ/compressor/SZInterpolationCompressor.hpp)
+// ---------------------------------------------------------------------------
+struct InterpEngine {
+    int N = 3;
+    size_t dims[4] = {0, 0, 0, 0};
+    size_t offs[4] = {0, 0, 0, 0};
+    size_t num = 0;
+    int levels = -1;
+    bool anchor = false;
+    size_t max_step = 0;
+    unsigned block_size = 32;
+    int interp_algo = 1, direction = 0;
+    double alpha = -1, beta = -1;
+    std::vector<uint8_t> ops, dirs;
+    std::vector<std::array<int, 4>> perms;
+
+    Quantizer quantizer;
+    std::vector<int> codes;
+    std::vector<size_t> code_pos;  // flat index of each pushed code (seq-index probe)
+    size_t code_index = 0;         // decompression cursor
+    bool decompressing = false;
+    int tuning = 0;
+
+    void set_params(const oracle_interp_params *p) {
+        N = p->N;
+        for (int i = 0; i < N; i++) dims[i] = p->dims[i];
+        max_step = p->max_step;
+        block_size = (unsigned) p->block_size;
+        interp_algo = p->interp_algo;
+        direction = p->direction;
+        alpha = p->alpha;
+        beta = p->beta;
+        ops.assign(p->ops, p->ops + p->levelwise);
+        dirs.assign(p->dirs, p->dirs + p->levelwise);
+        quantizer.radius = p->radius;
+        quantizer.set_eb(p->eb);
+    }
+
+    // init (SIC:820-852)
+    void init() {
+        num = 1;
+        levels = -1;
+        for (int i = 0; i < N; i++) {
+            if (levels < std::ceil(std::log2((double) dims[i]))) levels = (unsigned) std::ceil(std::log2((double) dims[i]));
+            num *= dims[i];
+        }
+        anchor = false;
+        if (max_step > 0) {
+            anchor = true;
+            levels += 1;
+            int max_level = (unsigned) std::log2((double) max_step) + 1;
+            if (max_level <= levels) levels = max_level;
+        }
+        offs[N - 1] = 1;
+        for (int i = N - 2; i >= 0; i--) offs[i] = offs[i + 1] * dims[i + 1];
+        perms.clear();
+        std::array<int, 4> seq = {0, 1, 2, 3};
+        do {
+            perms.push_back(seq);
+        } while (std::next_permutation(seq.begin(), seq.begin() + N));
+    }
+
+    // level-wise error bound (SIC:386-406, 113-137)
+    void set_level_eb(double eb, unsigned level) {
+        if (alpha < 0) {
+            quantizer.set_eb(level >= 3 ? eb * 0.5 : eb);
+        } else if (alpha >= 1) {
+            double r = std::pow(alpha, level - 1);
+            if (r > beta) r = beta;
+            quantizer.set_eb(eb / r);
+        } else {
+            double r = 1 - (level - 1) * alpha;
+            if (r < beta) r = beta;
+            quantizer.set_eb(eb * r);
+        }
+    }
+
+    void level_op(unsigned level, int &interp, int &dir) const {  // SIC:407-422
+        int L = (int) ops.size();
+        if (L == 0) {
+            interp = interp_algo;
+            dir = direction;
+        } else if ((int) level - 1 < L) {
+            interp = ops[level - 1];
+            dir = dirs[level - 1];
+        } else {
+            interp = ops[L - 1];
+            dir = dirs[L - 1];
+        }
+    }
+
+    // one predicted point: quantize (SIC:924-938) / quantize_tuning (SIC:940-974) / recover (SIC:976-978)
+    double point(float *data, float *d, float pred) {
+        if (decompressing) {
+            *d = quantizer.recover(pred, codes[code_index++]);
+            return 0;
+        }
+        if (tuning == 0) {
+            codes.push_back(quantizer.quantize_and_overwrite(*d, pred));
+            code_pos.push_back((size_t) (d - data));
+            return 0;
+        }
+        if (tuning == 1) {
+            float orig = *d;
+            codes.push_back(quantizer.quantize_and_overwrite(*d, pred));
+            code_pos.push_back((size_t) (d - data));
+            return (*d - orig) * (*d - orig);
+        }
+        double pred_error = std::fabs(*d - pred);
+        quantizer.quantize_and_overwrite(*d, pred, false);
+        return pred_error;
+    }
+
+    // block_interpolation_1d (SIC:1306-1417)
+    double line(float *data, size_t begin, size_t end, size_t stride, bool linear) {
+        size_t n = (end - begin) / stride + 1;
+        if (n <= 1) return 0;
+        double err = 0;
+        size_t s3 = 3 * stride, s5 = 5 * stride;
+        if (linear || n < 5) {
+            for (size_t i = 1; i + 1 < n; i += 2) {
+                float *d = data + begin + i * stride;
+                err += point(data, d, interp_linear(*(d - stride), *(d + stride)));
+            }
+            if (n % 2 == 0) {
+                float *d = data + begin + (n - 1) * stride;
+                if (n < 4)
+                    err += point(data, d, *(d - stride));
+                else
+                    err += point(data, d, interp_linear1(*(d - s3), *(d - stride)));
+            }
+        } else {
+            float *d;
+            size_t i;
+            for (i = 3; i + 3 < n; i += 2) {
+                d = data + begin + i * stride;
+                err += point(data, d, interp_cubic(*(d - s3), *(d - stride), *(d + stride), *(d + s3)));
+            }
+            d = data + begin + stride;
+            err += point(data, d, interp_quad_1(*(d - stride), *(d + stride), *(d + s3)));
+            d = data + begin + i * stride;
+            err += point(data, d, interp_quad_2(*(d - s3), *(d - stride), *(d + stride)));
+            if (n % 2 == 0) {
+                d = data + begin + (n - 1) * stride;
+                err += point(data, d, interp_quad_3(*(d - s5), *(d - s3), *(d - stride)));
+            }
+        }
+        return err;
+    }
+
+    // block_interpolation<N> (SIC:3076-3442): pass P runs along axis p[P];
+    // axes p[Q<P] step by stride from (begin? begin+stride : 0), axes p[Q>P]
+    // step by 2*stride from (begin? begin+2*stride : 0); earlier axes outer.
+    double block(float *data, const size_t *begin, const size_t *end, bool linear, int dir, size_t stride) {
+        const std::array<int, 4> &p = perms[dir];
+        double err = 0;
+        for (int P = 0; P < N; P++) {
+            int other[3];
+            int no = 0;
+            for (int Q = 0; Q < N; Q++)
+                if (Q != P) other[no++] = Q;
+            size_t st[3], lo[3], hi[3], c[3];
+            for (int k = 0; k < no; k++) {
+                int Q = other[k];
+                int ax = p[Q];
+                st[k] = Q < P ? stride : 2 * stride;
+                lo[k] = begin[ax] ? begin[ax] + st[k] : 0;
+                hi[k] = end[ax];
+                c[k] = lo[k];
+            }
+            int ax = p[P];
+            // odometer over the outer coordinates (earlier permutation index outer)
+            bool empty = false;
+            for (int k = 0; k < no; k++)
+                if (lo[k] > hi[k]) empty = true;
+            if (empty) continue;
+            while (true) {
+                size_t off = begin[ax] * offs[ax];
+                for (int k = 0; k < no; k++) off += c[k] * offs[p[other[k]]];
+                err += line(data, off, off + (end[ax] - begin[ax]) * offs[ax], stride * offs[ax], linear);
+                int k = no - 1;
+                while (k >= 0) {
+                    c[k] += st[k];
+                    if (c[k] <= hi[k]) break;
+                    c[k] = lo[k];
+                    k--;
+                }
+                if (k < 0) break;
+            }
+        }
+        return err;
+    }
+
+    // one level: blocks in row-major raster of multi_dimensional_range (IT:361-453)
+    double level_pass(float *data, unsigned level) {
+        int interp, dir;
+        level_op(level, interp, dir);
+        size_t stride = (size_t) 1U << (level - 1);
+        size_t bs = (size_t) block_size * stride;
+        size_t nb[4], bi[4] = {0, 0, 0, 0};
+        size_t total = 1;
+        for (int i = 0; i < N; i++) {
+            nb[i] = (dims[i] - 1) / bs + 1;
+            total *= nb[i];
+        }
+        double err = 0;
+        for (size_t b = 0; b < total; b++) {
+            size_t begin[4], end[4];
+            for (int i = 0; i < N; i++) {
+                begin[i] = bi[i] * bs;
+                end[i] = begin[i] + bs;
+                if (end[i] > dims[i] - 1) end[i] = dims[i] - 1;
+            }
+            err += block(data, begin, end, interp == 0, dir, stride);
+            for (int i = N - 1; i >= 0; i--) {
+                if (++bi[i] < nb[i]) break;
+                bi[i] = 0;
+            }
+        }
+        return err;
+    }
+
+    // build_grid (SIC:854-900): only N == 2 and N == 3 have anchors
+    void build_grid(float *data) {
+        if (tuning > 1) return;
+        size_t s0 = max_step * (tuning == 1);
+        if (N == 2) {
+            for (size_t x = s0; x < dims[0]; x += max_step)
+                for (size_t y = s0; y < dims[1]; y += max_step) {
+                    quantizer.unpred.push_back(data[x * dims[1] + y]);
+                    codes.push_back(0);
+                    code_pos.push_back(x * dims[1] + y);
+                }
+        } else if (N == 3) {
+            for (size_t x = s0; x < dims[0]; x += max_step)
+                for (size_t y = s0; y < dims[1]; y += max_step)
+                    for (size_t z = s0; z < dims[2]; z += max_step) {
+                        size_t idx = x * offs[0] + y * offs[1] + z;
+                        quantizer.unpred.push_back(data[idx]);
+                        codes.push_back(0);
+                        code_pos.push_back(idx);
+                    }
+        }
+    }
+
+    // compress (SIC:328-476), returns the summed loss of the tuning modes
+    double compress(float *data, int tuning_, int start_level, int end_level, std::vector<size_t> &level_counts) {
+        tuning = tuning_;
+        init();
+        codes.clear();
+        code_pos.clear();
+        quantizer.unpred.clear();
+        double eb = quantizer.eb;
+        if (tuning) level_counts.assign(levels, 0);
+        if (start_level <= 0 || start_level > levels) start_level = levels;
+        if (end_level >= start_level || end_level < 0) end_level = 0;
+        if (!anchor) {
+            codes.push_back(quantizer.quantize_and_overwrite(*data, 0));
+            code_pos.push_back(0);
+        } else if (start_level == levels) {
+            if (tuning) level_counts[start_level - 1] = codes.size();
+            build_grid(data);
+            start_level--;
+        }
+        double err = 0;
+        for (unsigned level = start_level; level > (unsigned) end_level && level <= (unsigned) start_level; level--) {
+            set_level_eb(eb, level);
+            err += level_pass(data, level);
+            if (tuning) level_counts[level - 1] = codes.size();
+        }
+        quantizer.set_eb(eb);
+        return err;
+    }
+
+    // decompress level loop (SIC:94-196)
+    void decompress(float *data) {
+        decompressing = true;
+        code_index = 0;
+        quantizer.index = 0;
+        init();
+        double eb = quantizer.eb;
+        if (!anchor) {
+            *data = quantizer.recover(0, codes[code_index++]);
+        } else {
+            if (N == 2) {
+                for (size_t x = 0; x < dims[0]; x += max_step)
+                    for (size_t y = 0; y < dims[1]; y += max_step) {
+                        data[x * offs[0] + y] = quantizer.unpred[quantizer.index++];
+                        code_index++;
+                    }
+            } else if (N == 3) {
+                for (size_t x = 0; x < dims[0]; x += max_step)
+                    for (size_t y = 0; y < dims[1]; y += max_step)
+                        for (size_t z = 0; z < dims[2]; z += max_step) {
+                            data[x * offs[0] + y * offs[1] + z] = quantizer.unpred[quantizer.index++];
+                            code_index++;
+                        }
+            }
+            levels--;
+        }
+        for (unsigned level = levels; level > 0 && level <= (unsigned) levels; level--) {
+            set_level_eb(eb, level);
+            level_pass(data, level);
+        }
+        quantizer.set_eb(eb);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// HuffmanEncoder<int> (include/QoZ/encoder/HuffmanEncoder.hpp)
+// ---------------------------------------------------------------------------
+
+// ska::unordered_map<int,size_t> iteration order after inserting the keys in
+// first-occurrence order (utils/ska_hash/unordered_map.hpp:254-303, 412-446,
+// 604-643; flat_hash_map.hpp:1268-1295): power-of-two buckets indexed by
+// Fibonacci hashing, growth to max(4, 2B) when count+1 > B, rehash walks old
+// buckets upward pushing each chain element to the front of its new chain,
+// new keys go to the chain front, iteration runs from the top bucket down.
+static std::vector<int> ska_iteration_order(const std::vector<int> &keys_in_first_occurrence) {
+    std::vector<std::vector<int>> chains;  // chain[i][0] = head
+    size_t B = 1;
+    int shift = 63;
+    bool allocated = false;
+    auto index = [&](int key) -> size_t {
+        return (size_t) ((11400714819323198485ull * (uint64_t) (size_t) (long) key) >> shift);
+    };
+    size_t count = 0;
+    for (int key: keys_in_first_occurrence) {
+        if (!allocated || (double) (count + 1) > (double) B * 1.0) {
+            size_t nb = std::max<size_t>(4, 2 * (allocated ? B : 1));
+            size_t p2 = 1;
+            while (p2 < nb) p2 <<= 1;
+            nb = std::max<size_t>(2, p2);
+            int log2nb = 0;
+            while (((size_t) 1 << log2nb) < nb) log2nb++;
+            std::vector<std::vector<int>> fresh(nb);
+            shift = 64 - log2nb;
+            if (allocated) {
+                for (size_t i = 0; i < B; i++)
+                    for (int e: chains[i]) {
+                        auto &c = fresh[index(e)];
+                        c.insert(c.begin(), e);
+                    }
+            }
+            chains.swap(fresh);
+            B = nb;
+            allocated = true;
+        }
+        auto &c = chains[index(key)];
+        c.insert(c.begin(), key);
+        count++;
+    }
+    std::vector<int> order;
+    for (size_t i = B; i-- > 0;)
+        for (int e: chains[i]) order.push_back(e);
+    return order;
+}
+
+struct Huffman {
+    struct Node {
+        int left = -1, right = -1;
+        uint64_t freq = 0;
+        uint8_t t = 0;
+        int c = 0;
+    };
+    int offset = 0;
+    int stateNum = 0;
+    std::vector<Node> pool;
+    std::vector<int> qq;  // 1-based heap of pool indices
+    int qend = 1;
+    std::vector<uint64_t> code;
+    std::vector<uint8_t> len;
+    std::vector<bool> has_code;
+    int root = -1;
+    unsigned nodeCount = 0;
+
+    int new_leaf(uint64_t freq, int c) {
+        Node n;
+        n.freq = freq;
+        n.c = c;
+        n.t = 1;
+        pool.push_back(n);
+        return (int) pool.size() - 1;
+    }
+    int new_inner(int a, int b) {
+        Node n;
+        n.left = a;
+        n.right = b;
+        n.freq = pool[a].freq + pool[b].freq;
+        n.t = 0;
+        pool.push_back(n);
+        return (int) pool.size() - 1;
+    }
+    void qinsert(int n) {  // HUF:455-463
+        int j, i = qend++;
+        if ((int) qq.size() <= i) qq.resize(i + 1);
+        while ((j = (i >> 1))) {
+            if (pool[qq[j]].freq <= pool[n].freq) break;
+            qq[i] = qq[j], i = j;
+        }
+        qq[i] = n;
+    }
+    int qremove() {  // HUF:465-485
+        int i = 1, l;
+        int n = qq[i];
+        if (qend < 2) return -1;
+        qend--;
+        qq[i] = qq[qend];
+        while ((l = (i << 1)) < qend) {
+            if (l + 1 < qend && pool[qq[l + 1]].freq < pool[qq[l]].freq) l++;
+            if (pool[qq[i]].freq > pool[qq[l]].freq) {
+                std::swap(qq[i], qq[l]);
+                i = l;
+            } else {
+                break;
+            }
+        }
+        return n;
+    }
+    bool build_code(int n, int length, uint64_t out1) {  // HUF:493-521 (codes <= 64 bits)
+        if (pool[n].t) {
+            if (length > 64) return false;
+            code[pool[n].c] = length == 0 ? 0 : (out1 << (64 - length));
+            len[pool[n].c] = (uint8_t) length;
+            has_code[pool[n].c] = true;
+            return true;
+        }
+        if (length >= 64) return false;
+        out1 <<= 1;
+        if (!build_code(pool[n].left, length + 1, out1)) return false;
+        return build_code(pool[n].right, length + 1, out1 | 1);
+    }
+
+    // init (HUF:529-566) + preprocess_encode (HUF:95-116)
+    bool init(const int32_t *s, size_t n) {
+        std::vector<int> first_order;
+        std::vector<uint64_t> freq;
+        int mn = s[0], mx = s[0];
+        for (size_t i = 0; i < n; i++) {
+            mn = std::min(mn, s[i]);
+            mx = std::max(mx, s[i]);
+        }
+        // counts and first-occurrence order (dense table over [mn, mx])
+        std::vector<uint64_t> cnt((size_t) ((int64_t) mx - mn + 1), 0);
+        for (size_t i = 0; i < n; i++) {
+            uint64_t &c = cnt[(size_t) ((int64_t) s[i] - mn)];
+            if (c == 0) first_order.push_back(s[i]);
+            c++;
+        }
+        offset = mn;
+        stateNum = mx - offset + 2;
+        pool.clear();
+        qq.assign(2, 0);
+        qend = 1;
+        std::vector<int> order = ska_iteration_order(first_order);
+        for (int key: order) qinsert(new_leaf(cnt[(size_t) ((int64_t) key - mn)], key - offset));
+        // qinsert(new_node(0, 0, qremove(), qremove())): g++ evaluates the two
+        // calls right to left, so the first removed node becomes the right child.
+        while (qend > 2) {
+            int first = qremove();
+            int second = qremove();
+            qinsert(new_inner(second, first));
+        }
+        root = qq[1];
+        code.assign(stateNum, 0);
+        len.assign(stateNum, 0);
+        has_code.assign(stateNum, false);
+        if (!build_code(root, 0, 0)) return false;
+        unsigned k = 0;
+        for (int i = 0; i < stateNum; i++)
+            if (has_code[i]) k++;
+        nodeCount = k * 2 - 1;
+        return true;
+    }
+
+    static void put_be32(uint8_t *b, int32_t v) {
+        b[0] = (uint8_t) (v >> 24);
+        b[1] = (uint8_t) (v >> 16);
+        b[2] = (uint8_t) (v >> 8);
+        b[3] = (uint8_t) v;
+    }
+
+    template<class T1>
+    void pad_tree(std::vector<T1> &L, std::vector<T1> &R, std::vector<int32_t> &C, std::vector<uint8_t> &t,
+                  unsigned i, int node, unsigned &n_inode) {  // HUF:569-584
+        C[i] = pool[node].c;
+        t[i] = pool[node].t;
+        if (pool[node].left >= 0) {
+            n_inode++;
+            L[i] = (T1) n_inode;
+            pad_tree(L, R, C, t, n_inode, pool[node].left, n_inode);
+        }
+        if (pool[node].right >= 0) {
+            n_inode++;
+            R[i] = (T1) n_inode;
+            pad_tree(L, R, C, t, n_inode, pool[node].right, n_inode);
+        }
+    }
+
+    template<class T1>
+    void tree_bytes(std::vector<uint8_t> &out) {  // HUF:607-633
+        std::vector<T1> L(nodeCount, 0), R(nodeCount, 0);
+        std::vector<int32_t> C(nodeCount, 0);
+        std::vector<uint8_t> t(nodeCount, 0);
+        unsigned n_inode = 0;
+        pad_tree(L, R, C, t, 0, root, n_inode);
+        out.push_back(0);  // little-endian marker
+        const uint8_t *p = (const uint8_t *) L.data();
+        out.insert(out.end(), p, p + nodeCount * sizeof(T1));
+        p = (const uint8_t *) R.data();
+        out.insert(out.end(), p, p + nodeCount * sizeof(T1));
+        p = (const uint8_t *) C.data();
+        out.insert(out.end(), p, p + nodeCount * sizeof(int32_t));
+        out.insert(out.end(), t.begin(), t.end());
+    }
+
+    // save (HUF:119-137)
+    void save(std::vector<uint8_t> &out) {
+        uint8_t b[4];
+        memcpy(b, &offset, 4);
+        out.insert(out.end(), b, b + 4);
+        put_be32(b, (int32_t) nodeCount);
+        out.insert(out.end(), b, b + 4);
+        put_be32(b, (int32_t) (stateNum / 2));
+        out.insert(out.end(), b, b + 4);
+        if (nodeCount <= 256)
+            tree_bytes<uint8_t>(out);
+        else if (nodeCount <= 65536)
+            tree_bytes<uint16_t>(out);
+        else
+            tree_bytes<uint32_t>(out);
+    }
+
+    // encode (HUF:150-227): size_t byte count, then the codes MSB-first
+    void encode(const int32_t *s, size_t n, std::vector<uint8_t> &out) {
+        size_t start = out.size();
+        out.resize(start + 8, 0);
+        std::vector<uint8_t> bits;
+        uint64_t acc = 0;
+        int nacc = 0;  // bits pending in acc (MSB-aligned at bit 63)
+        for (size_t i = 0; i < n; i++) {
+            int st = s[i] - offset;
+            int l = len[st];
+            uint64_t c = code[st];
+            for (int k = 0; k < l; k++) {
+                uint64_t bit = (c >> (63 - k)) & 1;
+                acc |= bit << (63 - nacc);
+                if (++nacc == 8) {
+                    bits.push_back((uint8_t) (acc >> 56));
+                    acc = 0;
+                    nacc = 0;
+                }
+            }
+        }
+        if (nacc) bits.push_back((uint8_t) (acc >> 56));
+        uint64_t outSize = bits.size();
+        memcpy(out.data() + start, &outSize, 8);
+        out.insert(out.end(), bits.begin(), bits.end());
+    }
+};
+
+// load + decode (HUF:236-296, 310-454)
+struct HuffDecoder {
+    struct Node {
+        int left = -1, right = -1;
+        int c = 0;
+        uint8_t t = 0;
+    };
+    std::vector<Node> nodes;
+    int offset = 0;
+
+    static int32_t get_be32(const uint8_t *b) {
+        return (int32_t) (((uint32_t) b[0] << 24) | ((uint32_t) b[1] << 16) | ((uint32_t) b[2] << 8) | b[3]);
+    }
+    template<class T1>
+    void unpad(const T1 *L, const T1 *R, const int32_t *C, const uint8_t *t, unsigned i, int node) {
+        if (nodes[node].t == 0) {
+            T1 l = L[i];
+            if (l != 0) {
+                Node m;
+                m.c = C[l];
+                m.t = t[l];
+                nodes.push_back(m);
+                int id = (int) nodes.size() - 1;
+                nodes[node].left = id;
+                unpad(L, R, C, t, l, id);
+            }
+            T1 r = R[i];
+            if (r != 0) {
+                Node m;
+                m.c = C[r];
+                m.t = t[r];
+                nodes.push_back(m);
+                int id = (int) nodes.size() - 1;
+                nodes[node].right = id;
+                unpad(L, R, C, t, r, id);
+            }
+        }
+    }
+    template<class T1>
+    size_t load_tree(const uint8_t *b, unsigned nc) {
+        std::vector<T1> L(nc), R(nc);
+        std::vector<int32_t> C(nc);
+        std::vector<uint8_t> t(nc);
+        memcpy(L.data(), b + 1, nc * sizeof(T1));
+        memcpy(R.data(), b + 1 + nc * sizeof(T1), nc * sizeof(T1));
+        memcpy(C.data(), b + 1 + 2 * nc * sizeof(T1), nc * 4);
+        memcpy(t.data(), b + 1 + 2 * nc * sizeof(T1) + nc * 4, nc);
+        nodes.clear();
+        Node root;
+        if (nc <= 256) {
+            root.c = C[0];
+            root.t = t[0];
+        }
+        nodes.push_back(root);
+        unpad(L.data(), R.data(), C.data(), t.data(), 0, 0);
+        return 1 + 2 * nc * sizeof(T1) + nc * 4 + nc;
+    }
+    // returns bytes consumed by load + decode, 0 on error
+    size_t decode(const uint8_t *buf, size_t n, int32_t *out, size_t num) {
+        if (n < 12) return 0;
+        memcpy(&offset, buf, 4);
+        unsigned nc = (unsigned) get_be32(buf + 4);
+        size_t used = 12;
+        if (nc <= 256)
+            used += load_tree<uint8_t>(buf + 12, nc);
+        else if (nc <= 65536)
+            used += load_tree<uint16_t>(buf + 12, nc);
+        else
+            used += load_tree<uint32_t>(buf + 12, nc);
+        uint64_t enc;
+        memcpy(&enc, buf + used, 8);
+        used += 8;
+        const uint8_t *bytes = buf + used;
+        if (nodes[0].t) {
+            for (size_t i = 0; i < num; i++) out[i] = nodes[0].c + offset;
+            return used + enc;
+        }
+        int nd = 0;
+        size_t count = 0;
+        for (size_t i = 0; count < num; i++) {
+            if ((i >> 3) >= enc) return 0;
+            int bit = (bytes[i >> 3] >> (7 - (i & 7))) & 1;
+            nd = bit ? nodes[nd].right : nodes[nd].left;
+            if (nd < 0) return 0;
+            if (nodes[nd].t) {
+                out[count++] = nodes[nd].c + offset;
+                nd = 0;
+            }
+        }
+        return used + enc;
+    }
+};
+
+template<class V>
+static void put(std::vector<uint8_t> &out, const V &v) {
+    const uint8_t *p = (const uint8_t *) &v;
+    out.insert(out.end(), p, p + sizeof(V));
+}
+
+template<class V>
+static V take(const uint8_t *&p) {
+    V v;
+    memcpy(&v, p, sizeof(V));
+    p += sizeof(V);
+    return v;
+}
+
+// engine header + quantizer + Huffman (SIC:485-518, Q:185-201)
+static bool interp_raw(const oracle_interp_params *prm, const float *input, std::vector<uint8_t> &out) {
+    InterpEngine e;
+    e.set_params(prm);
+    std::vector<float> data(input, input + [&] {
+        size_t n = 1;
+        for (int i = 0; i < prm->N; i++) n *= prm->dims[i];
+        return n;
+    }());
+    std::vector<size_t> lc;
+    e.compress(data.data(), 0, 0, 0, lc);
+    Huffman h;
+    if (!h.init(e.codes.data(), e.codes.size())) return false;
+    out.clear();
+    for (int i = 0; i < prm->N; i++) put(out, (size_t) prm->dims[i]);
+    put(out, (unsigned) prm->block_size);
+    put(out, (int) prm->interp_algo);
+    put(out, (int) prm->direction);
+    put(out, prm->alpha);
+    put(out, prm->beta);
+    put(out, (size_t) prm->max_step);
+    put(out, (int) prm->levelwise);
+    put(out, (bool) false);   // blockwiseTuning
+    put(out, (int) 0);        // fixBlockSize
+    put(out, (size_t) 0);     // cross_block
+    put(out, (int) 0);        // trimToZero
+    put(out, (int) 0);        // blockOrder
+    out.insert(out.end(), prm->ops, prm->ops + prm->levelwise);
+    out.insert(out.end(), prm->dirs, prm->dirs + prm->levelwise);
+    out.push_back(2);
+    put(out, prm->eb);
+    put(out, (int) prm->radius);
+    put(out, (size_t) e.quantizer.unpred.size());
+    const uint8_t *up = (const uint8_t *) e.quantizer.unpred.data();
+    out.insert(out.end(), up, up + e.quantizer.unpred.size() * sizeof(float));
+    h.save(out);
+    h.encode(e.codes.data(), e.codes.size(), out);
+    return true;
+}
+
+static size_t interp_header_len(int N, int levels) {
+    return N * 8 + 4 + 4 + 4 + 8 + 8 + 8 + 4 + 1 + 4 + 8 + 4 + 4 + 2 * (size_t) levels;
+}
+
+static bool interp_decompress_raw(const uint8_t *raw, size_t n, int N, float *out, size_t num) {
+    const uint8_t *p = raw;
+    oracle_interp_params prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.N = N;
+    size_t total = 1;
+    for (int i = 0; i < N; i++) {
+        prm.dims[i] = take<size_t>(p);
+        total *= prm.dims[i];
+    }
+    if (total != num) return false;
+    prm.block_size = (int) take<unsigned>(p);
+    prm.interp_algo = take<int>(p);
+    prm.direction = take<int>(p);
+    prm.alpha = take<double>(p);
+    prm.beta = take<double>(p);
+    prm.max_step = take<size_t>(p);
+    prm.levelwise = take<int>(p);
+    bool blockwise = take<bool>(p);
+    int fixbs = take<int>(p);
+    size_t cross = take<size_t>(p);
+    int ttz = take<int>(p);
+    int border = take<int>(p);
+    if (blockwise || fixbs || cross || ttz || border) return false;  // out-of-scope engine modes
+    if (prm.levelwise < 0 || prm.levelwise > 32) return false;
+    memcpy(prm.ops, p, prm.levelwise);
+    p += prm.levelwise;
+    memcpy(prm.dirs, p, prm.levelwise);
+    p += prm.levelwise;
+    p += 1;
+    prm.eb = take<double>(p);
+    prm.radius = take<int>(p);
+    size_t U = take<size_t>(p);
+    InterpEngine e;
+    e.set_params(&prm);
+    e.quantizer.unpred.assign((const float *) p, (const float *) p + U);
+    p += U * sizeof(float);
+    e.codes.resize(num);
+    HuffDecoder hd;
+    if (!hd.decode(p, n - (p - raw), e.codes.data(), num)) return false;
+    e.decompress(out);
+    return true;
+}
+
+// QoZ::Config trailer for an ALGO_INTERP stream (Config.hpp:240-309)
+static void config_save(std::vector<uint8_t> &out, const oracle_interp_params *p, double absEB, double relEB) {
+    size_t num = 1;
+    for (int i = 0; i < p->N; i++) num *= p->dims[i];
+    put(out, (char) p->N);
+    for (int i = 0; i < p->N; i++) put(out, (size_t) p->dims[i]);
+    put(out, num);
+    put(out, (uint8_t) 2);  // cmprAlgo = ALGO_INTERP
+    put(out, (uint8_t) 0);  // errorBoundMode = EB_ABS (after calAbsErrorBound)
+    put(out, (uint8_t) 0);  // tuningTarget = RD
+    put(out, absEB);
+    put(out, relEB);
+    put(out, p->alpha);
+    put(out, p->beta);
+    put(out, (bool) true);   // lorenzo
+    put(out, (bool) false);  // lorenzo2
+    put(out, (bool) true);   // regression
+    put(out, (bool) false);  // regression2
+    put(out, (uint8_t) p->interp_algo);
+    put(out, (uint8_t) p->direction);
+    put(out, (int) p->block_size);
+    put(out, (uint8_t) 1);  // lossless = zstd
+    put(out, (uint8_t) 1);  // encoder = Huffman
+    put(out, (int) (p->radius * 2));
+    int blockSize = p->N == 1 ? 128 : (p->N == 2 ? 16 : 6);
+    put(out, blockSize);
+    put(out, (int) p->levelwise);
+    put(out, (bool) false);  // blockwiseTuning
+    put(out, blockSize);     // stride
+    put(out, (size_t) p->max_step);
+    put(out, (int) p->N);    // pred_dim
+    put(out, (bool) false);  // openmp
+    put(out, (int) 0);       // fixBlockSize
+    put(out, (int) 0);       // blockwiseSampleBlockSize
+    put(out, (int) 0);       // crossBlock
+    put(out, (int) 0);       // wavelet
+    put(out, (size_t) 0);    // firstSize (indeterminate in the reference)
+    put(out, (int) 0);       // offsetPredictor
+    put(out, num);           // coeffs_num
+    put(out, (size_t) p->N);
+    for (int i = 0; i < p->N; i++) put(out, (size_t) p->dims[i]);
+    put(out, (int) 0);       // pid
+    put(out, (int) 0);       // blockOrder
+    put(out, (int) -1);      // sperr
+    put(out, (bool) false);  // sperrWithoutWave
+    put(out, (bool) true);   // pyBind
+    put(out, (size_t) 0);    // metadata size
+    put(out, (int) 0);       // conditioning
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_interp_compress(const oracle_interp_params *p, float *data, int tuning, int start_level, int end_level,
+                           int32_t *codes, size_t *n_codes, float *unpred, size_t *n_unpred, double *loss,
+                           size_t *level_counts, int *n_levels) {
+    if (!p || p->N < 1 || p->N > 4) return -1;
+    InterpEngine e;
+    e.set_params(p);
+    std::vector<size_t> lc;
+    double err = e.compress(data, tuning, start_level, end_level, lc);
+    if (codes) memcpy(codes, e.codes.data(), e.codes.size() * sizeof(int32_t));
+    if (n_codes) *n_codes = e.codes.size();
+    if (unpred) memcpy(unpred, e.quantizer.unpred.data(), e.quantizer.unpred.size() * sizeof(float));
+    if (n_unpred) *n_unpred = e.quantizer.unpred.size();
+    if (loss) *loss = err;
+    if (level_counts) {
+        for (size_t i = 0; i < lc.size() && i < 64; i++) level_counts[i] = lc[i];
+    }
+    if (n_levels) *n_levels = e.levels;
+    return 0;
+}
+
+int oracle_interp_raw(const oracle_interp_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out) {
+    std::vector<uint8_t> buf;
+    if (!interp_raw(p, data, buf)) return -2;
+    *n_out = buf.size();
+    if (buf.size() > cap) return -3;
+    memcpy(out, buf.data(), buf.size());
+    return 0;
+}
+
+int oracle_interp_decompress_raw(const uint8_t *raw, size_t n, int N, float *out, size_t num) {
+    return interp_decompress_raw(raw, n, N, out, num) ? 0 : -2;
+}
+
+int oracle_huffman_encode(const int32_t *codes, size_t n, uint8_t *out, size_t cap, size_t *n_out) {
+    if (n == 0) return -1;
+    Huffman h;
+    if (!h.init(codes, n)) return -2;
+    std::vector<uint8_t> buf;
+    h.save(buf);
+    h.encode(codes, n, buf);
+    *n_out = buf.size();
+    if (buf.size() > cap) return -3;
+    memcpy(out, buf.data(), buf.size());
+    return 0;
+}
+
+int oracle_huffman_decode(const uint8_t *buf, size_t n, int32_t *out, size_t num, size_t *n_used) {
+    HuffDecoder d;
+    size_t used = d.decode(buf, n, out, num);
+    if (!used) return -2;
+    if (n_used) *n_used = used;
+    return 0;
+}
+
+int oracle_compress_stream(const oracle_interp_params *p, int rel_mode, double eb_in, const float *data,
+                           uint8_t *out, size_t cap, size_t *n_out) {
+    size_t num = 1;
+    for (int i = 0; i < p->N; i++) num *= p->dims[i];
+    oracle_interp_params q = *p;
+    double rel = -1.0;
+    if (rel_mode) {
+        float mn, mx;
+        oracle_range(data, num, &mn, &mx);
+        double rng = (double) (float) (mx - mn);
+        rel = eb_in;
+        q.eb = rel * rng;
+    } else {
+        q.eb = eb_in;
+    }
+    std::vector<uint8_t> raw;
+    if (!interp_raw(&q, data, raw)) return -2;
+    std::vector<uint8_t> s(8 + ZSTD_compressBound(raw.size()));
+    uint64_t rawLen = raw.size();
+    memcpy(s.data(), &rawLen, 8);
+    size_t z = ZSTD_compress(s.data() + 8, s.size() - 8, raw.data(), raw.size(), 3);
+    if (ZSTD_isError(z)) return -4;
+    s.resize(8 + z);
+    size_t before = s.size();
+    config_save(s, &q, q.eb, rel);
+    put(s, (int) (s.size() - before));
+    *n_out = s.size();
+    if (s.size() > cap) return -3;
+    memcpy(out, s.data(), s.size());
+    return 0;
+}
+
+int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t num) {
+    int confSize;
+    memcpy(&confSize, stream + n - 4, 4);
+    const uint8_t *c = stream + n - 4 - confSize;
+    int N = (int) (signed char) c[0];
+    uint64_t rawLen;
+    memcpy(&rawLen, stream, 8);
+    std::vector<uint8_t> raw(rawLen);
+    size_t z = ZSTD_decompress(raw.data(), rawLen, stream + 8, (n - 4 - confSize) - 8);
+    if (ZSTD_isError(z) || z != rawLen) return -4;
+    return interp_decompress_raw(raw.data(), raw.size(), N, out, num) ? 0 : -2;
+}
+
+int oracle_range(const float *data, size_t n, float *mn, float *mx) {
+    float a = data[0], b = data[0];
+    for (size_t i = 1; i < n; i++) {
+        if (b < data[i]) b = data[i];
+        if (a > data[i]) a = data[i];
+    }
+    *mn = a;
+    *mx = b;
+    return 0;
+}
+
+int oracle_interp_seq_index(const oracle_interp_params *p, int64_t *seq) {
+    InterpEngine e;
+    e.set_params(p);
+    size_t num = 1;
+    for (int i = 0; i < p->N; i++) num *= p->dims[i];
+    std::vector<float> data(num, 0.0f);
+    std::vector<size_t> lc;
+    e.compress(data.data(), 0, 0, 0, lc);
+    for (size_t i = 0; i < num; i++) seq[i] = -1;
+    for (size_t k = 0; k < e.code_pos.size(); k++) seq[e.code_pos[k]] = (int64_t) k;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+/* CPU oracle for the FAZ compression hot path — TEST INFRASTRUCTURE ONLY.
+ *
+ * A scalar restatement of the reference algorithm, used by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+ * The product (faz_amd/) never links, loads or calls it.
+ *
+ * All functions return 0 on success, a negative value on error.
+ */
+#ifndef FAZ_ORACLE_H
+#define FAZ_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors the QoZ::Config fields the interpolation engine reads
+ * (include/QoZ/utils/Config.hpp:394-499, SIC:333-345). */
+typedef struct {
+    int N;
+    size_t dims[4];
+    double eb;            /* absolute error bound (after calAbsErrorBound) */
+    int interp_algo;      /* 0 linear, 1 cubic */
+    int direction;        /* index into next_permutation order of axes */
+    double alpha, beta;
+    size_t max_step;      /* anchor stride, 0 = no anchors */
+    int block_size;       /* interpBlockSize */
+    int radius;           /* quantbinCnt / 2 */
+    int levelwise;        /* entries in ops/dirs (0 = none) */
+    uint8_t ops[32];
+    uint8_t dirs[32];
+} oracle_interp_params;
+
+/* SZInterpolationCompressor::compress in tuning mode 0/1/2 (SIC:328-476).
+ * data is overwritten with the reconstruction as the reference does.
+ * codes: capacity n (mode 2 pushes none); unpred capacity n.
+ * level_counts: capacity 64, filled in tuning modes. */
+int oracle_interp_compress(const oracle_interp_params *p, float *data, int tuning, int start_level, int end_level,
+                           int32_t *codes, size_t *n_codes, float *unpred, size_t *n_unpred,
+                           double *loss, size_t *level_counts, int *n_levels);
+
+/* Pre-zstd engine buffer (SIC:485-518): header, quantizer, Huffman tree, bits. */
+int oracle_interp_raw(const oracle_interp_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out);
+
+/* SZInterpolationCompressor::decompress on a pre-zstd buffer (SIC:42-199). */
+int oracle_interp_decompress_raw(const uint8_t *raw, size_t n, int N, float *out, size_t num);
+
+/* HuffmanEncoder preprocess_encode + save + encode (HUF:103-227). */
+int oracle_huffman_encode(const int32_t *codes, size_t n, uint8_t *out, size_t cap, size_t *n_out);
+/* HuffmanEncoder load + decode (HUF:236-296). *n_used = bytes consumed. */
+int oracle_huffman_decode(const uint8_t *buf, size_t n, int32_t *out, size_t num, size_t *n_used);
+
+/* Full SZ_compress stream for ALGO_INTERP (sz.hpp:54-91): the error bound
+ * mode is applied first (rel_mode != 0 → eb is relative, Statistic.hpp:34-66). */
+int oracle_compress_stream(const oracle_interp_params *p, int rel_mode, double eb_in, const float *data,
+                           uint8_t *out, size_t cap, size_t *n_out);
+int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t num);
+
+/* data_range (Statistic.hpp:11-21). */
+int oracle_range(const float *data, size_t n, float *mn, float *mx);
+
+/* The block-order sequence index of every point (where its code lands in
+ * the code stream), -1 for none. Used by tests to pin the closed form. */
+int oracle_interp_seq_index(const oracle_interp_params *p, int64_t *seq);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+// Golden-vector driver around the reference FAZ engine classes.
+//
+// Test infrastructure only: this file is compiled (by oracle/Makefile) against
+// the reference headers where they lie and writes its binary into oracle/_ref/.
+// It composes the reference classes exactly as the reference drivers do and
+// dumps what the product must reproduce bit-for-bit:
+//   * SZ_compress_Interp (include/QoZ/api/impl/SZInterp.hpp:387-414) +
+//     SZ_compress trailer (include/QoZ/api/sz.hpp:54-91), restated here because
+//     SZInterp.hpp pulls in PolyRegressionPredictor.hpp whose
+//     PolyRegressionCoeffAux.hpp is absent from the reference tree;
+//   * the pre-zstd engine buffer (same engine, Lossless_bypass);
+//   * codes decoded from that buffer with the reference HuffmanEncoder;
+//   * tuning modes 1/2 of SZInterpolationCompressor::compress (SIC:328-476);
+//   * SZ_decompress for ALGO_INTERP (sz.hpp:159-205, SZInterp.hpp:417-431);
+//   * the Huffman coder alone (HuffmanEncoder.hpp:95-227);
+//   * the Lorenzo/regression compressors (SZLorenzoReg.hpp:27-114).
+#include "QoZ/utils/Config.hpp"
+#include "QoZ/utils/Statistic.hpp"  // QoZ::factorial, used by the engine header below
+#include "QoZ/compressor/SZInterpolationCompressor.hpp"
+#include "QoZ/compressor/SZGeneralCompressor.hpp"
+#include "QoZ/frontend/SZFastFrontend.hpp"
+#include "QoZ/frontend/SZGeneralFrontend.hpp"
+#include "QoZ/predictor/ComposedPredictor.hpp"
+#include "QoZ/predictor/LorenzoPredictor.hpp"
+#include "QoZ/predictor/RegressionPredictor.hpp"
+#include "QoZ/quantizer/IntegerQuantizer.hpp"
+#include "QoZ/encoder/HuffmanEncoder.hpp"
+#include "QoZ/lossless/Lossless_zstd.hpp"
+#include "QoZ/lossless/Lossless_bypass.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+typedef std::map<std::string, std::string> Args;
+
+static Args parse_args(int argc, char **argv, int first) {
+    Args a;
+    for (int i = first; i < argc; i++) {
+        std::string s(argv[i]);
+        size_t eq = s.find('=');
+        if (eq == std::string::npos) {
+            fprintf(stderr, "bad argument %s\n", argv[i]);
+            exit(2);
+        }
+        a[s.substr(0, eq)] = s.substr(eq + 1);
+    }
+    return a;
+}
+
+static std::string get(const Args &a, const char *k, const char *def = nullptr) {
+    auto it = a.find(k);
+    if (it == a.end()) {
+        if (!def) {
+            fprintf(stderr, "missing argument %s\n", k);
+            exit(2);
+        }
+        return def;
+    }
+    return it->second;
+}
+
+static std::vector<size_t> parse_list(const std::string &s) {
+    std::vector<size_t> v;
+    size_t p = 0;
+    while (p < s.size()) {
+        size_t q = s.find(',', p);
+        if (q == std::string::npos) q = s.size();
+        if (q > p) v.push_back(strtoull(s.substr(p, q - p).c_str(), nullptr, 10));
+        p = q + 1;
+    }
+    return v;
+}
+
+template<class T>
+static std::vector<T> read_vec(const std::string &path) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) { perror(path.c_str()); exit(2); }
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    std::vector<T> v(sz / sizeof(T));
+    if (!v.empty() && fread(v.data(), sizeof(T), v.size(), f) != v.size()) { perror("read"); exit(2); }
+    fclose(f);
+    return v;
+}
+
+static void write_bytes(const std::string &path, const void *p, size_t n) {
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) { perror(path.c_str()); exit(2); }
+    if (n && fwrite(p, 1, n, f) != n) { perror("write"); exit(2); }
+    fclose(f);
+}
+
+static QoZ::Config make_config(const Args &a) {
+    std::vector<size_t> dims = parse_list(get(a, "dims"));
+    QoZ::Config conf;
+    conf.setDims(dims.begin(), dims.end());
+    conf.cmprAlgo = QoZ::ALGO_INTERP;
+    std::string mode = get(a, "mode", "abs");
+    if (mode == "abs") {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = atof(get(a, "eb").c_str());
+    } else {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = atof(get(a, "eb").c_str());
+    }
+    conf.interpAlgo = (uint8_t) atoi(get(a, "algo", "1").c_str());
+    conf.interpDirection = (uint8_t) atoi(get(a, "dir", "0").c_str());
+    conf.alpha = atof(get(a, "alpha", "-1").c_str());
+    conf.beta = atof(get(a, "beta", "-1").c_str());
+    conf.maxStep = strtoull(get(a, "maxstep", "0").c_str(), nullptr, 10);
+    conf.interpBlockSize = atoi(get(a, "bs", "32").c_str());
+    conf.quantbinCnt = atoi(get(a, "qbins", "65536").c_str());
+    std::vector<size_t> ops = parse_list(get(a, "ops", ""));
+    std::vector<size_t> dirs = parse_list(get(a, "dirs", ""));
+    conf.interpAlgo_list.assign(ops.begin(), ops.end());
+    conf.interpDirection_list.assign(dirs.begin(), dirs.end());
+    conf.levelwisePredictionSelection = (int) ops.size();
+    conf.firstSize = 0;  // never initialised by the reference when wavelet==0
+    conf.meta.fill(0);
+    return conf;
+}
+
+// Header length of the interp engine buffer (SIC:489-512).
+template<QoZ::uint N>
+static size_t interp_header_len(size_t levels) {
+    return N * sizeof(size_t) + sizeof(QoZ::uint) + 2 * sizeof(int) + 2 * sizeof(double) + sizeof(size_t) +
+           sizeof(int) + sizeof(bool) + sizeof(int) + sizeof(size_t) + 2 * sizeof(int) + 2 * levels;
+}
+
+template<QoZ::uint N>
+static int run_interp(const Args &a) {
+    typedef float T;
+    std::string out = get(a, "out");
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    QoZ::Config conf = make_config(a);
+    if (input.size() != conf.num) { fprintf(stderr, "size mismatch\n"); return 2; }
+    int tuning = atoi(get(a, "tuning", "0").c_str());
+    int start = atoi(get(a, "start", "0").c_str());
+    int end = atoi(get(a, "end", "0").c_str());
+
+    if (tuning) {
+        QoZ::calAbsErrorBound(conf, input.data());
+        std::vector<T> data(input);
+        auto sz = QoZ::SZInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_bypass>(
+                QoZ::LinearQuantizer<T>(conf.absErrorBound, conf.quantbinCnt / 2), QoZ::HuffmanEncoder<int>(),
+                QoZ::Lossless_bypass());
+        size_t outSize = 0;
+        conf.verbose = 0;
+        QoZ::uchar *buf = sz.compress(conf, data.data(), outSize, tuning, start, end);
+        delete[] buf;
+        write_bytes(out + ".codes", conf.quant_bins.data(), conf.quant_bins.size() * sizeof(int));
+        write_bytes(out + ".recon", data.data(), data.size() * sizeof(T));
+        write_bytes(out + ".loss", &conf.decomp_square_error, sizeof(double));
+        write_bytes(out + ".levelcounts", conf.quant_bin_counts.data(), conf.quant_bin_counts.size() * sizeof(size_t));
+        return 0;
+    }
+
+    // Pre-zstd engine buffer: same engine with the bypass lossless stage.
+    {
+        QoZ::Config c2(conf);
+        QoZ::calAbsErrorBound(c2, input.data());
+        std::vector<T> data(input);
+        auto sz = QoZ::SZInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_bypass>(
+                QoZ::LinearQuantizer<T>(c2.absErrorBound, c2.quantbinCnt / 2), QoZ::HuffmanEncoder<int>(),
+                QoZ::Lossless_bypass());
+        size_t rawSize = 0;
+        c2.verbose = 0;
+        QoZ::uchar *raw = sz.compress(c2, data.data(), rawSize);
+        write_bytes(out + ".raw", raw, rawSize);
+        write_bytes(out + ".recon", data.data(), data.size() * sizeof(T));
+        write_bytes(out + ".eb", &c2.absErrorBound, sizeof(double));
+        // codes and unpredictable list, decoded with the reference classes
+        size_t L = c2.interpAlgo_list.size();
+        const QoZ::uchar *pos = raw + interp_header_len<N>(L);
+        size_t U;
+        memcpy(&U, pos + 1 + sizeof(double) + sizeof(int), sizeof(size_t));
+        const QoZ::uchar *up = pos + 1 + sizeof(double) + sizeof(int) + sizeof(size_t);
+        write_bytes(out + ".unpred", up, U * sizeof(T));
+        pos = up + U * sizeof(T);
+        size_t remaining = rawSize - (pos - raw);
+        QoZ::HuffmanEncoder<int> enc;
+        enc.load(pos, remaining);
+        std::vector<int> codes = enc.decode(pos, c2.num);
+        write_bytes(out + ".codes", codes.data(), codes.size() * sizeof(int));
+        delete[] raw;
+    }
+
+    // Full stream as SZ_compress writes it: zstd(engine buffer) + Config trailer + int.
+    {
+        QoZ::Config c3(conf);
+        std::vector<T> data(input);
+        QoZ::calAbsErrorBound(c3, data.data());
+        c3.verbose = 0;
+        auto sz = QoZ::SZInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_zstd>(
+                QoZ::LinearQuantizer<T>(c3.absErrorBound, c3.quantbinCnt / 2), QoZ::HuffmanEncoder<int>(),
+                QoZ::Lossless_zstd());
+        size_t outSize = 0;
+        char *cmpData = (char *) sz.compress(c3, data.data(), outSize);
+        QoZ::uchar *cmpDataPos = (QoZ::uchar *) cmpData + outSize;
+        c3.save(cmpDataPos);
+        size_t newSize = (char *) cmpDataPos - cmpData;
+        QoZ::write(int(newSize - outSize), cmpDataPos);
+        outSize = (char *) cmpDataPos - cmpData;
+        write_bytes(out + ".stream", cmpData, outSize);
+        delete[] cmpData;
+    }
+    return 0;
+}
+
+template<QoZ::uint N>
+static int run_decompress_n(QoZ::Config &conf, char *cmpData, size_t cmpSize, float *dec) {
+    auto sz = QoZ::SZInterpolationCompressor<float, N, QoZ::LinearQuantizer<float>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_zstd>(
+            QoZ::LinearQuantizer<float>(), QoZ::HuffmanEncoder<int>(), QoZ::Lossless_zstd());
+    sz.decompress((QoZ::uchar *) cmpData, cmpSize, dec);
+    return 0;
+}
+
+static int run_decompress(const Args &a) {
+    std::vector<char> stream = read_vec<char>(get(a, "in"));
+    int confSize;
+    memcpy(&confSize, stream.data() + stream.size() - sizeof(int), sizeof(int));
+    const QoZ::uchar *pos = (const QoZ::uchar *) stream.data() + (stream.size() - sizeof(int) - confSize);
+    QoZ::Config conf;
+    conf.load(pos);
+    if (conf.cmprAlgo != QoZ::ALGO_INTERP) { fprintf(stderr, "only ALGO_INTERP streams\n"); return 2; }
+    std::vector<float> dec(conf.num);
+    size_t cmpSize = stream.size() - sizeof(int) - confSize;
+    switch (conf.N) {
+        case 1: run_decompress_n<1>(conf, stream.data(), cmpSize, dec.data()); break;
+        case 2: run_decompress_n<2>(conf, stream.data(), cmpSize, dec.data()); break;
+        case 3: run_decompress_n<3>(conf, stream.data(), cmpSize, dec.data()); break;
+        case 4: run_decompress_n<4>(conf, stream.data(), cmpSize, dec.data()); break;
+        default: return 2;
+    }
+    write_bytes(get(a, "out"), dec.data(), dec.size() * sizeof(float));
+    return 0;
+}
+
+static int run_huffman(const Args &a) {
+    std::vector<int> codes = read_vec<int>(get(a, "in"));
+    QoZ::HuffmanEncoder<int> enc;
+    enc.preprocess_encode(codes, 0);
+    std::vector<QoZ::uchar> buf(64 + enc.size_est() + codes.size() * 8 + 64);
+    QoZ::uchar *pos = buf.data();
+    enc.save(pos);
+    enc.encode(codes, pos);
+    enc.postprocess_encode();
+    write_bytes(get(a, "out"), buf.data(), pos - buf.data());
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    if (argc < 2) {
+        fprintf(stderr, "usage: ref_harness interp|decompress|huffman key=value...\n");
+        return 2;
+    }
+    std::string cmd(argv[1]);
+    Args a = parse_args(argc, argv, 2);
+    if (cmd == "interp") {
+        size_t N = parse_list(get(a, "dims")).size();
+        switch (N) {
+            case 1: return run_interp<1>(a);
+            case 2: return run_interp<2>(a);
+            case 3: return run_interp<3>(a);
+            case 4: return run_interp<4>(a);
+            default: return 2;
+        }
+    }
+    if (cmd == "decompress") return run_decompress(a);
+    if (cmd == "huffman") return run_huffman(a);
+    fprintf(stderr, "unknown command %s\n", argv[1]);
+    return 2;
+}
