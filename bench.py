@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Throughput of the FAZ interpolation + quantization + Huffman path on one MI355X.
+
+Workload (plain GPU drop-in, single field): a 512x512x512 float32 Gaussian
+random field with power spectrum P(k) ~ k^-11/3, min-max normalised to [0,1]
+plus N(0, (1e-4)^2) white noise, plus 16 randomly placed tanh fronts (width 4
+cells, amplitude 0.5); REL error bound 1e-3, cubic interpolation, anchor
+stride 32 (the reference's QoZ 3-D defaults).  Synthesised on the GPU from a
+seed; the field is resident in HBM before the timed region starts.
+
+A step is one device-resident compression of the field: value range,
+interpolation + quantization (one launch per level and pass), histogram and
+outlier compaction, host Huffman tree, GPU encode.  `value` = 4 * n bytes of
+float32 input per step / step time (GB/s).  The ratio and PSNR are measured
+once after the timed steps on the full stream (zstd + Config trailer).
+
+    python bench.py [--steps K] [--warmup W] [--rel 1e-3] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "compress GB/s + ratio at REL 1e-3 (bit-exact); learned vs exhaustive tuner"
+PEAK_HBM_GBS = 8000.0
+# Algorithmic HBM bytes per predicted point of the interpolation pass kernel:
+# read the original float (4) + write the reconstruction (4) + write the
+# 16-bit code (2).  Stencil neighbour re-reads are not counted.
+PASS_BYTES_PER_POINT = 10.0
+
+
+def make_field(shape, seed=0, fronts=16, slope=11.0 / 3.0, noise=1e-4, width=4.0, amp=0.5):
+    """k^-slope Gaussian random field + tanh fronts on the GPU (float32, contiguous, flat)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    n0, n1, n2 = shape
+    kx = torch.fft.fftfreq(n0, device="cuda").view(-1, 1, 1)
+    ky = torch.fft.fftfreq(n1, device="cuda").view(1, -1, 1)
+    kz = torch.fft.rfftfreq(n2, device="cuda").view(1, 1, -1)
+    k = torch.sqrt(kx * kx + ky * ky + kz * kz)
+    k[0, 0, 0] = 1.0
+    ampl = k.pow(-slope / 2.0)
+    ampl[0, 0, 0] = 0.0
+    del k
+    re = torch.randn(ampl.shape, generator=g, device="cuda")
+    im = torch.randn(ampl.shape, generator=g, device="cuda")
+    spec = torch.complex(re * ampl, im * ampl)
+    del re, im, ampl
+    f = torch.fft.irfftn(spec, s=shape)
+    del spec
+    f = (f - f.min()) / (f.max() - f.min())
+    f += noise * torch.randn(shape, generator=g, device="cuda")
+    if fronts:
+        i = torch.arange(n0, device="cuda", dtype=torch.float32).view(-1, 1, 1)
+        j = torch.arange(n1, device="cuda", dtype=torch.float32).view(1, -1, 1)
+        l = torch.arange(n2, device="cuda", dtype=torch.float32).view(1, 1, -1)
+        for _ in range(fronts):
+            nv = torch.randn(3, generator=g, device="cuda")
+            nv = nv / nv.norm()
+            pnt = torch.rand(3, generator=g, device="cuda") * torch.tensor(shape, device="cuda",
+                                                                           dtype=torch.float32)
+            c = pnt.dot(nv)
+            f += amp * torch.tanh((i * nv[0] + j * nv[1] + l * nv[2] - c) / width)
+    out = f.to(torch.float32).contiguous().view(-1)
+    torch.cuda.synchronize()
+    return out
+
+
+def cpu_baseline(x, dims, rel):
+    """Reference CPU compressor (oracle/_ref, built from the reference sources)
+    on this host, one thread; falls back to the oracle port when absent."""
+    import oracle
+    sample = "full %s field, REL %g, ALGO_INTERP cubic maxStep 32, SZ_compress incl. zstd, best of 2" % (
+        "x".join(map(str, dims)), rel)
+    host = x.cpu().numpy()
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "field.f32")
+        host.tofile(path)
+        n = host.size
+        if os.path.exists(oracle.REF_HARNESS):
+            out = subprocess.run([oracle.REF_HARNESS, "time", f"in={path}", "dims=" + ",".join(map(str, dims)),
+                                  "mode=rel", f"eb={rel!r}", "algo=1", "maxstep=32", "reps=2"],
+                                 capture_output=True, text=True, timeout=600)
+            if out.returncode == 0:
+                r = json.loads(out.stdout.strip().splitlines()[-1])
+                return {"value": round(4.0 * n / r["seconds"] / 1e9, 4), "unit": "GB/s", "cores": 1,
+                        "kind": "reference", "sample": sample,
+                        "ratio": round(4.0 * n / r["bytes"], 3)}
+    # oracle port: time one interpolation pass over a 256^3 crop (single thread)
+    crop = host.reshape(dims)[:256, :256, :256].copy()
+    mn, mx = oracle.data_range(crop)
+    p = oracle.make_params(crop.shape, rel * float((mx - mn)), algo=1, max_step=32)
+    t0 = time.perf_counter()
+    oracle.compress_stream(crop, p, rel, rel=True)
+    dt = time.perf_counter() - t0
+    return {"value": round(4.0 * crop.size / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "256^3 crop of the field, REL %g, oracle port incl. zstd" % rel}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rel", type=float, default=1e-3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the compression of one field on one MI355X; "
+              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import faz_amd
+
+    dims = (args.n, args.n, args.n)
+    n = args.n ** 3
+    x = make_field(dims, seed=args.seed)
+    eng = faz_amd.Engine(0)
+    p = faz_amd.make_params(dims, 0.0, algo=1, max_step=32)
+    recon = torch.empty_like(x)
+
+    for _ in range(args.warmup):
+        eng.compress_device(x, p, True, args.rel, recon=recon)
+    torch.cuda.synchronize()
+    eng.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        raw_size = eng.compress_device(x, p, True, args.rel, recon=recon)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof_ms, launches, points = eng.profile()
+    eng.set_profiling(False)
+    timings = eng.timings()
+
+    dt = (t1 - t0) / args.steps
+    value = 4.0 * n / dt / 1e9
+    avg_launch_ms = prof_ms / max(launches, 1)
+    achieved = PASS_BYTES_PER_POINT * points / (prof_ms / 1e3) / 1e9 if prof_ms > 0 else 0.0
+
+    # quality of the stream (outside the timed region)
+    mn, mx = eng.range(x)
+    eb = args.rel * float(torch.tensor(mx - mn, dtype=torch.float32).item())
+    stream = eng.sz_compress(x, p, args.rel, rel=True)
+    err = (recon.double() - x.double())
+    max_err = err.abs().max().item()
+    mse = (err * err).mean().item()
+    psnr = 20 * math.log10(float(mx - mn)) - 10 * math.log10(mse) if mse > 0 else float("inf")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded k^-11/3 Gaussian random field + 16 tanh fronts, generated on the GPU",
+        "config": {"workload": "single field 512x512x512 f32, REL %g, cubic interpolation, maxStep 32, "
+                               "interp+quantize+Huffman on GPU (ALGO_INTERP drop-in)" % args.rel,
+                   "dims": list(dims), "rel_eb": args.rel, "interp": "cubic", "max_step": 32},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "interp_pass_kernel<3>", "launches_per_step": launches // max(args.steps, 1),
+                     "avg_launch_ms": round(avg_launch_ms, 5),
+                     "bytes_per_point": PASS_BYTES_PER_POINT},
+        "quality": {"ratio": round(4.0 * n / len(stream), 3), "stream_bytes": len(stream), "raw_bytes": raw_size,
+                    "max_abs_err": max_err, "eb": eb, "psnr_db": round(psnr, 3)},
+        "phases_ms": {"range": round(timings[0], 4), "interp": round(timings[1], 4),
+                      "hist_outliers": round(timings[2], 4), "tree": round(timings[3], 4),
+                      "encode": round(timings[4], 4), "device_total": round(timings[5], 4)},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline(x, dims, args.rel)
+        except Exception as e:  # the baseline is a report, not the measurement
+            line["cpu_baseline"] = {"value": None, "error": str(e)[:200]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,331 @@
+"""Python bindings of the MI355X FAZ engine (ctypes over the C-ABI in include/faz_gpu.h).
+
+The compute path is the HIP library ``faz_amd/libfaz_amd.so``; importing this
+package never falls back to anything else.  PyTorch is used only to hold
+device memory (``tensor.data_ptr()`` is handed to the C-ABI).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfaz_amd.so")
+CLI_PATH = os.path.join(HERE, "faz")
+
+FAZ_OK = 0
+ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "unsupported", -4: "capacity", -5: "format"}
+
+
+class FazError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status}: {ERRORS.get(status, '?')})")
+        self.status = status
+
+
+class InterpParams(ctypes.Structure):
+    """Mirror of faz_interp_params."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("eb", ctypes.c_double),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("max_step", ctypes.c_size_t),
+        ("block_size", ctypes.c_int),
+        ("radius", ctypes.c_int),
+        ("levelwise", ctypes.c_int),
+        ("ops", ctypes.c_uint8 * 32),
+        ("dirs", ctypes.c_uint8 * 32),
+        ("tuning", ctypes.c_int),
+        ("start_level", ctypes.c_int),
+        ("end_level", ctypes.c_int),
+    ]
+
+
+class InterpOut(ctypes.Structure):
+    _fields_ = [
+        ("d_codes", ctypes.c_void_p),
+        ("n_codes", ctypes.c_size_t),
+        ("d_unpred", ctypes.c_void_p),
+        ("n_unpred", ctypes.c_size_t),
+        ("loss", ctypes.c_double),
+        ("level_counts", ctypes.c_size_t * 64),
+        ("levels", ctypes.c_int),
+    ]
+
+
+class HuffTable(ctypes.Structure):
+    _fields_ = [
+        ("offset", ctypes.c_int32),
+        ("n_states", ctypes.c_int32),
+        ("code", ctypes.c_void_p),
+        ("len", ctypes.c_void_p),
+    ]
+
+
+def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0, block_size=32,
+                radius=32768, ops=(), dirs=()):
+    p = InterpParams()
+    p.N = len(dims)
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    p.interp_algo = int(algo)
+    p.direction = int(direction)
+    p.alpha = float(alpha)
+    p.beta = float(beta)
+    p.max_step = int(max_step)
+    p.block_size = int(block_size)
+    p.radius = int(radius)
+    p.levelwise = len(ops)
+    for i, (o, d) in enumerate(zip(ops, dirs)):
+        p.ops[i] = int(o)
+        p.dirs[i] = int(d)
+    return p
+
+
+# Exported symbols declared in include/faz_gpu.h (checked by the CPU tests).
+EXPORTS = [
+    "faz_gpu_ctx_create", "faz_gpu_ctx_destroy", "faz_gpu_last_error", "faz_gpu_ctx_device",
+    "faz_gpu_range_f32", "faz_gpu_interp_compress_f32", "faz_gpu_huffman_hist", "faz_gpu_huffman_encode",
+    "faz_huffman_build", "faz_gpu_interp_decompress_raw", "faz_gpu_interp_raw_f32",
+    "faz_sz_compress_f32", "faz_sz_decompress_f32", "faz_free", "faz_gpu_last_timings",
+    "faz_gpu_compress_f32", "faz_gpu_fetch_raw", "faz_gpu_set_profiling", "faz_gpu_profile_read",
+]
+
+_lib = None
+_hip = None
+
+
+def lib():
+    """Loads libfaz_amd.so; raises when the HIP extension has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FazError(-2, f"{LIB_PATH} is missing: run __graft_entry__.build() (make -C faz_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        L.faz_gpu_ctx_create.argtypes = [ctypes.c_int, P(vp)]
+        L.faz_gpu_ctx_destroy.argtypes = [vp]
+        L.faz_gpu_ctx_destroy.restype = None
+        L.faz_gpu_last_error.argtypes = [vp]
+        L.faz_gpu_last_error.restype = ctypes.c_char_p
+        L.faz_gpu_ctx_device.argtypes = [vp]
+        L.faz_gpu_range_f32.argtypes = [vp, vp, sz, P(ctypes.c_float), P(ctypes.c_float)]
+        L.faz_gpu_interp_compress_f32.argtypes = [vp, vp, vp, P(InterpParams), P(InterpOut)]
+        L.faz_gpu_huffman_hist.argtypes = [vp, vp, sz, vp, vp]
+        L.faz_gpu_huffman_encode.argtypes = [vp, vp, sz, P(HuffTable), vp, sz, P(sz)]
+        L.faz_huffman_build.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_int32), vp, vp, vp, sz, P(sz)]
+        L.faz_gpu_interp_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
+        L.faz_gpu_interp_raw_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, P(sz)]
+        L.faz_sz_compress_f32.argtypes = [vp, P(InterpParams), ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
+                                          P(vp), P(sz)]
+        L.faz_sz_decompress_f32.argtypes = [vp, vp, sz, vp, sz, ctypes.c_int]
+        L.faz_free.argtypes = [vp]
+        L.faz_free.restype = None
+        L.faz_gpu_last_timings.argtypes = [vp, P(ctypes.c_double), ctypes.c_int]
+        L.faz_gpu_compress_f32.argtypes = [vp, vp, vp, P(InterpParams), ctypes.c_int, ctypes.c_double, P(sz)]
+        L.faz_gpu_fetch_raw.argtypes = [vp, vp, sz]
+        L.faz_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
+        L.faz_gpu_profile_read.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def hip():
+    """HIP runtime, for copying context-owned device arrays to the host."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipDeviceSynchronize.argtypes = []
+    return _hip
+
+
+def d2h(ptr, nbytes, dtype):
+    out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+    if nbytes:
+        rc = hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), nbytes, 2)
+        if rc != 0:
+            raise FazError(-2, f"hipMemcpy D2H failed ({rc})")
+    return out
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def huffman_build(freq, first):
+    """Host Huffman tree (faz_huffman_build): returns (offset, code, len, tree_bytes)."""
+    freq = np.ascontiguousarray(freq, np.uint32)
+    first = np.ascontiguousarray(first, np.uint32)
+    assert freq.size == 65536 and first.size == 65536
+    code = np.zeros(65536, np.uint64)
+    ln = np.zeros(65536, np.uint8)
+    off = ctypes.c_int32()
+    ns = ctypes.c_int32()
+    cap = 4 * 1024 * 1024
+    tree = np.zeros(cap, np.uint8)
+    tsz = ctypes.c_size_t()
+    rc = lib().faz_huffman_build(_np_ptr(freq), _np_ptr(first), ctypes.byref(off), ctypes.byref(ns),
+                                 _np_ptr(code), _np_ptr(ln), _np_ptr(tree), cap, ctypes.byref(tsz))
+    if rc:
+        raise FazError(rc, "faz_huffman_build")
+    return off.value, code[:ns.value].copy(), ln[:ns.value].copy(), tree[:tsz.value].tobytes()
+
+
+class Engine:
+    """One GPU context (stream + device scratch) of the C-ABI."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.faz_gpu_ctx_create(int(device), ctypes.byref(h))
+        if rc:
+            raise FazError(rc, f"faz_gpu_ctx_create(device={device})")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.faz_gpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self._L.faz_gpu_last_error(self.h)
+            raise FazError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    @staticmethod
+    def _dev(t):
+        import torch
+        assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()
+        return ctypes.c_void_p(t.data_ptr())
+
+    def range(self, x):
+        import torch
+        torch.cuda.synchronize()
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        self._check(self._L.faz_gpu_range_f32(self.h, self._dev(x), x.numel(), ctypes.byref(mn),
+                                              ctypes.byref(mx)), "range")
+        return mn.value, mx.value
+
+    def interp_compress(self, x, params, recon=None):
+        """Returns dict(codes uint16, unpred float32, recon tensor, levels)."""
+        import torch
+        if recon is None:
+            recon = torch.empty_like(x)
+        torch.cuda.synchronize()
+        out = InterpOut()
+        self._check(self._L.faz_gpu_interp_compress_f32(self.h, self._dev(x), self._dev(recon),
+                                                        ctypes.byref(params), ctypes.byref(out)),
+                    "interp_compress")
+        codes = d2h(out.d_codes, out.n_codes * 2, np.uint16)
+        unpred = d2h(out.d_unpred, out.n_unpred * 4, np.float32)
+        return dict(codes=codes, unpred=unpred, recon=recon, levels=out.levels, d_codes=out.d_codes,
+                    n_codes=out.n_codes)
+
+    def huffman_hist(self, d_codes, n):
+        freq = np.zeros(65536, np.uint32)
+        first = np.zeros(65536, np.uint32)
+        self._check(self._L.faz_gpu_huffman_hist(self.h, ctypes.c_void_p(d_codes), n, _np_ptr(freq),
+                                                 _np_ptr(first)), "huffman_hist")
+        return freq, first
+
+    def huffman_encode(self, d_codes, n, offset, code, ln, out_tensor):
+        t = HuffTable()
+        t.offset = offset
+        t.n_states = code.size
+        t.code = code.ctypes.data
+        t.len = ln.ctypes.data
+        nb = ctypes.c_size_t()
+        self._check(self._L.faz_gpu_huffman_encode(self.h, ctypes.c_void_p(d_codes), n, ctypes.byref(t),
+                                                   self._dev(out_tensor), out_tensor.numel() *
+                                                   out_tensor.element_size(), ctypes.byref(nb)),
+                    "huffman_encode")
+        return nb.value
+
+    def interp_raw(self, x, params, recon=None):
+        import torch
+        torch.cuda.synchronize()
+        need = ctypes.c_size_t()
+        rp = self._dev(recon) if recon is not None else None
+        rc = self._L.faz_gpu_interp_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params), None, 0,
+                                            ctypes.byref(need))
+        if rc != -4:
+            self._check(rc, "interp_raw")
+        buf = np.zeros(need.value, np.uint8)
+        self._check(self._L.faz_gpu_interp_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params),
+                                                   _np_ptr(buf), buf.size, ctypes.byref(need)), "interp_raw")
+        return buf[:need.value].tobytes()
+
+    def decompress_raw(self, raw, N, out):
+        import torch
+        b = np.frombuffer(raw, np.uint8).copy()
+        torch.cuda.synchronize()
+        self._check(self._L.faz_gpu_interp_decompress_raw(self.h, _np_ptr(b), b.size, int(N), self._dev(out),
+                                                          out.numel()), "decompress_raw")
+        return out
+
+    def sz_compress(self, data, params, eb, rel=True):
+        """SZ_compress stream of `data` (numpy on host, or a device tensor)."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        if isinstance(data, np.ndarray):
+            a = np.ascontiguousarray(data, np.float32)
+            ptr, ondev = _np_ptr(a), 0
+        else:
+            import torch
+            torch.cuda.synchronize()
+            ptr, ondev = self._dev(data), 1
+        self._check(self._L.faz_sz_compress_f32(self.h, ctypes.byref(params), 1 if rel else 0, float(eb), ptr,
+                                                ondev, ctypes.byref(p), ctypes.byref(n)), "sz_compress")
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            self._L.faz_free(p)
+
+    def sz_decompress(self, stream, num):
+        out = np.zeros(num, np.float32)
+        b = np.frombuffer(stream, np.uint8).copy()
+        self._check(self._L.faz_sz_decompress_f32(self.h, _np_ptr(b), b.size, _np_ptr(out), num, 0),
+                    "sz_decompress")
+        return out
+
+    def compress_device(self, x, params, rel, eb, recon=None):
+        """Device-resident compression; returns the engine buffer size."""
+        n = ctypes.c_size_t()
+        rp = self._dev(recon) if recon is not None else None
+        self._check(self._L.faz_gpu_compress_f32(self.h, self._dev(x), rp, ctypes.byref(params), 1 if rel else 0,
+                                                 float(eb), ctypes.byref(n)), "compress_device")
+        return n.value
+
+    def fetch_raw(self, size):
+        buf = np.zeros(size, np.uint8)
+        self._check(self._L.faz_gpu_fetch_raw(self.h, _np_ptr(buf), size), "fetch_raw")
+        return buf.tobytes()
+
+    def set_profiling(self, on):
+        self._check(self._L.faz_gpu_set_profiling(self.h, 1 if on else 0), "set_profiling")
+
+    def profile(self):
+        ms, la, pts = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._L.faz_gpu_profile_read(self.h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(pts)),
+                    "profile_read")
+        return ms.value, la.value, pts.value
+
+    def timings(self):
+        t = (ctypes.c_double * 8)()
+        self._L.faz_gpu_last_timings(self.h, t, 8)
+        return list(t)

@@ -1,0 +1,854 @@
+// Host orchestration of the gfx950 hot path behind the C-ABI (faz_gpu.h).
+//
+// The interpolation engine runs level by level and, within a level, pass by
+// pass: every (level, pass) is one data-parallel launch over all blocks,
+// because a pass only reads points fixed by earlier passes or coarser levels
+// (in the reference's sequential block order those are exactly the points
+// already written).  The host computes, per level, the number of codes each
+// block emits so every kernel thread can place its code at the position the
+// reference would push it (level, block raster, pass, line, stencil order).
+#include "faz_gpu.h"
+#include "faz_internal.h"
+#include "../host/huffman_host.h"
+#include "../host/engine_format.h"
+#include "../host/raw_api.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace faz;
+
+namespace {
+
+template<class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(n, 1);
+        hipError_t e = hipMalloc((void **) &p, want * sizeof(T));
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Engine buffer waiting in a context: everything but the device arrays.
+struct PendingRaw {
+    faz::EngineHeader h;
+    double eb = 0;
+    int radius = 0;
+    uint64_t n_unpred = 0;
+    faz::HuffTable tab;
+    size_t nbytes = 0;
+    bool valid = false;
+};
+
+}  // namespace
+
+struct faz_gpu_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevBuf<uint16_t> codes;
+    DevBuf<float> outval, unpred, work_in, work_out;
+    DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
+    DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code;
+    DevBuf<uint8_t> enc_len;
+    hipEvent_t ev[8] = {};
+    double timings[8] = {};
+    // histogram of the codes last produced by interp_compress
+    const uint16_t *hist_codes = nullptr;
+    size_t hist_n = 0;
+    std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
+    PendingRaw pending;
+    // per-launch timing of the interpolation pass kernels
+    bool profile = false;
+    double prof_ms = 0;
+    uint64_t prof_launches = 0, prof_points = 0;
+    std::vector<hipEvent_t> prof_ev;
+};
+
+#define CTX_FAIL(code, ...)                                          \
+    do {                                                             \
+        char _m[512];                                                \
+        snprintf(_m, sizeof(_m), __VA_ARGS__);                       \
+        ctx->err = _m;                                               \
+        return code;                                                 \
+    } while (0)
+
+#define HIPCHECK(x)                                                                               \
+    do {                                                                                          \
+        hipError_t _e = (x);                                                                      \
+        if (_e != hipSuccess) CTX_FAIL(FAZ_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_e)); \
+    } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Engine plan: SZInterpolationCompressor::init (SIC:820-852), the level-wise
+// error bounds and operators (SIC:384-422) and the per-level launches.
+// ---------------------------------------------------------------------------
+struct LevelPlan {
+    unsigned level;
+    double eb, recip;
+    int linear;
+    int dir;
+    uint64_t base;             // codes before this level
+    uint64_t codes;            // codes of this level
+    size_t prefix_at;          // offset into Plan::prefix
+    std::vector<PassDesc> passes;
+};
+
+struct Plan {
+    int N;
+    uint32_t dims[kMaxN];
+    uint64_t offs[kMaxN];
+    uint64_t num;
+    int levels;                // interpolation_level after init
+    bool anchor;
+    uint64_t head_codes;       // anchors or the first point
+    uint64_t anchor_count;
+    uint64_t total_codes;
+    std::vector<std::array<int, kMaxN>> perms;
+    std::vector<LevelPlan> lv;
+    std::vector<uint64_t> prefix;
+};
+
+int make_plan(const faz_interp_params *p, Plan &pl, std::string &why) {
+    pl.N = p->N;
+    if (pl.N < 1 || pl.N > 4) {
+        why = "N must be 1..4";
+        return FAZ_ERR_INVALID;
+    }
+    pl.num = 1;
+    for (int i = 0; i < pl.N; i++) {
+        if (p->dims[i] == 0 || p->dims[i] >= (1ull << 31)) {
+            why = "dimension out of range";
+            return FAZ_ERR_INVALID;
+        }
+        pl.dims[i] = (uint32_t) p->dims[i];
+        pl.num *= p->dims[i];
+    }
+    if (pl.num >= (1ull << 32)) {
+        why = "more than 2^32-1 points per field";
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    if (p->block_size < 2 || (p->block_size & 1)) {
+        why = "interpBlockSize must be even and >= 2";
+        return FAZ_ERR_INVALID;
+    }
+    if (p->radius < 1 || p->radius > 32768) {
+        why = "quantbinCnt/2 must be in 1..32768 (16-bit codes)";
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    if (!(p->eb > 0)) {
+        why = "error bound must be positive";
+        return FAZ_ERR_INVALID;
+    }
+    if (p->levelwise < 0 || p->levelwise > 32) {
+        why = "levelwise list too long";
+        return FAZ_ERR_INVALID;
+    }
+    if (p->tuning != 0) {
+        why = "tuning modes run through the batched tuner";
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    if (p->max_step > 0 && pl.N != 2 && pl.N != 3) {
+        why = "anchors (maxStep) exist only for 2-D and 3-D fields in the reference";
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    // init(): level count and anchors
+    int levels = -1;
+    for (int i = 0; i < pl.N; i++)
+        if (levels < std::ceil(std::log2((double) pl.dims[i]))) levels = (unsigned) std::ceil(std::log2((double) pl.dims[i]));
+    pl.anchor = false;
+    if (p->max_step > 0) {
+        pl.anchor = true;
+        levels += 1;
+        int max_level = (unsigned) std::log2((double) p->max_step) + 1;
+        if (max_level <= levels) levels = max_level;
+    }
+    pl.levels = levels;
+    pl.offs[pl.N - 1] = 1;
+    for (int i = pl.N - 2; i >= 0; i--) pl.offs[i] = pl.offs[i + 1] * pl.dims[i + 1];
+    pl.perms.clear();
+    std::array<int, kMaxN> seq = {0, 1, 2, 3};
+    do {
+        pl.perms.push_back(seq);
+    } while (std::next_permutation(seq.begin(), seq.begin() + pl.N));
+    for (int i = 0; i < p->levelwise; i++)
+        if (p->dirs[i] >= pl.perms.size() || p->ops[i] > 1) {
+            why = "levelwise operator/direction out of range";
+            return FAZ_ERR_INVALID;
+        }
+    if ((size_t) p->direction >= pl.perms.size() || p->interp_algo < 0 || p->interp_algo > 1) {
+        why = "interpolator or direction out of range";
+        return FAZ_ERR_INVALID;
+    }
+
+    // head: anchors (build_grid, SIC:854-900) or the first point
+    unsigned start = (unsigned) levels;
+    if (pl.anchor) {
+        uint64_t c = 1;
+        for (int i = 0; i < pl.N; i++) c *= (pl.dims[i] - 1) / p->max_step + 1;
+        pl.anchor_count = c;
+        pl.head_codes = c;
+        start--;
+    } else {
+        pl.anchor_count = 0;
+        pl.head_codes = 1;
+    }
+    unsigned stop = (unsigned) std::max(p->end_level, 0);
+    if (p->start_level > 0 && p->start_level <= levels) {
+        why = "partial level ranges run through the batched tuner";
+        return FAZ_ERR_UNSUPPORTED;
+    }
+
+    pl.lv.clear();
+    pl.prefix.clear();
+    uint64_t base = pl.head_codes;
+    const double eb = p->eb;
+    for (unsigned level = start; level > stop && level <= start; level--) {
+        LevelPlan L;
+        L.level = level;
+        if (p->alpha < 0) {
+            L.eb = level >= 3 ? eb * 0.5 : eb;
+        } else if (p->alpha >= 1) {
+            double r = std::pow(p->alpha, level - 1);
+            if (r > p->beta) r = p->beta;
+            L.eb = eb / r;
+        } else {
+            double r = 1 - (level - 1) * p->alpha;
+            if (r < p->beta) r = p->beta;
+            L.eb = eb * r;
+        }
+        L.recip = 1.0 / L.eb;
+        int interp = p->interp_algo, dir = p->direction;
+        if (p->levelwise > 0) {
+            int k = (int) level - 1 < p->levelwise ? (int) level - 1 : p->levelwise - 1;
+            interp = p->ops[k];
+            dir = p->dirs[k];
+        }
+        L.linear = interp == 0;
+        L.dir = dir;
+        const auto &perm = pl.perms[dir];
+        const uint64_t s = (uint64_t) 1 << (level - 1);
+        const uint64_t bs = (uint64_t) p->block_size * s;
+        if (bs >= (1ull << 31)) {
+            why = "block edge exceeds 2^31";
+            return FAZ_ERR_UNSUPPORTED;
+        }
+        // per-axis block tables
+        uint32_t nb[kMaxN] = {1, 1, 1, 1};
+        std::vector<uint64_t> tn[kMaxN], tg1[kMaxN], tg2[kMaxN];
+        uint64_t nblocks = 1;
+        for (int a = 0; a < pl.N; a++) {
+            nb[a] = (uint32_t) ((pl.dims[a] - 1) / bs + 1);
+            nblocks *= nb[a];
+            for (uint32_t k = 0; k < nb[a]; k++) {
+                uint64_t b = k * bs;
+                uint64_t e = std::min<uint64_t>(b + bs, pl.dims[a] - 1);
+                tn[a].push_back((e - b) / s + 1);
+                tg1[a].push_back(b ? (e - b) / s : e / s + 1);
+                tg2[a].push_back(b ? (e - b) / (2 * s) : e / (2 * s) + 1);
+            }
+        }
+        // codes per block in raster order → exclusive prefix
+        L.prefix_at = pl.prefix.size();
+        uint64_t run = 0;
+        uint32_t k[kMaxN] = {0, 0, 0, 0};
+        for (uint64_t blk = 0; blk < nblocks; blk++) {
+            pl.prefix.push_back(run);
+            uint64_t cnt = 0;
+            for (int P = 0; P < pl.N; P++) {
+                uint64_t c = tn[perm[P]][k[perm[P]]] / 2;
+                for (int Q = 0; Q < pl.N; Q++) {
+                    if (Q == P) continue;
+                    c *= Q < P ? tg1[perm[Q]][k[perm[Q]]] : tg2[perm[Q]][k[perm[Q]]];
+                }
+                cnt += c;
+            }
+            run += cnt;
+            for (int a = pl.N - 1; a >= 0; a--) {
+                if (++k[a] < nb[a]) break;
+                k[a] = 0;
+            }
+        }
+        L.base = base;
+        L.codes = run;
+        // one launch per pass over the pass lattice
+        uint64_t lattice_total = 0;
+        for (int P = 0; P < pl.N; P++) {
+            PassDesc d;
+            memset(&d, 0, sizeof(d));
+            d.N = pl.N;
+            d.P = P;
+            d.axis = perm[P];
+            for (int a = 0; a < kMaxN; a++) d.perm[a] = a < pl.N ? perm[a] : a;
+            uint64_t total = 1;
+            for (int a = 0; a < pl.N; a++) {
+                d.dims[a] = pl.dims[a];
+                d.offs[a] = pl.offs[a];
+                d.nb[a] = nb[a];
+                int Q = 0;
+                while (perm[Q] != a) Q++;
+                uint64_t M = (pl.dims[a] - 1) / s;
+                if (Q == P) {
+                    d.kind[a] = kPassAxis;
+                    d.lat[a] = (uint32_t) ((M + 1) / 2);
+                } else if (Q < P) {
+                    d.kind[a] = kG1;
+                    d.lat[a] = (uint32_t) (M + 1);
+                } else {
+                    d.kind[a] = kG2;
+                    d.lat[a] = (uint32_t) ((pl.dims[a] - 1) / (2 * s) + 1);
+                }
+                total *= d.lat[a];
+            }
+            d.s = (uint32_t) s;
+            d.bs = (uint32_t) bs;
+            d.total = total;
+            d.level_base = base;
+            d.eb = L.eb;
+            d.recip = L.recip;
+            d.radius = p->radius;
+            d.linear = L.linear;
+            d.tuning = 0;
+            lattice_total += total;
+            L.passes.push_back(d);
+        }
+        if (lattice_total != run) {
+            why = "internal: pass lattice and block counts disagree";
+            return FAZ_ERR_INVALID;
+        }
+        base += run;
+        pl.lv.push_back(L);
+    }
+    pl.total_codes = base;
+    return FAZ_OK;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
+    if (!out) return FAZ_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return FAZ_ERR_HIP;
+    if (device < 0 || device >= count) return FAZ_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return FAZ_ERR_HIP;
+    faz_gpu_ctx *ctx = new faz_gpu_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return FAZ_ERR_HIP;
+    }
+    for (auto &e: ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            delete ctx;
+            return FAZ_ERR_HIP;
+        }
+    }
+    *out = ctx;
+    return FAZ_OK;
+}
+
+void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
+    if (!ctx) return;
+    (void) hipSetDevice(ctx->device);
+    (void) hipStreamSynchronize(ctx->stream);
+    ctx->codes.release();
+    ctx->outval.release();
+    ctx->unpred.release();
+    ctx->work_in.release();
+    ctx->work_out.release();
+    ctx->freq.release();
+    ctx->first.release();
+    ctx->chunk_zeros.release();
+    ctx->err_flag.release();
+    ctx->minmax.release();
+    ctx->words.release();
+    ctx->chunk_base.release();
+    ctx->scalars.release();
+    ctx->prefix.release();
+    ctx->chunk_bits.release();
+    ctx->chunk_off.release();
+    ctx->enc_code.release();
+    ctx->enc_len.release();
+    for (auto &e: ctx->ev)
+        if (e) (void) hipEventDestroy(e);
+    for (auto &e: ctx->prof_ev) (void) hipEventDestroy(e);
+    (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *faz_gpu_last_error(const faz_gpu_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int faz_gpu_ctx_device(const faz_gpu_ctx *ctx) { return ctx ? ctx->device : -1; }
+
+faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n) {
+    if (!ctx || !ms) return FAZ_ERR_INVALID;
+    for (int i = 0; i < n && i < 8; i++) ms[i] = ctx->timings[i];
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, float *h_min, float *h_max) {
+    if (!ctx || !d_in || !n || !h_min || !h_max) return FAZ_ERR_INVALID;
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(ctx->minmax.ensure(2));
+    launch_range(d_in, n, ctx->minmax.p, ctx->stream);
+    HIPCHECK(hipGetLastError());
+    uint32_t k[2];
+    HIPCHECK(hipMemcpyAsync(k, ctx->minmax.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    auto key2f = [](uint32_t key) {
+        uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    };
+    *h_min = key2f(k[0]);
+    *h_max = key2f(k[1]);
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
+                                       const faz_interp_params *p, faz_interp_out *out) {
+    if (!ctx || !d_in || !d_recon || !p || !out) return FAZ_ERR_INVALID;
+    Plan pl;
+    int rc = make_plan(p, pl, ctx->err);
+    if (rc) return rc;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    HIPCHECK(ctx->codes.ensure(pl.total_codes));
+    HIPCHECK(ctx->outval.ensure(pl.total_codes));
+    HIPCHECK(ctx->err_flag.ensure(1));
+    HIPCHECK(ctx->prefix.ensure(pl.prefix.size()));
+    HIPCHECK(ctx->scalars.ensure(4));
+    const uint32_t nchunks = (uint32_t) ((pl.total_codes + kHistChunk - 1) / kHistChunk);
+    HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
+    HIPCHECK(ctx->chunk_base.ensure(nchunks));
+    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->first.ensure(65536));
+    HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
+    launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
+
+    HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    if (pl.anchor) {
+        launch_anchor_grid(d_in, d_recon, ctx->codes.p, ctx->outval.p, pl.N, pl.dims, pl.offs,
+                           (uint32_t) p->max_step, pl.anchor_count, st);
+    } else {
+        double eb0 = p->eb;
+        launch_first_point(d_in, d_recon, ctx->codes.p, ctx->outval.p, eb0, 1.0 / eb0, p->radius, ctx->err_flag.p, st);
+    }
+    size_t npass = 0;
+    for (auto &L: pl.lv) npass += L.passes.size();
+    if (ctx->profile && ctx->prof_ev.size() < 2 * npass) {
+        size_t have = ctx->prof_ev.size();
+        ctx->prof_ev.resize(2 * npass);
+        for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
+    }
+    size_t k = 0;
+    for (auto &L: pl.lv) {
+        for (auto &d: L.passes) {
+            d.block_prefix = ctx->prefix.p + L.prefix_at;
+            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            launch_interp_pass(d, d_in, d_recon, ctx->codes.p, ctx->outval.p, nullptr, ctx->err_flag.p, st);
+            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+            k++;
+        }
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    // histogram, first positions and outlier compaction
+    launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st);
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->scalars.p, st);
+    HIPCHECK(hipGetLastError());
+    uint64_t n_unpred = 0;
+    uint32_t errflag = 0;
+    HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&errflag, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
+    HIPCHECK(ctx->unpred.ensure(n_unpred));
+    launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
+                         ctx->unpred.p, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[2], st));
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
+    ctx->timings[6] = ctx->timings[1];
+    if (ctx->profile) {
+        size_t j = 0;
+        for (auto &L: pl.lv)
+            for (auto &d: L.passes) {
+                if (d.total) {
+                    ctx->prof_ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
+                    ctx->prof_launches++;
+                    ctx->prof_points += d.total;
+                }
+                j++;
+            }
+    }
+    ctx->hist_codes = ctx->codes.p;
+    ctx->hist_n = pl.total_codes;
+
+    out->d_codes = ctx->codes.p;
+    out->n_codes = pl.total_codes;
+    out->d_unpred = ctx->unpred.p;
+    out->n_unpred = n_unpred;
+    out->loss = 0;
+    memset(out->level_counts, 0, sizeof(out->level_counts));
+    out->levels = pl.levels;
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, uint32_t *h_freq,
+                                uint32_t *h_first) {
+    if (!ctx || !d_codes || !n || !h_freq || !h_first) return FAZ_ERR_INVALID;
+    if (n >= (1ull << 32)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "more than 2^32-1 codes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    if (!(ctx->hist_codes == d_codes && ctx->hist_n == n)) {
+        const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+        HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
+        HIPCHECK(ctx->freq.ensure(65536));
+        HIPCHECK(ctx->first.ensure(65536));
+        launch_hist(d_codes, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
+        HIPCHECK(hipGetLastError());
+        ctx->hist_codes = d_codes;
+        ctx->hist_n = n;
+    }
+    HIPCHECK(hipMemcpyAsync(h_freq, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(h_first, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return FAZ_OK;
+}
+
+faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, int32_t *offset, int32_t *n_states,
+                             uint64_t *code, uint8_t *len, uint8_t *tree, size_t tree_cap, size_t *tree_size) {
+    if (!h_freq || !h_first || !offset || !n_states || !code || !len || !tree_size) return FAZ_ERR_INVALID;
+    HuffTable t;
+    if (build_huffman(h_freq, h_first, 65536, t)) return FAZ_ERR_INVALID;
+    *offset = t.offset;
+    *n_states = t.n_states;
+    memcpy(code, t.code.data(), t.code.size() * 8);
+    memcpy(len, t.len.data(), t.len.size());
+    *tree_size = t.tree.size();
+    if (!tree || t.tree.size() > tree_cap) return FAZ_ERR_CAPACITY;
+    memcpy(tree, t.tree.data(), t.tree.size());
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, const faz_huff_table *t,
+                                  uint8_t *d_bits, size_t cap_bytes, size_t *nbytes) {
+    if (!ctx || !d_codes || !n || !t || !nbytes || !t->code || !t->len || t->n_states <= 0) return FAZ_ERR_INVALID;
+    if (n >= (1ull << 32)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "more than 2^32-1 codes");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    HIPCHECK(ctx->enc_code.ensure((size_t) t->n_states));
+    HIPCHECK(ctx->enc_len.ensure((size_t) t->n_states));
+    HIPCHECK(ctx->chunk_bits.ensure(nchunks));
+    HIPCHECK(ctx->chunk_off.ensure(nchunks));
+    HIPCHECK(ctx->scalars.ensure(4));
+    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, t->code, (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ctx->enc_len.p, t->len, (size_t) t->n_states, hipMemcpyHostToDevice, st));
+    launch_encode_sizes(d_codes, n, ctx->enc_len.p, t->offset, ctx->chunk_bits.p, st);
+    launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->scalars.p + 1, st);
+    HIPCHECK(hipGetLastError());
+    uint64_t bits = 0;
+    HIPCHECK(hipMemcpyAsync(&bits, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const size_t bytes = (size_t) ((bits + 7) / 8);
+    const size_t words = (size_t) ((bits + 31) / 32);
+    *nbytes = bytes;
+    if (!d_bits || cap_bytes < words * 4) CTX_FAIL(FAZ_ERR_CAPACITY, "bit buffer needs %zu bytes", words * 4);
+    uint32_t *w = reinterpret_cast<uint32_t *>(d_bits);
+    if (((uintptr_t) d_bits & 3) != 0) CTX_FAIL(FAZ_ERR_INVALID, "bit buffer must be 4-byte aligned");
+    launch_fill_u32(w, words, 0u, st);
+    launch_encode_pack(d_codes, n, ctx->enc_code.p, ctx->enc_len.p, t->offset, ctx->chunk_off.p, w, st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    return FAZ_OK;
+}
+
+}  // extern "C"
+
+float *faz_ctx_work_in(faz_gpu_ctx *ctx, size_t n) {
+    if (hipSetDevice(ctx->device) != hipSuccess || ctx->work_in.ensure(n) != hipSuccess) return nullptr;
+    return ctx->work_in.p;
+}
+
+hipStream_t faz_ctx_stream(faz_gpu_ctx *ctx) { return ctx->stream; }
+
+void faz_ctx_set_error(faz_gpu_ctx *ctx, const char *msg) { ctx->err = msg; }
+
+// Device part of the engine buffer: interpolation + quantization, histogram
+// and outliers, host tree, encode.  Results stay in the context until fetched.
+static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p) {
+    if (!ctx || !d_in || !p) return FAZ_ERR_INVALID;
+    ctx->pending.valid = false;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    float *recon = d_recon;
+    if (!recon) {
+        size_t num = 1;
+        for (int i = 0; i < p->N && i < 4; i++) num *= p->dims[i];
+        HIPCHECK(ctx->work_out.ensure(num));
+        recon = ctx->work_out.p;
+    }
+    faz_interp_out io;
+    int rc = faz_gpu_interp_compress_f32(ctx, d_in, recon, p, &io);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(ctx->ev[3], st));
+    HIPCHECK(hipMemcpyAsync(ctx->h_freq.data(), ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(ctx->h_first.data(), ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    PendingRaw &pr = ctx->pending;
+    if (build_huffman(ctx->h_freq.data(), ctx->h_first.data(), 65536, pr.tab))
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
+    HIPCHECK(hipEventRecord(ctx->ev[4], st));
+    faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
+    uint64_t maxbits = 0;
+    for (auto l: pr.tab.len) maxbits = std::max<uint64_t>(maxbits, l);
+    size_t words_cap = (size_t) ((io.n_codes * maxbits + 31) / 32 + 2);
+    HIPCHECK(ctx->words.ensure(words_cap));
+    size_t nbytes = 0;
+    rc = faz_gpu_huffman_encode(ctx, io.d_codes, io.n_codes, &ht, reinterpret_cast<uint8_t *>(ctx->words.p),
+                                ctx->words.cap * 4, &nbytes);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(ctx->ev[5], st));
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->timings[3] = elapsed(ctx->ev[3], ctx->ev[4]);
+    ctx->timings[4] = elapsed(ctx->ev[4], ctx->ev[5]);
+    ctx->timings[5] = elapsed(ctx->ev[0], ctx->ev[5]);
+    // engine buffer header (SIC:489-512)
+    EngineHeader &h = pr.h;
+    h.N = p->N;
+    for (int i = 0; i < p->N; i++) h.dims[i] = p->dims[i];
+    h.block_size = (unsigned) p->block_size;
+    h.interp_algo = p->interp_algo;
+    h.direction = p->direction;
+    h.alpha = p->alpha;
+    h.beta = p->beta;
+    h.max_step = p->max_step;
+    h.levelwise = p->levelwise;
+    h.ops.assign(p->ops, p->ops + p->levelwise);
+    h.dirs.assign(p->dirs, p->dirs + p->levelwise);
+    pr.eb = p->eb;
+    pr.radius = p->radius;
+    pr.n_unpred = io.n_unpred;
+    pr.nbytes = nbytes;
+    pr.valid = true;
+    return FAZ_OK;
+}
+
+static size_t pending_size(const faz_gpu_ctx *ctx) {
+    const PendingRaw &pr = ctx->pending;
+    return engine_raw_size(pr.h, pr.n_unpred, pr.tab.tree.size(), pr.nbytes);
+}
+
+// Host copy of the pending engine buffer (SIC:489-518, Q:185-201, HUF:119-227).
+static int fetch_raw(faz_gpu_ctx *ctx, uint8_t *out) {
+    const PendingRaw &pr = ctx->pending;
+    hipStream_t st = ctx->stream;
+    uint8_t *w = out;
+    write_engine_header(pr.h, w);
+    write_quantizer_head(pr.eb, pr.radius, pr.n_unpred, w);
+    HIPCHECK(hipMemcpyAsync(w, ctx->unpred.p, pr.n_unpred * sizeof(float), hipMemcpyDeviceToHost, st));
+    w += pr.n_unpred * sizeof(float);
+    memcpy(w, pr.tab.tree.data(), pr.tab.tree.size());
+    w += pr.tab.tree.size();
+    uint64_t nb64 = pr.nbytes;
+    memcpy(w, &nb64, 8);
+    w += 8;
+    HIPCHECK(hipMemcpyAsync(w, ctx->words.p, pr.nbytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return FAZ_OK;
+}
+
+int faz_compress_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                         std::vector<uint8_t> &raw) {
+    int rc = compress_device(ctx, d_in, d_recon, p);
+    if (rc) return rc;
+    raw.resize(pending_size(ctx));
+    return fetch_raw(ctx, raw.data());
+}
+
+extern "C" {
+
+faz_status faz_gpu_interp_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                                  uint8_t *h_raw, size_t cap, size_t *raw_size) {
+    if (!ctx || !d_in || !p || !raw_size) return FAZ_ERR_INVALID;
+    std::vector<uint8_t> raw;
+    int rc = faz_compress_raw_vec(ctx, d_in, d_recon, p, raw);
+    if (rc) return rc;
+    *raw_size = raw.size();
+    if (!h_raw || cap < raw.size()) CTX_FAIL(FAZ_ERR_CAPACITY, "raw buffer needs %zu bytes", raw.size());
+    memcpy(h_raw, raw.data(), raw.size());
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                                int rel_mode, double eb, size_t *raw_size) {
+    if (!ctx || !d_in || !p || !raw_size || p->N < 1 || p->N > 4) return FAZ_ERR_INVALID;
+    faz_interp_params q = *p;
+    ctx->timings[0] = 0;
+    if (rel_mode) {
+        size_t num = 1;
+        for (int i = 0; i < p->N; i++) num *= p->dims[i];
+        HIPCHECK(hipSetDevice(ctx->device));
+        HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
+        float mn, mx;
+        int rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx);
+        if (rc) return rc;
+        HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
+        HIPCHECK(hipEventSynchronize(ctx->ev[7]));
+        ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);
+        q.eb = eb * (double) (float) (mx - mn);
+    } else {
+        q.eb = eb;
+    }
+    int rc = compress_device(ctx, d_in, d_recon, &q);
+    if (rc) return rc;
+    *raw_size = pending_size(ctx);
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_fetch_raw(faz_gpu_ctx *ctx, uint8_t *h_raw, size_t cap) {
+    if (!ctx || !ctx->pending.valid) return FAZ_ERR_INVALID;
+    size_t need = pending_size(ctx);
+    if (!h_raw || cap < need) CTX_FAIL(FAZ_ERR_CAPACITY, "raw buffer needs %zu bytes", need);
+    HIPCHECK(hipSetDevice(ctx->device));
+    return fetch_raw(ctx, h_raw);
+}
+
+faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on) {
+    if (!ctx) return FAZ_ERR_INVALID;
+    ctx->profile = on != 0;
+    ctx->prof_ms = 0;
+    ctx->prof_launches = 0;
+    ctx->prof_points = 0;
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points) {
+    if (!ctx || !ms || !launches || !points) return FAZ_ERR_INVALID;
+    *ms = ctx->prof_ms;
+    *launches = ctx->prof_launches;
+    *points = ctx->prof_points;
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
+                                         size_t num) {
+    if (!ctx || !h_raw || !d_out || N < 1 || N > 4) return FAZ_ERR_INVALID;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    EngineHeader h;
+    const uint8_t *pos = h_raw;
+    if (!read_engine_header(pos, n, N, h)) CTX_FAIL(FAZ_ERR_FORMAT, "engine header truncated or unsupported");
+    size_t total = 1;
+    for (int i = 0; i < N; i++) total *= h.dims[i];
+    if (total != num) CTX_FAIL(FAZ_ERR_INVALID, "stream holds %zu points, caller expects %zu", total, num);
+    double eb;
+    int radius;
+    uint64_t U;
+    if (!read_quantizer_head(pos, n - (pos - h_raw), eb, radius, U)) CTX_FAIL(FAZ_ERR_FORMAT, "quantizer truncated");
+    if ((size_t) (n - (pos - h_raw)) < U * sizeof(float)) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
+    const float *unpred_h = reinterpret_cast<const float *>(pos);
+    pos += U * sizeof(float);
+    std::vector<int32_t> codes;
+    if (!decode_huffman(pos, n - (pos - h_raw), codes, num)) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman section malformed");
+    faz_interp_params p;
+    memset(&p, 0, sizeof(p));
+    p.N = N;
+    for (int i = 0; i < N; i++) p.dims[i] = h.dims[i];
+    p.eb = eb;
+    p.interp_algo = h.interp_algo;
+    p.direction = h.direction;
+    p.alpha = h.alpha;
+    p.beta = h.beta;
+    p.max_step = h.max_step;
+    p.block_size = (int) h.block_size;
+    p.radius = radius;
+    p.levelwise = h.levelwise;
+    memcpy(p.ops, h.ops.data(), h.ops.size());
+    memcpy(p.dirs, h.dirs.data(), h.dirs.size());
+    Plan pl;
+    int rc = make_plan(&p, pl, ctx->err);
+    if (rc) return rc;
+    std::vector<uint16_t> c16(num);
+    uint64_t zeros = 0;
+    for (size_t i = 0; i < num; i++) {
+        if (codes[i] < 0 || codes[i] > 65535) CTX_FAIL(FAZ_ERR_FORMAT, "code out of 16-bit range");
+        c16[i] = (uint16_t) codes[i];
+        zeros += c16[i] == 0;
+    }
+    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
+                             (unsigned long long) U, (unsigned long long) zeros);
+    const uint32_t nchunks = (uint32_t) ((num + kHistChunk - 1) / kHistChunk);
+    HIPCHECK(ctx->codes.ensure(num));
+    HIPCHECK(ctx->unpred.ensure(U));
+    HIPCHECK(ctx->outval.ensure(num));
+    HIPCHECK(ctx->prefix.ensure(pl.prefix.size()));
+    HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
+    HIPCHECK(ctx->chunk_base.ensure(nchunks));
+    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->first.ensure(65536));
+    HIPCHECK(ctx->scalars.ensure(4));
+    HIPCHECK(hipMemcpyAsync(ctx->codes.p, c16.data(), num * 2, hipMemcpyHostToDevice, st));
+    if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
+    launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->scalars.p, st);
+    launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
+    ctx->hist_codes = nullptr;
+    // head: anchors / first point come straight from the unpredictable list
+    if (pl.anchor) {
+        // the anchors are the first anchor_count codes (all zero) and take the
+        // leading unpredictable values
+        if (U < pl.anchor_count) CTX_FAIL(FAZ_ERR_FORMAT, "fewer unpredictable values than anchors");
+        launch_anchor_scatter(ctx->unpred.p, d_out, N, pl.dims, pl.offs, (uint32_t) h.max_step, pl.anchor_count, st);
+    } else {
+        float v0 = c16[0] ? (float) ((double) 0.0f + (double) (2 * ((int) c16[0] - radius)) * eb) : unpred_h[0];
+        HIPCHECK(hipMemcpyAsync(d_out, &v0, 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    for (auto &L: pl.lv)
+        for (auto &d: L.passes) {
+            d.block_prefix = ctx->prefix.p + L.prefix_at;
+            launch_recover_pass(d, d_out, ctx->codes.p, ctx->outval.p, st);
+        }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    return FAZ_OK;
+}
+
+}  // extern "C"

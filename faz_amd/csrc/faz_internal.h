@@ -1,0 +1,78 @@
+// Internal interfaces shared by the HIP kernels (faz_kernels.hip) and the
+// host orchestration (faz_gpu.hip, host/*.cpp).  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace faz {
+
+// Largest supported rank and level count.
+constexpr int kMaxN = 4;
+constexpr int kMaxLevels = 64;
+
+// Axis roles inside one interpolation pass (SIC:3143-3185 generalised to N):
+// the pass axis carries the odd multiples of the stride, axes earlier in the
+// permutation run over all multiples of the stride (G1), later ones over
+// multiples of twice the stride (G2).
+enum AxisKind : uint8_t { kPassAxis = 0, kG1 = 1, kG2 = 2 };
+
+// One (level, pass) launch of the interpolation predictor.
+struct PassDesc {
+    int N;
+    int P;                    // pass number within the block (0 = first)
+    int axis;                 // natural axis interpolated along
+    int perm[kMaxN];          // dimension_sequences[direction]
+    uint32_t dims[kMaxN];
+    uint64_t offs[kMaxN];     // row-major element strides
+    uint32_t lat[kMaxN];      // lattice extent per natural axis
+    uint8_t kind[kMaxN];      // AxisKind per natural axis
+    uint32_t nb[kMaxN];       // blocks per natural axis at this level
+    uint32_t s;               // stride 2^(level-1)
+    uint32_t bs;              // block edge interpBlockSize * stride
+    uint64_t total;           // lattice points (= codes of this pass)
+    uint64_t level_base;      // codes pushed before this level
+    const uint64_t *block_prefix;  // codes of earlier blocks in raster order
+    double eb, recip;         // level error bound and 1.0 / eb
+    int radius;
+    int linear;               // 1: linear interpolator, 0: cubic
+    int tuning;               // 0 / 1 / 2
+};
+
+// Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
+void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st);
+void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
+                        const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st);
+void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint32_t *dims, const uint64_t *offs,
+                           uint32_t step, uint64_t count, hipStream_t st);
+void launch_first_point(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, double eb,
+                        double recip, int radius, uint32_t *d_err, hipStream_t st);
+void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
+                        double *d_loss, uint32_t *d_err, hipStream_t st);
+void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_codes, const float *d_unpred_at,
+                         hipStream_t st);
+void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
+                 int radius, hipStream_t st);
+void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_total,
+                      hipStream_t st);
+void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
+                          const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st);
+void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint8_t *d_len, int32_t offset,
+                         uint64_t *d_chunk_bits, hipStream_t st);
+void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_total, hipStream_t st);
+void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_code, const uint8_t *d_len,
+                        int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st);
+void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st);
+void launch_bswap_words(uint32_t *d, size_t count, hipStream_t st);
+void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
+                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st);
+
+// Chunk geometry of the histogram / outlier / encode kernels.
+constexpr uint32_t kHistThreads = 256;
+constexpr uint32_t kHistPerThread = 64;
+constexpr uint32_t kHistChunk = kHistThreads * kHistPerThread;  // codes per workgroup
+constexpr uint32_t kEncThreads = 256;
+constexpr uint32_t kEncPerThread = 16;
+constexpr uint32_t kEncChunk = kEncThreads * kEncPerThread;
+
+}  // namespace faz

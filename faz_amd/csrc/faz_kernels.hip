@@ -1,0 +1,680 @@
+// gfx950 kernels of the FAZ compression hot path.
+//
+// Bit-exactness with the reference x86-64 build: no FMA contraction (this
+// file is compiled with -ffp-contract=off and the pragma below), FP32
+// denormals preserved, every float/double operation in the reference's order.
+#include "faz_internal.h"
+
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#pragma clang fp contract(off)
+
+namespace faz {
+
+// ---------------------------------------------------------------------------
+// Scalar pieces shared by the pass kernels
+// ---------------------------------------------------------------------------
+
+// (int) of a non-negative double the way x86-64 cvttsd2si converts it: values
+// beyond INT_MAX (and NaN) become INT_MIN (IntegerQuantizer.hpp:81 relies on it).
+__device__ __forceinline__ int32_t x86_trunc_i32(double v) {
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t) v : INT32_MIN;
+}
+
+// LinearQuantizer::quantize_and_overwrite with trimToZero == 0
+// (IntegerQuantizer.hpp:69-108).  Returns the code; *rec gets the value the
+// reference leaves in its data buffer (the original when unpredictable).
+__device__ __forceinline__ int quantize_point(float orig, float pred, double eb, double recip, int radius,
+                                              float *rec) {
+    float diff = orig - pred;
+    uint32_t qi = (uint32_t) x86_trunc_i32((double) fabsf(diff) * recip) + 1u;
+    if ((int32_t) qi < radius * 2) {
+        int32_t q = ((int32_t) qi) >> 1;
+        int32_t half = q;
+        uint32_t qs = (uint32_t) q << 1;
+        int32_t shifted;
+        if (diff < 0) {
+            qs = 0u - qs;
+            shifted = radius - half;
+        } else {
+            shifted = radius + half;
+        }
+        float dec = (float) ((double) pred + (double) (int32_t) qs * eb);
+        if ((double) fabsf(dec - orig) > eb) {
+            *rec = orig;
+            return 0;
+        }
+        *rec = dec;
+        return shifted;
+    }
+    *rec = orig;
+    return 0;
+}
+
+// LinearQuantizer::recover (IntegerQuantizer.hpp:160-175)
+__device__ __forceinline__ float recover_point(float pred, int code, int radius, double eb) {
+    return (float) ((double) pred + (double) (2 * (code - radius)) * eb);
+}
+
+// Interpolators.hpp:9-36, evaluated in float left to right; linear1 in double.
+__device__ __forceinline__ float i_linear(float a, float b) { return (a + b) / 2; }
+__device__ __forceinline__ float i_linear1(float a, float b) { return (float) (-0.5 * (double) a + 1.5 * (double) b); }
+__device__ __forceinline__ float i_quad1(float a, float b, float c) { return (3 * a + 6 * b - c) / 8; }
+__device__ __forceinline__ float i_quad2(float a, float b, float c) { return (-a + 6 * b + 3 * c) / 8; }
+__device__ __forceinline__ float i_quad3(float a, float b, float c) { return (3 * a - 10 * b + 15 * c) / 8; }
+__device__ __forceinline__ float i_cubic(float a, float b, float c, float d) { return (-a + 9 * b + 9 * c - d) / 16; }
+
+// Where one predicted point sits in the reference's push order and which
+// stencil it uses (block_interpolation_1d, SIC:1306-1417): the line holds n
+// points, the point is at odd local position i.
+struct PointSlot {
+    uint64_t seq;   // index in the code stream
+    uint32_t i, n;  // local position / line length along the pass axis
+    uint32_t exit;  // cubic: position predicted with quad_2
+};
+
+template<int N>
+__device__ __forceinline__ bool locate(const PassDesc &d, uint32_t t, uint32_t *x, PointSlot &ps) {
+    // lattice coordinates, last natural axis fastest
+    uint32_t r = t;
+#pragma unroll
+    for (int a = N - 1; a >= 0; --a) {
+        uint32_t l = (a == 0) ? r : r % d.lat[a];
+        if (a) r /= d.lat[a];
+        uint32_t k = d.kind[a];
+        x[a] = (k == kPassAxis) ? d.s * (2 * l + 1) : (k == kG1 ? d.s * l : 2 * d.s * l);
+    }
+    // owning block along each axis (multi_dimensional_range raster, IT:361-453;
+    // a G1/G2 coordinate equal to a block start belongs to the previous block)
+    uint32_t kb[N], b[N], nn[N], g1[N], g2[N];
+    uint32_t kl = 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        uint32_t xa = x[a];
+        uint32_t k = (d.kind[a] == kPassAxis) ? xa / d.bs : (xa ? (xa - 1) / d.bs : 0);
+        kb[a] = k;
+        b[a] = k * d.bs;
+        uint32_t e = min(b[a] + d.bs, d.dims[a] - 1);
+        nn[a] = (e - b[a]) / d.s + 1;
+        g1[a] = b[a] ? (e - b[a]) / d.s : e / d.s + 1;
+        g2[a] = b[a] ? (e - b[a]) / (2 * d.s) : e / (2 * d.s) + 1;
+        kl = kl * d.nb[a] + k;
+    }
+    // codes of the earlier passes of this block
+    uint64_t off = 0;
+    for (int Pp = 0; Pp < d.P; ++Pp) {
+        uint64_t c = nn[d.perm[Pp]] / 2;
+#pragma unroll
+        for (int Q = 0; Q < N; ++Q) {
+            if (Q == Pp) continue;
+            c *= (Q < Pp) ? g1[d.perm[Q]] : g2[d.perm[Q]];
+        }
+        off += c;
+    }
+    // line index: outer axes in permutation order, earlier ones outer
+    uint64_t line = 0;
+#pragma unroll
+    for (int Q = 0; Q < N; ++Q) {
+        if (Q == d.P) continue;
+        int ax = d.perm[Q];
+        uint32_t idx, size;
+        if (Q < d.P) {
+            idx = b[ax] ? (x[ax] - b[ax]) / d.s - 1 : x[ax] / d.s;
+            size = g1[ax];
+        } else {
+            idx = b[ax] ? (x[ax] - b[ax]) / (2 * d.s) - 1 : x[ax] / (2 * d.s);
+            size = g2[ax];
+        }
+        line = line * size + idx;
+    }
+    const int ax = d.axis;
+    uint32_t i = (x[ax] - b[ax]) / d.s;
+    uint32_t n = nn[ax];
+    uint32_t rank, exit = 0;
+    if (d.linear || n < 5) {
+        rank = (i - 1) / 2;
+    } else {
+        uint32_t m = (n - 3) / 2 - 1;  // interior cubic points
+        exit = 3 + 2 * m;
+        if (i == 1)
+            rank = m;
+        else if (i == exit)
+            rank = m + 1;
+        else if (i == n - 1)
+            rank = m + 2;
+        else
+            rank = (i - 3) / 2;
+    }
+    ps.seq = d.level_base + d.block_prefix[kl] + off + line * (uint64_t) (n / 2) + rank;
+    ps.i = i;
+    ps.n = n;
+    ps.exit = exit;
+    return true;
+}
+
+__device__ __forceinline__ float predict(const float *v, uint64_t f, uint64_t S, const PointSlot &ps,
+                                         int linear) {
+    const uint32_t i = ps.i, n = ps.n;
+    if (linear || n < 5) {
+        if (i + 1 < n) return i_linear(v[f - S], v[f + S]);
+        if (n < 4) return v[f - S];
+        return i_linear1(v[f - 3 * S], v[f - S]);
+    }
+    if (i == 1) return i_quad1(v[f - S], v[f + S], v[f + 3 * S]);
+    if (i == ps.exit) return i_quad2(v[f - 3 * S], v[f - S], v[f + S]);
+    if (i == n - 1) return i_quad3(v[f - 5 * S], v[f - 3 * S], v[f - S]);
+    return i_cubic(v[f - 3 * S], v[f - S], v[f + S], v[f + 3 * S]);
+}
+
+// ---------------------------------------------------------------------------
+// One (level, pass) of the interpolation predictor: every lattice point of
+// the pass is independent, so the launch covers all blocks at once; each point
+// lands at its closed-form position in the code stream.
+// ---------------------------------------------------------------------------
+template<int N>
+__global__ void __launch_bounds__(256) interp_pass_kernel(PassDesc d, const float *in, float *recon,
+                                                          uint16_t *__restrict__ codes,
+                                                          float *__restrict__ outval, uint32_t *__restrict__ err) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.total) return;
+    uint32_t x[N];
+    PointSlot ps;
+    locate<N>(d, t, x, ps);
+    uint64_t f = 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a) f += (uint64_t) x[a] * d.offs[a];
+    const uint64_t S = (uint64_t) d.s * d.offs[d.axis];
+    float pred = predict(recon, f, S, ps, d.linear);
+    float orig = in[f];
+    float rec;
+    int code = quantize_point(orig, pred, d.eb, d.recip, d.radius, &rec);
+    recon[f] = rec;
+    if ((unsigned) code > 65535u) atomicOr(err, 1u);
+    codes[ps.seq] = (uint16_t) code;
+    if (code == 0) outval[ps.seq] = orig;
+}
+
+// Same walk in recover mode (PB_recover, SIC:110-196 with Q:160-180).
+template<int N>
+__global__ void __launch_bounds__(256) recover_pass_kernel(PassDesc d, float *__restrict__ data,
+                                                           const uint16_t *__restrict__ codes,
+                                                           const float *__restrict__ unpred_at) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.total) return;
+    uint32_t x[N];
+    PointSlot ps;
+    locate<N>(d, t, x, ps);
+    uint64_t f = 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a) f += (uint64_t) x[a] * d.offs[a];
+    const uint64_t S = (uint64_t) d.s * d.offs[d.axis];
+    float pred = predict(data, f, S, ps, d.linear);
+    int code = codes[ps.seq];
+    data[f] = code ? recover_point(pred, code, d.radius, d.eb) : unpred_at[ps.seq];
+}
+
+void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
+                        double *, uint32_t *d_err, hipStream_t st) {
+    if (d.total == 0) return;
+    dim3 grid((unsigned) ((d.total + 255) / 256)), block(256);
+    switch (d.N) {
+        case 1: hipLaunchKernelGGL(interp_pass_kernel<1>, grid, block, 0, st, d, d_in, d_recon, d_codes, d_outval, d_err); break;
+        case 2: hipLaunchKernelGGL(interp_pass_kernel<2>, grid, block, 0, st, d, d_in, d_recon, d_codes, d_outval, d_err); break;
+        case 3: hipLaunchKernelGGL(interp_pass_kernel<3>, grid, block, 0, st, d, d_in, d_recon, d_codes, d_outval, d_err); break;
+        default: hipLaunchKernelGGL(interp_pass_kernel<4>, grid, block, 0, st, d, d_in, d_recon, d_codes, d_outval, d_err); break;
+    }
+}
+
+void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_codes, const float *d_unpred_at,
+                         hipStream_t st) {
+    if (d.total == 0) return;
+    dim3 grid((unsigned) ((d.total + 255) / 256)), block(256);
+    switch (d.N) {
+        case 1: hipLaunchKernelGGL(recover_pass_kernel<1>, grid, block, 0, st, d, d_data, d_codes, d_unpred_at); break;
+        case 2: hipLaunchKernelGGL(recover_pass_kernel<2>, grid, block, 0, st, d, d_data, d_codes, d_unpred_at); break;
+        case 3: hipLaunchKernelGGL(recover_pass_kernel<3>, grid, block, 0, st, d, d_data, d_codes, d_unpred_at); break;
+        default: hipLaunchKernelGGL(recover_pass_kernel<4>, grid, block, 0, st, d, d_data, d_codes, d_unpred_at); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Anchors (build_grid, SIC:854-900) and the un-anchored first point (SIC:372-374)
+// ---------------------------------------------------------------------------
+__global__ void anchor_kernel(const float *__restrict__ in, float *__restrict__ recon, uint16_t *__restrict__ codes,
+                              float *__restrict__ outval, int N, uint32_t d0, uint32_t d1, uint32_t d2, uint64_t o0,
+                              uint64_t o1, uint32_t step, uint32_t count) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint64_t f;
+    if (N == 2) {
+        uint32_t ny = (d1 - 1) / step + 1;
+        f = (uint64_t) (t / ny) * step * o0 + (uint64_t) (t % ny) * step;
+    } else {
+        uint32_t ny = (d1 - 1) / step + 1, nz = (d2 - 1) / step + 1;
+        uint32_t z = t % nz, y = (t / nz) % ny, xx = t / (nz * ny);
+        f = (uint64_t) xx * step * o0 + (uint64_t) y * step * o1 + (uint64_t) z * step;
+    }
+    float v = in[f];
+    recon[f] = v;
+    codes[t] = 0;
+    outval[t] = v;
+}
+
+void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
+                        const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st) {
+    if (!count) return;
+    hipLaunchKernelGGL(anchor_kernel, dim3((unsigned) ((count + 255) / 256)), dim3(256), 0, st, d_in, d_recon, d_codes,
+                       d_outval, N, dims[0], dims[1], N > 2 ? dims[2] : 1, offs[0], N > 2 ? offs[1] : 1, step,
+                       (uint32_t) count);
+}
+
+__global__ void anchor_scatter_kernel(const float *__restrict__ vals, float *__restrict__ out, int N, uint32_t d1,
+                                      uint32_t d2, uint64_t o0, uint64_t o1, uint32_t step, uint32_t count) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    uint64_t f;
+    if (N == 2) {
+        uint32_t ny = (d1 - 1) / step + 1;
+        f = (uint64_t) (t / ny) * step * o0 + (uint64_t) (t % ny) * step;
+    } else {
+        uint32_t ny = (d1 - 1) / step + 1, nz = (d2 - 1) / step + 1;
+        uint32_t z = t % nz, y = (t / nz) % ny, xx = t / (nz * ny);
+        f = (uint64_t) xx * step * o0 + (uint64_t) y * step * o1 + (uint64_t) z * step;
+    }
+    out[f] = vals[t];
+}
+
+// recover_grid (SIC:902-923): anchors take the leading unpredictable values
+void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint32_t *dims, const uint64_t *offs,
+                           uint32_t step, uint64_t count, hipStream_t st) {
+    if (!count) return;
+    hipLaunchKernelGGL(anchor_scatter_kernel, dim3((unsigned) ((count + 255) / 256)), dim3(256), 0, st, d_vals, d_out,
+                       N, dims[1], N > 2 ? dims[2] : 1, offs[0], N > 2 ? offs[1] : 1, step, (uint32_t) count);
+}
+
+__global__ void first_point_kernel(const float *in, float *recon, uint16_t *codes, float *outval, double eb,
+                                   double recip, int radius, uint32_t *err) {
+    float rec;
+    int code = quantize_point(in[0], 0.0f, eb, recip, radius, &rec);
+    recon[0] = rec;
+    if ((unsigned) code > 65535u) atomicOr(err, 1u);
+    codes[0] = (uint16_t) code;
+    if (code == 0) outval[0] = in[0];
+}
+
+void launch_first_point(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, double eb, double recip,
+                        int radius, uint32_t *d_err, hipStream_t st) {
+    hipLaunchKernelGGL(first_point_kernel, dim3(1), dim3(1), 0, st, d_in, d_recon, d_codes, d_outval, eb, recip, radius,
+                       d_err);
+}
+
+// ---------------------------------------------------------------------------
+// data_range: min/max as order-preserving uint32 keys
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in, size_t n, uint32_t *minmax) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    size_t n4 = n / 4;
+    const float4 *in4 = reinterpret_cast<const float4 *>(in);
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+        float4 v = in4[i];
+        uint32_t a = f2key(v.x), b = f2key(v.y), c = f2key(v.z), e = f2key(v.w);
+        lo = min(lo, min(min(a, b), min(c, e)));
+        hi = max(hi, max(max(a, b), max(c, e)));
+    }
+    for (size_t i = n4 * 4 + (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        uint32_t a = f2key(in[i]);
+        lo = min(lo, a);
+        hi = max(hi, a);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t) __shfl_xor((int) lo, o));
+        hi = max(hi, (uint32_t) __shfl_xor((int) hi, o));
+    }
+    __shared__ uint32_t slo[4], shi[4];
+    int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        slo[w] = lo;
+        shi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            lo = min(lo, slo[k]);
+            hi = max(hi, shi[k]);
+        }
+        atomicMin(&minmax[0], lo);
+        atomicMax(&minmax[1], hi);
+    }
+}
+
+__global__ void init_minmax_kernel(uint32_t *minmax) {
+    minmax[0] = 0xFFFFFFFFu;
+    minmax[1] = 0u;
+}
+
+void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st) {
+    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(1), 0, st, d_minmax);
+    size_t want = (n / 4 + 255) / 256;
+    unsigned grid = (unsigned) (want < 4096 ? (want ? want : 1) : 4096);
+    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax);
+}
+
+// ---------------------------------------------------------------------------
+// Histogram + first occurrence + per-chunk outlier counts
+// ---------------------------------------------------------------------------
+constexpr int kWin = 4096;  // LDS bins around the quantizer radius
+
+__global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                           uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
+                                                           uint32_t *__restrict__ chunk_zeros, int lo) {
+    __shared__ uint32_t s_cnt[kWin];
+    __shared__ uint32_t s_first[kWin];
+    __shared__ uint32_t s_zero, s_zero_first;
+    for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
+        s_cnt[i] = 0;
+        s_first[i] = 0xFFFFFFFFu;
+    }
+    if (threadIdx.x == 0) {
+        s_zero = 0;
+        s_zero_first = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    for (uint32_t j = base + threadIdx.x; j < end; j += kHistThreads) {
+        uint32_t v = codes[j];
+        uint32_t li = j - base;
+        if (v == 0) {
+            atomicAdd(&s_zero, 1u);
+            if (s_zero_first > li) atomicMin(&s_zero_first, li);
+        } else {
+            uint32_t w = v - (uint32_t) lo;
+            if (w < (uint32_t) kWin) {
+                atomicAdd(&s_cnt[w], 1u);
+                if (s_first[w] > li) atomicMin(&s_first[w], li);
+            } else {
+                atomicAdd(&freq[v], 1u);
+                atomicMin(&first[v], j);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
+        uint32_t c = s_cnt[i];
+        if (c) {
+            atomicAdd(&freq[lo + i], c);
+            atomicMin(&first[lo + i], base + s_first[i]);
+        }
+    }
+    if (threadIdx.x == 0) {
+        chunk_zeros[blockIdx.x] = s_zero;
+        if (s_zero) {
+            atomicAdd(&freq[0], s_zero);
+            atomicMin(&first[0], base + s_zero_first);
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t *d, size_t count, uint32_t value) {
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) d[i] = value;
+}
+
+void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st) {
+    if (!count) return;
+    size_t want = (count + 255) / 256;
+    unsigned grid = (unsigned) (want < 8192 ? want : 8192);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, st, d, count, value);
+}
+
+void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
+                 int radius, hipStream_t st) {
+    launch_fill_u32(d_freq, 65536, 0u, st);
+    launch_fill_u32(d_first, 65536, 0xFFFFFFFFu, st);
+    uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    int lo = radius - kWin / 2;
+    if (lo < 1) lo = 1;
+    if (nchunks)
+        hipLaunchKernelGGL(hist_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
+                           d_first, d_chunk_zeros, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scans over chunk totals (single workgroup; chunk counts are small)
+// ---------------------------------------------------------------------------
+template<typename TIn>
+__global__ void __launch_bounds__(1024) scan_kernel(const TIn *__restrict__ in, uint64_t *__restrict__ out,
+                                                    uint32_t count, uint64_t *total) {
+    __shared__ uint64_t s_part[1024];
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = min(lo + per, count);
+    uint64_t sum = 0;
+    for (uint32_t i = lo; i < hi; i++) sum += (uint64_t) in[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint64_t v = threadIdx.x >= (unsigned) off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (uint32_t i = lo; i < hi; i++) {
+        uint64_t v = (uint64_t) in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023 && total) *total = s_part[1023];
+}
+
+void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_total,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(scan_kernel<uint32_t>, dim3(1), dim3(1024), 0, st, d_chunk_zeros, d_chunk_base, nchunks,
+                       d_total);
+}
+
+void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_total, hipStream_t st) {
+    hipLaunchKernelGGL(scan_kernel<uint64_t>, dim3(1), dim3(1024), 0, st, d_in, d_out, count, d_total);
+}
+
+// ---------------------------------------------------------------------------
+// Outlier compaction: the unpredictable list is the values at code-0
+// positions in code order.  Only chunks that hold outliers do any work;
+// ranks come from wavefront ballots and a workgroup prefix.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kHistThreads) unpred_gather_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                    const float *__restrict__ outval,
+                                                                    const uint32_t *__restrict__ chunk_zeros,
+                                                                    const uint64_t *__restrict__ chunk_base,
+                                                                    float *__restrict__ unpred) {
+    if (chunk_zeros[blockIdx.x] == 0) return;
+    __shared__ uint32_t s_wave[kHistThreads / 64];
+    const uint32_t base = blockIdx.x * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    uint64_t run = chunk_base[blockIdx.x];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
+        uint32_t j = j0 + threadIdx.x;
+        bool z = j < end && codes[j] == 0;
+        uint64_t m = __ballot(z);
+        uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wave] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (int w = 0; w < kHistThreads / 64; w++) {
+            if (w < wave) before += s_wave[w];
+            tot += s_wave[w];
+        }
+        if (z) unpred[run + before + below] = outval[j];
+        run += tot;
+        __syncthreads();
+    }
+}
+
+void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
+                          const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st) {
+    uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    if (nchunks)
+        hipLaunchKernelGGL(unpred_gather_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
+                           d_outval, d_chunk_zeros, d_chunk_base, d_unpred);
+}
+
+// Inverse of the gather for decompression: unpredictable value of every
+// code-0 position, indexed by code position.
+__global__ void __launch_bounds__(kHistThreads) unpred_scatter_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                     const float *__restrict__ unpred,
+                                                                     const uint32_t *__restrict__ chunk_zeros,
+                                                                     const uint64_t *__restrict__ chunk_base,
+                                                                     float *__restrict__ unpred_at) {
+    if (chunk_zeros[blockIdx.x] == 0) return;
+    __shared__ uint32_t s_wave[kHistThreads / 64];
+    const uint32_t base = blockIdx.x * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    uint64_t run = chunk_base[blockIdx.x];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
+        uint32_t j = j0 + threadIdx.x;
+        bool z = j < end && codes[j] == 0;
+        uint64_t m = __ballot(z);
+        uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wave] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (int w = 0; w < kHistThreads / 64; w++) {
+            if (w < wave) before += s_wave[w];
+            tot += s_wave[w];
+        }
+        if (z) unpred_at[j] = unpred[run + before + below];
+        run += tot;
+        __syncthreads();
+    }
+}
+
+void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
+                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st) {
+    uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    if (nchunks)
+        hipLaunchKernelGGL(unpred_scatter_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
+                           d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at);
+}
+
+// ---------------------------------------------------------------------------
+// Huffman encode: chunk bit totals, exclusive scan, then per-chunk packing in
+// LDS (MSB-first) flushed to the output words in big-endian byte order.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kEncThreads) encode_sizes_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                  const uint8_t *__restrict__ len, int32_t offset,
+                                                                  uint64_t *__restrict__ chunk_bits) {
+    const uint32_t base = blockIdx.x * kEncChunk;
+    const uint32_t end = min(base + kEncChunk, n);
+    uint32_t sum = 0;
+    for (uint32_t j = base + threadIdx.x; j < end; j += kEncThreads) sum += len[(int32_t) codes[j] - offset];
+    for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
+    __shared__ uint32_t s[kEncThreads / 64];
+    if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kEncThreads / 64; w++) t += s[w];
+        chunk_bits[blockIdx.x] = t;
+    }
+}
+
+void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint8_t *d_len, int32_t offset,
+                         uint64_t *d_chunk_bits, hipStream_t st) {
+    uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    if (nchunks)
+        hipLaunchKernelGGL(encode_sizes_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_len,
+                           offset, d_chunk_bits);
+}
+
+__device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
+
+constexpr uint32_t kEncWords = kEncChunk * 64 / 32 + 2;  // worst case: 64-bit codes
+
+__global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                 const uint64_t *__restrict__ code,
+                                                                 const uint8_t *__restrict__ len, int32_t offset,
+                                                                 const uint64_t *__restrict__ chunk_off,
+                                                                 uint32_t *__restrict__ words) {
+    __shared__ uint32_t s_words[kEncWords];
+    __shared__ uint32_t s_wave[kEncThreads / 64];
+    const uint32_t base = blockIdx.x * kEncChunk;
+    const uint32_t end = min(base + kEncChunk, n);
+    // thread t owns codes [base + t*kEncPerThread, +kEncPerThread)
+    const uint32_t my0 = base + threadIdx.x * kEncPerThread;
+    uint32_t mybits = 0;
+    uint16_t mine[kEncPerThread];
+#pragma unroll
+    for (uint32_t k = 0; k < kEncPerThread; k++) {
+        uint32_t j = my0 + k;
+        mine[k] = j < end ? codes[j] : 0;
+        mybits += j < end ? len[(int32_t) mine[k] - offset] : 0;
+    }
+    // workgroup exclusive prefix of mybits
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    uint32_t incl = mybits;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t v = (uint32_t) __shfl_up((int) incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    for (uint32_t i = threadIdx.x; i < kEncWords; i += kEncThreads) s_words[i] = 0;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < kEncThreads / 64; w++) {
+        if (w < wave) before += s_wave[w];
+        total += s_wave[w];
+    }
+    const uint64_t g0 = chunk_off[blockIdx.x];
+    const uint32_t head = (uint32_t) (g0 & 31);
+    uint32_t lp = head + before + (incl - mybits);
+#pragma unroll
+    for (uint32_t k = 0; k < kEncPerThread; k++) {
+        uint32_t j = my0 + k;
+        if (j >= end) break;
+        int st = (int32_t) mine[k] - offset;
+        uint32_t L = len[st];
+        uint64_t c = code[st];
+        while (L) {
+            uint32_t w = lp >> 5, o = lp & 31;
+            uint32_t take = min(L, 32u - o);
+            uint32_t bits = (uint32_t) (c >> (64 - take));
+            atomicOr(&s_words[w], bits << (32 - o - take));
+            c <<= take;
+            L -= take;
+            lp += take;
+        }
+    }
+    __syncthreads();
+    const uint32_t nw = (head + total + 31) / 32;
+    const uint64_t gw = g0 >> 5;
+    for (uint32_t w = threadIdx.x; w < nw; w += kEncThreads) {
+        uint32_t v = bswap32(s_words[w]);
+        bool edge = (w == 0 && head) || (w == nw - 1 && ((head + total) & 31));
+        if (edge)
+            atomicOr(&words[gw + w], v);
+        else
+            words[gw + w] = v;
+    }
+}
+
+void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_code, const uint8_t *d_len,
+                        int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st) {
+    uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    if (nchunks)
+        hipLaunchKernelGGL(encode_pack_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_code,
+                           d_len, offset, d_chunk_off, d_words);
+}
+
+void launch_bswap_words(uint32_t *, size_t, hipStream_t) {}
+
+}  // namespace faz

@@ -1,0 +1,279 @@
+// `faz` command line of the MI355X build, accepting the reference's options
+// (test/faz.cpp:21-620): -z/-x/-i/-o/-s/-f/-d/-M/-A/-R/-S/-N/-T/-C/-B/-q/-F/-l/
+// -c/-1..-4/-a/-b/-t/-h/-v.  Compression and decompression run on the GPU
+// engine through SZ_compress / SZ_decompress.  Errors print a message and
+// exit with status 1 (the reference exits 0).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "faz/config.hpp"
+#include "faz/sz.hpp"
+
+namespace {
+
+void usage() {
+    printf("Usage: faz <options>\n"
+           "Options:\n"
+           "* general options:\n"
+           "\t-h: print the help information\n"
+           "\t-v: print the version number\n"
+           "\t-a : print compression results such as distortions\n"
+           "* input and output:\n"
+           "\t-i <path> : original binary input file\n"
+           "\t-o <path> : decompressed output file\n"
+           "\t-z <path> : compressed output (w -i) or input (w/o -i) file\n"
+           "\t-x <path> : decompressed output file\n"
+           "\t-s <path> : compressed input file (SZ2 style)\n"
+           "\t-t : store decompressed output in text format\n"
+           "* data type:\n"
+           "\t-f: single precision (float type)\n"
+           "\t-d: double precision (not served by the GPU engine)\n"
+           "* configuration file: \n"
+           "\t-c <configuration file> : configuration file faz.config\n"
+           "* error control:\n"
+           "\t-M <ABS|REL|PSNR|NORM|ABS_AND_REL|ABS_OR_REL> [error bound]\n"
+           "\t-A <absolute error bound>\n"
+           "\t-R <value-range relative error bound>\n"
+           "\t-S <PSNR>\n"
+           "\t-N <normErr>\n"
+           "\t-q: QoZ mode\n"
+           "\t-F <1/0>: FAZ features\n"
+           "\t-T <PSNR|CR|SSIM|AC>: tuning target\n"
+           "\t-C <maxStep>  -B <sampleBlockSize>  -l: test Lorenzo\n"
+           "* dimensions (fastest last): -1 nx | -2 nx ny | -3 nx ny nz | -4 nx ny nz nw\n");
+    exit(0);
+}
+
+[[noreturn]] void die(const char *m) {
+    fprintf(stderr, "faz: %s\n", m);
+    exit(1);
+}
+
+std::vector<char> read_file(const char *path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) die((std::string("cannot open ") + path).c_str());
+    return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+void write_file(const char *path, const void *p, size_t n) {
+    FILE *f = fopen(path, "wb");
+    if (!f || fwrite(p, 1, n, f) != n) die((std::string("cannot write ") + path).c_str());
+    fclose(f);
+}
+
+// verify (include/QoZ/utils/Statistic.hpp:118-181): distortion report in FP64
+void verify(const float *ori, const float *dec, size_t n) {
+    double Max = ori[0], Min = ori[0], diffMax = std::fabs((double) dec[0] - ori[0]);
+    double sum1 = 0, sum2 = 0, l2sum = 0;
+    for (size_t i = 0; i < n; i++) {
+        sum1 += ori[i];
+        sum2 += dec[i];
+        l2sum += (double) dec[i] * dec[i];
+    }
+    double mean1 = sum1 / n, mean2 = sum2 / n;
+    double sum3 = 0, sum4 = 0, sum = 0, prodSum = 0, maxpw = 0;
+    for (size_t i = 0; i < n; i++) {
+        double o = ori[i], d = dec[i];
+        if (Max < o) Max = o;
+        if (Min > o) Min = o;
+        double err = std::fabs(d - o);
+        if (o != 0 && maxpw < err / std::fabs(o)) maxpw = err / std::fabs(o);
+        if (diffMax < err) diffMax = err;
+        prodSum += (o - mean1) * (d - mean2);
+        sum3 += (o - mean1) * (o - mean1);
+        sum4 += (d - mean2) * (d - mean2);
+        sum += err * err;
+    }
+    double acEff = (prodSum / n) / std::sqrt(sum3 / n) / std::sqrt(sum4 / n);
+    double mse = sum / n, range = Max - Min;
+    double psnr = 20 * std::log10(range) - 10 * std::log10(mse);
+    printf("Min=%.20G, Max=%.20G, range=%.20G\n", Min, Max, range);
+    printf("Max absolute error = %.2G\n", diffMax);
+    printf("Max relative error = %.2G\n", diffMax / range);
+    printf("Max pw relative error = %.2G\n", maxpw);
+    printf("PSNR = %f, NRMSE= %.10G\n", psnr, std::sqrt(mse) / range);
+    printf("normError = %f, normErr_norm = %f\n", std::sqrt(sum), std::sqrt(sum) / std::sqrt(l2sum));
+    printf("acEff=%f\n", acEff);
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+    bool binaryOutput = true, compression = false, decompression = false, printCmpResults = false;
+    bool doubleType = false, qoz = false, testLorenzo = false, delCmpPath = false;
+    int faz = 1, maxStep = 0, sampleBlockSize = 0;
+    char *inPath = nullptr, *cmpPath = nullptr, *conPath = nullptr, *decPath = nullptr;
+    char *errBoundMode = nullptr, *errBound = nullptr, *absEB = nullptr, *relEB = nullptr, *psnrEB = nullptr,
+         *normEB = nullptr, *tuningTarget = nullptr;
+    size_t r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+    if (argc == 1) usage();
+    for (int i = 1; i < argc; i++) {
+        if (argv[i][0] != '-' || (argv[i][1] && argv[i][2])) usage();
+        auto need = [&]() -> char * {
+            if (++i == argc) usage();
+            return argv[i];
+        };
+        switch (argv[i][1]) {
+            case 'h': usage();
+            case 'v': printf("version: faz-mi355x 0.1.0.0\n"); return 0;
+            case 'b': binaryOutput = true; break;
+            case 't': binaryOutput = false; break;
+            case 'a': printCmpResults = true; break;
+            case 'z':
+                compression = true;
+                if (i + 1 < argc && argv[i + 1][0] != '-') cmpPath = argv[++i];
+                break;
+            case 'x':
+                decompression = true;
+                if (i + 1 < argc && argv[i + 1][0] != '-') decPath = argv[++i];
+                break;
+            case 'f': doubleType = false; break;
+            case 'd': doubleType = true; break;
+            case 'i': inPath = need(); break;
+            case 'q': qoz = true; break;
+            case 'F': faz = atoi(need()); break;
+            case 'l': testLorenzo = true; break;
+            case 'o': decPath = need(); break;
+            case 's': cmpPath = need(); break;
+            case 'c': conPath = need(); break;
+            case '1': r1 = strtoull(need(), nullptr, 10); break;
+            case '2': r1 = strtoull(need(), nullptr, 10); r2 = strtoull(need(), nullptr, 10); break;
+            case '3':
+                r1 = strtoull(need(), nullptr, 10);
+                r2 = strtoull(need(), nullptr, 10);
+                r3 = strtoull(need(), nullptr, 10);
+                break;
+            case '4':
+                r1 = strtoull(need(), nullptr, 10);
+                r2 = strtoull(need(), nullptr, 10);
+                r3 = strtoull(need(), nullptr, 10);
+                r4 = strtoull(need(), nullptr, 10);
+                break;
+            case 'M':
+                errBoundMode = need();
+                if (i + 1 < argc && argv[i + 1][0] != '-') errBound = argv[++i];
+                break;
+            case 'A': absEB = need(); break;
+            case 'R': relEB = need(); break;
+            case 'N': normEB = need(); break;
+            case 'S': psnrEB = need(); break;
+            case 'T': tuningTarget = need(); break;
+            case 'C': maxStep = atoi(need()); break;
+            case 'B': sampleBlockSize = atoi(need()); break;
+            default: usage();
+        }
+    }
+    if (!inPath && !cmpPath) die("specify a raw binary input (-i) or a compressed file (-z/-s)");
+    if (inPath && cmpPath && !decompression) compression = true;
+    if (cmpPath && decPath) decompression = true;
+    std::string tmpCmp;
+    if (inPath && !cmpPath && decPath) {
+        compression = decompression = true;
+        tmpCmp = std::string(inPath) + ".faz.tmp";
+        cmpPath = &tmpCmp[0];
+        delCmpPath = true;
+    }
+    if (!inPath || !errBoundMode) compression = false;
+    if (!compression && !decompression) usage();
+    if (doubleType) die("double precision is not served by the GPU engine");
+
+    QoZ::Config conf;
+    std::vector<size_t> dims;
+    if (r2 == 0) dims = {r1};
+    else if (r3 == 0) dims = {r2, r1};
+    else if (r4 == 0) dims = {r3, r2, r1};
+    else dims = {r4, r3, r2, r1};
+    conf.setDims(dims.begin(), dims.end());
+    if (qoz) conf.QoZ = 1;
+    if (faz) conf.FZ = 1;
+    if (testLorenzo) conf.testLorenzo = 1;
+    if (maxStep > 0) conf.maxStep = maxStep;
+    if (sampleBlockSize > 0) conf.sampleBlockSize = sampleBlockSize;
+    if (compression && conPath && !conf.loadcfg(conPath)) die("can't load cfg file");
+    if (errBoundMode) {
+        if (relEB) conf.relErrorBound = atof(relEB);
+        if (absEB) conf.absErrorBound = atof(absEB);
+        if (psnrEB) conf.psnrErrorBound = atof(psnrEB);
+        if (normEB) conf.l2normErrorBound = atof(normEB);
+        std::string m(errBoundMode);
+        if (m == "ABS") {
+            conf.errorBoundMode = QoZ::EB_ABS;
+            if (errBound) conf.absErrorBound = atof(errBound);
+        } else if (m == "REL" || m == "VR_REL") {
+            conf.errorBoundMode = QoZ::EB_REL;
+            if (errBound) conf.relErrorBound = atof(errBound);
+        } else if (m == "PSNR") {
+            conf.errorBoundMode = QoZ::EB_PSNR;
+            if (errBound) conf.psnrErrorBound = atof(errBound);
+        } else if (m == "NORM") {
+            conf.errorBoundMode = QoZ::EB_L2NORM;
+            if (errBound) conf.l2normErrorBound = atof(errBound);
+        } else if (m == "ABS_AND_REL") {
+            conf.errorBoundMode = QoZ::EB_ABS_AND_REL;
+        } else if (m == "ABS_OR_REL") {
+            conf.errorBoundMode = QoZ::EB_ABS_OR_REL;
+        } else {
+            die("wrong error bound mode setting by using the option '-M'");
+        }
+    }
+    if (tuningTarget) {
+        std::string t(tuningTarget);
+        if (t == "PSNR") conf.tuningTarget = QoZ::TUNING_TARGET_RD;
+        else if (t == "CR") conf.tuningTarget = QoZ::TUNING_TARGET_CR;
+        else if (t == "SSIM") conf.tuningTarget = QoZ::TUNING_TARGET_SSIM;
+        else if (t == "AC") conf.tuningTarget = QoZ::TUNING_TARGET_AC;
+        else die("wrong tuning target setting by using the option '-T'");
+    }
+    try {
+        if (compression) {
+            std::vector<char> raw = read_file(inPath);
+            if (raw.size() != conf.num * sizeof(float)) die("input size does not match the dimensions");
+            size_t outSize = 0;
+            auto t0 = std::chrono::steady_clock::now();
+            char *bytes = SZ_compress<float>(conf, reinterpret_cast<const float *>(raw.data()), outSize);
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::string out = cmpPath ? std::string(cmpPath) : std::string(inPath) + ".faz";
+            write_file(out.c_str(), bytes, outSize);
+            printf("compression ratio = %.2f \n", conf.num * 1.0 * sizeof(float) / outSize);
+            printf("compression time = %f\n", secs);
+            printf("compressed data file = %s\n", out.c_str());
+            delete[] bytes;
+        }
+        if (decompression) {
+            if (printCmpResults && !inPath) die("-a needs the original data (-i)");
+            std::vector<char> cmp = read_file(cmpPath);
+            auto t0 = std::chrono::steady_clock::now();
+            QoZ::Config dconf;
+            float *dec = SZ_decompress<float>(dconf, cmp.data(), cmp.size());
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::string out = decPath ? std::string(decPath) : std::string(cmpPath) + ".out";
+            if (binaryOutput) {
+                write_file(out.c_str(), dec, dconf.num * sizeof(float));
+            } else {
+                FILE *f = fopen(out.c_str(), "w");
+                if (!f) die("cannot write output");
+                for (size_t i = 0; i < dconf.num; i++) fprintf(f, "%.10G\n", dec[i]);
+                fclose(f);
+            }
+            if (printCmpResults) {
+                std::vector<char> ori = read_file(inPath);
+                if (ori.size() != dconf.num * sizeof(float)) die("original size mismatch");
+                verify(reinterpret_cast<const float *>(ori.data()), dec, dconf.num);
+            }
+            printf("compression ratio = %f\n", dconf.num * sizeof(float) * 1.0 / cmp.size());
+            printf("decompression time = %f seconds.\n", secs);
+            printf("decompressed file = %s\n", out.c_str());
+            delete[] dec;
+        }
+    } catch (const faz::Error &e) {
+        die(e.what());
+    }
+    if (delCmpPath) remove(cmpPath);
+    return 0;
+}

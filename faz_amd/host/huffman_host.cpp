@@ -1,0 +1,334 @@
+// Host side of the Huffman stage: tree construction, code table, tree
+// serialisation and bit-serial decoding, reproducing HuffmanEncoder<int>
+// (include/QoZ/encoder/HuffmanEncoder.hpp) byte for byte.
+//
+// The GPU supplies per-symbol counts and the first code position of every
+// symbol.  The reference counts symbols in a ska::unordered_map and builds its
+// heap in that map's iteration order, which depends only on the order in
+// which distinct keys were first inserted; the first positions give that
+// order.
+#include "huffman_host.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace faz {
+
+namespace {
+
+// ska::unordered_map<int, size_t> bucket/chain structure
+// (utils/ska_hash/unordered_map.hpp:254-303, 412-446, 604-643;
+// flat_hash_map.hpp:1268-1295): Fibonacci hashing into a power-of-two bucket
+// array, growth to max(4, 2*buckets) when size+1 would exceed the bucket count
+// (max load factor 1), new keys at the head of their chain, rehash walking the
+// old buckets upwards and pushing each element to the head of its new chain,
+// iteration from the top bucket down and along each chain.
+class SkaOrder {
+public:
+    explicit SkaOrder(size_t expected) { next_.reserve(expected); keys_.reserve(expected); }
+
+    void insert(int key) {
+        if (heads_.empty() || keys_.size() + 1 > heads_.size()) grow();
+        int e = (int) keys_.size();
+        keys_.push_back(key);
+        next_.push_back(-1);
+        link_front(e);
+    }
+
+    std::vector<int> iteration_order() const {
+        std::vector<int> out;
+        out.reserve(keys_.size());
+        for (size_t b = heads_.size(); b-- > 0;)
+            for (int e = heads_[b]; e >= 0; e = next_[e]) out.push_back(keys_[e]);
+        return out;
+    }
+
+private:
+    std::vector<int> heads_;
+    std::vector<int> next_;
+    std::vector<int> keys_;
+    int shift_ = 63;
+
+    size_t bucket(int key) const {
+        return (size_t) ((0x9E3779B97F4A7C15ull * (uint64_t) (int64_t) key) >> shift_);
+    }
+    void link_front(int e) {
+        size_t b = bucket(keys_[e]);
+        next_[e] = heads_[b];
+        heads_[b] = e;
+    }
+    void grow() {
+        size_t old = heads_.empty() ? 1 : heads_.size();
+        size_t want = std::max<size_t>(4, 2 * old);
+        size_t nb = 2;
+        while (nb < want) nb <<= 1;
+        int lg = 0;
+        while (((size_t) 1 << lg) < nb) lg++;
+        std::vector<int> old_heads;
+        old_heads.swap(heads_);
+        heads_.assign(nb, -1);
+        shift_ = 64 - lg;
+        for (size_t b = 0; b < old_heads.size(); b++) {
+            int e = old_heads[b];
+            while (e >= 0) {
+                int nx = next_[e];
+                link_front(e);
+                e = nx;
+            }
+        }
+    }
+};
+
+struct Node {
+    int left = -1, right = -1;
+    uint64_t freq = 0;
+    int c = 0;
+    bool leaf = false;
+};
+
+// HuffmanEncoder's 1-based binary heap (qinsert/qremove, HUF:455-485)
+class Heap {
+public:
+    explicit Heap(const std::vector<Node> &pool) : pool_(pool) { q_.assign(2, -1); }
+    void insert(int n) {
+        int i = end_++, j;
+        if ((int) q_.size() <= i) q_.resize(2 * i + 2, -1);
+        while ((j = i >> 1)) {
+            if (pool_[q_[j]].freq <= pool_[n].freq) break;
+            q_[i] = q_[j];
+            i = j;
+        }
+        q_[i] = n;
+    }
+    int remove() {
+        int i = 1, l;
+        int n = q_[1];
+        end_--;
+        q_[1] = q_[end_];
+        while ((l = i << 1) < end_) {
+            if (l + 1 < end_ && pool_[q_[l + 1]].freq < pool_[q_[l]].freq) l++;
+            if (pool_[q_[i]].freq > pool_[q_[l]].freq) {
+                std::swap(q_[i], q_[l]);
+                i = l;
+            } else {
+                break;
+            }
+        }
+        return n;
+    }
+    int size() const { return end_ - 1; }
+    int top() const { return q_[1]; }
+
+private:
+    const std::vector<Node> &pool_;
+    std::vector<int> q_;
+    int end_ = 1;
+};
+
+template<class T1>
+void serialise(const std::vector<Node> &pool, int root, unsigned nodeCount, std::vector<uint8_t> &out) {
+    // pre-order numbering (pad_tree, HUF:569-584); internal nodes carry C = 0
+    std::vector<T1> L(nodeCount, 0), R(nodeCount, 0);
+    std::vector<int32_t> C(nodeCount, 0);
+    std::vector<uint8_t> t(nodeCount, 0);
+    unsigned counter = 0;
+    struct Frame { int node; unsigned slot; int stage; };
+    std::vector<Frame> st;
+    st.push_back({root, 0, 0});
+    while (!st.empty()) {
+        Frame &f = st.back();
+        const Node &nd = pool[f.node];
+        if (f.stage == 0) {
+            C[f.slot] = nd.leaf ? nd.c : 0;
+            t[f.slot] = nd.leaf ? 1 : 0;
+            f.stage = 1;
+            if (nd.left >= 0) {
+                counter++;
+                L[f.slot] = (T1) counter;
+                st.push_back({nd.left, counter, 0});
+                continue;
+            }
+        }
+        if (f.stage == 1) {
+            f.stage = 2;
+            if (nd.right >= 0) {
+                counter++;
+                R[f.slot] = (T1) counter;
+                st.push_back({nd.right, counter, 0});
+                continue;
+            }
+        }
+        st.pop_back();
+    }
+    out.push_back(0);  // endianness marker: little-endian
+    auto app = [&out](const void *p, size_t n) {
+        const uint8_t *b = (const uint8_t *) p;
+        out.insert(out.end(), b, b + n);
+    };
+    app(L.data(), nodeCount * sizeof(T1));
+    app(R.data(), nodeCount * sizeof(T1));
+    app(C.data(), nodeCount * sizeof(int32_t));
+    app(t.data(), nodeCount);
+}
+
+void put_be32(std::vector<uint8_t> &out, uint32_t v) {
+    out.push_back((uint8_t) (v >> 24));
+    out.push_back((uint8_t) (v >> 16));
+    out.push_back((uint8_t) (v >> 8));
+    out.push_back((uint8_t) v);
+}
+
+}  // namespace
+
+int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
+    std::vector<std::pair<uint32_t, int>> present;
+    for (int s = 0; s < nsym; s++)
+        if (freq[s]) present.push_back({first[s], s});
+    if (present.empty()) return -1;
+    std::sort(present.begin(), present.end());
+    int mn = present[0].second, mx = present[0].second;
+    for (auto &p: present) {
+        mn = std::min(mn, p.second);
+        mx = std::max(mx, p.second);
+    }
+    tab.offset = mn;
+    tab.n_states = mx - mn + 1;
+    const int stateNum = mx - mn + 2;  // HUF:551
+
+    SkaOrder ska(present.size());
+    for (auto &p: present) ska.insert(p.second);
+    std::vector<int> order = ska.iteration_order();
+
+    std::vector<Node> pool;
+    pool.reserve(2 * present.size());
+    Heap heap(pool);
+    for (int key: order) {
+        Node n;
+        n.leaf = true;
+        n.freq = freq[key];
+        n.c = key - mn;
+        pool.push_back(n);
+        heap.insert((int) pool.size() - 1);
+    }
+    // qinsert(new_node(0, 0, qremove(), qremove())) in the reference's g++
+    // build evaluates the arguments right to left: the first node removed is
+    // the right child.
+    while (heap.size() > 1) {
+        int right = heap.remove();
+        int left = heap.remove();
+        Node n;
+        n.left = left;
+        n.right = right;
+        n.freq = pool[left].freq + pool[right].freq;
+        pool.push_back(n);
+        heap.insert((int) pool.size() - 1);
+    }
+    const int root = heap.top();
+
+    // codes: left = 0, right = 1, MSB-aligned (build_code, HUF:493-521)
+    tab.code.assign(tab.n_states, 0);
+    tab.len.assign(tab.n_states, 0);
+    struct Item { int node; int depth; uint64_t bits; };
+    std::vector<Item> st;
+    st.push_back({root, 0, 0});
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        const Node &nd = pool[it.node];
+        if (nd.leaf) {
+            if (it.depth > 64) return -2;  // codes beyond 64 bits need > 2.7e13 symbols
+            tab.code[nd.c] = it.depth ? it.bits << (64 - it.depth) : 0;
+            tab.len[nd.c] = (uint8_t) it.depth;
+            continue;
+        }
+        if (it.depth >= 64) return -2;
+        st.push_back({nd.right, it.depth + 1, (it.bits << 1) | 1});
+        st.push_back({nd.left, it.depth + 1, it.bits << 1});
+    }
+
+    // save (HUF:119-137)
+    const unsigned nodeCount = (unsigned) (2 * present.size() - 1);
+    tab.tree.clear();
+    uint8_t b4[4];
+    memcpy(b4, &mn, 4);
+    tab.tree.insert(tab.tree.end(), b4, b4 + 4);
+    put_be32(tab.tree, nodeCount);
+    put_be32(tab.tree, (uint32_t) (stateNum / 2));
+    if (nodeCount <= 256)
+        serialise<uint8_t>(pool, root, nodeCount, tab.tree);
+    else if (nodeCount <= 65536)
+        serialise<uint16_t>(pool, root, nodeCount, tab.tree);
+    else
+        serialise<uint32_t>(pool, root, nodeCount, tab.tree);
+    return 0;
+}
+
+// HuffmanEncoder::load + decode (HUF:236-296, 310-454)
+size_t decode_huffman(const uint8_t *buf, size_t n, std::vector<int32_t> &out, size_t num) {
+    if (n < 12) return 0;
+    int32_t offset;
+    memcpy(&offset, buf, 4);
+    unsigned nc = ((unsigned) buf[4] << 24) | ((unsigned) buf[5] << 16) | ((unsigned) buf[6] << 8) | buf[7];
+    if (nc == 0 || nc > (1u << 26)) return 0;
+    size_t w = nc <= 256 ? 1 : (nc <= 65536 ? 2 : 4);
+    size_t treeBytes = 1 + 2 * nc * w + nc * 4 + nc;
+    if (12 + treeBytes + 8 > n) return 0;
+    const uint8_t *tb = buf + 12;
+    auto idx = [&](const uint8_t *arr, unsigned i) -> unsigned {
+        if (w == 1) return arr[i];
+        if (w == 2) {
+            uint16_t v;
+            memcpy(&v, arr + 2 * i, 2);
+            return v;
+        }
+        uint32_t v;
+        memcpy(&v, arr + 4 * i, 4);
+        return v;
+    };
+    const uint8_t *L = tb + 1, *R = tb + 1 + nc * w, *C = tb + 1 + 2 * nc * w, *T = C + nc * 4;
+    // flatten into child tables indexed by serialised slot (unpad_tree walks
+    // the same links); slot 0 is the root
+    std::vector<int32_t> left(nc, -1), right(nc, -1), sym(nc, 0);
+    std::vector<uint8_t> leaf(nc, 0);
+    for (unsigned i = 0; i < nc; i++) {
+        int32_t c;
+        memcpy(&c, C + 4 * i, 4);
+        sym[i] = c;
+        leaf[i] = T[i];
+    }
+    if (nc > 256) leaf[0] = 0;  // reference root is rebuilt as an inner node
+    for (unsigned i = 0; i < nc; i++) {
+        if (leaf[i]) continue;
+        unsigned l = idx(L, i), r = idx(R, i);
+        if (l >= nc || r >= nc) return 0;
+        left[i] = l ? (int32_t) l : -1;
+        right[i] = r ? (int32_t) r : -1;
+    }
+    uint64_t enc;
+    memcpy(&enc, tb + treeBytes, 8);
+    size_t used = 12 + treeBytes + 8;
+    if (used + enc > n) return 0;
+    const uint8_t *bytes = buf + used;
+    out.resize(num);
+    if (leaf[0]) {
+        for (size_t i = 0; i < num; i++) out[i] = sym[0] + offset;
+        return used + enc;
+    }
+    int nd = 0;
+    size_t count = 0;
+    uint64_t nbits = enc * 8;
+    for (uint64_t i = 0; count < num; i++) {
+        if (i >= nbits) return 0;
+        int bit = (bytes[i >> 3] >> (7 - (i & 7))) & 1;
+        nd = bit ? right[nd] : left[nd];
+        if (nd < 0) return 0;
+        if (leaf[nd]) {
+            out[count++] = sym[nd] + offset;
+            nd = 0;
+        }
+    }
+    return used + enc;
+}
+
+}  // namespace faz

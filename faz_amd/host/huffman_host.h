@@ -1,0 +1,26 @@
+// Host Huffman stage (tree, code table, serialisation, decoding).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <vector>
+
+namespace faz {
+
+struct HuffTable {
+    int32_t offset = 0;            // smallest symbol
+    int32_t n_states = 0;          // max - offset + 1
+    std::vector<uint64_t> code;    // MSB-aligned codewords by symbol - offset
+    std::vector<uint8_t> len;      // codeword lengths
+    std::vector<uint8_t> tree;     // HuffmanEncoder::save bytes
+};
+
+// Builds the reference Huffman tree from counts and first positions of
+// symbols 0..nsym-1.  Returns 0, or < 0 for an empty histogram / codes longer
+// than 64 bits.
+int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab);
+
+// Loads a saved tree followed by a size_t-prefixed bitstream and decodes num
+// symbols.  Returns the bytes consumed, 0 on malformed input.
+size_t decode_huffman(const uint8_t *buf, size_t n, std::vector<int32_t> &out, size_t num);
+
+}  // namespace faz

@@ -1,0 +1,18 @@
+// Internal entry points shared by the GPU orchestration (csrc/faz_gpu.hip)
+// and the stream layer (host/stream.cpp).
+#pragma once
+#include <stdint.h>
+#include <vector>
+
+#include "faz_gpu.h"
+
+extern "C++" {
+// Whole engine buffer (header, quantizer, Huffman tree and bits) of one field
+// into `raw`, sized exactly.
+int faz_compress_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                         std::vector<uint8_t> &raw);
+// Device scratch of the context, grown on demand (n floats).
+float *faz_ctx_work_in(faz_gpu_ctx *ctx, size_t n);
+hipStream_t faz_ctx_stream(faz_gpu_ctx *ctx);
+void faz_ctx_set_error(faz_gpu_ctx *ctx, const char *msg);
+}

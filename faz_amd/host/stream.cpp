@@ -1,0 +1,463 @@
+// Stream layer of the MI355X build: SZ_compress / SZ_decompress
+// (include/QoZ/api/sz.hpp:54-205), the ALGO_INTERP driver
+// (SZInterp.hpp:387-431), calAbsErrorBound (Statistic.hpp:34-66), the zstd
+// wrapper (Lossless_zstd.hpp:22-47) and the Config trailer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "faz/config.hpp"
+#include "faz/sz.hpp"
+#include "faz_gpu.h"
+#include "raw_api.h"
+
+extern "C" {
+// libzstd (system shared library libzstd.so.1); prototypes as published in zstd.h
+size_t ZSTD_compress(void *dst, size_t dstCapacity, const void *src, size_t srcSize, int compressionLevel);
+size_t ZSTD_decompress(void *dst, size_t dstCapacity, const void *src, size_t compressedSize);
+size_t ZSTD_compressBound(size_t srcSize);
+unsigned ZSTD_isError(size_t code);
+}
+
+namespace {
+
+constexpr int kZstdLevel = 3;  // Lossless_zstd::compression_level
+
+// zstd stage: size_t raw length, then the zstd frame (Lossless_zstd.hpp:22-31)
+bool zstd_wrap(const std::vector<uint8_t> &raw, std::vector<uint8_t> &out, size_t reserve_tail) {
+    out.resize(8 + ZSTD_compressBound(raw.size()) + reserve_tail);
+    uint64_t len = raw.size();
+    memcpy(out.data(), &len, 8);
+    size_t z = ZSTD_compress(out.data() + 8, out.size() - 8, raw.data(), raw.size(), kZstdLevel);
+    if (ZSTD_isError(z)) return false;
+    out.resize(8 + z);
+    return true;
+}
+
+bool zstd_unwrap(const uint8_t *p, size_t n, std::vector<uint8_t> &raw) {
+    if (n < 8) return false;
+    uint64_t len;
+    memcpy(&len, p, 8);
+    if (len > (1ull << 40)) return false;
+    raw.resize(len);
+    size_t z = ZSTD_decompress(raw.data(), len, p + 8, n - 8);
+    return !ZSTD_isError(z) && z == len;
+}
+
+void append_trailer(const QoZ::Config &conf, std::vector<uint8_t> &out) {
+    size_t before = out.size();
+    out.resize(before + QoZ::Config::size_est() + conf.metadata.size() + 8 * conf.coeffs_dims.size() + 64);
+    QoZ::uchar *w = out.data() + before;
+    conf.save(w);
+    int confSize = (int) (w - (out.data() + before));
+    memcpy(w, &confSize, 4);
+    w += 4;
+    out.resize(w - out.data());
+}
+
+// The Config as SZ_compress stores it for an ALGO_INTERP field.
+QoZ::Config interp_config(const faz_interp_params *p, double absEB, double relEB) {
+    QoZ::Config c;
+    c.setDims(p->dims, p->dims + p->N);
+    c.cmprAlgo = QoZ::ALGO_INTERP;
+    c.errorBoundMode = QoZ::EB_ABS;
+    c.absErrorBound = absEB;
+    c.relErrorBound = relEB;
+    c.alpha = p->alpha;
+    c.beta = p->beta;
+    c.interpAlgo = (uint8_t) p->interp_algo;
+    c.interpDirection = (uint8_t) p->direction;
+    c.interpBlockSize = p->block_size;
+    c.quantbinCnt = p->radius * 2;
+    c.levelwisePredictionSelection = p->levelwise;
+    c.interpAlgo_list.assign(p->ops, p->ops + p->levelwise);
+    c.interpDirection_list.assign(p->dirs, p->dirs + p->levelwise);
+    c.maxStep = p->max_step;
+    return c;
+}
+
+faz_interp_params params_from_config(const QoZ::Config &c) {
+    faz_interp_params p;
+    memset(&p, 0, sizeof(p));
+    p.N = c.N;
+    for (int i = 0; i < c.N && i < 4; i++) p.dims[i] = c.dims[i];
+    p.eb = c.absErrorBound;
+    p.interp_algo = c.interpAlgo;
+    p.direction = c.interpDirection;
+    p.alpha = c.alpha;
+    p.beta = c.beta;
+    p.max_step = c.maxStep;
+    p.block_size = c.interpBlockSize;
+    p.radius = c.quantbinCnt / 2;
+    p.levelwise = (int) std::min<size_t>(c.interpAlgo_list.size(), 32);
+    for (int i = 0; i < p.levelwise; i++) {
+        p.ops[i] = c.interpAlgo_list[i];
+        p.dirs[i] = i < (int) c.interpDirection_list.size() ? c.interpDirection_list[i] : 0;
+    }
+    return p;
+}
+
+// calAbsErrorBound (Statistic.hpp:34-66) with the value range from the GPU
+int cal_abs_error_bound(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data) {
+    if (conf.errorBoundMode == QoZ::EB_ABS) return FAZ_OK;
+    auto range = [&](double &rng) -> int {
+        float mn, mx;
+        int rc = faz_gpu_range_f32(ctx, d_data, conf.num, &mn, &mx);
+        if (rc) return rc;
+        rng = (double) (float) (mx - mn);
+        conf.rng = rng;
+        return FAZ_OK;
+    };
+    double rng;
+    int rc;
+    switch (conf.errorBoundMode) {
+        case QoZ::EB_REL:
+            conf.errorBoundMode = QoZ::EB_ABS;
+            if ((rc = range(rng))) return rc;
+            conf.absErrorBound = conf.relErrorBound * rng;
+            return FAZ_OK;
+        case QoZ::EB_PSNR: {
+            conf.errorBoundMode = QoZ::EB_ABS;
+            if ((rc = range(rng))) return rc;
+            double v1 = conf.psnrErrorBound + 10 * std::log10(1 - 2.0 / 3.0 * 0.99);
+            double v2 = v1 / (-20);
+            double v3 = std::pow(10, v2);
+            conf.absErrorBound = rng * v3;
+            conf.relErrorBound = conf.absErrorBound / rng;
+            return FAZ_OK;
+        }
+        case QoZ::EB_L2NORM:
+            conf.errorBoundMode = QoZ::EB_ABS;
+            conf.absErrorBound = std::sqrt(3.0 / conf.num) * conf.l2normErrorBound;
+            return FAZ_OK;
+        case QoZ::EB_ABS_AND_REL:
+            conf.errorBoundMode = QoZ::EB_ABS;
+            if ((rc = range(rng))) return rc;
+            conf.absErrorBound = std::min(conf.absErrorBound, conf.relErrorBound * rng);
+            return FAZ_OK;
+        case QoZ::EB_ABS_OR_REL:
+            conf.errorBoundMode = QoZ::EB_ABS;
+            if ((rc = range(rng))) return rc;
+            conf.absErrorBound = std::max(conf.absErrorBound, conf.relErrorBound * rng);
+            return FAZ_OK;
+        default:
+            return FAZ_ERR_INVALID;
+    }
+}
+
+// Stages host data on the device (no copy when already there).
+const float *to_device(faz_gpu_ctx *ctx, const float *data, size_t n, bool on_device, int &rc) {
+    rc = FAZ_OK;
+    if (on_device) return data;
+    float *d = faz_ctx_work_in(ctx, n);
+    if (!d) {
+        rc = FAZ_ERR_HIP;
+        return nullptr;
+    }
+    if (hipMemcpyAsync(d, data, n * sizeof(float), hipMemcpyHostToDevice, faz_ctx_stream(ctx)) != hipSuccess ||
+        hipStreamSynchronize(faz_ctx_stream(ctx)) != hipSuccess) {
+        rc = FAZ_ERR_HIP;
+        return nullptr;
+    }
+    return d;
+}
+
+// Everything SZ_compress does for one ALGO_INTERP field after the Config copy.
+int compress_interp_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
+    int rc = cal_abs_error_bound(ctx, conf, d_data);
+    if (rc) return rc;
+    faz_interp_params p = params_from_config(conf);
+    if (conf.fixBlockSize || conf.blockwiseTuning || conf.crossBlock || conf.trimToZero || conf.blockOrder) {
+        faz_ctx_set_error(ctx, "engine mode outside the GPU path (fixBlockSize/blockwiseTuning/crossBlock/trimToZero/blockOrder)");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    std::vector<uint8_t> raw;
+    rc = faz_compress_raw_vec(ctx, d_data, nullptr, &p, raw);
+    if (rc) return rc;
+    if (!zstd_wrap(raw, out, QoZ::Config::size_est())) {
+        faz_ctx_set_error(ctx, "zstd compression failed");
+        return FAZ_ERR_INVALID;
+    }
+    append_trailer(conf, out);
+    return FAZ_OK;
+}
+
+int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Config &conf, float *d_out,
+                      size_t num) {
+    if (n < 4) return FAZ_ERR_FORMAT;
+    int confSize;
+    memcpy(&confSize, stream + n - 4, 4);
+    if (confSize <= 0 || (size_t) confSize + 4 > n) return FAZ_ERR_FORMAT;
+    const QoZ::uchar *c = stream + (n - 4 - confSize);
+    conf.load(c);
+    if (conf.N < 1 || conf.N > 4 || conf.num != num) {
+        faz_ctx_set_error(ctx, "stream shape does not match the output buffer");
+        return FAZ_ERR_FORMAT;
+    }
+    if (conf.cmprAlgo != QoZ::ALGO_INTERP || conf.wavelet != 0) {
+        faz_ctx_set_error(ctx, "only ALGO_INTERP streams without wavelets decode on the GPU path");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    std::vector<uint8_t> raw;
+    if (!zstd_unwrap(stream, n - 4 - confSize, raw)) {
+        faz_ctx_set_error(ctx, "zstd frame malformed");
+        return FAZ_ERR_FORMAT;
+    }
+    return faz_gpu_interp_decompress_raw(ctx, raw.data(), raw.size(), conf.N, d_out, num);
+}
+
+// Process-wide context of the C++ API (one per host thread).
+faz_gpu_ctx *api_ctx() {
+    thread_local std::unique_ptr<faz_gpu_ctx, void (*)(faz_gpu_ctx *)> ctx(nullptr, faz_gpu_ctx_destroy);
+    if (!ctx) {
+        const char *dev = getenv("FAZ_DEVICE");
+        faz_gpu_ctx *c = nullptr;
+        int rc = faz_gpu_ctx_create(dev ? atoi(dev) : 0, &c);
+        if (rc) throw faz::Error(rc, "no HIP device available for the FAZ GPU engine");
+        ctx.reset(c);
+    }
+    return ctx.get();
+}
+
+[[noreturn]] void fail(faz_gpu_ctx *ctx, int rc, const char *what) {
+    std::string m = std::string(what) + ": " + faz_gpu_last_error(ctx);
+    throw faz::Error(rc, m);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C++ surface (include/faz/sz.hpp)
+// ---------------------------------------------------------------------------
+template<>
+char *SZ_compress<float>(QoZ::Config &config, const float *data, size_t &outSize) {
+    QoZ::Config conf(config);
+    if (conf.N < 1 || conf.N > 4) throw faz::Error(FAZ_ERR_INVALID, "Data dimension higher than 4 is not supported.");
+    if (conf.cmprAlgo != QoZ::ALGO_INTERP)
+        throw faz::Error(FAZ_ERR_UNSUPPORTED, std::string("GPU engine does not serve ") + QoZ::ALGO_STR[conf.cmprAlgo]);
+    faz_gpu_ctx *ctx = api_ctx();
+    int rc;
+    const float *d = to_device(ctx, data, conf.num, false, rc);
+    if (rc) fail(ctx, rc, "staging input");
+    std::vector<uint8_t> out;
+    rc = compress_interp_stream(ctx, conf, d, out);
+    if (rc) fail(ctx, rc, "SZ_compress");
+    char *buf = new char[out.size()];
+    memcpy(buf, out.data(), out.size());
+    outSize = out.size();
+    return buf;
+}
+
+template<>
+void SZ_decompress<float>(QoZ::Config &config, char *cmpData, size_t cmpSize, float *&decData) {
+    QoZ::Config conf(config);
+    if (cmpSize < 4) throw faz::Error(FAZ_ERR_FORMAT, "stream too short");
+    int confSize;
+    memcpy(&confSize, cmpData + cmpSize - 4, 4);
+    if (confSize <= 0 || (size_t) confSize + 4 > cmpSize) throw faz::Error(FAZ_ERR_FORMAT, "bad Config trailer");
+    const QoZ::uchar *c = (const QoZ::uchar *) cmpData + (cmpSize - 4 - confSize);
+    conf.load(c);
+    faz_gpu_ctx *ctx = api_ctx();
+    float *dev = faz_ctx_work_in(ctx, conf.num);
+    if (!dev) fail(ctx, FAZ_ERR_HIP, "device buffer");
+    QoZ::Config loaded;
+    int rc = decompress_stream(ctx, (const uint8_t *) cmpData, cmpSize, loaded, dev, conf.num);
+    if (rc) fail(ctx, rc, "SZ_decompress");
+    if (decData == nullptr) decData = new float[conf.num];
+    if (hipMemcpy(decData, dev, conf.num * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw faz::Error(FAZ_ERR_HIP, "copying the reconstruction to the host");
+    config = loaded;
+}
+
+// ---------------------------------------------------------------------------
+// INI configuration (Config.hpp:91-233)
+// ---------------------------------------------------------------------------
+namespace {
+std::string trim(const std::string &s) {
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+bool as_bool(const std::string &v, bool def) {
+    std::string l;
+    for (char ch: v) l += (char) tolower(ch);
+    if (l == "true" || l == "yes" || l == "on" || l == "1") return true;
+    if (l == "false" || l == "no" || l == "off" || l == "0") return false;
+    return def;
+}
+}  // namespace
+
+bool QoZ::Config::loadcfg(const std::string &cfgpath) {
+    std::ifstream f(cfgpath);
+    if (!f) return false;
+    std::string section, line;
+    std::vector<std::pair<std::string, std::string>> kv;  // "section.key" → value
+    while (std::getline(f, line)) {
+        std::string t = trim(line);
+        if (t.empty() || t[0] == ';' || t[0] == '#') continue;
+        if (t[0] == '[') {
+            size_t e = t.find(']');
+            if (e == std::string::npos) return false;
+            section = trim(t.substr(1, e - 1));
+            continue;
+        }
+        size_t eq = t.find_first_of("=:");
+        if (eq == std::string::npos) return false;
+        std::string val = trim(t.substr(eq + 1));
+        size_t cmt = val.find(" ;");
+        if (cmt != std::string::npos) val = trim(val.substr(0, cmt));
+        std::string key = section + "." + trim(t.substr(0, eq));
+        for (auto &ch: key) ch = (char) tolower(ch);
+        kv.push_back({key, val});
+    }
+    auto get = [&](const char *sec, const char *key, std::string &out) {
+        std::string k = std::string(sec) + "." + key;
+        for (auto &ch: k) ch = (char) tolower(ch);
+        bool found = false;
+        for (auto &p: kv)
+            if (p.first == k) {
+                out = p.second;
+                found = true;
+            }
+        return found;
+    };
+    std::string v;
+    if (get("GlobalSettings", "CmprAlgo", v))
+        for (int i = 0; i < 4; i++)
+            if (v == ALGO_STR[i]) cmprAlgo = (uint8_t) i;
+    if (get("GlobalSettings", "ErrorBoundMode", v))
+        for (int i = 0; i < 6; i++)
+            if (v == EB_STR[i]) errorBoundMode = (uint8_t) i;
+    if (get("GlobalSettings", "tuningTarget", v))
+        for (int i = 0; i < 4; i++)
+            if (v == TUNING_TARGET_STR[i]) tuningTarget = (uint8_t) i;
+    auto real = [&](const char *sec, const char *key, double &dst) {
+        if (get(sec, key, v)) dst = strtod(v.c_str(), nullptr);
+    };
+    auto integer = [&](const char *sec, const char *key, auto &dst) {
+        if (get(sec, key, v)) dst = (std::remove_reference_t<decltype(dst)>) strtol(v.c_str(), nullptr, 0);
+    };
+    auto boolean = [&](const char *sec, const char *key, bool &dst) {
+        if (get(sec, key, v)) dst = as_bool(v, dst);
+    };
+    real("GlobalSettings", "AbsErrorBound", absErrorBound);
+    real("GlobalSettings", "RelErrorBound", relErrorBound);
+    real("GlobalSettings", "PSNRErrorBound", psnrErrorBound);
+    real("GlobalSettings", "L2NormErrorBound", l2normErrorBound);
+    real("AlgoSettings", "alpha", alpha);
+    real("AlgoSettings", "beta", beta);
+    real("AlgoSettings", "autoTuningRate", autoTuningRate);
+    real("AlgoSettings", "predictorTuningRate", predictorTuningRate);
+    real("AlgoSettings", "wavelet_rel_coeff", wavelet_rel_coeff);
+    real("AlgoSettings", "preTrim", preTrim);
+    real("AlgoSettings", "waveletTuningRate", waveletTuningRate);
+    real("AlgoSettings", "waveletBrFix", waveletBrFix);
+    real("AlgoSettings", "waveletBrFix2", waveletBrFix2);
+    real("AlgoSettings", "waveletMseFix", waveletMseFix);
+    real("AlgoSettings", "waveletMseFix2", waveletMseFix2);
+    real("AlgoSettings", "lorenzoBrFix", lorenzoBrFix);
+    boolean("GlobalSettings", "OpenMP", openmp);
+    boolean("AlgoSettings", "Lorenzo", lorenzo);
+    boolean("AlgoSettings", "Lorenzo2ndOrder", lorenzo2);
+    boolean("AlgoSettings", "Regression", regression);
+    boolean("AlgoSettings", "Regression2ndOrder", regression2);
+    if (get("AlgoSettings", "InterpolationAlgo", v)) {
+        if (v == INTERP_ALGO_STR[0]) interpAlgo = INTERP_ALGO_LINEAR;
+        if (v == INTERP_ALGO_STR[1]) interpAlgo = INTERP_ALGO_CUBIC;
+    }
+    integer("AlgoSettings", "InterpDirection", interpDirection);
+    integer("AlgoSettings", "InterpBlockSize", interpBlockSize);
+    integer("AlgoSettings", "BlockSize", blockSize);
+    integer("AlgoSettings", "QuantizationBinTotal", quantbinCnt);
+    integer("AlgoSettings", "maxStep", maxStep);
+    integer("AlgoSettings", "sampleBlockSize", sampleBlockSize);
+    integer("AlgoSettings", "levelwisePredictionSelection", levelwisePredictionSelection);
+    integer("AlgoSettings", "testLorenzo", testLorenzo);
+    integer("AlgoSettings", "profiling", profiling);
+    integer("AlgoSettings", "SSIMBlockSize", SSIMBlockSize);
+    integer("AlgoSettings", "fixBlockSize", fixBlockSize);
+    boolean("AlgoSettings", "verbose", verbose);
+    boolean("AlgoSettings", "QoZ", QoZ);
+    boolean("AlgoSettings", "FZ", FZ);
+    boolean("AlgoSettings", "sperrWithoutWave", sperrWithoutWave);
+    boolean("AlgoSettings", "pyBind", pyBind);
+    integer("AlgoSettings", "blockwiseSampleBlockSize", blockwiseSampleBlockSize);
+    integer("AlgoSettings", "abList", abList);
+    integer("AlgoSettings", "crossBlock", crossBlock);
+    integer("AlgoSettings", "sampleBlockSampleBlockSize", sampleBlockSampleBlockSize);
+    boolean("AlgoSettings", "peTracking", peTracking);
+    integer("AlgoSettings", "wavelet", wavelet);
+    integer("AlgoSettings", "offsetPredictor", offsetPredictor);
+    integer("AlgoSettings", "trimToZero", trimToZero);
+    integer("AlgoSettings", "pid", pid);
+    integer("AlgoSettings", "blockOrder", blockOrder);
+    integer("AlgoSettings", "coeffTracking", coeffTracking);
+    integer("AlgoSettings", "waveletTest", waveletTest);
+    integer("AlgoSettings", "waveletAutoTuning", waveletAutoTuning);
+    integer("AlgoSettings", "var_first", var_first);
+    integer("AlgoSettings", "profStride", profStride);
+    integer("AlgoSettings", "sperr", sperr);
+    integer("AlgoSettings", "waveAutoFix", waveAutoFix);
+    integer("AlgoSettings", "conditioning", conditioning);
+    integer("AlgoSettings", "fixWave", fixWave);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI forms (faz_gpu.h)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+faz_status faz_sz_compress_f32(faz_gpu_ctx *ctx, const faz_interp_params *p, int rel_mode, double eb,
+                               const float *data, int data_on_device, uint8_t **out, size_t *out_size) {
+    if (!ctx || !p || !data || !out || !out_size || p->N < 1 || p->N > 4) return FAZ_ERR_INVALID;
+    size_t num = 1;
+    for (int i = 0; i < p->N; i++) num *= p->dims[i];
+    QoZ::Config conf = interp_config(p, 0.0, -1.0);
+    if (rel_mode) {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = eb;
+    } else {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = eb;
+    }
+    int rc;
+    const float *d = to_device(ctx, data, num, data_on_device != 0, rc);
+    if (rc) return rc;
+    std::vector<uint8_t> buf;
+    rc = compress_interp_stream(ctx, conf, d, buf);
+    if (rc) return rc;
+    uint8_t *m = (uint8_t *) malloc(buf.size());
+    if (!m) return FAZ_ERR_INVALID;
+    memcpy(m, buf.data(), buf.size());
+    *out = m;
+    *out_size = buf.size();
+    return FAZ_OK;
+}
+
+faz_status faz_sz_decompress_f32(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, float *out, size_t num,
+                                 int out_on_device) {
+    if (!ctx || !stream || !out) return FAZ_ERR_INVALID;
+    float *d = out;
+    if (!out_on_device) {
+        d = faz_ctx_work_in(ctx, num);
+        if (!d) return FAZ_ERR_HIP;
+    }
+    QoZ::Config conf;
+    int rc = decompress_stream(ctx, stream, n, conf, d, num);
+    if (rc) return rc;
+    if (!out_on_device && hipMemcpy(out, d, num * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return FAZ_ERR_HIP;
+    return FAZ_OK;
+}
+
+void faz_free(void *p) { free(p); }
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+/* faz_gpu.h — C-ABI of the MI355X (gfx950) FAZ compression hot path.
+ *
+ * The reference (JLiu-1/FAZ, namespace QoZ) has no FFI: its hot path is the
+ * C++ template engine SZInterpolationCompressor<T,N,LinearQuantizer,
+ * HuffmanEncoder,Lossless_zstd>.  Each entry point below replaces one piece of
+ * that engine and names the reference code it stands in for.  The C++ host
+ * layer (include/faz/sz.hpp: QoZ::Config, SZ_compress, SZ_decompress) and the
+ * `faz` command line sit on top of these calls.
+ *
+ * Conventions
+ *   - every call returns faz_status (0 = ok, < 0 = error; never exits);
+ *     faz_gpu_last_error() gives a message for the last failing call;
+ *   - d_ pointers are device (HBM) pointers, h_ pointers host pointers;
+ *   - a context owns its device scratch and stream; output device pointers
+ *     returned through a context stay valid until the next call on it;
+ *   - one context per host thread; calls are stream-ordered and synchronous
+ *     at return.
+ */
+#ifndef FAZ_GPU_H
+#define FAZ_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int faz_status;
+#define FAZ_OK 0
+#define FAZ_ERR_INVALID (-1)     /* bad argument */
+#define FAZ_ERR_HIP (-2)         /* HIP runtime error or no device */
+#define FAZ_ERR_UNSUPPORTED (-3) /* engine mode outside the GPU path */
+#define FAZ_ERR_CAPACITY (-4)    /* caller buffer too small (size written back) */
+#define FAZ_ERR_FORMAT (-5)      /* malformed or truncated stream */
+
+typedef struct faz_gpu_ctx faz_gpu_ctx;
+
+/* Interpolation engine parameters: the QoZ::Config fields read by
+ * SZInterpolationCompressor::compress (include/QoZ/compressor/
+ * SZInterpolationCompressor.hpp:333-345) plus its tuning arguments. */
+typedef struct {
+    int N;                  /* 1..4 */
+    size_t dims[4];         /* slowest first, last dimension contiguous */
+    double eb;              /* absolute error bound */
+    int interp_algo;        /* 0 linear, 1 cubic (Config::interpAlgo) */
+    int direction;          /* Config::interpDirection: permutation index */
+    double alpha, beta;     /* level-wise error-bound scaling */
+    size_t max_step;        /* anchor stride (Config::maxStep), 0 = none */
+    int block_size;         /* Config::interpBlockSize (even) */
+    int radius;             /* Config::quantbinCnt / 2, <= 32768 */
+    int levelwise;          /* entries used in ops/dirs */
+    uint8_t ops[32];        /* Config::interpAlgo_list */
+    uint8_t dirs[32];       /* Config::interpDirection_list */
+    int tuning;             /* 0 compress, 1 codes + squared error, 2 |prediction error| */
+    int start_level;        /* 0 = top level */
+    int end_level;          /* 0 = down to level 1 */
+} faz_interp_params;
+
+/* Results of faz_gpu_interp_compress_f32 (device pointers owned by ctx). */
+typedef struct {
+    uint16_t *d_codes;      /* quantization codes in the reference's push order */
+    size_t n_codes;
+    float *d_unpred;        /* unpredictable values, same order */
+    size_t n_unpred;
+    double loss;            /* tuning 1: sum of squared errors; 2: sum |pred error| */
+    size_t level_counts[64];/* tuning modes: codes pushed after each level */
+    int levels;             /* interpolation levels (SIC:820-852) */
+} faz_interp_out;
+
+/* Huffman code table built on the host from the GPU histogram. */
+typedef struct {
+    int32_t offset;         /* smallest symbol (HuffmanEncoder::offset) */
+    int32_t n_states;       /* max - offset + 1 entries in code/len */
+    const uint64_t *code;   /* MSB-aligned codewords, indexed by symbol - offset */
+    const uint8_t *len;     /* codeword lengths (0..64) */
+} faz_huff_table;
+
+/* --- context ------------------------------------------------------------ */
+faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out);
+void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx);
+const char *faz_gpu_last_error(const faz_gpu_ctx *ctx);
+/* Device id the context runs on, or -1. */
+int faz_gpu_ctx_device(const faz_gpu_ctx *ctx);
+
+/* --- A1: QoZ::data_range (include/QoZ/utils/Statistic.hpp:11-21) -------- */
+faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, float *h_min, float *h_max);
+
+/* --- A2..A10: SZInterpolationCompressor::compress level/block/pass loop,
+ * quantize / quantize_tuning and LinearQuantizer::quantize_and_overwrite
+ * (SIC:328-476, 924-974, 1306-1417, 3076-3442; IntegerQuantizer.hpp:69-108).
+ * d_in is read only; d_recon (n floats, may equal d_in) receives the values
+ * the reference leaves in its data buffer. */
+faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
+                                       const faz_interp_params *p, faz_interp_out *out);
+
+/* --- A12 input: the frequency loop of HuffmanEncoder::init
+ * (include/QoZ/encoder/HuffmanEncoder.hpp:529-538) plus the first sequence
+ * index of every symbol, which fixes ska::unordered_map's iteration order.
+ * h_freq and h_first have 65536 entries; h_first is UINT32_MAX for absent. */
+faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, uint32_t *h_freq,
+                                uint32_t *h_first);
+
+/* --- A14: HuffmanEncoder::encode (HuffmanEncoder.hpp:150-227).  Writes the
+ * MSB-first bitstream (without the size_t prefix) to d_bits (capacity
+ * cap_bytes) and its byte count to *nbytes. */
+faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, const faz_huff_table *t,
+                                  uint8_t *d_bits, size_t cap_bytes, size_t *nbytes);
+
+/* --- A12/A13 host side: HuffmanEncoder::init + save (HuffmanEncoder.hpp:
+ * 103-137, 455-633).  From per-symbol counts and first indices (65536 entries
+ * each) builds the tree exactly as the reference does and writes the code
+ * table (code/len, 65536 entries each, indexed by symbol - *offset) and the
+ * serialised tree bytes (capacity tree_cap, size to *tree_size). */
+faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, int32_t *offset, int32_t *n_states,
+                             uint64_t *code, uint8_t *len, uint8_t *tree, size_t tree_cap, size_t *tree_size);
+
+/* --- A17: HuffmanEncoder::decode + SZInterpolationCompressor::decompress
+ * (HuffmanEncoder.hpp:236-296; SIC:42-199) on a pre-zstd engine buffer. */
+faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
+                                         size_t num);
+
+/* --- A11..A16 on one GPU: the whole engine buffer that
+ * SZInterpolationCompressor::compress hands to zstd (SIC:485-518): header,
+ * LinearQuantizer::save (IntegerQuantizer.hpp:185-201), Huffman tree and
+ * bits.  Written to h_raw (capacity cap), size to *raw_size. */
+faz_status faz_gpu_interp_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                                  uint8_t *h_raw, size_t cap, size_t *raw_size);
+
+/* --- The same engine buffer in two steps, so the device work can be timed
+ * on its own: faz_gpu_compress_f32 runs the value range (rel_mode != 0:
+ * eb is relative to max-min, Statistic.hpp:34-41), interpolation +
+ * quantization, histogram, outlier compaction, the host tree and the encode
+ * (SIC:328-518) and leaves the result in the context; *raw_size gets the
+ * engine buffer size.  faz_gpu_fetch_raw copies it to the host. */
+faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                                int rel_mode, double eb, size_t *raw_size);
+faz_status faz_gpu_fetch_raw(faz_gpu_ctx *ctx, uint8_t *h_raw, size_t cap);
+
+/* --- SZ_compress / SZ_decompress for ALGO_INTERP streams
+ * (include/QoZ/api/sz.hpp:54-205, SZInterp.hpp:387-431,
+ * Lossless_zstd.hpp:22-47, Config.hpp:240-388).  `rel_mode` != 0 applies
+ * calAbsErrorBound's EB_REL conversion (Statistic.hpp:34-41) to eb.
+ * data is a host pointer unless data_on_device.  *out is malloc'ed; free it
+ * with faz_free. */
+faz_status faz_sz_compress_f32(faz_gpu_ctx *ctx, const faz_interp_params *p, int rel_mode, double eb,
+                               const float *data, int data_on_device, uint8_t **out, size_t *out_size);
+faz_status faz_sz_decompress_f32(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, float *out, size_t num,
+                                 int out_on_device);
+void faz_free(void *p);
+
+/* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
+ * milliseconds, measured with HIP events on the context stream:
+ * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
+ * [3] host tree build, [4] encode, [5] device total, [6] dominant kernel
+ * (interpolation pass launches) summed. */
+faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
+
+/* Per-launch HIP-event timing of the interpolation pass kernels (the
+ * dominant kernel), accumulated while enabled: total milliseconds, launches
+ * and predicted points.  Enabling resets the counters. */
+faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on);
+faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -28,6 +28,7 @@
 #include "QoZ/lossless/Lossless_zstd.hpp"
 #include "QoZ/lossless/Lossless_bypass.hpp"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -241,6 +242,40 @@ static int run_decompress(const Args &a) {
     return 0;
 }
 
+// One SZ_compress-equivalent ALGO_INTERP compression (range, engine, zstd,
+// trailer) timed with a steady clock; prints seconds and the stream size.
+template<QoZ::uint N>
+static int run_time(const Args &a) {
+    typedef float T;
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    QoZ::Config conf = make_config(a);
+    if (input.size() != conf.num) { fprintf(stderr, "size mismatch\n"); return 2; }
+    int reps = atoi(get(a, "reps", "1").c_str());
+    double best = 1e30;
+    size_t outSize = 0;
+    for (int r = 0; r < reps; r++) {
+        QoZ::Config c3(conf);
+        c3.verbose = 0;
+        auto t0 = std::chrono::steady_clock::now();
+        std::vector<T> data(input);  // SZ_compress copies its input (sz.hpp:57)
+        QoZ::calAbsErrorBound(c3, data.data());
+        auto sz = QoZ::SZInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_zstd>(
+                QoZ::LinearQuantizer<T>(c3.absErrorBound, c3.quantbinCnt / 2), QoZ::HuffmanEncoder<int>(),
+                QoZ::Lossless_zstd());
+        char *cmpData = (char *) sz.compress(c3, data.data(), outSize);
+        QoZ::uchar *pos = (QoZ::uchar *) cmpData + outSize;
+        c3.save(pos);
+        size_t newSize = (char *) pos - cmpData;
+        QoZ::write(int(newSize - outSize), pos);
+        outSize = (char *) pos - cmpData;
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        delete[] cmpData;
+        if (secs < best) best = secs;
+    }
+    printf("{\"seconds\": %.6f, \"bytes\": %zu, \"points\": %zu}\n", best, outSize, conf.num);
+    return 0;
+}
+
 static int run_huffman(const Args &a) {
     std::vector<int> codes = read_vec<int>(get(a, "in"));
     QoZ::HuffmanEncoder<int> enc;
@@ -268,6 +303,15 @@ int main(int argc, char **argv) {
             case 2: return run_interp<2>(a);
             case 3: return run_interp<3>(a);
             case 4: return run_interp<4>(a);
+            default: return 2;
+        }
+    }
+    if (cmd == "time") {
+        switch (parse_list(get(a, "dims")).size()) {
+            case 1: return run_time<1>(a);
+            case 2: return run_time<2>(a);
+            case 3: return run_time<3>(a);
+            case 4: return run_time<4>(a);
             default: return 2;
         }
     }

@@ -1,0 +1,150 @@
+"""Writes the golden fixtures under tests/golden/ from the reference engine.
+
+Runs only where the reference tree is mounted and oracle/_ref/ref_harness has
+been built from it (``make -C oracle ref``).  Each fixture stores the input,
+the parameters and what the reference engine produced for it: quantization
+codes, unpredictable list, reconstruction, the pre-zstd engine buffer and the
+full stream (zstd frame from the reference's vendored zstd 1.4.5 + Config
+trailer).  Huffman fixtures store a symbol sequence and the reference
+HuffmanEncoder's save+encode bytes; tuning fixtures the codes, loss and level
+counts of SZInterpolationCompressor::compress in tuning modes 1 and 2.
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def smooth(rng, shape, k=3):
+    x = rng.standard_normal(shape)
+    for ax in range(len(shape)):
+        for _ in range(k):
+            x = np.cumsum(x, axis=ax)
+    x = (x - x.min()) / (x.max() - x.min() + 1e-30)
+    return x.astype(np.float32)
+
+
+def cases():
+    r = np.random.default_rng(20261203)
+    c = []
+    add = lambda name, x, **kw: c.append((name, x, kw))
+    add("c3d_17_cubic_rel1e-3", smooth(r, (17, 17, 17)), mode="rel", eb=1e-3)
+    add("c3d_17_anchor32", smooth(r, (17, 17, 17)), mode="rel", eb=1e-3, maxstep=32)
+    add("c3d_33x20x9_levelwise", smooth(r, (33, 20, 9), 2), mode="rel", eb=1e-2, maxstep=8, alpha=1.5,
+        beta=3.0, ops=[1, 0, 1], dirs=[0, 5, 2], dir=2)
+    add("c3d_65x33x17_alpha", smooth(r, (65, 33, 17)), mode="rel", eb=1e-4, maxstep=32, alpha=1.25, beta=2.0)
+    add("c3d_33cube_degenerate", smooth(r, (33, 33, 33), 2), mode="rel", eb=1e-3, dir=4)
+    add("c3d_24x31x20_bs16", smooth(r, (24, 31, 20)), mode="rel", eb=5e-4, bs=16, dir=1, maxstep=16)
+    add("c3d_20_qbins1024", smooth(r, (20, 20, 20), 1), mode="rel", eb=1e-5, qbins=1024)
+    add("c3d_13_outliers", r.random((13, 13, 13)).astype(np.float32), mode="abs", eb=1e-9)
+    spikes = smooth(r, (11, 12, 13))
+    spikes.flat[r.choice(spikes.size, 20, replace=False)] = np.float32(3e30)
+    add("c3d_spikes_overflow", spikes, mode="abs", eb=1e-3)
+    add("c3d_const_9", np.full((9, 9, 9), 3.25, np.float32), mode="abs", eb=1e-2)
+    add("c3d_linear_dir3", smooth(r, (19, 23, 29), 2), mode="rel", eb=1e-3, algo=0, dir=3)
+    add("c3d_alpha_lt1", smooth(r, (21, 18, 27)), mode="rel", eb=1e-3, alpha=0.5, beta=0.2, maxstep=16)
+    add("c2d_45x37_anchor", smooth(r, (45, 37)), mode="rel", eb=1e-2, maxstep=32)
+    add("c2d_100x37_linear", smooth(r, (100, 37), 2), mode="rel", eb=1e-4, algo=0, dir=1, alpha=0.5, beta=0.2)
+    add("c2d_90x130_cubic", smooth(r, (90, 130), 2), mode="rel", eb=1e-3, maxstep=64, ops=[1] * 6,
+        dirs=[0, 1, 0, 1, 0, 1])
+    add("c1d_1000", smooth(r, (1000,), 2), mode="rel", eb=1e-3)
+    add("c1d_4097_linear", smooth(r, (4097,), 1), mode="rel", eb=1e-4, algo=0)
+    add("c4d_9x10x11x12", smooth(r, (9, 10, 11, 12), 1), mode="rel", eb=1e-2, dir=17)
+    add("c3d_1x40x40_flat_axis", smooth(r, (1, 40, 40), 2), mode="rel", eb=1e-3, maxstep=32)
+    add("c3d_negative_values", (smooth(r, (16, 17, 18)) - 0.7) * 1e3, mode="rel", eb=1e-3, maxstep=32)
+    return c
+
+
+def huffman_cases():
+    r = np.random.default_rng(7)
+    c = []
+    c.append(("h_geometric", (32768 + np.round(r.laplace(0, 6, 20000))).astype(np.int32)))
+    c.append(("h_600_symbols", (30000 + r.integers(0, 600, 30000)).astype(np.int32)))
+    z = r.integers(0, 40000, 200000).astype(np.int32)
+    c.append(("h_40000_symbols_u32_tree", z))
+    c.append(("h_single", np.full(1000, 32768, np.int32)))
+    c.append(("h_two", np.array([5, 9, 9, 5, 9] * 40, np.int32)))
+    c.append(("h_with_zero", np.where(r.random(5000) < 0.1, 0, 32768 + r.integers(-3, 4, 5000)).astype(np.int32)))
+    c.append(("h_negative", r.integers(-50, 50, 3000).astype(np.int32)))
+    fib = []
+    a, b = 1, 1
+    for s in range(30):
+        fib += [s] * a
+        a, b = b, a + b
+    c.append(("h_fibonacci_deep", np.array(fib, np.int32)[r.permutation(len(fib))][:200000]))
+    return c
+
+
+def tuning_cases():
+    r = np.random.default_rng(11)
+    c = []
+    x = smooth(r, (33, 33, 33), 2)
+    c.append(("t3d_33_mode1", x, dict(mode="abs", eb=1e-3, maxstep=32, tuning=1)))
+    c.append(("t3d_33_mode2_L3", x, dict(mode="abs", eb=1e-3, maxstep=32, tuning=2, start=3, end=2, algo=0)))
+    c.append(("t3d_33_mode2_L1", x, dict(mode="abs", eb=1e-3, maxstep=32, tuning=2, start=1, end=0, dir=5)))
+    c.append(("t3d_17_mode1_lw", smooth(r, (17, 17, 17)), dict(mode="abs", eb=2e-3, maxstep=16, tuning=1,
+                                                               alpha=1.5, beta=2, ops=[1, 1, 0, 1],
+                                                               dirs=[0, 5, 0, 5])))
+    c.append(("t2d_65_mode1", smooth(r, (65, 65)), dict(mode="abs", eb=1e-3, maxstep=64, tuning=1)))
+    return c
+
+
+def harness_args(inp, out, dims, kw):
+    a = [HARNESS, "interp", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, dims)),
+         f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"algo={kw.get('algo', 1)}",
+         f"dir={kw.get('dir', 0)}", f"alpha={kw.get('alpha', -1.0)!r}", f"beta={kw.get('beta', -1.0)!r}",
+         f"maxstep={kw.get('maxstep', 0)}", f"bs={kw.get('bs', 32)}", f"qbins={kw.get('qbins', 65536)}"]
+    if kw.get("ops"):
+        a += ["ops=" + ",".join(map(str, kw["ops"])), "dirs=" + ",".join(map(str, kw["dirs"]))]
+    if kw.get("tuning"):
+        a += [f"tuning={kw['tuning']}", f"start={kw.get('start', 0)}", f"end={kw.get('end', 0)}"]
+    return a
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("oracle/_ref/ref_harness missing: build it with `make -C oracle ref`")
+    tmp = tempfile.mkdtemp()
+    for name, x, kw in cases():
+        inp = os.path.join(tmp, "in.f32")
+        out = os.path.join(tmp, "o")
+        x.astype(np.float32).tofile(inp)
+        subprocess.check_call(harness_args(inp, out, x.shape, kw))
+        ld = lambda ext, dt: np.fromfile(out + ext, dt)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32),
+                            codes=ld(".codes", np.int32), recon=ld(".recon", np.float32),
+                            unpred=ld(".unpred", np.float32), raw=ld(".raw", np.uint8),
+                            stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
+                            params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
+        print("wrote", name)
+    for name, codes in huffman_cases():
+        inp = os.path.join(tmp, "codes.i32")
+        out = os.path.join(tmp, "huff.bin")
+        codes.tofile(inp)
+        subprocess.check_call([HARNESS, "huffman", f"in={inp}", f"out={out}"])
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), codes=codes, encoded=np.fromfile(out, np.uint8))
+        print("wrote", name)
+    for name, x, kw in tuning_cases():
+        inp = os.path.join(tmp, "in.f32")
+        out = os.path.join(tmp, "t")
+        x.astype(np.float32).tofile(inp)
+        subprocess.check_call(harness_args(inp, out, x.shape, kw))
+        ld = lambda ext, dt: np.fromfile(out + ext, dt)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32),
+                            codes=ld(".codes", np.int32), recon=ld(".recon", np.float32),
+                            loss=ld(".loss", np.float64), level_counts=ld(".levelcounts", np.uint64),
+                            params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,41 @@
+"""Loading helpers for the golden fixtures written by tests/golden/make_golden.py."""
+import glob
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def names(prefix):
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, prefix + "*.npz")))
+
+
+INTERP = names("c")
+HUFFMAN = names("h_")
+TUNING = names("t")
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    d = {k: z[k] for k in z.files}
+    if "params" in d:
+        d["params"] = json.loads(str(d["params"]))
+    return d
+
+
+def param_kwargs(case):
+    """Engine parameters of a fixture, with its absolute error bound."""
+    p = case["params"]
+    if "eb" in case and np.asarray(case["eb"]).size:
+        eb = float(np.asarray(case["eb"]).ravel()[0])
+    elif p.get("mode", "abs") == "abs":
+        eb = float(p["eb"])
+    else:
+        x = case["input"].ravel()
+        eb = float(p["eb"]) * float(np.float32(x.max() - x.min()))
+    return dict(dims=p["dims"], eb=eb, algo=p.get("algo", 1), direction=p.get("dir", 0),
+                alpha=p.get("alpha", -1.0), beta=p.get("beta", -1.0), max_step=p.get("maxstep", 0),
+                block_size=p.get("bs", 32), radius=p.get("qbins", 65536) // 2,
+                ops=tuple(p.get("ops", ())), dirs=tuple(p.get("dirs", ())))

@@ -1,0 +1,149 @@
+"""GPU parity of the interpolation + quantization + Huffman path (needs an MI355X).
+
+Bar: bit-exact codes, unpredictable lists, reconstructions, pre-zstd engine
+buffers and Config trailers against the reference's golden vectors and the
+oracle; the GPU decoder reads the reference's own streams; at the full
+512^3 size, size-independent properties (error bound in FP64, decode of the
+stream reproduces the reconstruction, code/outlier counts agree).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from golden_cases import INTERP, load, param_kwargs
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _faz_params(case):
+    import faz_amd
+    return faz_amd.make_params(**param_kwargs(case))
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x, np.float32).ravel()).cuda()
+
+
+@pytest.mark.parametrize("name", INTERP)
+def test_golden_codes_unpred_recon(engine, name):
+    c = load(name)
+    print(f"[gpu] golden case {name} dims={c['params']['dims']}", flush=True)
+    r = engine.interp_compress(_dev(c["input"]), _faz_params(c))
+    assert np.array_equal(r["codes"].astype(np.int32), c["codes"])
+    assert np.array_equal(r["unpred"].view(np.uint32), c["unpred"].view(np.uint32))
+    rec = r["recon"].cpu().numpy()
+    assert np.array_equal(rec.view(np.uint32), c["recon"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", INTERP)
+def test_golden_raw_buffer(engine, name):
+    c = load(name)
+    print(f"[gpu] raw buffer {name}", flush=True)
+    raw = engine.interp_raw(_dev(c["input"]), _faz_params(c))
+    assert raw == c["raw"].tobytes()
+
+
+@pytest.mark.parametrize("name", INTERP)
+def test_golden_stream_and_reference_stream_decode(engine, name):
+    c = load(name)
+    p = c["params"]
+    print(f"[gpu] stream {name}", flush=True)
+    import faz_amd
+    kw = param_kwargs(c)
+    s = engine.sz_compress(c["input"], faz_amd.make_params(**kw), p["eb"], rel=p.get("mode", "abs") == "rel")
+    ref = c["stream"].tobytes()
+    conf = int(np.frombuffer(ref[-4:], np.int32)[0])
+    assert s[-4 - conf:] == ref[-4 - conf:]
+    num = int(np.prod(p["dims"]))
+    # our stream decodes on the oracle, the reference's stream on the GPU
+    assert np.array_equal(oracle.decompress_stream(s, num).view(np.uint32), c["recon"].view(np.uint32))
+    dec = engine.sz_decompress(ref, num)
+    assert np.array_equal(dec.view(np.uint32), c["recon"].view(np.uint32))
+
+
+def _field(shape, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(shape)
+    for ax in range(len(shape)):
+        x = np.cumsum(x, axis=ax)
+    x += 0.05 * rng.standard_normal(shape) * x.std()
+    return x.astype(np.float32)
+
+
+CONFIGS = [
+    ((64, 96, 80), dict(algo=1, direction=0, max_step=32), 1e-3),
+    ((97, 65, 130), dict(algo=1, direction=5, max_step=32, alpha=1.25, beta=2.0), 1e-4),
+    ((129, 129, 129), dict(algo=0, direction=2), 1e-3),
+    ((70, 200, 33), dict(algo=1, direction=3, max_step=16, ops=(1, 0, 1, 1), dirs=(0, 5, 1, 2)), 1e-2),
+    ((300, 421), dict(algo=1, direction=1, max_step=64), 1e-3),
+    ((100000,), dict(algo=1), 1e-4),
+    ((12, 20, 24, 30), dict(algo=1, direction=7), 1e-3),
+    ((48, 48, 48), dict(algo=1, max_step=32, block_size=8), 1e-3),
+    ((40, 50, 60), dict(algo=1, radius=256), 1e-5),
+]
+
+
+@pytest.mark.parametrize("i", range(len(CONFIGS)))
+def test_random_fields_against_oracle(engine, i):
+    import faz_amd
+    shape, kw, rel = CONFIGS[i]
+    x = _field(shape, 100 + i)
+    eb = rel * float(np.float32(x.max() - x.min()))
+    print(f"[gpu] oracle parity {shape} {kw} eb={eb}", flush=True)
+    o = oracle.interp_compress(x, oracle.make_params(shape, eb, **kw))
+    r = engine.interp_compress(_dev(x), faz_amd.make_params(shape, eb, **kw))
+    assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
+    assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
+    assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+    raw = engine.interp_raw(_dev(x), faz_amd.make_params(shape, eb, **kw))
+    assert raw == oracle.interp_raw(x, oracle.make_params(shape, eb, **kw))
+    out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+    engine.decompress_raw(raw, len(shape), out)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+
+
+def test_range_matches_oracle(engine):
+    x = _field((33, 45, 57), 5) - 3.0
+    mn, mx = engine.range(_dev(x))
+    assert (mn, mx) == oracle.data_range(x)
+
+
+def test_rejects_unsupported_modes(engine):
+    import faz_amd
+    x = _dev(np.zeros(64, np.float32))
+    with pytest.raises(faz_amd.FazError):
+        engine.interp_compress(x, faz_amd.make_params((64,), 1e-3, max_step=16))  # 1-D anchors
+    with pytest.raises(faz_amd.FazError):
+        engine.interp_compress(x, faz_amd.make_params((64,), 1e-3, radius=65536))
+    with pytest.raises(faz_amd.FazError):
+        engine.interp_compress(x, faz_amd.make_params((64,), 0.0))
+
+
+def _grf_512(seed=0):
+    """512^3 GRF with P(k) ~ k^-11/3 plus 16 tanh fronts, synthesised on the GPU."""
+    import bench
+    return bench.make_field((512, 512, 512), seed=seed, fronts=16)
+
+
+@pytest.mark.parametrize("rel", [1e-3, 1e-5])
+def test_full_size_512_properties(engine, rel):
+    import faz_amd
+    x = _grf_512()
+    mn, mx = engine.range(x)
+    eb = rel * float(np.float32(mx - mn))
+    print(f"[gpu] 512^3 rel={rel} eb={eb}", flush=True)
+    p = faz_amd.make_params((512, 512, 512), eb, algo=1, max_step=32)
+    recon = torch.empty_like(x)
+    raw = engine.interp_raw(x, p, recon=recon)
+    err = (recon.double() - x.double()).abs().max().item()
+    assert err <= eb
+    out = torch.empty_like(x)
+    engine.decompress_raw(raw, 3, out)
+    assert torch.equal(out.view(torch.int32), recon.view(torch.int32))
+    r = engine.interp_compress(x, p)
+    assert r["n_codes"] == x.numel()
+    assert r["unpred"].size == int((r["codes"] == 0).sum())
+    del x, recon, out
+    torch.cuda.empty_cache()
