@@ -19,6 +19,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -68,7 +69,7 @@ struct faz_gpu_ctx {
     DevBuf<uint16_t> codes;
     DevBuf<float> outval, unpred, work_in, work_out;
     DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
-    DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code;
+    DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     DevBuf<uint8_t> enc_len;
     hipEvent_t ev[8] = {};
     double timings[8] = {};
@@ -82,6 +83,8 @@ struct faz_gpu_ctx {
     double prof_ms = 0;
     uint64_t prof_launches = 0, prof_points = 0;
     std::vector<hipEvent_t> prof_ev;
+    // 3-D fields: one fused launch per level (0 = per-pass launches)
+    bool fused3d = true;
 };
 
 #define CTX_FAIL(code, ...)                                          \
@@ -106,6 +109,8 @@ namespace {
 // ---------------------------------------------------------------------------
 struct LevelPlan {
     unsigned level;
+    uint32_t nb[kMaxN];        // blocks per axis in the raster
+    uint64_t bs;
     double eb, recip;
     int linear;
     int dir;
@@ -292,6 +297,8 @@ int make_plan(const faz_interp_params *p, Plan &pl, std::string &why) {
         }
         L.base = base;
         L.codes = run;
+        for (int a = 0; a < kMaxN; a++) L.nb[a] = nb[a];
+        L.bs = bs;
         // one launch per pass over the pass lattice
         uint64_t lattice_total = 0;
         for (int P = 0; P < pl.N; P++) {
@@ -363,6 +370,8 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return FAZ_ERR_HIP;
     faz_gpu_ctx *ctx = new faz_gpu_ctx();
     ctx->device = device;
+    const char *kern = getenv("FAZ_INTERP_KERNEL");
+    if (kern && strcmp(kern, "pass") == 0) ctx->fused3d = false;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return FAZ_ERR_HIP;
@@ -399,6 +408,7 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->chunk_off.release();
     ctx->enc_code.release();
     ctx->enc_len.release();
+    ctx->tiles.release();
     for (auto &e: ctx->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e: ctx->prof_ev) (void) hipEventDestroy(e);
@@ -472,8 +482,33 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
         ctx->prof_ev.resize(2 * npass);
         for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
     }
+    const bool fused = ctx->fused3d && pl.N == 3 && (uint32_t) p->block_size <= kFusedMaxBlock;
     size_t k = 0;
     for (auto &L: pl.lv) {
+        if (fused) {
+            Level3D g;
+            for (int a = 0; a < 3; a++) {
+                g.dims[a] = pl.dims[a];
+                g.offs[a] = pl.offs[a];
+                g.nb[a] = L.nb[a];
+                g.nbw[a] = pl.dims[a] == 1 ? 1 : (uint32_t) ((pl.dims[a] - 1 + L.bs - 1) / L.bs);
+                g.p[a] = pl.perms[L.dir][a];
+            }
+            g.s = (uint32_t) ((uint64_t) 1 << (L.level - 1));
+            g.bs = (uint32_t) L.bs;
+            g.linear = L.linear;
+            g.eb = L.eb;
+            g.recip = L.recip;
+            g.radius = p->radius;
+            g.level_base = L.base;
+            g.block_prefix = ctx->prefix.p + L.prefix_at;
+            g.code_limit = pl.total_codes;
+            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            launch_interp_level3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
+            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+            k++;
+            continue;
+        }
         for (auto &d: L.passes) {
             d.block_prefix = ctx->prefix.p + L.prefix_at;
             if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
@@ -486,13 +521,15 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
     HIPCHECK(hipEventRecord(ctx->ev[1], st));
     // histogram, first positions and outlier compaction
     launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st);
-    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->scalars.p, st);
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     HIPCHECK(hipGetLastError());
     uint64_t n_unpred = 0;
     uint32_t errflag = 0;
     HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(&errflag, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
     HIPCHECK(ctx->unpred.ensure(n_unpred));
     launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
@@ -505,7 +542,14 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
     ctx->timings[6] = ctx->timings[1];
     if (ctx->profile) {
         size_t j = 0;
-        for (auto &L: pl.lv)
+        for (auto &L: pl.lv) {
+            if (fused) {
+                ctx->prof_ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
+                ctx->prof_launches++;
+                ctx->prof_points += L.codes;
+                j++;
+                continue;
+            }
             for (auto &d: L.passes) {
                 if (d.total) {
                     ctx->prof_ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
@@ -514,6 +558,7 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
                 }
                 j++;
             }
+        }
     }
     ctx->hist_codes = ctx->codes.p;
     ctx->hist_n = pl.total_codes;
@@ -580,7 +625,8 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, t->code, (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->enc_len.p, t->len, (size_t) t->n_states, hipMemcpyHostToDevice, st));
     launch_encode_sizes(d_codes, n, ctx->enc_len.p, t->offset, ctx->chunk_bits.p, st);
-    launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->scalars.p + 1, st);
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
     HIPCHECK(hipGetLastError());
     uint64_t bits = 0;
     HIPCHECK(hipMemcpyAsync(&bits, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
@@ -827,7 +873,8 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
     launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
-    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->scalars.p, st);
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
     ctx->hist_codes = nullptr;
     // head: anchors / first point come straight from the unpredictable list

@@ -39,7 +39,28 @@ struct PassDesc {
     int tuning;               // 0 / 1 / 2
 };
 
+// One level of the 3-D engine as a single launch: one workgroup per block.
+struct Level3D {
+    uint32_t dims[3];
+    uint64_t offs[3];
+    uint32_t s, bs;           // stride and block edge
+    uint32_t nb[3];           // blocks per axis in the reference raster
+    uint32_t nbw[3];          // launched blocks per axis (trailing one-plane blocks skipped)
+    int p[3];                 // dimension_sequences[direction]
+    int linear;
+    double eb, recip;
+    int radius;
+    uint64_t level_base;
+    const uint64_t *block_prefix;
+    uint64_t code_limit;      // codes in the whole stream (bounds every write)
+};
+
+// Largest interpBlockSize the fused 3-D kernel stages in LDS.
+constexpr uint32_t kFusedMaxBlock = 32;
+
 // Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
+void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
+                           uint32_t *d_err, hipStream_t st);
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st);
 void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
                         const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st);
@@ -53,13 +74,16 @@ void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_cod
                          hipStream_t st);
 void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
                  int radius, hipStream_t st);
-void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_total,
-                      hipStream_t st);
+void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
+                      uint64_t *d_total, hipStream_t st);
 void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
                           const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st);
 void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint8_t *d_len, int32_t offset,
                          uint64_t *d_chunk_bits, hipStream_t st);
-void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_total, hipStream_t st);
+void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
+                     hipStream_t st);
+// tiles of the two-level scans (scratch of this many uint64 per scan)
+uint32_t scan_tiles(uint32_t count);
 void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_code, const uint8_t *d_len,
                         int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st);
 void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st);

@@ -4,6 +4,7 @@
 // file is compiled with -ffp-contract=off and the pragma below), FP32
 // denormals preserved, every float/double operation in the reference's order.
 #include "faz_internal.h"
+#include "faz_device.h"
 
 #include <hip/hip_runtime.h>
 #include <climits>
@@ -11,59 +12,6 @@
 #pragma clang fp contract(off)
 
 namespace faz {
-
-// ---------------------------------------------------------------------------
-// Scalar pieces shared by the pass kernels
-// ---------------------------------------------------------------------------
-
-// (int) of a non-negative double the way x86-64 cvttsd2si converts it: values
-// beyond INT_MAX (and NaN) become INT_MIN (IntegerQuantizer.hpp:81 relies on it).
-__device__ __forceinline__ int32_t x86_trunc_i32(double v) {
-    return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t) v : INT32_MIN;
-}
-
-// LinearQuantizer::quantize_and_overwrite with trimToZero == 0
-// (IntegerQuantizer.hpp:69-108).  Returns the code; *rec gets the value the
-// reference leaves in its data buffer (the original when unpredictable).
-__device__ __forceinline__ int quantize_point(float orig, float pred, double eb, double recip, int radius,
-                                              float *rec) {
-    float diff = orig - pred;
-    uint32_t qi = (uint32_t) x86_trunc_i32((double) fabsf(diff) * recip) + 1u;
-    if ((int32_t) qi < radius * 2) {
-        int32_t q = ((int32_t) qi) >> 1;
-        int32_t half = q;
-        uint32_t qs = (uint32_t) q << 1;
-        int32_t shifted;
-        if (diff < 0) {
-            qs = 0u - qs;
-            shifted = radius - half;
-        } else {
-            shifted = radius + half;
-        }
-        float dec = (float) ((double) pred + (double) (int32_t) qs * eb);
-        if ((double) fabsf(dec - orig) > eb) {
-            *rec = orig;
-            return 0;
-        }
-        *rec = dec;
-        return shifted;
-    }
-    *rec = orig;
-    return 0;
-}
-
-// LinearQuantizer::recover (IntegerQuantizer.hpp:160-175)
-__device__ __forceinline__ float recover_point(float pred, int code, int radius, double eb) {
-    return (float) ((double) pred + (double) (2 * (code - radius)) * eb);
-}
-
-// Interpolators.hpp:9-36, evaluated in float left to right; linear1 in double.
-__device__ __forceinline__ float i_linear(float a, float b) { return (a + b) / 2; }
-__device__ __forceinline__ float i_linear1(float a, float b) { return (float) (-0.5 * (double) a + 1.5 * (double) b); }
-__device__ __forceinline__ float i_quad1(float a, float b, float c) { return (3 * a + 6 * b - c) / 8; }
-__device__ __forceinline__ float i_quad2(float a, float b, float c) { return (-a + 6 * b + 3 * c) / 8; }
-__device__ __forceinline__ float i_quad3(float a, float b, float c) { return (3 * a - 10 * b + 15 * c) / 8; }
-__device__ __forceinline__ float i_cubic(float a, float b, float c, float d) { return (-a + 9 * b + 9 * c - d) / 16; }
 
 // Where one predicted point sits in the reference's push order and which
 // stencil it uses (block_interpolation_1d, SIC:1306-1417): the line holds n
@@ -368,15 +316,44 @@ void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t s
 
 // ---------------------------------------------------------------------------
 // Histogram + first occurrence + per-chunk outlier counts
+//
+// Codes cluster tightly around the quantizer radius, so the 16 symbols
+// nearest it are counted in per-thread packed 8-bit registers (a thread sees
+// at most kHistPerThread <= 255 codes per chunk) and folded once per chunk;
+// the rest of a 4096-symbol window goes to LDS atomics, anything beyond it to
+// global atomics.  First positions: LDS atomicMin guarded by a read, so each
+// symbol costs at most a few atomics per chunk.
 // ---------------------------------------------------------------------------
 constexpr int kWin = 4096;  // LDS bins around the quantizer radius
+constexpr int kHot = 16;    // register-counted symbols [hot0, hot0 + 16)
+static_assert(kHistPerThread <= 255, "8-bit register counters");
+
+__device__ __forceinline__ void hist_cold(uint32_t v, uint32_t li, uint32_t base, uint32_t lo, uint32_t *s_cnt,
+                                          uint32_t *s_first, uint32_t *freq, uint32_t *first, uint32_t *s_zero,
+                                          uint32_t *s_zero_first) {
+    if (v == 0) {
+        atomicAdd(s_zero, 1u);
+        if (*s_zero_first > li) atomicMin(s_zero_first, li);
+        return;
+    }
+    uint32_t w = v - lo;
+    if (w < (uint32_t) kWin) {
+        atomicAdd(&s_cnt[w], 1u);
+        if (s_first[w] > li) atomicMin(&s_first[w], li);
+    } else {
+        atomicAdd(&freq[v], 1u);
+        atomicMin(&first[v], base + li);
+    }
+}
 
 __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                            uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
-                                                           uint32_t *__restrict__ chunk_zeros, int lo) {
+                                                           uint32_t *__restrict__ chunk_zeros, int lo_,
+                                                           int hot0_, int aligned) {
     __shared__ uint32_t s_cnt[kWin];
     __shared__ uint32_t s_first[kWin];
     __shared__ uint32_t s_zero, s_zero_first;
+    const uint32_t lo = (uint32_t) lo_, hot0 = (uint32_t) hot0_;
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
         s_cnt[i] = 0;
         s_first[i] = 0xFFFFFFFFu;
@@ -388,22 +365,46 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
     __syncthreads();
     const uint32_t base = blockIdx.x * kHistChunk;
     const uint32_t end = min(base + kHistChunk, n);
-    for (uint32_t j = base + threadIdx.x; j < end; j += kHistThreads) {
-        uint32_t v = codes[j];
-        uint32_t li = j - base;
-        if (v == 0) {
-            atomicAdd(&s_zero, 1u);
-            if (s_zero_first > li) atomicMin(&s_zero_first, li);
+    uint64_t hot_a = 0, hot_b = 0;  // 8-bit counters of symbols hot0..hot0+7, hot0+8..hot0+15
+    uint32_t seen = 0;              // hot symbols whose first position this thread already offered
+    // each thread takes 8 consecutive codes per step (one 16-byte load)
+    for (uint32_t j = base + threadIdx.x * 8; j < end; j += kHistThreads * 8) {
+        uint16_t v[8];
+        if (aligned && j + 8 <= end) {
+            uint4 q = *reinterpret_cast<const uint4 *>(codes + j);
+            v[0] = (uint16_t) q.x; v[1] = (uint16_t) (q.x >> 16);
+            v[2] = (uint16_t) q.y; v[3] = (uint16_t) (q.y >> 16);
+            v[4] = (uint16_t) q.z; v[5] = (uint16_t) (q.z >> 16);
+            v[6] = (uint16_t) q.w; v[7] = (uint16_t) (q.w >> 16);
         } else {
-            uint32_t w = v - (uint32_t) lo;
-            if (w < (uint32_t) kWin) {
-                atomicAdd(&s_cnt[w], 1u);
-                if (s_first[w] > li) atomicMin(&s_first[w], li);
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = j + k < end ? codes[j + k] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            if (j + k >= end) break;
+            const uint32_t li = j + k - base;
+            const uint32_t d = (uint32_t) v[k] - hot0;
+            if (d < (uint32_t) kHot) {
+                const uint64_t one = 1ull << (8 * (d & 7));
+                if (d < 8) hot_a += one; else hot_b += one;
+                if (!((seen >> d) & 1u)) {
+                    seen |= 1u << d;
+                    uint32_t w = v[k] - lo;
+                    if (s_first[w] > li) atomicMin(&s_first[w], li);
+                }
             } else {
-                atomicAdd(&freq[v], 1u);
-                atomicMin(&first[v], j);
+                hist_cold(v[k], li, base, lo, s_cnt, s_first, freq, first, &s_zero, &s_zero_first);
             }
         }
+    }
+    // fold the register counters: wave reduction, then one LDS add per wave
+    const int lane = threadIdx.x % 64;
+#pragma unroll
+    for (int d = 0; d < kHot; d++) {
+        uint32_t c = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
+        for (int o = 32; o > 0; o >>= 1) c += (uint32_t) __shfl_xor((int) c, o);
+        if (lane == 0 && c) atomicAdd(&s_cnt[hot0 + d - lo], c);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
@@ -439,51 +440,111 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     launch_fill_u32(d_freq, 65536, 0u, st);
     launch_fill_u32(d_first, 65536, 0xFFFFFFFFu, st);
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    // window [lo, lo + kWin) inside 1..65535, hot symbols inside the window
     int lo = radius - kWin / 2;
     if (lo < 1) lo = 1;
+    if (lo + kWin > 65536) lo = 65536 - kWin;
+    int hot0 = radius - kHot / 2;
+    if (hot0 < lo) hot0 = lo;
+    if (hot0 + kHot > lo + kWin) hot0 = lo + kWin - kHot;
     if (nchunks)
         hipLaunchKernelGGL(hist_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
-                           d_first, d_chunk_zeros, lo);
+                           d_first, d_chunk_zeros, lo, hot0, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scans over chunk totals (single workgroup; chunk counts are small)
+// Exclusive scans over chunk totals: tiles of kScanTile entries are reduced,
+// then every tile scans itself and adds the sum of the tiles before it.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanPer = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanPer;
+
 template<typename TIn>
-__global__ void __launch_bounds__(1024) scan_kernel(const TIn *__restrict__ in, uint64_t *__restrict__ out,
-                                                    uint32_t count, uint64_t *total) {
-    __shared__ uint64_t s_part[1024];
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    const uint32_t hi = min(lo + per, count);
+__global__ void __launch_bounds__(kScanThreads) tile_reduce_kernel(const TIn *__restrict__ in, uint32_t count,
+                                                                  uint64_t *__restrict__ tile_sum) {
+    const uint32_t i0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
     uint64_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += (uint64_t) in[i];
-    s_part[threadIdx.x] = sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++)
+        if (i0 + k < count) sum += (uint64_t) in[i0 + k];
+    for (int o = 32; o > 0; o >>= 1) sum += (uint64_t) __shfl_xor((long long) sum, o);
+    __shared__ uint64_t s[kScanThreads / 64];
+    if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = sum;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint64_t v = threadIdx.x >= (unsigned) off ? s_part[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; w++) t += s[w];
+        tile_sum[blockIdx.x] = t;
     }
-    uint64_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
-    for (uint32_t i = lo; i < hi; i++) {
-        uint64_t v = (uint64_t) in[i];
-        out[i] = run;
-        run += v;
-    }
-    if (threadIdx.x == 1023 && total) *total = s_part[1023];
 }
 
-void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_total,
-                      hipStream_t st) {
-    hipLaunchKernelGGL(scan_kernel<uint32_t>, dim3(1), dim3(1024), 0, st, d_chunk_zeros, d_chunk_base, nchunks,
-                       d_total);
+template<typename TIn>
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const TIn *__restrict__ in, uint32_t count,
+                                                                 const uint64_t *__restrict__ tile_sum,
+                                                                 uint32_t ntiles, uint64_t *__restrict__ out,
+                                                                 uint64_t *total) {
+    __shared__ uint64_t s_wave[kScanThreads / 64];
+    __shared__ uint64_t s_tile_base;
+    // sum of the tiles before this one
+    uint64_t tb = 0;
+    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += kScanThreads) tb += tile_sum[k];
+    for (int o = 32; o > 0; o >>= 1) tb += (uint64_t) __shfl_xor((long long) tb, o);
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    if (lane == 0) s_wave[wave] = tb;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t w = 0; w < kScanThreads / 64; w++) t += s_wave[w];
+        s_tile_base = t;
+    }
+    __syncthreads();
+    const uint32_t i0 = blockIdx.x * kScanTile + threadIdx.x * kScanPer;
+    uint64_t v[kScanPer];
+    uint64_t mine = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        v[k] = i0 + k < count ? (uint64_t) in[i0 + k] : 0;
+        mine += v[k];
+    }
+    uint64_t incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t t = (uint64_t) __shfl_up((long long) incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint64_t run = s_tile_base + incl - mine;
+    for (int w = 0; w < wave; w++) run += s_wave[w];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k++) {
+        if (i0 + k < count) out[i0 + k] = run;
+        run += v[k];
+    }
+    if (total && blockIdx.x == ntiles - 1 && threadIdx.x == kScanThreads - 1) *total = run;
 }
 
-void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_total, hipStream_t st) {
-    hipLaunchKernelGGL(scan_kernel<uint64_t>, dim3(1), dim3(1024), 0, st, d_in, d_out, count, d_total);
+template<typename TIn>
+static void scan_two_level(const TIn *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
+                           hipStream_t st) {
+    uint32_t ntiles = (count + kScanTile - 1) / kScanTile;
+    if (ntiles == 0) return;
+    hipLaunchKernelGGL(tile_reduce_kernel<TIn>, dim3(ntiles), dim3(kScanThreads), 0, st, d_in, count, d_tiles);
+    hipLaunchKernelGGL(scan_apply_kernel<TIn>, dim3(ntiles), dim3(kScanThreads), 0, st, d_in, count, d_tiles, ntiles,
+                       d_out, d_total);
 }
+
+void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
+                      uint64_t *d_total, hipStream_t st) {
+    scan_two_level<uint32_t>(d_chunk_zeros, d_chunk_base, nchunks, d_tiles, d_total, st);
+}
+
+void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
+                     hipStream_t st) {
+    scan_two_level<uint64_t>(d_in, d_out, count, d_tiles, d_total, st);
+}
+
+uint32_t scan_tiles(uint32_t count) { return (count + kScanTile - 1) / kScanTile; }
 
 // ---------------------------------------------------------------------------
 // Outlier compaction: the unpredictable list is the values at code-0
@@ -568,15 +629,40 @@ void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unp
 
 // ---------------------------------------------------------------------------
 // Huffman encode: chunk bit totals, exclusive scan, then per-chunk packing in
-// LDS (MSB-first) flushed to the output words in big-endian byte order.
+// LDS (MSB-first) flushed to the output words in big-endian byte order.  A
+// thread owns kEncPerThread consecutive codes, loaded with 16-byte loads.
 // ---------------------------------------------------------------------------
+static_assert(kEncPerThread == 16, "two 16-byte loads per thread");
+
+__device__ __forceinline__ void load16(const uint16_t *__restrict__ codes, uint32_t j, uint32_t end, bool aligned,
+                                       uint16_t *v) {
+    if (aligned && j + 16 <= end) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(codes + j);
+        uint4 a = q[0], b = q[1];
+        uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            v[2 * k] = (uint16_t) w[k];
+            v[2 * k + 1] = (uint16_t) (w[k] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = j + k < end ? codes[j + k] : 0;
+    }
+}
+
 __global__ void __launch_bounds__(kEncThreads) encode_sizes_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                                   const uint8_t *__restrict__ len, int32_t offset,
-                                                                  uint64_t *__restrict__ chunk_bits) {
+                                                                  uint64_t *__restrict__ chunk_bits, int aligned) {
     const uint32_t base = blockIdx.x * kEncChunk;
     const uint32_t end = min(base + kEncChunk, n);
+    const uint32_t my0 = base + threadIdx.x * kEncPerThread;
+    uint16_t v[16];
+    load16(codes, my0, end, aligned, v);
     uint32_t sum = 0;
-    for (uint32_t j = base + threadIdx.x; j < end; j += kEncThreads) sum += len[(int32_t) codes[j] - offset];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        if (my0 + k < end) sum += len[(int32_t) v[k] - offset];
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
     __shared__ uint32_t s[kEncThreads / 64];
     if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = sum;
@@ -593,7 +679,7 @@ void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint8_t *d_len
     uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (nchunks)
         hipLaunchKernelGGL(encode_sizes_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_len,
-                           offset, d_chunk_bits);
+                           offset, d_chunk_bits, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
@@ -604,27 +690,24 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
                                                                  const uint64_t *__restrict__ code,
                                                                  const uint8_t *__restrict__ len, int32_t offset,
                                                                  const uint64_t *__restrict__ chunk_off,
-                                                                 uint32_t *__restrict__ words) {
+                                                                 uint32_t *__restrict__ words, int aligned) {
     __shared__ uint32_t s_words[kEncWords];
     __shared__ uint32_t s_wave[kEncThreads / 64];
     const uint32_t base = blockIdx.x * kEncChunk;
     const uint32_t end = min(base + kEncChunk, n);
-    // thread t owns codes [base + t*kEncPerThread, +kEncPerThread)
     const uint32_t my0 = base + threadIdx.x * kEncPerThread;
+    uint16_t v[16];
+    load16(codes, my0, end, aligned, v);
     uint32_t mybits = 0;
-    uint16_t mine[kEncPerThread];
 #pragma unroll
-    for (uint32_t k = 0; k < kEncPerThread; k++) {
-        uint32_t j = my0 + k;
-        mine[k] = j < end ? codes[j] : 0;
-        mybits += j < end ? len[(int32_t) mine[k] - offset] : 0;
-    }
+    for (int k = 0; k < 16; k++)
+        if (my0 + k < end) mybits += len[(int32_t) v[k] - offset];
     // workgroup exclusive prefix of mybits
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     uint32_t incl = mybits;
     for (int o = 1; o < 64; o <<= 1) {
-        uint32_t v = (uint32_t) __shfl_up((int) incl, o);
-        if (lane >= o) incl += v;
+        uint32_t t = (uint32_t) __shfl_up((int) incl, o);
+        if (lane >= o) incl += t;
     }
     if (lane == 63) s_wave[wave] = incl;
     for (uint32_t i = threadIdx.x; i < kEncWords; i += kEncThreads) s_words[i] = 0;
@@ -636,34 +719,57 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
     }
     const uint64_t g0 = chunk_off[blockIdx.x];
     const uint32_t head = (uint32_t) (g0 & 31);
-    uint32_t lp = head + before + (incl - mybits);
+    const uint32_t start = head + before + (incl - mybits);
+    // register accumulator: pending bits MSB-aligned in acc, nacc of them;
+    // the first output word may be shared with the previous thread and the
+    // last with the next one (atomicOr), words in between are ours alone.
+    uint32_t w = start >> 5;
+    uint64_t acc = 0;
+    uint32_t nacc = start & 31;
+    bool first_word = true;
+    auto flush = [&](uint32_t word) {
+        if (first_word && (start & 31))
+            atomicOr(&s_words[w], word);
+        else
+            s_words[w] = word;
+        first_word = false;
+        w++;
+    };
 #pragma unroll
-    for (uint32_t k = 0; k < kEncPerThread; k++) {
-        uint32_t j = my0 + k;
-        if (j >= end) break;
-        int st = (int32_t) mine[k] - offset;
-        uint32_t L = len[st];
-        uint64_t c = code[st];
-        while (L) {
-            uint32_t w = lp >> 5, o = lp & 31;
-            uint32_t take = min(L, 32u - o);
-            uint32_t bits = (uint32_t) (c >> (64 - take));
-            atomicOr(&s_words[w], bits << (32 - o - take));
-            c <<= take;
-            L -= take;
-            lp += take;
+    for (int k = 0; k < 16; k++) {
+        if (my0 + k >= end) break;
+        const int st = (int32_t) v[k] - offset;
+        const uint32_t L = len[st];
+        if (!L) continue;
+        const uint64_t c = code[st];
+        if (nacc + L <= 64) {
+            acc |= c >> nacc;
+            nacc += L;
+        } else {
+            const uint32_t take = 64 - nacc;
+            acc |= c >> nacc;
+            flush((uint32_t) (acc >> 32));
+            flush((uint32_t) acc);
+            acc = c << take;
+            nacc = L - take;
+        }
+        while (nacc >= 32) {
+            flush((uint32_t) (acc >> 32));
+            acc <<= 32;
+            nacc -= 32;
         }
     }
+    if (nacc > (first_word ? (start & 31) : 0u)) atomicOr(&s_words[w], (uint32_t) (acc >> 32));
     __syncthreads();
     const uint32_t nw = (head + total + 31) / 32;
     const uint64_t gw = g0 >> 5;
-    for (uint32_t w = threadIdx.x; w < nw; w += kEncThreads) {
-        uint32_t v = bswap32(s_words[w]);
-        bool edge = (w == 0 && head) || (w == nw - 1 && ((head + total) & 31));
+    for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
+        uint32_t val = bswap32(s_words[i]);
+        bool edge = (i == 0 && head) || (i == nw - 1 && ((head + total) & 31));
         if (edge)
-            atomicOr(&words[gw + w], v);
+            atomicOr(&words[gw + i], val);
         else
-            words[gw + w] = v;
+            words[gw + i] = val;
     }
 }
 
@@ -672,7 +778,7 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_cod
     uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (nchunks)
         hipLaunchKernelGGL(encode_pack_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_code,
-                           d_len, offset, d_chunk_off, d_words);
+                           d_len, offset, d_chunk_off, d_words, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
 void launch_bswap_words(uint32_t *, size_t, hipStream_t) {}

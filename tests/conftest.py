@@ -19,3 +19,23 @@ def engine():
     e = faz_amd.Engine(0)
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_pass():
+    """A context forced onto the per-pass launches (no fused 3-D level kernel)."""
+    import os
+
+    import faz_amd
+
+    old = os.environ.get("FAZ_INTERP_KERNEL")
+    os.environ["FAZ_INTERP_KERNEL"] = "pass"
+    try:
+        e = faz_amd.Engine(0)
+    finally:
+        if old is None:
+            del os.environ["FAZ_INTERP_KERNEL"]
+        else:
+            os.environ["FAZ_INTERP_KERNEL"] = old
+    yield e
+    e.close()

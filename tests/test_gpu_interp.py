@@ -104,6 +104,36 @@ def test_random_fields_against_oracle(engine, i):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
 
 
+FUSED3D = [
+    ((65, 66, 67), dict(algo=1, direction=d, max_step=32), 1e-3) for d in range(6)
+] + [
+    ((33, 97, 40), dict(algo=0, direction=5, max_step=16), 1e-4),
+    ((1, 70, 90), dict(algo=1, direction=0, max_step=32), 1e-3),
+    ((70, 1, 90), dict(algo=1, direction=4), 1e-3),
+    ((2, 3, 129), dict(algo=1, direction=1), 1e-3),
+    ((64, 64, 64), dict(algo=1, direction=0, block_size=16, max_step=16), 1e-3),
+    ((50, 45, 40), dict(algo=1, direction=2, alpha=1.5, beta=2.0, ops=(1, 0, 1), dirs=(0, 5, 3)), 1e-3),
+    ((96, 96, 96), dict(algo=1, direction=5, block_size=8), 1e-2),
+    ((40, 40, 40), dict(algo=1, radius=128), 1e-6),
+]
+
+
+@pytest.mark.parametrize("path", ["fused", "pass"])
+@pytest.mark.parametrize("i", range(len(FUSED3D)))
+def test_3d_level_kernels_against_oracle(engine, engine_pass, path, i):
+    import faz_amd
+    eng = engine if path == "fused" else engine_pass
+    shape, kw, rel = FUSED3D[i]
+    x = _field(shape, 300 + i)
+    eb = rel * float(np.float32(x.max() - x.min()))
+    print(f"[gpu] 3-D {path} {shape} {kw} eb={eb}", flush=True)
+    o = oracle.interp_compress(x, oracle.make_params(shape, eb, **kw))
+    r = eng.interp_compress(_dev(x), faz_amd.make_params(shape, eb, **kw))
+    assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
+    assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
+    assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+
+
 def test_range_matches_oracle(engine):
     x = _field((33, 45, 57), 5) - 3.0
     mn, mx = engine.range(_dev(x))
