@@ -31,9 +31,10 @@ if ROOT not in sys.path:
 
 METRIC = "compress GB/s + ratio at REL 1e-3 (bit-exact); learned vs exhaustive tuner"
 PEAK_HBM_GBS = 8000.0
-# Algorithmic HBM bytes per predicted point of the interpolation pass kernel:
-# read the original float (4) + write the reconstruction (4) + write the
-# 16-bit code (2).  Stencil neighbour re-reads are not counted.
+# Algorithmic HBM bytes per predicted point of the interpolation kernels (the
+# fused level kernel and the per-pass kernel alike): read the original float
+# (4) + write the reconstruction (4) + write the 16-bit code (2).  Stencil
+# neighbour re-reads are not counted.
 PASS_BYTES_PER_POINT = 10.0
 
 
@@ -141,7 +142,7 @@ def main():
         raw_size = eng.compress_device(x, p, True, args.rel, recon=recon)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof_ms, launches, points = eng.profile()
+    prof_ms, launches, points, kname = eng.profile()
     eng.set_profiling(False)
     timings = eng.timings()
 
@@ -177,7 +178,7 @@ def main():
                    "dims": list(dims), "rel_eb": args.rel, "interp": "cubic", "max_step": 32},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "interp_pass_kernel<3>", "launches_per_step": launches // max(args.steps, 1),
+                     "kernel": kname, "launches_per_step": launches // max(args.steps, 1),
                      "avg_launch_ms": round(avg_launch_ms, 5),
                      "bytes_per_point": PASS_BYTES_PER_POINT},
         "quality": {"ratio": round(4.0 * n / len(stream), 3), "stream_bytes": len(stream), "raw_bytes": raw_size,

@@ -95,6 +95,7 @@ EXPORTS = [
     "faz_huffman_build", "faz_gpu_interp_decompress_raw", "faz_gpu_interp_raw_f32",
     "faz_sz_compress_f32", "faz_sz_decompress_f32", "faz_free", "faz_gpu_last_timings",
     "faz_gpu_compress_f32", "faz_gpu_fetch_raw", "faz_gpu_set_profiling", "faz_gpu_profile_read",
+    "faz_gpu_profile_kernel",
 ]
 
 _lib = None
@@ -133,6 +134,7 @@ def lib():
         L.faz_gpu_fetch_raw.argtypes = [vp, vp, sz]
         L.faz_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
         L.faz_gpu_profile_read.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
+        L.faz_gpu_profile_kernel.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -323,7 +325,9 @@ class Engine:
         ms, la, pts = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
         self._check(self._L.faz_gpu_profile_read(self.h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(pts)),
                     "profile_read")
-        return ms.value, la.value, pts.value
+        kind = self._L.faz_gpu_profile_kernel(self.h)
+        name = "interp_level3d_kernel" if kind == 0 else "interp_pass_kernel"
+        return ms.value, la.value, pts.value, name
 
     def timings(self):
         t = (ctypes.c_double * 8)()

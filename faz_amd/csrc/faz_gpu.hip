@@ -80,8 +80,10 @@ struct faz_gpu_ctx {
     PendingRaw pending;
     // per-launch timing of the interpolation pass kernels
     bool profile = false;
-    double prof_ms = 0;
-    uint64_t prof_launches = 0, prof_points = 0;
+    struct ProfBucket {
+        double ms = 0;
+        uint64_t launches = 0, points = 0;
+    } prof[2];
     std::vector<hipEvent_t> prof_ev;
     // 3-D fields: one fused launch per level (0 = per-pass launches)
     bool fused3d = true;
@@ -483,9 +485,18 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
         for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
     }
     const bool fused = ctx->fused3d && pl.N == 3 && (uint32_t) p->block_size <= kFusedMaxBlock;
+    // levels with fewer blocks than this run as per-pass launches, which
+    // spread a coarse level's points over the whole device
+    constexpr uint64_t kFusedMinBlocks = 256;
+    auto level_fused = [&](const LevelPlan &L) {
+        if (!fused) return false;
+        uint64_t nbw = 1;
+        for (int a = 0; a < 3; a++) nbw *= pl.dims[a] == 1 ? 1 : (pl.dims[a] - 1 + L.bs - 1) / L.bs;
+        return nbw >= kFusedMinBlocks;
+    };
     size_t k = 0;
     for (auto &L: pl.lv) {
-        if (fused) {
+        if (level_fused(L)) {
             Level3D g;
             for (int a = 0; a < 3; a++) {
                 g.dims[a] = pl.dims[a];
@@ -541,20 +552,21 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
     ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
     ctx->timings[6] = ctx->timings[1];
     if (ctx->profile) {
+        // bucket 0: fused level launches, bucket 1: per-pass launches
         size_t j = 0;
         for (auto &L: pl.lv) {
-            if (fused) {
-                ctx->prof_ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
-                ctx->prof_launches++;
-                ctx->prof_points += L.codes;
+            if (level_fused(L)) {
+                ctx->prof[0].ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
+                ctx->prof[0].launches++;
+                ctx->prof[0].points += L.codes;
                 j++;
                 continue;
             }
             for (auto &d: L.passes) {
                 if (d.total) {
-                    ctx->prof_ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
-                    ctx->prof_launches++;
-                    ctx->prof_points += d.total;
+                    ctx->prof[1].ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
+                    ctx->prof[1].launches++;
+                    ctx->prof[1].points += d.total;
                 }
                 j++;
             }
@@ -798,18 +810,24 @@ faz_status faz_gpu_fetch_raw(faz_gpu_ctx *ctx, uint8_t *h_raw, size_t cap) {
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on) {
     if (!ctx) return FAZ_ERR_INVALID;
     ctx->profile = on != 0;
-    ctx->prof_ms = 0;
-    ctx->prof_launches = 0;
-    ctx->prof_points = 0;
+    ctx->prof[0] = faz_gpu_ctx::ProfBucket();
+    ctx->prof[1] = faz_gpu_ctx::ProfBucket();
     return FAZ_OK;
 }
 
 faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points) {
     if (!ctx || !ms || !launches || !points) return FAZ_ERR_INVALID;
-    *ms = ctx->prof_ms;
-    *launches = ctx->prof_launches;
-    *points = ctx->prof_points;
+    // the dominant kernel: whichever launch family took longer
+    const auto &b = ctx->prof[0].ms >= ctx->prof[1].ms ? ctx->prof[0] : ctx->prof[1];
+    *ms = b.ms;
+    *launches = b.launches;
+    *points = b.points;
     return FAZ_OK;
+}
+
+int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx) {
+    if (!ctx) return -1;
+    return ctx->prof[0].ms >= ctx->prof[1].ms ? 0 : 1;
 }
 
 faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,

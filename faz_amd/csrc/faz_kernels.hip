@@ -268,9 +268,21 @@ __device__ __forceinline__ uint32_t f2key(float f) {
 __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in, size_t n, uint32_t *minmax) {
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
     size_t stride = (size_t) gridDim.x * blockDim.x;
-    size_t n4 = n / 4;
+    size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
     const float4 *in4 = reinterpret_cast<const float4 *>(in);
-    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+        float4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = in4[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            uint32_t a = f2key(v[u].x), b = f2key(v[u].y), c = f2key(v[u].z), e = f2key(v[u].w);
+            lo = min(lo, min(min(a, b), min(c, e)));
+            hi = max(hi, max(max(a, b), max(c, e)));
+        }
+    }
+    for (; i < n4; i += stride) {
         float4 v = in4[i];
         uint32_t a = f2key(v.x), b = f2key(v.y), c = f2key(v.z), e = f2key(v.w);
         lo = min(lo, min(min(a, b), min(c, e)));
@@ -556,12 +568,14 @@ __global__ void __launch_bounds__(kHistThreads) unpred_gather_kernel(const uint1
                                                                     const uint32_t *__restrict__ chunk_zeros,
                                                                     const uint64_t *__restrict__ chunk_base,
                                                                     float *__restrict__ unpred) {
-    if (chunk_zeros[blockIdx.x] == 0) return;
     __shared__ uint32_t s_wave[kHistThreads / 64];
-    const uint32_t base = blockIdx.x * kHistChunk;
-    const uint32_t end = min(base + kHistChunk, n);
-    uint64_t run = chunk_base[blockIdx.x];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    if (chunk_zeros[c] == 0) continue;
+    const uint32_t base = c * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    uint64_t run = chunk_base[c];
     for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
         uint32_t j = j0 + threadIdx.x;
         bool z = j < end && codes[j] == 0;
@@ -578,11 +592,13 @@ __global__ void __launch_bounds__(kHistThreads) unpred_gather_kernel(const uint1
         run += tot;
         __syncthreads();
     }
+    }
 }
 
 void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
                           const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st) {
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    if (nchunks > 1024) nchunks = 1024;
     if (nchunks)
         hipLaunchKernelGGL(unpred_gather_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
                            d_outval, d_chunk_zeros, d_chunk_base, d_unpred);
@@ -595,12 +611,14 @@ __global__ void __launch_bounds__(kHistThreads) unpred_scatter_kernel(const uint
                                                                      const uint32_t *__restrict__ chunk_zeros,
                                                                      const uint64_t *__restrict__ chunk_base,
                                                                      float *__restrict__ unpred_at) {
-    if (chunk_zeros[blockIdx.x] == 0) return;
     __shared__ uint32_t s_wave[kHistThreads / 64];
-    const uint32_t base = blockIdx.x * kHistChunk;
-    const uint32_t end = min(base + kHistChunk, n);
-    uint64_t run = chunk_base[blockIdx.x];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    if (chunk_zeros[c] == 0) continue;
+    const uint32_t base = c * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    uint64_t run = chunk_base[c];
     for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
         uint32_t j = j0 + threadIdx.x;
         bool z = j < end && codes[j] == 0;
@@ -617,11 +635,13 @@ __global__ void __launch_bounds__(kHistThreads) unpred_scatter_kernel(const uint
         run += tot;
         __syncthreads();
     }
+    }
 }
 
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
                            const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st) {
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    if (nchunks > 1024) nchunks = 1024;
     if (nchunks)
         hipLaunchKernelGGL(unpred_scatter_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
                            d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at);
@@ -710,7 +730,6 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
         if (lane >= o) incl += t;
     }
     if (lane == 63) s_wave[wave] = incl;
-    for (uint32_t i = threadIdx.x; i < kEncWords; i += kEncThreads) s_words[i] = 0;
     __syncthreads();
     uint32_t before = 0, total = 0;
     for (int w = 0; w < kEncThreads / 64; w++) {
@@ -719,6 +738,8 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
     }
     const uint64_t g0 = chunk_off[blockIdx.x];
     const uint32_t head = (uint32_t) (g0 & 31);
+    for (uint32_t i = threadIdx.x; i < (head + total + 31) / 32 + 1; i += kEncThreads) s_words[i] = 0;
+    __syncthreads();
     const uint32_t start = head + before + (incl - mybits);
     // register accumulator: pending bits MSB-aligned in acc, nacc of them;
     // the first output word may be shared with the previous thread and the

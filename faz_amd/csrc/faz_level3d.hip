@@ -36,6 +36,21 @@ struct Blk {
     uint32_t cntA, cntB, total;
 };
 
+// x / d for x < 2^20 through a float reciprocal, corrected by one step
+struct FastDiv {
+    uint32_t d;
+    float r;
+    __device__ __forceinline__ explicit FastDiv(uint32_t d_) : d(d_), r(1.0f / (float) d_) {}
+    __device__ __forceinline__ uint32_t div(uint32_t x) const {
+        uint32_t q = (uint32_t) ((float) x * r);
+        if (q * d > x)
+            q--;
+        else if ((q + 1) * d <= x)
+            q++;
+        return q;
+    }
+};
+
 __device__ __forceinline__ uint32_t idx_g1(uint32_t l, uint32_t first) { return first ? l : l - 1; }
 __device__ __forceinline__ uint32_t idx_g2(uint32_t l, uint32_t first) { return first ? l / 2 : l / 2 - 1; }
 
@@ -75,7 +90,7 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
     // ---- block geometry ----------------------------------------------------
     Blk B;
     {
-        uint32_t r = blockIdx.x;
+        uint32_t r = blockIdx.x;  // once per workgroup
         uint32_t k2 = r % L.nbw[2];
         r /= L.nbw[2];
         uint32_t k1 = r % L.nbw[1];
@@ -108,20 +123,31 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
     }
 
     // ---- load E: coarse points from recon, the rest from the input ---------
+    // (flat index over E with axis 2 fastest; R loads in flight per thread)
     {
         const uint32_t m2 = B.m[2], m1 = B.m[1];
-        const uint32_t rows = B.m[0] * m1;
-        const uint32_t rpw = 64 / m2;                     // rows per wave
-        const uint32_t lane = tid & 63, wave = tid >> 6;
-        const uint32_t sub = lane / m2, e2 = lane % m2;
-        if (sub < rpw) {
-            for (uint32_t row = wave * rpw + sub; row < rows; row += (kThreads / 64) * rpw) {
-                uint32_t e0 = row / m1, e1 = row % m1;
-                uint32_t l0 = P2 == 0 ? 2 * e0 : e0, l1 = P2 == 1 ? 2 * e1 : e1, l2 = P2 == 2 ? 2 * e2 : e2;
-                uint64_t f = gaddr(L, B, l0, l1, l2);
-                bool coarse = !((l0 | l1 | l2) & 1);
-                E[e0 * B.es[0] + e1 * B.es[1] + e2] = coarse ? recon[f] : in[f];
+        const uint32_t total = B.m[0] * m1 * m2;
+        const FastDiv d2(m2), d1(m1);
+        constexpr int R = 8;
+        for (uint32_t base = 0; base < total; base += kThreads * R) {
+            float v[R];
+            uint32_t dst[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t idx = base + r * kThreads + tid;
+                dst[r] = 0xFFFFFFFFu;
+                if (idx < total) {
+                    uint32_t row = d2.div(idx), e2 = idx - row * m2;
+                    uint32_t e0 = d1.div(row), e1 = row - e0 * m1;
+                    uint32_t l0 = P2 == 0 ? 2 * e0 : e0, l1 = P2 == 1 ? 2 * e1 : e1, l2 = P2 == 2 ? 2 * e2 : e2;
+                    const float *src = ((l0 | l1 | l2) & 1) ? in : recon;
+                    v[r] = src[gaddr(L, B, l0, l1, l2)];
+                    dst[r] = e0 * B.es[0] + e1 * B.es[1] + e2;
+                }
             }
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (dst[r] != 0xFFFFFFFFu) E[dst[r]] = v[r];
         }
     }
     __syncthreads();
@@ -131,9 +157,11 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
         const uint32_t cn = B.n[p0] / 2, hj = (B.n[p1] + 1) / 2, hk = (B.n[P2] + 1) / 2;
         const uint32_t cnt = cn * hj * hk;
         const uint32_t st = B.es[p0];
+        const FastDiv dcn(cn ? cn : 1), dhj(hj);
         for (uint32_t t = tid; t < cnt; t += kThreads) {
-            uint32_t q = t % cn, r = t / cn;
-            uint32_t j = 2 * (r % hj), kk = 2 * (r / hj);
+            uint32_t r = dcn.div(t), q = t - r * cn;
+            uint32_t r2 = dhj.div(r);
+            uint32_t j = 2 * (r - r2 * hj), kk = 2 * r2;
             uint32_t i = 2 * q + 1;
             uint32_t l[3];
             l[p0] = i;
@@ -158,9 +186,11 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
         const uint32_t cn = B.n[p1] / 2, ni = B.n[p0], hk = (B.n[P2] + 1) / 2;
         const uint32_t cnt = cn * ni * hk;
         const uint32_t st = B.es[p1];
+        const FastDiv dcn(cn ? cn : 1), dhk(hk);
         for (uint32_t t = tid; t < cnt; t += kThreads) {
-            uint32_t q = t % cn, r = t / cn;
-            uint32_t kk = 2 * (r % hk), i0 = r / hk;
+            uint32_t r = dcn.div(t), q = t - r * cn;
+            uint32_t i0 = dhk.div(r);
+            uint32_t kk = 2 * (r - i0 * hk);
             uint32_t i = 2 * q + 1;
             uint32_t l[3];
             l[p0] = i0;
@@ -186,35 +216,50 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
     const uint32_t stC = B.es[P2];
     const uint32_t baseC = B.cntA + B.cntB;
     if (P2 == 2) {
-        // rows along the contiguous axis: a row of lanes covers the even/odd
-        // pairs (2j, 2j+1); every lane writes its even value, the odd ones
-        // are predicted
-        const uint32_t lpr = (B.n[2] + 1) / 2;       // lanes per row
-        const uint32_t rpw = 64 / lpr;
+        // rows along the contiguous axis, 16 lanes per row: lane jj writes
+        // the even value 2jj (lane 15 also 32) and predicts 2jj + 1; RB rows
+        // per lane are loaded before any is computed
         const uint32_t lane = tid & 63, wave = tid >> 6;
-        const uint32_t sub = lane / lpr, jj = lane % lpr;
+        const uint32_t sub = lane >> 4, jj = lane & 15;
         const uint32_t rows = B.n[0] * B.n[1];
-        if (sub < rpw) {
-            for (uint32_t row = wave * rpw + sub; row < rows; row += (kThreads / 64) * rpw) {
-                uint32_t l0 = row / B.n[1], l1 = row % B.n[1];
-                uint32_t ie = 2 * jj, i = ie + 1;
-                uint32_t eb0 = l0 * B.es[0] + l1 * B.es[1];
-                bool wrow = l0 <= B.whi[0] && l1 <= B.whi[1];
-                uint64_t f = gaddr(L, B, l0, l1, ie);
+        const FastDiv dn1(B.n[1]);
+        const uint32_t ie = 2 * jj, i = ie + 1;
+        const bool odd_ok = i < B.n[2];
+        const bool tail = jj == 15 && B.n[2] > 32;  // even position 32
+        constexpr int RB = 4;
+        constexpr uint32_t kRowsPerStep = (kThreads / 64) * 4;
+        for (uint32_t r0 = wave * 4 + sub; r0 < rows; r0 += kRowsPerStep * RB) {
+            float orig[RB];
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const uint32_t row = r0 + b * kRowsPerStep;
+                if (row < rows && odd_ok) {
+                    uint32_t l0 = dn1.div(row), l1 = row - l0 * B.n[1];
+                    orig[b] = in[gaddr(L, B, l0, l1, i)];
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < RB; b++) {
+                const uint32_t row = r0 + b * kRowsPerStep;
+                if (row >= rows) break;
+                uint32_t l0 = dn1.div(row), l1 = row - l0 * B.n[1];
+                const uint32_t eb0 = l0 * B.es[0] + l1 * B.es[1];
+                const bool wrow = l0 <= B.whi[0] && l1 <= B.whi[1];
+                const uint64_t f = gaddr(L, B, l0, l1, ie);
                 if (wrow && ie <= B.whi[2]) recon[f] = E[eb0 + jj];
-                if (i < B.n[2]) {
-                    float orig = in[f + L.s];
-                    uint32_t ei0 = eb0 + jj;  // E index of position i - 1
+                if (wrow && tail && 32 <= B.whi[2]) recon[gaddr(L, B, l0, l1, 32)] = E[eb0 + 16];
+                if (odd_ok) {
+                    const uint32_t ei0 = eb0 + jj;  // E index of position i - 1
                     float pred = stencil_pred(i, B.n[2], L.linear, [&](int d) {
                         return E[(int) ei0 + ((d + 1) / 2) * (int) stC];
                     });
                     float rec;
-                    int code = quantize_point(orig, pred, L.eb, L.recip, L.radius, &rec);
-                    if (wrow && i <= B.whi[2]) recon[f + L.s] = rec;
-                    uint32_t lq0 = p0 == 0 ? l0 : l1, lq1 = p0 == 0 ? l1 : l0;  // (p0, p1) coordinates
+                    int code = quantize_point(orig[b], pred, L.eb, L.recip, L.radius, &rec);
+                    if (wrow && i <= B.whi[2]) recon[gaddr(L, B, l0, l1, i)] = rec;
+                    const uint32_t lq0 = p0 == 0 ? l0 : l1, lq1 = p0 == 0 ? l1 : l0;  // (p0, p1) coordinates
                     if ((B.first[p0] || lq0) && (B.first[p1] || lq1)) {
                         uint32_t line = idx_g1(lq0, B.first[p0]) * B.g1[p1] + idx_g1(lq1, B.first[p1]);
-                        store_code(s_codes, baseC + line * cnC + line_rank(i, B.n[2], L.linear), code, orig, B,
+                        store_code(s_codes, baseC + line * cnC + line_rank(i, B.n[2], L.linear), code, orig[b], B,
                                    outval, err);
                     }
                 }
@@ -226,21 +271,20 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
         const int q = P2 == 0 ? 1 : 0;  // the other non-contiguous axis
         {
             const uint32_t m2 = B.m[2], m1 = B.m[1];
-            const uint32_t rows = B.m[0] * m1;
-            const uint32_t rpw = 64 / m2;
-            const uint32_t lane = tid & 63, wave = tid >> 6;
-            const uint32_t sub = lane / m2, e2 = lane % m2;
-            if (sub < rpw && e2 <= B.whi[2]) {
-                for (uint32_t row = wave * rpw + sub; row < rows; row += (kThreads / 64) * rpw) {
-                    uint32_t e0 = row / m1, e1 = row % m1;
-                    uint32_t l0 = P2 == 0 ? 2 * e0 : e0, l1 = P2 == 1 ? 2 * e1 : e1;
-                    if (l0 <= B.whi[0] && l1 <= B.whi[1]) recon[gaddr(L, B, l0, l1, e2)] = E[e0 * B.es[0] + e1 * B.es[1] + e2];
-                }
+            const uint32_t total = B.m[0] * m1 * m2;
+            const FastDiv d2(m2), d1(m1);
+            for (uint32_t idx = tid; idx < total; idx += kThreads) {
+                uint32_t row = d2.div(idx), e2 = idx - row * m2;
+                uint32_t e0 = d1.div(row), e1 = row - e0 * m1;
+                uint32_t l0 = P2 == 0 ? 2 * e0 : e0, l1 = P2 == 1 ? 2 * e1 : e1;
+                if (l0 <= B.whi[0] && l1 <= B.whi[1] && e2 <= B.whi[2])
+                    recon[gaddr(L, B, l0, l1, e2)] = E[e0 * B.es[0] + e1 * B.es[1] + e2];
             }
         }
         const uint32_t cols = B.n[q] * B.n[2];
+        const FastDiv dn2(B.n[2]);
         for (uint32_t c = tid; c < cols; c += kThreads) {
-            uint32_t lq = c / B.n[2], l2 = c % B.n[2];
+            uint32_t lq = dn2.div(c), l2 = c - lq * B.n[2];
             float orig[kHalf];
 #pragma unroll
             for (uint32_t jj = 0; jj < (uint32_t) kHalf - 1; jj++) {

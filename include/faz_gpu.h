@@ -156,11 +156,15 @@ void faz_free(void *p);
  * (interpolation pass launches) summed. */
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
-/* Per-launch HIP-event timing of the interpolation pass kernels (the
- * dominant kernel), accumulated while enabled: total milliseconds, launches
- * and predicted points.  Enabling resets the counters. */
+/* Per-launch HIP-event timing of the interpolation kernels, accumulated while
+ * enabled for the fused 3-D level kernel and the per-pass kernel separately;
+ * faz_gpu_profile_read reports the family that took longer (the dominant
+ * kernel): total milliseconds, launches and predicted points, and
+ * faz_gpu_profile_kernel which family that is (0 fused level kernel,
+ * 1 per-pass kernel).  Enabling resets the counters. */
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on);
 faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points);
+int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx);
 
 #ifdef __cplusplus
 }
