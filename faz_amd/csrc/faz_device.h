@@ -19,13 +19,19 @@ __device__ __forceinline__ int32_t x86_trunc_i32(double v) {
     return (v > -2147483649.0 && v < 2147483648.0) ? (int32_t) v : INT32_MIN;
 }
 
+// The same for v = |diff| * recip with recip > 0 (v is >= 0 or NaN): one
+// comparison sends NaN and everything from 2^31 up to INT_MIN.
+__device__ __forceinline__ int32_t x86_trunc_i32_nonneg(double v) {
+    return v < 2147483648.0 ? (int32_t) v : INT32_MIN;
+}
+
 // LinearQuantizer::quantize_and_overwrite with trimToZero == 0
-// (IntegerQuantizer.hpp:69-108).  Returns the code; *rec gets the value the
+// (IntegerQuantizer.hpp:69-108); recip = 1 / eb > 0.  Returns the code; *rec gets the value the
 // reference leaves in its data buffer (the original when unpredictable).
 __device__ __forceinline__ int quantize_point(float orig, float pred, double eb, double recip, int radius,
                                               float *rec) {
     float diff = orig - pred;
-    uint32_t qi = (uint32_t) x86_trunc_i32((double) fabsf(diff) * recip) + 1u;
+    uint32_t qi = (uint32_t) x86_trunc_i32_nonneg((double) fabsf(diff) * recip) + 1u;
     if ((int32_t) qi < radius * 2) {
         int32_t q = ((int32_t) qi) >> 1;
         int32_t half = q;
@@ -47,6 +53,26 @@ __device__ __forceinline__ int quantize_point(float orig, float pred, double eb,
     }
     *rec = orig;
     return 0;
+}
+
+// quantize_point without branches: every step is computed and the outcome
+// selected, so a wavefront never splits on the in-range test.  ebf is the
+// largest float not above eb: for a float f, (double) f > eb exactly when
+// f > ebf, so the final bound check needs no conversion to double.
+__device__ __forceinline__ int quantize_select(float orig, float pred, double eb, double recip, float ebf, int radius,
+                                               float *rec) {
+    const float diff = orig - pred;
+    const int32_t qi = (int32_t) ((uint32_t) x86_trunc_i32_nonneg((double) fabsf(diff) * recip) + 1u);
+    const bool in_range = qi < radius * 2;
+    const int32_t half = qi >> 1;
+    const uint32_t qs = (uint32_t) qi & ~1u;  // (qi >> 1) << 1
+    const bool neg = diff < 0;
+    const int32_t qsigned = (int32_t) (neg ? 0u - qs : qs);
+    const int32_t shifted = neg ? radius - half : radius + half;
+    const float dec = (float) ((double) pred + (double) qsigned * eb);
+    const bool ok = in_range && !(fabsf(dec - orig) > ebf);
+    *rec = ok ? dec : orig;
+    return ok ? shifted : 0;
 }
 
 // LinearQuantizer::recover (IntegerQuantizer.hpp:160-175)

@@ -87,6 +87,12 @@ struct faz_gpu_ctx {
     std::vector<hipEvent_t> prof_ev;
     // 3-D fields: one fused launch per level (0 = per-pass launches)
     bool fused3d = true;
+    // levels that qualify run on the whole-tile kernel (persistent, tile_grid
+    // workgroups: one per compute unit, a multiple of 8 XCDs)
+    bool tile3d = true;
+    uint32_t tile_grid = 256;
+    // levels with fewer blocks than this run as per-pass launches
+    uint64_t fused_min_blocks = 256;
 };
 
 #define CTX_FAIL(code, ...)                                          \
@@ -372,8 +378,17 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return FAZ_ERR_HIP;
     faz_gpu_ctx *ctx = new faz_gpu_ctx();
     ctx->device = device;
+    // FAZ_INTERP_KERNEL selects the 3-D level kernels: "pass" = per-pass
+    // launches only, "level" = the per-block level kernel without the
+    // whole-tile fast path, "tile" = level kernels on every level whatever
+    // its block count (the whole-tile kernel where a level qualifies)
     const char *kern = getenv("FAZ_INTERP_KERNEL");
     if (kern && strcmp(kern, "pass") == 0) ctx->fused3d = false;
+    if (kern && (strcmp(kern, "pass") == 0 || strcmp(kern, "level") == 0)) ctx->tile3d = false;
+    if (kern && strcmp(kern, "tile") == 0) ctx->fused_min_blocks = 1;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu >= 8)
+        ctx->tile_grid = (uint32_t) (ncu / 8 * 8);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return FAZ_ERR_HIP;
@@ -485,14 +500,13 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
         for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
     }
     const bool fused = ctx->fused3d && pl.N == 3 && (uint32_t) p->block_size <= kFusedMaxBlock;
-    // levels with fewer blocks than this run as per-pass launches, which
-    // spread a coarse level's points over the whole device
-    constexpr uint64_t kFusedMinBlocks = 256;
+    // levels with few blocks run as per-pass launches, which spread a coarse
+    // level's points over the whole device
     auto level_fused = [&](const LevelPlan &L) {
         if (!fused) return false;
         uint64_t nbw = 1;
         for (int a = 0; a < 3; a++) nbw *= pl.dims[a] == 1 ? 1 : (pl.dims[a] - 1 + L.bs - 1) / L.bs;
-        return nbw >= kFusedMinBlocks;
+        return nbw >= ctx->fused_min_blocks;
     };
     size_t k = 0;
     for (auto &L: pl.lv) {
@@ -510,12 +524,17 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
             g.linear = L.linear;
             g.eb = L.eb;
             g.recip = L.recip;
+            g.ebf = float_floor(L.eb);
             g.radius = p->radius;
             g.level_base = L.base;
             g.block_prefix = ctx->prefix.p + L.prefix_at;
             g.code_limit = pl.total_codes;
             if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
-            launch_interp_level3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
+            if (ctx->tile3d && tile3d_eligible(g))
+                launch_interp_tile3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid,
+                                     st);
+            else
+                launch_interp_level3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
             if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
             k++;
             continue;

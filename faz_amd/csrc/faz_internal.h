@@ -2,6 +2,7 @@
 // host orchestration (faz_gpu.hip, host/*.cpp).  Not part of the C-ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -49,14 +50,33 @@ struct Level3D {
     int p[3];                 // dimension_sequences[direction]
     int linear;
     double eb, recip;
+    float ebf;                // largest float <= eb
     int radius;
     uint64_t level_base;
     const uint64_t *block_prefix;
     uint64_t code_limit;      // codes in the whole stream (bounds every write)
 };
 
+// Largest float not above a positive double.
+inline float float_floor(double x) {
+    float f = (float) x;
+    if ((double) f > x) f = nextafterf(f, 0.0f);
+    return f;
+}
+
 // Largest interpBlockSize the fused 3-D kernel stages in LDS.
 constexpr uint32_t kFusedMaxBlock = 32;
+
+// Axis order of 3-D direction d: std::next_permutation order of (0, 1, 2)
+// (dimension_sequences, SIC:820-852).
+constexpr int kPerm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// Whole-tile level kernel: a level qualifies when every block it launches
+// spans 32 or 33 lattice points on each axis (interpBlockSize 32) and every
+// tile offset fits 32 bits.  `grid` persistent workgroups share the blocks.
+bool tile3d_eligible(const Level3D &L);
+void launch_interp_tile3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
+                          uint32_t *d_err, uint32_t grid, hipStream_t st);
 
 // Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
 void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,

@@ -32,8 +32,7 @@ constexpr int kHalf = (kTile + 1) / 2;               // 17 even positions
 constexpr int kEMax = kTile * kTile * kHalf;         // 18513 floats
 constexpr int kCodeMax = kTile * kTile * kTile - kHalf * kHalf * kHalf;  // 31024 codes
 
-// axis order of direction DIR: std::next_permutation order of (0, 1, 2)
-constexpr int kPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr auto &kPerm = kPerm3;
 
 // stencil kinds of block_interpolation_1d (SIC:1306-1417)
 enum Kind : uint32_t { kCubic = 0, kQuad1, kQuad2, kQuad3, kLin, kCopy, kLin1 };

@@ -1,0 +1,432 @@
+// One level of the 3-D interpolation engine over whole tiles: the fast path
+// for interpBlockSize 32.
+//
+// A block of the level (block_interpolation<3>, SIC:3143-3185) spans 32 or 33
+// lattice points per axis.  Its whole tile (at most 33^3 floats, 152 KB with
+// the padded strides below; gfx950 gives a workgroup up to 160 KB of LDS)
+// sits in LDS, so passes A (along p0), B (along p1) and C (along p2) never
+// touch HBM.  Workgroups are persistent: while one tile is predicted the next
+// tile's values are already in flight into registers, so the HBM reads of a
+// block overlap the arithmetic of the previous one.
+//
+// Work mapping inside a pass: eight lanes per line, lane t (= lane & 7)
+// taking the odd positions i = 4t+1 and 4t+3, whose five even neighbours are
+// read once and shared by the two stencils.  Lines have
+// 32 or 33 points only, so every position has a fixed stencil (cubic inside,
+// quad_1 at i = 1, quad_2 at the loop exit, quad_3 at n-1 for even n:
+// SIC:1360-1414) and a fixed emission rank.
+//
+// A block recomputes the faces that the reference assigns to the previous
+// blocks (identical inputs give identical bits) and emits codes only for the
+// lines it owns, straight into its segment of the code stream.
+// Reconstructed values go back for the half-open block [b, b + bs) (through
+// the last point for the last block along an axis), so every point of the
+// level is written by exactly one workgroup.
+#include <algorithm>
+
+#include "faz_device.h"
+#include "faz_internal.h"
+
+namespace faz {
+
+namespace {
+
+constexpr int kThreads = 1024;               // 16 waves: four per SIMD, up to 128 VGPRs each
+constexpr int kRow = 35;                     // LDS stride of axis 1 (33 + 2 pad)
+constexpr int kPlane = 1157;                 // LDS stride of axis 0 (33 * 35 + 2 pad)
+constexpr int kTileFloats = 33 * kPlane;     // 38181 floats = 152724 bytes
+constexpr int kEdge = 33 * 32 + 33 * 33;     // l0 = 32 plane (l2 < 32) and l2 = 32 column
+constexpr int kEdgeSlots = (kEdge + kThreads - 1) / kThreads;
+constexpr int kLanesPerLine = 8;             // lanes sharing one line of 16 odd positions
+constexpr int kPerLane = 16 / kLanesPerLine;  // odd positions per lane
+constexpr int kLinesPerRound = kThreads / kLanesPerLine;
+
+__host__ __device__ constexpr int lds_stride(int axis) { return axis == 0 ? kPlane : (axis == 1 ? kRow : 1); }
+
+// Flag bits of an edge or coarse slot: the slot exists, and which of its
+// coordinates sit at tile position 32 (present only when that axis has 33).
+enum : uint32_t { kHas = 1, kAt32_0 = 2, kAt32_1 = 4, kAt32_2 = 8, kAllEven = 16 };
+
+struct Tile {
+    uint32_t n[3];
+    uint32_t first;     // bit a set when the block starts at 0 on axis a
+    uint32_t miss;      // kAt32_a set when axis a has no position 32
+    uint32_t nown;      // kAt32_a set when this block does not write position 32 of axis a
+    uint64_t gbase;     // element offset of the block origin
+    uint64_t seg;       // first code of the block
+    uint32_t cnt0, cnt1, total;
+    uint32_t emit;      // 1 when the block's code segment lies inside the stream
+};
+
+__device__ __forceinline__ uint32_t first_of(const Tile &B, int a) { return (B.first >> a) & 1; }
+// lines over all positions / over even positions of axis a that the block owns
+__device__ __forceinline__ uint32_t g1_of(const Tile &B, int a) { return B.n[a] - 1 + first_of(B, a); }
+__device__ __forceinline__ uint32_t g2_of(const Tile &B, int a) { return (B.n[a] - 1) / 2 + first_of(B, a); }
+
+// Origin and missing-position mask of block blk (what the loads need).
+__device__ __forceinline__ void tile_origin(const Level3D &L, uint32_t blk, uint64_t &gbase, uint32_t &miss) {
+    uint32_t r = blk;
+    const uint32_t k2 = r % L.nbw[2];
+    r /= L.nbw[2];
+    const uint32_t k1 = r % L.nbw[1];
+    const uint32_t k0 = r / L.nbw[1];
+    const uint32_t k[3] = {k0, k1, k2};
+    uint64_t g = 0;
+    miss = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const uint32_t b = k[a] * L.bs;
+        const uint32_t e = min(b + L.bs, L.dims[a] - 1);
+        if ((e - b) / L.s + 1 != 33) miss |= kAt32_0 << a;
+        g += (uint64_t) b * (a == 2 ? 1 : L.offs[a]);
+    }
+    gbase = g;
+}
+
+template<int DIR>
+__device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
+    constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
+    Tile B;
+    uint32_t r = blk;
+    const uint32_t k2 = r % L.nbw[2];
+    r /= L.nbw[2];
+    const uint32_t k1 = r % L.nbw[1];
+    const uint32_t k0 = r / L.nbw[1];
+    const uint32_t k[3] = {k0, k1, k2};
+    uint32_t b[3];
+    B.miss = 0;
+    B.nown = 0;
+    B.first = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        b[a] = k[a] * L.bs;
+        const uint32_t e = min(b[a] + L.bs, L.dims[a] - 1);
+        B.n[a] = (e - b[a]) / L.s + 1;
+        const uint32_t whi = e == L.dims[a] - 1 ? B.n[a] - 1 : B.n[a] - 2;
+        if (b[a] == 0) B.first |= 1u << a;
+        if (B.n[a] != 33) B.miss |= kAt32_0 << a;
+        if (whi != 32) B.nown |= kAt32_0 << a;
+    }
+    B.gbase = (uint64_t) b[0] * L.offs[0] + (uint64_t) b[1] * L.offs[1] + b[2];
+    B.seg = L.level_base + L.block_prefix[((uint64_t) k0 * L.nb[1] + k1) * L.nb[2] + k2];
+    B.cnt0 = 16 * g2_of(B, p1) * g2_of(B, p2);
+    B.cnt1 = g1_of(B, p0) * 16 * g2_of(B, p2);
+    B.total = B.cnt0 + B.cnt1 + g1_of(B, p0) * g1_of(B, p1) * 16;
+    B.emit = B.seg + B.total <= L.code_limit ? 1u : 0u;
+    return B;
+}
+
+// Per-thread placement of the tile loads, fixed for the whole launch.
+//  main: l0 = tid >> 5, l2 = tid & 31, l1 = 0..32 over the unrolled loop
+//  edge: the l0 = 32 plane and the l2 = 32 column, kEdgeSlots per thread
+// Points with every coordinate even come from the reconstruction of the
+// coarser levels, all others from the original data.
+struct Place {
+    uint32_t gl, ll;                    // main: global byte / LDS offset of (tid >> 5, 0, l2)
+    uint32_t lane_even;                 // l0 and l2 both even
+    uint32_t eg[kEdgeSlots];            // edge: global byte offsets
+    uint32_t ef[kEdgeSlots];            // edge: LDS index << 8 | flags
+};
+
+__device__ __forceinline__ Place place_of(uint32_t tid, uint32_t S0, uint32_t S1, uint32_t s) {
+    Place P;
+    const uint32_t l0 = tid >> 5, l2 = tid & 31;
+    P.gl = 4 * (l0 * S0 + l2 * s);
+    P.ll = l0 * kPlane + l2;
+    P.lane_even = ((l0 | l2) & 1) == 0 ? 1u : 0u;
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++) {
+        const uint32_t e = tid + kThreads * j;
+        uint32_t c0 = 0, c1 = 0, c2 = 0, f = 0;
+        if (e < 33 * 32) {
+            c0 = 32, c1 = e >> 5, c2 = e & 31, f = kHas;
+        } else if (e < (uint32_t) kEdge) {
+            const uint32_t x = e - 33 * 32;
+            c0 = x / 33, c1 = x % 33, c2 = 32, f = kHas;
+        }
+        f |= (c0 == 32 ? kAt32_0 : 0) | (c1 == 32 ? kAt32_1 : 0) | (c2 == 32 ? kAt32_2 : 0) |
+             (((c0 | c1 | c2) & 1) == 0 ? kAllEven : 0);
+        P.eg[j] = 4 * (c0 * S0 + c1 * S1 + c2 * s);
+        P.ef[j] = (c0 * kPlane + c1 * kRow + c2) << 8 | f;
+    }
+    return P;
+}
+
+__device__ __forceinline__ bool slot_on(uint32_t f, uint32_t off) { return (f & kHas) && !(f & off); }
+
+// Buffer resources over a block's origin: every access is a 32-bit lane
+// offset plus a uniform (scalar) offset, with no 64-bit address arithmetic
+// per lane.  The record limit is the whole 32-bit range; the host bounds the
+// offsets of a tile below 2^31 bytes.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rsrc_at(const void *p, int bytes = -1) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short) 0, bytes, 0x00020000);
+}
+// A store at this offset falls outside every bounded resource and is dropped
+// by the hardware: predicated stores without branches.
+constexpr uint32_t kDropped = 0x80000000u;
+__device__ __forceinline__ float ld(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+// Loads of the all-even points from the reconstruction.  They carry the
+// non-temporal hint, which also keeps the compiler from merging them with the
+// original-data load of the other lanes into one load through a per-lane
+// selected (non-uniform) resource.
+__device__ __forceinline__ float ld_coarse(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+__device__ __forceinline__ void st(Rsrc r, float v, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+
+// Values of one tile in flight (registers).
+struct Pre {
+    float m[33];
+    float ed[kEdgeSlots];
+};
+
+__device__ __forceinline__ void issue_loads(Pre &R, const Place &P, uint64_t gbase, uint32_t miss,
+                                            const float *__restrict__ in, const float *__restrict__ recon,
+                                            uint32_t S0, uint32_t S1) {
+    const Rsrc ib = rsrc_at(in + gbase);
+    const Rsrc rb = rsrc_at(recon + gbase);
+#pragma unroll
+    for (int k = 0; k < 33; k++) {
+        if (k == 32 && (miss & kAt32_1)) break;
+        const uint32_t so = 4 * k * S1;
+        if (!(k & 1) && P.lane_even)
+            R.m[k] = ld_coarse(rb, P.gl, so);
+        else
+            R.m[k] = ld(ib, P.gl, so);
+    }
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++) {
+        if (slot_on(P.ef[j], miss)) {
+            if (P.ef[j] & kAllEven)
+                R.ed[j] = ld_coarse(rb, P.eg[j], 0);
+            else
+                R.ed[j] = ld(ib, P.eg[j], 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void fill_tile(float *T, const Pre &R, const Place &P, uint32_t miss) {
+#pragma unroll
+    for (int k = 0; k < 32; k++) T[P.ll + k * kRow] = R.m[k];
+    if (!(miss & kAt32_1)) T[P.ll + 32 * kRow] = R.m[32];
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++)
+        if (slot_on(P.ef[j], miss)) T[P.ef[j] >> 8] = R.ed[j];
+}
+
+__device__ __forceinline__ void store_tile(const float *T, const Place &P, const Tile &B, float *__restrict__ recon,
+                                           uint32_t S0, uint32_t S1) {
+    const Rsrc rb = rsrc_at(recon + B.gbase);
+#pragma unroll
+    for (int k = 0; k < 32; k++) st(rb, T[P.ll + k * kRow], P.gl, 4 * k * S1);
+    if (!(B.nown & kAt32_1)) st(rb, T[P.ll + 32 * kRow], P.gl, 4 * 32 * S1);
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++)
+        if (slot_on(P.ef[j], B.nown)) st(rb, T[P.ef[j] >> 8], P.eg[j], 0);
+}
+
+// One pass of the block over the tile in LDS.
+template<int DIR, bool LINEAR, int PASS>
+__device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D &L, Rsrc cb, Rsrc ob,
+                                          uint32_t tid) {
+    constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
+    constexpr int A = PASS == 0 ? p0 : (PASS == 1 ? p1 : p2);  // interpolated axis
+    constexpr int U = PASS == 0 ? p1 : p0;                     // slow line axis (code order)
+    constexpr int W = PASS == 2 ? p1 : p2;                     // fast line axis
+    constexpr bool UE = PASS == 0, WE = PASS != 2;             // even positions only
+    constexpr uint32_t NU = UE ? 17 : 33, NW = WE ? 17 : 33;
+    constexpr int ESA = lds_stride(A), ESU = lds_stride(U) * (UE ? 2 : 1), ESW = lds_stride(W) * (WE ? 2 : 1);
+    constexpr uint32_t LINES = NU * NW;
+    constexpr int ROUNDS = (int) ((LINES + kLinesPerRound - 1) / kLinesPerRound);
+
+    constexpr int NE = kPerLane + 3;                           // even neighbours per lane
+    const uint32_t t = tid % kLanesPerLine, line0 = tid / kLanesPerLine;
+    const uint32_t na = B.n[A];
+    const uint32_t umax = UE ? (B.n[U] - 1) / 2 : B.n[U] - 1;
+    const uint32_t wmax = WE ? (B.n[W] - 1) / 2 : B.n[W] - 1;
+    const uint32_t fu = first_of(B, U), fw = first_of(B, W);
+    const uint32_t gw = WE ? g2_of(B, W) : g1_of(B, W);
+    const uint32_t pbase = PASS == 0 ? 0 : (PASS == 1 ? B.cnt0 : B.cnt0 + B.cnt1);
+    const bool first_lane = t == 0, last_lane = t == kLanesPerLine - 1;
+    // emission ranks of the lane's points q = kPerLane t + j (line_rank for
+    // n = 32, 33: interior cubic points first, then q = 0, then the rest)
+    uint32_t rk[kPerLane];
+    {
+        const uint32_t ncub = na == 33 ? 14u : 13u;
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) {
+            const uint32_t q = kPerLane * t + j;
+            rk[j] = LINEAR ? q : (q == 0 ? ncub : (q <= ncub ? q - 1 : q));
+        }
+    }
+    // lane t reads the even positions 2 kPerLane t - 2 + 2k, k < NE; the first
+    // and last of them fall outside the line for the first and last lane,
+    // which read a position inside it instead and never use the value
+    const int olo = first_lane ? 0 : -2 * ESA;
+    const int ohi = last_lane ? 2 * kPerLane * ESA : (2 * kPerLane + 2) * ESA;
+#pragma unroll 1
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t line = line0 + (uint32_t) kLinesPerRound * r;
+        const uint32_t u = line / NW, w = line - u * NW;
+        if ((LINES % kLinesPerRound == 0 || line < LINES) && u <= umax && w <= wmax) {
+            const int base = (int) (u * ESU + w * ESW) + (int) t * 2 * kPerLane * ESA;
+            float e[NE], o[kPerLane], pr[kPerLane];
+            e[0] = LINEAR ? 0.f : T[base + olo];
+#pragma unroll
+            for (int k = 1; k < NE - 1; k++) e[k] = T[base + (2 * k - 2) * ESA];
+            e[NE - 1] = LINEAR ? 0.f : T[base + ohi];
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) o[j] = T[base + (2 * j + 1) * ESA];
+            constexpr int h = kPerLane;  // e[h - 2 .. h + 1] hold positions 26 .. 32 for the last lane
+            if (LINEAR) {
+#pragma unroll
+                for (int j = 0; j < kPerLane; j++) pr[j] = i_linear(e[j + 1], e[j + 2]);
+                if (na == 32) {
+                    const float l1 = i_linear1(e[h - 1], e[h]);
+                    pr[h - 1] = last_lane ? l1 : pr[h - 1];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < kPerLane; j++) pr[j] = i_cubic(e[j], e[j + 1], e[j + 2], e[j + 3]);
+                const float q1 = i_quad1(e[1], e[2], e[3]);
+                pr[0] = first_lane ? q1 : pr[0];
+                if (na == 33) {
+                    const float q2 = i_quad2(e[h - 1], e[h], e[h + 1]);
+                    pr[h - 1] = last_lane ? q2 : pr[h - 1];
+                } else {
+                    const float q2 = i_quad2(e[h - 2], e[h - 1], e[h]);
+                    const float q3 = i_quad3(e[h - 2], e[h - 1], e[h]);
+                    pr[h - 2] = last_lane ? q2 : pr[h - 2];
+                    pr[h - 1] = last_lane ? q3 : pr[h - 1];
+                }
+            }
+            const bool own = B.emit && (fu | u) && (fw | w);
+            const uint32_t cl = pbase + ((u + fu - 1) * gw + (w + fw - 1)) * 16;
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) {
+                float rec;
+                const int code = quantize_select(o[j], pr[j], L.eb, L.recip, L.ebf, L.radius, &rec);
+                T[base + (2 * j + 1) * ESA] = rec;
+                const uint32_t slot = cl + rk[j];
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t) code, cb, own ? 2 * slot : kDropped, 0, 0);
+                st(ob, o[j], own && code == 0 ? 4 * slot : kDropped, 0);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+template<int DIR, bool LINEAR>
+__global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, const float *__restrict__ in,
+                                                                 float *__restrict__ recon,
+                                                                 uint16_t *__restrict__ codes,
+                                                                 float *__restrict__ outval,
+                                                                 uint32_t *__restrict__ err) {
+    __shared__ float T[kTileFloats];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
+    // blocks of one XCD (workgroup id mod 8) form a contiguous raster range,
+    // so neighbouring tiles share that XCD's L2 for their common faces
+    uint32_t blk, end, step;
+    if (gridDim.x % 8 == 0) {
+        const uint32_t per = gridDim.x / 8, xcd = blockIdx.x % 8, chunk = (nblk + 7) / 8;
+        blk = xcd * chunk + blockIdx.x / 8;
+        end = min(nblk, (xcd + 1) * chunk);
+        step = per;
+    } else {
+        blk = blockIdx.x;
+        end = nblk;
+        step = gridDim.x;
+    }
+    if (blk >= end) return;
+    const uint32_t S0 = L.s * (uint32_t) L.offs[0], S1 = L.s * (uint32_t) L.offs[1];
+    const Place P = place_of(tid, S0, S1, L.s);
+
+    Pre R;
+    {
+        uint64_t g;
+        uint32_t miss;
+        tile_origin(L, blk, g, miss);
+        issue_loads(R, P, g, miss, in, recon, S0, S1);
+    }
+    for (;;) {
+        const Tile B = tile_of<DIR>(L, blk);
+        if (!B.emit && tid == 0) atomicOr(err, 2u);
+        fill_tile(T, R, P, B.miss);
+        __syncthreads();
+        const uint32_t nxt = blk + step;
+        if (nxt < end) {
+            uint64_t g;
+            uint32_t miss;
+            tile_origin(L, nxt, g, miss);
+            issue_loads(R, P, g, miss, in, recon, S0, S1);
+        }
+        // bounded to the block's segment of the code stream; the segment is
+        // read through a vector load, so its uniformity is stated explicitly
+        const uint64_t seg = ((uint64_t) __builtin_amdgcn_readfirstlane((uint32_t) (B.seg >> 32)) << 32) |
+                             __builtin_amdgcn_readfirstlane((uint32_t) B.seg);
+        const int total = __builtin_amdgcn_readfirstlane((int) B.total);
+        const Rsrc cb = rsrc_at(codes + seg, 2 * total);
+        const Rsrc ob = rsrc_at(outval + seg, 4 * total);
+        tile_pass<DIR, LINEAR, 0>(T, B, L, cb, ob, tid);
+        __syncthreads();
+        tile_pass<DIR, LINEAR, 1>(T, B, L, cb, ob, tid);
+        __syncthreads();
+        tile_pass<DIR, LINEAR, 2>(T, B, L, cb, ob, tid);
+        __syncthreads();
+        store_tile(T, P, B, recon, S0, S1);
+        if (nxt >= end) break;
+        __syncthreads();
+        blk = nxt;
+    }
+}
+
+bool tile3d_eligible(const Level3D &L) {
+    if (L.s == 0 || L.bs != 32 * L.s) return false;
+    for (int a = 0; a < 3; a++) {
+        if (L.dims[a] < 33 || L.nbw[a] == 0) return false;
+        const uint64_t b = (uint64_t) (L.nbw[a] - 1) * L.bs;
+        const uint64_t e = std::min<uint64_t>(b + L.bs, L.dims[a] - 1);
+        const uint64_t n = (e - b) / L.s + 1;
+        if (n != 32 && n != 33) return false;
+    }
+    // every in-tile byte offset stays below 2^31
+    const uint64_t span = 32ull * L.s * (L.offs[0] + L.offs[1] + 1);
+    return span < (1ull << 29);
+}
+
+void launch_interp_tile3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
+                          uint32_t *d_err, uint32_t grid, hipStream_t st) {
+    const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
+    if (!nblk || !grid) return;
+    dim3 g(std::min(grid, nblk)), block(kThreads);
+    int dir = 0;
+    for (int d = 0; d < 6; d++)
+        if (kPerm3[d][0] == L.p[0] && kPerm3[d][1] == L.p[1] && kPerm3[d][2] == L.p[2]) dir = d;
+#define FAZ_T3D(D)                                                                                               \
+    do {                                                                                                         \
+        if (L.linear)                                                                                            \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, true>), g, block, 0, st, L, d_in, d_recon, d_codes,      \
+                               d_outval, d_err);                                                                 \
+        else                                                                                                     \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, false>), g, block, 0, st, L, d_in, d_recon, d_codes,     \
+                               d_outval, d_err);                                                                 \
+    } while (0)
+    switch (dir) {
+        case 0: FAZ_T3D(0); break;
+        case 1: FAZ_T3D(1); break;
+        case 2: FAZ_T3D(2); break;
+        case 3: FAZ_T3D(3); break;
+        case 4: FAZ_T3D(4); break;
+        default: FAZ_T3D(5); break;
+    }
+#undef FAZ_T3D
+}
+
+}  // namespace faz

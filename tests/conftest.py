@@ -21,21 +21,40 @@ def engine():
     e.close()
 
 
-@pytest.fixture(scope="session")
-def engine_pass():
-    """A context forced onto the per-pass launches (no fused 3-D level kernel)."""
-    import os
-
+def _engine_with(kernel):
     import faz_amd
 
     old = os.environ.get("FAZ_INTERP_KERNEL")
-    os.environ["FAZ_INTERP_KERNEL"] = "pass"
+    os.environ["FAZ_INTERP_KERNEL"] = kernel
     try:
-        e = faz_amd.Engine(0)
+        return faz_amd.Engine(0)
     finally:
         if old is None:
             del os.environ["FAZ_INTERP_KERNEL"]
         else:
             os.environ["FAZ_INTERP_KERNEL"] = old
+
+
+@pytest.fixture(scope="session")
+def engine_pass():
+    """A context forced onto the per-pass launches (no fused 3-D level kernel)."""
+    e = _engine_with("pass")
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_level():
+    """A context on the per-block level kernel (no whole-tile fast path)."""
+    e = _engine_with("level")
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_tile():
+    """A context that runs every 3-D level through the level kernels whatever
+    its block count, so small fields exercise the whole-tile kernel."""
+    e = _engine_with("tile")
     yield e
     e.close()

@@ -120,15 +120,45 @@ FUSED3D = [
 
 @pytest.mark.parametrize("path", ["fused", "pass"])
 @pytest.mark.parametrize("i", range(len(FUSED3D)))
-def test_3d_level_kernels_against_oracle(engine, engine_pass, path, i):
+def test_3d_level_kernels_against_oracle(engine_tile, engine_pass, path, i):
     import faz_amd
-    eng = engine if path == "fused" else engine_pass
+    eng = engine_tile if path == "fused" else engine_pass
     shape, kw, rel = FUSED3D[i]
     x = _field(shape, 300 + i)
     eb = rel * float(np.float32(x.max() - x.min()))
     print(f"[gpu] 3-D {path} {shape} {kw} eb={eb}", flush=True)
     o = oracle.interp_compress(x, oracle.make_params(shape, eb, **kw))
     r = eng.interp_compress(_dev(x), faz_amd.make_params(shape, eb, **kw))
+    assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
+    assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
+    assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+
+
+# Fields whose every dimension is 32k or 32k + 1: their fine levels run on the
+# whole-tile kernel (blocks of 32 or 33 points per axis), coarser ones on the
+# per-block level kernel.
+TILE3D = [
+    ((64, 65, 96), dict(algo=1, direction=d, max_step=32), 1e-3) for d in range(6)
+] + [
+    ((97, 64, 33), dict(algo=0, direction=3, max_step=32), 1e-4),
+    ((65, 65, 65), dict(algo=1, direction=0), 1e-3),
+    ((128, 96, 65), dict(algo=1, direction=5, max_step=32, alpha=1.5, beta=2.0), 1e-3),
+    ((33, 128, 64), dict(algo=0, direction=1, max_step=16), 1e-2),
+    ((96, 64, 64), dict(algo=1, max_step=32, ops=(1, 0, 1), dirs=(0, 5, 3)), 1e-3),
+    ((64, 64, 64), dict(algo=1, max_step=32, radius=128), 1e-6),
+    ((64, 97, 64), dict(algo=0, direction=4, max_step=32), 1e-3),
+]
+
+
+@pytest.mark.parametrize("i", range(len(TILE3D)))
+def test_tile_kernel_against_oracle(engine_tile, i):
+    import faz_amd
+    shape, kw, rel = TILE3D[i]
+    x = _field(shape, 500 + i)
+    eb = rel * float(np.float32(x.max() - x.min()))
+    print(f"[gpu] tile {shape} {kw} eb={eb}", flush=True)
+    o = oracle.interp_compress(x, oracle.make_params(shape, eb, **kw))
+    r = engine_tile.interp_compress(_dev(x), faz_amd.make_params(shape, eb, **kw))
     assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
     assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
     assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
@@ -179,19 +209,22 @@ def test_full_size_512_properties(engine, rel):
     torch.cuda.empty_cache()
 
 
-def test_full_size_fused_equals_per_pass(engine, engine_pass):
-    """At 512^3 the fused level kernel and the per-pass launches agree bit for bit."""
+def test_full_size_kernels_agree(engine, engine_level, engine_pass):
+    """At 512^3 the whole-tile kernel, the per-block level kernel and the
+    per-pass launches agree bit for bit."""
     import faz_amd
     x = _grf_512(seed=1)
     mn, mx = engine.range(x)
     eb = 1e-3 * float(np.float32(mx - mn))
-    for d in (0, 5):
-        p = faz_amd.make_params((512, 512, 512), eb, algo=1, direction=d, max_step=32)
-        print(f"[gpu] 512^3 fused vs per-pass dir={d}", flush=True)
+    for d, algo in ((0, 1), (5, 1), (3, 0)):
+        p = faz_amd.make_params((512, 512, 512), eb, algo=algo, direction=d, max_step=32)
+        print(f"[gpu] 512^3 tile vs level vs per-pass dir={d} algo={algo}", flush=True)
         a = engine.interp_compress(x, p)
-        b = engine_pass.interp_compress(x, p)
-        assert np.array_equal(a["codes"], b["codes"])
-        assert np.array_equal(a["unpred"].view(np.uint32), b["unpred"].view(np.uint32))
-        assert torch.equal(a["recon"].view(torch.int32), b["recon"].view(torch.int32))
+        for other in (engine_level, engine_pass):
+            b = other.interp_compress(x, p)
+            assert np.array_equal(a["codes"], b["codes"])
+            assert np.array_equal(a["unpred"].view(np.uint32), b["unpred"].view(np.uint32))
+            assert torch.equal(a["recon"].view(torch.int32), b["recon"].view(torch.int32))
+            del b
     del x
     torch.cuda.empty_cache()
