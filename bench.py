@@ -8,11 +8,14 @@ cells, amplitude 0.5); REL error bound 1e-3, cubic interpolation, anchor
 stride 32 (the reference's QoZ 3-D defaults).  Synthesised on the GPU from a
 seed; the field is resident in HBM before the timed region starts.
 
-A step is one device-resident compression of the field: value range,
-interpolation + quantization (one launch per level and pass), histogram and
-outlier compaction, host Huffman tree, GPU encode.  `value` = 4 * n bytes of
-float32 input per step / step time (GB/s).  The ratio and PSNR are measured
-once after the timed steps on the full stream (zstd + Config trailer).
+A step is one device-resident compression of the field, what SZ_compress
+does before zstd: value range, interpolation + quantization (one launch per
+level; per pass on the coarse levels), histogram and outlier compaction,
+host Huffman tree, GPU encode.  Like SZ_compress it returns the engine
+buffer and no reconstruction, so the finest level writes no reconstructed
+values.  `value` = 4 * n bytes of float32 input per step / step time (GB/s).
+The ratio, PSNR and max error are measured once after the timed steps on a
+compression that also returns the reconstruction (same engine buffer).
 
     python bench.py [--steps K] [--warmup W] [--rel 1e-3] [--no-cpu-baseline]
 """
@@ -31,11 +34,6 @@ if ROOT not in sys.path:
 
 METRIC = "compress GB/s + ratio at REL 1e-3 (bit-exact); learned vs exhaustive tuner"
 PEAK_HBM_GBS = 8000.0
-# Algorithmic HBM bytes per predicted point of the interpolation kernels (the
-# fused level kernel and the per-pass kernel alike): read the original float
-# (4) + write the reconstruction (4) + write the 16-bit code (2).  Stencil
-# neighbour re-reads are not counted.
-PASS_BYTES_PER_POINT = 10.0
 
 
 def make_field(shape, seed=0, fronts=16, slope=11.0 / 3.0, noise=1e-4, width=4.0, amp=0.5):
@@ -130,28 +128,33 @@ def main():
     x = make_field(dims, seed=args.seed)
     eng = faz_amd.Engine(0)
     p = faz_amd.make_params(dims, 0.0, algo=1, max_step=32)
-    recon = torch.empty_like(x)
 
     for _ in range(args.warmup):
-        eng.compress_device(x, p, True, args.rel, recon=recon)
+        eng.compress_device(x, p, True, args.rel)
     torch.cuda.synchronize()
     eng.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        raw_size = eng.compress_device(x, p, True, args.rel, recon=recon)
+        raw_size = eng.compress_device(x, p, True, args.rel)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof_ms, launches, points, kname = eng.profile()
+    prof_ms, launches, points, kname, prof_bytes = eng.profile()
     eng.set_profiling(False)
     timings = eng.timings()
+    raw = eng.fetch_raw(raw_size)
 
     dt = (t1 - t0) / args.steps
     value = 4.0 * n / dt / 1e9
     avg_launch_ms = prof_ms / max(launches, 1)
-    achieved = PASS_BYTES_PER_POINT * points / (prof_ms / 1e3) / 1e9 if prof_ms > 0 else 0.0
+    achieved = prof_bytes / (prof_ms / 1e3) / 1e9 if prof_ms > 0 else 0.0
 
-    # quality of the stream (outside the timed region)
+    # quality of the stream (outside the timed region), from a compression that
+    # also returns the reconstruction; its engine buffer must be the same
+    recon = torch.empty_like(x)
+    raw_size2 = eng.compress_device(x, p, True, args.rel, recon=recon)
+    if eng.fetch_raw(raw_size2) != raw:
+        raise SystemExit("engine buffer differs between the runs with and without the reconstruction")
     mn, mx = eng.range(x)
     eb = args.rel * float(torch.tensor(mx - mn, dtype=torch.float32).item())
     stream = eng.sz_compress(x, p, args.rel, rel=True)
@@ -180,7 +183,8 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": kname, "launches_per_step": launches // max(args.steps, 1),
                      "avg_launch_ms": round(avg_launch_ms, 5),
-                     "bytes_per_point": PASS_BYTES_PER_POINT},
+                     "bytes_per_launch": round(prof_bytes / max(launches, 1)),
+                     "bytes_per_point": round(prof_bytes / max(points, 1), 4)},
         "quality": {"ratio": round(4.0 * n / len(stream), 3), "stream_bytes": len(stream), "raw_bytes": raw_size,
                     "max_abs_err": max_err, "eb": eb, "psnr_db": round(psnr, 3)},
         "phases_ms": {"range": round(timings[0], 4), "interp": round(timings[1], 4),

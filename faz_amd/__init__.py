@@ -96,6 +96,7 @@ EXPORTS = [
     "faz_sz_compress_f32", "faz_sz_decompress_f32", "faz_free", "faz_gpu_last_timings",
     "faz_gpu_compress_f32", "faz_gpu_fetch_raw", "faz_gpu_set_profiling", "faz_gpu_profile_read",
     "faz_gpu_profile_kernel",
+    "faz_gpu_profile_bytes",
 ]
 
 _lib = None
@@ -135,6 +136,8 @@ def lib():
         L.faz_gpu_set_profiling.argtypes = [vp, ctypes.c_int]
         L.faz_gpu_profile_read.argtypes = [vp, P(ctypes.c_double), P(ctypes.c_uint64), P(ctypes.c_uint64)]
         L.faz_gpu_profile_kernel.argtypes = [vp]
+        L.faz_gpu_profile_bytes.argtypes = [vp]
+        L.faz_gpu_profile_bytes.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -326,8 +329,8 @@ class Engine:
         self._check(self._L.faz_gpu_profile_read(self.h, ctypes.byref(ms), ctypes.byref(la), ctypes.byref(pts)),
                     "profile_read")
         kind = self._L.faz_gpu_profile_kernel(self.h)
-        name = "interp_level3d_kernel" if kind == 0 else "interp_pass_kernel"
-        return ms.value, la.value, pts.value, name
+        name = {0: "interp_tile3d_kernel", 1: "interp_pass_kernel", 2: "interp_level3d_kernel"}.get(kind, "?")
+        return ms.value, la.value, pts.value, name, self._L.faz_gpu_profile_bytes(self.h)
 
     def timings(self):
         t = (ctypes.c_double * 8)()

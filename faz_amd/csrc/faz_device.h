@@ -75,6 +75,23 @@ __device__ __forceinline__ int quantize_select(float orig, float pred, double eb
     return ok ? shifted : 0;
 }
 
+// quantize_select when |diff| * recip is known to stay below 2^30 (the host
+// bounds it from max |x| and the level's error bound): the truncation never
+// overflows, so the x86 out-of-range path is dead, and the code follows from
+// the signed quantization index as radius + qsigned / 2.
+__device__ __forceinline__ int quantize_bounded(float orig, float pred, double eb, double recip, float ebf,
+                                                int radius, float *rec) {
+    const float diff = orig - pred;
+    const int32_t qi = (int32_t) ((double) fabsf(diff) * recip) + 1;
+    const bool in_range = qi < radius * 2;
+    const int32_t qs = qi & ~1;  // (qi >> 1) << 1
+    const int32_t qsigned = diff < 0 ? -qs : qs;
+    const float dec = (float) ((double) pred + (double) qsigned * eb);
+    const bool ok = in_range && !(fabsf(dec - orig) > ebf);
+    *rec = ok ? dec : orig;
+    return ok ? radius + (qsigned >> 1) : 0;
+}
+
 // LinearQuantizer::recover (IntegerQuantizer.hpp:160-175)
 __device__ __forceinline__ float recover_point(float pred, int code, int radius, double eb) {
     return (float) ((double) pred + (double) (2 * (code - radius)) * eb);

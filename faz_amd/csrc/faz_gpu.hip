@@ -68,6 +68,8 @@ struct faz_gpu_ctx {
     std::string err;
     DevBuf<uint16_t> codes;
     DevBuf<float> outval, unpred, work_in, work_out;
+    // compact lattices handed from one whole-tile level to the next
+    DevBuf<float> lat[2];
     DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     DevBuf<uint8_t> enc_len;
@@ -81,9 +83,9 @@ struct faz_gpu_ctx {
     // per-launch timing of the interpolation pass kernels
     bool profile = false;
     struct ProfBucket {
-        double ms = 0;
+        double ms = 0, bytes = 0;  // bytes: algorithmic HBM bytes of the launches
         uint64_t launches = 0, points = 0;
-    } prof[2];
+    } prof[3];
     std::vector<hipEvent_t> prof_ev;
     // 3-D fields: one fused launch per level (0 = per-pass launches)
     bool fused3d = true;
@@ -412,6 +414,8 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->unpred.release();
     ctx->work_in.release();
     ctx->work_out.release();
+    ctx->lat[0].release();
+    ctx->lat[1].release();
     ctx->freq.release();
     ctx->first.release();
     ctx->chunk_zeros.release();
@@ -463,9 +467,15 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
     return FAZ_OK;
 }
 
-faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
-                                       const faz_interp_params *p, faz_interp_out *out) {
-    if (!ctx || !d_in || !d_recon || !p || !out) return FAZ_ERR_INVALID;
+}  // extern "C"
+
+// Interpolation + quantization of one field.  d_recon is a full-size working
+// array that receives every reconstructed point when final_recon is set;
+// otherwise it only carries the coarse levels (the compression itself needs
+// no reconstruction of the finest level).
+// absmax: max |x| of the input when the caller knows it, else negative.
+static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                           faz_interp_out *out, bool final_recon, double absmax) {
     Plan pl;
     int rc = make_plan(p, pl, ctx->err);
     if (rc) return rc;
@@ -508,33 +518,105 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
         for (int a = 0; a < 3; a++) nbw *= pl.dims[a] == 1 ? 1 : (pl.dims[a] - 1 + L.bs - 1) / L.bs;
         return nbw >= ctx->fused_min_blocks;
     };
-    size_t k = 0;
-    for (auto &L: pl.lv) {
-        if (level_fused(L)) {
-            Level3D g;
-            for (int a = 0; a < 3; a++) {
-                g.dims[a] = pl.dims[a];
-                g.offs[a] = pl.offs[a];
-                g.nb[a] = L.nb[a];
-                g.nbw[a] = pl.dims[a] == 1 ? 1 : (uint32_t) ((pl.dims[a] - 1 + L.bs - 1) / L.bs);
-                g.p[a] = pl.perms[L.dir][a];
+    // kernel per level (coarse to fine): 0 per-pass launches, 1 per-block
+    // level kernel, 2 whole-tile kernel
+    const size_t nlv = pl.lv.size();
+    std::vector<Level3D> g3(nlv);
+    std::vector<int> kind(nlv, 0);
+    for (size_t i = 0; i < nlv; i++) {
+        const LevelPlan &L = pl.lv[i];
+        if (!level_fused(L)) continue;
+        Level3D &g = g3[i];
+        memset(&g, 0, sizeof(g));
+        for (int a = 0; a < 3; a++) {
+            g.dims[a] = pl.dims[a];
+            g.offs[a] = pl.offs[a];
+            g.nb[a] = L.nb[a];
+            g.nbw[a] = pl.dims[a] == 1 ? 1 : (uint32_t) ((pl.dims[a] - 1 + L.bs - 1) / L.bs);
+            g.p[a] = pl.perms[L.dir][a];
+        }
+        g.s = (uint32_t) ((uint64_t) 1 << (L.level - 1));
+        g.bs = (uint32_t) L.bs;
+        g.linear = L.linear;
+        g.eb = L.eb;
+        g.recip = L.recip;
+        g.ebf = float_floor(L.eb);
+        g.radius = p->radius;
+        g.level_base = L.base;
+        g.block_prefix = ctx->prefix.p + L.prefix_at;
+        g.code_limit = pl.total_codes;
+        kind[i] = ctx->tile3d && tile3d_eligible(g) ? 2 : 1;
+    }
+    // The bounded quantizer needs |x - pred| / eb < 2^30 on the level.  Every
+    // prediction is a stencil of reconstructed values of magnitude at most
+    // max|x| + eb with coefficient sum at most 28/8 (quad_3), so
+    // |x - pred| <= 4.5 max|x| + 3.5 eb; half of the range is kept as margin
+    // for rounding.  A non-finite or huge max |x| keeps the exact quantizer.
+    if (std::find(kind.begin(), kind.end(), 2) != kind.end()) {
+        if (!(absmax >= 0)) {
+            float mn = 0, mx = 0;
+            int rr = faz_gpu_range_f32(ctx, d_in, pl.num, &mn, &mx);
+            if (rr) return rr;
+            absmax = std::max(std::fabs((double) mn), std::fabs((double) mx));
+        }
+        for (size_t i = 0; i < nlv; i++) {
+            if (kind[i] != 2) continue;
+            const double v = 4.5 * absmax * g3[i].recip + 8.0;
+            g3[i].bounded = std::isfinite(absmax) && absmax < 1e37 && v < 1073741824.0;
+        }
+    }
+    // Whole-tile levels read the coarser lattice and write their own either
+    // at full resolution in d_recon or, between two whole-tile levels, as a
+    // compact array (lattice index c at c0 m1 m2 + c1 m2 + c2).
+    {
+        int cur = -1;  // lat[] buffer with the previous level's compact lattice
+        for (size_t i = 0; i < nlv; i++) {
+            if (kind[i] != 2) {
+                cur = -1;
+                continue;
             }
-            g.s = (uint32_t) ((uint64_t) 1 << (L.level - 1));
-            g.bs = (uint32_t) L.bs;
-            g.linear = L.linear;
-            g.eb = L.eb;
-            g.recip = L.recip;
-            g.ebf = float_floor(L.eb);
-            g.radius = p->radius;
-            g.level_base = L.base;
-            g.block_prefix = ctx->prefix.p + L.prefix_at;
-            g.code_limit = pl.total_codes;
+            Level3D &g = g3[i];
+            const uint64_t s = g.s;
+            auto compact = [&](uint64_t step, uint64_t *str, uint64_t &count) {
+                uint64_t m[3];
+                for (int a = 0; a < 3; a++) m[a] = (pl.dims[a] - 1) / step + 1;
+                str[0] = m[1] * m[2];
+                str[1] = m[2];
+                str[2] = 1;
+                count = m[0] * m[1] * m[2];
+            };
+            uint64_t count = 0;
+            if (cur < 0) {
+                g.csrc = d_recon;
+                for (int a = 0; a < 3; a++) g.cstr[a] = 2 * s * pl.offs[a];
+            } else {
+                g.csrc = ctx->lat[cur].p;
+                compact(2 * s, g.cstr, count);
+            }
+            const bool last = i + 1 == nlv;
+            const bool next_tile = !last && kind[i + 1] == 2;
+            if (next_tile) {
+                const int nb = cur < 0 ? 0 : 1 - cur;
+                compact(s, g.dstr, count);
+                HIPCHECK(ctx->lat[nb].ensure(count));
+                g.dst = ctx->lat[nb].p;
+                cur = nb;
+            } else {
+                g.dst = last && !final_recon ? nullptr : d_recon;
+                for (int a = 0; a < 3; a++) g.dstr[a] = s * pl.offs[a];
+                cur = -1;
+            }
+        }
+    }
+    size_t k = 0;
+    for (size_t i = 0; i < nlv; i++) {
+        LevelPlan &L = pl.lv[i];
+        if (kind[i]) {
             if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
-            if (ctx->tile3d && tile3d_eligible(g))
-                launch_interp_tile3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid,
-                                     st);
+            if (kind[i] == 2)
+                launch_interp_tile3d(g3[i], d_in, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid, st);
             else
-                launch_interp_level3d(g, d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
+                launch_interp_level3d(g3[i], d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
             if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
             k++;
             continue;
@@ -571,13 +653,25 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
     ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
     ctx->timings[6] = ctx->timings[1];
     if (ctx->profile) {
-        // bucket 0: fused level launches, bucket 1: per-pass launches
+        // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block level launches
         size_t j = 0;
-        for (auto &L: pl.lv) {
-            if (level_fused(L)) {
-                ctx->prof[0].ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
-                ctx->prof[0].launches++;
-                ctx->prof[0].points += L.codes;
+        for (size_t i = 0; i < nlv; i++) {
+            const LevelPlan &L = pl.lv[i];
+            if (kind[i]) {
+                auto &b = ctx->prof[kind[i] == 2 ? 0 : 2];
+                b.ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
+                b.launches++;
+                b.points += L.codes;
+                // read each original and write its code; the reconstruction
+                // traffic is the coarser lattice read and this level's
+                // lattice written, where that happens
+                uint64_t lat = 1, coarse = 1;
+                for (int a = 0; a < 3; a++) {
+                    lat *= (pl.dims[a] - 1) / g3[i].s + 1;
+                    coarse *= (pl.dims[a] - 1) / (2 * g3[i].s) + 1;
+                }
+                const bool writes = kind[i] == 1 || g3[i].dst != nullptr;
+                b.bytes += 6.0 * L.codes + 4.0 * coarse + (writes ? 4.0 * lat : 0.0);
                 j++;
                 continue;
             }
@@ -586,6 +680,7 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
                     ctx->prof[1].ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
                     ctx->prof[1].launches++;
                     ctx->prof[1].points += d.total;
+                    ctx->prof[1].bytes += 10.0 * d.total;  // original in, reconstruction and code out
                 }
                 j++;
             }
@@ -602,6 +697,14 @@ faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, floa
     memset(out->level_counts, 0, sizeof(out->level_counts));
     out->levels = pl.levels;
     return FAZ_OK;
+}
+
+extern "C" {
+
+faz_status faz_gpu_interp_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
+                                       const faz_interp_params *p, faz_interp_out *out) {
+    if (!ctx || !d_in || !d_recon || !p || !out) return FAZ_ERR_INVALID;
+    return interp_compress(ctx, d_in, d_recon, p, out, true, -1.0);
 }
 
 faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, uint32_t *h_freq,
@@ -688,7 +791,8 @@ void faz_ctx_set_error(faz_gpu_ctx *ctx, const char *msg) { ctx->err = msg; }
 
 // Device part of the engine buffer: interpolation + quantization, histogram
 // and outliers, host tree, encode.  Results stay in the context until fetched.
-static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p) {
+static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
+                           double absmax = -1.0) {
     if (!ctx || !d_in || !p) return FAZ_ERR_INVALID;
     ctx->pending.valid = false;
     HIPCHECK(hipSetDevice(ctx->device));
@@ -701,7 +805,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         recon = ctx->work_out.p;
     }
     faz_interp_out io;
-    int rc = faz_gpu_interp_compress_f32(ctx, d_in, recon, p, &io);
+    int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax);
     if (rc) return rc;
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
     HIPCHECK(hipMemcpyAsync(ctx->h_freq.data(), ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
@@ -797,6 +901,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
     if (!ctx || !d_in || !p || !raw_size || p->N < 1 || p->N > 4) return FAZ_ERR_INVALID;
     faz_interp_params q = *p;
     ctx->timings[0] = 0;
+    double absmax = -1.0;
     if (rel_mode) {
         size_t num = 1;
         for (int i = 0; i < p->N; i++) num *= p->dims[i];
@@ -809,10 +914,11 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         HIPCHECK(hipEventSynchronize(ctx->ev[7]));
         ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);
         q.eb = eb * (double) (float) (mx - mn);
+        absmax = std::max(std::fabs((double) mn), std::fabs((double) mx));
     } else {
         q.eb = eb;
     }
-    int rc = compress_device(ctx, d_in, d_recon, &q);
+    int rc = compress_device(ctx, d_in, d_recon, &q, absmax);
     if (rc) return rc;
     *raw_size = pending_size(ctx);
     return FAZ_OK;
@@ -829,24 +935,31 @@ faz_status faz_gpu_fetch_raw(faz_gpu_ctx *ctx, uint8_t *h_raw, size_t cap) {
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on) {
     if (!ctx) return FAZ_ERR_INVALID;
     ctx->profile = on != 0;
-    ctx->prof[0] = faz_gpu_ctx::ProfBucket();
-    ctx->prof[1] = faz_gpu_ctx::ProfBucket();
+    for (auto &b: ctx->prof) b = faz_gpu_ctx::ProfBucket();
     return FAZ_OK;
 }
 
 faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points) {
     if (!ctx || !ms || !launches || !points) return FAZ_ERR_INVALID;
-    // the dominant kernel: whichever launch family took longer
-    const auto &b = ctx->prof[0].ms >= ctx->prof[1].ms ? ctx->prof[0] : ctx->prof[1];
+    // the dominant kernel: whichever launch family took longest
+    const auto &b = ctx->prof[faz_gpu_profile_kernel(ctx)];
     *ms = b.ms;
     *launches = b.launches;
     *points = b.points;
     return FAZ_OK;
 }
 
+double faz_gpu_profile_bytes(const faz_gpu_ctx *ctx) {
+    if (!ctx) return 0;
+    return ctx->prof[faz_gpu_profile_kernel(ctx)].bytes;
+}
+
 int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx) {
     if (!ctx) return -1;
-    return ctx->prof[0].ms >= ctx->prof[1].ms ? 0 : 1;
+    int best = 0;
+    for (int i = 1; i < 3; i++)
+        if (ctx->prof[i].ms > ctx->prof[best].ms) best = i;
+    return best;
 }
 
 faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,

@@ -51,10 +51,19 @@ struct Level3D {
     int linear;
     double eb, recip;
     float ebf;                // largest float <= eb
+    int bounded;              // |x - pred| / eb provably below 2^30 on this level
     int radius;
     uint64_t level_base;
     const uint64_t *block_prefix;
     uint64_t code_limit;      // codes in the whole stream (bounds every write)
+    // Whole-tile kernel only: where the points of the coarser lattice (every
+    // coordinate a multiple of 2s) are read, and where this level's lattice
+    // (multiples of s) is written.  Lattice point c (per axis) lives at
+    // csrc[sum c_a * cstr_a] / dst[sum c_a * dstr_a]; dst may be null.
+    const float *csrc;
+    uint64_t cstr[3];
+    float *dst;
+    uint64_t dstr[3];
 };
 
 // Largest float not above a positive double.
@@ -75,8 +84,8 @@ constexpr int kPerm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0,
 // spans 32 or 33 lattice points on each axis (interpBlockSize 32) and every
 // tile offset fits 32 bits.  `grid` persistent workgroups share the blocks.
 bool tile3d_eligible(const Level3D &L);
-void launch_interp_tile3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
-                          uint32_t *d_err, uint32_t grid, hipStream_t st);
+void launch_interp_tile3d(const Level3D &L, const float *d_in, uint16_t *d_codes, float *d_outval, uint32_t *d_err,
+                          uint32_t grid, hipStream_t st);
 
 // Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
 void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,

@@ -9,9 +9,9 @@
 // tile's values are already in flight into registers, so the HBM reads of a
 // block overlap the arithmetic of the previous one.
 //
-// Work mapping inside a pass: eight lanes per line, lane t (= lane & 7)
-// taking the odd positions i = 4t+1 and 4t+3, whose five even neighbours are
-// read once and shared by the two stencils.  Lines have
+// Work mapping inside a pass: four lanes per line, lane t (= lane & 3)
+// taking the odd positions i = 8t+1, 8t+3, 8t+5, 8t+7, whose seven even
+// neighbours are read once and shared by the four stencils.  Lines have
 // 32 or 33 points only, so every position has a fixed stencil (cubic inside,
 // quad_1 at i = 1, quad_2 at the loop exit, quad_3 at n-1 for even n:
 // SIC:1360-1414) and a fixed emission rank.
@@ -37,7 +37,7 @@ constexpr int kPlane = 1157;                 // LDS stride of axis 0 (33 * 35 + 
 constexpr int kTileFloats = 33 * kPlane;     // 38181 floats = 152724 bytes
 constexpr int kEdge = 33 * 32 + 33 * 33;     // l0 = 32 plane (l2 < 32) and l2 = 32 column
 constexpr int kEdgeSlots = (kEdge + kThreads - 1) / kThreads;
-constexpr int kLanesPerLine = 8;             // lanes sharing one line of 16 odd positions
+constexpr int kLanesPerLine = 4;             // lanes sharing one line of 16 odd positions
 constexpr int kPerLane = 16 / kLanesPerLine;  // odd positions per lane
 constexpr int kLinesPerRound = kThreads / kLanesPerLine;
 
@@ -52,7 +52,9 @@ struct Tile {
     uint32_t first;     // bit a set when the block starts at 0 on axis a
     uint32_t miss;      // kAt32_a set when axis a has no position 32
     uint32_t nown;      // kAt32_a set when this block does not write position 32 of axis a
-    uint64_t gbase;     // element offset of the block origin
+    uint64_t gbase;     // element offset of the block origin in the input
+    uint64_t cbase;     // ... in the coarser lattice
+    uint64_t dbase;     // ... in the destination lattice
     uint64_t seg;       // first code of the block
     uint32_t cnt0, cnt1, total;
     uint32_t emit;      // 1 when the block's code segment lies inside the stream
@@ -63,15 +65,16 @@ __device__ __forceinline__ uint32_t first_of(const Tile &B, int a) { return (B.f
 __device__ __forceinline__ uint32_t g1_of(const Tile &B, int a) { return B.n[a] - 1 + first_of(B, a); }
 __device__ __forceinline__ uint32_t g2_of(const Tile &B, int a) { return (B.n[a] - 1) / 2 + first_of(B, a); }
 
-// Origin and missing-position mask of block blk (what the loads need).
-__device__ __forceinline__ void tile_origin(const Level3D &L, uint32_t blk, uint64_t &gbase, uint32_t &miss) {
+// Origins and missing-position mask of block blk (what the loads need).
+__device__ __forceinline__ void tile_origin(const Level3D &L, uint32_t blk, uint64_t &gbase, uint64_t &cbase,
+                                            uint32_t &miss) {
     uint32_t r = blk;
     const uint32_t k2 = r % L.nbw[2];
     r /= L.nbw[2];
     const uint32_t k1 = r % L.nbw[1];
     const uint32_t k0 = r / L.nbw[1];
     const uint32_t k[3] = {k0, k1, k2};
-    uint64_t g = 0;
+    uint64_t g = 0, c = 0;
     miss = 0;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -79,8 +82,10 @@ __device__ __forceinline__ void tile_origin(const Level3D &L, uint32_t blk, uint
         const uint32_t e = min(b + L.bs, L.dims[a] - 1);
         if ((e - b) / L.s + 1 != 33) miss |= kAt32_0 << a;
         g += (uint64_t) b * (a == 2 ? 1 : L.offs[a]);
+        c += (uint64_t) (b / (2 * L.s)) * L.cstr[a];
     }
     gbase = g;
+    cbase = c;
 }
 
 template<int DIR>
@@ -108,6 +113,12 @@ __device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
         if (whi != 32) B.nown |= kAt32_0 << a;
     }
     B.gbase = (uint64_t) b[0] * L.offs[0] + (uint64_t) b[1] * L.offs[1] + b[2];
+    B.cbase = B.dbase = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        B.cbase += (uint64_t) (b[a] / (2 * L.s)) * L.cstr[a];
+        B.dbase += (uint64_t) (b[a] / L.s) * L.dstr[a];
+    }
     B.seg = L.level_base + L.block_prefix[((uint64_t) k0 * L.nb[1] + k1) * L.nb[2] + k2];
     B.cnt0 = 16 * g2_of(B, p1) * g2_of(B, p2);
     B.cnt1 = g1_of(B, p0) * 16 * g2_of(B, p2);
@@ -122,16 +133,26 @@ __device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
 // Points with every coordinate even come from the reconstruction of the
 // coarser levels, all others from the original data.
 struct Place {
-    uint32_t gl, ll;                    // main: global byte / LDS offset of (tid >> 5, 0, l2)
+    uint32_t gl, ll;                    // main: input byte / LDS offset of (tid >> 5, 0, l2)
+    uint32_t cgl, dgl;                  // main: coarse-lattice / destination byte offset
     uint32_t lane_even;                 // l0 and l2 both even
-    uint32_t eg[kEdgeSlots];            // edge: global byte offsets
+    uint32_t eg[kEdgeSlots];            // edge: input (or coarse-lattice, all even) byte offsets
+    uint32_t dg[kEdgeSlots];            // edge: destination byte offsets
     uint32_t ef[kEdgeSlots];            // edge: LDS index << 8 | flags
 };
 
-__device__ __forceinline__ Place place_of(uint32_t tid, uint32_t S0, uint32_t S1, uint32_t s) {
+// Element strides per tile step: in the input (S), the coarser lattice per
+// even step (C) and the destination lattice (D).
+struct Strides {
+    uint32_t S[3], C[3], D[3];
+};
+
+__device__ __forceinline__ Place place_of(uint32_t tid, const Strides &X) {
     Place P;
     const uint32_t l0 = tid >> 5, l2 = tid & 31;
-    P.gl = 4 * (l0 * S0 + l2 * s);
+    P.gl = 4 * (l0 * X.S[0] + l2 * X.S[2]);
+    P.cgl = 4 * ((l0 >> 1) * X.C[0] + (l2 >> 1) * X.C[2]);
+    P.dgl = 4 * (l0 * X.D[0] + l2 * X.D[2]);
     P.ll = l0 * kPlane + l2;
     P.lane_even = ((l0 | l2) & 1) == 0 ? 1u : 0u;
 #pragma unroll
@@ -144,9 +165,11 @@ __device__ __forceinline__ Place place_of(uint32_t tid, uint32_t S0, uint32_t S1
             const uint32_t x = e - 33 * 32;
             c0 = x / 33, c1 = x % 33, c2 = 32, f = kHas;
         }
-        f |= (c0 == 32 ? kAt32_0 : 0) | (c1 == 32 ? kAt32_1 : 0) | (c2 == 32 ? kAt32_2 : 0) |
-             (((c0 | c1 | c2) & 1) == 0 ? kAllEven : 0);
-        P.eg[j] = 4 * (c0 * S0 + c1 * S1 + c2 * s);
+        const bool even = ((c0 | c1 | c2) & 1) == 0;
+        f |= (c0 == 32 ? kAt32_0 : 0) | (c1 == 32 ? kAt32_1 : 0) | (c2 == 32 ? kAt32_2 : 0) | (even ? kAllEven : 0);
+        P.eg[j] = even ? 4 * ((c0 >> 1) * X.C[0] + (c1 >> 1) * X.C[1] + (c2 >> 1) * X.C[2])
+                       : 4 * (c0 * X.S[0] + c1 * X.S[1] + c2 * X.S[2]);
+        P.dg[j] = 4 * (c0 * X.D[0] + c1 * X.D[1] + c2 * X.D[2]);
         P.ef[j] = (c0 * kPlane + c1 * kRow + c2) << 8 | f;
     }
     return P;
@@ -185,19 +208,18 @@ struct Pre {
     float ed[kEdgeSlots];
 };
 
-__device__ __forceinline__ void issue_loads(Pre &R, const Place &P, uint64_t gbase, uint32_t miss,
-                                            const float *__restrict__ in, const float *__restrict__ recon,
-                                            uint32_t S0, uint32_t S1) {
+__device__ __forceinline__ void issue_loads(Pre &R, const Place &P, const Strides &X, uint64_t gbase,
+                                            uint64_t cbase, uint32_t miss, const float *__restrict__ in,
+                                            const float *__restrict__ csrc) {
     const Rsrc ib = rsrc_at(in + gbase);
-    const Rsrc rb = rsrc_at(recon + gbase);
+    const Rsrc rb = rsrc_at(csrc + cbase);
 #pragma unroll
     for (int k = 0; k < 33; k++) {
         if (k == 32 && (miss & kAt32_1)) break;
-        const uint32_t so = 4 * k * S1;
         if (!(k & 1) && P.lane_even)
-            R.m[k] = ld_coarse(rb, P.gl, so);
+            R.m[k] = ld_coarse(rb, P.cgl, 4 * (k >> 1) * X.C[1]);
         else
-            R.m[k] = ld(ib, P.gl, so);
+            R.m[k] = ld(ib, P.gl, 4 * k * X.S[1]);
     }
 #pragma unroll
     for (int j = 0; j < kEdgeSlots; j++) {
@@ -219,21 +241,23 @@ __device__ __forceinline__ void fill_tile(float *T, const Pre &R, const Place &P
         if (slot_on(P.ef[j], miss)) T[P.ef[j] >> 8] = R.ed[j];
 }
 
-__device__ __forceinline__ void store_tile(const float *T, const Place &P, const Tile &B, float *__restrict__ recon,
-                                           uint32_t S0, uint32_t S1) {
-    const Rsrc rb = rsrc_at(recon + B.gbase);
+// The block's part of this level's lattice: [b, b + bs) per axis, through the
+// last point for the last block along an axis.
+__device__ __forceinline__ void store_tile(const float *T, const Place &P, const Strides &X, const Tile &B,
+                                           float *__restrict__ dst) {
+    const Rsrc db = rsrc_at(dst + B.dbase);
 #pragma unroll
-    for (int k = 0; k < 32; k++) st(rb, T[P.ll + k * kRow], P.gl, 4 * k * S1);
-    if (!(B.nown & kAt32_1)) st(rb, T[P.ll + 32 * kRow], P.gl, 4 * 32 * S1);
+    for (int k = 0; k < 32; k++) st(db, T[P.ll + k * kRow], P.dgl, 4 * k * X.D[1]);
+    if (!(B.nown & kAt32_1)) st(db, T[P.ll + 32 * kRow], P.dgl, 4 * 32 * X.D[1]);
 #pragma unroll
     for (int j = 0; j < kEdgeSlots; j++)
-        if (slot_on(P.ef[j], B.nown)) st(rb, T[P.ef[j] >> 8], P.eg[j], 0);
+        if (slot_on(P.ef[j], B.nown)) st(db, T[P.ef[j] >> 8], P.dg[j], 0);
 }
 
 // One pass of the block over the tile in LDS.
-template<int DIR, bool LINEAR, int PASS>
+template<int DIR, bool LINEAR, bool BOUNDED, int PASS>
 __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D &L, Rsrc cb, Rsrc ob,
-                                          uint32_t tid) {
+                                          uint32_t tid, bool &wide) {
     constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
     constexpr int A = PASS == 0 ? p0 : (PASS == 1 ? p1 : p2);  // interpolated axis
     constexpr int U = PASS == 0 ? p1 : p0;                     // slow line axis (code order)
@@ -307,14 +331,16 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
             }
             const bool own = B.emit && (fu | u) && (fw | w);
             const uint32_t cl = pbase + ((u + fu - 1) * gw + (w + fw - 1)) * 16;
+            const uint32_t coff = own ? 2 * cl : kDropped;  // codes of lines owned elsewhere are dropped
 #pragma unroll
             for (int j = 0; j < kPerLane; j++) {
                 float rec;
-                const int code = quantize_select(o[j], pr[j], L.eb, L.recip, L.ebf, L.radius, &rec);
+                const int code = BOUNDED ? quantize_bounded(o[j], pr[j], L.eb, L.recip, L.ebf, L.radius, &rec)
+                                         : quantize_select(o[j], pr[j], L.eb, L.recip, L.ebf, L.radius, &rec);
+                if (!BOUNDED) wide |= (uint32_t) code > 65535u;
                 T[base + (2 * j + 1) * ESA] = rec;
-                const uint32_t slot = cl + rk[j];
-                __builtin_amdgcn_raw_buffer_store_b16((uint16_t) code, cb, own ? 2 * slot : kDropped, 0, 0);
-                st(ob, o[j], own && code == 0 ? 4 * slot : kDropped, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t) code, cb, coff + 2 * rk[j], 0, 0);
+                if (own && code == 0) st(ob, o[j], 4 * (cl + rk[j]), 0);
             }
         }
     }
@@ -322,9 +348,8 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
 
 }  // namespace
 
-template<int DIR, bool LINEAR>
+template<int DIR, bool LINEAR, bool BOUNDED>
 __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, const float *__restrict__ in,
-                                                                 float *__restrict__ recon,
                                                                  uint16_t *__restrict__ codes,
                                                                  float *__restrict__ outval,
                                                                  uint32_t *__restrict__ err) {
@@ -345,15 +370,22 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
         step = gridDim.x;
     }
     if (blk >= end) return;
-    const uint32_t S0 = L.s * (uint32_t) L.offs[0], S1 = L.s * (uint32_t) L.offs[1];
-    const Place P = place_of(tid, S0, S1, L.s);
+    Strides X;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        X.S[a] = L.s * (uint32_t) (a == 2 ? 1 : L.offs[a]);
+        X.C[a] = (uint32_t) L.cstr[a];
+        X.D[a] = (uint32_t) L.dstr[a];
+    }
+    const Place P = place_of(tid, X);
 
+    bool wide = false;  // a code beyond 16 bits (possible only without the bound)
     Pre R;
     {
-        uint64_t g;
+        uint64_t g, c;
         uint32_t miss;
-        tile_origin(L, blk, g, miss);
-        issue_loads(R, P, g, miss, in, recon, S0, S1);
+        tile_origin(L, blk, g, c, miss);
+        issue_loads(R, P, X, g, c, miss, in, L.csrc);
     }
     for (;;) {
         const Tile B = tile_of<DIR>(L, blk);
@@ -362,10 +394,10 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
         __syncthreads();
         const uint32_t nxt = blk + step;
         if (nxt < end) {
-            uint64_t g;
+            uint64_t g, c;
             uint32_t miss;
-            tile_origin(L, nxt, g, miss);
-            issue_loads(R, P, g, miss, in, recon, S0, S1);
+            tile_origin(L, nxt, g, c, miss);
+            issue_loads(R, P, X, g, c, miss, in, L.csrc);
         }
         // bounded to the block's segment of the code stream; the segment is
         // read through a vector load, so its uniformity is stated explicitly
@@ -374,17 +406,18 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
         const int total = __builtin_amdgcn_readfirstlane((int) B.total);
         const Rsrc cb = rsrc_at(codes + seg, 2 * total);
         const Rsrc ob = rsrc_at(outval + seg, 4 * total);
-        tile_pass<DIR, LINEAR, 0>(T, B, L, cb, ob, tid);
+        tile_pass<DIR, LINEAR, BOUNDED, 0>(T, B, L, cb, ob, tid, wide);
         __syncthreads();
-        tile_pass<DIR, LINEAR, 1>(T, B, L, cb, ob, tid);
+        tile_pass<DIR, LINEAR, BOUNDED, 1>(T, B, L, cb, ob, tid, wide);
         __syncthreads();
-        tile_pass<DIR, LINEAR, 2>(T, B, L, cb, ob, tid);
+        tile_pass<DIR, LINEAR, BOUNDED, 2>(T, B, L, cb, ob, tid, wide);
         __syncthreads();
-        store_tile(T, P, B, recon, S0, S1);
+        if (L.dst) store_tile(T, P, X, B, L.dst);
         if (nxt >= end) break;
         __syncthreads();
         blk = nxt;
     }
+    if (!BOUNDED && wide) atomicOr(err, 1u);
 }
 
 bool tile3d_eligible(const Level3D &L) {
@@ -401,22 +434,29 @@ bool tile3d_eligible(const Level3D &L) {
     return span < (1ull << 29);
 }
 
-void launch_interp_tile3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
-                          uint32_t *d_err, uint32_t grid, hipStream_t st) {
+void launch_interp_tile3d(const Level3D &L, const float *d_in, uint16_t *d_codes, float *d_outval, uint32_t *d_err,
+                          uint32_t grid, hipStream_t st) {
     const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
     if (!nblk || !grid) return;
     dim3 g(std::min(grid, nblk)), block(kThreads);
     int dir = 0;
     for (int d = 0; d < 6; d++)
         if (kPerm3[d][0] == L.p[0] && kPerm3[d][1] == L.p[1] && kPerm3[d][2] == L.p[2]) dir = d;
-#define FAZ_T3D(D)                                                                                               \
+#define FAZ_T3D_B(D, LIN)                                                                                        \
     do {                                                                                                         \
-        if (L.linear)                                                                                            \
-            hipLaunchKernelGGL((interp_tile3d_kernel<D, true>), g, block, 0, st, L, d_in, d_recon, d_codes,      \
-                               d_outval, d_err);                                                                 \
+        if (L.bounded)                                                                                           \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, true>), g, block, 0, st, L, d_in, d_codes, d_outval, \
+                               d_err);                                                                           \
         else                                                                                                     \
-            hipLaunchKernelGGL((interp_tile3d_kernel<D, false>), g, block, 0, st, L, d_in, d_recon, d_codes,     \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, false>), g, block, 0, st, L, d_in, d_codes,         \
                                d_outval, d_err);                                                                 \
+    } while (0)
+#define FAZ_T3D(D)                 \
+    do {                           \
+        if (L.linear)              \
+            FAZ_T3D_B(D, true);    \
+        else                       \
+            FAZ_T3D_B(D, false);   \
     } while (0)
     switch (dir) {
         case 0: FAZ_T3D(0); break;
@@ -427,6 +467,7 @@ void launch_interp_tile3d(const Level3D &L, const float *d_in, float *d_recon, u
         default: FAZ_T3D(5); break;
     }
 #undef FAZ_T3D
+#undef FAZ_T3D_B
 }
 
 }  // namespace faz

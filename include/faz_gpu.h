@@ -128,7 +128,10 @@ faz_status faz_gpu_interp_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_
                                   uint8_t *h_raw, size_t cap, size_t *raw_size);
 
 /* --- The same engine buffer in two steps, so the device work can be timed
- * on its own: faz_gpu_compress_f32 runs the value range (rel_mode != 0:
+ * on its own.  d_recon may be NULL: SZ_compress returns no reconstruction,
+ * and without one the finest level writes none (the coarse levels still keep
+ * theirs in context scratch); with one it receives every reconstructed
+ * point.  faz_gpu_compress_f32 runs the value range (rel_mode != 0:
  * eb is relative to max-min, Statistic.hpp:34-41), interpolation +
  * quantization, histogram, outlier compaction, the host tree and the encode
  * (SIC:328-518) and leaves the result in the context; *raw_size gets the
@@ -157,14 +160,19 @@ void faz_free(void *p);
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
 /* Per-launch HIP-event timing of the interpolation kernels, accumulated while
- * enabled for the fused 3-D level kernel and the per-pass kernel separately;
- * faz_gpu_profile_read reports the family that took longer (the dominant
- * kernel): total milliseconds, launches and predicted points, and
- * faz_gpu_profile_kernel which family that is (0 fused level kernel,
- * 1 per-pass kernel).  Enabling resets the counters. */
+ * enabled for each kernel family separately; faz_gpu_profile_read reports the
+ * family that took longest (the dominant kernel): total milliseconds,
+ * launches and predicted points, and faz_gpu_profile_kernel which family that
+ * is (0 whole-tile 3-D level kernel, 1 per-pass kernel, 2 per-block 3-D level
+ * kernel).  Enabling resets the counters. */
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on);
 faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points);
 int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx);
+/* Algorithmic HBM bytes of the dominant family's launches: 4 per original
+ * read and 2 per code written, plus 4 per reconstructed lattice point read
+ * from the coarser level and 4 per point of the level's lattice written
+ * (when it is written). */
+double faz_gpu_profile_bytes(const faz_gpu_ctx *ctx);
 
 #ifdef __cplusplus
 }

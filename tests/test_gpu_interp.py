@@ -164,6 +164,37 @@ def test_tile_kernel_against_oracle(engine_tile, i):
     assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
 
 
+@pytest.mark.parametrize("algo", [0, 1])
+def test_tile_kernel_exact_quantizer(engine_tile, algo):
+    """An error bound so small against max |x| that |x - pred| / eb may pass
+    2^31: the whole-tile levels keep the quantizer's x86 out-of-range path."""
+    import faz_amd
+    shape = (64, 65, 64)
+    x = _field(shape, 800 + algo) + np.float32(1e4)
+    eb = 1e-9 * float(np.float32(x.max() - x.min()))
+    kw = dict(algo=algo, direction=2, max_step=32)
+    print(f"[gpu] tile exact quantizer {shape} eb={eb}", flush=True)
+    o = oracle.interp_compress(x, oracle.make_params(shape, eb, **kw))
+    r = engine_tile.interp_compress(_dev(x), faz_amd.make_params(shape, eb, **kw))
+    assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
+    assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
+    assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+
+
+@pytest.mark.parametrize("i", [0, 2, 7, 9, 10, 11])
+def test_compression_without_reconstruction(engine_tile, i):
+    """SZ_compress returns no reconstruction: without one the whole-tile levels
+    hand compact lattices to each other and the finest level writes nothing;
+    the engine buffer stays bit-exact."""
+    import faz_amd
+    shape, kw, rel = TILE3D[i]
+    x = _field(shape, 700 + i)
+    eb = rel * float(np.float32(x.max() - x.min()))
+    print(f"[gpu] no-reconstruction path {shape} {kw}", flush=True)
+    size = engine_tile.compress_device(_dev(x), faz_amd.make_params(shape, eb, **kw), False, eb)
+    assert engine_tile.fetch_raw(size) == oracle.interp_raw(x, oracle.make_params(shape, eb, **kw))
+
+
 def test_range_matches_oracle(engine):
     x = _field((33, 45, 57), 5) - 3.0
     mn, mx = engine.range(_dev(x))
@@ -205,6 +236,9 @@ def test_full_size_512_properties(engine, rel):
     r = engine.interp_compress(x, p)
     assert r["n_codes"] == x.numel()
     assert r["unpred"].size == int((r["codes"] == 0).sum())
+    # the bench path: no reconstruction requested, same engine buffer
+    size = engine.compress_device(x, p, False, eb)
+    assert engine.fetch_raw(size) == raw
     del x, recon, out
     torch.cuda.empty_cache()
 
