@@ -72,13 +72,13 @@ struct faz_gpu_ctx {
     DevBuf<float> lat[2];
     DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
-    DevBuf<uint8_t> enc_len;
     hipEvent_t ev[8] = {};
     double timings[8] = {};
     // histogram of the codes last produced by interp_compress
     const uint16_t *hist_codes = nullptr;
     size_t hist_n = 0;
     std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
+    std::vector<uint64_t> h_tab;  // packed code table of the last encode
     PendingRaw pending;
     // per-launch timing of the interpolation pass kernels
     bool profile = false;
@@ -428,7 +428,6 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->chunk_bits.release();
     ctx->chunk_off.release();
     ctx->enc_code.release();
-    ctx->enc_len.release();
     ctx->tiles.release();
     for (auto &e: ctx->ev)
         if (e) (void) hipEventDestroy(e);
@@ -751,14 +750,19 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    // one packed entry per symbol: codeword right-aligned above an 8-bit length
+    ctx->h_tab.resize((size_t) t->n_states);
+    for (int32_t i = 0; i < t->n_states; i++) {
+        const uint32_t L = t->len[i];
+        if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
+        ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
+    }
     HIPCHECK(ctx->enc_code.ensure((size_t) t->n_states));
-    HIPCHECK(ctx->enc_len.ensure((size_t) t->n_states));
     HIPCHECK(ctx->chunk_bits.ensure(nchunks));
     HIPCHECK(ctx->chunk_off.ensure(nchunks));
     HIPCHECK(ctx->scalars.ensure(4));
-    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, t->code, (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(ctx->enc_len.p, t->len, (size_t) t->n_states, hipMemcpyHostToDevice, st));
-    launch_encode_sizes(d_codes, n, ctx->enc_len.p, t->offset, ctx->chunk_bits.p, st);
+    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, ctx->h_tab.data(), (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
+    launch_encode_sizes(d_codes, n, ctx->enc_code.p, (uint32_t) t->n_states, t->offset, ctx->chunk_bits.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
     HIPCHECK(hipGetLastError());
@@ -772,7 +776,7 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     uint32_t *w = reinterpret_cast<uint32_t *>(d_bits);
     if (((uintptr_t) d_bits & 3) != 0) CTX_FAIL(FAZ_ERR_INVALID, "bit buffer must be 4-byte aligned");
     launch_fill_u32(w, words, 0u, st);
-    launch_encode_pack(d_codes, n, ctx->enc_code.p, ctx->enc_len.p, t->offset, ctx->chunk_off.p, w, st);
+    launch_encode_pack(d_codes, n, ctx->enc_code.p, (uint32_t) t->n_states, t->offset, ctx->chunk_off.p, w, st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
     return FAZ_OK;

@@ -271,12 +271,12 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
     size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
     const float4 *in4 = reinterpret_cast<const float4 *>(in);
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n4; i += 4 * stride) {
-        float4 v[4];
+    for (; i + 7 * stride < n4; i += 8 * stride) {
+        float4 v[8];
 #pragma unroll
-        for (int u = 0; u < 4; u++) v[u] = in4[i + u * stride];
+        for (int u = 0; u < 8; u++) v[u] = in4[i + u * stride];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < 8; u++) {
             uint32_t a = f2key(v[u].x), b = f2key(v[u].y), c = f2key(v[u].z), e = f2key(v[u].w);
             lo = min(lo, min(min(a, b), min(c, e)));
             hi = max(hi, max(max(a, b), max(c, e)));
@@ -322,7 +322,7 @@ __global__ void init_minmax_kernel(uint32_t *minmax) {
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st) {
     hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(1), 0, st, d_minmax);
     size_t want = (n / 4 + 255) / 256;
-    unsigned grid = (unsigned) (want < 4096 ? (want ? want : 1) : 4096);
+    unsigned grid = (unsigned) (want < 2048 ? (want ? want : 1) : 2048);
     hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax);
 }
 
@@ -334,104 +334,113 @@ void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t s
 // at most kHistPerThread <= 255 codes per chunk) and folded once per chunk;
 // the rest of a 4096-symbol window goes to LDS atomics, anything beyond it to
 // global atomics.  First positions: LDS atomicMin guarded by a read, so each
-// symbol costs at most a few atomics per chunk.
+// symbol costs at most a few atomics per workgroup.
 // ---------------------------------------------------------------------------
 constexpr int kWin = 4096;  // LDS bins around the quantizer radius
 constexpr int kHot = 16;    // register-counted symbols [hot0, hot0 + 16)
 static_assert(kHistPerThread <= 255, "8-bit register counters");
 
-__device__ __forceinline__ void hist_cold(uint32_t v, uint32_t li, uint32_t base, uint32_t lo, uint32_t *s_cnt,
-                                          uint32_t *s_first, uint32_t *freq, uint32_t *first, uint32_t *s_zero,
-                                          uint32_t *s_zero_first) {
-    if (v == 0) {
-        atomicAdd(s_zero, 1u);
-        if (*s_zero_first > li) atomicMin(s_zero_first, li);
-        return;
-    }
-    uint32_t w = v - lo;
-    if (w < (uint32_t) kWin) {
-        atomicAdd(&s_cnt[w], 1u);
-        if (s_first[w] > li) atomicMin(&s_first[w], li);
-    } else {
-        atomicAdd(&freq[v], 1u);
-        atomicMin(&first[v], base + li);
-    }
-}
-
+// Persistent: a workgroup keeps its LDS window and register counters across
+// the chunks it takes (in increasing order), so the window is cleared and
+// flushed to the global histogram once per workgroup, not once per chunk.
 __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                            uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
                                                            uint32_t *__restrict__ chunk_zeros, int lo_,
                                                            int hot0_, int aligned) {
     __shared__ uint32_t s_cnt[kWin];
-    __shared__ uint32_t s_first[kWin];
+    __shared__ uint32_t s_first[kWin];  // first global position per window symbol
     __shared__ uint32_t s_zero, s_zero_first;
     const uint32_t lo = (uint32_t) lo_, hot0 = (uint32_t) hot0_;
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
         s_cnt[i] = 0;
         s_first[i] = 0xFFFFFFFFu;
     }
-    if (threadIdx.x == 0) {
-        s_zero = 0;
-        s_zero_first = 0xFFFFFFFFu;
-    }
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kHistChunk;
-    const uint32_t end = min(base + kHistChunk, n);
-    uint64_t hot_a = 0, hot_b = 0;  // 8-bit counters of symbols hot0..hot0+7, hot0+8..hot0+15
-    uint32_t seen = 0;              // hot symbols whose first position this thread already offered
-    // each thread takes 8 consecutive codes per step (one 16-byte load)
-    for (uint32_t j = base + threadIdx.x * 8; j < end; j += kHistThreads * 8) {
-        uint16_t v[8];
-        if (aligned && j + 8 <= end) {
-            uint4 q = *reinterpret_cast<const uint4 *>(codes + j);
-            v[0] = (uint16_t) q.x; v[1] = (uint16_t) (q.x >> 16);
-            v[2] = (uint16_t) q.y; v[3] = (uint16_t) (q.y >> 16);
-            v[4] = (uint16_t) q.z; v[5] = (uint16_t) (q.z >> 16);
-            v[6] = (uint16_t) q.w; v[7] = (uint16_t) (q.w >> 16);
-        } else {
+    if (threadIdx.x == 0) s_zero_first = 0xFFFFFFFFu;
+    uint32_t seen = 0;  // hot symbols whose first position this thread already offered
+    uint32_t zeros_total = 0;
+    const int lane = threadIdx.x % 64;
+    const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        if (threadIdx.x == 0) s_zero = 0;
+        __syncthreads();
+        const uint32_t base = c * kHistChunk;
+        const uint32_t end = min(base + kHistChunk, n);
+        uint64_t hot_a = 0, hot_b = 0;  // 8-bit counters of hot0..hot0+7, hot0+8..hot0+15 (this chunk)
+        // the thread's 8 groups of 8 consecutive codes (16-byte loads), all
+        // issued before any is used
+        constexpr int kGroups = kHistPerThread / 8;
+        uint4 q[kGroups];
+        const bool full = aligned && end - base == kHistChunk;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = j + k < end ? codes[j + k] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            if (j + k >= end) break;
-            const uint32_t li = j + k - base;
-            const uint32_t d = (uint32_t) v[k] - hot0;
-            if (d < (uint32_t) kHot) {
-                const uint64_t one = 1ull << (8 * (d & 7));
-                if (d < 8) hot_a += one; else hot_b += one;
-                if (!((seen >> d) & 1u)) {
-                    seen |= 1u << d;
-                    uint32_t w = v[k] - lo;
-                    if (s_first[w] > li) atomicMin(&s_first[w], li);
-                }
+        for (int g = 0; g < kGroups; g++) {
+            const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
+            if (full) {
+                q[g] = *reinterpret_cast<const uint4 *>(codes + j);
             } else {
-                hist_cold(v[k], li, base, lo, s_cnt, s_first, freq, first, &s_zero, &s_zero_first);
+                uint32_t t[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) t[k] = j + k < end ? codes[j + k] : 0;
+                q[g] = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
             }
         }
-    }
-    // fold the register counters: wave reduction, then one LDS add per wave
-    const int lane = threadIdx.x % 64;
 #pragma unroll
-    for (int d = 0; d < kHot; d++) {
-        uint32_t c = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
-        for (int o = 32; o > 0; o >>= 1) c += (uint32_t) __shfl_xor((int) c, o);
-        if (lane == 0 && c) atomicAdd(&s_cnt[hot0 + d - lo], c);
+        for (int g = 0; g < kGroups; g++) {
+            const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
+            const uint16_t v[8] = {(uint16_t) q[g].x, (uint16_t) (q[g].x >> 16), (uint16_t) q[g].y,
+                                   (uint16_t) (q[g].y >> 16), (uint16_t) q[g].z, (uint16_t) (q[g].z >> 16),
+                                   (uint16_t) q[g].w, (uint16_t) (q[g].w >> 16)};
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (j + k >= end) break;
+                const uint32_t gi = j + k;
+                const uint32_t d = (uint32_t) v[k] - hot0;
+                if (d < (uint32_t) kHot) {
+                    const uint64_t one = 1ull << (8 * (d & 7));
+                    if (d < 8) hot_a += one; else hot_b += one;
+                    if (!((seen >> d) & 1u)) {
+                        seen |= 1u << d;
+                        const uint32_t w = v[k] - lo;
+                        if (s_first[w] > gi) atomicMin(&s_first[w], gi);
+                    }
+                } else if (v[k] == 0) {
+                    atomicAdd(&s_zero, 1u);
+                    if (s_zero_first > gi) atomicMin(&s_zero_first, gi);
+                } else {
+                    const uint32_t w = (uint32_t) v[k] - lo;
+                    if (w < (uint32_t) kWin) {
+                        atomicAdd(&s_cnt[w], 1u);
+                        if (s_first[w] > gi) atomicMin(&s_first[w], gi);
+                    } else {
+                        atomicAdd(&freq[v[k]], 1u);
+                        atomicMin(&first[v[k]], gi);
+                    }
+                }
+            }
+        }
+        // fold this chunk's register counters: wave reduction, one LDS add per wave
+#pragma unroll
+        for (int d = 0; d < kHot; d++) {
+            uint32_t cnt = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
+            for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t) __shfl_xor((int) cnt, o);
+            if (lane == 0 && cnt) atomicAdd(&s_cnt[hot0 + d - lo], cnt);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            chunk_zeros[c] = s_zero;
+            zeros_total += s_zero;
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
-        uint32_t c = s_cnt[i];
-        if (c) {
-            atomicAdd(&freq[lo + i], c);
-            atomicMin(&first[lo + i], base + s_first[i]);
+        const uint32_t cnt = s_cnt[i];
+        if (cnt) {
+            atomicAdd(&freq[lo + i], cnt);
+            atomicMin(&first[lo + i], s_first[i]);
         }
     }
-    if (threadIdx.x == 0) {
-        chunk_zeros[blockIdx.x] = s_zero;
-        if (s_zero) {
-            atomicAdd(&freq[0], s_zero);
-            atomicMin(&first[0], base + s_zero_first);
-        }
+    if (threadIdx.x == 0 && zeros_total) {
+        atomicAdd(&freq[0], zeros_total);
+        atomicMin(&first[0], s_zero_first);
     }
 }
 
@@ -459,8 +468,10 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     int hot0 = radius - kHot / 2;
     if (hot0 < lo) hot0 = lo;
     if (hot0 + kHot > lo + kWin) hot0 = lo + kWin - kHot;
+    // persistent workgroups: a few per compute unit
+    const uint32_t grid = nchunks < 1024 ? nchunks : 1024;
     if (nchunks)
-        hipLaunchKernelGGL(hist_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
+        hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
                            d_first, d_chunk_zeros, lo, hot0, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
@@ -647,161 +658,5 @@ void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unp
                            d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at);
 }
 
-// ---------------------------------------------------------------------------
-// Huffman encode: chunk bit totals, exclusive scan, then per-chunk packing in
-// LDS (MSB-first) flushed to the output words in big-endian byte order.  A
-// thread owns kEncPerThread consecutive codes, loaded with 16-byte loads.
-// ---------------------------------------------------------------------------
-static_assert(kEncPerThread == 16, "two 16-byte loads per thread");
-
-__device__ __forceinline__ void load16(const uint16_t *__restrict__ codes, uint32_t j, uint32_t end, bool aligned,
-                                       uint16_t *v) {
-    if (aligned && j + 16 <= end) {
-        const uint4 *q = reinterpret_cast<const uint4 *>(codes + j);
-        uint4 a = q[0], b = q[1];
-        uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            v[2 * k] = (uint16_t) w[k];
-            v[2 * k + 1] = (uint16_t) (w[k] >> 16);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 16; k++) v[k] = j + k < end ? codes[j + k] : 0;
-    }
-}
-
-__global__ void __launch_bounds__(kEncThreads) encode_sizes_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                  const uint8_t *__restrict__ len, int32_t offset,
-                                                                  uint64_t *__restrict__ chunk_bits, int aligned) {
-    const uint32_t base = blockIdx.x * kEncChunk;
-    const uint32_t end = min(base + kEncChunk, n);
-    const uint32_t my0 = base + threadIdx.x * kEncPerThread;
-    uint16_t v[16];
-    load16(codes, my0, end, aligned, v);
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++)
-        if (my0 + k < end) sum += len[(int32_t) v[k] - offset];
-    for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
-    __shared__ uint32_t s[kEncThreads / 64];
-    if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < kEncThreads / 64; w++) t += s[w];
-        chunk_bits[blockIdx.x] = t;
-    }
-}
-
-void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint8_t *d_len, int32_t offset,
-                         uint64_t *d_chunk_bits, hipStream_t st) {
-    uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
-    if (nchunks)
-        hipLaunchKernelGGL(encode_sizes_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_len,
-                           offset, d_chunk_bits, (int) (((uintptr_t) d_codes & 15) == 0));
-}
-
-__device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
-
-constexpr uint32_t kEncWords = kEncChunk * 64 / 32 + 2;  // worst case: 64-bit codes
-
-__global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                 const uint64_t *__restrict__ code,
-                                                                 const uint8_t *__restrict__ len, int32_t offset,
-                                                                 const uint64_t *__restrict__ chunk_off,
-                                                                 uint32_t *__restrict__ words, int aligned) {
-    __shared__ uint32_t s_words[kEncWords];
-    __shared__ uint32_t s_wave[kEncThreads / 64];
-    const uint32_t base = blockIdx.x * kEncChunk;
-    const uint32_t end = min(base + kEncChunk, n);
-    const uint32_t my0 = base + threadIdx.x * kEncPerThread;
-    uint16_t v[16];
-    load16(codes, my0, end, aligned, v);
-    uint32_t mybits = 0;
-#pragma unroll
-    for (int k = 0; k < 16; k++)
-        if (my0 + k < end) mybits += len[(int32_t) v[k] - offset];
-    // workgroup exclusive prefix of mybits
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    uint32_t incl = mybits;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = (uint32_t) __shfl_up((int) incl, o);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) s_wave[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (int w = 0; w < kEncThreads / 64; w++) {
-        if (w < wave) before += s_wave[w];
-        total += s_wave[w];
-    }
-    const uint64_t g0 = chunk_off[blockIdx.x];
-    const uint32_t head = (uint32_t) (g0 & 31);
-    for (uint32_t i = threadIdx.x; i < (head + total + 31) / 32 + 1; i += kEncThreads) s_words[i] = 0;
-    __syncthreads();
-    const uint32_t start = head + before + (incl - mybits);
-    // register accumulator: pending bits MSB-aligned in acc, nacc of them;
-    // the first output word may be shared with the previous thread and the
-    // last with the next one (atomicOr), words in between are ours alone.
-    uint32_t w = start >> 5;
-    uint64_t acc = 0;
-    uint32_t nacc = start & 31;
-    bool first_word = true;
-    auto flush = [&](uint32_t word) {
-        if (first_word && (start & 31))
-            atomicOr(&s_words[w], word);
-        else
-            s_words[w] = word;
-        first_word = false;
-        w++;
-    };
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-        if (my0 + k >= end) break;
-        const int st = (int32_t) v[k] - offset;
-        const uint32_t L = len[st];
-        if (!L) continue;
-        const uint64_t c = code[st];
-        if (nacc + L <= 64) {
-            acc |= c >> nacc;
-            nacc += L;
-        } else {
-            const uint32_t take = 64 - nacc;
-            acc |= c >> nacc;
-            flush((uint32_t) (acc >> 32));
-            flush((uint32_t) acc);
-            acc = c << take;
-            nacc = L - take;
-        }
-        while (nacc >= 32) {
-            flush((uint32_t) (acc >> 32));
-            acc <<= 32;
-            nacc -= 32;
-        }
-    }
-    if (nacc > (first_word ? (start & 31) : 0u)) atomicOr(&s_words[w], (uint32_t) (acc >> 32));
-    __syncthreads();
-    const uint32_t nw = (head + total + 31) / 32;
-    const uint64_t gw = g0 >> 5;
-    for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
-        uint32_t val = bswap32(s_words[i]);
-        bool edge = (i == 0 && head) || (i == nw - 1 && ((head + total) & 31));
-        if (edge)
-            atomicOr(&words[gw + i], val);
-        else
-            words[gw + i] = val;
-    }
-}
-
-void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_code, const uint8_t *d_len,
-                        int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st) {
-    uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
-    if (nchunks)
-        hipLaunchKernelGGL(encode_pack_kernel, dim3(nchunks), dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_code,
-                           d_len, offset, d_chunk_off, d_words, (int) (((uintptr_t) d_codes & 15) == 0));
-}
-
-void launch_bswap_words(uint32_t *, size_t, hipStream_t) {}
 
 }  // namespace faz
