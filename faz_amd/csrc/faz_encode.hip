@@ -5,8 +5,9 @@
 // MSB-first stream, which is the same byte sequence).
 //
 // Workgroups are persistent over chunks of kEncChunk codes, so the code table
-// is staged in LDS (sized to the table, at most kTabLds entries) once per
-// workgroup.  Every code is placed on its own: its
+// is staged in LDS once per workgroup: the window of at most kTabLds entries
+// that holds every symbol but 0 (the outlier code, far from the others, kept
+// in a register).  Tables whose symbols spread wider read global memory.  Every code is placed on its own: its
 // bit position is the chunk offset plus a workgroup prefix of lengths, and its
 // bits are OR-ed into at most three words of an LDS bit buffer, with no
 // sequential accumulator per thread.  Interior words then go out with plain
@@ -24,36 +25,60 @@ constexpr uint32_t kEncGridPerCu = 8;
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
 
-// 16 consecutive codes from position j (zero past end), 16-byte loads when
-// the codes are 16-byte aligned.
-__device__ __forceinline__ void load16(const uint16_t *__restrict__ codes, uint32_t j, uint32_t end, bool aligned,
-                                       uint16_t *v) {
+// 16 consecutive codes from position j as two 16-byte words (16-byte loads
+// when the codes are 16-byte aligned; zero past end).  Loaded one chunk
+// ahead of their use, so a workgroup's code reads overlap its packing.
+struct Codes16 {
+    uint4 a, b;
+    __device__ __forceinline__ uint16_t operator[](int k) const {
+        const uint4 &w = k < 8 ? a : b;
+        const uint32_t word = (k & 6) == 0 ? w.x : (k & 6) == 2 ? w.y : (k & 6) == 4 ? w.z : w.w;
+        return (uint16_t) (k & 1 ? word >> 16 : word);
+    }
+};
+
+__device__ __forceinline__ Codes16 fetch16(const uint16_t *__restrict__ codes, uint32_t j, uint32_t end,
+                                           bool aligned) {
+    Codes16 r;
     if (aligned && j + 16 <= end) {
         const uint4 *q = reinterpret_cast<const uint4 *>(codes + j);
-        const uint4 a = q[0], b = q[1];
-        const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            v[2 * k] = (uint16_t) w[k];
-            v[2 * k + 1] = (uint16_t) (w[k] >> 16);
-        }
+        r.a = q[0];
+        r.b = q[1];
     } else {
+        uint32_t t[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++) v[k] = j + k < end ? codes[j + k] : 0;
+        for (int k = 0; k < 16; k++) t[k] = j + k < end ? codes[j + k] : 0;
+        r.a = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
+        r.b = make_uint4(t[8] | t[9] << 16, t[10] | t[11] << 16, t[12] | t[13] << 16, t[14] | t[15] << 16);
     }
+    return r;
+}
+
+__device__ __forceinline__ Codes16 fetch_chunk(const uint16_t *__restrict__ codes, uint32_t c, uint32_t n,
+                                               bool aligned) {
+    const uint32_t base = c * kEncChunk;
+    return fetch16(codes, base + threadIdx.x * kEncPerThread, min(base + kEncChunk, n), aligned);
+}
+
+// Table entries [win, win + nwin) live in LDS; e0 is the entry of symbol 0.
+struct TabWindow {
+    uint32_t win, nwin;
+    uint64_t e0;
+};
+
+template<bool LDS_TAB>
+__device__ __forceinline__ uint64_t entry(const uint64_t *s_tab, const uint64_t *__restrict__ tab, const TabWindow &w,
+                                          uint16_t sym, int32_t offset) {
+    if (!LDS_TAB) return tab[(int32_t) sym - offset];
+    const uint64_t e = s_tab[min((uint32_t) ((int32_t) sym - offset) - w.win, w.nwin - 1)];
+    return sym == 0 ? w.e0 : e;
 }
 
 template<bool LDS_TAB>
-__device__ __forceinline__ uint64_t entry(const uint64_t *s_tab, const uint64_t *__restrict__ tab, int32_t i) {
-    return LDS_TAB ? s_tab[i] : tab[i];
-}
-
-template<bool LDS_TAB>
-__device__ __forceinline__ void stage_table(uint64_t *s_tab, const uint64_t *__restrict__ tab, uint32_t n_states) {
-    if (LDS_TAB) {
-        for (uint32_t i = threadIdx.x; i < n_states; i += kEncThreads) s_tab[i] = tab[i];
-        __syncthreads();
-    }
+__device__ __forceinline__ void stage_table(uint64_t *s_tab, const uint64_t *__restrict__ tab, const TabWindow &w) {
+    if (!LDS_TAB) return;
+    for (uint32_t i = threadIdx.x; i < w.nwin; i += kEncThreads) s_tab[i] = tab[w.win + i];
+    __syncthreads();
 }
 
 // Workgroup exclusive prefix of x; *total gets the sum.
@@ -78,35 +103,26 @@ __device__ __forceinline__ uint32_t wg_exclusive(uint32_t x, uint32_t *s_wave, u
     return before + incl - x;
 }
 
-template<bool LDS_TAB>
-__global__ void __launch_bounds__(kEncThreads) encode_sizes_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                  const uint64_t *__restrict__ tab,
-                                                                  uint32_t n_states, int32_t offset,
-                                                                  uint64_t *__restrict__ chunk_bits, int aligned) {
-    extern __shared__ uint64_t s_tab[];
-    __shared__ uint32_t s_red[kEncThreads / 64];
-    stage_table<LDS_TAB>(s_tab, tab, n_states);
-    const uint32_t nchunks = (n + kEncChunk - 1) / kEncChunk;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const uint32_t base = c * kEncChunk, end = min(base + kEncChunk, n);
-        const uint32_t my0 = base + threadIdx.x * kEncPerThread;
-        uint16_t v[16];
-        load16(codes, my0, end, aligned != 0, v);
-        uint32_t sum = 0;
+// Bit total of chunk c (codes v of this thread; unless FULL, valid below end).
+template<bool LDS_TAB, bool FULL>
+__device__ __forceinline__ void sizes_chunk(uint32_t c, const Codes16 &v, uint32_t end, const uint64_t *s_tab,
+                                            const uint64_t *__restrict__ tab, const TabWindow &tw, int32_t offset,
+                                            uint32_t *s_red, uint64_t *__restrict__ chunk_bits) {
+    const uint32_t my0 = c * kEncChunk + threadIdx.x * kEncPerThread;
+    uint32_t sum = 0;
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-            if (my0 + k < end) sum += (uint32_t) entry<LDS_TAB>(s_tab, tab, (int32_t) v[k] - offset) & 0xFF;
+    for (int k = 0; k < 16; k++)
+        if (FULL || my0 + k < end) sum += (uint32_t) entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) & 0xFF;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
-        if (threadIdx.x % 64 == 0) s_red[threadIdx.x / 64] = sum;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t t = 0;
-            for (uint32_t w = 0; w < kEncThreads / 64; w++) t += s_red[w];
-            chunk_bits[c] = t;
-        }
-        __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
+    if (threadIdx.x % 64 == 0) s_red[threadIdx.x / 64] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (uint32_t w = 0; w < kEncThreads / 64; w++) t += s_red[w];
+        chunk_bits[c] = t;
     }
+    __syncthreads();
 }
 
 // OR a code of L bits (right-aligned in c, 1 <= L <= 56) into the MSB-first
@@ -121,67 +137,123 @@ __device__ __forceinline__ void put_code(uint64_t c, uint32_t L, uint32_t pos, W
     if (sh + L > 64) word_or(w + 2, (uint32_t) (hi << (32 - sh)));
 }
 
+// Bits of chunk c, which starts at bit g0 of the stream.
+template<bool LDS_TAB, bool FULL>
+__device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_t end, uint64_t g0,
+                                           const uint64_t *s_tab, const uint64_t *__restrict__ tab,
+                                           const TabWindow &tw, int32_t offset, uint32_t *s_words, uint32_t *s_wave,
+                                           uint32_t *__restrict__ words) {
+    const uint32_t my0 = c * kEncChunk + threadIdx.x * kEncPerThread;
+    uint64_t e[16];
+    uint32_t mybits = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        e[k] = FULL || my0 + k < end ? entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) : 0;
+        mybits += (uint32_t) e[k] & 0xFF;
+    }
+    uint32_t total;
+    const uint32_t before = wg_exclusive(mybits, s_wave, &total);
+    const uint32_t head = (uint32_t) (g0 & 31);
+    const uint32_t nw = (head + total + 31) / 32;
+    const uint64_t gw = g0 >> 5;
+    uint32_t pos = head + before;
+    if (nw <= kPackWords) {
+        for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t L = (uint32_t) e[k] & 0xFF;
+            if (L) put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&s_words[w], bits); });
+            pos += L;
+        }
+        __syncthreads();
+        const bool open_tail = ((head + total) & 31) != 0;
+        for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
+            const uint32_t val = bswap32(s_words[i]);
+            if ((i == 0 && head) || (i == nw - 1 && open_tail))
+                atomicOr(&words[gw + i], val);
+            else
+                words[gw + i] = val;
+        }
+    } else {
+        // long codewords: straight into the zeroed output
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t L = (uint32_t) e[k] & 0xFF;
+            if (L)
+                put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&words[gw + w], bswap32(bits)); });
+            pos += L;
+        }
+    }
+    __syncthreads();
+}
+
+// Full chunks with 16-byte aligned codes: the next chunk's codes (and, for
+// the packer, its bit offset) are in flight while the current one is
+// processed, with a fixed number of loads per step so the wait before use
+// covers the current chunk only.  The last, partial chunk and unaligned
+// codes take element loads.
+__device__ __forceinline__ Codes16 fetch_full(const uint16_t *__restrict__ codes, uint32_t c) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(codes + (size_t) c * kEncChunk + threadIdx.x * kEncPerThread);
+    Codes16 r;
+    r.a = q[0];
+    r.b = q[1];
+    return r;
+}
+
+template<bool LDS_TAB>
+__global__ void __launch_bounds__(kEncThreads) encode_sizes_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                  const uint64_t *__restrict__ tab, TabWindow tw,
+                                                                  int32_t offset, uint64_t *__restrict__ chunk_bits,
+                                                                  int aligned) {
+    extern __shared__ uint64_t s_tab[];
+    __shared__ uint32_t s_red[kEncThreads / 64];
+    stage_table<LDS_TAB>(s_tab, tab, tw);
+    const uint32_t nchunks = (n + kEncChunk - 1) / kEncChunk;
+    const uint32_t nfull = aligned ? n / kEncChunk : 0;
+    uint32_t c = blockIdx.x;
+    if (c < nfull) {
+        Codes16 v = fetch_full(codes, c);
+        for (; c < nfull; c += gridDim.x) {
+            // the next full chunk (the last one again at the end: unused)
+            const Codes16 next = fetch_full(codes, min(c + gridDim.x, nfull - 1));
+            sizes_chunk<LDS_TAB, true>(c, v, 0, s_tab, tab, tw, offset, s_red, chunk_bits);
+            v = next;
+        }
+    }
+    for (; c < nchunks; c += gridDim.x)
+        sizes_chunk<LDS_TAB, false>(c, fetch_chunk(codes, c, n, aligned != 0), min(c * kEncChunk + kEncChunk, n),
+                                    s_tab, tab, tw, offset, s_red, chunk_bits);
+}
+
 template<bool LDS_TAB>
 __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                 const uint64_t *__restrict__ tab,
-                                                                 uint32_t n_states, int32_t offset,
-                                                                 const uint64_t *__restrict__ chunk_off,
+                                                                 const uint64_t *__restrict__ tab, TabWindow tw,
+                                                                 int32_t offset, const uint64_t *__restrict__ chunk_off,
                                                                  uint32_t *__restrict__ words, int aligned) {
     extern __shared__ uint64_t s_tab[];
     __shared__ uint32_t s_words[kPackWords];
     __shared__ uint32_t s_wave[kEncThreads / 64];
-    stage_table<LDS_TAB>(s_tab, tab, n_states);
+    stage_table<LDS_TAB>(s_tab, tab, tw);
     const uint32_t nchunks = (n + kEncChunk - 1) / kEncChunk;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const uint32_t base = c * kEncChunk, end = min(base + kEncChunk, n);
-        const uint32_t my0 = base + threadIdx.x * kEncPerThread;
-        uint16_t v[16];
-        load16(codes, my0, end, aligned != 0, v);
-        uint64_t e[16];
-        uint32_t mybits = 0;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            e[k] = my0 + k < end ? entry<LDS_TAB>(s_tab, tab, (int32_t) v[k] - offset) : 0;
-            mybits += (uint32_t) e[k] & 0xFF;
+    const uint32_t nfull = aligned ? n / kEncChunk : 0;
+    uint32_t c = blockIdx.x;
+    if (c < nfull) {
+        Codes16 v = fetch_full(codes, c);
+        uint64_t g0 = chunk_off[c];
+        for (; c < nfull; c += gridDim.x) {
+            // the next full chunk (the last one again at the end: unused)
+            const uint32_t cn = min(c + gridDim.x, nfull - 1);
+            const Codes16 next = fetch_full(codes, cn);
+            const uint64_t g0n = chunk_off[cn];
+            pack_chunk<LDS_TAB, true>(c, v, 0, g0, s_tab, tab, tw, offset, s_words, s_wave, words);
+            v = next;
+            g0 = g0n;
         }
-        uint32_t total;
-        const uint32_t before = wg_exclusive(mybits, s_wave, &total);
-        const uint64_t g0 = chunk_off[c];
-        const uint32_t head = (uint32_t) (g0 & 31);
-        const uint32_t nw = (head + total + 31) / 32;
-        const uint64_t gw = g0 >> 5;
-        uint32_t pos = head + before;
-        if (nw <= kPackWords) {
-            for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const uint32_t L = (uint32_t) e[k] & 0xFF;
-                if (L) put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&s_words[w], bits); });
-                pos += L;
-            }
-            __syncthreads();
-            const bool open_tail = ((head + total) & 31) != 0;
-            for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
-                const uint32_t val = bswap32(s_words[i]);
-                if ((i == 0 && head) || (i == nw - 1 && open_tail))
-                    atomicOr(&words[gw + i], val);
-                else
-                    words[gw + i] = val;
-            }
-        } else {
-            // long codewords: straight into the zeroed output
-#pragma unroll
-            for (int k = 0; k < 16; k++) {
-                const uint32_t L = (uint32_t) e[k] & 0xFF;
-                if (L)
-                    put_code(e[k] >> 8, L, pos,
-                             [&](uint32_t w, uint32_t bits) { atomicOr(&words[gw + w], bswap32(bits)); });
-                pos += L;
-            }
-        }
-        __syncthreads();
     }
+    for (; c < nchunks; c += gridDim.x)
+        pack_chunk<LDS_TAB, false>(c, fetch_chunk(codes, c, n, aligned != 0), min(c * kEncChunk + kEncChunk, n),
+                                   chunk_off[c], s_tab, tab, tw, offset, s_words, s_wave, words);
 }
 
 uint32_t encode_grid(uint32_t nchunks) {
@@ -199,32 +271,34 @@ uint32_t encode_grid(uint32_t nchunks) {
 
 }  // namespace
 
-void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, uint32_t n_states, int32_t offset,
-                         uint64_t *d_chunk_bits, hipStream_t st) {
+void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
+                         int32_t offset, uint64_t *d_chunk_bits, hipStream_t st) {
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
-    const dim3 grid(encode_grid(nchunks)), block(kEncThreads);
-    if (n_states <= kTabLds)
-        hipLaunchKernelGGL(encode_sizes_kernel<true>, grid, block, n_states * 8, st, d_codes, (uint32_t) n, d_tab,
-                           n_states, offset, d_chunk_bits, aligned);
+    const TabWindow tw{w.win, w.nwin, w.e0};
+    if (w.nwin && w.nwin <= kTabLds)
+        hipLaunchKernelGGL(encode_sizes_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
+                           d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_bits, aligned);
     else
-        hipLaunchKernelGGL(encode_sizes_kernel<false>, grid, block, 0, st, d_codes, (uint32_t) n, d_tab, n_states,
-                           offset, d_chunk_bits, aligned);
+        hipLaunchKernelGGL(encode_sizes_kernel<false>, dim3(encode_grid(nchunks)), dim3(kEncThreads), 0, st, d_codes,
+                           (uint32_t) n, d_tab, tw, offset, d_chunk_bits, aligned);
 }
 
-void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, uint32_t n_states, int32_t offset,
-                        const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st) {
+void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
+                        int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st) {
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
-    const dim3 grid(encode_grid(nchunks)), block(kEncThreads);
-    if (n_states <= kTabLds)
-        hipLaunchKernelGGL(encode_pack_kernel<true>, grid, block, n_states * 8, st, d_codes, (uint32_t) n, d_tab,
-                           n_states, offset, d_chunk_off, d_words, aligned);
+    const TabWindow tw{w.win, w.nwin, w.e0};
+    if (w.nwin && w.nwin <= kTabLds)
+        hipLaunchKernelGGL(encode_pack_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
+                           d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_off, d_words, aligned);
     else
-        hipLaunchKernelGGL(encode_pack_kernel<false>, grid, block, 0, st, d_codes, (uint32_t) n, d_tab, n_states,
-                           offset, d_chunk_off, d_words, aligned);
+        hipLaunchKernelGGL(encode_pack_kernel<false>, dim3(encode_grid(nchunks)), dim3(kEncThreads), 0, st, d_codes,
+                           (uint32_t) n, d_tab, tw, offset, d_chunk_off, d_words, aligned);
 }
+
+uint32_t encode_lds_entries() { return kTabLds; }
 
 }  // namespace faz

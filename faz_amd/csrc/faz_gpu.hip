@@ -752,17 +752,28 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     // one packed entry per symbol: codeword right-aligned above an 8-bit length
     ctx->h_tab.resize((size_t) t->n_states);
+    int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
     for (int32_t i = 0; i < t->n_states; i++) {
         const uint32_t L = t->len[i];
         if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
         ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
+        if (L && i + t->offset != 0) {
+            if (lo < 0) lo = i;
+            hi = i;
+        }
+    }
+    EncodeWindow ew{0, 0, 0};
+    if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
+        ew.win = (uint32_t) lo;
+        ew.nwin = (uint32_t) (hi - lo + 1);
+        ew.e0 = (t->offset <= 0 && -t->offset < t->n_states) ? ctx->h_tab[-t->offset] : 0;
     }
     HIPCHECK(ctx->enc_code.ensure((size_t) t->n_states));
     HIPCHECK(ctx->chunk_bits.ensure(nchunks));
     HIPCHECK(ctx->chunk_off.ensure(nchunks));
     HIPCHECK(ctx->scalars.ensure(4));
     HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, ctx->h_tab.data(), (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
-    launch_encode_sizes(d_codes, n, ctx->enc_code.p, (uint32_t) t->n_states, t->offset, ctx->chunk_bits.p, st);
+    launch_encode_sizes(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_bits.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
     HIPCHECK(hipGetLastError());
@@ -776,7 +787,7 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     uint32_t *w = reinterpret_cast<uint32_t *>(d_bits);
     if (((uintptr_t) d_bits & 3) != 0) CTX_FAIL(FAZ_ERR_INVALID, "bit buffer must be 4-byte aligned");
     launch_fill_u32(w, words, 0u, st);
-    launch_encode_pack(d_codes, n, ctx->enc_code.p, (uint32_t) t->n_states, t->offset, ctx->chunk_off.p, w, st);
+    launch_encode_pack(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_off.p, w, st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
     return FAZ_OK;

@@ -109,14 +109,22 @@ void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outv
                           const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st);
 // Huffman encode (faz_encode.hip).  tab holds one entry per symbol - offset:
 // the codeword right-aligned in bits 8..63 and its length (<= 56) in bits 0..7.
-void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, uint32_t n_states, int32_t offset,
-                         uint64_t *d_chunk_bits, hipStream_t st);
+// When every present symbol but 0 lies in table indices [win, win + nwin)
+// with nwin <= encode_lds_entries(), that window is staged in LDS and e0 is
+// symbol 0's entry; nwin = 0 reads the whole table from global memory.
+struct EncodeWindow {
+    uint32_t win, nwin;
+    uint64_t e0;
+};
+uint32_t encode_lds_entries();
+void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
+                         int32_t offset, uint64_t *d_chunk_bits, hipStream_t st);
 void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
                      hipStream_t st);
 // tiles of the two-level scans (scratch of this many uint64 per scan)
 uint32_t scan_tiles(uint32_t count);
-void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, uint32_t n_states, int32_t offset,
-                        const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st);
+void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
+                        int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st);
 void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st);
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
                            const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st);

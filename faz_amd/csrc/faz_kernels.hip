@@ -572,90 +572,112 @@ uint32_t scan_tiles(uint32_t count) { return (count + kScanTile - 1) / kScanTile
 // ---------------------------------------------------------------------------
 // Outlier compaction: the unpredictable list is the values at code-0
 // positions in code order.  Only chunks that hold outliers do any work;
-// ranks come from wavefront ballots and a workgroup prefix.
+// ranks come from a workgroup prefix of per-thread counts.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kHistThreads) unpred_gather_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                    const float *__restrict__ outval,
-                                                                    const uint32_t *__restrict__ chunk_zeros,
-                                                                    const uint64_t *__restrict__ chunk_base,
-                                                                    float *__restrict__ unpred) {
+// 64 consecutive codes from start as eight 16-byte words (past the end: 1,
+// which is never an outlier).
+__device__ __forceinline__ void load64(const uint16_t *__restrict__ codes, uint32_t start, uint32_t end, bool aligned,
+                                       uint4 *q) {
+    if (aligned && start + 64 <= end) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(codes + start);
+#pragma unroll
+        for (int g = 0; g < 8; g++) q[g] = p[g];
+    } else {
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+            uint32_t t[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t j = start + 8 * g + k;
+                t[k] = j < end ? codes[j] : 1u;  // past the end: never an outlier
+            }
+            q[g] = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
+        }
+    }
+}
+
+__device__ __forceinline__ uint16_t code_at(const uint4 *q, int k) {
+    const uint4 &w = q[k >> 3];
+    const uint32_t word = (k & 6) == 0 ? w.x : (k & 6) == 2 ? w.y : (k & 6) == 4 ? w.z : w.w;
+    return (uint16_t) (k & 1 ? word >> 16 : word);
+}
+
+// Workgroup exclusive prefix (kHistThreads threads).
+__device__ __forceinline__ uint32_t chunk_exclusive(uint32_t x, uint32_t *s_wave) {
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    uint32_t incl = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = (uint32_t) __shfl_up((int) incl, o);
+        if (lane >= (uint32_t) o) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kHistThreads / 64; w++) before += w < wave ? s_wave[w] : 0;
+    return before + incl - x;
+}
+
+// The outliers of a chunk in code order: a thread takes 64 consecutive codes
+// (kHistPerThread), its rank comes from a workgroup prefix of per-thread
+// outlier counts.  Only chunks that hold outliers do any work.
+template<bool GATHER>
+__global__ void __launch_bounds__(kHistThreads) unpred_move_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                  const float *__restrict__ src,
+                                                                  const uint32_t *__restrict__ chunk_zeros,
+                                                                  const uint64_t *__restrict__ chunk_base,
+                                                                  float *__restrict__ dst, int aligned) {
+    static_assert(kHistPerThread == 64, "one load64 per thread and chunk");
     __shared__ uint32_t s_wave[kHistThreads / 64];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    if (chunk_zeros[c] == 0) continue;
-    const uint32_t base = c * kHistChunk;
-    const uint32_t end = min(base + kHistChunk, n);
-    uint64_t run = chunk_base[c];
-    for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
-        uint32_t j = j0 + threadIdx.x;
-        bool z = j < end && codes[j] == 0;
-        uint64_t m = __ballot(z);
-        uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wave[wave] = (uint32_t) __popcll(m);
-        __syncthreads();
-        uint32_t before = 0, tot = 0;
-        for (int w = 0; w < kHistThreads / 64; w++) {
-            if (w < wave) before += s_wave[w];
-            tot += s_wave[w];
+        if (chunk_zeros[c] == 0) continue;
+        const uint32_t base = c * kHistChunk;
+        const uint32_t end = min(base + kHistChunk, n);
+        const uint32_t my0 = base + threadIdx.x * kHistPerThread;
+        uint4 q[8];
+        load64(codes, my0, end, aligned != 0, q);
+        uint32_t z = 0;
+#pragma unroll
+        for (int k = 0; k < 64; k++) z += code_at(q, k) == 0;
+        uint64_t run = chunk_base[c] + chunk_exclusive(z, s_wave);
+        if (z) {
+#pragma unroll
+            for (int k = 0; k < 64; k++) {
+                if (code_at(q, k) == 0) {
+                    // gather: the list in code order; scatter: back to code positions
+                    if (GATHER)
+                        dst[run] = src[my0 + k];
+                    else
+                        dst[my0 + k] = src[run];
+                    run++;
+                }
+            }
         }
-        if (z) unpred[run + before + below] = outval[j];
-        run += tot;
         __syncthreads();
-    }
     }
 }
 
 void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
                           const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st) {
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
-    if (nchunks > 1024) nchunks = 1024;
+    if (nchunks > 2048) nchunks = 2048;
     if (nchunks)
-        hipLaunchKernelGGL(unpred_gather_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
-                           d_outval, d_chunk_zeros, d_chunk_base, d_unpred);
+        hipLaunchKernelGGL(unpred_move_kernel<true>, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
+                           d_outval, d_chunk_zeros, d_chunk_base, d_unpred, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
-// Inverse of the gather for decompression: unpredictable value of every
-// code-0 position, indexed by code position.
-__global__ void __launch_bounds__(kHistThreads) unpred_scatter_kernel(const uint16_t *__restrict__ codes, uint32_t n,
-                                                                     const float *__restrict__ unpred,
-                                                                     const uint32_t *__restrict__ chunk_zeros,
-                                                                     const uint64_t *__restrict__ chunk_base,
-                                                                     float *__restrict__ unpred_at) {
-    __shared__ uint32_t s_wave[kHistThreads / 64];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    if (chunk_zeros[c] == 0) continue;
-    const uint32_t base = c * kHistChunk;
-    const uint32_t end = min(base + kHistChunk, n);
-    uint64_t run = chunk_base[c];
-    for (uint32_t j0 = base; j0 < end; j0 += kHistThreads) {
-        uint32_t j = j0 + threadIdx.x;
-        bool z = j < end && codes[j] == 0;
-        uint64_t m = __ballot(z);
-        uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wave[wave] = (uint32_t) __popcll(m);
-        __syncthreads();
-        uint32_t before = 0, tot = 0;
-        for (int w = 0; w < kHistThreads / 64; w++) {
-            if (w < wave) before += s_wave[w];
-            tot += s_wave[w];
-        }
-        if (z) unpred_at[j] = unpred[run + before + below];
-        run += tot;
-        __syncthreads();
-    }
-    }
-}
-
+// The inverse for decompression: unpredictable value of every code-0
+// position, indexed by code position (unpred_move_kernel<false>).
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
                            const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st) {
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
-    if (nchunks > 1024) nchunks = 1024;
+    if (nchunks > 2048) nchunks = 2048;
     if (nchunks)
-        hipLaunchKernelGGL(unpred_scatter_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
-                           d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at);
+        hipLaunchKernelGGL(unpred_move_kernel<false>, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes,
+                           (uint32_t) n, d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at,
+                           (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
 
