@@ -74,6 +74,18 @@ def make_field(shape, seed=0, fronts=16, slope=11.0 / 3.0, noise=1e-4, width=4.0
     return out
 
 
+def measured_traffic(kernel, dims):
+    """HBM bytes per launch of `kernel` on this workload, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/ (None when
+    no summary matches)."""
+    import re
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, "x".join(map(str, dims))))
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(x, dims, rel):
     """Reference CPU compressor (oracle/_ref, built from the reference sources)
     on this host, one thread; falls back to the oracle port when absent."""
@@ -114,6 +126,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one device copy of the field (4n bytes read, 4n written) before the steps, the known-size "
+                         "kernel that calibrates FETCH_SIZE / WRITE_SIZE in rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the compression of one field on one MI355X; "
@@ -126,6 +141,11 @@ def main():
     dims = (args.n, args.n, args.n)
     n = args.n ** 3
     x = make_field(dims, seed=args.seed)
+    if args.copy_probe:
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del y
     eng = faz_amd.Engine(0)
     p = faz_amd.make_params(dims, 0.0, algo=1, max_step=32)
 
@@ -180,7 +200,7 @@ def main():
                                "interp+quantize+Huffman on GPU (ALGO_INTERP drop-in)" % args.rel,
                    "dims": list(dims), "rel_eb": args.rel, "interp": "cubic", "max_step": 32},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, dims),
                      "kernel": kname, "launches_per_step": launches // max(args.steps, 1),
                      "avg_launch_ms": round(avg_launch_ms, 5),
                      "bytes_per_launch": round(prof_bytes / max(launches, 1)),
