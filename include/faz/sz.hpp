@@ -8,8 +8,8 @@
 //   * errors throw faz::Error instead of printing and calling exit(0);
 //   * SZ_decompress does not write the reference's debug file
 //     "waved.qoz.dec.final2" (sz.hpp:198);
-//   * the GPU engine serves ALGO_INTERP and ALGO_LORENZO_REG streams in float;
-//     other algorithms and types throw faz::Error (FAZ_ERR_UNSUPPORTED).
+//   * the GPU engine serves ALGO_INTERP streams in float; other algorithms
+//     and types throw faz::Error (FAZ_ERR_UNSUPPORTED).
 #pragma once
 
 #include <stdexcept>
