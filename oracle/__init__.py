@@ -57,6 +57,36 @@ def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0
     return p
 
 
+class LRParams(ctypes.Structure):
+    """Mirror of oracle_lr_params (faz_oracle.h)."""
+
+    _fields_ = [
+        ("dims", ctypes.c_size_t * 3),
+        ("eb", ctypes.c_double),
+        ("block_size", ctypes.c_int),
+        ("pred_dim", ctypes.c_int),
+        ("lorenzo", ctypes.c_int),
+        ("lorenzo2", ctypes.c_int),
+        ("regression", ctypes.c_int),
+        ("radius", ctypes.c_int),
+    ]
+
+
+def make_lr_params(dims, eb, block_size=6, pred_dim=3, lorenzo=True, lorenzo2=False,
+                   regression=True, radius=32768):
+    p = LRParams()
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    p.block_size = int(block_size)
+    p.pred_dim = int(pred_dim)
+    p.lorenzo = int(bool(lorenzo))
+    p.lorenzo2 = int(bool(lorenzo2))
+    p.regression = int(bool(regression))
+    p.radius = int(radius)
+    return p
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE, "all"])
 
@@ -91,6 +121,12 @@ def lib():
         L.oracle_range.argtypes = [ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_float),
                                    P(ctypes.c_float)]
         L.oracle_interp_seq_index.argtypes = [P(InterpParams), ctypes.c_void_p]
+        vp, sz = ctypes.c_void_p, P(ctypes.c_size_t)
+        L.oracle_lr_compress.argtypes = [P(LRParams), vp, vp, vp, vp, sz, vp, vp, vp, sz, vp, sz, vp, sz]
+        L.oracle_lr_raw.argtypes = [P(LRParams), vp, vp, ctypes.c_size_t, sz]
+        L.oracle_lr_decompress_raw.argtypes = [vp, ctypes.c_size_t, vp, vp]
+        L.oracle_lr_compress_stream.argtypes = [P(LRParams), ctypes.c_int, ctypes.c_double, vp, vp,
+                                                ctypes.c_size_t, sz]
         _lib = L
     return _lib
 
@@ -190,3 +226,63 @@ def seq_index(params):
     seq = np.zeros(num, np.int64)
     lib().oracle_interp_seq_index(ctypes.byref(params), _ptr(seq))
     return seq
+
+
+def lr_compress(data, params):
+    """3-D Lorenzo/regression engine. Returns dict(codes, recon, indicator,
+    raw_coeffs, reg_codes, reg_coeffs, reg_unpred, unpred)."""
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    n = data.size
+    bs = params.block_size
+    nb = 1
+    for i in range(3):
+        nb *= (params.dims[i] - 1) // bs + 1
+    codes = np.zeros(n, np.int32)
+    recon = np.zeros(n, np.float32)
+    ind = np.zeros(nb, np.int32)
+    raw_c = np.zeros(4 * nb, np.float32)
+    reg_codes = np.zeros(4 * nb, np.int32)
+    reg_coeffs = np.zeros(4 * nb, np.float32)
+    reg_unpred = np.zeros(4 * nb, np.float32)
+    unpred = np.zeros(n, np.float32)
+    nbk, nreg, nru, nu = (ctypes.c_size_t() for _ in range(4))
+    rc = lib().oracle_lr_compress(ctypes.byref(params), _ptr(data), _ptr(codes), _ptr(recon), _ptr(ind),
+                                  ctypes.byref(nbk), _ptr(raw_c), _ptr(reg_codes), _ptr(reg_coeffs),
+                                  ctypes.byref(nreg), _ptr(reg_unpred), ctypes.byref(nru), _ptr(unpred),
+                                  ctypes.byref(nu))
+    assert rc == 0, rc
+    assert nbk.value == nb
+    r = nreg.value
+    return dict(codes=codes, recon=recon, indicator=ind, raw_coeffs=raw_c.reshape(-1, 4),
+                reg_codes=reg_codes[:4 * r].copy(), reg_coeffs=reg_coeffs[:4 * r].reshape(-1, 4).copy(),
+                reg_unpred=reg_unpred[:nru.value].copy(), unpred=unpred[:nu.value].copy())
+
+
+def lr_raw(data, params):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().oracle_lr_raw(ctypes.byref(params), _ptr(data), _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def lr_decompress_raw(raw, dims):
+    buf = np.frombuffer(raw, np.uint8).copy()
+    d = np.array(dims, np.uint64)
+    out = np.zeros(int(np.prod(dims)), np.float32)
+    rc = lib().oracle_lr_decompress_raw(_ptr(buf), buf.size, _ptr(d), _ptr(out))
+    assert rc == 0, rc
+    return out
+
+
+def lr_compress_stream(data, params, eb, rel=True):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = lib().oracle_lr_compress_stream(ctypes.byref(params), 1 if rel else 0, float(eb), _ptr(data),
+                                         _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()

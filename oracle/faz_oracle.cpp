@@ -858,23 +858,32 @@ static bool interp_decompress_raw(const uint8_t *raw, size_t n, int N, float *ou
     return true;
 }
 
-// QoZ::Config trailer for an ALGO_INTERP stream (Config.hpp:240-309)
-static void config_save(std::vector<uint8_t> &out, const oracle_interp_params *p, double absEB, double relEB) {
+// Config fields of the trailer that differ between the two stream kinds.
+struct TrailerFields {
+    uint8_t cmprAlgo = 2;  // ALGO_INTERP (Config.hpp:24-25: LORENZO_REG=0, INTERP_LORENZO=1, INTERP=2)
+    bool lorenzo = true, lorenzo2 = false, regression = true;
+    int blockSize = -1;    // -1: the Config constructor default for N
+    int pred_dim = -1;     // -1: N
+};
+
+// QoZ::Config trailer (Config.hpp:240-309)
+static void config_save(std::vector<uint8_t> &out, const oracle_interp_params *p, double absEB, double relEB,
+                        const TrailerFields &tf = TrailerFields()) {
     size_t num = 1;
     for (int i = 0; i < p->N; i++) num *= p->dims[i];
     put(out, (char) p->N);
     for (int i = 0; i < p->N; i++) put(out, (size_t) p->dims[i]);
     put(out, num);
-    put(out, (uint8_t) 2);  // cmprAlgo = ALGO_INTERP
+    put(out, tf.cmprAlgo);
     put(out, (uint8_t) 0);  // errorBoundMode = EB_ABS (after calAbsErrorBound)
     put(out, (uint8_t) 0);  // tuningTarget = RD
     put(out, absEB);
     put(out, relEB);
     put(out, p->alpha);
     put(out, p->beta);
-    put(out, (bool) true);   // lorenzo
-    put(out, (bool) false);  // lorenzo2
-    put(out, (bool) true);   // regression
+    put(out, tf.lorenzo);
+    put(out, tf.lorenzo2);
+    put(out, tf.regression);
     put(out, (bool) false);  // regression2
     put(out, (uint8_t) p->interp_algo);
     put(out, (uint8_t) p->direction);
@@ -882,13 +891,14 @@ static void config_save(std::vector<uint8_t> &out, const oracle_interp_params *p
     put(out, (uint8_t) 1);  // lossless = zstd
     put(out, (uint8_t) 1);  // encoder = Huffman
     put(out, (int) (p->radius * 2));
-    int blockSize = p->N == 1 ? 128 : (p->N == 2 ? 16 : 6);
+    int blockSize = tf.blockSize >= 0 ? tf.blockSize : (p->N == 1 ? 128 : (p->N == 2 ? 16 : 6));
+    int stride = p->N == 1 ? 128 : (p->N == 2 ? 16 : 6);  // set by setDims only
     put(out, blockSize);
     put(out, (int) p->levelwise);
     put(out, (bool) false);  // blockwiseTuning
-    put(out, blockSize);     // stride
+    put(out, stride);
     put(out, (size_t) p->max_step);
-    put(out, (int) p->N);    // pred_dim
+    put(out, (int) (tf.pred_dim >= 0 ? tf.pred_dim : p->N));
     put(out, (bool) false);  // openmp
     put(out, (int) 0);       // fixBlockSize
     put(out, (int) 0);       // blockwiseSampleBlockSize
@@ -906,6 +916,364 @@ static void config_save(std::vector<uint8_t> &out, const oracle_interp_params *p
     put(out, (bool) true);   // pyBind
     put(out, (size_t) 0);    // metadata size
     put(out, (int) 0);       // conditioning
+}
+
+// ---------------------------------------------------------------------------
+// 3-D Lorenzo/regression engine: SZFastFrontend (include/QoZ/frontend/
+// SZFastFrontend.hpp:176-345, 429-530) with the predictors of
+// MetaLorenzoPredictor.hpp and MetaRegressionPredictor.hpp and the constants
+// of MetaDef.hpp:16-31, composed by SZGeneralCompressor (SZGeneralCompressor.hpp:30-71).
+// ---------------------------------------------------------------------------
+enum { SEL_LORENZO = 0, SEL_REGRESSION = 1, SEL_LORENZO_2LAYER = 2 };
+
+struct LREngine {
+    size_t d1 = 0, d2 = 0, d3 = 0, s0 = 0, s1 = 0;
+    int bs = 6, pred_dim = 3;
+    bool lo = true, lo2 = false, reg = true;
+    double eb = 0;
+    float prec_f = 0;          // meta_params' float copy of the bound (MD:44-66)
+    float reg_linear = 0, reg_indep = 0;
+    size_t nx = 0, ny = 0, nz = 0, nblocks = 0;
+    Quantizer quantizer;
+    std::vector<int32_t> codes, indicator, reg_types;
+    std::vector<float> reg_unpred, reg_coeffs, raw_coeffs, recon;
+
+    void set(const oracle_lr_params *p) {
+        d1 = p->dims[0], d2 = p->dims[1], d3 = p->dims[2];
+        s0 = d2 * d3, s1 = d3;
+        bs = p->block_size, pred_dim = p->pred_dim;
+        lo = p->lorenzo != 0, lo2 = p->lorenzo2 != 0, reg = p->regression != 0;
+        eb = p->eb;
+        prec_f = (float) eb;
+        // meta_params(): reg_eb_base(1) * precision / RegCoeffNum3d / (float) bs, all float (MD:59-65)
+        reg_linear = 1.0f * prec_f / 4 / (float) bs;
+        reg_indep = (float) bs * reg_linear;
+        nx = (d1 - 1) / bs + 1, ny = (d2 - 1) / bs + 1, nz = (d3 - 1) / bs + 1;
+        nblocks = nx * ny * nz;
+        quantizer.set_eb(eb);
+        quantizer.radius = p->radius;
+    }
+
+    // reconstructed value, 0 below the low faces (the zero-padded pred_buffer, FF:228-243)
+    float rv(long x, long y, long z) const {
+        if (x < 0 || y < 0 || z < 0) return 0.0f;
+        return recon[(size_t) x * s0 + (size_t) y * s1 + (size_t) z];
+    }
+
+    // MetaLorenzoPredictor.hpp:9-77 on an accessor f(dx,dy,dz) of offsets <= 0
+    // (dim0 = x, dim1 = y, unit stride = z); float arithmetic left to right.
+    template<class F>
+    static float lorenzo(F f, int pd, bool two) {
+        if (!two) {
+            if (pd == 3)
+                return f(0, 0, -1) + f(0, -1, 0) + f(-1, 0, 0) - f(0, -1, -1) - f(-1, 0, -1) - f(-1, -1, 0) +
+                       f(-1, -1, -1);
+            if (pd == 2) return f(0, 0, -1) + f(-1, 0, 0) - f(-1, 0, -1);
+            return f(0, 0, -1);
+        }
+        if (pd == 3)
+            return 2 * f(0, 0, -1) - f(0, 0, -2) + 2 * f(0, -1, 0) - 4 * f(0, -1, -1) + 2 * f(0, -1, -2) -
+                   f(0, -2, 0) + 2 * f(0, -2, -1) - f(0, -2, -2) + 2 * f(-1, 0, 0) - 4 * f(-1, 0, -1) +
+                   2 * f(-1, 0, -2) - 4 * f(-1, -1, 0) + 8 * f(-1, -1, -1) - 4 * f(-1, -1, -2) +
+                   2 * f(-1, -2, 0) - 4 * f(-1, -2, -1) + 2 * f(-1, -2, -2) - f(-2, 0, 0) + 2 * f(-2, 0, -1) -
+                   f(-2, 0, -2) + 2 * f(-2, -1, 0) - 4 * f(-2, -1, -1) + 2 * f(-2, -1, -2) - f(-2, -2, 0) +
+                   2 * f(-2, -2, -1) - f(-2, -2, -2);
+        if (pd == 2)
+            return 2 * f(-1, 0, 0) - f(-2, 0, 0) + 2 * f(0, 0, -1) - 4 * f(-1, 0, -1) + 2 * f(-2, 0, -1) -
+                   f(0, 0, -2) + 2 * f(-1, 0, -2) - f(-2, 0, -2);
+        return 2 * f(0, 0, -1) - f(0, 0, -2);
+    }
+
+    // compute_regression_coeffcients_3d (MRP:117-145), float sums in loop order
+    void regression_coeffs(size_t x0, size_t y0, size_t z0, int sx, int sy, int sz, const float *data, float *c) const {
+        float fx = 0, fy = 0, fz = 0, f = 0, sum_x, sum_y;
+        for (int i = 0; i < sx; i++) {
+            sum_x = 0;
+            for (int j = 0; j < sy; j++) {
+                sum_y = 0;
+                const float *row = data + (x0 + i) * s0 + (y0 + j) * s1 + z0;
+                for (int k = 0; k < sz; k++) {
+                    float v = row[k];
+                    sum_y += v;
+                    fz += v * k;
+                }
+                fy += sum_y * j;
+                sum_x += sum_y;
+            }
+            fx += sum_x * i;
+            f += sum_x;
+        }
+        float coeff = (float) (1.0 / (sx * sy * sz));
+        c[0] = (2 * fx / (sx - 1) - f) * 6 * coeff / (sx + 1);
+        c[1] = (2 * fy / (sy - 1) - f) * 6 * coeff / (sy + 1);
+        c[2] = (2 * fz / (sz - 1) - f) * 6 * coeff / (sz + 1);
+        c[3] = f * coeff - ((sx - 1) * c[0] / 2 + (sy - 1) * c[1] / 2 + (sz - 1) * c[2] / 2);
+    }
+
+    static float reg_predict(const float *c, int x, int y, int z) {  // MRP:148-152, 169-171
+        return c[0] * (float) x + c[1] * (float) y + c[2] * (float) z + c[3];
+    }
+
+    // meta_blockwise_selection_3d + meta_block_error_estimation_3d (FF:429-530):
+    // samples of the ORIGINAL data inside the block.
+    int select(size_t x0, size_t y0, size_t z0, int min_size, bool use_reg, const float *c, const float *data) const {
+        double e_lo = 0, e_lo2 = 0, e_reg = 0;
+        auto est = [&](int x, int y, int z) {
+            const float *pos = data + (x0 + x) * s0 + (y0 + y) * s1 + (z0 + z);
+            auto f = [&](int dx, int dy, int dz) { return pos[(long) dx * (long) s0 + (long) dy * (long) s1 + dz]; };
+            float cur = *pos;
+            float noise = 0, noise2 = 0;
+            if (use_reg) e_reg += std::fabs(cur - reg_predict(c, x, y, z));
+            double lp = 0, lp2 = 0;
+            const double n1 = pred_dim == 3 ? 1.22 : (pred_dim == 2 ? 0.81 : 0.5);
+            const double n2 = pred_dim == 3 ? 6.8 : (pred_dim == 2 ? 2.76 : 1.08);
+            if (lo2) {
+                lp2 = lorenzo(f, pred_dim, true);
+                noise2 = (float) (n2 * prec_f);
+            }
+            if (lo) {
+                lp = lorenzo(f, pred_dim, false);
+                noise = (float) (n1 * prec_f);
+            }
+            e_lo += std::fabs((double) cur - lp) + noise;
+            e_lo2 += std::fabs((double) cur - lp2) + noise2;
+        };
+        for (int i = 2; i < min_size - 1; i++) {
+            int b = min_size - i;
+            est(i, i, i);
+            est(i, i, b);
+            est(i, b, i);
+            est(i, b, b);
+        }
+        if (min_size > 3) est(min_size - 1, min_size - 1, min_size - 1);
+        if (use_reg && (!lo || e_reg <= e_lo) && (!lo2 || e_reg < e_lo2)) return SEL_REGRESSION;
+        if (lo2 && (!lo || e_lo2 <= e_lo) && (!use_reg || e_lo2 <= e_reg)) return SEL_LORENZO_2LAYER;
+        return SEL_LORENZO;
+    }
+
+    // quantize_reg_coeff (MRP:10-27)
+    int quantize_coeff(float pred, float cur, double precision, double recip, float *dec) {
+        double diff = cur - pred;
+        double qd = std::fabs(diff) * recip + 1;
+        if (qd < 65536) {
+            qd = (diff > 0) ? qd : -qd;
+            int qi = (int) (qd / 2) + 32768;
+            float d = (float) (pred + 2 * (qi - 32768) * precision);
+            *dec = d;
+            if (std::fabs(d - cur) <= precision) return qi;
+        }
+        *dec = cur;
+        reg_unpred.push_back(cur);
+        return 0;
+    }
+
+    // compress_3d (FF:176-345)
+    void compress(const float *data) {
+        size_t n = d1 * d2 * d3;
+        codes.clear(), indicator.clear(), reg_types.clear(), reg_unpred.clear(), reg_coeffs.clear();
+        quantizer.unpred.clear();
+        raw_coeffs.assign(4 * nblocks, 0.0f);
+        recon.assign(n, 0.0f);
+        codes.reserve(n);
+        float prev[4] = {0, 0, 0, 0};
+        const float prc[4] = {reg_linear, reg_linear, reg_linear, reg_indep};
+        float rcp[4];
+        for (int i = 0; i < 4; i++) rcp[i] = (float) (1.0 / prc[i]);
+        size_t b = 0;
+        for (size_t bi = 0; bi < nx; bi++)
+            for (size_t bj = 0; bj < ny; bj++)
+                for (size_t bk = 0; bk < nz; bk++, b++) {
+                    size_t x0 = bi * bs, y0 = bj * bs, z0 = bk * bs;
+                    int sx = (bi + 1) * bs < d1 ? bs : (int) (d1 - x0);
+                    int sy = (bj + 1) * bs < d2 ? bs : (int) (d2 - y0);
+                    int sz = (bk + 1) * bs < d3 ? bs : (int) (d3 - z0);
+                    int min_size = std::min(std::min(sx, sy), sz);
+                    bool use_reg = reg && min_size >= 2;
+                    float c[4] = {0, 0, 0, 0};
+                    if (use_reg) regression_coeffs(x0, y0, z0, sx, sy, sz, data, c);
+                    memcpy(&raw_coeffs[4 * b], c, sizeof(c));
+                    int sel = select(x0, y0, z0, min_size, use_reg, c, data);
+                    indicator.push_back(sel);
+                    if (sel == SEL_REGRESSION) {
+                        for (int t = 0; t < 4; t++) {
+                            float d;
+                            reg_types.push_back(quantize_coeff(prev[t], c[t], prc[t], rcp[t], &d));
+                            c[t] = d;
+                        }
+                        memcpy(prev, c, sizeof(c));
+                        reg_coeffs.insert(reg_coeffs.end(), c, c + 4);
+                    }
+                    for (int i = 0; i < sx; i++)
+                        for (int j = 0; j < sy; j++)
+                            for (int k = 0; k < sz; k++) {
+                                size_t x = x0 + i, y = y0 + j, z = z0 + k;
+                                float pred;
+                                if (sel == SEL_REGRESSION) {
+                                    pred = reg_predict(c, i, j, k);
+                                } else {
+                                    auto f = [&](int dx, int dy, int dz) {
+                                        return rv((long) x + dx, (long) y + dy, (long) z + dz);
+                                    };
+                                    pred = lorenzo(f, pred_dim, sel == SEL_LORENZO_2LAYER);
+                                }
+                                float v = data[x * s0 + y * s1 + z];
+                                codes.push_back(quantizer.quantize_and_overwrite(v, pred));
+                                recon[x * s0 + y * s1 + z] = v;
+                            }
+                }
+    }
+
+    // decompress_3d (FF:251-318) given codes, indicator, decoded coefficients, unpred lists
+    void decompress(const int32_t *cds, float *out) {
+        recon.assign(d1 * d2 * d3, 0.0f);
+        size_t b = 0, ci = 0, ri = 0;
+        for (size_t bi = 0; bi < nx; bi++)
+            for (size_t bj = 0; bj < ny; bj++)
+                for (size_t bk = 0; bk < nz; bk++, b++) {
+                    size_t x0 = bi * bs, y0 = bj * bs, z0 = bk * bs;
+                    int sx = (bi + 1) * bs < d1 ? bs : (int) (d1 - x0);
+                    int sy = (bj + 1) * bs < d2 ? bs : (int) (d2 - y0);
+                    int sz = (bk + 1) * bs < d3 ? bs : (int) (d3 - z0);
+                    int sel = indicator[b];
+                    const float *c = sel == SEL_REGRESSION ? &reg_coeffs[4 * ri++] : nullptr;
+                    for (int i = 0; i < sx; i++)
+                        for (int j = 0; j < sy; j++)
+                            for (int k = 0; k < sz; k++) {
+                                size_t x = x0 + i, y = y0 + j, z = z0 + k;
+                                int code = cds[ci++];
+                                float v;
+                                if (code == 0) {
+                                    v = quantizer.recover(0.0f, 0);
+                                } else {
+                                    float pred;
+                                    if (c) {
+                                        pred = reg_predict(c, i, j, k);
+                                    } else {
+                                        auto f = [&](int dx, int dy, int dz) {
+                                            return rv((long) x + dx, (long) y + dy, (long) z + dz);
+                                        };
+                                        pred = lorenzo(f, pred_dim, sel == SEL_LORENZO_2LAYER);
+                                    }
+                                    v = quantizer.recover(pred, code);
+                                }
+                                recon[x * s0 + y * s1 + z] = v;
+                            }
+                }
+        memcpy(out, recon.data(), recon.size() * sizeof(float));
+    }
+};
+
+// SZFastFrontend::save (FF:50-84) + SZGeneralCompressor::compress (SGC:30-71)
+static bool lr_raw(const oracle_lr_params *p, const float *data, std::vector<uint8_t> &out) {
+    LREngine e;
+    e.set(p);
+    e.compress(data);
+    out.clear();
+    // meta_params as a raw struct (MD:33-69): padding bytes written as 0
+    put(out, (int) e.bs);
+    put(out, (int) e.pred_dim);
+    put(out, (bool) e.lo);
+    put(out, (bool) e.lo2);
+    put(out, (bool) e.reg);
+    put(out, (uint8_t) 0);
+    put(out, (int) 2);       // lorenzo_padding_layer
+    put(out, (int) 0);       // capacity
+    put(out, e.reg_indep);
+    put(out, e.reg_linear);
+    put(out, 1.0f);          // reg_eb_base
+    put(out, (float) e.bs);  // reg_eb_1
+    put(out, 1.0f);          // sample_ratio
+    put(out, (bool) true);   // lossless
+    put(out, (uint8_t) 0), put(out, (uint8_t) 0), put(out, (uint8_t) 0);
+    put(out, e.eb);          // precision
+    put(out, (bool) false);  // meanInfo.use_mean
+    put(out, (uint8_t) 0), put(out, (uint8_t) 0), put(out, (uint8_t) 0);
+    put(out, 0.0f);          // meanInfo.mean
+    size_t reg_count = e.reg_coeffs.size() / 4;
+    put(out, reg_count);
+    Huffman hi;
+    if (!hi.init(e.indicator.data(), e.indicator.size())) return false;
+    hi.save(out);
+    hi.encode(e.indicator.data(), e.indicator.size(), out);
+    if (reg_count) {  // encode_regression_coefficients (MRP:96-108)
+        put(out, (size_t) e.reg_unpred.size());
+        const uint8_t *u = (const uint8_t *) e.reg_unpred.data();
+        out.insert(out.end(), u, u + e.reg_unpred.size() * sizeof(float));
+        Huffman hr;
+        if (!hr.init(e.reg_types.data(), e.reg_types.size())) return false;
+        hr.save(out);
+        hr.encode(e.reg_types.data(), e.reg_types.size(), out);
+    }
+    out.push_back(2);  // LinearQuantizer::save (Q:185-201)
+    put(out, e.eb);
+    put(out, (int) e.quantizer.radius);
+    put(out, (size_t) e.quantizer.unpred.size());
+    const uint8_t *up = (const uint8_t *) e.quantizer.unpred.data();
+    out.insert(out.end(), up, up + e.quantizer.unpred.size() * sizeof(float));
+    Huffman h;
+    if (!h.init(e.codes.data(), e.codes.size())) return false;
+    h.save(out);
+    h.encode(e.codes.data(), e.codes.size(), out);
+    return true;
+}
+
+// SZFastFrontend::load + decompress_3d (FF:86-131, 251-318), MRP:55-94
+static bool lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, float *out) {
+    const uint8_t *p = raw, *end = raw + n;
+    if (n < 44 + 8 + 8 + 8) return false;
+    oracle_lr_params prm;
+    memset(&prm, 0, sizeof(prm));
+    for (int i = 0; i < 3; i++) prm.dims[i] = dims[i];
+    prm.block_size = take<int>(p);
+    prm.pred_dim = take<int>(p);
+    prm.lorenzo = take<bool>(p);
+    prm.lorenzo2 = take<bool>(p);
+    prm.regression = take<bool>(p);
+    p += 1 + 4 + 4 + 4 * 5 + 1 + 3;
+    prm.eb = take<double>(p);
+    p += 8;  // meanInfo
+    size_t reg_count = take<size_t>(p);
+    if (prm.block_size <= 0) return false;
+    LREngine e;
+    e.set(&prm);
+    e.indicator.resize(e.nblocks);
+    HuffDecoder hi;
+    size_t used = hi.decode(p, end - p, e.indicator.data(), e.nblocks);
+    if (!used) return false;
+    p += used;
+    if (reg_count) {
+        size_t U = take<size_t>(p);
+        const float *ru = (const float *) p;
+        p += U * sizeof(float);
+        std::vector<int32_t> types(4 * reg_count);
+        HuffDecoder hr;
+        used = hr.decode(p, end - p, types.data(), types.size());
+        if (!used) return false;
+        p += used;
+        const float prc[4] = {e.reg_linear, e.reg_linear, e.reg_linear, e.reg_indep};
+        float prev[4] = {0, 0, 0, 0};
+        size_t ui = 0;
+        for (size_t r = 0; r < reg_count; r++)
+            for (int t = 0; t < 4; t++) {  // recover_reg_coeff (MRP:29-37)
+                int tv = types[4 * r + t];
+                float v = tv == 0 ? ru[ui++] : (float) (prev[t] + 2 * (tv - 32768) * (double) prc[t]);
+                e.reg_coeffs.push_back(v);
+                prev[t] = v;
+            }
+        (void) U;
+    }
+    p += 1;  // quantizer uid
+    e.quantizer.set_eb(take<double>(p));
+    e.quantizer.radius = take<int>(p);
+    size_t U = take<size_t>(p);
+    e.quantizer.unpred.assign((const float *) p, (const float *) p + U);
+    p += U * sizeof(float);
+    size_t num = dims[0] * dims[1] * dims[2];
+    std::vector<int32_t> codes(num);
+    HuffDecoder hd;
+    if (!hd.decode(p, end - p, codes.data(), num)) return false;
+    e.decompress(codes.data(), out);
+    return true;
 }
 
 }  // namespace
@@ -1003,12 +1371,102 @@ int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t
     memcpy(&confSize, stream + n - 4, 4);
     const uint8_t *c = stream + n - 4 - confSize;
     int N = (int) (signed char) c[0];
+    size_t dims[4] = {0, 0, 0, 0};
+    if (N < 1 || N > 4) return -1;
+    memcpy(dims, c + 1, N * sizeof(size_t));
+    uint8_t algo = c[1 + N * 8 + 8];
     uint64_t rawLen;
     memcpy(&rawLen, stream, 8);
     std::vector<uint8_t> raw(rawLen);
     size_t z = ZSTD_decompress(raw.data(), rawLen, stream + 8, (n - 4 - confSize) - 8);
     if (ZSTD_isError(z) || z != rawLen) return -4;
+    if (algo == 0) {
+        if (N != 3 || dims[0] * dims[1] * dims[2] != num) return -1;
+        return lr_decompress_raw(raw.data(), raw.size(), dims, out) ? 0 : -2;
+    }
     return interp_decompress_raw(raw.data(), raw.size(), N, out, num) ? 0 : -2;
+}
+
+int oracle_lr_compress(const oracle_lr_params *p, const float *data, int32_t *codes, float *recon,
+                       int32_t *indicator, size_t *n_blocks, float *raw_coeffs,
+                       int32_t *reg_codes, float *reg_coeffs, size_t *n_reg,
+                       float *reg_unpred, size_t *n_reg_unpred, float *unpred, size_t *n_unpred) {
+    if (!p || p->block_size < 1 || p->pred_dim < 1 || p->pred_dim > 3) return -1;
+    if (!p->dims[0] || !p->dims[1] || !p->dims[2]) return -1;
+    LREngine e;
+    e.set(p);
+    e.compress(data);
+    if (codes) memcpy(codes, e.codes.data(), e.codes.size() * sizeof(int32_t));
+    if (recon) memcpy(recon, e.recon.data(), e.recon.size() * sizeof(float));
+    if (indicator) memcpy(indicator, e.indicator.data(), e.indicator.size() * sizeof(int32_t));
+    if (n_blocks) *n_blocks = e.nblocks;
+    if (raw_coeffs) memcpy(raw_coeffs, e.raw_coeffs.data(), e.raw_coeffs.size() * sizeof(float));
+    if (reg_codes) memcpy(reg_codes, e.reg_types.data(), e.reg_types.size() * sizeof(int32_t));
+    if (reg_coeffs) memcpy(reg_coeffs, e.reg_coeffs.data(), e.reg_coeffs.size() * sizeof(float));
+    if (n_reg) *n_reg = e.reg_coeffs.size() / 4;
+    if (reg_unpred) memcpy(reg_unpred, e.reg_unpred.data(), e.reg_unpred.size() * sizeof(float));
+    if (n_reg_unpred) *n_reg_unpred = e.reg_unpred.size();
+    if (unpred) memcpy(unpred, e.quantizer.unpred.data(), e.quantizer.unpred.size() * sizeof(float));
+    if (n_unpred) *n_unpred = e.quantizer.unpred.size();
+    return 0;
+}
+
+int oracle_lr_raw(const oracle_lr_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out) {
+    if (!p || p->block_size < 1 || p->pred_dim < 1 || p->pred_dim > 3) return -1;
+    std::vector<uint8_t> buf;
+    if (!lr_raw(p, data, buf)) return -2;
+    *n_out = buf.size();
+    if (buf.size() > cap) return -3;
+    memcpy(out, buf.data(), buf.size());
+    return 0;
+}
+
+int oracle_lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, float *out) {
+    return lr_decompress_raw(raw, n, dims, out) ? 0 : -2;
+}
+
+int oracle_lr_compress_stream(const oracle_lr_params *p, int rel_mode, double eb_in, const float *data,
+                              uint8_t *out, size_t cap, size_t *n_out) {
+    if (!p || p->block_size < 1 || p->pred_dim < 1 || p->pred_dim > 3) return -1;
+    size_t num = p->dims[0] * p->dims[1] * p->dims[2];
+    oracle_lr_params q = *p;
+    double rel = -1.0;
+    if (rel_mode) {
+        float mn, mx;
+        oracle_range(data, num, &mn, &mx);
+        rel = eb_in;
+        q.eb = rel * (double) (float) (mx - mn);
+    } else {
+        q.eb = eb_in;
+    }
+    std::vector<uint8_t> raw;
+    if (!lr_raw(&q, data, raw)) return -2;
+    std::vector<uint8_t> s(8 + ZSTD_compressBound(raw.size()));
+    uint64_t rawLen = raw.size();
+    memcpy(s.data(), &rawLen, 8);
+    size_t z = ZSTD_compress(s.data() + 8, s.size() - 8, raw.data(), raw.size(), 3);
+    if (ZSTD_isError(z)) return -4;
+    s.resize(8 + z);
+    oracle_interp_params ip;
+    memset(&ip, 0, sizeof(ip));
+    ip.N = 3;
+    for (int i = 0; i < 3; i++) ip.dims[i] = q.dims[i];
+    ip.interp_algo = 1;       // Config defaults the trailer carries (Config.hpp:394-499)
+    ip.alpha = -1, ip.beta = -1;
+    ip.block_size = 32;
+    ip.radius = q.radius;
+    TrailerFields tf;
+    tf.cmprAlgo = 0;          // ALGO_LORENZO_REG
+    tf.lorenzo = q.lorenzo != 0, tf.lorenzo2 = q.lorenzo2 != 0, tf.regression = q.regression != 0;
+    tf.blockSize = q.block_size;
+    tf.pred_dim = q.pred_dim;
+    size_t before = s.size();
+    config_save(s, &ip, q.eb, rel, tf);
+    put(s, (int) (s.size() - before));
+    *n_out = s.size();
+    if (s.size() > cap) return -3;
+    memcpy(out, s.data(), s.size());
+    return 0;
 }
 
 int oracle_range(const float *data, size_t n, float *mn, float *mx) {

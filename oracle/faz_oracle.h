@@ -55,6 +55,7 @@ int oracle_huffman_decode(const uint8_t *buf, size_t n, int32_t *out, size_t num
  * mode is applied first (rel_mode != 0 → eb is relative, Statistic.hpp:34-66). */
 int oracle_compress_stream(const oracle_interp_params *p, int rel_mode, double eb_in, const float *data,
                            uint8_t *out, size_t cap, size_t *n_out);
+/* SZ_decompress of an ALGO_INTERP or (3-D) ALGO_LORENZO_REG stream. */
 int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t num);
 
 /* data_range (Statistic.hpp:11-21). */
@@ -63,6 +64,39 @@ int oracle_range(const float *data, size_t n, float *mn, float *mx);
 /* The block-order sequence index of every point (where its code lands in
  * the code stream), -1 for none. Used by tests to pin the closed form. */
 int oracle_interp_seq_index(const oracle_interp_params *p, int64_t *seq);
+
+/* Config fields the 3-D Lorenzo/regression engine reads: SZFastFrontend
+ * (include/QoZ/frontend/SZFastFrontend.hpp:25-31) and the quantizer radius
+ * (SZLorenzoReg.hpp:102). */
+typedef struct {
+    size_t dims[3];
+    double eb;            /* absolute error bound (after calAbsErrorBound) */
+    int block_size;       /* conf.blockSize */
+    int pred_dim;         /* conf.pred_dim: 3, 2 or 1 */
+    int lorenzo, lorenzo2, regression;
+    int radius;           /* quantbinCnt / 2 */
+} oracle_lr_params;
+
+/* SZFastFrontend::compress_3d (FF:176-345). Every output is optional (NULL).
+ * codes, recon: n points (codes block-major); indicator: one selector per
+ * block; raw_coeffs: 4 per block, the unquantized regression coefficients
+ * (0 where regression is off for the block); reg_codes / reg_coeffs: 4 per
+ * regression block, quantized; reg_unpred and unpred: capacity n. */
+int oracle_lr_compress(const oracle_lr_params *p, const float *data, int32_t *codes, float *recon,
+                       int32_t *indicator, size_t *n_blocks, float *raw_coeffs,
+                       int32_t *reg_codes, float *reg_coeffs, size_t *n_reg,
+                       float *reg_unpred, size_t *n_reg_unpred, float *unpred, size_t *n_unpred);
+
+/* Pre-zstd buffer of SZGeneralCompressor<SZFastFrontend> (SGC:30-71, FF:50-84). */
+int oracle_lr_raw(const oracle_lr_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out);
+/* SGC:115-140 on a pre-zstd buffer. */
+int oracle_lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, float *out);
+
+/* Full SZ_compress stream for ALGO_LORENZO_REG (SZDispatcher.hpp:13-20,
+ * SZLorenzoReg.hpp:95-114, sz.hpp:54-91); p->eb is replaced by the bound
+ * the error-bound mode gives. */
+int oracle_lr_compress_stream(const oracle_lr_params *p, int rel_mode, double eb_in, const float *data,
+                              uint8_t *out, size_t cap, size_t *n_out);
 
 #ifdef __cplusplus
 }

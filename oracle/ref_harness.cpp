@@ -276,6 +276,75 @@ static int run_time(const Args &a) {
     return 0;
 }
 
+// Lorenzo/regression path for 3-D float fields as SZ_compress runs it with
+// cmprAlgo = ALGO_LORENZO_REG (SZDispatcher.hpp:13-20, SZLorenzoReg.hpp:95-114):
+// calAbsErrorBound, then SZGeneralCompressor over SZFastFrontend with a
+// LinearQuantizer of radius quantbinCnt/2, HuffmanEncoder<int> and zstd.
+// Dumps the frontend codes, the pre-zstd buffer (bypass lossless stage), the
+// reconstruction decoded from it, the SZ_compress stream and the bound.
+static int run_lorenzo(const Args &a) {
+    typedef float T;
+    std::string out = get(a, "out");
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    QoZ::Config conf = make_config(a);
+    if (conf.N != 3 || input.size() != conf.num) { fprintf(stderr, "3-D field of dims size expected\n"); return 2; }
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    conf.blockSize = atoi(get(a, "block", "6").c_str());
+    conf.pred_dim = atoi(get(a, "pd", "3").c_str());
+    conf.lorenzo = atoi(get(a, "lo", "1").c_str()) != 0;
+    conf.lorenzo2 = atoi(get(a, "lo2", "0").c_str()) != 0;
+    conf.regression = atoi(get(a, "reg", "1").c_str()) != 0;
+    conf.regression2 = false;
+    conf.verbose = 0;
+    QoZ::calAbsErrorBound(conf, input.data());
+    write_bytes(out + ".eb", &conf.absErrorBound, sizeof(double));
+    {
+        QoZ::Config c(conf);
+        std::vector<T> data(input);
+        auto fe = QoZ::make_sz_fast_frontend<T, 3>(c, QoZ::LinearQuantizer<T>(c.absErrorBound, c.quantbinCnt / 2));
+        std::vector<int> codes = fe.compress(data.data());
+        write_bytes(out + ".codes", codes.data(), codes.size() * sizeof(int));
+    }
+    {
+        QoZ::Config c(conf);
+        std::vector<T> data(input);
+        auto *sz = QoZ::make_sz_general_compressor<T, 3>(
+                QoZ::make_sz_fast_frontend<T, 3>(c, QoZ::LinearQuantizer<T>(c.absErrorBound, c.quantbinCnt / 2)),
+                QoZ::HuffmanEncoder<int>(), QoZ::Lossless_bypass());
+        size_t rawSize = 0;
+        QoZ::uchar *raw = sz->compress(c, data.data(), rawSize);
+        write_bytes(out + ".raw", raw, rawSize);
+        delete sz;
+        QoZ::Config c2(conf);
+        QoZ::LinearQuantizer<T> q;
+        auto *dz = QoZ::make_sz_general_compressor<T, 3>(QoZ::make_sz_fast_frontend<T, 3>(c2, q),
+                                                          QoZ::HuffmanEncoder<int>(), QoZ::Lossless_bypass());
+        std::vector<T> dec(conf.num);
+        dz->decompress(raw, rawSize, dec.data());
+        write_bytes(out + ".recon", dec.data(), dec.size() * sizeof(T));
+        delete dz;
+        delete[] raw;
+    }
+    {
+        QoZ::Config c(conf);
+        std::vector<T> data(input);
+        auto *sz = QoZ::make_sz_general_compressor<T, 3>(
+                QoZ::make_sz_fast_frontend<T, 3>(c, QoZ::LinearQuantizer<T>(c.absErrorBound, c.quantbinCnt / 2)),
+                QoZ::HuffmanEncoder<int>(), QoZ::Lossless_zstd());
+        size_t outSize = 0;
+        char *cmpData = (char *) sz->compress(c, data.data(), outSize);
+        QoZ::uchar *pos = (QoZ::uchar *) cmpData + outSize;
+        c.save(pos);
+        size_t newSize = (char *) pos - cmpData;
+        QoZ::write(int(newSize - outSize), pos);
+        outSize = (char *) pos - cmpData;
+        write_bytes(out + ".stream", cmpData, outSize);
+        delete sz;
+        delete[] cmpData;
+    }
+    return 0;
+}
+
 static int run_huffman(const Args &a) {
     std::vector<int> codes = read_vec<int>(get(a, "in"));
     QoZ::HuffmanEncoder<int> enc;
@@ -317,6 +386,7 @@ int main(int argc, char **argv) {
     }
     if (cmd == "decompress") return run_decompress(a);
     if (cmd == "huffman") return run_huffman(a);
+    if (cmd == "lorenzo") return run_lorenzo(a);
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

@@ -7,7 +7,10 @@ codes, unpredictable list, reconstruction, the pre-zstd engine buffer and the
 full stream (zstd frame from the reference's vendored zstd 1.4.5 + Config
 trailer).  Huffman fixtures store a symbol sequence and the reference
 HuffmanEncoder's save+encode bytes; tuning fixtures the codes, loss and level
-counts of SZInterpolationCompressor::compress in tuning modes 1 and 2.
+counts of SZInterpolationCompressor::compress in tuning modes 1 and 2;
+Lorenzo/regression fixtures (``l*``) the codes, reconstruction, pre-zstd
+buffer and stream of SZ_compress with cmprAlgo = ALGO_LORENZO_REG on 3-D
+fields (SZFastFrontend).
 
     python tests/golden/make_golden.py
 """
@@ -98,6 +101,38 @@ def tuning_cases():
     return c
 
 
+def lorenzo_cases():
+    """3-D Lorenzo/regression fixtures (SZFastFrontend through SZ_compress)."""
+    r = np.random.default_rng(424242)
+    c = []
+    add = lambda name, x, **kw: c.append((name, x, kw))
+    add("l3d_20x17x23_default", smooth(r, (20, 17, 23)), mode="rel", eb=1e-3)
+    add("l3d_18_lo2_reg", smooth(r, (18, 18, 18), 2), mode="rel", eb=1e-3, lo2=1)
+    add("l3d_19x14x22_pd2_lo2", smooth(r, (19, 14, 22), 2), mode="rel", eb=1e-4, pd=2, lo2=1)
+    add("l3d_16x21x13_pd1", smooth(r, (16, 21, 13), 2), mode="rel", eb=1e-3, pd=1, lo2=1)
+    add("l3d_26x11x31_bs5", smooth(r, (26, 11, 31)), mode="rel", eb=1e-3, block=5)
+    add("l3d_21x24x20_bs8_lo2only", smooth(r, (21, 24, 20), 1), mode="rel", eb=1e-3, block=8, lo=0, lo2=1, reg=0)
+    add("l3d_17x19x15_regonly", smooth(r, (17, 19, 15)), mode="rel", eb=1e-2, lo=0, reg=1)
+    add("l3d_15_noise_tiny_eb", r.random((15, 15, 15)).astype(np.float32), mode="abs", eb=1e-9, lo2=1)
+    add("l3d_20_qbins1024", smooth(r, (20, 20, 20), 1), mode="rel", eb=1e-6, qbins=1024, lo2=1)
+    spikes = smooth(r, (14, 13, 12))
+    spikes.flat[r.choice(spikes.size, 25, replace=False)] = np.float32(3e30)
+    add("l3d_spikes_overflow", spikes, mode="abs", eb=1e-3, lo2=1)
+    add("l3d_const_9", np.full((9, 9, 9), -2.5, np.float32), mode="abs", eb=1e-2, lo2=1)
+    # piecewise field: linear ramps (regression wins), smooth bumps (Lorenzo 2-layer) and noise
+    g = np.stack(np.meshgrid(*[np.arange(n, dtype=np.float64) for n in (24, 30, 27)], indexing="ij"))
+    mixed = 0.01 * g[0] - 0.02 * g[1] + 0.005 * g[2]
+    mixed[:12] = np.sin(g[0][:12] / 3.0) * np.cos(g[2][:12] / 4.0)
+    mixed[:, :, 18:] += 0.3 * r.standard_normal(mixed[:, :, 18:].shape)
+    add("l3d_24x30x27_mixed", mixed.astype(np.float32), mode="rel", eb=1e-3, lo2=1)
+    add("l3d_negative_abs", ((smooth(r, (13, 25, 14)) - 0.6) * 1e4).astype(np.float32), mode="abs", eb=0.5)
+    add("l3d_1x40x40_thin", smooth(r, (1, 40, 40), 2), mode="rel", eb=1e-3, lo2=1)
+    add("l3d_40x2x33_thin", smooth(r, (40, 2, 33), 2), mode="rel", eb=1e-3)
+    add("l3d_37x3x7_bs3", smooth(r, (37, 3, 7), 2), mode="rel", eb=1e-3, block=3, lo2=1)
+    add("l3d_12_bs12_single_block", smooth(r, (12, 12, 12)), mode="rel", eb=1e-3, block=12, lo2=1)
+    return c
+
+
 def harness_args(inp, out, dims, kw):
     a = [HARNESS, "interp", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, dims)),
          f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"algo={kw.get('algo', 1)}",
@@ -132,6 +167,21 @@ def main():
         codes.tofile(inp)
         subprocess.check_call([HARNESS, "huffman", f"in={inp}", f"out={out}"])
         np.savez_compressed(os.path.join(HERE, name + ".npz"), codes=codes, encoded=np.fromfile(out, np.uint8))
+        print("wrote", name)
+    for name, x, kw in lorenzo_cases():
+        inp = os.path.join(tmp, "in.f32")
+        out = os.path.join(tmp, "l")
+        x.astype(np.float32).tofile(inp)
+        a = [HARNESS, "lorenzo", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, x.shape)),
+             f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"qbins={kw.get('qbins', 65536)}",
+             f"block={kw.get('block', 6)}", f"pd={kw.get('pd', 3)}", f"lo={kw.get('lo', 1)}",
+             f"lo2={kw.get('lo2', 0)}", f"reg={kw.get('reg', 1)}"]
+        subprocess.check_call(a)
+        ld = lambda ext, dt: np.fromfile(out + ext, dt)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32),
+                            codes=ld(".codes", np.int32), recon=ld(".recon", np.float32),
+                            raw=ld(".raw", np.uint8), stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
+                            params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
         print("wrote", name)
     for name, x, kw in tuning_cases():
         inp = os.path.join(tmp, "in.f32")

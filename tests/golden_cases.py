@@ -15,6 +15,7 @@ def names(prefix):
 INTERP = names("c")
 HUFFMAN = names("h_")
 TUNING = names("t")
+LORENZO = names("l")
 
 
 def load(name):
@@ -39,3 +40,26 @@ def param_kwargs(case):
                 alpha=p.get("alpha", -1.0), beta=p.get("beta", -1.0), max_step=p.get("maxstep", 0),
                 block_size=p.get("bs", 32), radius=p.get("qbins", 65536) // 2,
                 ops=tuple(p.get("ops", ())), dirs=tuple(p.get("dirs", ())))
+
+
+def lr_kwargs(case):
+    """Lorenzo/regression engine parameters of an ``l*`` fixture, absolute bound."""
+    p = case["params"]
+    return dict(dims=p["dims"], eb=float(np.asarray(case["eb"]).ravel()[0]), block_size=p.get("block", 6),
+                pred_dim=p.get("pd", 3), lorenzo=p.get("lo", 1), lorenzo2=p.get("lo2", 0),
+                regression=p.get("reg", 1), radius=p.get("qbins", 65536) // 2)
+
+
+# Struct padding the reference writes with whatever its memory held
+# (meta_params bytes 11 and 41-43, meanInfo bytes 1-3 at offset 52).
+LR_RAW_PADDING = (11, 41, 42, 43, 53, 54, 55)
+
+
+def lr_raw_equal(a, b):
+    a = np.frombuffer(a, np.uint8).copy()
+    b = np.frombuffer(b, np.uint8).copy()
+    if a.size != b.size:
+        return False
+    a[list(LR_RAW_PADDING)] = 0
+    b[list(LR_RAW_PADDING)] = 0
+    return bool(np.array_equal(a, b))

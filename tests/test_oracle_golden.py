@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import HUFFMAN, INTERP, TUNING, load, param_kwargs
+from golden_cases import HUFFMAN, INTERP, LORENZO, TUNING, load, lr_kwargs, lr_raw_equal, param_kwargs
 
 
 def _params(case):
@@ -81,3 +81,61 @@ def test_tuning_modes(name):
     assert np.array_equal(r["recon"].view(np.uint32), c["recon"].view(np.uint32))
     assert r["loss"] == float(c["loss"][0])
     assert np.array_equal(r["level_counts"], c["level_counts"])
+
+
+@pytest.mark.parametrize("name", LORENZO)
+def test_lorenzo_codes_recon(name):
+    """SZFastFrontend codes (block-major) and the reconstruction the decoder rebuilds."""
+    c = load(name)
+    r = oracle.lr_compress(c["input"], oracle.make_lr_params(**lr_kwargs(c)))
+    assert np.array_equal(r["codes"], c["codes"])
+    assert np.array_equal(r["recon"].view(np.uint32), c["recon"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", LORENZO)
+def test_lorenzo_raw_buffer(name):
+    """Pre-zstd buffer: meta_params, indicator tree, coefficients, quantizer, codes."""
+    c = load(name)
+    raw = oracle.lr_raw(c["input"], oracle.make_lr_params(**lr_kwargs(c)))
+    assert lr_raw_equal(raw, c["raw"].tobytes())
+
+
+@pytest.mark.parametrize("name", LORENZO)
+def test_lorenzo_decode_reference(name):
+    c = load(name)
+    dims = c["params"]["dims"]
+    rec = oracle.lr_decompress_raw(c["raw"].tobytes(), dims)
+    assert np.array_equal(rec.view(np.uint32), c["recon"].view(np.uint32))
+    rec2 = oracle.decompress_stream(c["stream"].tobytes(), int(np.prod(dims)))
+    assert np.array_equal(rec2.view(np.uint32), c["recon"].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", LORENZO)
+def test_lorenzo_stream_trailer(name):
+    c = load(name)
+    p = c["params"]
+    kw = lr_kwargs(c)
+    s = oracle.lr_compress_stream(c["input"], oracle.make_lr_params(**kw), p["eb"], rel=p.get("mode") == "rel")
+    ref = c["stream"].tobytes()
+    conf_ref = int(np.frombuffer(ref[-4:], np.int32)[0])
+    conf_ours = int(np.frombuffer(s[-4:], np.int32)[0])
+    assert conf_ref == conf_ours
+    assert s[-4 - conf_ours:] == ref[-4 - conf_ref:]
+    out = oracle.decompress_stream(s, int(np.prod(p["dims"])))
+    assert np.array_equal(out.view(np.uint32), c["recon"].view(np.uint32))
+
+
+def test_lorenzo_fixtures_cover_every_branch():
+    """The fixtures exercise all three selectors, both unpredictable lists,
+    partial blocks and blocks too thin for regression."""
+    seen = np.zeros(3, int)
+    unpred = reg_unpred = thin = 0
+    for name in LORENZO:
+        c = load(name)
+        kw = lr_kwargs(c)
+        r = oracle.lr_compress(c["input"], oracle.make_lr_params(**kw))
+        seen += np.bincount(r["indicator"], minlength=3)
+        unpred += r["unpred"].size
+        reg_unpred += r["reg_unpred"].size
+        thin += any(d % kw["block_size"] == 1 or d == 1 for d in kw["dims"])
+    assert (seen > 10).all() and unpred > 100 and reg_unpred > 10 and thin >= 2
