@@ -67,6 +67,48 @@ class HuffTable(ctypes.Structure):
     ]
 
 
+class LorenzoParams(ctypes.Structure):
+    """Mirror of faz_lorenzo_params."""
+
+    _fields_ = [
+        ("dims", ctypes.c_size_t * 3),
+        ("eb", ctypes.c_double),
+        ("block_size", ctypes.c_int),
+        ("pred_dim", ctypes.c_int),
+        ("lorenzo", ctypes.c_int),
+        ("lorenzo2", ctypes.c_int),
+        ("regression", ctypes.c_int),
+        ("radius", ctypes.c_int),
+    ]
+
+
+class LorenzoOut(ctypes.Structure):
+    _fields_ = [
+        ("d_codes", ctypes.c_void_p),
+        ("n_codes", ctypes.c_size_t),
+        ("d_unpred", ctypes.c_void_p),
+        ("n_unpred", ctypes.c_size_t),
+        ("d_selector", ctypes.c_void_p),
+        ("n_blocks", ctypes.c_size_t),
+        ("n_regression", ctypes.c_size_t),
+    ]
+
+
+def make_lorenzo_params(dims, eb, block_size=6, pred_dim=3, lorenzo=True, lorenzo2=False, regression=True,
+                        radius=32768):
+    p = LorenzoParams()
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    p.block_size = int(block_size)
+    p.pred_dim = int(pred_dim)
+    p.lorenzo = int(bool(lorenzo))
+    p.lorenzo2 = int(bool(lorenzo2))
+    p.regression = int(bool(regression))
+    p.radius = int(radius)
+    return p
+
+
 def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0, block_size=32,
                 radius=32768, ops=(), dirs=()):
     p = InterpParams()
@@ -97,6 +139,8 @@ EXPORTS = [
     "faz_gpu_compress_f32", "faz_gpu_fetch_raw", "faz_gpu_set_profiling", "faz_gpu_profile_read",
     "faz_gpu_profile_kernel",
     "faz_gpu_profile_bytes",
+    "faz_gpu_lorenzo_compress_f32", "faz_gpu_lorenzo_raw_f32", "faz_gpu_lorenzo_decompress_raw",
+    "faz_sz_compress_lorenzo_f32",
 ]
 
 _lib = None
@@ -138,6 +182,11 @@ def lib():
         L.faz_gpu_profile_kernel.argtypes = [vp]
         L.faz_gpu_profile_bytes.argtypes = [vp]
         L.faz_gpu_profile_bytes.restype = ctypes.c_double
+        L.faz_gpu_lorenzo_compress_f32.argtypes = [vp, vp, vp, P(LorenzoParams), P(LorenzoOut)]
+        L.faz_gpu_lorenzo_raw_f32.argtypes = [vp, vp, vp, P(LorenzoParams), vp, sz, P(sz)]
+        L.faz_gpu_lorenzo_decompress_raw.argtypes = [vp, vp, sz, vp, vp]
+        L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
+                                                  ctypes.c_int, P(vp), P(sz)]
         _lib = L
     return _lib
 
@@ -336,3 +385,60 @@ class Engine:
         t = (ctypes.c_double * 8)()
         self._L.faz_gpu_last_timings(self.h, t, 8)
         return list(t)
+
+    # --- 3-D Lorenzo/regression engine -------------------------------------
+    def lorenzo_compress(self, x, params, recon=None):
+        """Returns dict(codes uint16 block-major, unpred, selector uint8, n_regression, recon tensor)."""
+        import torch
+        if recon is None:
+            recon = torch.empty_like(x)
+        torch.cuda.synchronize()
+        out = LorenzoOut()
+        self._check(self._L.faz_gpu_lorenzo_compress_f32(self.h, self._dev(x), self._dev(recon),
+                                                         ctypes.byref(params), ctypes.byref(out)),
+                    "lorenzo_compress")
+        return dict(codes=d2h(out.d_codes, out.n_codes * 2, np.uint16),
+                    unpred=d2h(out.d_unpred, out.n_unpred * 4, np.float32),
+                    selector=d2h(out.d_selector, out.n_blocks, np.uint8), n_regression=out.n_regression,
+                    recon=recon)
+
+    def lorenzo_raw(self, x, params, recon=None):
+        import torch
+        torch.cuda.synchronize()
+        need = ctypes.c_size_t()
+        rp = self._dev(recon) if recon is not None else None
+        rc = self._L.faz_gpu_lorenzo_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params), None, 0,
+                                             ctypes.byref(need))
+        if rc != -4:
+            self._check(rc, "lorenzo_raw")
+        buf = np.zeros(need.value, np.uint8)
+        self._check(self._L.faz_gpu_lorenzo_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params), _np_ptr(buf),
+                                                    buf.size, ctypes.byref(need)), "lorenzo_raw")
+        return buf[:need.value].tobytes()
+
+    def lorenzo_decompress_raw(self, raw, dims, out):
+        import torch
+        b = np.frombuffer(raw, np.uint8).copy()
+        d = np.array(dims, np.uint64)
+        torch.cuda.synchronize()
+        self._check(self._L.faz_gpu_lorenzo_decompress_raw(self.h, _np_ptr(b), b.size, _np_ptr(d), self._dev(out)),
+                    "lorenzo_decompress_raw")
+        return out
+
+    def sz_compress_lorenzo(self, data, params, eb, rel=True):
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        if isinstance(data, np.ndarray):
+            a = np.ascontiguousarray(data, np.float32)
+            ptr, ondev = _np_ptr(a), 0
+        else:
+            import torch
+            torch.cuda.synchronize()
+            ptr, ondev = self._dev(data), 1
+        self._check(self._L.faz_sz_compress_lorenzo_f32(self.h, ctypes.byref(params), 1 if rel else 0, float(eb),
+                                                        ptr, ondev, ctypes.byref(p), ctypes.byref(n)),
+                    "sz_compress_lorenzo")
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            self._L.faz_free(p)

@@ -1,0 +1,109 @@
+// Context of the C-ABI (faz_gpu.h): device scratch, stream, events and the
+// engine buffer waiting to be fetched.  Shared by the host orchestration of
+// the interpolation engine (faz_gpu.hip) and of the Lorenzo/regression engine
+// (faz_lorenzo.hip).  Not part of the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "faz_gpu.h"
+#include "../host/engine_format.h"
+#include "../host/huffman_host.h"
+
+namespace faz {
+
+template<class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap && p) return hipSuccess;
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(n, 1);
+        hipError_t e = hipMalloc((void **) &p, want * sizeof(T));
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Engine buffer waiting in a context: everything but the device arrays.
+struct PendingRaw {
+    faz::EngineHeader h;
+    double eb = 0;
+    int radius = 0;
+    uint64_t n_unpred = 0;
+    faz::HuffTable tab;
+    size_t nbytes = 0;
+    bool valid = false;
+};
+
+}  // namespace faz
+
+struct faz_gpu_ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    std::string err;
+    faz::DevBuf<uint16_t> codes;
+    faz::DevBuf<float> outval, unpred, work_in, work_out;
+    // compact lattices handed from one whole-tile level to the next
+    faz::DevBuf<float> lat[2];
+    faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
+    faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
+    hipEvent_t ev[8] = {};
+    double timings[8] = {};
+    // histogram of the codes last produced by interp_compress
+    const uint16_t *hist_codes = nullptr;
+    size_t hist_n = 0;
+    std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
+    std::vector<uint64_t> h_tab;  // packed code table of the last encode
+    faz::PendingRaw pending;
+    // per-launch timing of the interpolation pass kernels
+    bool profile = false;
+    struct ProfBucket {
+        double ms = 0, bytes = 0;  // bytes: algorithmic HBM bytes of the launches
+        uint64_t launches = 0, points = 0;
+    } prof[3];
+    std::vector<hipEvent_t> prof_ev;
+    // 3-D fields: one fused launch per level (0 = per-pass launches)
+    bool fused3d = true;
+    // levels that qualify run on the whole-tile kernel (persistent, tile_grid
+    // workgroups: one per compute unit, a multiple of 8 XCDs)
+    bool tile3d = true;
+    uint32_t tile_grid = 256;
+    // levels with fewer blocks than this run as per-pass launches
+    uint64_t fused_min_blocks = 256;
+    // Lorenzo/regression engine: per-block selector, coefficients, regression
+    // ranks, the in-block hyperplane order table and its offsets
+    faz::DevBuf<uint8_t> lr_ind;
+    faz::DevBuf<float4> lr_coef, lr_regc;
+    faz::DevBuf<uint32_t> lr_flag, lr_diag;
+    faz::DevBuf<uint64_t> lr_rank;
+    faz::DevBuf<uint16_t> lr_tab;
+    uint32_t lr_tab_bs = 0;  // block size the table was built for
+};
+
+#define CTX_FAIL(code, ...)                                          \
+    do {                                                             \
+        char _m[512];                                                \
+        snprintf(_m, sizeof(_m), __VA_ARGS__);                       \
+        ctx->err = _m;                                               \
+        return code;                                                 \
+    } while (0)
+
+#define HIPCHECK(x)                                                                               \
+    do {                                                                                          \
+        hipError_t _e = (x);                                                                      \
+        if (_e != hipSuccess) CTX_FAIL(FAZ_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_e)); \
+    } while (0)
+

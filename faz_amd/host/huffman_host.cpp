@@ -265,6 +265,36 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
 }
 
 // HuffmanEncoder::load + decode (HUF:236-296, 310-454)
+void encode_huffman(const HuffTable &tab, const int32_t *s, size_t n, std::vector<uint8_t> &out) {
+    uint64_t total = 0;
+    for (size_t i = 0; i < n; i++) total += tab.len[s[i] - tab.offset];
+    const uint64_t nbytes = (total + 7) / 8;
+    const size_t at = out.size();
+    out.resize(at + 8 + nbytes, 0);
+    memcpy(out.data() + at, &nbytes, 8);
+    uint8_t *w = out.data() + at + 8;
+    uint64_t acc = 0;  // pending bits, MSB-aligned
+    int nacc = 0;
+    for (size_t i = 0; i < n; i++) {
+        const int st = s[i] - tab.offset;
+        int l = tab.len[st];
+        uint64_t c = tab.code[st];
+        while (l > 0) {
+            const int take = std::min(l, 64 - nacc);
+            acc |= take == 64 ? c : (c >> nacc) & (~0ull << (64 - nacc - take));
+            if (take < 64) c <<= take;
+            l -= take;
+            nacc += take;
+            while (nacc >= 8) {
+                *w++ = (uint8_t) (acc >> 56);
+                acc <<= 8;
+                nacc -= 8;
+            }
+        }
+    }
+    if (nacc) *w++ = (uint8_t) (acc >> 56);
+}
+
 size_t decode_huffman(const uint8_t *buf, size_t n, std::vector<int32_t> &out, size_t num) {
     if (n < 12) return 0;
     int32_t offset;

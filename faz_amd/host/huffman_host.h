@@ -19,6 +19,11 @@ struct HuffTable {
 // than 64 bits.
 int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab);
 
+// HuffmanEncoder::encode (HuffmanEncoder.hpp:150-227) on the host: appends
+// the size_t byte count and the MSB-first bitstream of s[0..n) to out.  Every
+// symbol must have a code in tab.
+void encode_huffman(const HuffTable &tab, const int32_t *s, size_t n, std::vector<uint8_t> &out);
+
 // Loads a saved tree followed by a size_t-prefixed bitstream and decodes num
 // symbols.  Returns the bytes consumed, 0 on malformed input.
 size_t decode_huffman(const uint8_t *buf, size_t n, std::vector<int32_t> &out, size_t num);

@@ -11,6 +11,10 @@ extern "C++" {
 // into `raw`, sized exactly.
 int faz_compress_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
                          std::vector<uint8_t> &raw);
+// The same for the 3-D Lorenzo/regression engine (SZGeneralCompressor over
+// SZFastFrontend).
+int faz_lorenzo_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_lorenzo_params *p,
+                        std::vector<uint8_t> &raw);
 // Device scratch of the context, grown on demand (n floats).
 float *faz_ctx_work_in(faz_gpu_ctx *ctx, size_t n);
 hipStream_t faz_ctx_stream(faz_gpu_ctx *ctx);

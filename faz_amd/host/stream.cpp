@@ -190,6 +190,47 @@ int compress_interp_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_d
     return FAZ_OK;
 }
 
+faz_lorenzo_params lorenzo_params_from_config(const QoZ::Config &c) {
+    faz_lorenzo_params p;
+    memset(&p, 0, sizeof(p));
+    for (int i = 0; i < 3; i++) p.dims[i] = c.dims[i];
+    p.eb = c.absErrorBound;
+    p.block_size = c.blockSize;
+    p.pred_dim = c.pred_dim;
+    p.lorenzo = c.lorenzo;
+    p.lorenzo2 = c.lorenzo2;
+    p.regression = c.regression;
+    p.radius = c.quantbinCnt / 2;
+    return p;
+}
+
+// SZ_compress for ALGO_LORENZO_REG on a 3-D field: calAbsErrorBound
+// (SZDispatcher.hpp:13-20), SZ_compress_LorenzoReg's fast frontend path
+// (SZLorenzoReg.hpp:95-110), zstd, trailer.
+int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
+    if (conf.N != 3 || conf.regression2) {
+        faz_ctx_set_error(ctx, "the GPU Lorenzo/regression engine serves 3-D fields without regression2 "
+                               "(SZFastFrontend)");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    if (conf.wavelet) {
+        faz_ctx_set_error(ctx, "wavelet preprocessing is outside the GPU path");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    int rc = cal_abs_error_bound(ctx, conf, d_data);
+    if (rc) return rc;
+    faz_lorenzo_params p = lorenzo_params_from_config(conf);
+    std::vector<uint8_t> raw;
+    rc = faz_lorenzo_raw_vec(ctx, d_data, nullptr, &p, raw);
+    if (rc) return rc;
+    if (!zstd_wrap(raw, out, QoZ::Config::size_est())) {
+        faz_ctx_set_error(ctx, "zstd compression failed");
+        return FAZ_ERR_INVALID;
+    }
+    append_trailer(conf, out);
+    return FAZ_OK;
+}
+
 int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Config &conf, float *d_out,
                       size_t num) {
     if (n < 4) return FAZ_ERR_FORMAT;
@@ -202,14 +243,20 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
         faz_ctx_set_error(ctx, "stream shape does not match the output buffer");
         return FAZ_ERR_FORMAT;
     }
-    if (conf.cmprAlgo != QoZ::ALGO_INTERP || conf.wavelet != 0) {
-        faz_ctx_set_error(ctx, "only ALGO_INTERP streams without wavelets decode on the GPU path");
+    const bool lorenzo = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && conf.N == 3 && !conf.regression2;
+    if (!(conf.cmprAlgo == QoZ::ALGO_INTERP || lorenzo) || conf.wavelet != 0) {
+        faz_ctx_set_error(ctx, "only ALGO_INTERP and 3-D ALGO_LORENZO_REG streams without wavelets decode on the "
+                               "GPU path");
         return FAZ_ERR_UNSUPPORTED;
     }
     std::vector<uint8_t> raw;
     if (!zstd_unwrap(stream, n - 4 - confSize, raw)) {
         faz_ctx_set_error(ctx, "zstd frame malformed");
         return FAZ_ERR_FORMAT;
+    }
+    if (lorenzo) {
+        size_t dims[3] = {conf.dims[0], conf.dims[1], conf.dims[2]};
+        return faz_gpu_lorenzo_decompress_raw(ctx, raw.data(), raw.size(), dims, d_out);
     }
     return faz_gpu_interp_decompress_raw(ctx, raw.data(), raw.size(), conf.N, d_out, num);
 }
@@ -241,14 +288,17 @@ template<>
 char *SZ_compress<float>(QoZ::Config &config, const float *data, size_t &outSize) {
     QoZ::Config conf(config);
     if (conf.N < 1 || conf.N > 4) throw faz::Error(FAZ_ERR_INVALID, "Data dimension higher than 4 is not supported.");
-    if (conf.cmprAlgo != QoZ::ALGO_INTERP)
+    if (conf.cmprAlgo != QoZ::ALGO_INTERP && conf.cmprAlgo != QoZ::ALGO_LORENZO_REG)
         throw faz::Error(FAZ_ERR_UNSUPPORTED, std::string("GPU engine does not serve ") + QoZ::ALGO_STR[conf.cmprAlgo]);
     faz_gpu_ctx *ctx = api_ctx();
     int rc;
     const float *d = to_device(ctx, data, conf.num, false, rc);
     if (rc) fail(ctx, rc, "staging input");
     std::vector<uint8_t> out;
-    rc = compress_interp_stream(ctx, conf, d, out);
+    if (conf.cmprAlgo == QoZ::ALGO_LORENZO_REG)
+        rc = compress_lorenzo_stream(ctx, conf, d, out);
+    else
+        rc = compress_interp_stream(ctx, conf, d, out);
     if (rc) fail(ctx, rc, "SZ_compress");
     char *buf = new char[out.size()];
     memcpy(buf, out.data(), out.size());
@@ -455,6 +505,38 @@ faz_status faz_sz_decompress_f32(faz_gpu_ctx *ctx, const uint8_t *stream, size_t
     if (rc) return rc;
     if (!out_on_device && hipMemcpy(out, d, num * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         return FAZ_ERR_HIP;
+    return FAZ_OK;
+}
+
+faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
+                                       const float *data, int data_on_device, uint8_t **out, size_t *out_size) {
+    if (!ctx || !p || !data || !out || !out_size) return FAZ_ERR_INVALID;
+    QoZ::Config conf(p->dims[0], p->dims[1], p->dims[2]);
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    conf.blockSize = p->block_size;
+    conf.pred_dim = p->pred_dim;
+    conf.lorenzo = p->lorenzo != 0;
+    conf.lorenzo2 = p->lorenzo2 != 0;
+    conf.regression = p->regression != 0;
+    conf.quantbinCnt = p->radius * 2;
+    if (rel_mode) {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = eb;
+    } else {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = eb;
+    }
+    int rc;
+    const float *d = to_device(ctx, data, conf.num, data_on_device != 0, rc);
+    if (rc) return rc;
+    std::vector<uint8_t> buf;
+    rc = compress_lorenzo_stream(ctx, conf, d, buf);
+    if (rc) return rc;
+    uint8_t *m = (uint8_t *) malloc(buf.size());
+    if (!m) return FAZ_ERR_INVALID;
+    memcpy(m, buf.data(), buf.size());
+    *out = m;
+    *out_size = buf.size();
     return FAZ_OK;
 }
 

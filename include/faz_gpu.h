@@ -152,6 +152,53 @@ faz_status faz_sz_decompress_f32(faz_gpu_ctx *ctx, const uint8_t *stream, size_t
                                  int out_on_device);
 void faz_free(void *p);
 
+/* --- A19: the 3-D Lorenzo/regression engine, SZFastFrontend
+ * (include/QoZ/frontend/SZFastFrontend.hpp:176-345, 429-530) as
+ * SZ_compress_LorenzoReg composes it for 3-D fields (SZLorenzoReg.hpp:95-114:
+ * SZGeneralCompressor, LinearQuantizer, HuffmanEncoder<int>).  The
+ * parameters are the QoZ::Config fields the frontend reads (FF:25-31). */
+typedef struct {
+    size_t dims[3];         /* slowest first, last dimension contiguous */
+    double eb;              /* absolute error bound */
+    int block_size;         /* Config::blockSize, 1..16 */
+    int pred_dim;           /* Config::pred_dim: 3, 2 or 1 */
+    int lorenzo;            /* Config::lorenzo */
+    int lorenzo2;           /* Config::lorenzo2 */
+    int regression;         /* Config::regression */
+    int radius;             /* Config::quantbinCnt / 2, <= 32768 */
+} faz_lorenzo_params;
+
+/* Results of faz_gpu_lorenzo_compress_f32 (device pointers owned by ctx). */
+typedef struct {
+    uint16_t *d_codes;      /* quantization codes, block-major (FF:191, 212) */
+    size_t n_codes;
+    float *d_unpred;        /* unpredictable values in code order */
+    size_t n_unpred;
+    const uint8_t *d_selector; /* per block: 0 Lorenzo, 1 regression, 2 2-layer Lorenzo */
+    size_t n_blocks;
+    size_t n_regression;    /* blocks predicted by regression */
+} faz_lorenzo_out;
+
+/* SZFastFrontend::compress_3d (FF:176-345).  d_recon (n floats) receives
+ * the reconstruction the reference keeps in its prediction buffer. */
+faz_status faz_gpu_lorenzo_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
+                                        const faz_lorenzo_params *p, faz_lorenzo_out *out);
+/* The buffer SZGeneralCompressor::compress hands to zstd (SGC:30-71): the
+ * frontend's save (FF:50-84: meta_params, bound, selector and coefficient
+ * Huffman sections, LinearQuantizer::save) then the codes' tree and bits.
+ * d_recon may be NULL. */
+faz_status faz_gpu_lorenzo_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_lorenzo_params *p,
+                                   uint8_t *h_raw, size_t cap, size_t *raw_size);
+/* SZGeneralCompressor::decompress after zstd (SGC:115-140) into d_out. */
+faz_status faz_gpu_lorenzo_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, const size_t *dims,
+                                          float *d_out);
+/* SZ_compress for cmprAlgo = ALGO_LORENZO_REG on a 3-D field
+ * (SZDispatcher.hpp:13-20, SZLorenzoReg.hpp:95-114, sz.hpp:54-91): bound
+ * conversion as faz_sz_compress_f32, zstd, Config trailer.  faz_sz_decompress_f32
+ * decodes the result. */
+faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
+                                       const float *data, int data_on_device, uint8_t **out, size_t *out_size);
+
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
