@@ -109,6 +109,61 @@ def make_lorenzo_params(dims, eb, block_size=6, pred_dim=3, lorenzo=True, lorenz
     return p
 
 
+class TuneIn(ctypes.Structure):
+    """Mirror of faz_tune_in."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("abs_eb", ctypes.c_double),
+        ("rel_eb", ctypes.c_double),
+        ("rng", ctypes.c_double),
+        ("interp_block_size", ctypes.c_int),
+        ("max_step", ctypes.c_size_t),
+        ("levelwise", ctypes.c_int),
+        ("sample_block_size", ctypes.c_int),
+    ]
+
+
+class TuneOut(ctypes.Structure):
+    """Mirror of faz_tune_out."""
+
+    _fields_ = [
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("levelwise", ctypes.c_int),
+        ("ops", ctypes.c_uint8 * 32),
+        ("dirs", ctypes.c_uint8 * 32),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("max_step", ctypes.c_size_t),
+        ("sample_block_size", ctypes.c_int),
+        ("n_blocks", ctypes.c_size_t),
+        ("block_starts", (ctypes.c_size_t * 3) * 4096),
+        ("pred_loss", (ctypes.c_double * 4) * 32),
+        ("n_tests", ctypes.c_int),
+        ("test_alpha", ctypes.c_double * 128),
+        ("test_beta", ctypes.c_double * 128),
+        ("test_eb", ctypes.c_double * 128),
+        ("test_bitrate", ctypes.c_double * 128),
+        ("test_psnr", ctypes.c_double * 128),
+        ("test_bytes", ctypes.c_size_t * 128),
+        ("ms", ctypes.c_double),
+    ]
+
+
+def tune_out_dict(o):
+    lw, nt, nb = o.levelwise, min(o.n_tests, 128), o.n_blocks
+    return dict(alpha=o.alpha, beta=o.beta, levelwise=lw, ops=list(o.ops[:lw]), dirs=list(o.dirs[:lw]),
+                interp_algo=o.interp_algo, direction=o.direction, max_step=o.max_step,
+                sample_block_size=o.sample_block_size, n_blocks=nb,
+                block_starts=np.array([list(o.block_starts[i]) for i in range(min(nb, 4096))], np.int64),
+                pred_loss=np.array([list(o.pred_loss[i]) for i in range(32)]),
+                tests=[dict(alpha=o.test_alpha[i], beta=o.test_beta[i], eb=o.test_eb[i], bitrate=o.test_bitrate[i],
+                            psnr=o.test_psnr[i], bytes=o.test_bytes[i]) for i in range(nt)],
+                ms=getattr(o, "ms", 0.0))
+
+
 def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0, block_size=32,
                 radius=32768, ops=(), dirs=()):
     p = InterpParams()
@@ -140,7 +195,7 @@ EXPORTS = [
     "faz_gpu_profile_kernel",
     "faz_gpu_profile_bytes",
     "faz_gpu_lorenzo_compress_f32", "faz_gpu_lorenzo_raw_f32", "faz_gpu_lorenzo_decompress_raw",
-    "faz_sz_compress_lorenzo_f32",
+    "faz_sz_compress_lorenzo_f32", "faz_gpu_tune_qoz_f32",
 ]
 
 _lib = None
@@ -187,6 +242,7 @@ def lib():
         L.faz_gpu_lorenzo_decompress_raw.argtypes = [vp, vp, sz, vp, vp]
         L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
+        L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
         _lib = L
     return _lib
 
@@ -442,3 +498,20 @@ class Engine:
             return ctypes.string_at(p, n.value)
         finally:
             self._L.faz_free(p)
+
+    # --- exhaustive QoZ tuner -----------------------------------------------
+    def tune_qoz(self, x, shape, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0,
+                 sample_block_size=0):
+        """faz -q tuner on the device field x (flat tensor of `shape`)."""
+        import torch
+        torch.cuda.synchronize()
+        t = TuneIn()
+        t.N = len(shape)
+        for i, d in enumerate(shape):
+            t.dims[i] = int(d)
+        t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
+        t.interp_block_size, t.max_step, t.levelwise, t.sample_block_size = interp_block_size, max_step, \
+            levelwise, sample_block_size
+        o = TuneOut()
+        self._check(self._L.faz_gpu_tune_qoz_f32(self.h, self._dev(x), ctypes.byref(t), ctypes.byref(o)), "tune_qoz")
+        return tune_out_dict(o)

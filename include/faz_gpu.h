@@ -199,6 +199,44 @@ faz_status faz_gpu_lorenzo_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw
 faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
                                        const float *data, int data_on_device, uint8_t **out, size_t *out_size);
 
+/* --- A18: the exhaustive QoZ sampling tuner (`faz -q`): Tuning() with
+ * conf.QoZ set, RD target, no wavelets (include/QoZ/api/impl/SZInterp.hpp:
+ * 1589-2557): sampleBlocks (SZI:993-1127), per-level interpolator and
+ * direction from tuning-mode-2 losses, alpha/beta from CompressTest
+ * (SZI:1131-1392) bitrate/PSNR, setABwithRelBound (SZI:1394-1468).  The
+ * sample blocks are read in place from d_in (the field on the device). */
+typedef struct {
+    int N;                  /* 2 or 3 */
+    size_t dims[4];
+    double abs_eb;          /* Config::absErrorBound after calAbsErrorBound */
+    double rel_eb;          /* Config::relErrorBound when Tuning starts */
+    double rng;             /* Config::rng (max - min) */
+    int interp_block_size;  /* Config::interpBlockSize */
+    size_t max_step;        /* Config::maxStep, 0: the QoZ default */
+    int levelwise;          /* Config::levelwisePredictionSelection, 0: the QoZ default */
+    int sample_block_size;  /* Config::sampleBlockSize, 0: the QoZ default */
+} faz_tune_in;
+
+typedef struct {
+    double alpha, beta;     /* chosen level-wise bound scaling */
+    int levelwise;          /* entries of ops/dirs */
+    uint8_t ops[32];        /* interpAlgo_list */
+    uint8_t dirs[32];       /* interpDirection_list */
+    int interp_algo;        /* Config::interpAlgo / interpDirection as Tuning leaves them */
+    int direction;
+    size_t max_step;
+    int sample_block_size;
+    size_t n_blocks;        /* sampled blocks and their starts */
+    size_t block_starts[4096][3];
+    double pred_loss[32][4];/* [level-1][op * 2 + (direction != 0)] summed mode-2 loss */
+    int n_tests;            /* CompressTest calls, in order */
+    double test_alpha[128], test_beta[128], test_eb[128], test_bitrate[128], test_psnr[128];
+    size_t test_bytes[128]; /* compressed bytes of each CompressTest */
+    double ms;              /* device+host time of the tuner */
+} faz_tune_out;
+
+faz_status faz_gpu_tune_qoz_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tune_out *out);
+
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,

@@ -286,3 +286,90 @@ def lr_compress_stream(data, params, eb, rel=True):
                                          _ptr(out), cap, ctypes.byref(n))
     assert rc == 0, rc
     return out[:n.value].tobytes()
+
+
+class TuneIn(ctypes.Structure):
+    """Mirror of oracle_tune_in (faz_oracle.h)."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("abs_eb", ctypes.c_double),
+        ("rel_eb", ctypes.c_double),
+        ("rng", ctypes.c_double),
+        ("interp_block_size", ctypes.c_int),
+        ("max_step", ctypes.c_size_t),
+        ("levelwise", ctypes.c_int),
+        ("sample_block_size", ctypes.c_int),
+    ]
+
+
+class TuneOut(ctypes.Structure):
+    _fields_ = [
+        ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double),
+        ("levelwise", ctypes.c_int),
+        ("ops", ctypes.c_uint8 * 32),
+        ("dirs", ctypes.c_uint8 * 32),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("max_step", ctypes.c_size_t),
+        ("sample_block_size", ctypes.c_int),
+        ("n_blocks", ctypes.c_size_t),
+        ("block_starts", (ctypes.c_size_t * 3) * 4096),
+        ("pred_loss", (ctypes.c_double * 4) * 32),
+        ("n_tests", ctypes.c_int),
+        ("test_alpha", ctypes.c_double * 128),
+        ("test_beta", ctypes.c_double * 128),
+        ("test_eb", ctypes.c_double * 128),
+        ("test_bitrate", ctypes.c_double * 128),
+        ("test_psnr", ctypes.c_double * 128),
+        ("test_bytes", ctypes.c_size_t * 128),
+    ]
+
+
+def qoz_tune(data, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0, sample_block_size=0):
+    """QoZ exhaustive tuner (faz -q).  Returns a dict of the decisions and of
+    every intermediate result."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    L = lib()
+    L.oracle_qoz_tune.argtypes = [ctypes.POINTER(TuneIn), ctypes.c_void_p, ctypes.POINTER(TuneOut)]
+    t = TuneIn()
+    t.N = data.ndim
+    for i, d in enumerate(data.shape):
+        t.dims[i] = d
+    t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
+    t.interp_block_size, t.max_step, t.levelwise, t.sample_block_size = interp_block_size, max_step, levelwise, \
+        sample_block_size
+    o = TuneOut()
+    rc = L.oracle_qoz_tune(ctypes.byref(t), _ptr(data), ctypes.byref(o))
+    assert rc == 0, rc
+    lw = o.levelwise
+    nt = min(o.n_tests, 128)
+    nb = o.n_blocks
+    return dict(alpha=o.alpha, beta=o.beta, levelwise=lw, ops=list(o.ops[:lw]), dirs=list(o.dirs[:lw]),
+                interp_algo=o.interp_algo, direction=o.direction, max_step=o.max_step,
+                sample_block_size=o.sample_block_size, n_blocks=nb,
+                block_starts=np.array([list(o.block_starts[i]) for i in range(min(nb, 4096))], np.int64),
+                pred_loss=np.array([list(o.pred_loss[i]) for i in range(32)]),
+                tests=[dict(alpha=o.test_alpha[i], beta=o.test_beta[i], eb=o.test_eb[i], bitrate=o.test_bitrate[i],
+                            psnr=o.test_psnr[i], bytes=o.test_bytes[i]) for i in range(nt)])
+
+
+def compress_test(field, starts, sbs, params, rng):
+    """CompressTest over the sample blocks at `starts`: (raw bytes, compressed size, squared error)."""
+    field = np.ascontiguousarray(field, dtype=np.float32)
+    L = lib()
+    vp = ctypes.c_void_p
+    L.oracle_compress_test.argtypes = [ctypes.POINTER(InterpParams), vp, vp, vp, ctypes.c_size_t, ctypes.c_size_t,
+                                       ctypes.c_double, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_double)]
+    fd = np.array(list(field.shape) + [1] * (3 - field.ndim), np.uint64)
+    st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+    cap = 64 * 1024 + st.shape[0] * (sbs + 1) ** field.ndim * 16
+    raw = np.zeros(cap, np.uint8)
+    rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+    rc = L.oracle_compress_test(ctypes.byref(params), _ptr(field), _ptr(fd), _ptr(st), st.shape[0], sbs, float(rng),
+                                _ptr(raw), cap, ctypes.byref(rs), ctypes.byref(by), ctypes.byref(sq))
+    assert rc == 0, rc
+    return raw[:rs.value].tobytes(), by.value, sq.value

@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <vector>
 
 extern "C" {
@@ -1276,6 +1277,258 @@ static bool lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, 
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// QoZ exhaustive sampling tuner (SZInterp.hpp:1589-2557, QoZ path, RD target,
+// no wavelets, no Lorenzo test).
+// ---------------------------------------------------------------------------
+struct TuneState {
+    int N;
+    std::vector<size_t> bdims;               // sample block shape, (sbs+1)^N
+    std::vector<std::vector<float>> blocks;  // sampled blocks
+    oracle_interp_params base;               // engine parameters shared by the runs
+};
+
+// sampleBlocks with profiling = 0, var_first = 0 (SZI:1056-1086) and
+// sample_blocks (Sample.hpp:79-139): blocks of (sbs+1)^N starting on a grid of
+// sbs, every (size_t)(1/rate)-th in raster order.
+static void sample_blocks(const float *data, int N, const size_t *dims, size_t sbs, double rate, TuneState &ts,
+                          oracle_tune_out *out) {
+    size_t stride = (size_t) (1.0 / rate);
+    if (stride <= 0) stride = 1;
+    size_t idx = 0;
+    ts.blocks.clear();
+    out->n_blocks = 0;
+    auto take = [&](size_t x0, size_t y0, size_t z0) {
+        std::vector<float> b;
+        if (N == 2) {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++) b.push_back(data[(x0 + i) * dims[1] + y0 + j]);
+        } else {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++)
+                    for (size_t k = 0; k <= sbs; k++)
+                        b.push_back(data[((x0 + i) * dims[1] + y0 + j) * dims[2] + z0 + k]);
+        }
+        if (out->n_blocks < 4096) {
+            out->block_starts[out->n_blocks][0] = x0;
+            out->block_starts[out->n_blocks][1] = y0;
+            out->block_starts[out->n_blocks][2] = z0;
+        }
+        out->n_blocks++;
+        ts.blocks.push_back(std::move(b));
+    };
+    if (N == 2) {
+        for (size_t x = 0; x < dims[0] - sbs; x += sbs)
+            for (size_t y = 0; y < dims[1] - sbs; y += sbs) {
+                if (idx % stride == 0) take(x, y, 0);
+                idx++;
+            }
+    } else if (N == 3) {
+        for (size_t x = 0; x < dims[0] - sbs; x += sbs)
+            for (size_t y = 0; y < dims[1] - sbs; y += sbs)
+                for (size_t z = 0; z < dims[2] - sbs; z += sbs) {
+                    if (idx % stride == 0) take(x, y, z);
+                    idx++;
+                }
+    }
+}
+
+// setABwithRelBound, configuration 0 (SZI:1394-1414)
+static void set_ab(double rel, double &a, double &b) {
+    if (rel >= 0.01) a = 2, b = 2;
+    else if (rel >= 0.007) a = 1.75, b = 2;
+    else if (rel >= 0.004) a = 1.5, b = 2;
+    else if (rel > 0.001) a = 1.25, b = 1.5;
+    else a = 1, b = 1;
+}
+
+// Summed mode-2 loss of one candidate over the sampled blocks (SZI:1952-1975).
+static double predictor_loss(const TuneState &ts, oracle_interp_params p, int start, int end) {
+    double loss = 0;
+    for (const auto &blk: ts.blocks) {
+        std::vector<float> cur(blk);
+        InterpEngine e;
+        e.set_params(&p);
+        std::vector<size_t> lc;
+        loss += e.compress(cur.data(), 2, start, end, lc);
+    }
+    return loss;
+}
+
+// CompressTest for ALGO_INTERP, RD target (SZI:1131-1392): mode-1 runs over
+// the blocks with one quantizer (outliers accumulate), codes interleaved
+// level by level across blocks, encoding_lossless (SIC:784-809) through
+// zstd, bitrate and PSNR.
+static void compress_test(const TuneState &ts, const oracle_interp_params &p, double rng, double &bitrate,
+                          double &psnr, size_t &bytes, std::vector<uint8_t> *raw_out = nullptr,
+                          double *sq_out = nullptr) {
+    double square_error = 0;
+    std::vector<std::vector<int>> bq;
+    std::vector<size_t> counts;
+    std::vector<float> unpred;
+    for (const auto &blk: ts.blocks) {
+        std::vector<float> cur(blk);
+        InterpEngine e;
+        e.set_params(&p);
+        e.quantizer.radius = 32768;  // LinearQuantizer<T>(eb): default radius
+        square_error += e.compress(cur.data(), 1, 0, 0, counts);
+        bq.push_back(e.codes);
+        unpred.insert(unpred.end(), e.quantizer.unpred.begin(), e.quantizer.unpred.end());
+    }
+    std::vector<int> q;
+    size_t last = 0;
+    for (int k = (int) counts.size() - 1; k >= 0; k--) {
+        for (size_t l = 0; l < bq.size(); l++)
+            for (size_t m = last; m < counts[k]; m++) q.push_back(bq[l][m]);
+        last = counts[k];
+    }
+    std::vector<uint8_t> raw;
+    raw.push_back(2);
+    put(raw, p.eb);
+    put(raw, (int) 32768);
+    put(raw, (size_t) unpred.size());
+    const uint8_t *u = (const uint8_t *) unpred.data();
+    raw.insert(raw.end(), u, u + unpred.size() * sizeof(float));
+    Huffman h;
+    h.init(q.data(), q.size());
+    h.save(raw);
+    h.encode(q.data(), q.size(), raw);
+    std::vector<uint8_t> z(ZSTD_compressBound(raw.size()));
+    size_t zs = ZSTD_compress(z.data(), z.size(), raw.data(), raw.size(), 3);
+    bytes = 8 + zs;
+    if (raw_out) *raw_out = raw;
+    if (sq_out) *sq_out = square_error;
+    size_t per = 1;
+    for (size_t d: ts.bdims) per *= d;
+    const size_t ele = ts.blocks.size() * per;
+    bitrate = 8 * double(bytes) / ele;
+    double mse = square_error / ele;
+    psnr = 20 * std::log10(rng) - 10 * std::log10(mse);
+}
+
+static int qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out *out) {
+    const int N = in->N;
+    if (N != 2 && N != 3) return -1;
+    memset(out, 0, sizeof(*out));
+    size_t num = 1;
+    for (int i = 0; i < N; i++) num *= in->dims[i];
+    const float rngf = (float) in->rng;  // Tuning keeps rng as T (SZI:1590)
+    const double rel = in->rel_eb > 0 ? in->rel_eb : in->abs_eb / rngf;
+    // QoZ defaults (SZI:1594-1613)
+    const double rate = N == 2 ? 0.01 : 0.005;
+    size_t max_step = in->max_step > 0 ? in->max_step : (N == 2 ? 64 : 32);
+    int levelwise = in->levelwise > 0 ? in->levelwise : (N == 2 ? 6 : 4);
+    long sbs = in->sample_block_size > 0 ? in->sample_block_size : (N == 2 ? 64 : 32);
+    int max_interp_level = -1;
+    for (int i = 0; i < N; i++)
+        if (max_interp_level < std::ceil(std::log2((double) in->dims[i])))
+            max_interp_level = (unsigned) std::ceil(std::log2((double) in->dims[i]));
+    if (max_step > 0) {
+        int t = (unsigned) std::log2((double) max_step);
+        if (t <= max_interp_level) max_interp_level = t;
+        if (levelwise > max_interp_level) levelwise = max_interp_level;
+    }
+    size_t shortest = in->dims[0];
+    for (int i = 0; i < N; i++) shortest = std::min(shortest, in->dims[i]);
+    // sample block size (SZI:1776-1784)
+    const long minimum_sbs = 16;
+    if (sbs < minimum_sbs) sbs = minimum_sbs;
+    while ((size_t) sbs >= shortest) sbs /= 2;
+    while (rate > 0 && sbs >= 2 * minimum_sbs && (std::pow((double) sbs, N) / (double) num) > 1.5 * rate) sbs /= 2;
+    TuneState ts;
+    ts.N = N;
+    sample_blocks(data, N, in->dims, (size_t) sbs, rate, ts, out);
+    ts.bdims.assign(N, (size_t) sbs + 1);
+    oracle_interp_params p;
+    memset(&p, 0, sizeof(p));
+    p.N = N;
+    for (int i = 0; i < N; i++) p.dims[i] = (size_t) sbs + 1;
+    p.eb = in->abs_eb;
+    p.max_step = max_step;
+    p.block_size = in->interp_block_size;
+    p.radius = 32768;
+    double a0, b0;
+    set_ab(rel, a0, b0);
+    p.alpha = a0, p.beta = b0;
+    const int ndir = N == 2 ? 2 : 6;
+    const int dir_cands[2] = {0, ndir - 1};
+    std::vector<uint8_t> ops(levelwise), dirs(levelwise);
+    // predictor tuning (SZI:1938-2000)
+    for (int level = levelwise; level > 0; level--) {
+        int start = level == levelwise ? 9999 : level, end = level - 1;
+        int best_op = 1, best_dir = 0;
+        double best = std::numeric_limits<double>::max();
+        for (int op = 0; op < 2; op++)
+            for (int di = 0; di < 2; di++) {
+                p.interp_algo = op;
+                p.direction = dir_cands[di];
+                double loss = predictor_loss(ts, p, start, end);
+                out->pred_loss[level - 1][op * 2 + di] = loss;
+                if (loss < best) {
+                    best = loss;
+                    best_op = op;
+                    best_dir = dir_cands[di];
+                }
+            }
+        ops[level - 1] = (uint8_t) best_op;
+        dirs[level - 1] = (uint8_t) best_dir;
+    }
+    out->interp_algo = p.interp_algo;  // the last candidate stays in the Config
+    out->direction = p.direction;
+    // alpha / beta search (SZI:2340-2420)
+    p.levelwise = levelwise;
+    memcpy(p.ops, ops.data(), levelwise);
+    memcpy(p.dirs, dirs.data(), levelwise);
+    const double alphas[5] = {1, 1.25, 1.5, 1.75, 2};
+    const double betas[4] = {1.5, 2, 3, 4};
+    double best_a = 1, best_b = 1, bestb = 9999, bestm = 0;
+    auto test = [&](double a, double b, double eb, double &br, double &ps) {
+        oracle_interp_params q = p;
+        q.alpha = a, q.beta = b, q.eb = eb;
+        size_t bytes;
+        compress_test(ts, q, in->rng, br, ps, bytes);
+        int t = out->n_tests;
+        if (t < 128) {
+            out->test_alpha[t] = a, out->test_beta[t] = b, out->test_eb[t] = eb;
+            out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = bytes;
+        }
+        out->n_tests++;
+    };
+    for (int i = 0; i < 5; i++) {
+        for (int j = 0; j < 4; j++) {
+            const double alpha = alphas[i], beta = betas[j];
+            if (alpha >= 1 && alpha > beta) continue;
+            double bitrate, metric;
+            test(alpha, beta, in->abs_eb, bitrate, metric);
+            const bool stop = (alpha >= 1 && std::pow(alpha, max_interp_level - 1) <= beta) ||
+                              (alpha < 1 && alpha * (max_interp_level - 1) <= beta);
+            if (metric >= bestm && bitrate <= bestb) {
+                best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
+            } else if (metric <= bestm && bitrate >= bestb) {
+                if (stop) break;
+                continue;
+            } else {
+                const double fix = bitrate / bestb;
+                double br_r, m_r;
+                test(alpha, beta, in->abs_eb * fix, br_r, m_r);
+                const double a = (metric - m_r) / (bitrate - br_r);
+                const double b = metric - a * bitrate;
+                const double reg = a * bestb + b;
+                if (reg > bestm) best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
+            }
+            if (stop) break;
+        }
+    }
+    out->alpha = best_a;
+    out->beta = best_b;
+    out->levelwise = levelwise;
+    memcpy(out->ops, ops.data(), levelwise);
+    memcpy(out->dirs, dirs.data(), levelwise);
+    out->max_step = max_step;
+    out->sample_block_size = (int) sbs;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1467,6 +1720,47 @@ int oracle_lr_compress_stream(const oracle_lr_params *p, int rel_mode, double eb
     if (s.size() > cap) return -3;
     memcpy(out, s.data(), s.size());
     return 0;
+}
+
+int oracle_compress_test(const oracle_interp_params *p, const float *field, const size_t *fdims, const size_t *starts,
+                         size_t nblocks, size_t sbs, double rng, uint8_t *raw, size_t cap, size_t *raw_size,
+                         size_t *bytes, double *square_error) {
+    if (!p || (p->N != 2 && p->N != 3)) return -1;
+    TuneState ts;
+    ts.N = p->N;
+    ts.bdims.assign(p->N, sbs + 1);
+    oracle_tune_out *tmp = new oracle_tune_out;
+    memset(tmp, 0, sizeof(*tmp));
+    for (size_t b = 0; b < nblocks; b++) {
+        TuneState one;
+        size_t d[3] = {fdims[0], fdims[1], p->N == 3 ? fdims[2] : 1};
+        // one block at the given start: the sampling walk with a stride of 1 over a 1-block grid
+        std::vector<float> blk;
+        const size_t *s0 = starts + 3 * b;
+        if (p->N == 2) {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++) blk.push_back(field[(s0[0] + i) * d[1] + s0[1] + j]);
+        } else {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++)
+                    for (size_t k = 0; k <= sbs; k++)
+                        blk.push_back(field[((s0[0] + i) * d[1] + s0[1] + j) * d[2] + s0[2] + k]);
+        }
+        ts.blocks.push_back(std::move(blk));
+    }
+    delete tmp;
+    double br, ps;
+    std::vector<uint8_t> r;
+    compress_test(ts, *p, rng, br, ps, *bytes, &r, square_error);
+    *raw_size = r.size();
+    if (r.size() > cap) return -3;
+    memcpy(raw, r.data(), r.size());
+    return 0;
+}
+
+int oracle_qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out *out) {
+    if (!in || !data || !out) return -1;
+    return qoz_tune(in, data, out);
 }
 
 int oracle_range(const float *data, size_t n, float *mn, float *mx) {

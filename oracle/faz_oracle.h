@@ -98,6 +98,47 @@ int oracle_lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, f
 int oracle_lr_compress_stream(const oracle_lr_params *p, int rel_mode, double eb_in, const float *data,
                               uint8_t *out, size_t cap, size_t *n_out);
 
+/* QoZ exhaustive sampling tuner (`faz -q`): Tuning() with conf.QoZ set,
+ * no wavelets, tuningTarget RD (include/QoZ/api/impl/SZInterp.hpp:1589-2557),
+ * with sampleBlocks (SZI:993-1127), CompressTest (SZI:1131-1392),
+ * setABwithRelBound (SZI:1394-1468) and init_alphalist/betalist
+ * (SZI:869-933).  N = 2 or 3. */
+typedef struct {
+    int N;
+    size_t dims[4];
+    double abs_eb;            /* conf.absErrorBound after calAbsErrorBound */
+    double rel_eb;            /* conf.relErrorBound when Tuning starts (> 0) */
+    double rng;               /* conf.rng */
+    int interp_block_size;    /* conf.interpBlockSize */
+    size_t max_step;          /* conf.maxStep (0: QoZ default) */
+    int levelwise;            /* conf.levelwisePredictionSelection (0: QoZ default) */
+    int sample_block_size;    /* conf.sampleBlockSize (0: QoZ default) */
+} oracle_tune_in;
+
+typedef struct {
+    double alpha, beta;       /* chosen */
+    int levelwise;
+    uint8_t ops[32], dirs[32];
+    int interp_algo, direction;   /* conf.interpAlgo / interpDirection left by the predictor loop */
+    size_t max_step;
+    int sample_block_size;
+    size_t n_blocks;          /* sampled blocks */
+    size_t block_starts[4096][3];
+    double pred_loss[32][4];  /* [level-1][op*2 + (dir != 0)] summed mode-2 loss */
+    int n_tests;              /* CompressTest calls in order */
+    double test_alpha[128], test_beta[128], test_eb[128], test_bitrate[128], test_psnr[128];
+    size_t test_bytes[128];   /* compressed bytes of encoding_lossless */
+} oracle_tune_out;
+
+int oracle_qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out *out);
+
+/* CompressTest (ALGO_INTERP, RD) on the blocks of (sbs+1)^N values starting
+ * at starts[3*b..] of a field: the pre-zstd encoding_lossless buffer, the
+ * compressed size and the summed squared error. */
+int oracle_compress_test(const oracle_interp_params *p, const float *field, const size_t *fdims, const size_t *starts,
+                         size_t nblocks, size_t sbs, double rng, uint8_t *raw, size_t cap, size_t *raw_size,
+                         size_t *bytes, double *square_error);
+
 #ifdef __cplusplus
 }
 #endif

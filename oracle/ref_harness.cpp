@@ -345,6 +345,70 @@ static int run_lorenzo(const Args &a) {
     return 0;
 }
 
+// CompressTest for ALGO_INTERP with the RD target (SZInterp.hpp:1131-1392),
+// composed from the reference classes it uses: one SZInterpolationCompressor
+// (LinearQuantizer(eb), HuffmanEncoder<int>, Lossless_zstd) compresses every
+// block in tuning mode 1, the codes are interleaved level by level across the
+// blocks and encoding_lossless (SIC:784-809) gives the compressed size.
+// blocks: a file of int64 start triples; prints bytes and the squared error.
+template<QoZ::uint N>
+static int run_ctest(const Args &a) {
+    typedef float T;
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    std::vector<size_t> fdims = parse_list(get(a, "fdims"));
+    std::vector<int64_t> starts = read_vec<int64_t>(get(a, "starts"));
+    size_t sbs = strtoull(get(a, "sbs").c_str(), nullptr, 10);
+    QoZ::Config conf = make_config(a);  // dims = block shape
+    conf.verbose = 0;
+    size_t per = conf.num;
+    std::vector<std::vector<T>> blocks;
+    for (size_t b = 0; b + 2 < starts.size(); b += 3) {
+        std::vector<T> blk;
+        if (N == 2) {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++) blk.push_back(input[(starts[b] + i) * fdims[1] + starts[b + 1] + j]);
+        } else {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++)
+                    for (size_t k = 0; k <= sbs; k++)
+                        blk.push_back(input[((starts[b] + i) * fdims[1] + starts[b + 1] + j) * fdims[2] + starts[b + 2] + k]);
+        }
+        blocks.push_back(blk);
+    }
+    auto sz = QoZ::SZInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>, QoZ::Lossless_zstd>(
+            QoZ::LinearQuantizer<T>(conf.absErrorBound), QoZ::HuffmanEncoder<int>(), QoZ::Lossless_zstd());
+    double square_error = 0;
+    std::vector<std::vector<int>> bq;
+    for (auto &blk: blocks) {
+        std::vector<T> cur(blk);
+        size_t outSize = 0;
+        QoZ::uchar *d = sz.compress(conf, cur.data(), outSize, 1);
+        delete[] d;
+        bq.push_back(conf.quant_bins);
+        square_error += conf.decomp_square_error;
+    }
+    std::vector<size_t> counts = conf.quant_bin_counts;
+    std::vector<int> q;
+    size_t last = 0;
+    for (int k = (int) counts.size() - 1; k >= 0; k--) {
+        for (size_t l = 0; l < bq.size(); l++)
+            for (size_t m = last; m < counts[k]; m++) q.push_back(bq[l][m]);
+        last = counts[k];
+    }
+    size_t total = 0;
+    QoZ::uchar *c = sz.encoding_lossless(total, q);
+    if (a.count("raw")) {  // the pre-zstd buffer, recovered from the zstd frame
+        size_t len = total;
+        QoZ::Lossless_zstd z;
+        QoZ::uchar *raw = z.decompress(c, len);
+        write_bytes(get(a, "raw"), raw, len);
+        delete[] raw;
+    }
+    delete[] c;
+    printf("{\"bytes\": %zu, \"square_error\": %.17g, \"points\": %zu}\n", total, square_error, per * blocks.size());
+    return 0;
+}
+
 static int run_huffman(const Args &a) {
     std::vector<int> codes = read_vec<int>(get(a, "in"));
     QoZ::HuffmanEncoder<int> enc;
@@ -387,6 +451,7 @@ int main(int argc, char **argv) {
     if (cmd == "decompress") return run_decompress(a);
     if (cmd == "huffman") return run_huffman(a);
     if (cmd == "lorenzo") return run_lorenzo(a);
+    if (cmd == "ctest") return parse_list(get(a, "dims")).size() == 2 ? run_ctest<2>(a) : run_ctest<3>(a);
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

@@ -133,6 +133,27 @@ def lorenzo_cases():
     return c
 
 
+def ctest_cases():
+    """CompressTest fixtures: a field, sample-block starts, the block edge and
+    the engine parameters of one tuner candidate."""
+    r = np.random.default_rng(99)
+    c = []
+    f3 = smooth(r, (34, 34, 51), 2)
+    c.append(("q3d_17_lw", f3, [(0, 0, 0), (16, 16, 16), (0, 16, 32), (16, 0, 34)], 16,
+              dict(eb=1e-3, alpha=1.25, beta=2.0, maxstep=16, ops=[1, 0, 1, 1], dirs=[0, 5, 5, 0])))
+    c.append(("q3d_17_noanchor", f3, [(0, 0, 0), (17, 17, 17)], 16,
+              dict(eb=2e-4, alpha=1.5, beta=3.0, maxstep=0, algo=1, dir=3)))
+    c.append(("q3d_17_small_eb", f3, [(5, 6, 7), (16, 16, 30)], 16,
+              dict(eb=3e-6, alpha=1.0, beta=1.5, maxstep=16, ops=[1, 1, 1, 1], dirs=[0, 0, 0, 0])))
+    f33 = smooth(r, (66, 34, 34), 2)
+    c.append(("q3d_33_qoz", f33, [(0, 0, 0), (32, 0, 0), (33, 1, 1)], 32,
+              dict(eb=5e-4, alpha=1.75, beta=4.0, maxstep=32, ops=[0, 1, 1, 0], dirs=[5, 0, 5, 0])))
+    f2 = smooth(r, (130, 131), 2)
+    c.append(("q2d_65_qoz", f2, [(0, 0), (64, 64), (65, 0)], 64,
+              dict(eb=1e-3, alpha=1.25, beta=1.5, maxstep=64, ops=[1, 0, 1, 1, 0, 1], dirs=[0, 1, 1, 0, 0, 1])))
+    return c
+
+
 def harness_args(inp, out, dims, kw):
     a = [HARNESS, "interp", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, dims)),
          f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"algo={kw.get('algo', 1)}",
@@ -182,6 +203,25 @@ def main():
                             codes=ld(".codes", np.int32), recon=ld(".recon", np.float32),
                             raw=ld(".raw", np.uint8), stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
                             params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
+        print("wrote", name)
+    for name, x, starts, sbs, kw in ctest_cases():
+        inp = os.path.join(tmp, "field.f32")
+        st = os.path.join(tmp, "starts.i64")
+        raw = os.path.join(tmp, "ct.raw")
+        x.astype(np.float32).tofile(inp)
+        s3 = np.array([list(t) + [0] * (3 - len(t)) for t in starts], np.int64)
+        s3.tofile(st)
+        bd = ",".join([str(sbs + 1)] * x.ndim)
+        a = [HARNESS, "ctest", f"in={inp}", "fdims=" + ",".join(map(str, x.shape)), f"starts={st}", f"sbs={sbs}",
+             "dims=" + bd, "mode=abs", f"eb={kw['eb']!r}", f"alpha={kw['alpha']!r}", f"beta={kw['beta']!r}",
+             f"maxstep={kw['maxstep']}", f"algo={kw.get('algo', 1)}", f"dir={kw.get('dir', 0)}", f"raw={raw}"]
+        if kw.get("ops"):
+            a += ["ops=" + ",".join(map(str, kw["ops"])), "dirs=" + ",".join(map(str, kw["dirs"]))]
+        res = json.loads(subprocess.check_output(a))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32), starts=s3,
+                            raw=np.fromfile(raw, np.uint8), square_error=np.array([res["square_error"]]),
+                            ref_zstd_bytes=np.array([res["bytes"]]),
+                            params=np.array(json.dumps(dict(kw, sbs=sbs, dims=list(x.shape)))))
         print("wrote", name)
     for name, x, kw in tuning_cases():
         inp = os.path.join(tmp, "in.f32")

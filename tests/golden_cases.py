@@ -16,6 +16,7 @@ INTERP = names("c")
 HUFFMAN = names("h_")
 TUNING = names("t")
 LORENZO = names("l")
+CTEST = names("q")
 
 
 def load(name):
@@ -63,3 +64,13 @@ def lr_raw_equal(a, b):
     a[list(LR_RAW_PADDING)] = 0
     b[list(LR_RAW_PADDING)] = 0
     return bool(np.array_equal(a, b))
+
+
+def ctest_params(case, make):
+    """Engine parameters (for `make`: oracle.make_params or faz_amd.make_params)
+    of a CompressTest fixture: the sample-block shape and the candidate."""
+    p = case["params"]
+    n = len(p["dims"])
+    return make((p["sbs"] + 1,) * n, p["eb"], algo=p.get("algo", 1), direction=p.get("dir", 0), alpha=p["alpha"],
+                beta=p["beta"], max_step=p["maxstep"], block_size=32, radius=32768, ops=tuple(p.get("ops", ())),
+                dirs=tuple(p.get("dirs", ())))

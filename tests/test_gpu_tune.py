@@ -1,0 +1,68 @@
+"""GPU parity of the exhaustive QoZ tuner (`faz -q`, needs an MI355X).
+
+Bar: the same sample blocks, bit-identical summed mode-2 losses for every
+(level, interpolator, direction), the same sequence of CompressTest calls with
+bit-identical bitrates and PSNRs (same compressed sizes), and the same final
+alpha, beta and level-wise lists as the oracle's restatement of Tuning().
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _grf(shape, slope=11 / 3, seed=0, fronts=0):
+    r = np.random.default_rng(seed)
+    k = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
+    kk = np.sqrt(sum(a * a for a in k))
+    kk[(0,) * len(shape)] = 1
+    f = (r.standard_normal(shape) + 1j * r.standard_normal(shape)) * kk ** (-slope / 2)
+    f[(0,) * len(shape)] = 0
+    x = np.real(np.fft.ifftn(f))
+    g = np.stack(np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij"))
+    for _ in range(fronts):
+        nrm = r.standard_normal(len(shape))
+        nrm /= np.linalg.norm(nrm)
+        x += 0.5 * x.std() * np.tanh((np.tensordot(nrm, g, 1) - r.uniform(0, max(shape))) / 4.0)
+    x = (x - x.min()) / (x.max() - x.min())
+    return (x + 1e-4 * r.standard_normal(shape)).astype(np.float32)
+
+
+CASES = [
+    ((96, 96, 96), 1e-3, 0),
+    ((200, 200, 200), 1e-3, 4),
+    ((130, 140, 150), 1e-4, 2),
+    ((300, 400), 1e-3, 0),
+    ((64, 80, 300), 1e-2, 1),
+    ((160, 160, 160), 1e-5, 3),
+]
+
+
+def _compare(g, o):
+    assert g["sample_block_size"] == o["sample_block_size"]
+    assert g["n_blocks"] == o["n_blocks"]
+    assert np.array_equal(g["block_starts"], o["block_starts"])
+    assert np.array_equal(g["pred_loss"], o["pred_loss"])
+    assert len(g["tests"]) == len(o["tests"])
+    for a, b in zip(g["tests"], o["tests"]):
+        assert (a["alpha"], a["beta"], a["eb"]) == (b["alpha"], b["beta"], b["eb"])
+        assert a["bytes"] == b["bytes"]
+        assert a["bitrate"] == b["bitrate"] and a["psnr"] == b["psnr"]
+    for k in ("alpha", "beta", "levelwise", "ops", "dirs", "interp_algo", "direction", "max_step"):
+        assert g[k] == o[k], k
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_tuner_matches_oracle(engine, i):
+    shape, rel, fronts = CASES[i]
+    x = _grf(shape, seed=40 + i, fronts=fronts)
+    rng = float(np.float32(x.max() - x.min()))
+    eb = rel * rng
+    print(f"[gpu] tuner {shape} rel={rel}", flush=True)
+    o = oracle.qoz_tune(x, eb, rel, rng)
+    g = engine.tune_qoz(torch.from_numpy(x.ravel()).cuda(), shape, eb, rel, rng)
+    _compare(g, o)

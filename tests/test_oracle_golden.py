@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import HUFFMAN, INTERP, LORENZO, TUNING, load, lr_kwargs, lr_raw_equal, param_kwargs
+from golden_cases import (CTEST, HUFFMAN, INTERP, LORENZO, TUNING, ctest_params, load, lr_kwargs, lr_raw_equal,
+                          param_kwargs)
 
 
 def _params(case):
@@ -139,3 +140,17 @@ def test_lorenzo_fixtures_cover_every_branch():
         reg_unpred += r["reg_unpred"].size
         thin += any(d % kw["block_size"] == 1 or d == 1 for d in kw["dims"])
     assert (seen > 10).all() and unpred > 100 and reg_unpred > 10 and thin >= 2
+
+
+@pytest.mark.parametrize("name", CTEST)
+def test_compress_test_buffer(name):
+    """CompressTest (SZInterp.hpp:1131-1392): the encoding_lossless buffer the
+    reference hands to zstd and the summed squared error, exactly; the zstd
+    frame size of the reference's vendored zstd is within 0.5%."""
+    c = load(name)
+    p = c["params"]
+    raw, nbytes, sq = oracle.compress_test(c["input"], c["starts"], p["sbs"], ctest_params(c, oracle.make_params),
+                                           1.0)
+    assert raw == c["raw"].tobytes()
+    assert sq == float(c["square_error"][0])
+    assert abs(nbytes - int(c["ref_zstd_bytes"][0])) <= 0.005 * nbytes + 8
