@@ -195,7 +195,7 @@ EXPORTS = [
     "faz_gpu_profile_kernel",
     "faz_gpu_profile_bytes",
     "faz_gpu_lorenzo_compress_f32", "faz_gpu_lorenzo_raw_f32", "faz_gpu_lorenzo_decompress_raw",
-    "faz_sz_compress_lorenzo_f32", "faz_gpu_tune_qoz_f32",
+    "faz_sz_compress_lorenzo_f32", "faz_gpu_tune_qoz_f32", "faz_gpu_compress_test_f32",
 ]
 
 _lib = None
@@ -243,6 +243,8 @@ def lib():
         L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
         L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
+        L.faz_gpu_compress_test_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, sz, ctypes.c_double, vp, sz,
+                                                P(sz), P(sz), P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -515,3 +517,18 @@ class Engine:
         o = TuneOut()
         self._check(self._L.faz_gpu_tune_qoz_f32(self.h, self._dev(x), ctypes.byref(t), ctypes.byref(o)), "tune_qoz")
         return tune_out_dict(o)
+
+    def compress_test(self, x, shape, starts, sbs, params, rng):
+        """One CompressTest: (pre-zstd buffer, compressed size, squared error)."""
+        import torch
+        torch.cuda.synchronize()
+        fd = np.array(list(shape) + [1] * (3 - len(shape)), np.uint64)
+        st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+        cap = 64 * 1024 + st.shape[0] * (sbs + 1) ** len(shape) * 16
+        raw = np.zeros(cap, np.uint8)
+        rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+        self._check(self._L.faz_gpu_compress_test_f32(self.h, self._dev(x), _np_ptr(fd), ctypes.byref(params),
+                                                      _np_ptr(st), st.shape[0], sbs, float(rng), _np_ptr(raw), cap,
+                                                      ctypes.byref(rs), ctypes.byref(by), ctypes.byref(sq)),
+                    "compress_test")
+        return raw[:rs.value].tobytes(), by.value, sq.value

@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(kTuneThreads) tune_kernel(TuneGeom G, const Tu
         for (int a = 0; a < G.N; a++) f += (uint64_t) x[a] * G.fstr[a];
         s_v[q] = field[f];
     }
+    __syncthreads();
     const uint64_t cbase = (uint64_t) blockIdx.x * G.ncodes;
     if (tid == 0) {
         s_total = 0.0;
@@ -461,7 +462,8 @@ struct Tuner {
     // CompressTest (SZI:1131-1392) for one mode-1 candidate already run as
     // candidate 0: outliers in block order, codes interleaved by level,
     // encoding_lossless (SIC:784-809) through zstd.
-    int compress_test(double eb, double &bitrate, double &psnr, size_t &bytes) {
+    int compress_test(double eb, double &bitrate, double &psnr, size_t &bytes,
+                      std::vector<uint8_t> *raw_out = nullptr, double *sq_out = nullptr) {
         hipStream_t st = ctx->stream;
         const uint32_t nb = (uint32_t) origins.size();
         const uint64_t n = (uint64_t) nb * pl.G.ncodes;
@@ -543,6 +545,8 @@ struct Tuner {
         const size_t zs = ZSTD_compress(z.data(), z.size(), raw.data(), raw.size(), 3);
         if (ZSTD_isError(zs)) CTX_FAIL(FAZ_ERR_INVALID, "tuner: zstd failed");
         bytes = 8 + zs;
+        if (raw_out) raw_out->swap(raw);
+        if (sq_out) *sq_out = square_error;
         size_t per = 1;
         for (int a = 0; a < pl.G.N; a++) per *= pl.G.bd[a];
         const size_t ele = origins.size() * per;
@@ -740,6 +744,66 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
 }
 
 }  // namespace faz
+
+namespace faz {
+
+// CompressTest alone on blocks at the given starts.
+int compress_test_at(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims, const faz_interp_params *p,
+                     const size_t *starts, size_t nblocks, size_t sbs, double rng, std::vector<uint8_t> &raw,
+                     size_t &bytes, double &sq) {
+    const int N = p->N;
+    if (N != 2 && N != 3) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "CompressTest samples 2-D and 3-D fields only");
+    if (sbs < 1 || (N == 3 && sbs + 1 > 33) || (N == 2 && (sbs + 1) * (sbs + 1) > kTuneMaxPoints))
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "sample blocks of edge %zu exceed the LDS-resident tuner", sbs + 1);
+    if (nblocks == 0) CTX_FAIL(FAZ_ERR_INVALID, "no blocks");
+    HIPCHECK(hipSetDevice(ctx->device));
+    uint64_t fstr[3] = {0, 0, 0};
+    fstr[N - 1] = 1;
+    for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * fdims[a + 1];
+    Tuner T;
+    T.ctx = ctx;
+    T.d_field = d_in;
+    T.rng = rng;
+    make_tune_plan(N, (uint32_t) sbs + 1, fstr, p->max_step, p->block_size, T.pl);
+    if (T.pl.G.levels > kTuneMaxLevels || T.pl.G.max_lines > kTuneMaxLines)
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "sample block outside the tuner's LDS tables");
+    for (size_t b = 0; b < nblocks; b++) {
+        for (int a = 0; a < N; a++)
+            if (starts[3 * b + a] + sbs >= fdims[a]) CTX_FAIL(FAZ_ERR_INVALID, "block %zu outside the field", b);
+        T.origins.push_back(starts[3 * b] * fstr[0] + starts[3 * b + 1] * fstr[1] + (N == 3 ? starts[3 * b + 2] : 0));
+    }
+    HIPCHECK(T.d_origins.ensure(T.origins.size()));
+    HIPCHECK(hipMemcpyAsync(T.d_origins.p, T.origins.data(), T.origins.size() * 8, hipMemcpyHostToDevice,
+                            ctx->stream));
+    TuneCand c;
+    memset(&c, 0, sizeof(c));
+    std::vector<uint8_t> ops(p->ops, p->ops + p->levelwise), dirs(p->dirs, p->dirs + p->levelwise);
+    fill_levels(c, T.pl.G.levels, p->eb, p->alpha, p->beta, p->interp_algo, p->direction, ops, dirs);
+    set_range(c, T.pl.G, 1, 0, 0);
+    int rc = T.run(std::vector<TuneCand>{c}, true);
+    if (rc) return rc;
+    double br, ps;
+    return T.compress_test(p->eb, br, ps, bytes, &raw, &sq);
+}
+
+}  // namespace faz
+
+extern "C" faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims,
+                                                const faz_interp_params *p, const size_t *starts, size_t nblocks,
+                                                size_t sbs, double rng, uint8_t *h_raw, size_t cap, size_t *raw_size,
+                                                size_t *bytes, double *square_error) {
+    if (!ctx || !d_in || !fdims || !p || !starts || !raw_size || !bytes || !square_error) return FAZ_ERR_INVALID;
+    std::vector<uint8_t> raw;
+    int rc = faz::compress_test_at(ctx, d_in, fdims, p, starts, nblocks, sbs, rng, raw, *bytes, *square_error);
+    if (rc) return rc;
+    *raw_size = raw.size();
+    if (!h_raw || cap < raw.size()) {
+        ctx->err = "raw buffer too small";
+        return FAZ_ERR_CAPACITY;
+    }
+    memcpy(h_raw, raw.data(), raw.size());
+    return FAZ_OK;
+}
 
 extern "C" faz_status faz_gpu_tune_qoz_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in,
                                            faz_tune_out *out) {

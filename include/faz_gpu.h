@@ -237,6 +237,17 @@ typedef struct {
 
 faz_status faz_gpu_tune_qoz_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tune_out *out);
 
+/* One CompressTest (SZI:1131-1392, ALGO_INTERP, RD) on the sample blocks of
+ * edge sbs+1 starting at starts[3*b..3*b+2] (b < nblocks) of the device field
+ * d_in (dims fdims, p->N of them): every block in tuning mode 1 with p's
+ * engine parameters (p->dims is ignored), then encoding_lossless
+ * (SIC:784-809).  Returns the pre-zstd buffer (h_raw, capacity cap), the
+ * zstd-framed size (*bytes) and the summed squared error. */
+faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims,
+                                     const faz_interp_params *p, const size_t *starts, size_t nblocks, size_t sbs,
+                                     double rng, uint8_t *h_raw, size_t cap, size_t *raw_size, size_t *bytes,
+                                     double *square_error);
+
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,

@@ -66,3 +66,21 @@ def test_tuner_matches_oracle(engine, i):
     o = oracle.qoz_tune(x, eb, rel, rng)
     g = engine.tune_qoz(torch.from_numpy(x.ravel()).cuda(), shape, eb, rel, rng)
     _compare(g, o)
+
+
+from golden_cases import CTEST, ctest_params, load  # noqa: E402
+
+
+@pytest.mark.parametrize("name", CTEST)
+def test_compress_test_golden(engine, name):
+    """GPU CompressTest against the reference's encoding_lossless buffer and
+    squared error (fixtures from the reference classes)."""
+    import faz_amd
+    c = load(name)
+    p = c["params"]
+    x = c["input"]
+    print(f"[gpu] CompressTest {name}", flush=True)
+    raw, nbytes, sq = engine.compress_test(torch.from_numpy(x.ravel()).cuda(), x.shape, c["starts"], p["sbs"],
+                                           ctest_params(c, faz_amd.make_params), 1.0)
+    assert raw == c["raw"].tobytes()
+    assert sq == float(c["square_error"][0])
