@@ -196,6 +196,7 @@ EXPORTS = [
     "faz_gpu_profile_bytes",
     "faz_gpu_lorenzo_compress_f32", "faz_gpu_lorenzo_raw_f32", "faz_gpu_lorenzo_decompress_raw",
     "faz_sz_compress_lorenzo_f32", "faz_gpu_tune_qoz_f32", "faz_gpu_compress_test_f32",
+    "faz_sz_compress_qoz_f32",
 ]
 
 _lib = None
@@ -243,6 +244,8 @@ def lib():
         L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
         L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
+        L.faz_sz_compress_qoz_f32.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
+                                              P(vp), P(sz)]
         L.faz_gpu_compress_test_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, sz, ctypes.c_double, vp, sz,
                                                 P(sz), P(sz), P(ctypes.c_double)]
         _lib = L
@@ -532,3 +535,22 @@ class Engine:
                                                       ctypes.byref(rs), ctypes.byref(by), ctypes.byref(sq)),
                     "compress_test")
         return raw[:rs.value].tobytes(), by.value, sq.value
+
+    def sz_compress_qoz(self, data, shape, eb, rel=True):
+        """`faz -q` stream: QoZ tuner + interpolation engine (numpy host data or a device tensor)."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        d = np.array(list(shape), np.uint64)
+        if isinstance(data, np.ndarray):
+            a = np.ascontiguousarray(data, np.float32)
+            ptr, ondev = _np_ptr(a), 0
+        else:
+            import torch
+            torch.cuda.synchronize()
+            ptr, ondev = self._dev(data), 1
+        self._check(self._L.faz_sz_compress_qoz_f32(self.h, len(shape), _np_ptr(d), 1 if rel else 0, float(eb), ptr,
+                                                    ondev, ctypes.byref(p), ctypes.byref(n)), "sz_compress_qoz")
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            self._L.faz_free(p)

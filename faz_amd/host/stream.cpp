@@ -231,6 +231,76 @@ int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_
     return FAZ_OK;
 }
 
+// SZ_compress_Interp_lorenzo with conf.QoZ set (SZInterp.hpp:2562-2899): the
+// exhaustive QoZ tuner (Tuning, SZI:1589-2557) on the GPU, then the
+// interpolation engine with the tuned Config.
+int compress_qoz_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
+    if (!conf.QoZ) {
+        faz_ctx_set_error(ctx, "ALGO_INTERP_LORENZO is served with the QoZ tuner (conf.QoZ, faz -q); the SZ3 "
+                               "sampling tuner is outside the GPU path");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    if ((conf.N != 2 && conf.N != 3) || conf.wavelet > 0 || conf.tuningTarget != QoZ::TUNING_TARGET_RD) {
+        faz_ctx_set_error(ctx, "the GPU QoZ tuner serves 2-D and 3-D fields, no wavelets, the PSNR target");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    int rc = cal_abs_error_bound(ctx, conf, d_data);
+    if (rc) return rc;
+    if (conf.rng < 0) {  // data_range (Statistic.hpp:11-21)
+        float mn, mx;
+        if ((rc = faz_gpu_range_f32(ctx, d_data, conf.num, &mn, &mx))) return rc;
+        conf.rng = (double) (float) (mx - mn);
+    }
+    if (conf.relErrorBound <= 0) conf.relErrorBound = conf.absErrorBound / conf.rng;
+    // the QoZ settings Tuning applies (SZI:1594-1613)
+    const int N = conf.N;
+    if (conf.autoTuningRate <= 0) conf.autoTuningRate = N == 2 ? 0.01 : 0.005;
+    if (conf.predictorTuningRate <= 0) conf.predictorTuningRate = N == 2 ? 0.01 : 0.005;
+    if (conf.maxStep <= 0) conf.maxStep = N == 2 ? 64 : 32;
+    if (conf.levelwisePredictionSelection <= 0) conf.levelwisePredictionSelection = N == 2 ? 6 : 4;
+    if (conf.sampleBlockSize <= 0) conf.sampleBlockSize = N == 2 ? 64 : 32;
+    if (conf.autoTuningRate != (N == 2 ? 0.01 : 0.005) || conf.predictorTuningRate != conf.autoTuningRate) {
+        faz_ctx_set_error(ctx, "the GPU QoZ tuner uses the QoZ default tuning rates");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    conf.profiling = 0;
+    conf.var_first = 0;
+    conf.testLorenzo = 0;
+    conf.waveletAutoTuning = 0;
+    conf.waveletTest = 0;
+    conf.waveAutoFix = 0;
+    conf.sperr = -1;
+    conf.conditioning = 0;
+    conf.pyBind = 0;
+    conf.fixWave = -1;
+    conf.sperrWithoutWave = false;
+    faz_tune_in tin;
+    memset(&tin, 0, sizeof(tin));
+    tin.N = N;
+    for (int i = 0; i < N; i++) tin.dims[i] = conf.dims[i];
+    tin.abs_eb = conf.absErrorBound;
+    tin.rel_eb = conf.relErrorBound;
+    tin.rng = conf.rng;
+    tin.interp_block_size = conf.interpBlockSize;
+    tin.max_step = conf.maxStep;
+    tin.levelwise = conf.levelwisePredictionSelection;
+    tin.sample_block_size = conf.sampleBlockSize;
+    std::unique_ptr<faz_tune_out> tout(new faz_tune_out);
+    if ((rc = faz_gpu_tune_qoz_f32(ctx, d_data, &tin, tout.get()))) return rc;
+    conf.levelwisePredictionSelection = tout->levelwise;
+    conf.sampleBlockSize = tout->sample_block_size;
+    conf.interpAlgo = (uint8_t) tout->interp_algo;
+    conf.interpDirection = (uint8_t) tout->direction;
+    conf.interpAlgo_list.assign(tout->ops, tout->ops + tout->levelwise);
+    conf.interpDirection_list.assign(tout->dirs, tout->dirs + tout->levelwise);
+    conf.alpha = tout->alpha;
+    conf.beta = tout->beta;
+    conf.wavelet_rel_coeff = 1;
+    conf.wavelet = 0;
+    conf.cmprAlgo = QoZ::ALGO_INTERP;
+    return compress_interp_stream(ctx, conf, d_data, out);
+}
+
 int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Config &conf, float *d_out,
                       size_t num) {
     if (n < 4) return FAZ_ERR_FORMAT;
@@ -288,7 +358,8 @@ template<>
 char *SZ_compress<float>(QoZ::Config &config, const float *data, size_t &outSize) {
     QoZ::Config conf(config);
     if (conf.N < 1 || conf.N > 4) throw faz::Error(FAZ_ERR_INVALID, "Data dimension higher than 4 is not supported.");
-    if (conf.cmprAlgo != QoZ::ALGO_INTERP && conf.cmprAlgo != QoZ::ALGO_LORENZO_REG)
+    if (conf.cmprAlgo != QoZ::ALGO_INTERP && conf.cmprAlgo != QoZ::ALGO_LORENZO_REG &&
+        conf.cmprAlgo != QoZ::ALGO_INTERP_LORENZO)
         throw faz::Error(FAZ_ERR_UNSUPPORTED, std::string("GPU engine does not serve ") + QoZ::ALGO_STR[conf.cmprAlgo]);
     faz_gpu_ctx *ctx = api_ctx();
     int rc;
@@ -297,6 +368,8 @@ char *SZ_compress<float>(QoZ::Config &config, const float *data, size_t &outSize
     std::vector<uint8_t> out;
     if (conf.cmprAlgo == QoZ::ALGO_LORENZO_REG)
         rc = compress_lorenzo_stream(ctx, conf, d, out);
+    else if (conf.cmprAlgo == QoZ::ALGO_INTERP_LORENZO)
+        rc = compress_qoz_stream(ctx, conf, d, out);
     else
         rc = compress_interp_stream(ctx, conf, d, out);
     if (rc) fail(ctx, rc, "SZ_compress");
@@ -531,6 +604,34 @@ faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_param
     if (rc) return rc;
     std::vector<uint8_t> buf;
     rc = compress_lorenzo_stream(ctx, conf, d, buf);
+    if (rc) return rc;
+    uint8_t *m = (uint8_t *) malloc(buf.size());
+    if (!m) return FAZ_ERR_INVALID;
+    memcpy(m, buf.data(), buf.size());
+    *out = m;
+    *out_size = buf.size();
+    return FAZ_OK;
+}
+
+faz_status faz_sz_compress_qoz_f32(faz_gpu_ctx *ctx, int N, const size_t *dims, int rel_mode, double eb,
+                                   const float *data, int data_on_device, uint8_t **out, size_t *out_size) {
+    if (!ctx || !dims || !data || !out || !out_size || N < 1 || N > 4) return FAZ_ERR_INVALID;
+    QoZ::Config conf;
+    conf.setDims(dims, dims + N);
+    conf.cmprAlgo = QoZ::ALGO_INTERP_LORENZO;
+    conf.QoZ = 1;
+    if (rel_mode) {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = eb;
+    } else {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = eb;
+    }
+    int rc;
+    const float *d = to_device(ctx, data, conf.num, data_on_device != 0, rc);
+    if (rc) return rc;
+    std::vector<uint8_t> buf;
+    rc = compress_qoz_stream(ctx, conf, d, buf);
     if (rc) return rc;
     uint8_t *m = (uint8_t *) malloc(buf.size());
     if (!m) return FAZ_ERR_INVALID;

@@ -248,6 +248,12 @@ faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const 
                                      double rng, uint8_t *h_raw, size_t cap, size_t *raw_size, size_t *bytes,
                                      double *square_error);
 
+/* SZ_compress with cmprAlgo = ALGO_INTERP_LORENZO and conf.QoZ (`faz -q`,
+ * SZInterp.hpp:2562-2899): the QoZ tuner above, then the interpolation
+ * engine with the tuned Config (stream as faz_sz_compress_f32). */
+faz_status faz_sz_compress_qoz_f32(faz_gpu_ctx *ctx, int N, const size_t *dims, int rel_mode, double eb,
+                                   const float *data, int data_on_device, uint8_t **out, size_t *out_size);
+
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,

@@ -84,3 +84,50 @@ def test_compress_test_golden(engine, name):
                                            ctest_params(c, faz_amd.make_params), 1.0)
     assert raw == c["raw"].tobytes()
     assert sq == float(c["square_error"][0])
+
+
+def _trailer(stream):
+    """The leading Config fields of a stream's trailer (Config.hpp:240-270)."""
+    import struct
+    n = int(np.frombuffer(stream[-4:], np.int32)[0])
+    c = stream[-4 - n:-4]
+    N = c[0]
+    o = 1 + 8 * N + 8
+    algo = c[o]
+    o += 3
+    absb, relb, alpha, beta = struct.unpack_from("<4d", c, o)
+    o += 32 + 4
+    interp, direction = c[o], c[o + 1]
+    o += 2
+    ibs, = struct.unpack_from("<i", c, o)
+    o += 4 + 2
+    qbins, bs, lw = struct.unpack_from("<3i", c, o)
+    o += 12 + 1
+    stride, = struct.unpack_from("<i", c, o)
+    o += 4
+    max_step, = struct.unpack_from("<Q", c, o)
+    return dict(algo=algo, abs=absb, rel=relb, alpha=alpha, beta=beta, interp=interp, direction=direction, ibs=ibs,
+                levelwise=lw, max_step=max_step)
+
+
+@pytest.mark.parametrize("shape,rel", [((96, 96, 96), 1e-3), ((150, 170, 190), 1e-4), ((300, 400), 1e-3)])
+def test_qoz_stream(engine, shape, rel):
+    """faz -q: tuned Config in the trailer, engine buffer decodes on the oracle
+    to the oracle's reconstruction with the oracle tuner's parameters."""
+    x = _grf(shape, seed=77, fronts=2)
+    rng = float(np.float32(x.max() - x.min()))
+    eb = rel * rng
+    print(f"[gpu] qoz stream {shape} rel={rel}", flush=True)
+    s = engine.sz_compress_qoz(x, shape, rel, rel=True)
+    o = oracle.qoz_tune(x, eb, rel, rng)
+    t = _trailer(s)
+    assert t["algo"] == 2 and t["abs"] == eb and t["rel"] == rel
+    assert (t["alpha"], t["beta"], t["levelwise"], t["max_step"]) == (o["alpha"], o["beta"], o["levelwise"],
+                                                                       o["max_step"])
+    assert (t["interp"], t["direction"]) == (o["interp_algo"], o["direction"])
+    p = oracle.make_params(shape, eb, algo=o["interp_algo"], direction=o["direction"], alpha=o["alpha"],
+                           beta=o["beta"], max_step=o["max_step"], ops=tuple(o["ops"]), dirs=tuple(o["dirs"]))
+    ref = oracle.interp_compress(x, p)["recon"]
+    dec = oracle.decompress_stream(s, x.size)
+    assert np.array_equal(dec.view(np.uint32), ref.view(np.uint32))
+    assert np.array_equal(engine.sz_decompress(s, x.size).view(np.uint32), ref.view(np.uint32))
