@@ -197,6 +197,8 @@ EXPORTS = [
     "faz_gpu_lorenzo_compress_f32", "faz_gpu_lorenzo_raw_f32", "faz_gpu_lorenzo_decompress_raw",
     "faz_sz_compress_lorenzo_f32", "faz_gpu_tune_qoz_f32", "faz_gpu_compress_test_f32",
     "faz_sz_compress_qoz_f32",
+    "faz_mlp_create", "faz_mlp_destroy", "faz_mlp_layout", "faz_mlp_forward", "faz_mlp_gradients",
+    "faz_mlp_train_step", "faz_mlp_get_params", "faz_mlp_set_params",
 ]
 
 _lib = None
@@ -244,6 +246,16 @@ def lib():
         L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
         L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
+        L.faz_mlp_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                     P(vp)]
+        L.faz_mlp_destroy.argtypes = [vp]
+        L.faz_mlp_destroy.restype = None
+        L.faz_mlp_layout.argtypes = [vp, P(ctypes.c_int), vp, vp]
+        L.faz_mlp_forward.argtypes = [vp, vp, vp, ctypes.c_int, vp]
+        L.faz_mlp_gradients.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, sz, P(ctypes.c_double)]
+        L.faz_mlp_train_step.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, P(ctypes.c_double)]
+        L.faz_mlp_get_params.argtypes = [vp, vp, vp, sz, P(sz)]
+        L.faz_mlp_set_params.argtypes = [vp, vp, vp, sz]
         L.faz_sz_compress_qoz_f32.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
                                               P(vp), P(sz)]
         L.faz_gpu_compress_test_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, sz, ctypes.c_double, vp, sz,
@@ -554,3 +566,70 @@ class Engine:
             return ctypes.string_at(p, n.value)
         finally:
             self._L.faz_free(p)
+
+
+class MLP:
+    """Device MLP of the learned tuner (faz_mlp_* C-ABI): BF16 matrix-core
+    products with FP32 accumulation, FP32 Adam."""
+
+    def __init__(self, engine, in_dim, hidden=256, hidden_layers=3, out_dim=2, seed=0):
+        self.e = engine
+        self.L = engine._L
+        h = ctypes.c_void_p()
+        engine._check(self.L.faz_mlp_create(engine.h, in_dim, hidden, hidden_layers, out_dim, seed, ctypes.byref(h)),
+                      "mlp_create")
+        self.h = h
+        self.in_dim, self.out_dim = in_dim, out_dim
+        n = ctypes.c_int()
+        K = np.zeros(32, np.int32)
+        N = np.zeros(32, np.int32)
+        self.L.faz_mlp_layout(self.h, ctypes.byref(n), _np_ptr(K), _np_ptr(N))
+        self.K, self.N = [int(v) for v in K[:n.value]], [int(v) for v in N[:n.value]]
+        self.n_params = sum(k * m + m for k, m in zip(self.K, self.N))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.faz_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, x):
+        import torch
+        x = x.contiguous()
+        y = torch.empty((x.shape[0], self.out_dim), dtype=torch.float32, device=x.device)
+        torch.cuda.synchronize()
+        self.e._check(self.L.faz_mlp_forward(self.e.h, self.h, self.e._dev(x), x.shape[0], self.e._dev(y)),
+                      "mlp_forward")
+        return y
+
+    def gradients(self, x, t):
+        import torch
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_double()
+        torch.cuda.synchronize()
+        self.e._check(self.L.faz_mlp_gradients(self.e.h, self.h, self.e._dev(x.contiguous()),
+                                               self.e._dev(t.contiguous()), x.shape[0], _np_ptr(g), g.size,
+                                               ctypes.byref(loss)), "mlp_gradients")
+        return loss.value, g
+
+    def train_step(self, x, t, lr):
+        loss = ctypes.c_double()
+        self.e._check(self.L.faz_mlp_train_step(self.e.h, self.h, self.e._dev(x), self.e._dev(t), x.shape[0],
+                                                float(lr), ctypes.byref(loss)), "mlp_train_step")
+        return loss.value
+
+    def get_params(self):
+        p = np.zeros(self.n_params, np.float32)
+        n = ctypes.c_size_t()
+        self.e._check(self.L.faz_mlp_get_params(self.e.h, self.h, _np_ptr(p), p.size, ctypes.byref(n)),
+                      "mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.e._check(self.L.faz_mlp_set_params(self.e.h, self.h, _np_ptr(p), p.size), "mlp_set_params")

@@ -254,6 +254,28 @@ faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const 
 faz_status faz_sz_compress_qoz_f32(faz_gpu_ctx *ctx, int N, const size_t *dims, int rel_mode, double eb,
                                    const float *data, int data_on_device, uint8_t **out, size_t *out_size);
 
+/* --- Learned tuner: MLP surrogate with BF16 matrix-core kernels (no
+ * counterpart in the reference, which tunes only exhaustively).
+ * in_dim -> hidden (x hidden_layers, ReLU) -> out_dim (linear); weights
+ * He-uniform from `seed`.  Inputs and targets are device fp32 row-major
+ * [batch x in_dim] / [batch x out_dim].  Parameters on the host are, per
+ * layer, W as [K x N] row-major (padded shapes from faz_mlp_layout) then b[N]. */
+typedef struct faz_mlp faz_mlp;
+faz_status faz_mlp_create(faz_gpu_ctx *ctx, int in_dim, int hidden, int hidden_layers, int out_dim, uint64_t seed,
+                          faz_mlp **out);
+void faz_mlp_destroy(faz_mlp *net);
+faz_status faz_mlp_layout(const faz_mlp *net, int *n_layers, int *K, int *N);
+faz_status faz_mlp_forward(faz_gpu_ctx *ctx, faz_mlp *net, const float *d_x, int batch, float *d_y);
+/* Mean-squared-error gradients of every parameter (host, layout above) and
+ * the loss, without updating. */
+faz_status faz_mlp_gradients(faz_gpu_ctx *ctx, faz_mlp *net, const float *d_x, const float *d_t, int batch,
+                             float *h_grad, size_t cap, double *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch. */
+faz_status faz_mlp_train_step(faz_gpu_ctx *ctx, faz_mlp *net, const float *d_x, const float *d_t, int batch,
+                              float lr, double *loss);
+faz_status faz_mlp_get_params(faz_gpu_ctx *ctx, const faz_mlp *net, float *h, size_t cap, size_t *n);
+faz_status faz_mlp_set_params(faz_gpu_ctx *ctx, faz_mlp *net, const float *h, size_t n);
+
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
