@@ -199,7 +199,11 @@ EXPORTS = [
     "faz_sz_compress_qoz_f32",
     "faz_mlp_create", "faz_mlp_destroy", "faz_mlp_layout", "faz_mlp_forward", "faz_mlp_gradients",
     "faz_mlp_train_step", "faz_mlp_get_params", "faz_mlp_set_params",
+    "faz_gpu_compress_test_batch_f32", "faz_gpu_block_features_f32", "faz_learned_input", "faz_learned_target", "faz_gpu_tune_learned_f32",
 ]
+
+LEARNED_FEATURES = 40
+LEARNED_INPUTS = 49
 
 _lib = None
 _hip = None
@@ -256,6 +260,13 @@ def lib():
         L.faz_mlp_train_step.argtypes = [vp, vp, vp, vp, ctypes.c_int, ctypes.c_float, P(ctypes.c_double)]
         L.faz_mlp_get_params.argtypes = [vp, vp, vp, sz, P(sz)]
         L.faz_mlp_set_params.argtypes = [vp, vp, vp, sz]
+        L.faz_gpu_compress_test_batch_f32.argtypes = [vp, vp, vp, vp, sz, vp, sz, sz, ctypes.c_double, vp, vp]
+        L.faz_gpu_block_features_f32.argtypes = [vp, vp, ctypes.c_int, vp, vp, sz, sz, ctypes.c_float,
+                                                 ctypes.c_float, ctypes.c_double, vp]
+        L.faz_learned_input.argtypes = [vp, sz, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, vp, vp, vp]
+        L.faz_learned_target.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
+        L.faz_gpu_tune_learned_f32.argtypes = [vp, vp, P(TuneIn), vp, vp, vp, P(TuneOut)]
         L.faz_sz_compress_qoz_f32.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
                                               P(vp), P(sz)]
         L.faz_gpu_compress_test_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, sz, ctypes.c_double, vp, sz,
@@ -310,6 +321,14 @@ class Engine:
     """One GPU context (stream + device scratch) of the C-ABI."""
 
     def __init__(self, device=0):
+        # the bindings hand torch device tensors to the library: let torch's
+        # HIP runtime claim the device first (it does not attach afterwards)
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
         self._L = lib()
         h = ctypes.c_void_p()
         rc = self._L.faz_gpu_ctx_create(int(device), ctypes.byref(h))
@@ -533,6 +552,21 @@ class Engine:
         self._check(self._L.faz_gpu_tune_qoz_f32(self.h, self._dev(x), ctypes.byref(t), ctypes.byref(o)), "tune_qoz")
         return tune_out_dict(o)
 
+    def compress_test_batch(self, x, shape, starts, sbs, params_list, rng):
+        """CompressTest of every candidate in params_list: (bytes, squared errors)."""
+        import torch
+        torch.cuda.synchronize()
+        fd = np.array(list(shape) + [1] * (3 - len(shape)), np.uint64)
+        st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+        arr = (InterpParams * len(params_list))(*params_list)
+        nb = np.zeros(len(params_list), np.uint64)
+        sq = np.zeros(len(params_list), np.float64)
+        self._check(self._L.faz_gpu_compress_test_batch_f32(self.h, self._dev(x), _np_ptr(fd), ctypes.cast(arr,
+                                                            ctypes.c_void_p), len(params_list), _np_ptr(st),
+                                                            st.shape[0], sbs, float(rng), _np_ptr(nb), _np_ptr(sq)),
+                    "compress_test_batch")
+        return nb.astype(np.int64), sq
+
     def compress_test(self, x, shape, starts, sbs, params, rng):
         """One CompressTest: (pre-zstd buffer, compressed size, squared error)."""
         import torch
@@ -566,6 +600,35 @@ class Engine:
             return ctypes.string_at(p, n.value)
         finally:
             self._L.faz_free(p)
+
+    # --- learned tuner ---------------------------------------------------------
+    def block_features(self, x, shape, starts, sbs, vmin, rng, eb_norm):
+        """FAZ_LEARNED_FEATURES statistics of every sample block (host array)."""
+        import torch
+        torch.cuda.synchronize()
+        d = np.array(list(shape), np.uint64)
+        st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+        out = np.zeros((st.shape[0], LEARNED_FEATURES), np.float32)
+        self._check(self._L.faz_gpu_block_features_f32(self.h, self._dev(x), len(shape), _np_ptr(d), _np_ptr(st),
+                                                       st.shape[0], sbs, float(vmin), float(rng), float(eb_norm),
+                                                       _np_ptr(out)), "block_features")
+        return out
+
+    def tune_learned(self, x, shape, abs_eb, rel_eb, rng, mlp, mu, sigma, interp_block_size=32):
+        import torch
+        torch.cuda.synchronize()
+        t = TuneIn()
+        t.N = len(shape)
+        for i, dd in enumerate(shape):
+            t.dims[i] = int(dd)
+        t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
+        t.interp_block_size = interp_block_size
+        mu = np.ascontiguousarray(mu, np.float32)
+        sigma = np.ascontiguousarray(sigma, np.float32)
+        o = TuneOut()
+        self._check(self._L.faz_gpu_tune_learned_f32(self.h, self._dev(x), ctypes.byref(t), mlp.h, _np_ptr(mu),
+                                                     _np_ptr(sigma), ctypes.byref(o)), "tune_learned")
+        return tune_out_dict(o)
 
 
 class MLP:
@@ -633,3 +696,24 @@ class MLP:
     def set_params(self, p):
         p = np.ascontiguousarray(p, np.float32)
         self.e._check(self.L.faz_mlp_set_params(self.e.h, self.h, _np_ptr(p), p.size), "mlp_set_params")
+
+
+def learned_input(block_feats, eb_norm, rel_eb, alpha, beta, eb_scale, ops, dirs):
+    """MLP input row of one candidate (faz_learned_input)."""
+    f = np.ascontiguousarray(block_feats, np.float32)
+    o = np.ascontiguousarray(ops, np.uint8)
+    d = np.ascontiguousarray(dirs, np.uint8)
+    row = np.zeros(LEARNED_INPUTS, np.float32)
+    rc = lib().faz_learned_input(_np_ptr(f), f.shape[0], float(eb_norm), float(rel_eb), float(alpha), float(beta),
+                                 float(eb_scale), len(o), _np_ptr(o), _np_ptr(d), _np_ptr(row))
+    if rc:
+        raise FazError(rc, "learned_input")
+    return row
+
+
+def learned_target(bitrate, psnr, eb_norm, eb_scale):
+    t = np.zeros(2, np.float32)
+    rc = lib().faz_learned_target(float(bitrate), float(psnr), float(eb_norm), float(eb_scale), _np_ptr(t))
+    if rc:
+        raise FazError(rc, "learned_target")
+    return t

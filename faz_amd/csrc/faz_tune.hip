@@ -463,12 +463,14 @@ struct Tuner {
     // candidate 0: outliers in block order, codes interleaved by level,
     // encoding_lossless (SIC:784-809) through zstd.
     int compress_test(double eb, double &bitrate, double &psnr, size_t &bytes,
-                      std::vector<uint8_t> *raw_out = nullptr, double *sq_out = nullptr) {
+                      std::vector<uint8_t> *raw_out = nullptr, double *sq_out = nullptr, uint32_t cand = 0) {
         hipStream_t st = ctx->stream;
         const uint32_t nb = (uint32_t) origins.size();
         const uint64_t n = (uint64_t) nb * pl.G.ncodes;
         double square_error = 0;
-        for (uint32_t b = 0; b < nb; b++) square_error += h_loss[b];
+        for (uint32_t b = 0; b < nb; b++) square_error += h_loss[(size_t) cand * nb + b];
+        const uint16_t *codes = d_codes.p + (size_t) cand * n;
+        const float *outval = d_outval.p + (size_t) cand * n;
         // outliers: code-0 slots in block-major order (the one quantizer of
         // CompressTest accumulates them block by block)
         const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
@@ -478,13 +480,13 @@ struct Tuner {
         HIPCHECK(ctx->first.ensure(65536));
         HIPCHECK(ctx->scalars.ensure(4));
         HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
-        launch_hist(d_codes.p, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
+        launch_hist(codes, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
         launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
         uint64_t U = 0;
         HIPCHECK(hipMemcpyAsync(&U, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(ctx->unpred.ensure(U));
-        launch_unpred_gather(d_codes.p, n, d_outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p, st);
+        launch_unpred_gather(codes, n, outval, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p, st);
         // codes in CompressTest order
         std::vector<QSeg> segs;
         uint32_t last = 0, q = 0;
@@ -508,7 +510,7 @@ struct Tuner {
         HIPCHECK(d_segs.ensure(segs.size()));
         HIPCHECK(d_q.ensure(n));
         HIPCHECK(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(QSeg), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(interleave_kernel, dim3((uint32_t) ((n + 255) / 256)), dim3(256), 0, st, d_codes.p, d_q.p,
+        hipLaunchKernelGGL(interleave_kernel, dim3((uint32_t) ((n + 255) / 256)), dim3(256), 0, st, codes, d_q.p,
                            nb, pl.G.ncodes, d_segs.p, (int) segs.size());
         launch_hist(d_q.p, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
         HIPCHECK(hipGetLastError());
@@ -568,8 +570,23 @@ void set_ab(double rel, double &a, double &b) {
 
 }  // namespace
 
-// Tuning() with conf.QoZ set, RD target, no wavelets (SZI:1589-2557).
-int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tune_out *out) {
+struct LearnedModel {
+    faz_mlp *net;
+    const float *mu, *sigma;  // input standardisation (FAZ_LEARNED_INPUTS each)
+};
+
+int learned_predict(faz_gpu_ctx *ctx, const LearnedModel &lm, const std::vector<float> &rows, size_t nrows,
+                    std::vector<float> &y);
+void learned_row(const float *block_feats, size_t nblocks, double eb_norm, double rel, double alpha, double beta,
+                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, float *row);
+int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origins, size_t nblocks,
+                   const float *d_field, float vmin, float rng, float eb_norm, std::vector<float> &feats);
+
+// Tuning() with conf.QoZ set, RD target, no wavelets (SZI:1589-2557).  With a
+// learned model the CompressTest calls of the alpha/beta search are replaced
+// by the MLP's predictions of the same bitrate and PSNR.
+int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tune_out *out,
+             const LearnedModel *lm = nullptr) {
     const int N = in->N;
     if (N != 2 && N != 3) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "the QoZ tuner samples blocks of 2-D and 3-D fields only");
     memset(out, 0, sizeof(*out));
@@ -687,7 +704,53 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
     const double alphas[5] = {1, 1.25, 1.5, 1.75, 2};
     const double betas[4] = {1.5, 2, 3, 4};
     double best_a = 1, best_b = 1, bestb = 9999, bestm = 0;
+    std::vector<float> feats;
+    const double eb_norm = in->abs_eb / in->rng;
+    if (lm) {
+        float mn, mx;
+        if ((rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx))) return rc;
+        if ((rc = block_features(ctx, T.pl.G, T.d_origins.p, nb, d_in, mn, (float) in->rng, (float) eb_norm, feats)))
+            return rc;
+    }
+    // every alpha/beta pair at the base bound in one batch; re-tests at a
+    // scaled bound are predicted as they come
+    std::vector<float> y_base;
+    if (lm) {
+        std::vector<float> rows;
+        for (int i = 0; i < 5; i++)
+            for (int j = 0; j < 4; j++) {
+                std::vector<float> row(FAZ_LEARNED_INPUTS);
+                learned_row(feats.data(), nb, eb_norm, rel, alphas[i], betas[j], 1.0, levelwise, ops.data(),
+                            dirs.data(), row.data());
+                rows.insert(rows.end(), row.begin(), row.end());
+            }
+        if ((rc = learned_predict(ctx, *lm, rows, 20, y_base))) return rc;
+    }
     auto test = [&](double a, double b, double eb, double &br, double &ps) -> int {
+        if (lm) {
+            std::vector<float> y;
+            if (eb == in->abs_eb) {
+                int i = 0, j = 0;
+                while (i < 4 && alphas[i] != a) i++;
+                while (j < 3 && betas[j] != b) j++;
+                y.assign(y_base.begin() + 2 * (4 * i + j), y_base.begin() + 2 * (4 * i + j) + 2);
+            } else {
+                std::vector<float> row(FAZ_LEARNED_INPUTS);
+                learned_row(feats.data(), nb, eb_norm, rel, a, b, eb / in->abs_eb, levelwise, ops.data(),
+                            dirs.data(), row.data());
+                int r = learned_predict(ctx, *lm, row, 1, y);
+                if (r) return r;
+            }
+            br = 4.0 * y[0];
+            ps = 5.0 * y[1] - 20 * std::log10(eb_norm * (eb / in->abs_eb));
+            const int t = out->n_tests;
+            if (t < 128) {
+                out->test_alpha[t] = a, out->test_beta[t] = b, out->test_eb[t] = eb;
+                out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = 0;
+            }
+            out->n_tests++;
+            return FAZ_OK;
+        }
         TuneCand c;
         memset(&c, 0, sizeof(c));
         fill_levels(c, T.pl.G.levels, eb, a, b, 1, dir_cands[1], ops, dirs);
@@ -788,6 +851,69 @@ int compress_test_at(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims, c
 
 }  // namespace faz
 
+namespace faz {
+
+// CompressTest of many candidates on the same blocks: one tuning-mode-1
+// launch for all (candidate, block) pairs, then each candidate's entropy stage.
+int compress_test_batch(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims, const faz_interp_params *ps,
+                        size_t ncand, const size_t *starts, size_t nblocks, size_t sbs, double rng, size_t *bytes,
+                        double *sq) {
+    if (!ncand) return FAZ_OK;
+    const int N = ps[0].N;
+    if (N != 2 && N != 3) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "CompressTest samples 2-D and 3-D fields only");
+    if (sbs < 1 || (N == 3 && sbs + 1 > 33) || (N == 2 && (sbs + 1) * (sbs + 1) > kTuneMaxPoints))
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "sample blocks of edge %zu exceed the LDS-resident tuner", sbs + 1);
+    if (nblocks == 0) CTX_FAIL(FAZ_ERR_INVALID, "no blocks");
+    for (size_t c = 1; c < ncand; c++)
+        if (ps[c].N != N || ps[c].max_step != ps[0].max_step || ps[c].block_size != ps[0].block_size)
+            CTX_FAIL(FAZ_ERR_INVALID, "batched candidates must share N, maxStep and interpBlockSize");
+    HIPCHECK(hipSetDevice(ctx->device));
+    uint64_t fstr[3] = {0, 0, 0};
+    fstr[N - 1] = 1;
+    for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * fdims[a + 1];
+    Tuner T;
+    T.ctx = ctx;
+    T.d_field = d_in;
+    T.rng = rng;
+    make_tune_plan(N, (uint32_t) sbs + 1, fstr, ps[0].max_step, ps[0].block_size, T.pl);
+    if (T.pl.G.levels > kTuneMaxLevels || T.pl.G.max_lines > kTuneMaxLines)
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "sample block outside the tuner's LDS tables");
+    for (size_t b = 0; b < nblocks; b++) {
+        for (int a = 0; a < N; a++)
+            if (starts[3 * b + a] + sbs >= fdims[a]) CTX_FAIL(FAZ_ERR_INVALID, "block %zu outside the field", b);
+        T.origins.push_back(starts[3 * b] * fstr[0] + starts[3 * b + 1] * fstr[1] + (N == 3 ? starts[3 * b + 2] : 0));
+    }
+    HIPCHECK(T.d_origins.ensure(T.origins.size()));
+    HIPCHECK(hipMemcpyAsync(T.d_origins.p, T.origins.data(), T.origins.size() * 8, hipMemcpyHostToDevice,
+                            ctx->stream));
+    std::vector<TuneCand> cands(ncand);
+    for (size_t c = 0; c < ncand; c++) {
+        memset(&cands[c], 0, sizeof(TuneCand));
+        std::vector<uint8_t> ops(ps[c].ops, ps[c].ops + ps[c].levelwise),
+                dirs(ps[c].dirs, ps[c].dirs + ps[c].levelwise);
+        fill_levels(cands[c], T.pl.G.levels, ps[c].eb, ps[c].alpha, ps[c].beta, ps[c].interp_algo, ps[c].direction,
+                    ops, dirs);
+        set_range(cands[c], T.pl.G, 1, 0, 0);
+    }
+    int rc = T.run(cands, true);
+    if (rc) return rc;
+    for (size_t c = 0; c < ncand; c++) {
+        double br, psnr;
+        if ((rc = T.compress_test(ps[c].eb, br, psnr, bytes[c], nullptr, &sq[c], (uint32_t) c))) return rc;
+    }
+    return FAZ_OK;
+}
+
+}  // namespace faz
+
+extern "C" faz_status faz_gpu_compress_test_batch_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims,
+                                                      const faz_interp_params *ps, size_t ncand, const size_t *starts,
+                                                      size_t nblocks, size_t sbs, double rng, size_t *bytes,
+                                                      double *square_error) {
+    if (!ctx || !d_in || !fdims || !ps || !starts || !bytes || !square_error) return FAZ_ERR_INVALID;
+    return faz::compress_test_batch(ctx, d_in, fdims, ps, ncand, starts, nblocks, sbs, rng, bytes, square_error);
+}
+
 extern "C" faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims,
                                                 const faz_interp_params *p, const size_t *starts, size_t nblocks,
                                                 size_t sbs, double rng, uint8_t *h_raw, size_t cap, size_t *raw_size,
@@ -810,3 +936,308 @@ extern "C" faz_status faz_gpu_tune_qoz_f32(faz_gpu_ctx *ctx, const float *d_in, 
     if (!ctx || !d_in || !in || !out) return FAZ_ERR_INVALID;
     return faz::tune_qoz(ctx, d_in, in, out);
 }
+
+// ---------------------------------------------------------------------------
+// Learned tuner: block statistics, then the MLP surrogate in place of the
+// CompressTest search over alpha/beta.
+// ---------------------------------------------------------------------------
+namespace faz {
+namespace {
+
+constexpr int kFeat = 40;
+constexpr int kFeatThreads = 256;
+constexpr int kFeatAcc = 36;  // double accumulators per thread
+
+// Statistics of one sample block u = (v - vmin) / rng (see faz_gpu.h for the
+// list).  Per-thread partial sums over a fixed point assignment, added in
+// thread order: the result does not depend on scheduling.
+__global__ void __launch_bounds__(kFeatThreads) block_features_kernel(TuneGeom G, const uint64_t *__restrict__ origins,
+                                                                      const float *__restrict__ field, float vmin,
+                                                                      float rng, float eb_norm,
+                                                                      float *__restrict__ out) {
+    __shared__ float s_u[kTuneMaxPoints];
+    __shared__ double s_part[kFeatThreads];
+    __shared__ float s_max[kFeatThreads];
+    __shared__ double s_res[kFeatAcc + 8];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t org = origins[blockIdx.x];
+    const int N = G.N;
+    const uint32_t d0 = G.bd[0], d1 = G.bd[1], d2 = N == 3 ? G.bd[2] : 1;
+    const uint32_t npts = d0 * d1 * d2;
+    for (uint32_t q = tid; q < npts; q += blockDim.x) {
+        const uint32_t z = q % d2, r = q / d2, y = r % d1, x = r / d1;
+        const uint64_t f = org + (uint64_t) x * G.fstr[0] + (uint64_t) y * G.fstr[1] + (N == 3 ? z : 0);
+        s_u[q] = (field[f] - vmin) / rng;
+    }
+    __syncthreads();
+    const uint32_t dd[3] = {d0, d1, d2};
+    const uint32_t st[3] = {d1 * d2, d2, 1};
+    double acc[kFeatAcc];
+#pragma unroll
+    for (int i = 0; i < kFeatAcc; i++) acc[i] = 0.0;
+    float mn = 3.0e38f, mx = -3.0e38f, lmax[3] = {0.f, 0.f, 0.f};
+    // acc: 0 sum, 1 sum2, [2 + 6*si + a] sum|d|, [5 + 6*si + a] sum d^2 (stride si),
+    // 20 + 2*si sum|L|, 21 + 2*si sum L^2, 26 + a lag1 cross, 29 + a lag2 cross,
+    // 32 sum L1^4, 33 + a count |d1| < eb
+    const int strides[3] = {1, 2, 4};
+    for (uint32_t q = tid; q < npts; q += blockDim.x) {
+        const uint32_t c[3] = {q / (d1 * d2), (q / d2) % d1, q % d2};
+        const float u = s_u[q];
+        acc[0] += u;
+        acc[1] += (double) u * u;
+        mn = fminf(mn, u);
+        mx = fmaxf(mx, u);
+#pragma unroll
+        for (int si = 0; si < 3; si++) {
+            const uint32_t s = strides[si];
+            bool interior = true;
+            float L = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if (a >= N) continue;
+                if (c[a] + s < dd[a]) {
+                    const float d = s_u[q + s * st[a]] - u;
+                    acc[2 + 6 * si + a] += fabsf(d);
+                    acc[5 + 6 * si + a] += (double) d * d;
+                    if (si == 0 && fabsf(d) < eb_norm) acc[33 + a] += 1.0;
+                }
+                if (c[a] >= s && c[a] + s < dd[a])
+                    L += s_u[q + s * st[a]] - 2.f * u + s_u[q - s * st[a]];
+                else
+                    interior = false;
+            }
+            if (interior) {
+                acc[20 + 2 * si] += fabsf(L);
+                acc[21 + 2 * si] += (double) L * L;
+                lmax[si] = fmaxf(lmax[si], fabsf(L));
+                if (si == 0) acc[32] += (double) L * L * L * L;
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            if (a >= N) continue;
+            if (c[a] + 1 < dd[a]) acc[26 + a] += (double) u * s_u[q + st[a]];
+            if (c[a] + 2 < dd[a]) acc[29 + a] += (double) u * s_u[q + 2 * st[a]];
+        }
+    }
+    // deterministic reductions: partials in LDS, summed by thread 0 in order
+#pragma unroll
+    for (int i = 0; i < kFeatAcc; i++) {
+        s_part[tid] = acc[i];
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0;
+            for (int k = 0; k < kFeatThreads; k++) t += s_part[k];
+            s_res[i] = t;
+        }
+        __syncthreads();
+    }
+    const float mxs[6] = {mn, mx, lmax[0], lmax[1], lmax[2], 0.f};
+    for (int i = 0; i < 5; i++) {
+        s_max[tid] = i == 0 ? -mxs[0] : mxs[i];
+        __syncthreads();
+        if (tid == 0) {
+            float t = s_max[0];
+            for (int k = 1; k < kFeatThreads; k++) t = fmaxf(t, s_max[k]);
+            s_res[kFeatAcc + i] = i == 0 ? -t : t;
+        }
+        __syncthreads();
+    }
+    // second pass: share of |L1| above twice its RMS
+    uint64_t n_int1 = 1;
+    for (int a = 0; a < N; a++) n_int1 *= dd[a] - 2;
+    const double rms1 = n_int1 ? sqrt(s_res[21] / (double) n_int1) : 0.0;
+    double big = 0;
+    for (uint32_t q = tid; q < npts; q += blockDim.x) {
+        const uint32_t c[3] = {q / (d1 * d2), (q / d2) % d1, q % d2};
+        bool interior = true;
+        float L = 0.f;
+        for (int a = 0; a < N; a++) {
+            if (c[a] >= 1 && c[a] + 1 < dd[a])
+                L += s_u[q + st[a]] - 2.f * s_u[q] + s_u[q - st[a]];
+            else
+                interior = false;
+        }
+        if (interior && fabsf(L) > 2.0 * rms1) big += 1.0;
+    }
+    s_part[tid] = big;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0;
+        for (int k = 0; k < kFeatThreads; k++) t += s_part[k];
+        const double n = (double) npts;
+        const double mean = s_res[0] / n, var = fmax(s_res[1] / n - mean * mean, 0.0);
+        float *o = out + (uint64_t) blockIdx.x * kFeat;
+        o[0] = (float) mean;
+        o[1] = (float) sqrt(var);
+        o[2] = (float) s_res[kFeatAcc + 0];
+        o[3] = (float) s_res[kFeatAcc + 1];
+        for (int si = 0; si < 3; si++) {
+            const uint32_t s = strides[si];
+            for (int a = 0; a < 3; a++) {
+                double cnt = 0;
+                if (a < N && dd[a] > s) cnt = n / dd[a] * (dd[a] - s);
+                o[4 + 9 * si + a] = cnt > 0 ? (float) (s_res[2 + 6 * si + a] / cnt) : 0.f;
+                o[7 + 9 * si + a] = cnt > 0 ? (float) sqrt(s_res[5 + 6 * si + a] / cnt) : 0.f;
+            }
+            double ni = 1;
+            for (int a = 0; a < N; a++) ni *= dd[a] > 2 * s ? (double) (dd[a] - 2 * s) : 0.0;
+            o[10 + 9 * si] = ni > 0 ? (float) (s_res[20 + 2 * si] / ni) : 0.f;
+            o[11 + 9 * si] = ni > 0 ? (float) sqrt(s_res[21 + 2 * si] / ni) : 0.f;
+            o[12 + 9 * si] = (float) s_res[kFeatAcc + 2 + si];
+        }
+        for (int a = 0; a < 3; a++) {
+            for (int lag = 1; lag <= 2; lag++) {
+                double cnt = 0, cr = 0;
+                if (a < N && dd[a] > (uint32_t) lag) {
+                    cnt = n / dd[a] * (dd[a] - lag);
+                    cr = var > 0 ? (s_res[(lag == 1 ? 26 : 29) + a] / cnt - mean * mean) / var : 0.0;
+                }
+                o[(lag == 1 ? 31 : 34) + a] = (float) cr;
+            }
+        }
+        const double ni1 = (double) n_int1;
+        const double m2 = ni1 > 0 ? s_res[21] / ni1 : 0.0;
+        o[37] = m2 > 0 ? (float) ((s_res[32] / ni1) / (m2 * m2)) : 0.f;
+        o[38] = ni1 > 0 ? (float) (t / ni1) : 0.f;
+        double fr = 0;
+        int na = 0;
+        for (int a = 0; a < N; a++) {
+            if (dd[a] > 1) {
+                fr += s_res[33 + a] / (n / dd[a] * (dd[a] - 1));
+                na++;
+            }
+        }
+        o[39] = na ? (float) (fr / na) : 0.f;
+    }
+}
+
+}  // namespace
+}  // namespace faz
+
+namespace faz {
+
+int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origins, size_t nblocks,
+                   const float *d_field, float vmin, float rng, float eb_norm, std::vector<float> &feats) {
+    DevBuf<float> d_out;
+    HIPCHECK(d_out.ensure(nblocks * kFeat));
+    hipLaunchKernelGGL(block_features_kernel, dim3((uint32_t) nblocks), dim3(kFeatThreads), 0, ctx->stream, G,
+                       d_origins, d_field, vmin, rng, eb_norm, d_out.p);
+    HIPCHECK(hipGetLastError());
+    feats.resize(nblocks * kFeat);
+    HIPCHECK(hipMemcpyAsync(feats.data(), d_out.p, feats.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    d_out.release();
+    return FAZ_OK;
+}
+
+// Input row of the MLP (FAZ_LEARNED_INPUTS values): the block statistics
+// averaged over the sample blocks, magnitudes as log2 of their ratio to the
+// normalised bound, then the candidate.
+void learned_row(const float *block_feats, size_t nblocks, double eb_norm, double rel, double alpha, double beta,
+                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, float *row) {
+    for (int i = 0; i < kFeat; i++) {
+        double s = 0;
+        for (size_t b = 0; b < nblocks; b++) {
+            double v = block_feats[b * kFeat + i];
+            const bool magnitude = i == 1 || (i >= 4 && i <= 30);
+            if (magnitude) v = std::log2(std::max(v, 1e-30) / eb_norm);
+            if (i == 37) v = std::log2(std::max(v, 1e-30));
+            s += v;
+        }
+        row[i] = (float) (s / (double) nblocks);
+    }
+    double mop = 0, mdir = 0;
+    for (int l = 0; l < levelwise; l++) {
+        mop += ops[l];
+        mdir += dirs[l] != 0;
+    }
+    row[40] = (float) (alpha - 1);
+    row[41] = (float) (beta / 4);
+    row[42] = (float) std::log2(beta);
+    row[43] = (float) ops[0];
+    row[44] = (float) (dirs[0] != 0);
+    row[45] = (float) (mop / levelwise);
+    row[46] = (float) (mdir / levelwise);
+    row[47] = (float) std::log10(rel);
+    row[48] = (float) std::log2(eb_scale);
+}
+
+int learned_predict(faz_gpu_ctx *ctx, const LearnedModel &lm, const std::vector<float> &rows, size_t nrows,
+                    std::vector<float> &y) {
+    std::vector<float> x(rows.size());
+    for (size_t r = 0; r < nrows; r++)
+        for (int i = 0; i < FAZ_LEARNED_INPUTS; i++) {
+            const size_t k = r * FAZ_LEARNED_INPUTS + i;
+            x[k] = (rows[k] - lm.mu[i]) / lm.sigma[i];
+        }
+    DevBuf<float> dx, dy;
+    HIPCHECK(dx.ensure(x.size()));
+    HIPCHECK(dy.ensure(nrows * 2));
+    HIPCHECK(hipMemcpyAsync(dx.p, x.data(), x.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    int rc = faz_mlp_forward(ctx, lm.net, dx.p, (int) nrows, dy.p);
+    if (rc) return rc;
+    y.resize(nrows * 2);
+    HIPCHECK(hipMemcpy(y.data(), dy.p, y.size() * 4, hipMemcpyDeviceToHost));
+    dx.release();
+    dy.release();
+    return FAZ_OK;
+}
+
+}  // namespace faz
+
+extern "C" {
+
+faz_status faz_gpu_block_features_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims,
+                                      const size_t *starts, size_t nblocks, size_t sbs, float vmin, float rng,
+                                      double eb_norm, float *h_feats) {
+    if (!ctx || !d_in || !dims || !starts || !h_feats || (N != 2 && N != 3) || nblocks == 0) return FAZ_ERR_INVALID;
+    if ((N == 3 && sbs + 1 > 33) || (N == 2 && (sbs + 1) * (sbs + 1) > faz::kTuneMaxPoints))
+        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "sample blocks of edge %zu exceed the LDS-resident kernels", sbs + 1);
+    HIPCHECK(hipSetDevice(ctx->device));
+    uint64_t fstr[3] = {0, 0, 0};
+    fstr[N - 1] = 1;
+    for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * dims[a + 1];
+    faz::TunePlan pl;
+    faz::make_tune_plan(N, (uint32_t) sbs + 1, fstr, 0, 32, pl);
+    std::vector<uint64_t> org;
+    for (size_t b = 0; b < nblocks; b++) {
+        for (int a = 0; a < N; a++)
+            if (starts[3 * b + a] + sbs >= dims[a]) CTX_FAIL(FAZ_ERR_INVALID, "block %zu outside the field", b);
+        org.push_back(starts[3 * b] * fstr[0] + starts[3 * b + 1] * fstr[1] + (N == 3 ? starts[3 * b + 2] : 0));
+    }
+    faz::DevBuf<uint64_t> d_org;
+    HIPCHECK(d_org.ensure(org.size()));
+    HIPCHECK(hipMemcpy(d_org.p, org.data(), org.size() * 8, hipMemcpyHostToDevice));
+    std::vector<float> f;
+    int rc = faz::block_features(ctx, pl.G, d_org.p, nblocks, d_in, vmin, rng, (float) eb_norm, f);
+    d_org.release();
+    if (rc) return rc;
+    memcpy(h_feats, f.data(), f.size() * 4);
+    return FAZ_OK;
+}
+
+faz_status faz_learned_input(const float *block_feats, size_t nblocks, double eb_norm, double rel_eb, double alpha,
+                             double beta, double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs,
+                             float *row) {
+    if (!block_feats || !nblocks || !ops || !dirs || !row || levelwise < 1 || !(eb_norm > 0) || !(rel_eb > 0))
+        return FAZ_ERR_INVALID;
+    faz::learned_row(block_feats, nblocks, eb_norm, rel_eb, alpha, beta, eb_scale, levelwise, ops, dirs, row);
+    return FAZ_OK;
+}
+
+faz_status faz_learned_target(double bitrate, double psnr, double eb_norm, double eb_scale, float *t) {
+    if (!t || !(eb_norm > 0) || !(eb_scale > 0)) return FAZ_ERR_INVALID;
+    t[0] = (float) (bitrate / 4);
+    t[1] = (float) ((psnr + 20 * std::log10(eb_norm * eb_scale)) / 5);
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_mlp *net,
+                                    const float *mu, const float *sigma, faz_tune_out *out) {
+    if (!ctx || !d_in || !in || !net || !mu || !sigma || !out) return FAZ_ERR_INVALID;
+    faz::LearnedModel lm{net, mu, sigma};
+    return faz::tune_qoz(ctx, d_in, in, out, &lm);
+}
+
+}  // extern "C"

@@ -1,0 +1,230 @@
+"""Learned tuner of the MI355X FAZ engine: data, training and evaluation.
+
+The exhaustive QoZ tuner (faz -q) ranks alpha/beta candidates by running
+CompressTest (tuning-mode-1 compression of the sample blocks, Huffman, zstd)
+for each.  The learned tuner keeps the tuner's sampling and its per-level
+interpolator choice and replaces those CompressTests by an MLP that predicts
+the same (bitrate, PSNR) from statistics of the sample blocks; the decision
+rule on top is the exhaustive tuner's.  The quantizer still enforces the
+error bound exactly: only the configuration choice is learned.
+
+Pipeline (all arithmetic in the engine's own kernels through the C-ABI):
+  * fields: seeded k^-slope Gaussian random fields with tanh fronts
+    (synthesised with torch.fft on the device),
+  * tuples: for each field, the exhaustive tuner's own CompressTests plus a
+    grid of CompressTests over every alpha/beta pair and bound scale, with
+    the tuner's interpolators, -> (block statistics, candidate) -> (bitrate,
+    PSNR),
+  * training: faz_mlp_train_step (BF16 matrix-core forward/backward, Adam),
+  * model: parameters + input standardisation in one .npz.
+"""
+import os
+import time
+
+import numpy as np
+
+import faz_amd
+
+ALPHAS = (1.0, 1.25, 1.5, 1.75, 2.0)
+BETAS = (1.5, 2.0, 3.0, 4.0)
+EB_SCALES = (0.8, 0.9, 1.0, 1.12, 1.25)
+TRAIN_GRID = (320, 320, 320)  # training fields are cut from GRFs synthesised on this grid
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "learned_tuner.npz")
+
+
+def grf_field(shape, seed, slope=11.0 / 3.0, fronts=0, noise=1e-4, width=4.0, amp=0.5, grid=None):
+    """k^-slope Gaussian random field in [0, 1] plus tanh fronts, float32 flat
+    device tensor (synthetic input data).  With `grid` (a shape at least
+    `shape` on every axis) the field is synthesised on that grid and a
+    seeded sub-box is cut out, so fields of many shapes share one FFT plan."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    full = tuple(grid) if grid else tuple(shape)
+    ks = [torch.fft.fftfreq(n, device="cuda") for n in full[:-1]] + [torch.fft.rfftfreq(full[-1], device="cuda")]
+    grids = torch.meshgrid(*ks, indexing="ij")
+    k = torch.sqrt(sum(a * a for a in grids))
+    k.view(-1)[0] = 1.0
+    ampl = k.pow(-slope / 2.0)
+    ampl.view(-1)[0] = 0.0
+    spec = torch.complex(torch.randn(ampl.shape, generator=g, device="cuda") * ampl,
+                         torch.randn(ampl.shape, generator=g, device="cuda") * ampl)
+    f = torch.fft.irfftn(spec, s=full)
+    if grid:
+        off = [int(torch.randint(0, n - m + 1, (1,), generator=g, device="cuda").item()) for n, m in zip(full, shape)]
+        f = f[tuple(slice(o, o + m) for o, m in zip(off, shape))].contiguous()
+    f = (f - f.min()) / (f.max() - f.min())
+    f += noise * torch.randn(shape, generator=g, device="cuda")
+    if fronts:
+        coords = torch.meshgrid(*[torch.arange(n, device="cuda", dtype=torch.float32) for n in shape], indexing="ij")
+        for _ in range(fronts):
+            nv = torch.randn(len(shape), generator=g, device="cuda")
+            nv = nv / nv.norm()
+            c = (torch.rand(len(shape), generator=g, device="cuda") *
+                 torch.tensor(shape, device="cuda", dtype=torch.float32)).dot(nv)
+            f += amp * torch.tanh((sum(coords[a] * nv[a] for a in range(len(shape))) - c) / width)
+    out = f.to(torch.float32).contiguous().view(-1)
+    torch.cuda.synchronize()
+    return out
+
+
+def field_tuples(engine, x, shape, rel):
+    """(rows, targets, info) of one field: the exhaustive tuner's tests and a
+    CompressTest grid over ALPHAS x BETAS x EB_SCALES."""
+    mn, mx = engine.range(x)
+    rng = float(np.float32(mx - mn))
+    eb = rel * rng
+    ex = engine.tune_qoz(x, shape, eb, rel, rng)
+    sbs, starts = ex["sample_block_size"], ex["block_starts"]
+    eb_norm = eb / rng
+    feats = engine.block_features(x, shape, starts, sbs, mn, rng, eb_norm)
+    rows, targets = [], []
+
+    def add(alpha, beta, eb_t, bitrate, psnr):
+        rows.append(faz_amd.learned_input(feats, eb_norm, rel, alpha, beta, eb_t / eb, ex["ops"], ex["dirs"]))
+        targets.append(faz_amd.learned_target(bitrate, psnr, eb_norm, eb_t / eb))
+
+    for t in ex["tests"]:
+        add(t["alpha"], t["beta"], t["eb"], t["bitrate"], t["psnr"])
+    bd = (sbs + 1,) * len(shape)
+    per = (sbs + 1) ** len(shape) * len(starts)
+    grid = [(a, b, s) for a in ALPHAS for b in BETAS if a <= b for s in EB_SCALES]
+    ps = [faz_amd.make_params(bd, eb * s, algo=ex["interp_algo"], direction=ex["direction"], alpha=a, beta=b,
+                              max_step=ex["max_step"], ops=tuple(ex["ops"]), dirs=tuple(ex["dirs"])) for a, b, s in grid]
+    nbytes, sq = engine.compress_test_batch(x, shape, starts, sbs, ps, rng)
+    for (a, b, s), nbt, sqe in zip(grid, nbytes, sq):
+        bitrate = 8.0 * float(nbt) / per
+        psnr = 20 * np.log10(rng) - 10 * np.log10(sqe / per) if sqe > 0 else 200.0
+        add(a, b, eb * s, bitrate, psnr)
+    return np.array(rows, np.float32), np.array(targets, np.float32), dict(ex=ex, rng=rng, eb=eb)
+
+
+def training_fields(n, seed=0):
+    """Seeded (shape, slope, fronts, rel) draws for the training set."""
+    r = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        shape = tuple(int(v) for v in r.integers(160, TRAIN_GRID[0] + 1, 3))
+        out.append(dict(shape=shape, seed=int(r.integers(1 << 40)), slope=float(r.uniform(2.0, 5.0)),
+                        fronts=int(r.integers(0, 17)), rel=float(10 ** r.uniform(-4.3, -2.7))))
+    return out
+
+
+def build_dataset(engine, specs, log=None):
+    import torch
+    X, T = [], []
+    t0 = time.time()
+    for i, sp in enumerate(specs):
+        x = grf_field(sp["shape"], sp["seed"], sp["slope"], sp["fronts"], grid=TRAIN_GRID)
+        rows, targets, _ = field_tuples(engine, x, sp["shape"], sp["rel"])
+        X.append(rows)
+        T.append(targets)
+        del x
+        if log and (i + 1) % 100 == 0:
+            log(f"fields {i + 1}/{len(specs)}: {sum(len(a) for a in X)} tuples, {time.time() - t0:.0f} s")
+            torch.cuda.empty_cache()
+    return np.concatenate(X), np.concatenate(T)
+
+
+def train(engine, X, T, epochs=40, batch=1024, lr=1e-3, seed=0, hidden=256, layers=3, log=None):
+    """Trains a fresh MLP on standardised inputs; returns (mlp, mu, sigma, history)."""
+    import torch
+    mu = X.mean(0)
+    sigma = X.std(0)
+    sigma[sigma < 1e-6] = 1.0
+    Xs = ((X - mu) / sigma).astype(np.float32)
+    net = faz_amd.MLP(engine, X.shape[1], hidden, layers, T.shape[1], seed)
+    xd = torch.from_numpy(Xs).cuda()
+    td = torch.from_numpy(np.ascontiguousarray(T, np.float32)).cuda()
+    r = np.random.default_rng(seed)
+    hist = []
+    n = X.shape[0]
+    for ep in range(epochs):
+        perm = torch.from_numpy(r.permutation(n)).cuda()
+        step_lr = lr * (0.5 * (1 + np.cos(np.pi * ep / epochs)))
+        tot = 0.0
+        for s in range(0, n - batch + 1, batch):
+            idx = perm[s:s + batch]
+            tot += net.train_step(xd[idx].contiguous(), td[idx].contiguous(), step_lr)
+        hist.append(tot / max(1, n // batch))
+        if log:
+            log(f"epoch {ep + 1}/{epochs} loss {hist[-1]:.5f}")
+    return net, mu.astype(np.float32), sigma.astype(np.float32), hist
+
+
+def save_model(path, net, mu, sigma, meta=None):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez_compressed(path, params=net.get_params(), mu=mu, sigma=sigma, K=np.array(net.K), N=np.array(net.N),
+                        in_dim=np.array([net.in_dim]), out_dim=np.array([net.out_dim]),
+                        meta=np.array(repr(meta or {})))
+
+
+def load_model(engine, path=MODEL_PATH, hidden=256, layers=3):
+    z = np.load(path, allow_pickle=False)
+    net = faz_amd.MLP(engine, int(z["in_dim"][0]), hidden, layers, int(z["out_dim"][0]), 0)
+    if list(z["K"]) != net.K or list(z["N"]) != net.N:
+        raise faz_amd.FazError(-1, f"model layout {list(z['K'])}x{list(z['N'])} does not match the network")
+    net.set_params(z["params"])
+    return net, z["mu"], z["sigma"]
+
+
+def final_stats(engine, x, shape, eb, params):
+    """Compressed stream size and PSNR of the engine with `params` on the field."""
+    import torch
+    recon = torch.empty_like(x)
+    raw = engine.interp_raw(x, params, recon=recon)
+    mse = torch.mean((recon.double() - x.double()) ** 2).item()
+    return len(raw), mse
+
+
+def evaluate(engine, net, mu, sigma, shape, seed, rel, slope=11.0 / 3.0, fronts=8):
+    """Exhaustive vs learned vs default (setABwithRelBound, no alpha/beta
+    search) on one field: tuning times, engine buffer sizes, PSNRs."""
+    import torch
+    x = grf_field(shape, seed, slope, fronts)
+    mn, mx = engine.range(x)
+    rng = float(np.float32(mx - mn))
+    eb = rel * rng
+    out = {}
+    for name in ("exhaustive", "learned"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if name == "exhaustive":
+            r = engine.tune_qoz(x, shape, eb, rel, rng)
+        else:
+            r = engine.tune_learned(x, shape, eb, rel, rng, net, mu, sigma)
+        torch.cuda.synchronize()
+        out[name] = dict(tune_ms=(time.perf_counter() - t0) * 1e3, alpha=r["alpha"], beta=r["beta"], ops=r["ops"],
+                         dirs=r["dirs"], ex=r)
+    ex = out["exhaustive"]["ex"]
+    a0, b0 = default_ab(rel)
+    out["default"] = dict(tune_ms=0.0, alpha=a0, beta=b0, ops=ex["ops"], dirs=ex["dirs"])
+    n = x.numel()
+    for name, o in out.items():
+        p = faz_amd.make_params(shape, eb, algo=ex["interp_algo"], direction=ex["direction"], alpha=o["alpha"],
+                                beta=o["beta"], max_step=ex["max_step"], ops=tuple(o["ops"]), dirs=tuple(o["dirs"]))
+        nbytes, mse = final_stats(engine, x, shape, eb, p)
+        o["bitrate"] = 8.0 * nbytes / n
+        o["psnr"] = 20 * np.log10(rng) - 10 * np.log10(mse)
+        o.pop("ex", None)
+    del x
+    return out
+
+
+def default_ab(rel):
+    """setABwithRelBound, configuration 0 (include/QoZ/api/impl/SZInterp.hpp:1394-1414)."""
+    if rel >= 0.01:
+        return 2.0, 2.0
+    if rel >= 0.007:
+        return 1.75, 2.0
+    if rel >= 0.004:
+        return 1.5, 2.0
+    if rel > 0.001:
+        return 1.25, 1.5
+    return 1.0, 1.0
+
+
+def ratio_at_equal_psnr(bitrate, psnr, ref_psnr, bits=32.0, slope_db_per_bit=6.02):
+    """Compression ratio shifted to the reference PSNR along the high-rate
+    rate-distortion slope (6.02 dB per bit)."""
+    return bits / (bitrate + (ref_psnr - psnr) / slope_db_per_bit)

@@ -247,6 +247,12 @@ faz_status faz_gpu_compress_test_f32(faz_gpu_ctx *ctx, const float *d_in, const 
                                      const faz_interp_params *p, const size_t *starts, size_t nblocks, size_t sbs,
                                      double rng, uint8_t *h_raw, size_t cap, size_t *raw_size, size_t *bytes,
                                      double *square_error);
+/* The same for ncand candidates (same N, maxStep, interpBlockSize) on the
+ * same blocks, in one launch: bytes[c] and square_error[c] per candidate. */
+faz_status faz_gpu_compress_test_batch_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims,
+                                           const faz_interp_params *ps, size_t ncand, const size_t *starts,
+                                           size_t nblocks, size_t sbs, double rng, size_t *bytes,
+                                           double *square_error);
 
 /* SZ_compress with cmprAlgo = ALGO_INTERP_LORENZO and conf.QoZ (`faz -q`,
  * SZInterp.hpp:2562-2899): the QoZ tuner above, then the interpolation
@@ -275,6 +281,34 @@ faz_status faz_mlp_train_step(faz_gpu_ctx *ctx, faz_mlp *net, const float *d_x, 
                               float lr, double *loss);
 faz_status faz_mlp_get_params(faz_gpu_ctx *ctx, const faz_mlp *net, float *h, size_t cap, size_t *n);
 faz_status faz_mlp_set_params(faz_gpu_ctx *ctx, faz_mlp *net, const float *h, size_t n);
+
+/* --- Learned tuner: the QoZ tuner with the alpha/beta CompressTest search
+ * replaced by the MLP's predictions of the same bitrate and PSNR (no
+ * counterpart in the reference).  Block statistics of the sample blocks
+ * (FAZ_LEARNED_FEATURES per block, of u = (v - vmin) / rng):
+ *   0 mean, 1 std, 2 min, 3 max; for strides 1, 2, 4 (base 4 + 9 * i): mean
+ *   |first difference| per axis (3), its RMS per axis (3), mean |Laplacian|,
+ *   RMS Laplacian, max |Laplacian|; 31-33 lag-1 and 34-36 lag-2 correlation
+ *   per axis; 37 kurtosis of the stride-1 Laplacian; 38 share of it above
+ *   twice its RMS; 39 share of stride-1 differences below eb_norm.
+ * The MLP input row (FAZ_LEARNED_INPUTS): those statistics averaged over the
+ * blocks (magnitudes as log2 of their ratio to eb_norm = eb / rng), alpha - 1,
+ * beta / 4, log2 beta, level-1 interpolator and direction flag, their means
+ * over the levels, log10 of the relative bound, log2 of the bound's scale.
+ * Targets: bitrate / 4 and (PSNR + 20 log10(eb_norm * scale)) / 5. */
+#define FAZ_LEARNED_FEATURES 40
+#define FAZ_LEARNED_INPUTS 49
+faz_status faz_gpu_block_features_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims,
+                                      const size_t *starts, size_t nblocks, size_t sbs, float vmin, float rng,
+                                      double eb_norm, float *h_feats);
+faz_status faz_learned_input(const float *block_feats, size_t nblocks, double eb_norm, double rel_eb, double alpha,
+                             double beta, double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs,
+                             float *row);
+faz_status faz_learned_target(double bitrate, double psnr, double eb_norm, double eb_scale, float *t);
+/* The tuner with the trained network (inputs standardised as (x - mu) / sigma);
+ * out as faz_gpu_tune_qoz_f32, with predicted tests (test_bytes 0). */
+faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_mlp *net,
+                                    const float *mu, const float *sigma, faz_tune_out *out);
 
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
  * milliseconds, measured with HIP events on the context stream:

@@ -131,3 +131,24 @@ def test_qoz_stream(engine, shape, rel):
     dec = oracle.decompress_stream(s, x.size)
     assert np.array_equal(dec.view(np.uint32), ref.view(np.uint32))
     assert np.array_equal(engine.sz_decompress(s, x.size).view(np.uint32), ref.view(np.uint32))
+
+
+def test_compress_test_batch_matches_oracle(engine):
+    """Batched CompressTest = one CompressTest per candidate, against the oracle."""
+    import faz_amd
+    x = _grf((130, 120, 110), seed=9, fronts=3)
+    rng = float(np.float32(x.max() - x.min()))
+    starts = np.array([[0, 0, 0], [64, 32, 16], [96, 80, 70]], np.int64)
+    sbs = 32
+    cands = [dict(eb=1e-3 * rng * s, alpha=a, beta=b) for a, b, s in
+             [(1.0, 1.5, 1.0), (1.5, 3.0, 0.9), (2.0, 4.0, 1.25), (1.25, 2.0, 1.0)]]
+    ops, dirs = (1, 0, 1, 1), (0, 5, 0, 5)
+    ps = [faz_amd.make_params((33, 33, 33), c["eb"], alpha=c["alpha"], beta=c["beta"], max_step=32, ops=ops,
+                              dirs=dirs) for c in cands]
+    print("[gpu] batched CompressTest", flush=True)
+    nb, sq = engine.compress_test_batch(torch.from_numpy(x.ravel()).cuda(), x.shape, starts, sbs, ps, rng)
+    for c, b, s in zip(cands, nb, sq):
+        p = oracle.make_params((33, 33, 33), c["eb"], alpha=c["alpha"], beta=c["beta"], max_step=32, ops=ops,
+                               dirs=dirs)
+        _, ob, osq = oracle.compress_test(x, starts, sbs, p, rng)
+        assert (b, s) == (ob, osq)
