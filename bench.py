@@ -27,6 +27,7 @@ import subprocess
 import sys
 import tempfile
 import time
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
@@ -117,6 +118,64 @@ def cpu_baseline(x, dims, rel):
             "sample": "256^3 crop of the field, REL %g, oracle port incl. zstd" % rel}
 
 
+def tuner_report(eng, x, dims, rel, mn, mx):
+    """The exhaustive QoZ tuner (faz -q) and the learned tuner on the bench
+    field: tuning times (best of 3, device + host), chosen alpha/beta, the
+    engine buffer size and PSNR of each choice, the ratio of the learned
+    choice moved to the exhaustive choice's PSNR (6.02 dB per bit), and the
+    oracle's CPU restatement of the exhaustive tuner on one thread."""
+    import torch
+    import faz_amd
+    from faz_amd import learned
+    net, mu, sigma = learned.load_model(eng)
+    rng = float(np.float32(mx - mn))
+    eb = rel * rng
+    n = x.numel()
+    res = {}
+    for name in ("exhaustive", "learned"):
+        best = None
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = eng.tune_qoz(x, dims, eb, rel, rng) if name == "exhaustive" else \
+                eng.tune_learned(x, dims, eb, rel, rng, net, mu, sigma)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            best = ms if best is None else min(best, ms)
+        p = faz_amd.make_params(dims, eb, algo=r["interp_algo"], direction=r["direction"], alpha=r["alpha"],
+                                beta=r["beta"], max_step=r["max_step"], ops=tuple(r["ops"]), dirs=tuple(r["dirs"]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        size = eng.compress_device(x, p, False, eb)
+        torch.cuda.synchronize()
+        cms = (time.perf_counter() - t0) * 1e3
+        nbytes, mse = learned.final_stats(eng, x, dims, eb, p)
+        res[name] = {"tune_ms": round(best, 3), "compress_ms": round(cms, 3), "alpha": r["alpha"], "beta": r["beta"],
+                     "ops": r["ops"], "dirs": r["dirs"], "compress_tests": len(r["tests"]), "raw_bytes": size,
+                     "bitrate": round(8.0 * nbytes / n, 5), "psnr_db": round(20 * math.log10(rng) -
+                                                                             10 * math.log10(mse), 4)}
+    ex, le = res["exhaustive"], res["learned"]
+    out = dict(res)
+    out["tuning_speedup"] = round(ex["tune_ms"] / le["tune_ms"], 2)
+    out["end_to_end_speedup"] = round((ex["tune_ms"] + ex["compress_ms"]) / (le["tune_ms"] + le["compress_ms"]), 2)
+    out["ratio_exhaustive"] = round(32.0 / ex["bitrate"], 4)
+    out["ratio_learned_at_equal_psnr"] = round(learned.ratio_at_equal_psnr(le["bitrate"], le["psnr_db"],
+                                                                           ex["psnr_db"]), 4)
+    net.close()
+    try:
+        import oracle
+        host = x.cpu().numpy().reshape(dims)
+        t0 = time.perf_counter()
+        o = oracle.qoz_tune(host, eb, rel, rng)
+        out["cpu_oracle_exhaustive_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["cpu_oracle_same_choice"] = (o["alpha"], o["beta"], o["ops"], o["dirs"]) == (ex["alpha"], ex["beta"],
+                                                                                         ex["ops"], ex["dirs"])
+    except Exception as e:
+        out["cpu_oracle_exhaustive_ms"] = None
+        out["cpu_oracle_error"] = str(e)[:200]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +185,7 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tuner", action="store_true", help="skip the exhaustive / learned tuner comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one device copy of the field (4n bytes read, 4n written) before the steps, the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE in rocprofv3 --pmc runs")
@@ -211,6 +271,11 @@ def main():
                       "hist_outliers": round(timings[2], 4), "tree": round(timings[3], 4),
                       "encode": round(timings[4], 4), "device_total": round(timings[5], 4)},
     }
+    if not args.no_tuner:
+        try:
+            line["tuner"] = tuner_report(eng, x, dims, args.rel, mn, mx)
+        except Exception as e:  # reported beside the measurement
+            line["tuner"] = {"error": str(e)[:200]}
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(x, dims, args.rel)
