@@ -373,3 +373,111 @@ def compress_test(field, starts, sbs, params, rng):
                                 _ptr(raw), cap, ctypes.byref(rs), ctypes.byref(by), ctypes.byref(sq))
     assert rc == 0, rc
     return raw[:rs.value].tobytes(), by.value, sq.value
+
+
+class GFParams(ctypes.Structure):
+    """Mirror of oracle_gf_params (faz_oracle.h)."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("eb", ctypes.c_double),
+        ("block_size", ctypes.c_int),
+        ("lorenzo", ctypes.c_int),
+        ("lorenzo2", ctypes.c_int),
+        ("radius", ctypes.c_int),
+    ]
+
+
+def make_gf_params(dims, eb, block_size=5, lorenzo=True, lorenzo2=True, radius=32768):
+    p = GFParams()
+    p.N = len(dims)
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    p.block_size = int(block_size)
+    p.lorenzo = int(bool(lorenzo))
+    p.lorenzo2 = int(bool(lorenzo2))
+    p.radius = int(radius)
+    return p
+
+
+def _gf_lib():
+    L = lib()
+    vp, sz = ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)
+    P = ctypes.POINTER(GFParams)
+    L.oracle_gf_compress.argtypes = [P, vp, vp, vp, vp, sz, vp, sz]
+    L.oracle_gf_raw.argtypes = [P, vp, vp, ctypes.c_size_t, sz]
+    L.oracle_gf_decompress_raw.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                           ctypes.c_size_t]
+    L.oracle_gf_compress_stream.argtypes = [P, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_size_t, sz]
+    L.oracle_compress_test_lorenzo.argtypes = [ctypes.c_int, ctypes.c_double, vp, vp, vp, ctypes.c_size_t,
+                                               ctypes.c_size_t, vp, ctypes.c_size_t, sz, sz,
+                                               ctypes.POINTER(ctypes.c_double)]
+    return L
+
+
+def gf_compress(data, params):
+    """General frontend with the Lorenzo predictors. Returns dict(codes,
+    recon, selection, unpred)."""
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    n = data.size
+    nb = 1
+    for i in range(params.N):
+        nb *= (params.dims[i] - 1) // params.block_size + 1
+    codes = np.zeros(n, np.int32)
+    recon = np.zeros(n, np.float32)
+    sel = np.zeros(nb, np.int32)
+    unpred = np.zeros(n, np.float32)
+    ns, nu = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = _gf_lib().oracle_gf_compress(ctypes.byref(params), _ptr(data), _ptr(codes), _ptr(recon), _ptr(sel),
+                                      ctypes.byref(ns), _ptr(unpred), ctypes.byref(nu))
+    assert rc == 0, rc
+    return dict(codes=codes, recon=recon, selection=sel[:ns.value].copy(), unpred=unpred[:nu.value].copy())
+
+
+def gf_raw(data, params):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = _gf_lib().oracle_gf_raw(ctypes.byref(params), _ptr(data), _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def gf_decompress_raw(raw, dims, lorenzo=True, lorenzo2=True):
+    buf = np.frombuffer(raw, np.uint8).copy()
+    num = int(np.prod(dims))
+    out = np.zeros(num, np.float32)
+    rc = _gf_lib().oracle_gf_decompress_raw(_ptr(buf), buf.size, len(dims), int(bool(lorenzo)),
+                                            int(bool(lorenzo2)), _ptr(out), num)
+    assert rc == 0, rc
+    return out
+
+
+def gf_compress_stream(data, params, eb, rel=True):
+    data = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    cap = 64 * 1024 + data.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    rc = _gf_lib().oracle_gf_compress_stream(ctypes.byref(params), 1 if rel else 0, float(eb), _ptr(data),
+                                             _ptr(out), cap, ctypes.byref(n))
+    assert rc == 0, rc
+    return out[:n.value].tobytes()
+
+
+def compress_test_lorenzo(field, starts, sbs, eb):
+    """CompressTest's Lorenzo variant over the sample blocks at `starts`:
+    (raw bytes, compressed size, squared error)."""
+    field = np.ascontiguousarray(field, dtype=np.float32)
+    fd = np.array(list(field.shape) + [1] * (3 - field.ndim), np.uint64)
+    st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+    cap = 64 * 1024 + st.shape[0] * (sbs + 1) ** field.ndim * 16
+    raw = np.zeros(cap, np.uint8)
+    rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+    rc = _gf_lib().oracle_compress_test_lorenzo(field.ndim, float(eb), _ptr(field), _ptr(fd), _ptr(st), st.shape[0],
+                                                sbs, _ptr(raw), cap, ctypes.byref(rs), ctypes.byref(by),
+                                                ctypes.byref(sq))
+    assert rc == 0, rc
+    return raw[:rs.value].tobytes(), by.value, sq.value

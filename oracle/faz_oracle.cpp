@@ -1278,6 +1278,294 @@ static bool lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, 
 }
 
 // ---------------------------------------------------------------------------
+// General frontend: SZGeneralFrontend (include/QoZ/frontend/
+// SZGeneralFrontend.hpp:36-71, 110-125) with ComposedPredictor{Lorenzo 1-layer,
+// Lorenzo 2-layer} (ComposedPredictor.hpp:45-63, 70-93, 203-249) or one of
+// them alone (make_lorenzo_regression_compressor, SZLorenzoReg.hpp:27-91),
+// N = 1..4.  Blocks of conf.blockSize in raster order; the data array is
+// overwritten with the reconstruction point by point, so predictions read
+// reconstructed values behind the current point and originals elsewhere.
+// ---------------------------------------------------------------------------
+struct GFEngine {
+    int N = 3;
+    size_t dims[4] = {1, 1, 1, 1}, st[4] = {0, 0, 0, 0};
+    int bs = 5;
+    int npred = 2;
+    int layer[2] = {1, 2};   // predictor list order: 1-layer first (LR:39-67)
+    float noise[3] = {0, 0, 0};
+    double eb = 0;
+    Quantizer quantizer;
+    std::vector<int32_t> codes, selection;
+
+    bool set(const oracle_gf_params *p) {
+        N = p->N;
+        if (N < 1 || N > 4 || p->block_size < 1) return false;
+        if (!p->lorenzo && !p->lorenzo2) return false;
+        if (p->lorenzo2 && p->block_size < 2) return false;  // a 2-layer read would leave the array
+        npred = 0;
+        if (p->lorenzo) layer[npred++] = 1;
+        if (p->lorenzo2) layer[npred++] = 2;
+        bs = p->block_size;
+        size_t s = 1;
+        for (int d = N - 1; d >= 0; d--) {
+            dims[d] = p->dims[d];
+            if (!dims[d]) return false;
+            st[d] = s;
+            s *= dims[d];
+        }
+        eb = p->eb;
+        // LorenzoPredictor(double eb) noise terms (LP:25-47), stored as T
+        static const double n1[4] = {0.5, 0.81, 1.22, 1.79};
+        static const double n2[4] = {1.08, 2.76, 6.8, 0.0};
+        noise[1] = (float) (n1[N - 1] * eb);
+        noise[2] = (float) (n2[N - 1] * eb);
+        quantizer.set_eb(eb);
+        quantizer.radius = p->radius;
+        return true;
+    }
+
+    size_t num() const {
+        size_t n = 1;
+        for (int d = 0; d < N; d++) n *= dims[d];
+        return n;
+    }
+
+    // iterator.prev (Iterator.hpp:283-294): 0 when an offset crosses the low
+    // face of the field (bs >= 2 for 2-layer reads keeps every other read in it)
+    static float prev(const float *v, const size_t *c, const size_t *st, int N, const int *a) {
+        size_t off = 0;
+        for (int d = 0; d < N; d++) {
+            if (c[d] < (size_t) a[d]) return 0.0f;
+            off += (size_t) a[d] * st[d];
+        }
+        return *(v - off);
+    }
+
+    // LorenzoPredictor::do_predict (LP:117-247), float, left to right;
+    // N = 4 uses the 1-layer formula for both layer counts (LP:170-177)
+    float predict(const float *v, const size_t *c, int L) const {
+        auto f1 = [&](int a) { int o[1] = {a}; return prev(v, c, st, 1, o); };
+        auto f2 = [&](int a, int b) { int o[2] = {a, b}; return prev(v, c, st, 2, o); };
+        auto f3 = [&](int a, int b, int e) { int o[3] = {a, b, e}; return prev(v, c, st, 3, o); };
+        auto f4 = [&](int a, int b, int e, int g) { int o[4] = {a, b, e, g}; return prev(v, c, st, 4, o); };
+        switch (N) {
+            case 1:
+                if (L == 1) return f1(1);
+                return 2 * f1(1) - f1(2);
+            case 2:
+                if (L == 1) return f2(0, 1) + f2(1, 0) - f2(1, 1);
+                return 2 * f2(0, 1) - f2(0, 2) + 2 * f2(1, 0) - 4 * f2(1, 1) + 2 * f2(1, 2) - f2(2, 0) +
+                       2 * f2(2, 1) - f2(2, 2);
+            case 3:
+                if (L == 1)
+                    return f3(0, 0, 1) + f3(0, 1, 0) + f3(1, 0, 0) - f3(0, 1, 1) - f3(1, 0, 1) - f3(1, 1, 0) +
+                           f3(1, 1, 1);
+                return 2 * f3(0, 0, 1) - f3(0, 0, 2) + 2 * f3(0, 1, 0) - 4 * f3(0, 1, 1) + 2 * f3(0, 1, 2) -
+                       f3(0, 2, 0) + 2 * f3(0, 2, 1) - f3(0, 2, 2) + 2 * f3(1, 0, 0) - 4 * f3(1, 0, 1) +
+                       2 * f3(1, 0, 2) - 4 * f3(1, 1, 0) + 8 * f3(1, 1, 1) - 4 * f3(1, 1, 2) + 2 * f3(1, 2, 0) -
+                       4 * f3(1, 2, 1) + 2 * f3(1, 2, 2) - f3(2, 0, 0) + 2 * f3(2, 0, 1) - f3(2, 0, 2) +
+                       2 * f3(2, 1, 0) - 4 * f3(2, 1, 1) + 2 * f3(2, 1, 2) - f3(2, 2, 0) + 2 * f3(2, 2, 1) -
+                       f3(2, 2, 2);
+            default:
+                return f4(0, 0, 0, 1) + f4(0, 0, 1, 0) - f4(0, 0, 1, 1) + f4(0, 1, 0, 0) - f4(0, 1, 0, 1) -
+                       f4(0, 1, 1, 0) + f4(0, 1, 1, 1) + f4(1, 0, 0, 0) - f4(1, 0, 0, 1) - f4(1, 0, 1, 0) +
+                       f4(1, 0, 1, 1) - f4(1, 1, 0, 0) + f4(1, 1, 0, 1) + f4(1, 1, 1, 0) - f4(1, 1, 1, 1);
+        }
+    }
+
+    // LorenzoPredictor::estimate_error (LP:102-104): |x - pred| + noise in T
+    float estimate(const float *data, const size_t *c, int L) const {
+        size_t idx = 0;
+        for (int d = 0; d < N; d++) idx += c[d] * st[d];
+        float x = data[idx];
+        return std::fabs(x - predict(data + idx, c, L)) + noise[L];
+    }
+
+    // ComposedPredictor::do_estimate_error (CP:203-249): predictor with the
+    // smallest summed estimate over the block diagonals, first on ties.
+    int select(const float *data, const size_t *o, const size_t *sz) const {
+        double pe[2] = {0, 0};
+        size_t m = sz[0];
+        for (int d = 1; d < N; d++) m = std::min(m, sz[d]);
+        size_t c[4];
+        if (N == 1) {
+            for (int p = 0; p < npred; p++) {
+                c[0] = o[0];
+                pe[p] += estimate(data, c, layer[p]);
+                c[0] = o[0] + m - 1;
+                pe[p] += estimate(data, c, layer[p]);
+            }
+        } else {
+            const int ndiag = 1 << (N - 1);
+            for (size_t k = 0; k + 2 < m; k++)
+                for (int p = 0; p < npred; p++)
+                    for (int t = 0; t < ndiag; t++) {
+                        c[0] = o[0] + k;
+                        for (int d = 1; d < N; d++)
+                            c[d] = o[d] + (((t >> (N - 1 - d)) & 1) ? m - 1 - k : k);
+                        pe[p] += estimate(data, c, layer[p]);
+                    }
+        }
+        int sid = 0;
+        for (int p = 1; p < npred; p++)
+            if (pe[p] < pe[sid]) sid = p;
+        return sid;
+    }
+
+    // SZGeneralFrontend::compress (GF:36-71) on one array, appending codes,
+    // selections and outliers to the engine's lists (the tuner reuses one
+    // compressor across its sample blocks).
+    void compress(float *data) {
+        size_t nb[4], b[4] = {0, 0, 0, 0};
+        size_t total = 1;
+        for (int d = 0; d < N; d++) {
+            nb[d] = (dims[d] - 1) / bs + 1;
+            total *= nb[d];
+        }
+        for (size_t t = 0; t < total; t++) {
+            size_t o[4] = {0, 0, 0, 0}, sz[4] = {1, 1, 1, 1};
+            for (int d = 0; d < N; d++) {
+                o[d] = b[d] * bs;
+                sz[d] = b[d] == nb[d] - 1 ? dims[d] - o[d] : (size_t) bs;
+            }
+            int L = layer[0];
+            if (npred > 1) {
+                int sid = select(data, o, sz);
+                selection.push_back(sid);
+                L = layer[sid];
+            }
+            size_t bn = 1;
+            for (int d = 0; d < N; d++) bn *= sz[d];
+            size_t l[4] = {0, 0, 0, 0};
+            for (size_t e = 0; e < bn; e++) {
+                size_t c[4], idx = 0;
+                for (int d = 0; d < N; d++) {
+                    c[d] = o[d] + l[d];
+                    idx += c[d] * st[d];
+                }
+                float pred = predict(data + idx, c, L);
+                codes.push_back(quantizer.quantize_and_overwrite(data[idx], pred));
+                for (int d = N - 1; d >= 0; d--) {
+                    if (++l[d] < sz[d]) break;
+                    l[d] = 0;
+                }
+            }
+            for (int d = N - 1; d >= 0; d--) {
+                if (++b[d] < nb[d]) break;
+                b[d] = 0;
+            }
+        }
+    }
+
+    // SZGeneralFrontend::decompress (GF:72-96) given the codes and selections
+    void decompress(const int32_t *cds, float *out) {
+        size_t nb[4], b[4] = {0, 0, 0, 0};
+        size_t total = 1;
+        for (int d = 0; d < N; d++) {
+            nb[d] = (dims[d] - 1) / bs + 1;
+            total *= nb[d];
+        }
+        size_t ci = 0;
+        quantizer.index = 0;
+        for (size_t t = 0; t < total; t++) {
+            size_t o[4] = {0, 0, 0, 0}, sz[4] = {1, 1, 1, 1};
+            for (int d = 0; d < N; d++) {
+                o[d] = b[d] * bs;
+                sz[d] = b[d] == nb[d] - 1 ? dims[d] - o[d] : (size_t) bs;
+            }
+            int L = npred > 1 ? layer[selection[t]] : layer[0];
+            size_t bn = 1;
+            for (int d = 0; d < N; d++) bn *= sz[d];
+            size_t l[4] = {0, 0, 0, 0};
+            for (size_t e = 0; e < bn; e++) {
+                size_t c[4], idx = 0;
+                for (int d = 0; d < N; d++) {
+                    c[d] = o[d] + l[d];
+                    idx += c[d] * st[d];
+                }
+                out[idx] = quantizer.recover(predict(out + idx, c, L), cds[ci++]);
+                for (int d = N - 1; d >= 0; d--) {
+                    if (++l[d] < sz[d]) break;
+                    l[d] = 0;
+                }
+            }
+            for (int d = N - 1; d >= 0; d--) {
+                if (++b[d] < nb[d]) break;
+                b[d] = 0;
+            }
+        }
+    }
+
+    // SZGeneralFrontend::save (GF:98-103) + quantizer, then the codes'
+    // Huffman tree and bits (SGC:30-71 / encoding_lossless SGC:72-91)
+    bool save(std::vector<uint8_t> &out) const {
+        for (int d = 0; d < N; d++) put(out, (uint64_t) dims[d]);
+        put(out, (uint32_t) bs);
+        for (int p = 0; p < npred; p++) out.push_back(1);  // LorenzoPredictor::save (LP:57-59)
+        if (npred > 1) {                                    // ComposedPredictor::save (CP:70-81)
+            put(out, (uint64_t) selection.size());
+            if (!selection.empty()) {
+                Huffman hs;
+                if (!hs.init(selection.data(), selection.size())) return false;
+                hs.save(out);
+                hs.encode(selection.data(), selection.size(), out);
+            }
+        }
+        out.push_back(2);  // LinearQuantizer::save (Q:185-201)
+        put(out, eb);
+        put(out, (int) quantizer.radius);
+        put(out, (size_t) quantizer.unpred.size());
+        const uint8_t *u = (const uint8_t *) quantizer.unpred.data();
+        out.insert(out.end(), u, u + quantizer.unpred.size() * sizeof(float));
+        Huffman h;
+        if (!h.init(codes.data(), codes.size())) return false;
+        h.save(out);
+        h.encode(codes.data(), codes.size(), out);
+        return true;
+    }
+};
+
+// SGC:115-140 + GF:105-116 + CP:82-93 on a pre-zstd buffer; N and the
+// predictor list come from the Config (lorenzo, lorenzo2).
+static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int lo2, float *out, size_t num) {
+    const uint8_t *p = raw, *end = raw + n;
+    oracle_gf_params prm;
+    memset(&prm, 0, sizeof(prm));
+    prm.N = N;
+    prm.lorenzo = lo, prm.lorenzo2 = lo2;
+    if (n < (size_t) N * 8 + 4) return false;
+    for (int d = 0; d < N; d++) prm.dims[d] = (size_t) take<uint64_t>(p);
+    prm.block_size = (int) take<uint32_t>(p);
+    prm.eb = 1.0;
+    prm.radius = 32768;
+    GFEngine e;
+    if (!e.set(&prm) || e.num() != num) return false;
+    p += e.npred;  // predictor ids
+    size_t nblk = 1;
+    for (int d = 0; d < N; d++) nblk *= (e.dims[d] - 1) / e.bs + 1;
+    if (e.npred > 1) {
+        size_t ns = take<uint64_t>(p);
+        if (ns != nblk) return false;
+        e.selection.resize(ns);
+        HuffDecoder hs;
+        size_t used = hs.decode(p, end - p, e.selection.data(), ns);
+        if (!used) return false;
+        p += used;
+    }
+    p += 1;  // quantizer uid
+    e.quantizer.set_eb(take<double>(p));
+    e.quantizer.radius = take<int>(p);
+    size_t U = take<size_t>(p);
+    e.quantizer.unpred.assign((const float *) p, (const float *) p + U);
+    p += U * sizeof(float);
+    std::vector<int32_t> codes(num);
+    HuffDecoder hd;
+    if (!hd.decode(p, end - p, codes.data(), num)) return false;
+    e.decompress(codes.data(), out);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // QoZ exhaustive sampling tuner (SZInterp.hpp:1589-2557, QoZ path, RD target,
 // no wavelets, no Lorenzo test).
 // ---------------------------------------------------------------------------
@@ -1634,8 +1922,15 @@ int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t
     size_t z = ZSTD_decompress(raw.data(), rawLen, stream + 8, (n - 4 - confSize) - 8);
     if (ZSTD_isError(z) || z != rawLen) return -4;
     if (algo == 0) {
-        if (N != 3 || dims[0] * dims[1] * dims[2] != num) return -1;
-        return lr_decompress_raw(raw.data(), raw.size(), dims, out) ? 0 : -2;
+        // SZ_decompress_LorenzoReg (LR:117-135): fast frontend for 3-D without
+        // regression2, else the general frontend with the Config's predictors
+        const uint8_t *f = c + 44 + 8 * N;  // lorenzo, lorenzo2, regression, regression2
+        if (N == 3 && !f[3]) {
+            if (dims[0] * dims[1] * dims[2] != num) return -1;
+            return lr_decompress_raw(raw.data(), raw.size(), dims, out) ? 0 : -2;
+        }
+        if (f[2] || f[3]) return -1;  // regression predictors: not restated
+        return gf_decompress_raw(raw.data(), raw.size(), N, f[0], f[1], out, num) ? 0 : -2;
     }
     return interp_decompress_raw(raw.data(), raw.size(), N, out, num) ? 0 : -2;
 }
@@ -1752,6 +2047,138 @@ int oracle_compress_test(const oracle_interp_params *p, const float *field, cons
     double br, ps;
     std::vector<uint8_t> r;
     compress_test(ts, *p, rng, br, ps, *bytes, &r, square_error);
+    *raw_size = r.size();
+    if (r.size() > cap) return -3;
+    memcpy(raw, r.data(), r.size());
+    return 0;
+}
+
+int oracle_gf_compress(const oracle_gf_params *p, const float *data, int32_t *codes, float *recon,
+                       int32_t *selection, size_t *n_sel, float *unpred, size_t *n_unpred) {
+    GFEngine e;
+    if (!p || !e.set(p)) return -1;
+    std::vector<float> v(data, data + e.num());
+    e.compress(v.data());
+    if (codes) memcpy(codes, e.codes.data(), e.codes.size() * sizeof(int32_t));
+    if (recon) memcpy(recon, v.data(), v.size() * sizeof(float));
+    if (selection) memcpy(selection, e.selection.data(), e.selection.size() * sizeof(int32_t));
+    if (n_sel) *n_sel = e.selection.size();
+    if (unpred) memcpy(unpred, e.quantizer.unpred.data(), e.quantizer.unpred.size() * sizeof(float));
+    if (n_unpred) *n_unpred = e.quantizer.unpred.size();
+    return 0;
+}
+
+static bool gf_raw(const oracle_gf_params *p, const float *data, std::vector<uint8_t> &out) {
+    GFEngine e;
+    if (!e.set(p)) return false;
+    std::vector<float> v(data, data + e.num());
+    e.compress(v.data());
+    out.clear();
+    return e.save(out);
+}
+
+int oracle_gf_raw(const oracle_gf_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out) {
+    if (!p) return -1;
+    std::vector<uint8_t> buf;
+    if (!gf_raw(p, data, buf)) return -2;
+    *n_out = buf.size();
+    if (buf.size() > cap) return -3;
+    memcpy(out, buf.data(), buf.size());
+    return 0;
+}
+
+int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, float *out,
+                             size_t num) {
+    return gf_decompress_raw(raw, n, N, lorenzo, lorenzo2, out, num) ? 0 : -2;
+}
+
+int oracle_gf_compress_stream(const oracle_gf_params *p, int rel_mode, double eb_in, const float *data,
+                              uint8_t *out, size_t cap, size_t *n_out) {
+    if (!p || p->N < 1 || p->N > 4) return -1;
+    size_t num = 1;
+    for (int d = 0; d < p->N; d++) num *= p->dims[d];
+    oracle_gf_params q = *p;
+    double rel = -1.0;
+    if (rel_mode) {
+        float mn, mx;
+        oracle_range(data, num, &mn, &mx);
+        rel = eb_in;
+        q.eb = rel * (double) (float) (mx - mn);
+    } else {
+        q.eb = eb_in;
+    }
+    std::vector<uint8_t> raw;
+    if (!gf_raw(&q, data, raw)) return -2;
+    std::vector<uint8_t> s(8 + ZSTD_compressBound(raw.size()));
+    uint64_t rawLen = raw.size();
+    memcpy(s.data(), &rawLen, 8);
+    size_t z = ZSTD_compress(s.data() + 8, s.size() - 8, raw.data(), raw.size(), 3);
+    if (ZSTD_isError(z)) return -4;
+    s.resize(8 + z);
+    oracle_interp_params ip;
+    memset(&ip, 0, sizeof(ip));
+    ip.N = q.N;
+    for (int i = 0; i < q.N; i++) ip.dims[i] = q.dims[i];
+    ip.interp_algo = 1;
+    ip.alpha = -1, ip.beta = -1;
+    ip.block_size = 32;
+    ip.radius = q.radius;
+    TrailerFields tf;
+    tf.cmprAlgo = 0;  // ALGO_LORENZO_REG
+    tf.lorenzo = q.lorenzo != 0, tf.lorenzo2 = q.lorenzo2 != 0, tf.regression = false;
+    tf.blockSize = q.block_size;
+    size_t before = s.size();
+    config_save(s, &ip, q.eb, rel, tf);
+    put(s, (int) (s.size() - before));
+    *n_out = s.size();
+    if (s.size() > cap) return -3;
+    memcpy(out, s.data(), s.size());
+    return 0;
+}
+
+int oracle_compress_test_lorenzo(int N, double eb, const float *field, const size_t *fdims, const size_t *starts,
+                                 size_t nblocks, size_t sbs, uint8_t *raw, size_t cap, size_t *raw_size,
+                                 size_t *bytes, double *square_error) {
+    if (N != 2 && N != 3) return -1;
+    oracle_gf_params p;
+    memset(&p, 0, sizeof(p));
+    p.N = N;
+    for (int d = 0; d < N; d++) p.dims[d] = sbs + 1;
+    p.eb = eb;
+    p.block_size = 5;
+    p.lorenzo = p.lorenzo2 = 1;
+    p.radius = 65536;
+    GFEngine e;
+    if (!e.set(&p)) return -1;
+    const size_t per = e.num();
+    double sq = 0;
+    std::vector<float> blk, cur;
+    for (size_t b = 0; b < nblocks; b++) {
+        const size_t *s0 = starts + 3 * b;
+        blk.clear();
+        if (N == 2) {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++) blk.push_back(field[(s0[0] + i) * fdims[1] + s0[1] + j]);
+        } else {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++)
+                    for (size_t k = 0; k <= sbs; k++)
+                        blk.push_back(field[((s0[0] + i) * fdims[1] + s0[1] + j) * fdims[2] + s0[2] + k]);
+        }
+        cur = blk;
+        e.compress(cur.data());
+        for (size_t j = 0; j < per; j++) {  // SZI:1287-1292: float difference, float square
+            float v = blk[j] - cur[j];
+            sq += v * v;
+        }
+    }
+    std::vector<uint8_t> r;
+    if (!e.save(r)) return -2;
+    std::vector<uint8_t> z(ZSTD_compressBound(r.size()));
+    size_t zs = ZSTD_compress(z.data(), z.size(), r.data(), r.size(), 3);
+    if (ZSTD_isError(zs)) return -4;
+    *bytes = 8 + zs;
+    *square_error = sq;
     *raw_size = r.size();
     if (r.size() > cap) return -3;
     memcpy(raw, r.data(), r.size());

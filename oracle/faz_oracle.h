@@ -98,6 +98,38 @@ int oracle_lr_decompress_raw(const uint8_t *raw, size_t n, const size_t *dims, f
 int oracle_lr_compress_stream(const oracle_lr_params *p, int rel_mode, double eb_in, const float *data,
                               uint8_t *out, size_t cap, size_t *n_out);
 
+/* Config fields of the general frontend with the Lorenzo predictors
+ * (SZGeneralFrontend.hpp, ComposedPredictor.hpp, LorenzoPredictor.hpp as
+ * composed by SZLorenzoReg.hpp:27-91), N = 1..4. */
+typedef struct {
+    int N;
+    size_t dims[4];
+    double eb;            /* absolute error bound */
+    int block_size;       /* conf.blockSize */
+    int lorenzo, lorenzo2;
+    int radius;           /* quantbinCnt / 2 */
+} oracle_gf_params;
+
+/* SZGeneralFrontend::compress (GF:36-71). Outputs optional: codes (n,
+ * block-major), recon (n), selection (one per block when both predictors are
+ * on), unpred (capacity n). */
+int oracle_gf_compress(const oracle_gf_params *p, const float *data, int32_t *codes, float *recon,
+                       int32_t *selection, size_t *n_sel, float *unpred, size_t *n_unpred);
+/* Pre-zstd buffer of SZGeneralCompressor<SZGeneralFrontend> (SGC:30-71, GF:98-103). */
+int oracle_gf_raw(const oracle_gf_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out);
+/* SGC:115-140 on a pre-zstd buffer; lorenzo/lorenzo2 as in the Config. */
+int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, float *out,
+                             size_t num);
+/* SZ_compress stream for ALGO_LORENZO_REG through the general frontend. */
+int oracle_gf_compress_stream(const oracle_gf_params *p, int rel_mode, double eb_in, const float *data,
+                              uint8_t *out, size_t cap, size_t *n_out);
+/* CompressTest's Lorenzo variant (useFast = false): one general compressor
+ * with lorenzo + lorenzo2, blockSize 5, radius 65536 over the sample blocks
+ * of (sbs+1)^N at `starts` (int64-free: size_t triples, N = 2 or 3). */
+int oracle_compress_test_lorenzo(int N, double eb, const float *field, const size_t *fdims, const size_t *starts,
+                                 size_t nblocks, size_t sbs, uint8_t *raw, size_t cap, size_t *raw_size,
+                                 size_t *bytes, double *square_error);
+
 /* QoZ exhaustive sampling tuner (`faz -q`): Tuning() with conf.QoZ set,
  * no wavelets, tuningTarget RD (include/QoZ/api/impl/SZInterp.hpp:1589-2557),
  * with sampleBlocks (SZI:993-1127), CompressTest (SZI:1131-1392),

@@ -13,7 +13,8 @@
 //   * tuning modes 1/2 of SZInterpolationCompressor::compress (SIC:328-476);
 //   * SZ_decompress for ALGO_INTERP (sz.hpp:159-205, SZInterp.hpp:417-431);
 //   * the Huffman coder alone (HuffmanEncoder.hpp:95-227);
-//   * the Lorenzo/regression compressors (SZLorenzoReg.hpp:27-114).
+//   * the Lorenzo/regression compressors (SZLorenzoReg.hpp:27-114), fast and
+//     general frontends, and CompressTest's Lorenzo variant.
 #include "QoZ/utils/Config.hpp"
 #include "QoZ/utils/Statistic.hpp"  // QoZ::factorial, used by the engine header below
 #include "QoZ/compressor/SZInterpolationCompressor.hpp"
@@ -409,6 +410,149 @@ static int run_ctest(const Args &a) {
     return 0;
 }
 
+// make_lorenzo_regression_compressor (SZLorenzoReg.hpp:27-91) with the
+// Lorenzo predictors only: one LorenzoPredictor when a single one is enabled,
+// else ComposedPredictor{1-layer, 2-layer}, in an SZGeneralFrontend.
+template<QoZ::uint N, class Lossless>
+static QoZ::concepts::CompressorInterface<float> *general_compressor(const QoZ::Config &conf, int radius,
+                                                                      Lossless lossless) {
+    typedef float T;
+    auto q = QoZ::LinearQuantizer<T>(conf.absErrorBound, radius);
+    if (conf.lorenzo && conf.lorenzo2) {
+        std::vector<std::shared_ptr<QoZ::concepts::PredictorInterface<T, N>>> ps;
+        ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 1>>(conf.absErrorBound));
+        ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 2>>(conf.absErrorBound));
+        return QoZ::make_sz_general_compressor<T, N>(
+                QoZ::make_sz_general_frontend<T, N>(conf, QoZ::ComposedPredictor<T, N>(ps), q),
+                QoZ::HuffmanEncoder<int>(), lossless);
+    }
+    if (conf.lorenzo2)
+        return QoZ::make_sz_general_compressor<T, N>(
+                QoZ::make_sz_general_frontend<T, N>(conf, QoZ::LorenzoPredictor<T, N, 2>(conf.absErrorBound), q),
+                QoZ::HuffmanEncoder<int>(), lossless);
+    return QoZ::make_sz_general_compressor<T, N>(
+            QoZ::make_sz_general_frontend<T, N>(conf, QoZ::LorenzoPredictor<T, N, 1>(conf.absErrorBound), q),
+            QoZ::HuffmanEncoder<int>(), lossless);
+}
+
+// SZ_compress_LorenzoReg through the general frontend (LR:95-114 else-branch):
+// codes, pre-zstd buffer, reconstruction decoded from it, SZ_compress stream.
+template<QoZ::uint N>
+static int run_general(const Args &a) {
+    typedef float T;
+    std::string out = get(a, "out");
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    QoZ::Config conf = make_config(a);
+    if (input.size() != conf.num) { fprintf(stderr, "size mismatch\n"); return 2; }
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    if (a.count("block")) conf.blockSize = atoi(get(a, "block").c_str());
+    conf.lorenzo = atoi(get(a, "lo", "1").c_str()) != 0;
+    conf.lorenzo2 = atoi(get(a, "lo2", "1").c_str()) != 0;
+    conf.regression = false;
+    conf.regression2 = false;
+    conf.verbose = 0;
+    QoZ::calAbsErrorBound(conf, input.data());
+    write_bytes(out + ".eb", &conf.absErrorBound, sizeof(double));
+    const int radius = conf.quantbinCnt / 2;
+    {
+        QoZ::Config c(conf);
+        std::vector<T> data(input);
+        auto *sz = general_compressor<N>(c, radius, QoZ::Lossless_bypass());
+        size_t rawSize = 0;
+        QoZ::uchar *raw = sz->compress(c, data.data(), rawSize);
+        write_bytes(out + ".raw", raw, rawSize);
+        write_bytes(out + ".overwrite", data.data(), data.size() * sizeof(T));
+        delete sz;
+        QoZ::Config c2(conf);
+        auto *dz = general_compressor<N>(c2, radius, QoZ::Lossless_bypass());
+        std::vector<T> dec(conf.num);
+        dz->decompress(raw, rawSize, dec.data());
+        write_bytes(out + ".recon", dec.data(), dec.size() * sizeof(T));
+        delete dz;
+        delete[] raw;
+    }
+    {
+        QoZ::Config c(conf);
+        std::vector<T> data(input);
+        auto *sz = general_compressor<N>(c, radius, QoZ::Lossless_zstd());
+        size_t outSize = 0;
+        char *cmpData = (char *) sz->compress(c, data.data(), outSize);
+        QoZ::uchar *pos = (QoZ::uchar *) cmpData + outSize;
+        c.save(pos);
+        size_t newSize = (char *) pos - cmpData;
+        QoZ::write(int(newSize - outSize), pos);
+        outSize = (char *) pos - cmpData;
+        write_bytes(out + ".stream", cmpData, outSize);
+        delete sz;
+        delete[] cmpData;
+    }
+    return 0;
+}
+
+// CompressTest for ALGO_LORENZO_REG with useFast = false (SZI:1136-1170,
+// 1254-1256, 1283-1294, 1353-1392): one general compressor (lorenzo +
+// lorenzo2, blockSize 5, quantbinCnt 131072) compresses every sample block in
+// tuning mode, squared error from the overwritten blocks, then
+// encoding_lossless over the accumulated codes.
+template<QoZ::uint N>
+static int run_ctest_lorenzo(const Args &a) {
+    typedef float T;
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    std::vector<size_t> fdims = parse_list(get(a, "fdims"));
+    std::vector<int64_t> starts = read_vec<int64_t>(get(a, "starts"));
+    size_t sbs = strtoull(get(a, "sbs").c_str(), nullptr, 10);
+    QoZ::Config conf = make_config(a);  // dims = block shape
+    conf.verbose = 0;
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    conf.lorenzo = true;
+    conf.lorenzo2 = true;
+    conf.regression = false;
+    conf.regression2 = false;
+    conf.openmp = false;
+    conf.blockSize = 5;
+    conf.quantbinCnt = 65536 * 2;
+    size_t per = conf.num;
+    std::vector<std::vector<T>> blocks;
+    for (size_t b = 0; b + 2 < starts.size(); b += 3) {
+        std::vector<T> blk;
+        if (N == 2) {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++) blk.push_back(input[(starts[b] + i) * fdims[1] + starts[b + 1] + j]);
+        } else {
+            for (size_t i = 0; i <= sbs; i++)
+                for (size_t j = 0; j <= sbs; j++)
+                    for (size_t k = 0; k <= sbs; k++)
+                        blk.push_back(input[((starts[b] + i) * fdims[1] + starts[b + 1] + j) * fdims[2] + starts[b + 2] + k]);
+        }
+        blocks.push_back(blk);
+    }
+    auto *sz = general_compressor<N>(conf, conf.quantbinCnt / 2, QoZ::Lossless_zstd());
+    double square_error = 0;
+    for (auto &blk: blocks) {
+        std::vector<T> cur(blk);
+        size_t outSize = 0;
+        QoZ::uchar *d = sz->compress(conf, cur.data(), outSize, 1);
+        delete[] d;
+        for (size_t j = 0; j < per; j++) {
+            T value = blk[j] - cur[j];
+            square_error += value * value;
+        }
+    }
+    size_t total = 0;
+    QoZ::uchar *c = sz->encoding_lossless(total);
+    if (a.count("raw")) {
+        size_t len = total;
+        QoZ::Lossless_zstd z;
+        QoZ::uchar *raw = z.decompress(c, len);
+        write_bytes(get(a, "raw"), raw, len);
+        delete[] raw;
+    }
+    delete[] c;
+    delete sz;
+    printf("{\"bytes\": %zu, \"square_error\": %.17g, \"points\": %zu}\n", total, square_error, per * blocks.size());
+    return 0;
+}
+
 static int run_huffman(const Args &a) {
     std::vector<int> codes = read_vec<int>(get(a, "in"));
     QoZ::HuffmanEncoder<int> enc;
@@ -451,7 +595,20 @@ int main(int argc, char **argv) {
     if (cmd == "decompress") return run_decompress(a);
     if (cmd == "huffman") return run_huffman(a);
     if (cmd == "lorenzo") return run_lorenzo(a);
-    if (cmd == "ctest") return parse_list(get(a, "dims")).size() == 2 ? run_ctest<2>(a) : run_ctest<3>(a);
+    if (cmd == "ctest") {
+        bool lor = get(a, "ctalgo", "interp") == "lorenzo";
+        if (parse_list(get(a, "dims")).size() == 2) return lor ? run_ctest_lorenzo<2>(a) : run_ctest<2>(a);
+        return lor ? run_ctest_lorenzo<3>(a) : run_ctest<3>(a);
+    }
+    if (cmd == "general") {
+        switch (parse_list(get(a, "dims")).size()) {
+            case 1: return run_general<1>(a);
+            case 2: return run_general<2>(a);
+            case 3: return run_general<3>(a);
+            case 4: return run_general<4>(a);
+            default: return 2;
+        }
+    }
     fprintf(stderr, "unknown command %s\n", argv[1]);
     return 2;
 }

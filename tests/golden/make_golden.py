@@ -154,6 +154,55 @@ def ctest_cases():
     return c
 
 
+def general_cases():
+    """General-frontend fixtures (SZGeneralFrontend with the Lorenzo
+    predictors, N = 1..4) through SZ_compress."""
+    r = np.random.default_rng(515151)
+    c = []
+    add = lambda name, x, **kw: c.append((name, x, kw))
+    add("g1d_1000_default", smooth(r, (1000,), 2), mode="rel", eb=1e-3)
+    add("g1d_777_bs5_lo2only", smooth(r, (777,), 2), mode="rel", eb=1e-4, block=5, lo=0, lo2=1)
+    add("g1d_500_bs7_loonly", smooth(r, (500,), 1), mode="rel", eb=1e-3, block=7, lo=1, lo2=0)
+    add("g1d_301_bs2", smooth(r, (301,), 2), mode="rel", eb=1e-3, block=2)
+    add("g2d_100x130_default", smooth(r, (100, 130), 2), mode="rel", eb=1e-3)
+    add("g2d_61x47_bs5", smooth(r, (61, 47), 2), mode="rel", eb=1e-4, block=5)
+    add("g2d_33x90_bs6_lo2only", smooth(r, (33, 90), 2), mode="rel", eb=1e-3, block=6, lo=0, lo2=1)
+    add("g2d_40x41_loonly_abs", ((smooth(r, (40, 41), 1) - 0.5) * 100).astype(np.float32), mode="abs", eb=0.05,
+        lo=1, lo2=0)
+    add("g2d_30x30_bs1_loonly", smooth(r, (30, 30), 2), mode="rel", eb=1e-3, block=1, lo=1, lo2=0)
+    add("g2d_35x29_bs2", smooth(r, (35, 29), 2), mode="rel", eb=1e-3, block=2)
+    add("g2d_25_noise_tiny_eb", r.random((25, 25)).astype(np.float32), mode="abs", eb=1e-9)
+    add("g2d_50x60_qbins1024", smooth(r, (50, 60), 1), mode="rel", eb=1e-6, qbins=1024)
+    spikes = smooth(r, (40, 37))
+    spikes.flat[r.choice(spikes.size, 20, replace=False)] = np.float32(3e30)
+    add("g2d_spikes_overflow", spikes, mode="abs", eb=1e-3)
+    add("g2d_const_20", np.full((20, 20), 1.75, np.float32), mode="abs", eb=1e-2)
+    add("g2d_1x200_thin", smooth(r, (1, 200), 2), mode="rel", eb=1e-3, block=5)
+    add("g2d_200x1_thin", smooth(r, (200, 1), 2), mode="rel", eb=1e-3, block=5)
+    add("g3d_20x23x17_bs5", smooth(r, (20, 23, 17), 2), mode="rel", eb=1e-3, block=5)
+    add("g3d_13x9x11_bs4_lo2only", smooth(r, (13, 9, 11), 2), mode="rel", eb=1e-3, block=4, lo=0, lo2=1)
+    add("g4d_9x10x11x12_default", smooth(r, (9, 10, 11, 12), 2), mode="rel", eb=1e-3)
+    add("g4d_8x7x9x10_bs3_lo2only", smooth(r, (8, 7, 9, 10), 2), mode="rel", eb=1e-3, block=3, lo=0, lo2=1)
+    add("g4d_6x7x5x8_bs2", smooth(r, (6, 7, 5, 8), 2), mode="rel", eb=1e-4, block=2)
+    return c
+
+
+def ctest_lorenzo_cases():
+    """CompressTest fixtures of the Lorenzo variant (general frontend,
+    lorenzo + lorenzo2, blockSize 5, radius 65536)."""
+    r = np.random.default_rng(77)
+    c = []
+    f3 = smooth(r, (34, 34, 51), 2)
+    c.append(("k3d_17", f3, [(0, 0, 0), (16, 16, 16), (0, 16, 32), (16, 0, 34)], 16, dict(eb=1e-3)))
+    c.append(("k3d_17_small_eb", f3, [(5, 6, 7), (16, 16, 30)], 16, dict(eb=3e-6)))
+    f33 = smooth(r, (66, 34, 34), 2)
+    c.append(("k3d_33", f33, [(0, 0, 0), (32, 0, 0), (33, 1, 1)], 32, dict(eb=5e-4)))
+    f2 = smooth(r, (130, 131), 2)
+    c.append(("k2d_65", f2, [(0, 0), (64, 64), (65, 0)], 64, dict(eb=1e-3)))
+    c.append(("k2d_65_wide_codes", f2, [(0, 0), (65, 66)], 64, dict(eb=2e-7)))
+    return c
+
+
 def harness_args(inp, out, dims, kw):
     a = [HARNESS, "interp", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, dims)),
          f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"algo={kw.get('algo', 1)}",
@@ -170,7 +219,8 @@ def main():
     if not os.path.exists(HARNESS):
         sys.exit("oracle/_ref/ref_harness missing: build it with `make -C oracle ref`")
     tmp = tempfile.mkdtemp()
-    for name, x, kw in cases():
+    only = sys.argv[1] if len(sys.argv) > 1 else ""  # name prefix: regenerate only those fixtures
+    for name, x, kw in (cases() if not only else []):
         inp = os.path.join(tmp, "in.f32")
         out = os.path.join(tmp, "o")
         x.astype(np.float32).tofile(inp)
@@ -182,14 +232,14 @@ def main():
                             stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
                             params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
         print("wrote", name)
-    for name, codes in huffman_cases():
+    for name, codes in (huffman_cases() if not only else []):
         inp = os.path.join(tmp, "codes.i32")
         out = os.path.join(tmp, "huff.bin")
         codes.tofile(inp)
         subprocess.check_call([HARNESS, "huffman", f"in={inp}", f"out={out}"])
         np.savez_compressed(os.path.join(HERE, name + ".npz"), codes=codes, encoded=np.fromfile(out, np.uint8))
         print("wrote", name)
-    for name, x, kw in lorenzo_cases():
+    for name, x, kw in (lorenzo_cases() if not only else []):
         inp = os.path.join(tmp, "in.f32")
         out = os.path.join(tmp, "l")
         x.astype(np.float32).tofile(inp)
@@ -204,6 +254,44 @@ def main():
                             raw=ld(".raw", np.uint8), stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
                             params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
         print("wrote", name)
+    for name, x, kw in general_cases():
+        if only and not name.startswith(only):
+            continue
+        inp = os.path.join(tmp, "in.f32")
+        out = os.path.join(tmp, "g")
+        x.astype(np.float32).tofile(inp)
+        a = [HARNESS, "general", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, x.shape)),
+             f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"qbins={kw.get('qbins', 65536)}",
+             f"lo={kw.get('lo', 1)}", f"lo2={kw.get('lo2', 1)}"]
+        if "block" in kw:
+            a.append(f"block={kw['block']}")
+        subprocess.check_call(a)
+        ld = lambda ext, dt: np.fromfile(out + ext, dt)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32),
+                            recon=ld(".recon", np.float32), raw=ld(".raw", np.uint8),
+                            stream=ld(".stream", np.uint8), eb=ld(".eb", np.float64),
+                            params=np.array(json.dumps(dict(kw, dims=list(x.shape)))))
+        print("wrote", name)
+    for name, x, starts, sbs, kw in ctest_lorenzo_cases():
+        if only and not name.startswith(only):
+            continue
+        inp = os.path.join(tmp, "field.f32")
+        st = os.path.join(tmp, "starts.i64")
+        raw = os.path.join(tmp, "ct.raw")
+        x.astype(np.float32).tofile(inp)
+        s3 = np.array([list(t) + [0] * (3 - len(t)) for t in starts], np.int64)
+        s3.tofile(st)
+        bd = ",".join([str(sbs + 1)] * x.ndim)
+        a = [HARNESS, "ctest", "ctalgo=lorenzo", f"in={inp}", "fdims=" + ",".join(map(str, x.shape)),
+             f"starts={st}", f"sbs={sbs}", "dims=" + bd, "mode=abs", f"eb={kw['eb']!r}", f"raw={raw}"]
+        res = json.loads(subprocess.check_output(a))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32), starts=s3,
+                            raw=np.fromfile(raw, np.uint8), square_error=np.array([res["square_error"]]),
+                            ref_zstd_bytes=np.array([res["bytes"]]),
+                            params=np.array(json.dumps(dict(kw, sbs=sbs, dims=list(x.shape)))))
+        print("wrote", name)
+    if only:
+        return
     for name, x, starts, sbs, kw in ctest_cases():
         inp = os.path.join(tmp, "field.f32")
         st = os.path.join(tmp, "starts.i64")

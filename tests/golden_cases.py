@@ -17,6 +17,8 @@ HUFFMAN = names("h_")
 TUNING = names("t")
 LORENZO = names("l")
 CTEST = names("q")
+GENERAL = names("g")
+CTEST_LORENZO = names("k")
 
 
 def load(name):
@@ -49,6 +51,22 @@ def lr_kwargs(case):
     return dict(dims=p["dims"], eb=float(np.asarray(case["eb"]).ravel()[0]), block_size=p.get("block", 6),
                 pred_dim=p.get("pd", 3), lorenzo=p.get("lo", 1), lorenzo2=p.get("lo2", 0),
                 regression=p.get("reg", 1), radius=p.get("qbins", 65536) // 2)
+
+
+def gf_kwargs(case):
+    """General-frontend parameters of a ``g*`` fixture, absolute bound; the
+    block size defaults to the Config's for N (Config.hpp:71)."""
+    p = case["params"]
+    n = len(p["dims"])
+    return dict(dims=p["dims"], eb=float(np.asarray(case["eb"]).ravel()[0]),
+                block_size=p.get("block", 128 if n == 1 else (16 if n == 2 else 6)),
+                lorenzo=p.get("lo", 1), lorenzo2=p.get("lo2", 1), radius=p.get("qbins", 65536) // 2)
+
+
+def trailer(stream):
+    """The Config trailer of an SZ_compress stream (sz.hpp:54-91)."""
+    n = int(np.frombuffer(stream[-4:], np.int32)[0])
+    return stream[-4 - n:]
 
 
 # Struct padding the reference writes with whatever its memory held

@@ -10,8 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import (CTEST, HUFFMAN, INTERP, LORENZO, TUNING, ctest_params, load, lr_kwargs, lr_raw_equal,
-                          param_kwargs)
+from golden_cases import (CTEST, CTEST_LORENZO, GENERAL, HUFFMAN, INTERP, LORENZO, TUNING, ctest_params, gf_kwargs,
+                          load, lr_kwargs, lr_raw_equal, param_kwargs, trailer)
 
 
 def _params(case):
@@ -154,3 +154,86 @@ def test_compress_test_buffer(name):
     assert raw == c["raw"].tobytes()
     assert sq == float(c["square_error"][0])
     assert abs(nbytes - int(c["ref_zstd_bytes"][0])) <= 0.005 * nbytes + 8
+
+
+@pytest.mark.parametrize("name", GENERAL)
+def test_general_raw_recon_decode(name):
+    """General frontend with the Lorenzo predictors (SZGeneralFrontend +
+    ComposedPredictor, N = 1..4): the pre-zstd buffer bit for bit, the
+    overwritten array equals the reference's decoded reconstruction, and the
+    oracle decodes the reference's buffer to it."""
+    c = load(name)
+    kw = gf_kwargs(c)
+    prm = oracle.make_gf_params(**kw)
+    assert oracle.gf_raw(c["input"], prm) == c["raw"].tobytes()
+    r = oracle.gf_compress(c["input"], prm)
+    assert np.array_equal(r["recon"].view(np.uint32), c["recon"].view(np.uint32))
+    dec = oracle.gf_decompress_raw(c["raw"].tobytes(), kw["dims"], kw["lorenzo"], kw["lorenzo2"])
+    assert np.array_equal(dec.view(np.uint32), c["recon"].view(np.uint32))
+    x = c["input"].astype(np.float64).ravel()
+    bad = np.abs(r["recon"].astype(np.float64) - x) > kw["eb"]
+    assert not bad.any()
+    assert r["unpred"].size == int((r["codes"] == 0).sum())
+
+
+@pytest.mark.parametrize("name", GENERAL)
+def test_general_stream(name):
+    """SZ_compress stream: the Config trailer bit for bit; for N != 3 (where
+    SZ_decompress takes the general frontend, LR:117-135) the oracle decodes
+    the reference's stream and its own to the reference's reconstruction."""
+    c = load(name)
+    kw = gf_kwargs(c)
+    p = c["params"]
+    s = oracle.gf_compress_stream(c["input"], oracle.make_gf_params(**kw), p["eb"], rel=p.get("mode") == "rel")
+    ref = c["stream"].tobytes()
+    assert trailer(s) == trailer(ref)
+    if len(kw["dims"]) != 3:
+        for st in (ref, s):
+            dec = oracle.decompress_stream(st, c["input"].size)
+            assert np.array_equal(dec.view(np.uint32), c["recon"].view(np.uint32))
+
+
+def test_general_fixtures_cover_every_branch():
+    """Both predictors selected, every N, partial and single-point blocks,
+    outliers, a blockSize of 1 and of 2."""
+    sel = np.zeros(2, int)
+    ns, unpred, thin = set(), 0, 0
+    for name in GENERAL:
+        c = load(name)
+        kw = gf_kwargs(c)
+        r = oracle.gf_compress(c["input"], oracle.make_gf_params(**kw))
+        sel += np.bincount(r["selection"], minlength=2)
+        ns.add(len(kw["dims"]))
+        unpred += r["unpred"].size
+        thin += any(d % kw["block_size"] == 1 for d in kw["dims"])
+    assert (sel > 50).all() and ns == {1, 2, 3, 4} and unpred > 100 and thin >= 3
+    assert {gf_kwargs(load(n))["block_size"] for n in GENERAL} >= {1, 2}
+
+
+@pytest.mark.parametrize("name", CTEST_LORENZO)
+def test_compress_test_lorenzo_buffer(name):
+    """CompressTest's Lorenzo variant (SZI:1136-1170, 1283-1294): the
+    encoding_lossless buffer and the squared error exactly; the zstd frame
+    size within 0.5% of the reference's vendored zstd."""
+    c = load(name)
+    p = c["params"]
+    raw, nbytes, sq = oracle.compress_test_lorenzo(c["input"], c["starts"], p["sbs"], p["eb"])
+    assert raw == c["raw"].tobytes()
+    assert sq == float(c["square_error"][0])
+    assert abs(nbytes - int(c["ref_zstd_bytes"][0])) <= 0.005 * nbytes + 8
+
+
+def test_compress_test_lorenzo_wide_codes():
+    """radius 65536: the wide-codes fixture holds codes above 65535."""
+    import struct
+    c = load("k2d_65_wide_codes")
+    raw = c["raw"].tobytes()
+    n = len(c["params"]["dims"])
+    off = 8 * n + 4 + 2                       # dims, blockSize, two predictor ids
+    nsel = struct.unpack_from("<Q", raw, off)[0]
+    assert nsel == len(c["starts"]) * ((c["params"]["sbs"] + 1 + 4) // 5) ** n
+    x = c["input"]
+    blocks = [x[s[0]:s[0] + 65, s[1]:s[1] + 65] for s in c["starts"]]
+    prm = oracle.make_gf_params((65, 65), c["params"]["eb"], block_size=5, radius=65536)
+    codes = np.concatenate([oracle.gf_compress(b, prm)["codes"] for b in blocks])
+    assert codes.max() > 65535
