@@ -94,6 +94,47 @@ class LorenzoOut(ctypes.Structure):
     ]
 
 
+class GeneralParams(ctypes.Structure):
+    """Mirror of faz_general_params."""
+
+    _fields_ = [
+        ("N", ctypes.c_int),
+        ("dims", ctypes.c_size_t * 4),
+        ("eb", ctypes.c_double),
+        ("block_size", ctypes.c_int),
+        ("lorenzo", ctypes.c_int),
+        ("lorenzo2", ctypes.c_int),
+        ("radius", ctypes.c_int),
+    ]
+
+
+class GeneralOut(ctypes.Structure):
+    _fields_ = [
+        ("d_codes", ctypes.c_void_p),
+        ("n_codes", ctypes.c_size_t),
+        ("d_outval", ctypes.c_void_p),
+        ("d_selection", ctypes.c_void_p),
+        ("n_selection", ctypes.c_size_t),
+    ]
+
+
+def make_general_params(dims, eb, block_size=None, lorenzo=True, lorenzo2=True, radius=32768):
+    """General-frontend parameters; block_size defaults to the Config's for N
+    (128, 16, 6, 6)."""
+    p = GeneralParams()
+    p.N = len(dims)
+    for i, d in enumerate(dims):
+        p.dims[i] = int(d)
+    p.eb = float(eb)
+    if block_size is None:
+        block_size = 128 if p.N == 1 else (16 if p.N == 2 else 6)
+    p.block_size = int(block_size)
+    p.lorenzo = int(bool(lorenzo))
+    p.lorenzo2 = int(bool(lorenzo2))
+    p.radius = int(radius)
+    return p
+
+
 def make_lorenzo_params(dims, eb, block_size=6, pred_dim=3, lorenzo=True, lorenzo2=False, regression=True,
                         radius=32768):
     p = LorenzoParams()
@@ -199,7 +240,10 @@ EXPORTS = [
     "faz_sz_compress_qoz_f32",
     "faz_mlp_create", "faz_mlp_destroy", "faz_mlp_layout", "faz_mlp_forward", "faz_mlp_gradients",
     "faz_mlp_train_step", "faz_mlp_get_params", "faz_mlp_set_params",
-    "faz_gpu_compress_test_batch_f32", "faz_gpu_block_features_f32", "faz_learned_input", "faz_learned_target", "faz_gpu_tune_learned_f32",
+    "faz_gpu_compress_test_batch_f32", "faz_gpu_block_features_f32", "faz_learned_input", "faz_learned_target",
+    "faz_gpu_tune_learned_f32",
+    "faz_gpu_general_compress_f32", "faz_gpu_general_raw_f32", "faz_gpu_general_decompress_raw",
+    "faz_gpu_compress_test_lorenzo_f32", "faz_sz_compress_general_f32",
 ]
 
 LEARNED_FEATURES = 40
@@ -250,6 +294,13 @@ def lib():
         L.faz_sz_compress_lorenzo_f32.argtypes = [vp, P(LorenzoParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
         L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
+        L.faz_gpu_general_compress_f32.argtypes = [vp, vp, vp, P(GeneralParams), P(GeneralOut)]
+        L.faz_gpu_general_raw_f32.argtypes = [vp, vp, vp, P(GeneralParams), vp, sz, P(sz)]
+        L.faz_gpu_general_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, sz]
+        L.faz_gpu_compress_test_lorenzo_f32.argtypes = [vp, vp, ctypes.c_int, vp, vp, sz, sz, ctypes.c_double, vp,
+                                                        sz, P(sz), P(sz), P(ctypes.c_double)]
+        L.faz_sz_compress_general_f32.argtypes = [vp, P(GeneralParams), ctypes.c_int, ctypes.c_double, vp,
+                                                  ctypes.c_int, P(vp), P(sz)]
         L.faz_mlp_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                      P(vp)]
         L.faz_mlp_destroy.argtypes = [vp]
@@ -534,6 +585,79 @@ class Engine:
             return ctypes.string_at(p, n.value)
         finally:
             self._L.faz_free(p)
+
+    # --- general frontend (Lorenzo predictors, any N) ------------------------
+    def general_compress(self, x, params, recon=None):
+        """Returns dict(codes uint32 block-major, outval at zero codes,
+        selection uint8, recon tensor)."""
+        import torch
+        if recon is None:
+            recon = torch.empty_like(x)
+        torch.cuda.synchronize()
+        out = GeneralOut()
+        self._check(self._L.faz_gpu_general_compress_f32(self.h, self._dev(x), self._dev(recon),
+                                                         ctypes.byref(params), ctypes.byref(out)),
+                    "general_compress")
+        codes = d2h(out.d_codes, out.n_codes * 4, np.uint32)
+        outval = d2h(out.d_outval, out.n_codes * 4, np.float32)
+        return dict(codes=codes, unpred=outval[codes == 0].copy(),
+                    selection=d2h(out.d_selection, out.n_selection, np.uint8), recon=recon)
+
+    def general_raw(self, x, params, recon=None):
+        import torch
+        torch.cuda.synchronize()
+        need = ctypes.c_size_t()
+        rp = self._dev(recon) if recon is not None else None
+        rc = self._L.faz_gpu_general_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params), None, 0,
+                                             ctypes.byref(need))
+        if rc != -4:
+            self._check(rc, "general_raw")
+        buf = np.zeros(need.value, np.uint8)
+        self._check(self._L.faz_gpu_general_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params), _np_ptr(buf),
+                                                    buf.size, ctypes.byref(need)), "general_raw")
+        return buf[:need.value].tobytes()
+
+    def general_decompress_raw(self, raw, dims, out, lorenzo=True, lorenzo2=True):
+        import torch
+        b = np.frombuffer(raw, np.uint8).copy()
+        torch.cuda.synchronize()
+        self._check(self._L.faz_gpu_general_decompress_raw(self.h, _np_ptr(b), b.size, len(dims), int(bool(lorenzo)),
+                                                           int(bool(lorenzo2)), self._dev(out), int(np.prod(dims))),
+                    "general_decompress_raw")
+        return out
+
+    def sz_compress_general(self, data, params, eb, rel=True):
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        if isinstance(data, np.ndarray):
+            a = np.ascontiguousarray(data, np.float32)
+            ptr, ondev = _np_ptr(a), 0
+        else:
+            import torch
+            torch.cuda.synchronize()
+            ptr, ondev = self._dev(data), 1
+        self._check(self._L.faz_sz_compress_general_f32(self.h, ctypes.byref(params), 1 if rel else 0, float(eb),
+                                                        ptr, ondev, ctypes.byref(p), ctypes.byref(n)),
+                    "sz_compress_general")
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            self._L.faz_free(p)
+
+    def compress_test_lorenzo(self, x, shape, starts, sbs, eb):
+        """CompressTest's Lorenzo variant: (pre-zstd buffer, compressed size, squared error)."""
+        import torch
+        torch.cuda.synchronize()
+        fd = np.array(list(shape) + [1] * (3 - len(shape)), np.uint64)
+        st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+        cap = 64 * 1024 + st.shape[0] * (sbs + 1) ** len(shape) * 16
+        raw = np.zeros(cap, np.uint8)
+        rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+        self._check(self._L.faz_gpu_compress_test_lorenzo_f32(self.h, self._dev(x), len(shape), _np_ptr(fd),
+                                                              _np_ptr(st), st.shape[0], sbs, float(eb), _np_ptr(raw),
+                                                              cap, ctypes.byref(rs), ctypes.byref(by),
+                                                              ctypes.byref(sq)), "compress_test_lorenzo")
+        return raw[:rs.value].tobytes(), by.value, sq.value
 
     # --- exhaustive QoZ tuner -----------------------------------------------
     def tune_qoz(self, x, shape, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0,

@@ -91,6 +91,12 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint64_t> lr_rank;
     faz::DevBuf<uint16_t> lr_tab;
     uint32_t lr_tab_bs = 0;  // block size the table was built for
+    // general-frontend Lorenzo engine: 32-bit codes, per-block selection,
+    // gathered sample blocks, the in-block hyperplane table for (N, bs)
+    faz::DevBuf<uint32_t> gf_codes, gf_tab, gf_diag;
+    faz::DevBuf<uint8_t> gf_sel;
+    faz::DevBuf<float> gf_fields, gf_recon;
+    uint32_t gf_tab_key = 0;  // N << 16 | bs of the table
 };
 
 #define CTX_FAIL(code, ...)                                          \

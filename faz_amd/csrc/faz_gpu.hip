@@ -352,6 +352,12 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->lr_diag.release();
     ctx->lr_rank.release();
     ctx->lr_tab.release();
+    ctx->gf_codes.release();
+    ctx->gf_tab.release();
+    ctx->gf_diag.release();
+    ctx->gf_sel.release();
+    ctx->gf_fields.release();
+    ctx->gf_recon.release();
     for (auto &e: ctx->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e: ctx->prof_ev) (void) hipEventDestroy(e);

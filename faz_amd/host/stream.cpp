@@ -207,10 +207,26 @@ faz_lorenzo_params lorenzo_params_from_config(const QoZ::Config &c) {
 // SZ_compress for ALGO_LORENZO_REG on a 3-D field: calAbsErrorBound
 // (SZDispatcher.hpp:13-20), SZ_compress_LorenzoReg's fast frontend path
 // (SZLorenzoReg.hpp:95-110), zstd, trailer.
+faz_general_params general_params_from_config(const QoZ::Config &c) {
+    faz_general_params p;
+    memset(&p, 0, sizeof(p));
+    p.N = c.N;
+    for (int i = 0; i < c.N; i++) p.dims[i] = c.dims[i];
+    p.eb = c.absErrorBound;
+    p.block_size = c.blockSize;
+    p.lorenzo = c.lorenzo;
+    p.lorenzo2 = c.lorenzo2;
+    p.radius = c.quantbinCnt / 2;
+    return p;
+}
+
 int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
-    if (conf.N != 3 || conf.regression2) {
-        faz_ctx_set_error(ctx, "the GPU Lorenzo/regression engine serves 3-D fields without regression2 "
-                               "(SZFastFrontend)");
+    // SZ_compress_LorenzoReg (LR:95-114): the fast frontend for 3-D fields
+    // without regression2, else the general frontend
+    const bool fast = conf.N == 3 && !conf.regression2;
+    if (!fast && (conf.regression || conf.regression2)) {
+        faz_ctx_set_error(ctx, "the general frontend runs on the GPU with the Lorenzo predictors only "
+                               "(regression / regression2 off)");
         return FAZ_ERR_UNSUPPORTED;
     }
     if (conf.wavelet) {
@@ -219,9 +235,14 @@ int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_
     }
     int rc = cal_abs_error_bound(ctx, conf, d_data);
     if (rc) return rc;
-    faz_lorenzo_params p = lorenzo_params_from_config(conf);
     std::vector<uint8_t> raw;
-    rc = faz_lorenzo_raw_vec(ctx, d_data, nullptr, &p, raw);
+    if (fast) {
+        faz_lorenzo_params p = lorenzo_params_from_config(conf);
+        rc = faz_lorenzo_raw_vec(ctx, d_data, nullptr, &p, raw);
+    } else {
+        faz_general_params p = general_params_from_config(conf);
+        rc = faz_general_raw_vec(ctx, d_data, nullptr, &p, raw);
+    }
     if (rc) return rc;
     if (!zstd_wrap(raw, out, QoZ::Config::size_est())) {
         faz_ctx_set_error(ctx, "zstd compression failed");
@@ -313,10 +334,13 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
         faz_ctx_set_error(ctx, "stream shape does not match the output buffer");
         return FAZ_ERR_FORMAT;
     }
+    // SZ_decompress_LorenzoReg (LR:117-135): fast frontend for 3-D without
+    // regression2, else the general frontend
     const bool lorenzo = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && conf.N == 3 && !conf.regression2;
-    if (!(conf.cmprAlgo == QoZ::ALGO_INTERP || lorenzo) || conf.wavelet != 0) {
-        faz_ctx_set_error(ctx, "only ALGO_INTERP and 3-D ALGO_LORENZO_REG streams without wavelets decode on the "
-                               "GPU path");
+    const bool general = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && !lorenzo && !conf.regression && !conf.regression2;
+    if (!(conf.cmprAlgo == QoZ::ALGO_INTERP || lorenzo || general) || conf.wavelet != 0) {
+        faz_ctx_set_error(ctx, "only ALGO_INTERP and ALGO_LORENZO_REG streams (3-D, or any N with the Lorenzo "
+                               "predictors only) without wavelets decode on the GPU path");
         return FAZ_ERR_UNSUPPORTED;
     }
     std::vector<uint8_t> raw;
@@ -328,6 +352,9 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
         size_t dims[3] = {conf.dims[0], conf.dims[1], conf.dims[2]};
         return faz_gpu_lorenzo_decompress_raw(ctx, raw.data(), raw.size(), dims, d_out);
     }
+    if (general)
+        return faz_general_decompress_raw_vec(ctx, raw.data(), raw.size(), conf.N, conf.lorenzo, conf.lorenzo2, d_out,
+                                              num);
     return faz_gpu_interp_decompress_raw(ctx, raw.data(), raw.size(), conf.N, d_out, num);
 }
 
@@ -591,6 +618,43 @@ faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_param
     conf.lorenzo = p->lorenzo != 0;
     conf.lorenzo2 = p->lorenzo2 != 0;
     conf.regression = p->regression != 0;
+    conf.quantbinCnt = p->radius * 2;
+    if (rel_mode) {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = eb;
+    } else {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = eb;
+    }
+    int rc;
+    const float *d = to_device(ctx, data, conf.num, data_on_device != 0, rc);
+    if (rc) return rc;
+    std::vector<uint8_t> buf;
+    rc = compress_lorenzo_stream(ctx, conf, d, buf);
+    if (rc) return rc;
+    uint8_t *m = (uint8_t *) malloc(buf.size());
+    if (!m) return FAZ_ERR_INVALID;
+    memcpy(m, buf.data(), buf.size());
+    *out = m;
+    *out_size = buf.size();
+    return FAZ_OK;
+}
+
+faz_status faz_sz_compress_general_f32(faz_gpu_ctx *ctx, const faz_general_params *p, int rel_mode, double eb,
+                                       const float *data, int data_on_device, uint8_t **out, size_t *out_size) {
+    if (!ctx || !p || !data || !out || !out_size || p->N < 1 || p->N > 4) return FAZ_ERR_INVALID;
+    if (p->N == 3) {
+        faz_ctx_set_error(ctx, "SZ_compress takes the fast frontend for 3-D Lorenzo (faz_sz_compress_lorenzo_f32)");
+        return FAZ_ERR_INVALID;
+    }
+    QoZ::Config conf;
+    conf.setDims(p->dims, p->dims + p->N);
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    conf.blockSize = p->block_size;
+    conf.lorenzo = p->lorenzo != 0;
+    conf.lorenzo2 = p->lorenzo2 != 0;
+    conf.regression = false;
+    conf.regression2 = false;
     conf.quantbinCnt = p->radius * 2;
     if (rel_mode) {
         conf.errorBoundMode = QoZ::EB_REL;

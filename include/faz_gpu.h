@@ -199,6 +199,62 @@ faz_status faz_gpu_lorenzo_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw
 faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
                                        const float *data, int data_on_device, uint8_t **out, size_t *out_size);
 
+/* --- A20: the general frontend with the Lorenzo predictors, N = 1..4:
+ * SZGeneralFrontend (include/QoZ/frontend/SZGeneralFrontend.hpp:36-116)
+ * over ComposedPredictor{LorenzoPredictor<1>, LorenzoPredictor<2>}
+ * (ComposedPredictor.hpp:45-93, 203-249; LorenzoPredictor.hpp:25-247) or one
+ * of them, as make_lorenzo_regression_compressor composes them with
+ * regression and regression2 off (SZLorenzoReg.hpp:27-91).  This is what
+ * SZ_compress_LorenzoReg runs for N != 3 (LR:103-110) and what the tuner's
+ * Lorenzo CompressTest runs for every N (SZI:1136-1170). */
+typedef struct {
+    int N;                  /* 1..4 */
+    size_t dims[4];
+    double eb;              /* absolute error bound */
+    int block_size;         /* Config::blockSize (N=1: <= 2^20, 2: <= 64, 3: <= 16, 4: <= 8) */
+    int lorenzo;            /* Config::lorenzo */
+    int lorenzo2;           /* Config::lorenzo2 (needs block_size >= 2) */
+    int radius;             /* Config::quantbinCnt / 2, <= 65536 */
+} faz_general_params;
+
+/* Results of faz_gpu_general_compress_f32 (device pointers owned by ctx). */
+typedef struct {
+    uint32_t *d_codes;          /* quantization codes, block-major (GF:47-55) */
+    size_t n_codes;
+    float *d_outval;            /* at every zero code, the unpredictable value */
+    const uint8_t *d_selection; /* per block, index into the predictor list */
+    size_t n_selection;         /* 0 when a single predictor is on */
+} faz_general_out;
+
+/* SZGeneralFrontend::compress (GF:36-71); d_recon (n floats) receives the
+ * array the reference leaves overwritten. */
+faz_status faz_gpu_general_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon,
+                                        const faz_general_params *p, faz_general_out *out);
+/* The buffer SZGeneralCompressor::compress hands to zstd (SGC:30-71): the
+ * frontend's save (GF:98-103, CP:70-81), LinearQuantizer::save, the codes'
+ * tree and bits.  Needs radius <= 32768.  d_recon may be NULL. */
+faz_status faz_gpu_general_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_general_params *p,
+                                   uint8_t *h_raw, size_t cap, size_t *raw_size);
+/* SZGeneralCompressor::decompress after zstd (SGC:115-140, GF:72-96, 105-116)
+ * into d_out (num floats); lorenzo / lorenzo2 come from the stream's Config. */
+faz_status faz_gpu_general_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, int lorenzo,
+                                          int lorenzo2, float *d_out, size_t num);
+/* CompressTest with algo = ALGO_LORENZO_REG, useFast = false (SZI:1136-1170,
+ * 1254-1256, 1283-1294, 1353-1392): one general compressor (lorenzo +
+ * lorenzo2, blockSize 5, radius 65536) over the (sbs+1)^N sample blocks at
+ * `starts` (size_t triples) of the N = 2 or 3 field d_in (dims fdims);
+ * *bytes = the zstd-framed encoding_lossless size, *square_error = the summed
+ * float squared errors.  h_raw (may be NULL) receives the pre-zstd buffer. */
+faz_status faz_gpu_compress_test_lorenzo_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *fdims,
+                                             const size_t *starts, size_t n_blocks, size_t sbs, double eb,
+                                             uint8_t *h_raw, size_t cap, size_t *raw_size, size_t *bytes,
+                                             double *square_error);
+/* SZ_compress for cmprAlgo = ALGO_LORENZO_REG through the general frontend
+ * (N != 3: LR:95-114; sz.hpp:54-91): bound conversion, zstd, Config trailer.
+ * faz_sz_decompress_f32 decodes the result. */
+faz_status faz_sz_compress_general_f32(faz_gpu_ctx *ctx, const faz_general_params *p, int rel_mode, double eb,
+                                       const float *data, int data_on_device, uint8_t **out, size_t *out_size);
+
 /* --- A18: the exhaustive QoZ sampling tuner (`faz -q`): Tuning() with
  * conf.QoZ set, RD target, no wavelets (include/QoZ/api/impl/SZInterp.hpp:
  * 1589-2557): sampleBlocks (SZI:993-1127), per-level interpolator and
