@@ -244,7 +244,26 @@ EXPORTS = [
     "faz_gpu_tune_learned_f32",
     "faz_gpu_general_compress_f32", "faz_gpu_general_raw_f32", "faz_gpu_general_decompress_raw",
     "faz_gpu_compress_test_lorenzo_f32", "faz_sz_compress_general_f32",
+    "faz_gpu_batch_create", "faz_gpu_batch_destroy", "faz_gpu_batch_compress", "faz_gpu_batch_fetch_raw",
+    "faz_gpu_batch_stats", "faz_sz_compress_batch_f32", "faz_gpu_batch_field",
 ]
+
+
+class BatchStats(ctypes.Structure):
+    """Mirror of faz_batch_stats."""
+
+    _fields_ = [
+        ("calls", ctypes.c_uint64),
+        ("captures", ctypes.c_uint64),
+        ("updates", ctypes.c_uint64),
+        ("instantiations", ctypes.c_uint64),
+        ("replays", ctypes.c_uint64),
+        ("tuned_fields", ctypes.c_uint64),
+        ("range_ms", ctypes.c_double),
+        ("pred_ms", ctypes.c_double),
+        ("encode_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+    ]
 
 LEARNED_FEATURES = 40
 LEARNED_INPUTS = 49
@@ -301,6 +320,14 @@ def lib():
                                                         sz, P(sz), P(sz), P(ctypes.c_double)]
         L.faz_sz_compress_general_f32.argtypes = [vp, P(GeneralParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
+        L.faz_gpu_batch_create.argtypes = [vp, P(InterpParams), P(vp), ctypes.c_int, ctypes.c_int, P(vp)]
+        L.faz_gpu_batch_destroy.argtypes = [vp]
+        L.faz_gpu_batch_destroy.restype = None
+        L.faz_gpu_batch_compress.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.faz_gpu_batch_fetch_raw.argtypes = [vp, ctypes.c_int, vp, sz]
+        L.faz_gpu_batch_stats.argtypes = [vp, P(BatchStats), vp]
+        L.faz_sz_compress_batch_f32.argtypes = [vp, vp, vp, vp, vp]
+        L.faz_gpu_batch_field.argtypes = [vp, ctypes.c_int, P(InterpParams)]
         L.faz_mlp_create.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                      P(vp)]
         L.faz_mlp_destroy.argtypes = [vp]
@@ -841,3 +868,83 @@ def learned_target(bitrate, psnr, eb_norm, eb_scale):
     if rc:
         raise FazError(rc, "learned_target")
     return t
+
+
+class Batch:
+    """Many fields of uneven shapes compressed together (faz_gpu_batch_*):
+    inputs bound at creation (device tensors the caller refills in place),
+    one graph of range launches, one graph of per-field prediction branches
+    replayed while the bounds stay the same, per-field launch schedules picked
+    by the autotuner."""
+
+    def __init__(self, engine, params, tensors, autotune=True):
+        import torch
+        self.engine = engine
+        self.n = len(params)
+        self._params = (InterpParams * self.n)(*params)
+        self._tensors = list(tensors)  # keeps the bound inputs alive
+        ptrs = (ctypes.c_void_p * self.n)(*[t.data_ptr() for t in self._tensors])
+        torch.cuda.synchronize()
+        h = ctypes.c_void_p()
+        engine._check(engine._L.faz_gpu_batch_create(engine.h, self._params, ptrs, self.n, 1 if autotune else 0,
+                                                     ctypes.byref(h)), "batch_create")
+        self.h = h
+
+    def _modes(self, rel, eb):
+        rel = [rel] * self.n if isinstance(rel, (bool, int)) else list(rel)
+        eb = [eb] * self.n if isinstance(eb, (float, int)) else list(eb)
+        return np.array([1 if r else 0 for r in rel], np.int32), np.array(eb, np.float64)
+
+    def compress_raw(self, rel=True, eb=1e-3):
+        """Engine buffers of every field: (list of bytes, absolute bounds, ranges)."""
+        import torch
+        m, e = self._modes(rel, eb)
+        sizes = np.zeros(self.n, np.uint64)
+        aeb = np.zeros(self.n, np.float64)
+        rng = np.zeros(self.n, np.float64)
+        torch.cuda.synchronize()
+        L = self.engine._L
+        self.engine._check(L.faz_gpu_batch_compress(self.h, _np_ptr(m), _np_ptr(e), _np_ptr(sizes), _np_ptr(aeb),
+                                                    _np_ptr(rng)), "batch_compress")
+        out = []
+        for i in range(self.n):
+            buf = np.zeros(int(sizes[i]), np.uint8)
+            self.engine._check(L.faz_gpu_batch_fetch_raw(self.h, i, _np_ptr(buf), buf.size), "batch_fetch_raw")
+            out.append(buf.tobytes())
+        return out, aeb, rng
+
+    def sz_compress(self, rel=True, eb=1e-3):
+        """SZ_compress stream of every field."""
+        import torch
+        m, e = self._modes(rel, eb)
+        ptrs = (ctypes.c_void_p * self.n)()
+        sizes = (ctypes.c_size_t * self.n)()
+        torch.cuda.synchronize()
+        self.engine._check(self.engine._L.faz_sz_compress_batch_f32(self.h, _np_ptr(m), _np_ptr(e), ptrs, sizes),
+                           "sz_compress_batch")
+        out = []
+        for i in range(self.n):
+            try:
+                out.append(ctypes.string_at(ptrs[i], sizes[i]))
+            finally:
+                self.engine._L.faz_free(ptrs[i])
+        return out
+
+    def stats(self):
+        s = BatchStats()
+        sched = np.zeros(self.n, np.int32)
+        self.engine._check(self.engine._L.faz_gpu_batch_stats(self.h, ctypes.byref(s), _np_ptr(sched)), "batch_stats")
+        d = {k: getattr(s, k) for k, _ in BatchStats._fields_}
+        d["schedules"] = sched.tolist()
+        return d
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.engine._L.faz_gpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

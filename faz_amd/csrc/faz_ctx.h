@@ -60,6 +60,7 @@ struct faz_gpu_ctx {
     faz::DevBuf<float> lat[2];
     faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
+    uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     hipEvent_t ev[8] = {};
     double timings[8] = {};
     // histogram of the codes last produced by interp_compress

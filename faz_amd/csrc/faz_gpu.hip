@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <numeric>
 #include <vector>
 
 using namespace faz;
@@ -402,13 +403,25 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
 // otherwise it only carries the coarse levels (the compression itself needs
 // no reconstruction of the finest level).
 // absmax: max |x| of the input when the caller knows it, else negative.
+// enqueue: device work only, for stream capture: no host synchronisation or
+// transfer and no allocation (a run with the same shape before has sized every
+// buffer and uploaded the block prefix table), outliers gathered into a
+// buffer of one slot per code, results read by the caller after the stream
+// drains (ctx->scalars[0] = outlier count, ctx->err_flag).
 static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
-                           faz_interp_out *out, bool final_recon, double absmax) {
+                           faz_interp_out *out, bool final_recon, double absmax, bool enqueue = false) {
     Plan pl;
     int rc = make_plan(p, pl, ctx->err);
     if (rc) return rc;
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
+    const uint64_t prefix_key = pl.prefix.size() * 0x9E3779B97F4A7C15ull ^
+                                std::accumulate(pl.prefix.begin(), pl.prefix.end(), (uint64_t) pl.total_codes);
+    if (enqueue) {
+        if (ctx->prefix_key != prefix_key || ctx->prefix.cap < pl.prefix.size() || ctx->codes.cap < pl.total_codes ||
+            ctx->unpred.cap < pl.total_codes || !(absmax >= 0))
+            CTX_FAIL(FAZ_ERR_INVALID, "internal: enqueue without a prepared run of this shape");
+    }
     HIPCHECK(ctx->codes.ensure(pl.total_codes));
     HIPCHECK(ctx->outval.ensure(pl.total_codes));
     HIPCHECK(ctx->err_flag.ensure(1));
@@ -419,10 +432,14 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
     HIPCHECK(ctx->freq.ensure(65536));
     HIPCHECK(ctx->first.ensure(65536));
-    HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    if (!enqueue) {
+        HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
+        ctx->prefix_key = prefix_key;
+    }
     launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
 
-    HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[0], st));
     if (pl.anchor) {
         launch_anchor_grid(d_in, d_recon, ctx->codes.p, ctx->outval.p, pl.N, pl.dims, pl.offs,
                            (uint32_t) p->max_step, pl.anchor_count, st);
@@ -432,7 +449,8 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     }
     size_t npass = 0;
     for (auto &L: pl.lv) npass += L.passes.size();
-    if (ctx->profile && ctx->prof_ev.size() < 2 * npass) {
+    const bool profile = ctx->profile && !enqueue;
+    if (profile && ctx->prof_ev.size() < 2 * npass) {
         size_t have = ctx->prof_ev.size();
         ctx->prof_ev.resize(2 * npass);
         for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
@@ -481,7 +499,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     // |x - pred| <= 4.5 max|x| + 3.5 eb; half of the range is kept as margin
     // for rounding.  A non-finite or huge max |x| keeps the exact quantizer.
     if (std::find(kind.begin(), kind.end(), 2) != kind.end()) {
-        if (!(absmax >= 0)) {
+        if (!(absmax >= 0)) {  // never in enqueue mode (checked above)
             float mn = 0, mx = 0;
             int rr = faz_gpu_range_f32(ctx, d_in, pl.num, &mn, &mx);
             if (rr) return rr;
@@ -540,30 +558,42 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     for (size_t i = 0; i < nlv; i++) {
         LevelPlan &L = pl.lv[i];
         if (kind[i]) {
-            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
             if (kind[i] == 2)
                 launch_interp_tile3d(g3[i], d_in, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid, st);
             else
                 launch_interp_level3d(g3[i], d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
-            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
             k++;
             continue;
         }
         for (auto &d: L.passes) {
             d.block_prefix = ctx->prefix.p + L.prefix_at;
-            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
             launch_interp_pass(d, d_in, d_recon, ctx->codes.p, ctx->outval.p, nullptr, ctx->err_flag.p, st);
-            if (ctx->profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
             k++;
         }
     }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[1], st));
     // histogram, first positions and outlier compaction
     launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st);
-    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     HIPCHECK(hipGetLastError());
+    if (enqueue) {
+        launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
+                             ctx->unpred.p, st);
+        HIPCHECK(hipGetLastError());
+        out->d_codes = ctx->codes.p;
+        out->n_codes = pl.total_codes;
+        out->d_unpred = ctx->unpred.p;
+        out->n_unpred = 0;
+        out->loss = 0;
+        memset(out->level_counts, 0, sizeof(out->level_counts));
+        out->levels = pl.levels;
+        return FAZ_OK;
+    }
     uint64_t n_unpred = 0;
     uint32_t errflag = 0;
     HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
@@ -671,6 +701,68 @@ faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, in
     memcpy(tree, t.tree.data(), t.tree.size());
     return FAZ_OK;
 }
+
+}  // extern "C"
+
+// Packed code table of t in ctx->h_tab (one entry per symbol: codeword
+// right-aligned above an 8-bit length) and the LDS window of the encoder.
+static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow &ew) {
+    ctx->h_tab.resize((size_t) t->n_states);
+    int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
+    for (int32_t i = 0; i < t->n_states; i++) {
+        const uint32_t L = t->len[i];
+        if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
+        ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
+        if (L && i + t->offset != 0) {
+            if (lo < 0) lo = i;
+            hi = i;
+        }
+    }
+    ew = EncodeWindow{0, 0, 0};
+    if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
+        ew.win = (uint32_t) lo;
+        ew.nwin = (uint32_t) (hi - lo + 1);
+        ew.e0 = (t->offset <= 0 && -t->offset < t->n_states) ? ctx->h_tab[-t->offset] : 0;
+    }
+    return FAZ_OK;
+}
+
+__global__ void zero_words_kernel(uint32_t *__restrict__ w, const uint64_t *__restrict__ bits, size_t cap) {
+    const size_t words = std::min<size_t>((size_t) ((bits[0] + 31) / 32), cap);
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t) gridDim.x * blockDim.x)
+        w[i] = 0u;
+}
+
+// Huffman encode of n codes into ctx->words (sized for the longest codeword
+// on every code) without a host synchronisation: the bit total stays in
+// ctx->scalars[1].  ctx->h_tab must stay untouched until the stream drains.
+static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, const faz_huff_table *t) {
+    if (n >= (1ull << 32)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "more than 2^32-1 codes");
+    hipStream_t st = ctx->stream;
+    const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    EncodeWindow ew;
+    int rc = encode_table(ctx, t, ew);
+    if (rc) return rc;
+    uint64_t maxbits = 0;
+    for (int32_t i = 0; i < t->n_states; i++) maxbits = std::max<uint64_t>(maxbits, t->len[i]);
+    const size_t cap = (size_t) ((n * maxbits + 31) / 32 + 2);
+    HIPCHECK(ctx->words.ensure(cap));
+    HIPCHECK(ctx->enc_code.ensure((size_t) t->n_states));
+    HIPCHECK(ctx->chunk_bits.ensure(nchunks));
+    HIPCHECK(ctx->chunk_off.ensure(nchunks));
+    HIPCHECK(ctx->scalars.ensure(4));
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, ctx->h_tab.data(), (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
+    launch_encode_sizes(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_bits.p, st);
+    launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
+    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned) std::min<size_t>((cap + 255) / 256, 8192)), dim3(256), 0, st,
+                       ctx->words.p, ctx->scalars.p + 1, ctx->words.cap);
+    launch_encode_pack(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_off.p, ctx->words.p, st);
+    HIPCHECK(hipGetLastError());
+    return FAZ_OK;
+}
+
+extern "C" {
 
 faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, const faz_huff_table *t,
                                   uint8_t *d_bits, size_t cap_bytes, size_t *nbytes) {
@@ -989,6 +1081,395 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
         }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
+    return FAZ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many fields of uneven shapes compressed together.  Every field has
+// a child context (its own buffers and stream).  One hipGraph holds the range
+// launches of every field, a second one every field's interpolation,
+// quantization, histogram and outlier compaction, one graph branch per field
+// (fork/join on the child streams during capture, so small fields run beside
+// big ones).  The host then builds each field's Huffman tree and enqueues
+// every field's encode before it waits once.  The prediction graph carries the
+// fields' bounds as kernel arguments: it is replayed as is while they stay the
+// same (absolute bounds, or unchanged ranges) and re-captured and updated in
+// place (hipGraphExecUpdate) when they change.  At creation an autotuner picks
+// each 3-D field's launch schedule (whole-tile kernel grid, per-block level
+// kernel or per-pass launches for the coarse levels) by timing the candidates
+// on the bound input; every candidate writes the same codes.
+// ---------------------------------------------------------------------------
+struct faz_batch {
+    faz_gpu_ctx *ctx = nullptr;  // parent: device, capture stream
+    int n = 0;
+    std::vector<faz_interp_params> prm;
+    std::vector<const float *> d_in;
+    std::vector<uint64_t> num, ncodes;
+    std::vector<faz_gpu_ctx *> kid;
+    std::vector<int> schedule;   // index into kSchedules per field
+    hipGraph_t g_range = nullptr, g_pred = nullptr;
+    hipGraphExec_t x_range = nullptr, x_pred = nullptr;
+    std::vector<double> key;     // bound and max |x| of each field inside x_pred
+    std::vector<double> eb, rng, absmax;
+    hipEvent_t ev[6] = {};
+    faz_batch_stats st;
+    uint32_t *h_minmax = nullptr;  // pinned, 2 per field
+};
+
+namespace {
+
+// Launch schedules the autotuner tries on 3-D fields: {tile3d, tile_grid
+// multiple of the CU count / 256, fused3d, fused_min_blocks}.
+struct Schedule {
+    bool tile3d;
+    uint32_t grid_mul;  // whole-tile grid = CUs * grid_mul
+    bool fused3d;
+    uint64_t fused_min_blocks;
+};
+constexpr Schedule kSchedules[] = {
+    {true, 1, true, 256},  {true, 2, true, 256}, {true, 1, true, 64},
+    {true, 2, true, 1024}, {false, 1, true, 256}, {false, 1, false, 256},
+};
+constexpr int kNumSchedules = sizeof(kSchedules) / sizeof(kSchedules[0]);
+
+void apply_schedule(faz_gpu_ctx *kid, const faz_gpu_ctx *parent, int s) {
+    const Schedule &c = kSchedules[s];
+    kid->tile3d = c.tile3d && parent->tile3d;
+    kid->fused3d = c.fused3d && parent->fused3d;
+    kid->tile_grid = parent->tile_grid * c.grid_mul;
+    kid->fused_min_blocks = c.fused_min_blocks;
+}
+
+double ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+float key2f(uint32_t key) {
+    uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+#define BFAIL(code, ...) do { char _m[512]; snprintf(_m, sizeof(_m), __VA_ARGS__); b->ctx->err = _m; return code; } while (0)
+#define BHIP(x)                                                                                    \
+    do {                                                                                           \
+        hipError_t _e = (x);                                                                       \
+        if (_e != hipSuccess) BFAIL(FAZ_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(_e));      \
+    } while (0)
+
+// Every field's range kernel, captured once.
+int batch_capture_range(faz_batch *b) {
+    hipStream_t cs = b->ctx->stream;
+    BHIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < b->n; i++) launch_range(b->d_in[i], b->num[i], b->kid[i]->minmax.p, cs);
+    BHIP(hipStreamEndCapture(cs, &b->g_range));
+    BHIP(hipGraphInstantiate(&b->x_range, b->g_range, nullptr, nullptr, 0));
+    return FAZ_OK;
+}
+
+// Every field's device pipeline on its own branch, bounds as in b->eb.
+int batch_capture_pred(faz_batch *b, hipGraph_t &g) {
+    hipStream_t cs = b->ctx->stream;
+    BHIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = FAZ_OK;
+    BHIP(hipEventRecord(b->ev[0], cs));
+    for (int i = 0; i < b->n && !rc; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        if (hipStreamWaitEvent(k->stream, b->ev[0], 0) != hipSuccess) {
+            rc = FAZ_ERR_HIP;
+            break;
+        }
+        faz_interp_params q = b->prm[i];
+        q.eb = b->eb[i];
+        faz_interp_out io;
+        rc = interp_compress(k, b->d_in[i], k->work_out.p, &q, &io, false, b->absmax[i], true);
+        if (rc) {
+            b->ctx->err = k->err;
+            break;
+        }
+        if (hipEventRecord(k->ev[7], k->stream) != hipSuccess || hipStreamWaitEvent(cs, k->ev[7], 0) != hipSuccess)
+            rc = FAZ_ERR_HIP;
+    }
+    hipGraph_t tmp = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &tmp);
+    if (rc) {
+        if (tmp) (void) hipGraphDestroy(tmp);
+        if (rc == FAZ_ERR_HIP) BFAIL(FAZ_ERR_HIP, "capturing the batch failed");
+        return rc;
+    }
+    BHIP(e);
+    g = tmp;
+    return FAZ_OK;
+}
+
+// One eager compression of field i with its current schedule: sizes every
+// buffer and uploads the block prefix table the captured pipeline relies on;
+// returns the interpolation time.
+int batch_prepare_field(faz_batch *b, int i, double &ms) {
+    faz_gpu_ctx *k = b->kid[i];
+    faz_interp_params q = b->prm[i];
+    q.eb = b->eb[i];
+    int rc = FAZ_OK;
+    if (hipSetDevice(k->device) != hipSuccess) BFAIL(FAZ_ERR_HIP, "hipSetDevice failed");
+    if (k->work_out.ensure(b->num[i]) != hipSuccess || k->unpred.ensure(b->num[i]) != hipSuccess)
+        BFAIL(FAZ_ERR_HIP, "batch: device allocation failed");
+    faz_interp_out io;
+    rc = interp_compress(k, b->d_in[i], k->work_out.p, &q, &io, false, b->absmax[i]);
+    if (rc) {
+        b->ctx->err = k->err;
+        return rc;
+    }
+    ms = k->timings[1];
+    b->ncodes[i] = io.n_codes;
+    return FAZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+faz_status faz_gpu_batch_create(faz_gpu_ctx *ctx, const faz_interp_params *fields, const float *const *d_in, int n,
+                                int autotune, faz_batch **out) {
+    if (!ctx || !fields || !d_in || n <= 0 || !out) return FAZ_ERR_INVALID;
+    *out = nullptr;
+    HIPCHECK(hipSetDevice(ctx->device));
+    faz_batch *b = new faz_batch();
+    b->ctx = ctx;
+    b->n = n;
+    b->prm.assign(fields, fields + n);
+    b->d_in.assign(d_in, d_in + n);
+    b->eb.assign(n, 0.0), b->rng.assign(n, 0.0), b->absmax.assign(n, 0.0), b->schedule.assign(n, 0);
+    b->ncodes.assign(n, 0);
+    memset(&b->st, 0, sizeof(b->st));
+    auto fail = [&](int rc) {
+        faz_gpu_batch_destroy(b);
+        return (faz_status) rc;
+    };
+    for (int i = 0; i < n; i++) {
+        const faz_interp_params &p = fields[i];
+        if (!d_in[i] || p.N < 1 || p.N > 4) {
+            ctx->err = "batch: field without input or with N outside 1..4";
+            return fail(FAZ_ERR_INVALID);
+        }
+        uint64_t m = 1;
+        for (int a = 0; a < p.N; a++) m *= p.dims[a];
+        b->num.push_back(m);
+        faz_gpu_ctx *k = nullptr;
+        int rc = faz_gpu_ctx_create(ctx->device, &k);
+        if (rc) return fail(rc);
+        k->tile3d = ctx->tile3d, k->fused3d = ctx->fused3d, k->tile_grid = ctx->tile_grid;
+        k->fused_min_blocks = ctx->fused_min_blocks;
+        b->kid.push_back(k);
+        if (k->minmax.ensure(2) != hipSuccess) return fail(FAZ_ERR_HIP);
+    }
+    for (auto &e: b->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(FAZ_ERR_HIP);
+    if (hipHostMalloc((void **) &b->h_minmax, 8 * (size_t) n, 0) != hipSuccess) return fail(FAZ_ERR_HIP);
+    int rc = batch_capture_range(b);
+    if (rc) return fail(rc);
+    // ranges of the bound inputs: the eager runs below need a bound and max |x|
+    if (hipGraphLaunch(b->x_range, ctx->stream) != hipSuccess) return fail(FAZ_ERR_HIP);
+    for (int i = 0; i < n; i++)
+        if (hipMemcpyAsync(b->h_minmax + 2 * i, b->kid[i]->minmax.p, 8, hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess)
+            return fail(FAZ_ERR_HIP);
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(FAZ_ERR_HIP);
+    for (int i = 0; i < n; i++) {
+        const float mn = key2f(b->h_minmax[2 * i]), mx = key2f(b->h_minmax[2 * i + 1]);
+        b->absmax[i] = std::max(std::fabs((double) mn), std::fabs((double) mx));
+        const double r = (double) (float) (mx - mn);
+        b->eb[i] = r > 0 ? 1e-3 * r : 1e-3;  // a representative bound for preparation and tuning
+    }
+    // autotune (3-D fields) and prepare every field
+    for (int i = 0; i < n; i++) {
+        double ms = 0;
+        if (autotune && fields[i].N == 3) {
+            double best = 1e30;
+            int pick = 0;
+            for (int s = 0; s < kNumSchedules; s++) {
+                apply_schedule(b->kid[i], ctx, s);
+                double t = 1e30;
+                for (int rep = 0; rep < 2; rep++) {  // first run allocates
+                    if ((rc = batch_prepare_field(b, i, ms))) return fail(rc);
+                    t = std::min(t, ms);
+                }
+                if (t < best) best = t, pick = s;
+            }
+            b->schedule[i] = pick;
+            b->st.tuned_fields++;
+        }
+        apply_schedule(b->kid[i], ctx, b->schedule[i]);
+        if ((rc = batch_prepare_field(b, i, ms))) return fail(rc);
+    }
+    *out = b;
+    return FAZ_OK;
+}
+
+void faz_gpu_batch_destroy(faz_batch *b) {
+    if (!b) return;
+    (void) hipSetDevice(b->ctx->device);
+    (void) hipStreamSynchronize(b->ctx->stream);
+    if (b->x_pred) (void) hipGraphExecDestroy(b->x_pred);
+    if (b->g_pred) (void) hipGraphDestroy(b->g_pred);
+    if (b->x_range) (void) hipGraphExecDestroy(b->x_range);
+    if (b->g_range) (void) hipGraphDestroy(b->g_range);
+    for (auto k: b->kid) faz_gpu_ctx_destroy(k);
+    for (auto &e: b->ev)
+        if (e) (void) hipEventDestroy(e);
+    if (b->h_minmax) (void) hipHostFree(b->h_minmax);
+    delete b;
+}
+
+faz_status faz_gpu_batch_compress(faz_batch *b, const int *rel_mode, const double *eb, size_t *raw_sizes,
+                                  double *abs_eb, double *rng) {
+    if (!b || !rel_mode || !eb || !raw_sizes) return FAZ_ERR_INVALID;
+    faz_gpu_ctx *ctx = b->ctx;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t cs = ctx->stream;
+    const int n = b->n;
+    // 1. ranges of every field (one graph)
+    HIPCHECK(hipEventRecord(b->ev[1], cs));
+    HIPCHECK(hipGraphLaunch(b->x_range, cs));
+    for (int i = 0; i < n; i++)
+        HIPCHECK(hipMemcpyAsync(b->h_minmax + 2 * i, b->kid[i]->minmax.p, 8, hipMemcpyDeviceToHost, cs));
+    HIPCHECK(hipStreamSynchronize(cs));
+    std::vector<double> key(2 * (size_t) n);
+    for (int i = 0; i < n; i++) {
+        const float mn = key2f(b->h_minmax[2 * i]), mx = key2f(b->h_minmax[2 * i + 1]);
+        b->absmax[i] = std::max(std::fabs((double) mn), std::fabs((double) mx));
+        b->rng[i] = (double) (float) (mx - mn);
+        // calAbsErrorBound (Config.hpp / SZ_compress): REL bound times the float range
+        b->eb[i] = rel_mode[i] ? eb[i] * b->rng[i] : eb[i];
+        if (!(b->eb[i] > 0)) CTX_FAIL(FAZ_ERR_INVALID, "field %d: error bound not positive", i);
+        key[2 * i] = b->eb[i];
+        key[2 * i + 1] = b->absmax[i];
+    }
+    // 2. prediction graph: replay, or re-capture and update in place
+    if (!b->x_pred || key != b->key) {
+        hipGraph_t g = nullptr;
+        int rc = batch_capture_pred(b, g);
+        if (rc) return (faz_status) rc;
+        bool updated = false;
+        if (b->x_pred) {
+            hipGraphNode_t err_node = nullptr;
+            hipGraphExecUpdateResult res;
+            if (hipGraphExecUpdate(b->x_pred, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+                updated = true;
+                b->st.updates++;
+            } else {
+                (void) hipGetLastError();
+                (void) hipGraphExecDestroy(b->x_pred);
+                b->x_pred = nullptr;
+            }
+        }
+        if (!updated) {
+            HIPCHECK(hipGraphInstantiate(&b->x_pred, g, nullptr, nullptr, 0));
+            b->st.instantiations++;
+        }
+        if (b->g_pred) (void) hipGraphDestroy(b->g_pred);
+        b->g_pred = g;
+        b->key = key;
+        b->st.captures++;
+    } else {
+        b->st.replays++;
+    }
+    HIPCHECK(hipEventRecord(b->ev[2], cs));
+    HIPCHECK(hipGraphLaunch(b->x_pred, cs));
+    HIPCHECK(hipEventRecord(b->ev[3], cs));
+    // 3. histograms to the host, trees, every encode enqueued, one wait
+    for (int i = 0; i < n; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        HIPCHECK(hipMemcpyAsync(k->h_freq.data(), k->freq.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
+        HIPCHECK(hipMemcpyAsync(k->h_first.data(), k->first.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
+    }
+    HIPCHECK(hipStreamSynchronize(cs));
+    for (int i = 0; i < n; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        PendingRaw &pr = k->pending;
+        pr.valid = false;
+        if (build_huffman(k->h_freq.data(), k->h_first.data(), 65536, pr.tab))
+            CTX_FAIL(FAZ_ERR_UNSUPPORTED, "field %d: Huffman tree could not be built", i);
+        faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
+        HIPCHECK(hipEventRecord(b->ev[4], cs));
+        HIPCHECK(hipStreamWaitEvent(k->stream, b->ev[4], 0));
+        int rc = encode_enqueue(k, k->codes.p, b->ncodes[i], &ht);
+        if (rc) {
+            ctx->err = k->err;
+            return (faz_status) rc;
+        }
+    }
+    for (int i = 0; i < n; i++) HIPCHECK(hipStreamSynchronize(b->kid[i]->stream));
+    HIPCHECK(hipEventRecord(b->ev[5], cs));
+    HIPCHECK(hipStreamSynchronize(cs));
+    // 4. sizes of every field's engine buffer
+    for (int i = 0; i < n; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        uint64_t sc[2];
+        uint32_t errflag = 0;
+        HIPCHECK(hipMemcpy(sc, k->scalars.p, 16, hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(&errflag, k->err_flag.p, 4, hipMemcpyDeviceToHost));
+        if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: field %d: a block segment fell outside the stream", i);
+        if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "field %d: a quantization code exceeded 16 bits", i);
+        const faz_interp_params &p = b->prm[i];
+        PendingRaw &pr = k->pending;
+        EngineHeader &h = pr.h;
+        h.N = p.N;
+        for (int a = 0; a < p.N; a++) h.dims[a] = p.dims[a];
+        h.block_size = (unsigned) p.block_size;
+        h.interp_algo = p.interp_algo;
+        h.direction = p.direction;
+        h.alpha = p.alpha;
+        h.beta = p.beta;
+        h.max_step = p.max_step;
+        h.levelwise = p.levelwise;
+        h.ops.assign(p.ops, p.ops + p.levelwise);
+        h.dirs.assign(p.dirs, p.dirs + p.levelwise);
+        pr.eb = b->eb[i];
+        pr.radius = p.radius;
+        pr.n_unpred = sc[0];
+        pr.nbytes = (size_t) ((sc[1] + 7) / 8);
+        pr.valid = true;
+        raw_sizes[i] = engine_raw_size(pr.h, pr.n_unpred, pr.tab.tree.size(), pr.nbytes);
+        if (abs_eb) abs_eb[i] = b->eb[i];
+        if (rng) rng[i] = b->rng[i];
+    }
+    b->st.range_ms = ev_ms(b->ev[1], b->ev[2]);
+    b->st.pred_ms = ev_ms(b->ev[2], b->ev[3]);
+    b->st.encode_ms = ev_ms(b->ev[3], b->ev[5]);
+    b->st.total_ms = ev_ms(b->ev[1], b->ev[5]);
+    b->st.calls++;
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_batch_fetch_raw(faz_batch *b, int i, uint8_t *h_raw, size_t cap) {
+    if (!b || i < 0 || i >= b->n) return FAZ_ERR_INVALID;
+    faz_gpu_ctx *k = b->kid[i];
+    if (!k->pending.valid) return FAZ_ERR_INVALID;
+    const size_t need = pending_size(k);
+    faz_gpu_ctx *ctx = b->ctx;
+    if (!h_raw || cap < need) CTX_FAIL(FAZ_ERR_CAPACITY, "raw buffer needs %zu bytes", need);
+    HIPCHECK(hipSetDevice(k->device));
+    int rc = fetch_raw(k, h_raw);
+    if (rc) ctx->err = k->err;
+    return (faz_status) rc;
+}
+
+faz_status faz_gpu_batch_field(const faz_batch *b, int i, faz_interp_params *p) {
+    if (!b || !p || i < 0 || i >= b->n) return FAZ_ERR_INVALID;
+    *p = b->prm[i];
+    return FAZ_OK;
+}
+
+faz_status faz_gpu_batch_stats(const faz_batch *b, faz_batch_stats *s, int *schedules) {
+    if (!b || !s) return FAZ_ERR_INVALID;
+    *s = b->st;
+    if (schedules)
+        for (int i = 0; i < b->n; i++) schedules[i] = b->schedule[i];
     return FAZ_OK;
 }
 

@@ -12,6 +12,7 @@
 #include <memory>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "faz/config.hpp"
@@ -637,6 +638,62 @@ faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_param
     memcpy(m, buf.data(), buf.size());
     *out = m;
     *out_size = buf.size();
+    return FAZ_OK;
+}
+
+faz_status faz_sz_compress_batch_f32(faz_batch *b, const int *rel_mode, const double *eb, uint8_t **out,
+                                     size_t *out_size) {
+    if (!b || !rel_mode || !eb || !out || !out_size) return FAZ_ERR_INVALID;
+    faz_batch_stats s;
+    faz_gpu_batch_stats(b, &s, nullptr);
+    int n = 0;
+    for (faz_interp_params p; faz_gpu_batch_field(b, n, &p) == FAZ_OK;) n++;
+    std::vector<size_t> sizes(n);
+    std::vector<double> aeb(n), rng(n);
+    int rc = faz_gpu_batch_compress(b, rel_mode, eb, sizes.data(), aeb.data(), rng.data());
+    if (rc) return rc;
+    std::vector<std::vector<uint8_t>> raw(n), streams(n);
+    for (int i = 0; i < n; i++) {
+        raw[i].resize(sizes[i]);
+        if ((rc = faz_gpu_batch_fetch_raw(b, i, raw[i].data(), sizes[i]))) return rc;
+    }
+    // zstd + Config trailer per field, as compress_interp_stream writes them
+    std::vector<int> ok(n, 0);
+    auto work = [&](int t, int nt) {
+        for (int i = t; i < n; i += nt) {
+            faz_interp_params p;
+            faz_gpu_batch_field(b, i, &p);
+            QoZ::Config conf = interp_config(&p, 0.0, -1.0);
+            if (rel_mode[i]) {
+                conf.errorBoundMode = QoZ::EB_ABS;  // cal_abs_error_bound after EB_REL
+                conf.relErrorBound = eb[i];
+                conf.rng = rng[i];
+                conf.absErrorBound = aeb[i];
+            } else {
+                conf.errorBoundMode = QoZ::EB_ABS;
+                conf.absErrorBound = eb[i];
+            }
+            if (!zstd_wrap(raw[i], streams[i], QoZ::Config::size_est())) continue;
+            append_trailer(conf, streams[i]);
+            ok[i] = 1;
+        }
+    };
+    const int nt = std::max(1, std::min(n, 16));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++) pool.emplace_back(work, t, nt);
+    for (auto &th: pool) th.join();
+    for (int i = 0; i < n; i++)
+        if (!ok[i]) return FAZ_ERR_INVALID;
+    for (int i = 0; i < n; i++) {
+        uint8_t *m = (uint8_t *) malloc(streams[i].size());
+        if (!m) {
+            for (int j = 0; j < i; j++) free(out[j]);
+            return FAZ_ERR_INVALID;
+        }
+        memcpy(m, streams[i].data(), streams[i].size());
+        out[i] = m;
+        out_size[i] = streams[i].size();
+    }
     return FAZ_OK;
 }
 

@@ -199,6 +199,47 @@ faz_status faz_gpu_lorenzo_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw
 faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
                                        const float *data, int data_on_device, uint8_t **out, size_t *out_size);
 
+/* --- Batches of fields of uneven shapes (ALGO_INTERP engine).  No
+ * counterpart in the reference, which compresses one field per call: this is
+ * SZ_compress (sz.hpp:54-91, SZInterp.hpp:387-414) over many fields at once,
+ * each field's stream identical to a single-field call's.  The inputs are
+ * bound at creation (device arrays that stay valid and that the caller
+ * refills); each field's range kernel sits in one hipGraph and its
+ * prediction/quantization/histogram/outlier launches on its own branch of a
+ * second, which is replayed while the bounds stay the same and re-captured
+ * and updated in place (hipGraphExecUpdate) when they change.  autotune != 0
+ * times the launch schedules of every 3-D field on its bound input and keeps
+ * the fastest (all produce the same bytes). */
+typedef struct faz_batch faz_batch;
+
+typedef struct {
+    uint64_t calls, captures, updates, instantiations, replays, tuned_fields;
+    double range_ms, pred_ms, encode_ms, total_ms;  /* phases of the last call */
+} faz_batch_stats;
+
+/* fields[i]: shape and engine options of field i (eb ignored); d_in[i]: its
+ * device input. */
+faz_status faz_gpu_batch_create(faz_gpu_ctx *ctx, const faz_interp_params *fields, const float *const *d_in, int n,
+                                int autotune, faz_batch **out);
+void faz_gpu_batch_destroy(faz_batch *b);
+/* Compresses every bound field: rel_mode[i] != 0 takes eb[i] relative to the
+ * field's range (calAbsErrorBound, SZ_compress), else absolute.  raw_sizes[i]
+ * receives the engine buffer size; abs_eb / rng (may be NULL) the absolute
+ * bound and the float range of each field. */
+faz_status faz_gpu_batch_compress(faz_batch *b, const int *rel_mode, const double *eb, size_t *raw_sizes,
+                                  double *abs_eb, double *rng);
+/* Engine buffer of field i from the last faz_gpu_batch_compress. */
+faz_status faz_gpu_batch_fetch_raw(faz_batch *b, int i, uint8_t *h_raw, size_t cap);
+/* Counters of the batch and (schedules, may be NULL: n ints) the launch
+ * schedule the autotuner picked per field. */
+faz_status faz_gpu_batch_stats(const faz_batch *b, faz_batch_stats *s, int *schedules);
+/* SZ_compress streams of every bound field (zstd + Config trailer per field,
+ * as faz_sz_compress_f32 writes them); out[i] is released with faz_free. */
+faz_status faz_sz_compress_batch_f32(faz_batch *b, const int *rel_mode, const double *eb, uint8_t **out,
+                                     size_t *out_size);
+/* Parameters field i was bound with. */
+faz_status faz_gpu_batch_field(const faz_batch *b, int i, faz_interp_params *p);
+
 /* --- A20: the general frontend with the Lorenzo predictors, N = 1..4:
  * SZGeneralFrontend (include/QoZ/frontend/SZGeneralFrontend.hpp:36-116)
  * over ComposedPredictor{LorenzoPredictor<1>, LorenzoPredictor<2>}
