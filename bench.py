@@ -176,6 +176,94 @@ def tuner_report(eng, x, dims, rel, mn, mx):
     return out
 
 
+def batch_fields(count, seed):
+    """The batched workload: `count` float32 fields drawn from a seed among
+    (i) 2-D 1800x3600 lat-lon fields: zonal cos(lat) bands + k^-3 GRF + 1e-3
+    noise, (ii) 3-D 100x500x500 k^-11/3 GRFs, (iii) 3-D 256x384x384 GRFs with
+    5% of the points in exact-zero masked boxes."""
+    import torch
+    from faz_amd.learned import grf_field
+    r = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        kind = int(r.integers(0, 3))
+        s = int(r.integers(1 << 40))
+        if kind == 0:
+            shape = (1800, 3600)
+            f = grf_field(shape, s, slope=3.0, noise=1e-3).view(shape)
+            lat = torch.linspace(-math.pi / 2, math.pi / 2, shape[0], device="cuda").view(-1, 1)
+            f = 0.5 * f + 0.5 * (torch.cos(lat) * (0.6 + 0.4 * torch.cos(6 * lat))).expand(shape)
+        elif kind == 1:
+            shape = (100, 500, 500)
+            f = grf_field(shape, s, slope=11.0 / 3.0).view(shape)
+        else:
+            shape = (256, 384, 384)
+            f = grf_field(shape, s, slope=11.0 / 3.0).view(shape)
+            g = torch.Generator(device="cuda")
+            g.manual_seed(s)
+            masked = 0
+            while masked < 0.05 * f.numel():
+                c = [int(torch.randint(0, n, (1,), generator=g, device="cuda").item()) for n in shape]
+                h = [int(torch.randint(8, 48, (1,), generator=g, device="cuda").item()) for _ in shape]
+                sl = tuple(slice(ci, min(n, ci + hi)) for ci, hi, n in zip(c, h, shape))
+                f[sl] = 0.0
+                masked = int((f == 0).sum().item())
+        out.append((shape, f.contiguous().view(-1)))
+    torch.cuda.synchronize()
+    return out
+
+
+def batch_report(eng, count, seed, rel, steps=3):
+    """Batched uneven fields: one hipGraph of per-field branches, autotuned
+    launch schedules, against the same fields compressed one call at a time."""
+    import torch
+    import faz_amd
+    fields = batch_fields(count, seed)
+    ps = [faz_amd.make_params(s, 0.0, algo=1, max_step=64 if len(s) == 2 else 32) for s, _ in fields]
+    t0 = time.perf_counter()
+    b = faz_amd.Batch(eng, ps, [f for _, f in fields], autotune=True)
+    create_s = time.perf_counter() - t0
+    try:
+        b.compress(True, rel)  # captures the prediction graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sizes = b.compress(True, rel)
+        torch.cuda.synchronize()
+        t_batch = (time.perf_counter() - t0) / steps
+        st = b.stats()
+        # the same fields one SZ_compress-equivalent call at a time
+        for (s, f), p in zip(fields[:2], ps[:2]):
+            eng.compress_device(f, p, True, rel)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        single_sizes = []
+        for _ in range(steps):
+            single_sizes = [eng.compress_device(f, p, True, rel) for (s, f), p in zip(fields, ps)]
+        torch.cuda.synchronize()
+        t_single = (time.perf_counter() - t0) / steps
+        same = all(int(a) == int(c) for a, c in zip(sizes, single_sizes))
+        # byte check on the last field compressed singly and on the first one
+        last = eng.fetch_raw(single_sizes[-1])
+        same = same and last == b.fetch(len(fields) - 1, sizes[-1])
+        nbytes = sum(4 * f.numel() for _, f in fields)
+        kinds = {}
+        for s, _ in fields:
+            kinds["x".join(map(str, s))] = kinds.get("x".join(map(str, s)), 0) + 1
+        return {"fields": len(fields), "shapes": kinds, "input_gb": round(nbytes / 1e9, 3), "rel_eb": rel,
+                "batch_ms": round(t_batch * 1e3, 3), "batch_gbs": round(nbytes / t_batch / 1e9, 2),
+                "single_calls_ms": round(t_single * 1e3, 3), "single_calls_gbs": round(nbytes / t_single / 1e9, 2),
+                "speedup": round(t_single / t_batch, 3), "ratio": round(nbytes / float(sum(sizes)), 3),
+                "engine_buffers_identical": bool(same), "create_s": round(create_s, 3),
+                "graph": {k: st[k] for k in ("captures", "updates", "instantiations", "replays")},
+                "phases_ms": {k: round(st[k], 3) for k in ("range_ms", "pred_ms", "encode_ms", "total_ms")},
+                "schedules": {str(k): st["schedules"].count(k) for k in sorted(set(st["schedules"]))}}
+    finally:
+        b.close()
+        del fields
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +274,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tuner", action="store_true", help="skip the exhaustive / learned tuner comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-fields report")
+    ap.add_argument("--batch-fields", type=int, default=48)
     ap.add_argument("--copy-probe", action="store_true",
                     help="one device copy of the field (4n bytes read, 4n written) before the steps, the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE in rocprofv3 --pmc runs")
@@ -276,6 +366,11 @@ def main():
             line["tuner"] = tuner_report(eng, x, dims, args.rel, mn, mx)
         except Exception as e:  # reported beside the measurement
             line["tuner"] = {"error": str(e)[:200]}
+    if not args.no_batch:
+        try:
+            line["batch"] = batch_report(eng, args.batch_fields, args.seed + 17, args.rel)
+        except Exception as e:  # reported beside the measurement
+            line["batch"] = {"error": str(e)[:200]}
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(x, dims, args.rel)

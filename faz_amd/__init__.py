@@ -895,6 +895,23 @@ class Batch:
         eb = [eb] * self.n if isinstance(eb, (float, int)) else list(eb)
         return np.array([1 if r else 0 for r in rel], np.int32), np.array(eb, np.float64)
 
+    def compress(self, rel=True, eb=1e-3):
+        """Compresses every field on the device; returns the engine buffer sizes
+        (the buffers stay on the device until compress_raw / fetch)."""
+        import torch
+        m, e = self._modes(rel, eb)
+        sizes = np.zeros(self.n, np.uint64)
+        torch.cuda.synchronize()
+        self.engine._check(self.engine._L.faz_gpu_batch_compress(self.h, _np_ptr(m), _np_ptr(e), _np_ptr(sizes), None,
+                                                                 None), "batch_compress")
+        return sizes
+
+    def fetch(self, i, size):
+        buf = np.zeros(int(size), np.uint8)
+        self.engine._check(self.engine._L.faz_gpu_batch_fetch_raw(self.h, i, _np_ptr(buf), buf.size),
+                           "batch_fetch_raw")
+        return buf.tobytes()
+
     def compress_raw(self, rel=True, eb=1e-3):
         """Engine buffers of every field: (list of bytes, absolute bounds, ranges)."""
         import torch

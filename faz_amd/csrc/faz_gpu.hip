@@ -25,6 +25,10 @@
 #include <string>
 #include <numeric>
 #include <vector>
+#include <atomic>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 
 using namespace faz;
 
@@ -1116,6 +1120,8 @@ struct faz_batch {
     hipEvent_t ev[6] = {};
     faz_batch_stats st;
     uint32_t *h_minmax = nullptr;  // pinned, 2 per field
+    uint32_t *h_hist = nullptr;    // pinned, freq then first (2 x 65536) per field
+    uint64_t *h_scal = nullptr;    // pinned, outlier count, bit total, error flag per field
 };
 
 namespace {
@@ -1166,8 +1172,18 @@ float key2f(uint32_t key) {
 int batch_capture_range(faz_batch *b) {
     hipStream_t cs = b->ctx->stream;
     BHIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < b->n; i++) launch_range(b->d_in[i], b->num[i], b->kid[i]->minmax.p, cs);
-    BHIP(hipStreamEndCapture(cs, &b->g_range));
+    BHIP(hipEventRecord(b->ev[0], cs));
+    bool ok = true;
+    for (int i = 0; i < b->n && ok; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        ok = hipStreamWaitEvent(k->stream, b->ev[0], 0) == hipSuccess;
+        if (!ok) break;
+        launch_range(b->d_in[i], b->num[i], k->minmax.p, k->stream);
+        ok = hipEventRecord(k->ev[6], k->stream) == hipSuccess && hipStreamWaitEvent(cs, k->ev[6], 0) == hipSuccess;
+    }
+    hipError_t e = hipStreamEndCapture(cs, &b->g_range);
+    if (!ok) BFAIL(FAZ_ERR_HIP, "capturing the range graph failed");
+    BHIP(e);
     BHIP(hipGraphInstantiate(&b->x_range, b->g_range, nullptr, nullptr, 0));
     return FAZ_OK;
 }
@@ -1269,7 +1285,10 @@ faz_status faz_gpu_batch_create(faz_gpu_ctx *ctx, const faz_interp_params *field
     }
     for (auto &e: b->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(FAZ_ERR_HIP);
-    if (hipHostMalloc((void **) &b->h_minmax, 8 * (size_t) n, 0) != hipSuccess) return fail(FAZ_ERR_HIP);
+    if (hipHostMalloc((void **) &b->h_minmax, 8 * (size_t) n, 0) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_hist, 2 * 65536 * 4 * (size_t) n, 0) != hipSuccess ||
+        hipHostMalloc((void **) &b->h_scal, 3 * 8 * (size_t) n, 0) != hipSuccess)
+        return fail(FAZ_ERR_HIP);
     int rc = batch_capture_range(b);
     if (rc) return fail(rc);
     // ranges of the bound inputs: the eager runs below need a bound and max |x|
@@ -1322,6 +1341,8 @@ void faz_gpu_batch_destroy(faz_batch *b) {
     for (auto &e: b->ev)
         if (e) (void) hipEventDestroy(e);
     if (b->h_minmax) (void) hipHostFree(b->h_minmax);
+    if (b->h_hist) (void) hipHostFree(b->h_hist);
+    if (b->h_scal) (void) hipHostFree(b->h_scal);
     delete b;
 }
 
@@ -1332,6 +1353,8 @@ faz_status faz_gpu_batch_compress(faz_batch *b, const int *rel_mode, const doubl
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t cs = ctx->stream;
     const int n = b->n;
+    char errbuf[512] = {0};
+    for (int i = 0; i < n; i++) b->h_scal[3 * i + 2] = 0;
     // 1. ranges of every field (one graph)
     HIPCHECK(hipEventRecord(b->ev[1], cs));
     HIPCHECK(hipGraphLaunch(b->x_range, cs));
@@ -1381,38 +1404,69 @@ faz_status faz_gpu_batch_compress(faz_batch *b, const int *rel_mode, const doubl
     HIPCHECK(hipEventRecord(b->ev[2], cs));
     HIPCHECK(hipGraphLaunch(b->x_pred, cs));
     HIPCHECK(hipEventRecord(b->ev[3], cs));
-    // 3. histograms to the host, trees, every encode enqueued, one wait
+    // 3. histograms to pinned host memory, trees built by a pool of host
+    //    threads, each field's encode enqueued as soon as its tree is ready,
+    //    one wait at the end
     for (int i = 0; i < n; i++) {
         faz_gpu_ctx *k = b->kid[i];
-        HIPCHECK(hipMemcpyAsync(k->h_freq.data(), k->freq.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
-        HIPCHECK(hipMemcpyAsync(k->h_first.data(), k->first.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
+        uint32_t *hh = b->h_hist + (size_t) i * 2 * 65536;
+        HIPCHECK(hipMemcpyAsync(hh, k->freq.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
+        HIPCHECK(hipMemcpyAsync(hh + 65536, k->first.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
     }
     HIPCHECK(hipStreamSynchronize(cs));
-    for (int i = 0; i < n; i++) {
+    std::vector<int> tree_rc(n, 1);  // 1 pending, 0 built, < 0 failed
+    std::mutex mu;
+    std::condition_variable cv;
+    std::atomic<int> next{0};
+    auto builder = [&]() {
+        for (int i; (i = next.fetch_add(1)) < n;) {
+            PendingRaw &pr = b->kid[i]->pending;
+            pr.valid = false;
+            const uint32_t *hh = b->h_hist + (size_t) i * 2 * 65536;
+            const int r = build_huffman(hh, hh + 65536, 65536, pr.tab) ? -1 : 0;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                tree_rc[i] = r;
+            }
+            cv.notify_all();
+        }
+    };
+    const int nthreads = std::max(1, std::min(n, 8));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++) pool.emplace_back(builder);
+    int rc = FAZ_OK;
+    for (int i = 0; i < n && !rc; i++) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return tree_rc[i] != 1; });
+        }
         faz_gpu_ctx *k = b->kid[i];
         PendingRaw &pr = k->pending;
-        pr.valid = false;
-        if (build_huffman(k->h_freq.data(), k->h_first.data(), 65536, pr.tab))
-            CTX_FAIL(FAZ_ERR_UNSUPPORTED, "field %d: Huffman tree could not be built", i);
-        faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
-        HIPCHECK(hipEventRecord(b->ev[4], cs));
-        HIPCHECK(hipStreamWaitEvent(k->stream, b->ev[4], 0));
-        int rc = encode_enqueue(k, k->codes.p, b->ncodes[i], &ht);
-        if (rc) {
-            ctx->err = k->err;
-            return (faz_status) rc;
+        if (tree_rc[i]) {
+            snprintf(errbuf, sizeof(errbuf), "field %d: Huffman tree could not be built", i);
+            rc = FAZ_ERR_UNSUPPORTED;
+            break;
         }
+        faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
+        rc = encode_enqueue(k, k->codes.p, b->ncodes[i], &ht);
+        if (rc) snprintf(errbuf, sizeof(errbuf), "%s", k->err.c_str());
     }
-    for (int i = 0; i < n; i++) HIPCHECK(hipStreamSynchronize(b->kid[i]->stream));
+    for (auto &th: pool) th.join();
+    if (rc) CTX_FAIL(rc, "%s", errbuf);
+    for (int i = 0; i < n; i++) {
+        faz_gpu_ctx *k = b->kid[i];
+        HIPCHECK(hipEventRecord(k->ev[6], k->stream));
+        HIPCHECK(hipStreamWaitEvent(cs, k->ev[6], 0));
+        HIPCHECK(hipMemcpyAsync(b->h_scal + 3 * i, k->scalars.p, 16, hipMemcpyDeviceToHost, cs));
+        HIPCHECK(hipMemcpyAsync(b->h_scal + 3 * i + 2, k->err_flag.p, 4, hipMemcpyDeviceToHost, cs));
+    }
     HIPCHECK(hipEventRecord(b->ev[5], cs));
     HIPCHECK(hipStreamSynchronize(cs));
     // 4. sizes of every field's engine buffer
     for (int i = 0; i < n; i++) {
         faz_gpu_ctx *k = b->kid[i];
-        uint64_t sc[2];
-        uint32_t errflag = 0;
-        HIPCHECK(hipMemcpy(sc, k->scalars.p, 16, hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(&errflag, k->err_flag.p, 4, hipMemcpyDeviceToHost));
+        const uint64_t *sc = b->h_scal + 3 * i;
+        const uint32_t errflag = (uint32_t) sc[2];
         if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: field %d: a block segment fell outside the stream", i);
         if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "field %d: a quantization code exceeded 16 bits", i);
         const faz_interp_params &p = b->prm[i];
