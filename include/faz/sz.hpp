@@ -8,8 +8,11 @@
 //   * errors throw faz::Error instead of printing and calling exit(0);
 //   * SZ_decompress does not write the reference's debug file
 //     "waved.qoz.dec.final2" (sz.hpp:198);
-//   * the GPU engine serves ALGO_INTERP streams in float; other algorithms
-//     and types throw faz::Error (FAZ_ERR_UNSUPPORTED).
+//   * the GPU engines serve float data: ALGO_INTERP; ALGO_LORENZO_REG (the
+//     3-D fast frontend, and the general frontend for other ranks with the
+//     Lorenzo predictors only); ALGO_INTERP_LORENZO with conf.QoZ set (the
+//     `faz -q` tuner).  Other algorithms, engine modes and types throw
+//     faz::Error (FAZ_ERR_UNSUPPORTED).
 #pragma once
 
 #include <stdexcept>
