@@ -481,3 +481,101 @@ def compress_test_lorenzo(field, starts, sbs, eb):
                                                 ctypes.byref(sq))
     assert rc == 0, rc
     return raw[:rs.value].tobytes(), by.value, sq.value
+
+
+class SZ3Out(ctypes.Structure):
+    """Mirror of oracle_sz3_out (faz_oracle.h)."""
+
+    _fields_ = [
+        ("sampled", ctypes.c_int),
+        ("sample_dims", ctypes.c_size_t * 4),
+        ("sampling_block", ctypes.c_size_t),
+        ("lorenzo_bytes", ctypes.c_size_t),
+        ("interp_bytes", ctypes.c_size_t * 3),
+        ("lorenzo_ratio", ctypes.c_double),
+        ("interp_ratio", ctypes.c_double),
+        ("use_interp", ctypes.c_int),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("quantbin_estimate", ctypes.c_int),
+        ("lorenzo_bytes_pd2", ctypes.c_size_t),
+        ("lorenzo_bytes_16384", ctypes.c_size_t),
+        ("quantbin", ctypes.c_int),
+        ("final_lorenzo_ratio", ctypes.c_double),
+    ]
+
+
+def sz3_out_dict(o):
+    d = {k: getattr(o, k) for k, _ in SZ3Out._fields_}
+    d["sample_dims"] = list(o.sample_dims)
+    d["interp_bytes"] = list(o.interp_bytes)
+    return d
+
+
+def _sz3_lib():
+    L = lib()
+    vp, sz = ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t)
+    L.oracle_sampling.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_size_t, vp, sz]
+    L.oracle_block_interp_test.argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_size_t, vp, ctypes.c_size_t, sz, sz]
+    L.oracle_quant_intervals.argtypes = [vp, vp, ctypes.c_double, sz]
+    L.oracle_sz3_tune.argtypes = [vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, ctypes.POINTER(SZ3Out)]
+    L.oracle_sz3_compress_stream.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_size_t,
+                                             sz, ctypes.POINTER(SZ3Out)]
+    return L
+
+
+def sampling(x):
+    """sampling<T,N>: (sample array or None, sample dims, sampling block)."""
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.array(x.shape, np.uint64)
+    out = np.zeros(x.size, np.float32)
+    sd = np.zeros(4, np.uint64)
+    sb = ctypes.c_size_t()
+    rc = _sz3_lib().oracle_sampling(_ptr(x), x.ndim, _ptr(d), _ptr(out), out.size, _ptr(sd), ctypes.byref(sb))
+    assert rc in (0, 1), rc
+    shape = tuple(int(v) for v in sd[:x.ndim])
+    return (None if rc == 1 else out[:int(np.prod(shape))].reshape(shape).copy()), shape, sb.value
+
+
+def block_interp_test(x, eb, algo, direction, block):
+    """Block interpolation test: (pre-zstd buffer, zstd-framed size)."""
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.array(x.shape, np.uint64)
+    cap = 64 * 1024 + x.size * 16
+    raw = np.zeros(cap, np.uint8)
+    rs, by = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = _sz3_lib().oracle_block_interp_test(_ptr(x), x.ndim, _ptr(d), float(eb), int(algo), int(direction), int(block),
+                                             _ptr(raw), cap, ctypes.byref(rs), ctypes.byref(by))
+    assert rc == 0, rc
+    return raw[:rs.value].tobytes(), by.value
+
+
+def quant_intervals(x, eb):
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.array(x.shape, np.uint64)
+    n = ctypes.c_size_t()
+    return _sz3_lib().oracle_quant_intervals(_ptr(x), _ptr(d), float(eb), ctypes.byref(n)), n.value
+
+
+def sz3_tune(x, eb, rel):
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.array(list(x.shape) + [1] * (4 - x.ndim), np.uint64)
+    o = SZ3Out()
+    rc = _sz3_lib().oracle_sz3_tune(_ptr(x), x.ndim, _ptr(d), float(eb), float(rel), ctypes.byref(o))
+    assert rc == 0, rc
+    return sz3_out_dict(o)
+
+
+def sz3_compress_stream(x, eb, rel=True):
+    """SZ_compress in the default mode: (stream, tuning decisions)."""
+    x = np.ascontiguousarray(x, np.float32)
+    d = np.array(list(x.shape) + [1] * (4 - x.ndim), np.uint64)
+    cap = 64 * 1024 + x.size * 16
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_size_t()
+    o = SZ3Out()
+    rc = _sz3_lib().oracle_sz3_compress_stream(_ptr(x), x.ndim, _ptr(d), 1 if rel else 0, float(eb), _ptr(out), cap,
+                                               ctypes.byref(n), ctypes.byref(o))
+    assert rc == 0, rc
+    return out[:n.value].tobytes(), sz3_out_dict(o)

@@ -1566,6 +1566,386 @@ static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int l
 }
 
 // ---------------------------------------------------------------------------
+// The default (SZ3) tuning mode: Tuning with predictorTuningRate = 0
+// (SZInterp.hpp:2096-2180) and SZ_compress_Interp_lorenzo's Lorenzo branch
+// (SZI:2783-2898): uniform sampling (Extraction.hpp:8-212), the deprecated
+// block interpolation compressor as the interpolation test
+// (compressor/deprecated/SZBlockInterpolationCompressor.hpp:110-458, driven by
+// do_not_use_this_interp_compress_block_test, SZI:558-576), the Lorenzo
+// compressor on the sample, optimize_quant_invl_3d (QuantOptimization.hpp:
+// 14-147).
+// ---------------------------------------------------------------------------
+
+// cal_sampling_ratio (Extraction.hpp:8-15): float result
+static float sz3_sampling_ratio(size_t block, size_t n, size_t dmin, const size_t *dims, int N) {
+    size_t sample_n = 1;
+    for (int d = 0; d < N; d++) sample_n *= dims[d] / dmin * 2 * block;
+    return (float) (sample_n * 1.0 / n);
+}
+
+// sampling<T,N> (Extraction.hpp:17-212): 2*sb points per dmin along each
+// axis, the sb points after the first sb and the sb points before the last
+// sb of every dmin-long segment.  Returns false (sample = whole field) when
+// sb < 9.
+static bool sz3_sampling(const float *data, int N, const size_t *dims, std::vector<float> &out, size_t *sdims,
+                         size_t &sb) {
+    size_t num = 1, dmin = dims[0];
+    for (int d = 0; d < N; d++) num *= dims[d], dmin = std::min(dmin, dims[d]);
+    sb = dmin;
+    while (sz3_sampling_ratio(sb, num, dmin, dims, N) > 0.035) sb--;
+    if (sb * 2 > dmin) sb = dmin / 2;
+    if (sb < 9) {
+        for (int d = 0; d < N; d++) sdims[d] = dims[d];
+        out.clear();
+        return false;
+    }
+    size_t sn = 1, sst[4], st[4];
+    for (int d = 0; d < N; d++) sdims[d] = dims[d] / dmin * 2 * sb, sn *= sdims[d];
+    sst[N - 1] = st[N - 1] = 1;
+    for (int d = N - 2; d >= 0; d--) sst[d] = sst[d + 1] * sdims[d + 1], st[d] = st[d + 1] * dims[d + 1];
+    out.assign(sn, 0.0f);
+    for (size_t q = 0; q < sn; q++) {
+        size_t src = 0;
+        for (int d = 0; d < N; d++) {
+            const size_t c = q / sst[d] % sdims[d], bd = c / (2 * sb), i = c % (2 * sb);
+            const size_t di = i < sb ? i + sb : dmin - 3 * sb + i;
+            src += (bd * dmin + di) * st[d];
+        }
+        out[q] = data[src];
+    }
+    return true;
+}
+
+// SZBlockInterpolationCompressor (deprecated, SBI:110-458) for N = 1..3:
+// independent blocks of blockSize, each with its first point (prediction
+// 0) and its own levels; the block's lines in the loop order of its
+// direction.
+struct BIEngine {
+    int N = 3;
+    size_t dims[4] = {1, 1, 1, 1}, st[4] = {0, 0, 0, 0};
+    size_t bs = 6;
+    int algo = 1, dir = 0;
+    double eb = 0;
+    Quantizer quantizer;
+    std::vector<int32_t> codes;
+
+    void quantize(float *d, float pred) { codes.push_back(quantizer.quantize_and_overwrite(*d, pred)); }
+
+    // block_interpolation_1d (SBI:205-262), predict-and-overwrite
+    void line(float *data, size_t begin, size_t end, size_t stride) {
+        const size_t n = (end - begin) / stride + 1;
+        if (n <= 1) return;
+        const size_t s3 = 3 * stride, s5 = 5 * stride;
+        if (algo == 0 || n < 5) {
+            for (size_t i = 1; i + 1 < n; i += 2) {
+                float *d = data + begin + i * stride;
+                quantize(d, interp_linear(*(d - stride), *(d + stride)));
+            }
+            if (n % 2 == 0) {
+                float *d = data + begin + (n - 1) * stride;
+                if (n < 4) quantize(d, *(d - stride));
+                else quantize(d, interp_linear1(*(d - s3), *(d - stride)));
+            }
+        } else {
+            float *d = data + begin + stride;
+            quantize(d, interp_quad_1(*(d - stride), *(d + stride), *(d + s3)));
+            for (size_t i = 3; i + 3 < n; i += 2) {
+                d = data + begin + i * stride;
+                quantize(d, interp_cubic(*(d - s3), *(d - stride), *(d + stride), *(d + s3)));
+            }
+            if (n % 2 == 0) {
+                d = data + begin + (n - 3) * stride;
+                quantize(d, interp_quad_2(*(d - s3), *(d - stride), *(d + stride)));
+                d += 2 * stride;
+                quantize(d, interp_quad_3(*(d - s5), *(d - s3), *(d - stride)));
+            } else {
+                d = data + begin + (n - 2) * stride;
+                quantize(d, interp_quad_2(*(d - s3), *(d - stride), *(d + stride)));
+            }
+        }
+    }
+
+    size_t off(size_t a, size_t b, size_t c) const { return a * st[0] + b * st[1] + c * st[2]; }
+
+    // block_interpolation (SBI:263-458): the three passes of one level in the
+    // direction's loop order (axis, outer axis and step, inner axis and step)
+    void level(float *data, const size_t *B, const size_t *E, size_t s) {
+        const size_t s2 = 2 * s;
+        if (N == 1) {
+            line(data, B[0], E[0], s);
+            return;
+        }
+        if (N == 2) {
+            if (dir == 0) {
+                for (size_t j = B[1]; j <= E[1]; j += s2) line(data, B[0] * st[0] + j, E[0] * st[0] + j, s * st[0]);
+                for (size_t i = B[0]; i <= E[0]; i += s) line(data, i * st[0] + B[1], i * st[0] + E[1], s);
+            } else {
+                for (size_t i = B[0]; i <= E[0]; i += s2) line(data, i * st[0] + B[1], i * st[0] + E[1], s);
+                for (size_t j = B[1]; j <= E[1]; j += s) line(data, B[0] * st[0] + j, E[0] * st[0] + j, s * st[0]);
+            }
+            return;
+        }
+        // N = 3: {axis, outer axis, outer step factor, inner axis, inner step factor} per pass
+        static const int P[6][3][5] = {
+            {{0, 1, 2, 2, 2}, {1, 0, 1, 2, 2}, {2, 0, 1, 1, 1}},
+            {{0, 1, 2, 2, 2}, {2, 0, 1, 1, 2}, {1, 0, 1, 2, 1}},
+            {{1, 0, 2, 2, 2}, {0, 1, 1, 2, 2}, {2, 0, 1, 1, 1}},
+            {{1, 0, 2, 2, 2}, {2, 0, 2, 1, 1}, {0, 1, 1, 2, 1}},
+            {{2, 0, 2, 1, 2}, {0, 1, 2, 2, 1}, {1, 0, 1, 2, 1}},
+            {{2, 0, 2, 1, 2}, {1, 0, 2, 2, 1}, {0, 1, 1, 2, 1}},
+        };
+        for (int p = 0; p < 3; p++) {
+            const int a = P[dir][p][0], u = P[dir][p][1], w = P[dir][p][3];
+            const size_t su = s * P[dir][p][2], sw = s * P[dir][p][4];
+            for (size_t x = B[u]; x <= E[u]; x += su)
+                for (size_t y = B[w]; y <= E[w]; y += sw) {
+                    size_t c0[3], c1[3];
+                    c0[u] = c1[u] = x;
+                    c0[w] = c1[w] = y;
+                    c0[a] = B[a];
+                    c1[a] = E[a];
+                    line(data, off(c0[0], c0[1], c0[2]), off(c1[0], c1[1], c1[2]), s * st[a]);
+                }
+        }
+    }
+
+    // compress (SBI:110-146)
+    void compress(float *data) {
+        size_t nb[4], b[4] = {0, 0, 0, 0}, total = 1;
+        for (int d = 0; d < N; d++) nb[d] = (dims[d] - 1) / bs + 1, total *= nb[d];
+        codes.clear();
+        quantizer.unpred.clear();
+        for (size_t t = 0; t < total; t++) {
+            size_t B[3] = {0, 0, 0}, E[3] = {0, 0, 0};
+            unsigned max_level = 1;
+            for (int d = 0; d < N; d++) {
+                B[d] = b[d] * bs;
+                const size_t bd = B[d] + bs > dims[d] ? dims[d] - B[d] : bs;
+                E[d] = B[d] + bd - 1;
+                if (max_level < std::ceil(std::log2((double) bd))) max_level = (unsigned) std::ceil(std::log2((double) bd));
+            }
+            size_t o = 0;
+            for (int d = 0; d < N; d++) o += B[d] * st[d];
+            quantize(data + o, 0.0f);
+            for (unsigned lv = max_level; lv > 0 && lv <= max_level; lv--) level(data, B, E, (size_t) 1 << (lv - 1));
+            for (int d = N - 1; d >= 0; d--) {
+                if (++b[d] < nb[d]) break;
+                b[d] = 0;
+            }
+        }
+    }
+
+    // compress's buffer (SBI:150-165): dims, blockSize, interpolator id,
+    // direction id, quantizer, Huffman tree and bits
+    bool save(std::vector<uint8_t> &out) const {
+        for (int d = 0; d < N; d++) put(out, (uint64_t) dims[d]);
+        put(out, (uint32_t) bs);
+        put(out, (int32_t) algo);
+        put(out, (int32_t) dir);
+        out.push_back(2);
+        put(out, eb);
+        put(out, (int) quantizer.radius);
+        put(out, (size_t) quantizer.unpred.size());
+        const uint8_t *u = (const uint8_t *) quantizer.unpred.data();
+        out.insert(out.end(), u, u + quantizer.unpred.size() * sizeof(float));
+        Huffman h;
+        if (!h.init(codes.data(), codes.size())) return false;
+        h.save(out);
+        h.encode(codes.data(), codes.size(), out);
+        return true;
+    }
+};
+
+// do_not_use_this_interp_compress_block_test (SZI:558-576): the zstd-framed
+// size of the block interpolation compressor on a copy of the data.
+static size_t sz3_block_test(const float *data, int N, const size_t *dims, double eb, int algo, int dir, size_t bs,
+                             std::vector<uint8_t> *raw_out = nullptr) {
+    BIEngine e;
+    e.N = N;
+    size_t s = 1, num = 1;
+    for (int d = N - 1; d >= 0; d--) e.dims[d] = dims[d], e.st[d] = s, s *= dims[d];
+    num = s;
+    e.bs = bs;
+    e.algo = algo;
+    e.dir = dir;
+    e.eb = eb;
+    e.quantizer.set_eb(eb);
+    e.quantizer.radius = 32768;
+    std::vector<float> v(data, data + num);
+    e.compress(v.data());
+    std::vector<uint8_t> raw;
+    if (!e.save(raw)) return 0;
+    std::vector<uint8_t> z(ZSTD_compressBound(raw.size()));
+    size_t zs = ZSTD_compress(z.data(), z.size(), raw.data(), raw.size(), 3);
+    if (raw_out) *raw_out = raw;
+    return 8 + zs;
+}
+
+// optimize_quant_invl_3d (QO:89-147): the quantizer bin count from a
+// histogram of 1-layer 3-D Lorenzo errors (MetaLorenzoPredictor.hpp:42-46)
+// at about every 100th point of the pointer walk; the mean estimate it also
+// makes does not reach the result.
+static int sz3_quant_intervals(const float *data, size_t r1, size_t r2, size_t r3, double precision,
+                               size_t *samples = nullptr) {
+    std::vector<size_t> intervals(32768, 0);
+    const size_t dist = 100, r23 = r2 * r3, len = r1 * r23;
+    size_t sample_count = 0, offset_count = dist - 2, offset_count_2 = 0, n1 = 1, n2 = 1;
+    const float *p = data + r23 + r3 + offset_count;
+    while ((size_t) (p - data) < len) {
+        const float pred = p[-1] + p[-(ptrdiff_t) r3] + p[-(ptrdiff_t) r23] - p[-(ptrdiff_t) r3 - 1] -
+                           p[-(ptrdiff_t) r23 - 1] - p[-(ptrdiff_t) r23 - (ptrdiff_t) r3] +
+                           p[-(ptrdiff_t) r23 - (ptrdiff_t) r3 - 1];
+        const float err = std::fabs(pred - *p);
+        size_t idx = (size_t) ((err / precision + 1) / 2);
+        if (idx >= intervals.size()) idx = intervals.size() - 1;
+        intervals[idx]++;
+        offset_count += dist;
+        if (offset_count >= r3) {
+            n2++;
+            if (n2 == r2) {
+                n1++;
+                n2 = 1;
+                p += r3;
+            }
+            offset_count_2 = (n1 + n2) % dist;
+            p += (r3 + dist - offset_count) + (dist - offset_count_2);
+            offset_count = dist - offset_count_2;
+            if (offset_count == 0) offset_count++;
+        } else {
+            p += dist;
+        }
+        sample_count++;
+    }
+    if (samples) *samples = sample_count;
+    // estimate_quantization_intervals (QO:25-39)
+    const size_t target = (size_t) (sample_count * 0.999);
+    size_t sum = 0, i = 0;
+    for (i = 0; i < intervals.size(); i++) {
+        sum += intervals[i];
+        if (sum > target) break;
+    }
+    if (i == intervals.size()) i = intervals.size() - 1;
+    unsigned acc = 2 * (unsigned) (i + 1), b = acc - 1;
+    b |= b >> 1, b |= b >> 2, b |= b >> 4, b |= b >> 8, b |= b >> 16;
+    const unsigned num_intervals = 2 * (b + 1);
+    return num_intervals > 32 ? (int) num_intervals : 32;
+}
+
+// Pre-zstd buffer of the general frontend on one array.
+static bool gf_raw(const oracle_gf_params *p, const float *data, std::vector<uint8_t> &out) {
+    GFEngine e;
+    if (!e.set(p)) return false;
+    std::vector<float> v(data, data + e.num());
+    e.compress(v.data());
+    out.clear();
+    return e.save(out);
+}
+
+// SZ_compress_LorenzoReg's zstd-framed size (LR:95-114) on an array: the
+// fast frontend for 3-D, the general frontend otherwise.
+static size_t sz3_lorenzo_size(const float *data, int N, const size_t *dims, double eb, int bs, int pred_dim,
+                               int radius) {
+    std::vector<uint8_t> raw;
+    if (N == 3) {
+        oracle_lr_params p;
+        memset(&p, 0, sizeof(p));
+        for (int d = 0; d < 3; d++) p.dims[d] = dims[d];
+        p.eb = eb;
+        p.block_size = bs;
+        p.pred_dim = pred_dim;
+        p.lorenzo = p.lorenzo2 = 1;
+        p.regression = 0;
+        p.radius = radius;
+        if (!lr_raw(&p, data, raw)) return 0;
+    } else {
+        oracle_gf_params p;
+        memset(&p, 0, sizeof(p));
+        p.N = N;
+        for (int d = 0; d < N; d++) p.dims[d] = dims[d];
+        p.eb = eb;
+        p.block_size = bs;
+        p.lorenzo = p.lorenzo2 = 1;
+        p.radius = radius;
+        if (!gf_raw(&p, data, raw)) return 0;
+    }
+    std::vector<uint8_t> z(ZSTD_compressBound(raw.size()));
+    return 8 + ZSTD_compress(z.data(), z.size(), raw.data(), raw.size(), 3);
+}
+
+static int default_block_size(int N) { return N == 1 ? 128 : (N == 2 ? 16 : 6); }
+
+// Tuning with predictorTuningRate = 0 (SZI:2096-2180) and, when Lorenzo
+// wins, SZ_compress_Interp_lorenzo's Lorenzo branch (SZI:2783-2898).
+static int sz3_tune(const float *data, int N, const size_t *dims, double eb, double rel, oracle_sz3_out *o) {
+    memset(o, 0, sizeof(*o));
+    size_t num = 1;
+    for (int d = 0; d < N; d++) num *= dims[d];
+    std::vector<float> sample;
+    size_t sdims[4] = {1, 1, 1, 1}, sb = 0;
+    const bool sampled = sz3_sampling(data, N, dims, sample, sdims, sb);
+    size_t snum = 1;
+    for (int d = 0; d < N; d++) snum *= sdims[d], o->sample_dims[d] = sdims[d];
+    o->sampling_block = sb;
+    o->sampled = sampled;
+    o->interp_algo = 1;  // Config defaults (CFG:417-419)
+    o->direction = 0;
+    o->use_interp = 1;
+    if (!sampled) return 0;
+    size_t bytes = sz3_lorenzo_size(sample.data(), N, sdims, eb, 5, N, 32768);
+    if (!bytes) return -2;
+    o->lorenzo_bytes = bytes;
+    double lorenzo_ratio = snum * 1.0 * sizeof(float) / bytes;
+    double best = 0;
+    for (int op = 0; op <= 1; op++) {
+        size_t b = sz3_block_test(sample.data(), N, sdims, eb, op, o->direction, sb);
+        if (!b) return -2;
+        o->interp_bytes[op] = b;
+        const double r = snum * 1.0 * sizeof(float) / b;
+        if (r > best) best = r, o->interp_algo = op;
+    }
+    int fact = 1;
+    for (int d = 2; d <= N; d++) fact *= d;
+    {
+        size_t b = sz3_block_test(sample.data(), N, sdims, eb, o->interp_algo, fact - 1, sb);
+        if (!b) return -2;
+        o->interp_bytes[2] = b;
+        const double r = snum * 1.0 * sizeof(float) / b;
+        if (r > best * 1.02) best = r, o->direction = fact - 1;
+    }
+    o->lorenzo_ratio = lorenzo_ratio;
+    o->interp_ratio = best;
+    o->use_interp = !(lorenzo_ratio > best && lorenzo_ratio < 80 && best < 80);
+    if (o->use_interp) return 0;
+    // the Lorenzo branch (SZI:2783-2850): setDims resets blockSize and
+    // pred_dim, so the tests run with the Config defaults for N and only the
+    // bin count survives into the final Config
+    int quantbin = 65536, pred_dim = N;
+    const int bsd = default_block_size(N);
+    if (N == 3) {
+        quantbin = sz3_quant_intervals(data, dims[0], dims[1], dims[2], eb);
+        o->quantbin_estimate = quantbin;
+        pred_dim = 2;
+        if (quantbin / 2 > 65536) return -3;
+        size_t b = sz3_lorenzo_size(sample.data(), N, sdims, eb, bsd, 2, quantbin / 2);
+        if (!b) return -2;
+        o->lorenzo_bytes_pd2 = b;
+        const double r = snum * 1.0 * sizeof(float) / b;
+        if (r > lorenzo_ratio * 1.02) lorenzo_ratio = r;
+        else pred_dim = 3;
+    }
+    if (rel < 1.01e-6 && lorenzo_ratio > 5 && quantbin != 16384) {
+        size_t b = sz3_lorenzo_size(sample.data(), N, sdims, eb, bsd, pred_dim, 8192);
+        if (!b) return -2;
+        o->lorenzo_bytes_16384 = b;
+        const double r = snum * 1.0 * sizeof(float) / b;
+        if (r > lorenzo_ratio * 1.02) lorenzo_ratio = r, quantbin = 16384;
+    }
+    o->quantbin = quantbin;
+    o->final_lorenzo_ratio = lorenzo_ratio;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // QoZ exhaustive sampling tuner (SZInterp.hpp:1589-2557, QoZ path, RD target,
 // no wavelets, no Lorenzo test).
 // ---------------------------------------------------------------------------
@@ -2068,15 +2448,6 @@ int oracle_gf_compress(const oracle_gf_params *p, const float *data, int32_t *co
     return 0;
 }
 
-static bool gf_raw(const oracle_gf_params *p, const float *data, std::vector<uint8_t> &out) {
-    GFEngine e;
-    if (!e.set(p)) return false;
-    std::vector<float> v(data, data + e.num());
-    e.compress(v.data());
-    out.clear();
-    return e.save(out);
-}
-
 int oracle_gf_raw(const oracle_gf_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out) {
     if (!p) return -1;
     std::vector<uint8_t> buf;
@@ -2182,6 +2553,127 @@ int oracle_compress_test_lorenzo(int N, double eb, const float *field, const siz
     *raw_size = r.size();
     if (r.size() > cap) return -3;
     memcpy(raw, r.data(), r.size());
+    return 0;
+}
+
+int oracle_sampling(const float *data, int N, const size_t *dims, float *out, size_t cap, size_t *sample_dims,
+                    size_t *sampling_block) {
+    if (N < 1 || N > 4) return -1;
+    std::vector<float> s;
+    bool sampled = sz3_sampling(data, N, dims, s, sample_dims, *sampling_block);
+    if (!sampled) return 1;
+    if (s.size() > cap) return -3;
+    memcpy(out, s.data(), s.size() * sizeof(float));
+    return 0;
+}
+
+int oracle_block_interp_test(const float *data, int N, const size_t *dims, double eb, int algo, int dir, size_t bs,
+                             uint8_t *raw, size_t cap, size_t *raw_size, size_t *bytes) {
+    if (N < 1 || N > 3) return -1;
+    std::vector<uint8_t> r;
+    *bytes = sz3_block_test(data, N, dims, eb, algo, dir, bs, &r);
+    if (!*bytes) return -2;
+    *raw_size = r.size();
+    if (raw) {
+        if (r.size() > cap) return -3;
+        memcpy(raw, r.data(), r.size());
+    }
+    return 0;
+}
+
+int oracle_quant_intervals(const float *data, const size_t *dims, double eb, size_t *samples) {
+    return sz3_quant_intervals(data, dims[0], dims[1], dims[2], eb, samples);
+}
+
+int oracle_sz3_tune(const float *data, int N, const size_t *dims, double eb, double rel, oracle_sz3_out *out) {
+    if (N < 1 || N > 3 || !out) return -1;
+    return sz3_tune(data, N, dims, eb, rel, out);
+}
+
+// SZ_compress for ALGO_INTERP_LORENZO without conf.QoZ / conf.FZ (the
+// default `faz` mode): calAbsErrorBound, range, the SZ3 tuning, then
+// SZ_compress_Interp or SZ_compress_LorenzoReg with the tuned Config.
+int oracle_sz3_compress_stream(const float *data, int N, const size_t *dims, int rel_mode, double eb_in,
+                               uint8_t *out, size_t cap, size_t *n_out, oracle_sz3_out *tune) {
+    if (N < 1 || N > 3) return -1;
+    size_t num = 1;
+    for (int d = 0; d < N; d++) num *= dims[d];
+    float mn, mx;
+    oracle_range(data, num, &mn, &mx);
+    const double rng = (double) (float) (mx - mn);
+    double eb, rel;
+    if (rel_mode) {
+        rel = eb_in;
+        eb = rel * rng;
+    } else {
+        eb = eb_in;
+        rel = eb / rng;  // SZI:2569-2570
+    }
+    oracle_sz3_out t;
+    int rc = sz3_tune(data, N, dims, eb, rel, &t);
+    if (rc) return rc;
+    if (tune) *tune = t;
+    std::vector<uint8_t> s;
+    oracle_interp_params ip;
+    memset(&ip, 0, sizeof(ip));
+    ip.N = N;
+    for (int d = 0; d < N; d++) ip.dims[d] = dims[d];
+    ip.interp_algo = t.interp_algo;
+    ip.direction = t.direction;
+    ip.alpha = -1, ip.beta = -1;
+    ip.block_size = 32;
+    ip.radius = 32768;
+    TrailerFields tf;
+    if (t.use_interp) {
+        ip.eb = eb;
+        std::vector<uint8_t> raw;
+        if (!interp_raw(&ip, data, raw)) return -2;
+        s.resize(8 + ZSTD_compressBound(raw.size()));
+        uint64_t rawLen = raw.size();
+        memcpy(s.data(), &rawLen, 8);
+        size_t z = ZSTD_compress(s.data() + 8, s.size() - 8, raw.data(), raw.size(), 3);
+        s.resize(8 + z);
+        tf.cmprAlgo = 2;
+    } else {
+        std::vector<uint8_t> raw;
+        const int bsd = default_block_size(N);
+        if (N == 3) {
+            oracle_lr_params p;
+            memset(&p, 0, sizeof(p));
+            for (int d = 0; d < 3; d++) p.dims[d] = dims[d];
+            p.eb = eb;
+            p.block_size = bsd;
+            p.pred_dim = 3;
+            p.lorenzo = p.lorenzo2 = 1;
+            p.radius = t.quantbin / 2;
+            if (!lr_raw(&p, data, raw)) return -2;
+        } else {
+            oracle_gf_params p;
+            memset(&p, 0, sizeof(p));
+            p.N = N;
+            for (int d = 0; d < N; d++) p.dims[d] = dims[d];
+            p.eb = eb;
+            p.block_size = bsd;
+            p.lorenzo = p.lorenzo2 = 1;
+            p.radius = t.quantbin / 2;
+            if (!gf_raw(&p, data, raw)) return -2;
+        }
+        s.resize(8 + ZSTD_compressBound(raw.size()));
+        uint64_t rawLen = raw.size();
+        memcpy(s.data(), &rawLen, 8);
+        size_t z = ZSTD_compress(s.data() + 8, s.size() - 8, raw.data(), raw.size(), 3);
+        s.resize(8 + z);
+        tf.cmprAlgo = 0;
+        tf.lorenzo = true, tf.lorenzo2 = true, tf.regression = false;
+        tf.blockSize = bsd;
+        ip.radius = t.quantbin / 2;
+    }
+    size_t before = s.size();
+    config_save(s, &ip, eb, rel, tf);
+    put(s, (int) (s.size() - before));
+    *n_out = s.size();
+    if (s.size() > cap) return -3;
+    memcpy(out, s.data(), s.size());
     return 0;
 }
 

@@ -130,6 +130,40 @@ int oracle_compress_test_lorenzo(int N, double eb, const float *field, const siz
                                  size_t nblocks, size_t sbs, uint8_t *raw, size_t cap, size_t *raw_size,
                                  size_t *bytes, double *square_error);
 
+/* The default (SZ3) tuning mode: Tuning with predictorTuningRate = 0
+ * (SZInterp.hpp:2096-2180) and the Lorenzo branch of
+ * SZ_compress_Interp_lorenzo (SZI:2783-2898). */
+typedef struct {
+    int sampled;                 /* 0: the field is its own sample (sampling block < 9) */
+    size_t sample_dims[4];
+    size_t sampling_block;
+    size_t lorenzo_bytes;        /* SZ_compress_LorenzoReg on the sample (bs 5, lorenzo + lorenzo2) */
+    size_t interp_bytes[3];      /* block interpolation test: linear, cubic (direction 0), best with N!-1 */
+    double lorenzo_ratio, interp_ratio;
+    int use_interp, interp_algo, direction;
+    int quantbin_estimate;       /* optimize_quant_invl_3d (N = 3, Lorenzo chosen) */
+    size_t lorenzo_bytes_pd2, lorenzo_bytes_16384;
+    int quantbin;                /* final quantbinCnt of the Lorenzo path */
+    double final_lorenzo_ratio;
+} oracle_sz3_out;
+
+/* sampling<T,N> (Extraction.hpp:17-212).  Returns 1 when the field is its
+ * own sample (nothing written). */
+int oracle_sampling(const float *data, int N, const size_t *dims, float *out, size_t cap, size_t *sample_dims,
+                    size_t *sampling_block);
+/* do_not_use_this_interp_compress_block_test (SZI:558-576) through the
+ * block interpolation compressor (deprecated/SZBlockInterpolationCompressor.hpp):
+ * *bytes = zstd-framed size; raw (may be NULL) = the pre-zstd buffer. */
+int oracle_block_interp_test(const float *data, int N, const size_t *dims, double eb, int algo, int dir, size_t bs,
+                             uint8_t *raw, size_t cap, size_t *raw_size, size_t *bytes);
+/* optimize_quant_invl_3d (QuantOptimization.hpp:89-147). */
+int oracle_quant_intervals(const float *data, const size_t *dims, double eb, size_t *samples);
+/* The SZ3 tuning decisions for a field (N = 1..3). */
+int oracle_sz3_tune(const float *data, int N, const size_t *dims, double eb, double rel, oracle_sz3_out *out);
+/* SZ_compress for ALGO_INTERP_LORENZO in the default mode (no -q, no FZ). */
+int oracle_sz3_compress_stream(const float *data, int N, const size_t *dims, int rel_mode, double eb_in,
+                               uint8_t *out, size_t cap, size_t *n_out, oracle_sz3_out *tune);
+
 /* QoZ exhaustive sampling tuner (`faz -q`): Tuning() with conf.QoZ set,
  * no wavelets, tuningTarget RD (include/QoZ/api/impl/SZInterp.hpp:1589-2557),
  * with sampleBlocks (SZI:993-1127), CompressTest (SZI:1131-1392),

@@ -28,6 +28,9 @@
 #include "QoZ/encoder/HuffmanEncoder.hpp"
 #include "QoZ/lossless/Lossless_zstd.hpp"
 #include "QoZ/lossless/Lossless_bypass.hpp"
+#include "QoZ/compressor/deprecated/SZBlockInterpolationCompressor.hpp"
+#include "QoZ/utils/Extraction.hpp"
+#include "QoZ/utils/QuantOptimization.hpp"
 
 #include <chrono>
 #include <cstdio>
@@ -553,6 +556,72 @@ static int run_ctest_lorenzo(const Args &a) {
     return 0;
 }
 
+// sampling<T,N> (Extraction.hpp): the sample, its shape and block.
+template<QoZ::uint N>
+static int run_sampling(const Args &a) {
+    typedef float T;
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    std::vector<size_t> dims = parse_list(get(a, "dims"));
+    size_t num = 0, block = 0;
+    std::vector<size_t> sdims(N);
+    std::vector<T> s = QoZ::sampling<T, N>(input.data(), dims, num, sdims, block);
+    write_bytes(get(a, "out"), s.data(), s.size() * sizeof(T));
+    printf("{\"sampled\": %d, \"sampling_block\": %zu, \"sample_num\": %zu, \"sample_dims\": [", s.empty() ? 0 : 1,
+           block, num);
+    for (QoZ::uint i = 0; i < N; i++) printf("%s%zu", i ? ", " : "", sdims[i]);
+    printf("]}\n");
+    return 0;
+}
+
+// do_not_use_this_interp_compress_block_test (SZInterp.hpp:558-576),
+// composed from the same classes: the block interpolation compressor's
+// buffer (Lossless_bypass) and its zstd-framed size.
+template<QoZ::uint N>
+static int run_blockinterp(const Args &a) {
+    typedef float T;
+    std::vector<T> input = read_vec<T>(get(a, "in"));
+    std::vector<size_t> dims = parse_list(get(a, "dims"));
+    double eb = atof(get(a, "eb").c_str());
+    QoZ::Config conf;
+    conf.absErrorBound = eb;
+    conf.setDims(dims.begin(), dims.end());
+    conf.blockSize = atoi(get(a, "block").c_str());
+    conf.interpAlgo = (uint8_t) atoi(get(a, "algo").c_str());
+    conf.interpDirection = (uint8_t) atoi(get(a, "dir").c_str());
+    size_t bytes = 0;
+    {
+        std::vector<T> d(input);
+        auto sz = QoZ::SZBlockInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>,
+                                                      QoZ::Lossless_zstd>(QoZ::LinearQuantizer<T>(eb),
+                                                                          QoZ::HuffmanEncoder<int>(),
+                                                                          QoZ::Lossless_zstd());
+        QoZ::uchar *c = sz.compress(conf, d.data(), bytes);
+        delete[] c;
+    }
+    if (a.count("raw")) {
+        std::vector<T> d(input);
+        auto sz = QoZ::SZBlockInterpolationCompressor<T, N, QoZ::LinearQuantizer<T>, QoZ::HuffmanEncoder<int>,
+                                                      QoZ::Lossless_bypass>(QoZ::LinearQuantizer<T>(eb),
+                                                                            QoZ::HuffmanEncoder<int>(),
+                                                                            QoZ::Lossless_bypass());
+        size_t n = 0;
+        QoZ::uchar *c = sz.compress(conf, d.data(), n);
+        write_bytes(get(a, "raw"), c, n);
+        delete[] c;
+    }
+    printf("{\"bytes\": %zu}\n", bytes);
+    return 0;
+}
+
+static int run_quantinvl(const Args &a) {
+    std::vector<float> input = read_vec<float>(get(a, "in"));
+    std::vector<size_t> dims = parse_list(get(a, "dims"));
+    double eb = atof(get(a, "eb").c_str());
+    int q = QoZ::optimize_quant_invl_3d<float>(input.data(), dims[0], dims[1], dims[2], eb);
+    printf("{\"quantbin\": %d}\n", q);
+    return 0;
+}
+
 static int run_huffman(const Args &a) {
     std::vector<int> codes = read_vec<int>(get(a, "in"));
     QoZ::HuffmanEncoder<int> enc;
@@ -600,6 +669,16 @@ int main(int argc, char **argv) {
         if (parse_list(get(a, "dims")).size() == 2) return lor ? run_ctest_lorenzo<2>(a) : run_ctest<2>(a);
         return lor ? run_ctest_lorenzo<3>(a) : run_ctest<3>(a);
     }
+    if (cmd == "sampling" || cmd == "blockinterp") {
+        const bool s = cmd == "sampling";
+        switch (parse_list(get(a, "dims")).size()) {
+            case 1: return s ? run_sampling<1>(a) : run_blockinterp<1>(a);
+            case 2: return s ? run_sampling<2>(a) : run_blockinterp<2>(a);
+            case 3: return s ? run_sampling<3>(a) : run_blockinterp<3>(a);
+            default: return 2;
+        }
+    }
+    if (cmd == "quantinvl") return run_quantinvl(a);
     if (cmd == "general") {
         switch (parse_list(get(a, "dims")).size()) {
             case 1: return run_general<1>(a);

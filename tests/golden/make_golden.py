@@ -203,6 +203,22 @@ def ctest_lorenzo_cases():
     return c
 
 
+def sz3_cases():
+    """Default-mode (SZ3) tuning fixtures: a field, its sample, block
+    interpolation buffers of the sample and optimize_quant_invl_3d."""
+    r = np.random.default_rng(31337)
+    c = []
+    c.append(("z1d_3000", smooth(r, (3000,), 2), 1e-3))
+    c.append(("z2d_300x410", smooth(r, (300, 410), 2), 1e-3))
+    c.append(("z2d_97x250_noisy", (smooth(r, (97, 250), 1) + 0.05 * r.standard_normal((97, 250))).astype(np.float32),
+              1e-2))
+    c.append(("z3d_66x64x70", smooth(r, (66, 64, 70), 2), 1e-3))
+    c.append(("z3d_48_noisy", (smooth(r, (48, 48, 48), 1) + 0.02 * r.standard_normal((48, 48, 48))).astype(np.float32),
+              1e-3))
+    c.append(("z3d_60x40x50", smooth(r, (60, 40, 50), 3), 1e-2))
+    return c
+
+
 def harness_args(inp, out, dims, kw):
     a = [HARNESS, "interp", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, dims)),
          f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"algo={kw.get('algo', 1)}",
@@ -289,6 +305,44 @@ def main():
                             raw=np.fromfile(raw, np.uint8), square_error=np.array([res["square_error"]]),
                             ref_zstd_bytes=np.array([res["bytes"]]),
                             params=np.array(json.dumps(dict(kw, sbs=sbs, dims=list(x.shape)))))
+        print("wrote", name)
+    for name, x, rel in sz3_cases():
+        if only and not name.startswith(only):
+            continue
+        inp = os.path.join(tmp, "in.f32")
+        out = os.path.join(tmp, "sample.f32")
+        x.astype(np.float32).tofile(inp)
+        dims = ",".join(map(str, x.shape))
+        res = json.loads(subprocess.check_output([HARNESS, "sampling", f"in={inp}", f"out={out}", f"dims={dims}"]))
+        sample = np.fromfile(out, np.float32) if res["sampled"] else x.ravel()
+        sdims = res["sample_dims"] if res["sampled"] else list(x.shape)
+        sample.astype(np.float32).tofile(out)
+        eb = rel * float(np.float32(x.max() - x.min()))
+        block = max(res["sampling_block"], 6)
+        nf = {1: 1, 2: 2, 3: 6}[x.ndim]
+        raws, sizes, keys = [], [], []
+        for algo in (0, 1):
+            for d in range(nf):
+                raw = os.path.join(tmp, "bi.raw")
+                b = json.loads(subprocess.check_output(
+                    [HARNESS, "blockinterp", f"in={out}", "dims=" + ",".join(map(str, sdims)), f"eb={eb!r}",
+                     f"block={block}", f"algo={algo}", f"dir={d}", f"raw={raw}"]))
+                raws.append(np.fromfile(raw, np.uint8))
+                sizes.append(b["bytes"])
+                keys.append((algo, d))
+        q = []
+        if x.ndim == 3:
+            for f in (1.0, 0.01):
+                q.append(json.loads(subprocess.check_output(
+                    [HARNESS, "quantinvl", f"in={inp}", f"dims={dims}", f"eb={eb * f!r}"]))["quantbin"])
+        lens = np.array([len(v) for v in raws], np.int64)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), input=x.astype(np.float32),
+                            sample=sample.astype(np.float32), raws=np.concatenate(raws), raw_lens=lens,
+                            ref_zstd_bytes=np.array(sizes, np.int64), keys=np.array(keys, np.int32),
+                            quantbin=np.array(q, np.int32),
+                            params=np.array(json.dumps(dict(rel=rel, eb=eb, block=block, dims=list(x.shape),
+                                                            sampled=res["sampled"], sample_dims=sdims,
+                                                            sampling_block=res["sampling_block"]))))
         print("wrote", name)
     if only:
         return

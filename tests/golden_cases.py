@@ -18,6 +18,7 @@ TUNING = names("t")
 LORENZO = names("l")
 CTEST = names("q")
 GENERAL = names("g")
+SZ3 = names("z")
 CTEST_LORENZO = names("k")
 
 

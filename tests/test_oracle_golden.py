@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import (CTEST, CTEST_LORENZO, GENERAL, HUFFMAN, INTERP, LORENZO, TUNING, ctest_params, gf_kwargs,
+from golden_cases import (CTEST, CTEST_LORENZO, GENERAL, HUFFMAN, INTERP, LORENZO, SZ3, TUNING, ctest_params, gf_kwargs,
                           load, lr_kwargs, lr_raw_equal, param_kwargs, trailer)
 
 
@@ -237,3 +237,47 @@ def test_compress_test_lorenzo_wide_codes():
     prm = oracle.make_gf_params((65, 65), c["params"]["eb"], block_size=5, radius=65536)
     codes = np.concatenate([oracle.gf_compress(b, prm)["codes"] for b in blocks])
     assert codes.max() > 65535
+
+
+def sz3_raws(c):
+    off = np.concatenate([[0], np.cumsum(c["raw_lens"])])
+    return [c["raws"][off[i]:off[i + 1]].tobytes() for i in range(len(c["raw_lens"]))]
+
+
+@pytest.mark.parametrize("name", SZ3)
+def test_sz3_sampling_block_interp_quant_intervals(name):
+    """The default (SZ3) tuning mode's pieces against the reference classes:
+    sampling<T,N> (Extraction.hpp), the block interpolation compressor's buffer
+    for every interpolator and direction (deprecated/
+    SZBlockInterpolationCompressor.hpp; zstd size within 0.5%) and
+    optimize_quant_invl_3d (QuantOptimization.hpp)."""
+    c = load(name)
+    p = c["params"]
+    x = c["input"]
+    s, sdims, sb = oracle.sampling(x)
+    assert sb == p["sampling_block"] and (s is not None) == bool(p["sampled"])
+    if s is not None:
+        assert list(sdims) == p["sample_dims"]
+        assert np.array_equal(s.ravel().view(np.uint32), c["sample"].view(np.uint32))
+    sample = c["sample"].reshape(p["sample_dims"])
+    for (algo, d), raw, ref in zip(c["keys"], sz3_raws(c), c["ref_zstd_bytes"]):
+        ours, nbytes = oracle.block_interp_test(sample, p["eb"], int(algo), int(d), p["block"])
+        assert ours == raw, (algo, d)
+        assert abs(nbytes - int(ref)) <= 0.005 * nbytes + 8
+    if x.ndim == 3:
+        for f, q in zip((1.0, 0.01), c["quantbin"]):
+            assert oracle.quant_intervals(x, p["eb"] * f)[0] == int(q)
+
+
+def test_sz3_block_interp_round_trip_bound():
+    """Block interpolation test: every point of every block gets exactly one
+    code, and the buffer's outlier count equals its zero codes."""
+    import struct
+    x = np.random.default_rng(5).standard_normal((40, 33, 27)).cumsum(0).cumsum(1).astype(np.float32)
+    eb = 1e-3 * float(x.max() - x.min())
+    raw, _ = oracle.block_interp_test(x, eb, 1, 3, 11)
+    o = 3 * 8 + 4 + 4 + 4
+    assert raw[o] == 2
+    nunpred = struct.unpack_from("<Q", raw, o + 1 + 8 + 4)[0]
+    codes, _ = oracle.huffman_decode(raw[o + 1 + 8 + 4 + 8 + 4 * nunpred:], x.size)
+    assert codes.size == x.size and int((codes == 0).sum()) == nunpred
