@@ -246,7 +246,31 @@ EXPORTS = [
     "faz_gpu_compress_test_lorenzo_f32", "faz_sz_compress_general_f32",
     "faz_gpu_batch_create", "faz_gpu_batch_destroy", "faz_gpu_batch_compress", "faz_gpu_batch_fetch_raw",
     "faz_gpu_batch_stats", "faz_sz_compress_batch_f32", "faz_gpu_batch_field",
+    "faz_gpu_sampling_f32", "faz_gpu_block_interp_test_f32", "faz_gpu_quant_intervals_f32", "faz_gpu_sz3_tune_f32",
+    "faz_sz_compress_sz3_f32",
 ]
+
+
+class SZ3Out(ctypes.Structure):
+    """Mirror of faz_sz3_out."""
+
+    _fields_ = [
+        ("sampled", ctypes.c_int),
+        ("sample_dims", ctypes.c_size_t * 4),
+        ("sampling_block", ctypes.c_size_t),
+        ("lorenzo_bytes", ctypes.c_size_t),
+        ("interp_bytes", ctypes.c_size_t * 3),
+        ("lorenzo_ratio", ctypes.c_double),
+        ("interp_ratio", ctypes.c_double),
+        ("use_interp", ctypes.c_int),
+        ("interp_algo", ctypes.c_int),
+        ("direction", ctypes.c_int),
+        ("quantbin_estimate", ctypes.c_int),
+        ("lorenzo_bytes_pd2", ctypes.c_size_t),
+        ("lorenzo_bytes_16384", ctypes.c_size_t),
+        ("quantbin", ctypes.c_int),
+        ("final_lorenzo_ratio", ctypes.c_double),
+    ]
 
 
 class BatchStats(ctypes.Structure):
@@ -320,6 +344,13 @@ def lib():
                                                         sz, P(sz), P(sz), P(ctypes.c_double)]
         L.faz_sz_compress_general_f32.argtypes = [vp, P(GeneralParams), ctypes.c_int, ctypes.c_double, vp,
                                                   ctypes.c_int, P(vp), P(sz)]
+        L.faz_gpu_sampling_f32.argtypes = [vp, vp, ctypes.c_int, vp, vp, sz, vp, P(sz), P(ctypes.c_int)]
+        L.faz_gpu_block_interp_test_f32.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_int,
+                                                    ctypes.c_int, sz, vp, sz, P(sz), P(sz)]
+        L.faz_gpu_quant_intervals_f32.argtypes = [vp, vp, vp, ctypes.c_double, P(ctypes.c_int), P(sz)]
+        L.faz_gpu_sz3_tune_f32.argtypes = [vp, vp, ctypes.c_int, vp, ctypes.c_double, ctypes.c_double, P(SZ3Out)]
+        L.faz_sz_compress_sz3_f32.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
+                                              P(vp), P(sz)]
         L.faz_gpu_batch_create.argtypes = [vp, P(InterpParams), P(vp), ctypes.c_int, ctypes.c_int, P(vp)]
         L.faz_gpu_batch_destroy.argtypes = [vp]
         L.faz_gpu_batch_destroy.restype = None
@@ -685,6 +716,76 @@ class Engine:
                                                               cap, ctypes.byref(rs), ctypes.byref(by),
                                                               ctypes.byref(sq)), "compress_test_lorenzo")
         return raw[:rs.value].tobytes(), by.value, sq.value
+
+    # --- default (SZ3) tuning mode --------------------------------------------
+    def sampling(self, x, shape):
+        """sampling<T,N> of a device field: (sample tensor or None, sample dims, sampling block)."""
+        import torch
+        torch.cuda.synchronize()
+        d = np.array(list(shape), np.uint64)
+        out = torch.empty(x.numel(), dtype=torch.float32, device=x.device)
+        sd = np.zeros(4, np.uint64)
+        sb = ctypes.c_size_t()
+        sampled = ctypes.c_int()
+        self._check(self._L.faz_gpu_sampling_f32(self.h, self._dev(x), len(shape), _np_ptr(d), self._dev(out),
+                                                 out.numel(), _np_ptr(sd), ctypes.byref(sb), ctypes.byref(sampled)),
+                    "sampling")
+        sdims = tuple(int(v) for v in sd[:len(shape)])
+        n = int(np.prod(sdims))
+        return (out[:n] if sampled.value else None), sdims, sb.value
+
+    def block_interp_test(self, x, shape, eb, algo, direction, block):
+        """Block interpolation test on a device array: (pre-zstd buffer, zstd-framed size)."""
+        import torch
+        torch.cuda.synchronize()
+        d = np.array(list(shape), np.uint64)
+        cap = 64 * 1024 + int(np.prod(shape)) * 16
+        raw = np.zeros(cap, np.uint8)
+        rs, by = ctypes.c_size_t(), ctypes.c_size_t()
+        self._check(self._L.faz_gpu_block_interp_test_f32(self.h, self._dev(x), len(shape), _np_ptr(d), float(eb),
+                                                          int(algo), int(direction), int(block), _np_ptr(raw), cap,
+                                                          ctypes.byref(rs), ctypes.byref(by)), "block_interp_test")
+        return raw[:rs.value].tobytes(), by.value
+
+    def quant_intervals(self, x, shape, eb):
+        import torch
+        torch.cuda.synchronize()
+        d = np.array(list(shape), np.uint64)
+        q, n = ctypes.c_int(), ctypes.c_size_t()
+        self._check(self._L.faz_gpu_quant_intervals_f32(self.h, self._dev(x), _np_ptr(d), float(eb), ctypes.byref(q),
+                                                        ctypes.byref(n)), "quant_intervals")
+        return q.value, n.value
+
+    def sz3_tune(self, x, shape, abs_eb, rel_eb):
+        import torch
+        torch.cuda.synchronize()
+        d = np.array(list(shape) + [1] * (4 - len(shape)), np.uint64)
+        o = SZ3Out()
+        self._check(self._L.faz_gpu_sz3_tune_f32(self.h, self._dev(x), len(shape), _np_ptr(d), float(abs_eb),
+                                                 float(rel_eb), ctypes.byref(o)), "sz3_tune")
+        r = {k: getattr(o, k) for k, _ in SZ3Out._fields_}
+        r["sample_dims"] = list(o.sample_dims)
+        r["interp_bytes"] = list(o.interp_bytes)
+        return r
+
+    def sz_compress_sz3(self, data, shape, eb, rel=True):
+        """`faz` default-mode stream (SZ3 tuning, then the chosen engine)."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        d = np.array(list(shape), np.uint64)
+        if isinstance(data, np.ndarray):
+            a = np.ascontiguousarray(data, np.float32)
+            ptr, ondev = _np_ptr(a), 0
+        else:
+            import torch
+            torch.cuda.synchronize()
+            ptr, ondev = self._dev(data), 1
+        self._check(self._L.faz_sz_compress_sz3_f32(self.h, len(shape), _np_ptr(d), 1 if rel else 0, float(eb), ptr,
+                                                    ondev, ctypes.byref(p), ctypes.byref(n)), "sz_compress_sz3")
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            self._L.faz_free(p)
 
     # --- exhaustive QoZ tuner -----------------------------------------------
     def tune_qoz(self, x, shape, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0,

@@ -256,11 +256,55 @@ int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_
 // SZ_compress_Interp_lorenzo with conf.QoZ set (SZInterp.hpp:2562-2899): the
 // exhaustive QoZ tuner (Tuning, SZI:1589-2557) on the GPU, then the
 // interpolation engine with the tuned Config.
+// SZ_compress_Interp_lorenzo in the default mode (SZI:2562-2899 with
+// predictorTuningRate = autoTuningRate = 0): the SZ3 tuning on the GPU
+// (faz_gpu_sz3_tune_f32), then SZ_compress_Interp or, when Lorenzo wins,
+// SZ_compress_LorenzoReg with the Config the Lorenzo branch leaves (its
+// setDims calls restore blockSize and pred_dim to the defaults for N; the bin
+// count of its tests stays).
+int compress_sz3_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
+    if (conf.N > 3 || conf.wavelet > 0 || conf.autoTuningRate > 0 || conf.predictorTuningRate > 0) {
+        faz_ctx_set_error(ctx, "the default tuning mode runs on 1- to 3-D fields without wavelets and with zero "
+                               "tuning rates (set conf.QoZ for the QoZ tuner)");
+        return FAZ_ERR_UNSUPPORTED;
+    }
+    int rc = cal_abs_error_bound(ctx, conf, d_data);
+    if (rc) return rc;
+    if (conf.rng < 0) {  // data_range (Statistic.hpp:11-21)
+        float mn, mx;
+        if ((rc = faz_gpu_range_f32(ctx, d_data, conf.num, &mn, &mx))) return rc;
+        conf.rng = (double) (float) (mx - mn);
+    }
+    if (conf.relErrorBound <= 0) conf.relErrorBound = conf.absErrorBound / conf.rng;
+    size_t dims[4] = {1, 1, 1, 1};
+    for (int i = 0; i < conf.N; i++) dims[i] = conf.dims[i];
+    faz_sz3_out o;
+    if ((rc = faz_gpu_sz3_tune_f32(ctx, d_data, conf.N, dims, conf.absErrorBound, conf.relErrorBound, &o))) return rc;
+    conf.interpAlgo = (uint8_t) o.interp_algo;
+    conf.interpDirection = (uint8_t) o.direction;
+    if (o.use_interp) {
+        conf.cmprAlgo = QoZ::ALGO_INTERP;
+        return compress_interp_stream(ctx, conf, d_data, out);
+    }
+    conf.cmprAlgo = QoZ::ALGO_LORENZO_REG;
+    conf.lorenzo = true;
+    conf.lorenzo2 = true;
+    conf.regression = false;
+    conf.regression2 = false;
+    conf.openmp = false;
+    conf.blockSize = conf.N == 1 ? 128 : (conf.N == 2 ? 16 : 6);
+    conf.pred_dim = conf.N;
+    conf.quantbinCnt = o.quantbin;
+    return compress_lorenzo_stream(ctx, conf, d_data, out);
+}
+
 int compress_qoz_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data, std::vector<uint8_t> &out) {
     if (!conf.QoZ) {
-        faz_ctx_set_error(ctx, "ALGO_INTERP_LORENZO is served with the QoZ tuner (conf.QoZ, faz -q); the SZ3 "
-                               "sampling tuner is outside the GPU path");
-        return FAZ_ERR_UNSUPPORTED;
+        if (conf.FZ) {
+            faz_ctx_set_error(ctx, "the FZ mode (wavelet tuning) is outside the GPU path");
+            return FAZ_ERR_UNSUPPORTED;
+        }
+        return compress_sz3_stream(ctx, conf, d_data, out);
     }
     if ((conf.N != 2 && conf.N != 3) || conf.wavelet > 0 || conf.tuningTarget != QoZ::TUNING_TARGET_RD) {
         faz_ctx_set_error(ctx, "the GPU QoZ tuner serves 2-D and 3-D fields, no wavelets, the PSNR target");
@@ -725,6 +769,33 @@ faz_status faz_sz_compress_general_f32(faz_gpu_ctx *ctx, const faz_general_param
     if (rc) return rc;
     std::vector<uint8_t> buf;
     rc = compress_lorenzo_stream(ctx, conf, d, buf);
+    if (rc) return rc;
+    uint8_t *m = (uint8_t *) malloc(buf.size());
+    if (!m) return FAZ_ERR_INVALID;
+    memcpy(m, buf.data(), buf.size());
+    *out = m;
+    *out_size = buf.size();
+    return FAZ_OK;
+}
+
+faz_status faz_sz_compress_sz3_f32(faz_gpu_ctx *ctx, int N, const size_t *dims, int rel_mode, double eb,
+                                   const float *data, int data_on_device, uint8_t **out, size_t *out_size) {
+    if (!ctx || !dims || !data || !out || !out_size || N < 1 || N > 4) return FAZ_ERR_INVALID;
+    QoZ::Config conf;
+    conf.setDims(dims, dims + N);
+    conf.cmprAlgo = QoZ::ALGO_INTERP_LORENZO;
+    if (rel_mode) {
+        conf.errorBoundMode = QoZ::EB_REL;
+        conf.relErrorBound = eb;
+    } else {
+        conf.errorBoundMode = QoZ::EB_ABS;
+        conf.absErrorBound = eb;
+    }
+    int rc;
+    const float *d = to_device(ctx, data, conf.num, data_on_device != 0, rc);
+    if (rc) return rc;
+    std::vector<uint8_t> buf;
+    rc = compress_sz3_stream(ctx, conf, d, buf);
     if (rc) return rc;
     uint8_t *m = (uint8_t *) malloc(buf.size());
     if (!m) return FAZ_ERR_INVALID;

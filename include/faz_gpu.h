@@ -199,6 +199,50 @@ faz_status faz_gpu_lorenzo_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw
 faz_status faz_sz_compress_lorenzo_f32(faz_gpu_ctx *ctx, const faz_lorenzo_params *p, int rel_mode, double eb,
                                        const float *data, int data_on_device, uint8_t **out, size_t *out_size);
 
+/* --- A18, the default (SZ3) tuning mode: Tuning with
+ * predictorTuningRate = 0 (include/QoZ/api/impl/SZInterp.hpp:2096-2180),
+ * what `faz` runs without -q: sampling (utils/Extraction.hpp:17-212), the
+ * Lorenzo compressor and the block interpolation compressor
+ * (compressor/deprecated/SZBlockInterpolationCompressor.hpp, driven by
+ * do_not_use_this_interp_compress_block_test, SZI:558-576) on the sample,
+ * and for the Lorenzo choice optimize_quant_invl_3d
+ * (utils/QuantOptimization.hpp:89-147) and the tests of SZI:2783-2850. */
+typedef struct {
+    int sampled;                 /* 0: the field is its own sample (sampling block < 9) */
+    size_t sample_dims[4];
+    size_t sampling_block;
+    size_t lorenzo_bytes;        /* Lorenzo compressor on the sample (blockSize 5, 1- and 2-layer) */
+    size_t interp_bytes[3];      /* block interpolation test: linear, cubic (direction 0), best with N!-1 */
+    double lorenzo_ratio, interp_ratio;
+    int use_interp, interp_algo, direction;
+    int quantbin_estimate;       /* optimize_quant_invl_3d (3-D, Lorenzo chosen) */
+    size_t lorenzo_bytes_pd2, lorenzo_bytes_16384;
+    int quantbin;                /* quantbinCnt of the Lorenzo path */
+    double final_lorenzo_ratio;
+} faz_sz3_out;
+
+/* sampling<T,N> of a device field (N <= 3) into d_out (cap floats);
+ * *sampled = 0 when the field is its own sample (nothing written). */
+faz_status faz_gpu_sampling_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims, float *d_out,
+                                size_t cap, size_t *sample_dims, size_t *sampling_block, int *sampled);
+/* do_not_use_this_interp_compress_block_test (SZI:558-576): the block
+ * interpolation compressor on a device array (N <= 3); *bytes = zstd-framed
+ * size, h_raw (may be NULL) = the pre-zstd buffer (SBI:150-165). */
+faz_status faz_gpu_block_interp_test_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims, double eb,
+                                         int algo, int direction, size_t block, uint8_t *h_raw, size_t cap,
+                                         size_t *raw_size, size_t *bytes);
+/* optimize_quant_invl_3d (QO:89-147) on a 3-D device field. */
+faz_status faz_gpu_quant_intervals_f32(faz_gpu_ctx *ctx, const float *d_in, const size_t *dims, double eb,
+                                       int *quantbin, size_t *samples);
+/* The default-mode tuning decisions for a device field (N <= 3). */
+faz_status faz_gpu_sz3_tune_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims, double abs_eb,
+                                double rel_eb, faz_sz3_out *out);
+/* SZ_compress for ALGO_INTERP_LORENZO in the default mode (no conf.QoZ, no
+ * conf.FZ; SZI:2562-2899): tuning, then the interpolation or Lorenzo engine
+ * with the tuned Config; zstd and trailer as the reference writes them. */
+faz_status faz_sz_compress_sz3_f32(faz_gpu_ctx *ctx, int N, const size_t *dims, int rel_mode, double eb,
+                                   const float *data, int data_on_device, uint8_t **out, size_t *out_size);
+
 /* --- Batches of fields of uneven shapes (ALGO_INTERP engine).  No
  * counterpart in the reference, which compresses one field per call: this is
  * SZ_compress (sz.hpp:54-91, SZInterp.hpp:387-414) over many fields at once,
