@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -15,6 +16,28 @@
 #include "../host/huffman_host.h"
 
 namespace faz {
+
+// Page-locked host array (transfers without a staging copy).
+template<class T>
+struct HostPinned {
+    T *p = nullptr;
+    bool pinned = false;
+    explicit HostPinned(size_t n) {
+        if (hipHostMalloc((void **) &p, n * sizeof(T), 0) == hipSuccess) {
+            pinned = true;
+        } else {
+            (void) hipGetLastError();
+            p = new T[n];
+        }
+    }
+    ~HostPinned() {
+        if (pinned) (void) hipHostFree(p);
+        else delete[] p;
+    }
+    HostPinned(const HostPinned &) = delete;
+    HostPinned &operator=(const HostPinned &) = delete;
+    T *data() { return p; }
+};
 
 template<class T>
 struct DevBuf {
@@ -48,6 +71,17 @@ struct PendingRaw {
     bool valid = false;
 };
 
+// Waits for a stream by polling it: the waits of the compression path are
+// short, and polling avoids the sleep and wake-up of a blocking wait.  After
+// about a second of polling it falls back to the blocking wait.
+inline hipError_t spin_wait(hipStream_t st) {
+    for (int i = 0; i < 2000000; i++) {
+        const hipError_t e = hipStreamQuery(st);
+        if (e != hipErrorNotReady) return e;
+    }
+    return hipStreamSynchronize(st);
+}
+
 }  // namespace faz
 
 struct faz_gpu_ctx {
@@ -61,12 +95,18 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
+    const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to
+    // engine plan of the last parameter set (see cached_plan in faz_gpu.hip)
+    std::shared_ptr<void> plan_cache;
+    std::vector<uint8_t> plan_key;
     hipEvent_t ev[8] = {};
     double timings[8] = {};
     // histogram of the codes last produced by interp_compress
     const uint16_t *hist_codes = nullptr;
     size_t hist_n = 0;
     std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
+    // pinned: histogram and first positions, then 8 words of scalars
+    faz::HostPinned<uint32_t> hp_hist{2 * 65536 + 8};
     std::vector<uint64_t> h_tab;  // packed code table of the last encode
     faz::PendingRaw pending;
     // per-launch timing of the interpolation pass kernels

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <numeric>
 #include <vector>
 #include <atomic>
@@ -282,6 +283,65 @@ int make_plan(const faz_interp_params *p, Plan &pl, std::string &why) {
     return FAZ_OK;
 }
 
+// The plan of the last parameter set a context compressed with: every
+// field but the bound keys it; the level bounds are set again from p->eb
+// (the same arithmetic as make_plan), so repeated compressions of one shape
+// skip the plan and the upload of its block prefix table.
+Plan &cached_plan(faz_gpu_ctx *ctx, const faz_interp_params *p, int &rc) {
+    std::vector<uint8_t> key;
+    auto add = [&](const void *v, size_t n) {
+        const uint8_t *b = static_cast<const uint8_t *>(v);
+        key.insert(key.end(), b, b + n);
+    };
+    add(&p->N, sizeof(p->N));
+    add(p->dims, sizeof(p->dims));
+    add(&p->interp_algo, sizeof(p->interp_algo));
+    add(&p->direction, sizeof(p->direction));
+    add(&p->alpha, sizeof(p->alpha));
+    add(&p->beta, sizeof(p->beta));
+    add(&p->max_step, sizeof(p->max_step));
+    add(&p->block_size, sizeof(p->block_size));
+    add(&p->radius, sizeof(p->radius));
+    add(&p->levelwise, sizeof(p->levelwise));
+    add(p->ops, sizeof(p->ops));
+    add(p->dirs, sizeof(p->dirs));
+    add(&p->tuning, sizeof(p->tuning));
+    add(&p->start_level, sizeof(p->start_level));
+    add(&p->end_level, sizeof(p->end_level));
+    Plan *pl = static_cast<Plan *>(ctx->plan_cache.get());
+    rc = FAZ_OK;
+    if (!pl || key != ctx->plan_key || !(p->eb > 0)) {
+        ctx->plan_cache.reset();
+        ctx->plan_key.clear();
+        std::shared_ptr<Plan> fresh = std::make_shared<Plan>();
+        rc = make_plan(p, *fresh, ctx->err);
+        ctx->plan_cache = fresh;
+        if (!rc) ctx->plan_key = key;
+        return *fresh;
+    }
+    const double eb = p->eb;
+    for (auto &L: pl->lv) {
+        const unsigned level = L.level;
+        if (p->alpha < 0) {
+            L.eb = level >= 3 ? eb * 0.5 : eb;
+        } else if (p->alpha >= 1) {
+            double r = std::pow(p->alpha, level - 1);
+            if (r > p->beta) r = p->beta;
+            L.eb = eb / r;
+        } else {
+            double r = 1 - (level - 1) * p->alpha;
+            if (r < p->beta) r = p->beta;
+            L.eb = eb * r;
+        }
+        L.recip = 1.0 / L.eb;
+        for (auto &d: L.passes) {
+            d.eb = L.eb;
+            d.recip = L.recip;
+        }
+    }
+    return *pl;
+}
+
 double elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
@@ -386,9 +446,9 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
     HIPCHECK(ctx->minmax.ensure(2));
     launch_range(d_in, n, ctx->minmax.p, ctx->stream);
     HIPCHECK(hipGetLastError());
-    uint32_t k[2];
+    uint32_t *k = ctx->hp_hist.data() + 2 * 65536 + 6;  // pinned
     HIPCHECK(hipMemcpyAsync(k, ctx->minmax.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    HIPCHECK(spin_wait(ctx->stream));
     auto key2f = [](uint32_t key) {
         uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
         float f;
@@ -412,10 +472,14 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
 // buffer and uploaded the block prefix table), outliers gathered into a
 // buffer of one slot per code, results read by the caller after the stream
 // drains (ctx->scalars[0] = outlier count, ctx->err_flag).
+// defer: no host synchronisation either, but allocations and the prefix
+// upload are allowed; the caller reads the outlier count and the error flag
+// after its own wait and computes timings[1..2] from ev[0..2].
 static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
-                           faz_interp_out *out, bool final_recon, double absmax, bool enqueue = false) {
-    Plan pl;
-    int rc = make_plan(p, pl, ctx->err);
+                           faz_interp_out *out, bool final_recon, double absmax, bool enqueue = false,
+                           bool defer = false) {
+    int rc = FAZ_OK;
+    Plan &pl = cached_plan(ctx, p, rc);
     if (rc) return rc;
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -437,9 +501,10 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(ctx->freq.ensure(65536));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
-    if (!enqueue) {
+    if (!enqueue && (ctx->prefix_key != prefix_key || ctx->prefix_at != ctx->prefix.p)) {
         HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
         ctx->prefix_key = prefix_key;
+        ctx->prefix_at = ctx->prefix.p;
     }
     launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
 
@@ -585,36 +650,43 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st);
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     HIPCHECK(hipGetLastError());
-    if (enqueue) {
+    uint64_t n_unpred = 0;
+    if (enqueue || defer) {
+        // outliers gathered into one slot per code: no wait for their count
+        if (defer) HIPCHECK(ctx->unpred.ensure(pl.total_codes));
         launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
                              ctx->unpred.p, st);
         HIPCHECK(hipGetLastError());
-        out->d_codes = ctx->codes.p;
-        out->n_codes = pl.total_codes;
-        out->d_unpred = ctx->unpred.p;
-        out->n_unpred = 0;
-        out->loss = 0;
-        memset(out->level_counts, 0, sizeof(out->level_counts));
-        out->levels = pl.levels;
-        return FAZ_OK;
+        if (enqueue) {
+            out->d_codes = ctx->codes.p;
+            out->n_codes = pl.total_codes;
+            out->d_unpred = ctx->unpred.p;
+            out->n_unpred = 0;
+            out->loss = 0;
+            memset(out->level_counts, 0, sizeof(out->level_counts));
+            out->levels = pl.levels;
+            return FAZ_OK;
+        }
+        HIPCHECK(hipEventRecord(ctx->ev[2], st));
+        if (profile) HIPCHECK(hipStreamSynchronize(st));
+    } else {
+        uint32_t errflag = 0;
+        HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(&errflag, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
+        if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
+        HIPCHECK(ctx->unpred.ensure(n_unpred));
+        launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
+                             ctx->unpred.p, st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ctx->ev[2], st));
+        HIPCHECK(hipStreamSynchronize(st));
+        ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
+        ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->timings[6] = ctx->timings[1];
     }
-    uint64_t n_unpred = 0;
-    uint32_t errflag = 0;
-    HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(&errflag, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
-    if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
-    HIPCHECK(ctx->unpred.ensure(n_unpred));
-    launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
-                         ctx->unpred.p, st);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ctx->ev[2], st));
-    HIPCHECK(hipStreamSynchronize(st));
-    ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
-    ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
-    ctx->timings[6] = ctx->timings[1];
-    if (ctx->profile) {
+    if (profile) {
         // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block level launches
         size_t j = 0;
         for (size_t i = 0; i < nlv; i++) {
@@ -844,29 +916,40 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         HIPCHECK(ctx->work_out.ensure(num));
         recon = ctx->work_out.p;
     }
+    // one wait after the histogram (counts, first positions, outlier count
+    // and error flag through pinned memory), one after the encode
     faz_interp_out io;
-    int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax);
+    int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax, false, true);
     if (rc) return rc;
-    HIPCHECK(hipEventRecord(ctx->ev[3], st));
-    HIPCHECK(hipMemcpyAsync(ctx->h_freq.data(), ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(ctx->h_first.data(), ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    uint32_t *hf = ctx->hp_hist.data(), *hs = ctx->hp_hist.data() + 2 * 65536;
+    HIPCHECK(hipMemcpyAsync(hf, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(spin_wait(st));
+    const uint32_t errflag = hs[2];
+    if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
+    if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
+    uint64_t n_unpred;
+    memcpy(&n_unpred, hs, 8);
     PendingRaw &pr = ctx->pending;
-    if (build_huffman(ctx->h_freq.data(), ctx->h_first.data(), 65536, pr.tab))
+    if (build_huffman(hf, hf + 65536, 65536, pr.tab))
         CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
-    uint64_t maxbits = 0;
-    for (auto l: pr.tab.len) maxbits = std::max<uint64_t>(maxbits, l);
-    size_t words_cap = (size_t) ((io.n_codes * maxbits + 31) / 32 + 2);
-    HIPCHECK(ctx->words.ensure(words_cap));
-    size_t nbytes = 0;
-    rc = faz_gpu_huffman_encode(ctx, io.d_codes, io.n_codes, &ht, reinterpret_cast<uint8_t *>(ctx->words.p),
-                                ctx->words.cap * 4, &nbytes);
+    rc = encode_enqueue(ctx, io.d_codes, io.n_codes, &ht);
     if (rc) return rc;
+    HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
-    HIPCHECK(hipStreamSynchronize(st));
-    ctx->timings[3] = elapsed(ctx->ev[3], ctx->ev[4]);
+    HIPCHECK(spin_wait(st));
+    uint64_t bits;
+    memcpy(&bits, hs + 4, 8);
+    const size_t nbytes = (size_t) ((bits + 7) / 8);
+    io.n_unpred = n_unpred;
+    ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
+    ctx->timings[6] = ctx->timings[1];
+    ctx->timings[3] = elapsed(ctx->ev[2], ctx->ev[4]);
     ctx->timings[4] = elapsed(ctx->ev[4], ctx->ev[5]);
     ctx->timings[5] = elapsed(ctx->ev[0], ctx->ev[5]);
     // engine buffer header (SIC:489-512)
@@ -1062,6 +1145,7 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     HIPCHECK(hipMemcpyAsync(ctx->codes.p, c16.data(), num * 2, hipMemcpyHostToDevice, st));
     if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
+    ctx->prefix_key = 0;  // the compression path's table is gone
     launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
