@@ -37,7 +37,10 @@ constexpr int kPlane = 1157;                 // LDS stride of axis 0 (33 * 35 + 
 constexpr int kTileFloats = 33 * kPlane;     // 38181 floats = 152724 bytes
 constexpr int kEdge = 33 * 32 + 33 * 33;     // l0 = 32 plane (l2 < 32) and l2 = 32 column
 constexpr int kEdgeSlots = (kEdge + kThreads - 1) / kThreads;
-constexpr int kLanesPerLine = 4;             // lanes sharing one line of 16 odd positions
+#ifndef FAZ_TILE_LANES
+#define FAZ_TILE_LANES 4
+#endif
+constexpr int kLanesPerLine = FAZ_TILE_LANES;  // lanes sharing one line of 16 odd positions
 constexpr int kPerLane = 16 / kLanesPerLine;  // odd positions per lane
 constexpr int kLinesPerRound = kThreads / kLanesPerLine;
 
