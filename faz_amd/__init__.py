@@ -247,7 +247,7 @@ EXPORTS = [
     "faz_gpu_batch_create", "faz_gpu_batch_destroy", "faz_gpu_batch_compress", "faz_gpu_batch_fetch_raw",
     "faz_gpu_batch_stats", "faz_sz_compress_batch_f32", "faz_gpu_batch_field",
     "faz_gpu_sampling_f32", "faz_gpu_block_interp_test_f32", "faz_gpu_quant_intervals_f32", "faz_gpu_sz3_tune_f32",
-    "faz_sz_compress_sz3_f32",
+    "faz_sz_compress_sz3_f32", "faz_gpu_huffman_decode",
 ]
 
 
@@ -317,6 +317,7 @@ def lib():
         L.faz_gpu_huffman_encode.argtypes = [vp, vp, sz, P(HuffTable), vp, sz, P(sz)]
         L.faz_huffman_build.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_int32), vp, vp, vp, sz, P(sz)]
         L.faz_gpu_interp_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
+        L.faz_gpu_huffman_decode.argtypes = [vp, vp, sz, sz, ctypes.c_uint32, vp, P(sz)]
         L.faz_gpu_interp_raw_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, P(sz)]
         L.faz_sz_compress_f32.argtypes = [vp, P(InterpParams), ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
                                           P(vp), P(sz)]
@@ -524,6 +525,21 @@ class Engine:
         self._check(self._L.faz_gpu_interp_raw_f32(self.h, self._dev(x), rp, ctypes.byref(params),
                                                    _np_ptr(buf), buf.size, ctypes.byref(need)), "interp_raw")
         return buf[:need.value].tobytes()
+
+    def huffman_decode(self, section, num, limit=0):
+        """HuffmanEncoder::load + decode (HuffmanEncoder.hpp:236-296) on the
+        device: `num` symbols of a saved Huffman section as an int32 tensor
+        on the context's GPU, and the bytes the section took.  limit > 0
+        rejects symbols outside [0, limit)."""
+        import torch
+        b = np.frombuffer(section, np.uint8).copy()
+        out = torch.empty(max(int(num), 1), dtype=torch.int32, device=f"cuda:{self.device}")
+        used = ctypes.c_size_t()
+        torch.cuda.synchronize()
+        self._check(self._L.faz_gpu_huffman_decode(self.h, _np_ptr(b), b.size, int(num), int(limit),
+                                                   ctypes.c_void_p(out.data_ptr()), ctypes.byref(used)),
+                    "huffman_decode")
+        return out[:int(num)], used.value
 
     def decompress_raw(self, raw, N, out):
         import torch

@@ -138,6 +138,12 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint8_t> gf_sel;
     faz::DevBuf<float> gf_fields, gf_recon;
     uint32_t gf_tab_key = 0;  // N << 16 | bs of the table
+    // Huffman decoder: the bitstream as words, the lookup table and tree,
+    // per-chunk start / exit / symbol offset and symbol counts + flags
+    faz::DevBuf<uint32_t> dec_words, dec_tab, dec_cnt;
+    faz::DevBuf<uint8_t> dec_leaf;
+    faz::DevBuf<uint64_t> dec_pos;
+    uint32_t dec_rounds = 0;  // synchronisation rounds of the last decode
 };
 
 #define CTX_FAIL(code, ...)                                          \

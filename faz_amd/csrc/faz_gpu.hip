@@ -1103,8 +1103,6 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     if ((size_t) (n - (pos - h_raw)) < U * sizeof(float)) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
     const float *unpred_h = reinterpret_cast<const float *>(pos);
     pos += U * sizeof(float);
-    std::vector<int32_t> codes;
-    if (!decode_huffman(pos, n - (pos - h_raw), codes, num)) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman section malformed");
     faz_interp_params p;
     memset(&p, 0, sizeof(p));
     p.N = N;
@@ -1123,15 +1121,6 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     Plan pl;
     int rc = make_plan(&p, pl, ctx->err);
     if (rc) return rc;
-    std::vector<uint16_t> c16(num);
-    uint64_t zeros = 0;
-    for (size_t i = 0; i < num; i++) {
-        if (codes[i] < 0 || codes[i] > 65535) CTX_FAIL(FAZ_ERR_FORMAT, "code out of 16-bit range");
-        c16[i] = (uint16_t) codes[i];
-        zeros += c16[i] == 0;
-    }
-    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
-                             (unsigned long long) U, (unsigned long long) zeros);
     const uint32_t nchunks = (uint32_t) ((num + kHistChunk - 1) / kHistChunk);
     HIPCHECK(ctx->codes.ensure(num));
     HIPCHECK(ctx->unpred.ensure(U));
@@ -1142,13 +1131,25 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     HIPCHECK(ctx->freq.ensure(65536));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->scalars.ensure(4));
-    HIPCHECK(hipMemcpyAsync(ctx->codes.p, c16.data(), num * 2, hipMemcpyHostToDevice, st));
+    // codes decoded on the device (HUF:236-268), then the outliers' ranks
+    int drc = gpu_huffman_decode<uint16_t>(ctx, pos, n - (pos - h_raw), num, 65536u, ctx->codes.p, nullptr);
+    if (drc) return drc;
     if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
     ctx->prefix_key = 0;  // the compression path's table is gone
     launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
+    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned: zero count, first code
+    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->codes.p, 2, hipMemcpyDeviceToHost, st));
+    HIPCHECK(spin_wait(st));
+    uint64_t zeros;
+    memcpy(&zeros, hs, 8);
+    uint16_t c0;
+    memcpy(&c0, hs + 2, 2);
+    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
+                             (unsigned long long) U, (unsigned long long) zeros);
     launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
     ctx->hist_codes = nullptr;
     // head: anchors / first point come straight from the unpredictable list
@@ -1158,7 +1159,7 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
         if (U < pl.anchor_count) CTX_FAIL(FAZ_ERR_FORMAT, "fewer unpredictable values than anchors");
         launch_anchor_scatter(ctx->unpred.p, d_out, N, pl.dims, pl.offs, (uint32_t) h.max_step, pl.anchor_count, st);
     } else {
-        float v0 = c16[0] ? (float) ((double) 0.0f + (double) (2 * ((int) c16[0] - radius)) * eb) : unpred_h[0];
+        float v0 = c0 ? (float) ((double) 0.0f + (double) (2 * ((int) c0 - radius)) * eb) : unpred_h[0];
         HIPCHECK(hipMemcpyAsync(d_out, &v0, 4, hipMemcpyHostToDevice, st));
         HIPCHECK(hipStreamSynchronize(st));
     }

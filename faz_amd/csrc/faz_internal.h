@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+struct faz_gpu_ctx;
+
 namespace faz {
 
 // Largest supported rank and level count.
@@ -128,6 +130,13 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab
 void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st);
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
                            const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st);
+
+// HuffmanEncoder::decode on the device (faz_decode.hip): num symbols into
+// d_out on ctx->stream, each symbol + the tree's offset checked to lie in
+// [0, limit); returns a faz_status with the stream drained.
+template<class OUT>
+int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
+                       size_t *used);
 
 // Chunk geometry of the histogram / outlier / encode kernels.
 constexpr uint32_t kHistThreads = 256;

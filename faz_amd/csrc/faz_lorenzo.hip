@@ -805,16 +805,6 @@ int lr_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, const size_t *
     const float *unpred_h = reinterpret_cast<const float *>(p);
     p += U * 4;
     const uint64_t num = (uint64_t) G.d1 * G.d2 * G.d3;
-    std::vector<int32_t> codes;
-    if (!decode_huffman(p, end - p, codes, num)) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman section malformed");
-    std::vector<uint16_t> c16(num);
-    uint64_t zeros = 0;
-    for (size_t i = 0; i < num; i++) {
-        if (codes[i] < 0 || codes[i] > 65535) CTX_FAIL(FAZ_ERR_FORMAT, "code out of 16-bit range");
-        c16[i] = (uint16_t) codes[i];
-        zeros += c16[i] == 0;
-    }
-    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count does not match the zero codes");
     std::vector<uint8_t> ind8(ind.begin(), ind.end());
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
@@ -831,12 +821,19 @@ int lr_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, const size_t *
     HIPCHECK(ctx->lr_ind.ensure(G.nblocks));
     HIPCHECK(ctx->lr_flag.ensure(G.nblocks));
     HIPCHECK(ctx->lr_rank.ensure(G.nblocks));
+    // codes decoded on the device (HUF:236-268), then the outliers' ranks
+    if ((rc = gpu_huffman_decode<uint16_t>(ctx, p, end - p, num, 65536u, ctx->codes.p, nullptr))) return rc;
     HIPCHECK(ctx->tiles.ensure(std::max(scan_tiles((uint32_t) G.nblocks), scan_tiles(nchunks))));
-    HIPCHECK(hipMemcpyAsync(ctx->codes.p, c16.data(), num * 2, hipMemcpyHostToDevice, st));
     if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->lr_ind.p, ind8.data(), G.nblocks, hipMemcpyHostToDevice, st));
     launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
+    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned: zero count
+    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(spin_wait(st));
+    uint64_t zeros;
+    memcpy(&zeros, hs, 8);
+    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count does not match the zero codes");
     launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
     ctx->hist_codes = nullptr;
     const LRQuant Q = lr_quant(eb, radius);

@@ -115,6 +115,18 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
 faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, int32_t *offset, int32_t *n_states,
                              uint64_t *code, uint8_t *len, uint8_t *tree, size_t tree_cap, size_t *tree_size);
 
+/* --- A15: HuffmanEncoder::load + decode (include/QoZ/encoder/
+ * HuffmanEncoder.hpp:236-296) on the device.  h_buf holds a saved tree
+ * (offset, node count, tree arrays) followed by the size_t byte count and the
+ * MSB-first bitstream (n bytes in all).  Writes num symbols (each symbol +
+ * the tree's offset, which must lie in [0, limit) as an unsigned value unless
+ * limit is 0) to d_out and the bytes of the section to *used.  A stream that
+ * runs out before num symbols, bits that leave the tree or a symbol out of
+ * range fail with FAZ_ERR_FORMAT (the reference reads past its buffer or
+ * prints "garbage input"). */
+faz_status faz_gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *h_buf, size_t n, size_t num, uint32_t limit,
+                                  int32_t *d_out, size_t *used);
+
 /* --- A17: HuffmanEncoder::decode + SZInterpolationCompressor::decompress
  * (HuffmanEncoder.hpp:236-296; SIC:42-199) on a pre-zstd engine buffer. */
 faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
