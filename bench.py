@@ -264,6 +264,33 @@ def batch_report(eng, count, seed, rel, steps=3):
         torch.cuda.empty_cache()
 
 
+def decompress_report(eng, raw, recon, steps=5):
+    """SZInterpolationCompressor::decompress of the step's engine buffer on
+    the GPU (Huffman decode, outlier scatter, level recovery) from the host
+    buffer, as SZ_decompress hands it over; the output must equal the
+    compression's reconstruction bit for bit."""
+    import torch
+    out = torch.empty_like(recon)
+    eng.decompress_raw(raw, 3, out)
+    torch.cuda.synchronize()
+    if not torch.equal(out.view(torch.int32), recon.view(torch.int32)):
+        raise RuntimeError("GPU decompression differs from the compression's reconstruction")
+    total = decode = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.decompress_raw(raw, 3, out)
+        t = eng.timings()
+        decode += t[6]
+        total += t[7]
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    n = recon.numel()
+    return {"ms": round(wall * 1e3, 4), "GB/s": round(4.0 * n / wall / 1e9, 2),
+            "huffman_decode_ms": round(decode / steps, 4), "call_ms": round(total / steps, 4),
+            "raw_bytes": len(raw), "equal_to_recon": True,
+            "input": "pre-zstd engine buffer in host memory (PCIe upload included)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +388,10 @@ def main():
                       "hist_outliers": round(timings[2], 4), "tree": round(timings[3], 4),
                       "encode": round(timings[4], 4), "device_total": round(timings[5], 4)},
     }
+    try:
+        line["decompress"] = decompress_report(eng, raw, recon)
+    except Exception as e:  # reported beside the measurement
+        line["decompress"] = {"error": str(e)[:200]}
     if not args.no_tuner:
         try:
             line["tuner"] = tuner_report(eng, x, dims, args.rel, mn, mx)

@@ -526,14 +526,16 @@ class Engine:
                                                    _np_ptr(buf), buf.size, ctypes.byref(need)), "interp_raw")
         return buf[:need.value].tobytes()
 
-    def huffman_decode(self, section, num, limit=0):
+    def huffman_decode(self, section, num, limit=0, out=None):
         """HuffmanEncoder::load + decode (HuffmanEncoder.hpp:236-296) on the
         device: `num` symbols of a saved Huffman section as an int32 tensor
-        on the context's GPU, and the bytes the section took.  limit > 0
-        rejects symbols outside [0, limit)."""
+        on the context's GPU (or into `out`), and the bytes the section took.
+        limit > 0 rejects symbols outside [0, limit)."""
         import torch
-        b = np.frombuffer(section, np.uint8).copy()
-        out = torch.empty(max(int(num), 1), dtype=torch.int32, device=f"cuda:{self.device}")
+        b = np.frombuffer(section, np.uint8)  # read only by the library
+        if out is None:
+            out = torch.empty(max(int(num), 1), dtype=torch.int32, device=f"cuda:{self.device}")
+        assert out.dtype == torch.int32 and out.is_contiguous() and out.numel() >= int(num)
         used = ctypes.c_size_t()
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_huffman_decode(self.h, _np_ptr(b), b.size, int(num), int(limit),
@@ -543,7 +545,7 @@ class Engine:
 
     def decompress_raw(self, raw, N, out):
         import torch
-        b = np.frombuffer(raw, np.uint8).copy()
+        b = np.frombuffer(raw, np.uint8)  # read only by the library
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_interp_decompress_raw(self.h, _np_ptr(b), b.size, int(N), self._dev(out),
                                                           out.numel()), "decompress_raw")
@@ -635,7 +637,7 @@ class Engine:
 
     def lorenzo_decompress_raw(self, raw, dims, out):
         import torch
-        b = np.frombuffer(raw, np.uint8).copy()
+        b = np.frombuffer(raw, np.uint8)  # read only by the library
         d = np.array(dims, np.uint64)
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_lorenzo_decompress_raw(self.h, _np_ptr(b), b.size, _np_ptr(d), self._dev(out)),
@@ -693,7 +695,7 @@ class Engine:
 
     def general_decompress_raw(self, raw, dims, out, lorenzo=True, lorenzo2=True):
         import torch
-        b = np.frombuffer(raw, np.uint8).copy()
+        b = np.frombuffer(raw, np.uint8)  # read only by the library
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_general_decompress_raw(self.h, _np_ptr(b), b.size, len(dims), int(bool(lorenzo)),
                                                            int(bool(lorenzo2)), self._dev(out), int(np.prod(dims))),

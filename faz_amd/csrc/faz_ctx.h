@@ -116,6 +116,15 @@ struct faz_gpu_ctx {
         uint64_t launches = 0, points = 0;
     } prof[3];
     std::vector<hipEvent_t> prof_ev;
+    // launches of the last compression whose event times are still to be read:
+    // bucket, event pair index, points, algorithmic bytes
+    struct ProfRec {
+        int bucket;
+        uint32_t ev;
+        uint64_t points;
+        double bytes;
+    };
+    std::vector<ProfRec> prof_pending;
     // 3-D fields: one fused launch per level (0 = per-pass launches)
     bool fused3d = true;
     // levels that qualify run on the whole-tile kernel (persistent, tile_grid
@@ -139,11 +148,13 @@ struct faz_gpu_ctx {
     faz::DevBuf<float> gf_fields, gf_recon;
     uint32_t gf_tab_key = 0;  // N << 16 | bs of the table
     // Huffman decoder: the bitstream as words, the lookup table and tree,
-    // per-chunk start / exit / symbol offset and symbol counts + flags
+    // per-chunk start and symbol offset, symbol counts + error flag, the
+    // per-chunk entry maps (two buffers) and counts per entry
     faz::DevBuf<uint32_t> dec_words, dec_tab, dec_cnt;
-    faz::DevBuf<uint8_t> dec_leaf;
+    faz::DevBuf<uint8_t> dec_leaf, dec_x;
+    faz::DevBuf<uint16_t> dec_n;
     faz::DevBuf<uint64_t> dec_pos;
-    uint32_t dec_rounds = 0;  // synchronisation rounds of the last decode
+    uint32_t dec_rounds = 0;  // composition steps of the last decode
 };
 
 #define CTX_FAIL(code, ...)                                          \

@@ -2,16 +2,18 @@
 // encoder/HuffmanEncoder.hpp:236-296) for the bitstreams the reference
 // writes (one MSB-first stream, no chunk index).
 //
-// The stream is cut into chunks of kChunkBits bits.  A thread decodes one
-// chunk from a start position until it passes the chunk's end; its exit
-// position is where the next chunk's first codeword begins if its own start
-// was right.  Chunk 0 starts at bit 0; every other chunk first starts at its
-// boundary, which is usually inside a codeword, but a Huffman decoder
-// resynchronises within a few codewords, so after one correction round the
-// exits almost always agree.  Rounds repeat until no chunk's start moves
-// (chunk i is exact once chunk i - 1 is, so at most one round per chunk).
-// Then the symbol counts are scanned and every chunk decodes again, writing
-// its symbols at its offset.
+// The stream is cut into chunks of kChunkBits bits, one thread each.  The
+// first codeword of chunk c starts at an entry offset e_c < L (L = the longest
+// codeword) past the chunk's boundary, and e_{c+1} is a function x_c of e_c.
+// Pass 1 tabulates x_c and the chunk's symbol count for every entry offset:
+// it decodes from offset 0, and every other offset decodes only until it
+// lands on a codeword start of an earlier trajectory (a Huffman decoder falls
+// into step within a few codewords, except inside long runs of one codeword,
+// where two or three phases persist and are decoded in full).  Pass 2 composes
+// the maps with a parallel prefix (Hillis-Steele, log2(chunks) steps) to get
+// every e_c from e_0 = 0; the counts at the true entries are scanned into
+// output offsets, and pass 3 decodes each chunk again, writing its symbols.
+// Every pass stages its workgroup's slice of the stream in LDS.
 //
 // Codewords of up to kTabBits bits come from one table lookup (LDS); longer
 // ones continue bit by bit down the tree from the node the table names.
@@ -22,32 +24,61 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <functional>
+#include <type_traits>
 #include <vector>
 
 namespace faz {
 namespace {
 
 constexpr int kTabBits = 12;
-constexpr uint32_t kChunkBits = 4096;
-// Table entry: a codeword of at most kTabBits bits -> leaf node (bits 0..23)
-// | length << 24; a longer one -> kLongFlag | node reached after kTabBits
-// bits; kOffTree: the prefix leaves the tree (malformed tree).
+constexpr uint32_t kChunkBits = 1024;
+constexpr uint32_t kDecThreads = 256;
+// stream words a workgroup stages: its chunks, then a margin for the last
+// codeword (at most 64 bits) and the 64-bit window read past it
+constexpr uint32_t kGroupWords = kDecThreads * kChunkBits / 32;
+constexpr uint32_t kStageWords = kGroupWords + 5;
+// one padding word per 32 keeps lanes that read the same word of their own
+// chunk on different banks
+constexpr uint32_t kStageSlots = kStageWords + kStageWords / 32 + 1;
+// codeword starts of a trajectory are kept for its first kMaskBits bits
+constexpr uint32_t kMaskBits = 128;
+// Lookup entries.  Primary table (kTabBits bits, LDS) and second-level
+// tables (kSubBits bits each, global memory, one per tree node reached after
+// kTabBits + j * kSubBits bits):
+//   leaf node (bits 0..23) | codeword bits still to consume << 24  (bit 31 clear)
+//   kLongFlag | index of the next second-level table
+//   kLongFlag | kWalkFlag | tree node: continue bit by bit (tables capped)
+//   kOffTree: the bits leave the tree (malformed tree)
+constexpr int kSubBits = 8;
 constexpr uint32_t kLongFlag = 0x80000000u;
+constexpr uint32_t kWalkFlag = 0x40000000u;
 constexpr uint32_t kOffTree = 0xFFFFFFFFu;
+// second-level entries allowed before deeper nodes fall back to the walk
+constexpr size_t kMaxSubEntries = (size_t) 1 << 22;
 
 struct Tree {
     const int32_t *left, *right, *sym;
     const uint8_t *leaf;
+    const uint32_t *sub;  // second-level tables
 };
 
+__device__ __forceinline__ uint32_t slot(uint32_t i) { return i + (i >> 5); }
+
 struct BitReader {
+    const uint32_t *lw;     // staged words (byte order swapped), see slot()
+    uint64_t base;          // stream word index of lw's first word
     const uint32_t *words;  // the stream as 32-bit words (bytes in stream order)
     uint64_t nwords;
     uint64_t buf;           // next bits, MSB first
     int have;               // valid bits in buf
     uint64_t next;          // next word to load
 
-    __device__ uint32_t word(uint64_t i) const { return i < nwords ? __builtin_bswap32(words[i]) : 0u; }
+    __device__ uint32_t word(uint64_t i) const {
+        const uint64_t li = i - base;
+        if (li < kStageWords) return lw[slot((uint32_t) li)];
+        return i < nwords ? __builtin_bswap32(words[i]) : 0u;
+    }
     __device__ void start(uint64_t pos) {
         next = pos / 32;
         const uint32_t skip = (uint32_t) (pos % 32);
@@ -73,23 +104,33 @@ struct BitReader {
 // tree.  Always advances pos.
 __device__ __forceinline__ int32_t decode_one(BitReader &r, const uint32_t *tab, const Tree &T, uint64_t &pos) {
     r.refill();
-    const uint32_t e = tab[r.peek(kTabBits)];
-    if (e == kOffTree) {
-        r.drop(1);
-        pos++;
-        return -1;
+    uint32_t e = tab[r.peek(kTabBits)];
+    int bits = kTabBits;
+    int depth = 0;
+    for (;;) {
+        if (e == kOffTree) {
+            r.drop(1);
+            pos++;
+            return -1;
+        }
+        if (!(e & kLongFlag)) {
+            const int len = (int) (e >> 24);
+            r.drop(len);
+            pos += (uint64_t) len;
+            return (int32_t) (e & 0xFFFFFFu);
+        }
+        r.drop(bits);
+        pos += (uint64_t) bits;
+        depth += bits;
+        if (e & kWalkFlag) break;
+        // codewords are at most 64 bits (the reference keeps them in 64-bit words)
+        if (depth >= 64) return -1;
+        r.refill();
+        e = T.sub[(e & 0x3FFFFFFFu) + r.peek(kSubBits)];
+        bits = kSubBits;
     }
-    if (!(e & kLongFlag)) {
-        const int len = (int) (e >> 24);
-        r.drop(len);
-        pos += (uint64_t) len;
-        return (int32_t) (e & 0xFFFFFFu);
-    }
-    r.drop(kTabBits);
-    pos += kTabBits;
-    int32_t nd = (int32_t) (e & 0x7FFFFFFFu);
-    // codewords are at most 64 bits (the reference keeps them in 64-bit words)
-    for (int depth = kTabBits; depth < 64; depth++) {
+    int32_t nd = (int32_t) (e & 0x3FFFFFFFu);
+    for (; depth < 64; depth++) {
         r.refill();
         const uint32_t bit = r.peek(1);
         r.drop(1);
@@ -101,83 +142,202 @@ __device__ __forceinline__ int32_t decode_one(BitReader &r, const uint32_t *tab,
     return -1;
 }
 
-__device__ __forceinline__ void load_table(uint32_t *tab, const uint32_t *__restrict__ gtab) {
-    for (int i = threadIdx.x; i < (1 << kTabBits); i += blockDim.x) tab[i] = gtab[i];
+// Loads the lookup table and the workgroup's slice of the stream into LDS.
+__device__ __forceinline__ void stage(uint32_t *tab, uint32_t *lw, const uint32_t *__restrict__ gtab,
+                                      const uint32_t *__restrict__ words, uint64_t nwords, uint64_t base) {
+    for (uint32_t i = threadIdx.x; i < (1u << kTabBits); i += blockDim.x) tab[i] = gtab[i];
+    for (uint32_t i = threadIdx.x; i < kStageWords; i += blockDim.x) {
+        const uint64_t g = base + i;
+        lw[slot(i)] = g < nwords ? __builtin_bswap32(words[g]) : 0u;
+    }
     __syncthreads();
 }
 
-// Round 0: every chunk from its boundary.  Later rounds: chunk c restarts
-// from chunk c - 1's exit where that differs from its start (changed[0] = 1).
-// Records exit[c] and the symbols that end inside the stream.  Bits that
-// leave the tree (only a mis-synchronised start can meet them in a well-formed
-// stream) restart at the root; the write pass reports them.
-__global__ void __launch_bounds__(256) huff_count_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
-                                                         uint64_t nbits, uint32_t nchunks,
-                                                         const uint32_t *__restrict__ gtab, Tree T,
-                                                         uint64_t *__restrict__ start, uint64_t *__restrict__ exitp,
-                                                         uint32_t *__restrict__ count, uint32_t *__restrict__ changed,
-                                                         int round) {
-    __shared__ uint32_t tab[1 << kTabBits];
-    load_table(tab, gtab);
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchunks) return;
-    uint64_t s;
-    if (round == 0) {
-        s = (uint64_t) c * kChunkBits;
-    } else {
-        if (c == 0) return;
-        s = exitp[c - 1];
-        if (s == start[c]) return;
-        atomicOr(changed, 1u);
-    }
-    const uint64_t end = min((uint64_t) (c + 1) * kChunkBits, nbits);
-    uint64_t pos = s;
-    uint32_t n = 0;
+// A decoding position with the symbols counted so far: those that end inside
+// the stream (a codeword cut by the stream's end is not a symbol).
+struct Cursor {
     BitReader r;
-    r.words = words, r.nwords = nwords;
-    r.start(pos);
-    while (pos < end) {
-        const int32_t k = decode_one(r, tab, T, pos);
-        if (k >= 0 && pos <= nbits) n++;
+    uint64_t pos;
+    uint32_t k;
+    __device__ void init(const BitReader &proto, uint64_t p) {
+        r = proto;
+        r.start(p);
+        pos = p;
+        k = 0;
     }
-    start[c] = s;
-    exitp[c] = pos;
-    count[c] = n;
+    __device__ void step(const uint32_t *tab, const Tree &T, uint64_t nbits) {
+        const int32_t nd = decode_one(r, tab, T, pos);
+        if (nd >= 0 && pos <= nbits) k++;
+    }
+};
+
+__device__ __forceinline__ bool mask_has(const uint64_t *m, uint32_t q) { return (m[q >> 6] >> (q & 63)) & 1u; }
+__device__ __forceinline__ uint32_t mask_below(const uint64_t *m, uint32_t q) {
+    if (q < 64) return (uint32_t) __popcll(m[0] & ((1ull << q) - 1));
+    if (q >= 128) return (uint32_t) (__popcll(m[0]) + __popcll(m[1]));
+    return (uint32_t) (__popcll(m[0]) + __popcll(m[1] & ((1ull << (q - 64)) - 1)));
 }
 
-// Chunk c writes its symbols from offset[c] on (those below num), each
-// sym + sym_offset, which must lie in [0, limit) unless limit is 0; err[0] = 1
-// otherwise or when the bits leave the tree.
-template<class OUT>
-__global__ void __launch_bounds__(256) huff_write_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
-                                                         uint64_t nbits, uint32_t nchunks,
-                                                         const uint32_t *__restrict__ gtab, Tree T,
-                                                         const uint64_t *__restrict__ start,
-                                                         const uint64_t *__restrict__ offset, int32_t sym_offset,
-                                                         uint32_t limit, uint64_t num, OUT *__restrict__ out,
-                                                         uint32_t *__restrict__ err) {
+// Pass 1: for chunk c and every entry offset o < K, the next chunk's entry
+// offset (xmap, clamped to K - 1) and the symbols that start in the chunk
+// (nmap).  A trajectory merges with an earlier one when it reaches one of its
+// codeword starts; the starts of the first two trajectories are kept as bit
+// masks over the chunk's first kMaskBits bits.
+__global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
+                                                               uint64_t nbits, uint32_t nchunks,
+                                                               const uint32_t *__restrict__ gtab, Tree T, uint32_t K,
+                                                               uint8_t *__restrict__ xmap,
+                                                               uint16_t *__restrict__ nmap) {
     __shared__ uint32_t tab[1 << kTabBits];
-    load_table(tab, gtab);
+    __shared__ uint32_t lw[kStageSlots];
+    const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
+    stage(tab, lw, gtab, words, nwords, base);
+    const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
+    if (c >= nchunks) return;
+    BitReader proto;
+    proto.lw = lw, proto.base = base, proto.words = words, proto.nwords = nwords;
+    const uint64_t b = (uint64_t) c * kChunkBits;
+    const uint64_t end = min(b + kChunkBits, nbits);
+    const uint64_t nxt = b + kChunkBits;
+    // masks are exact only where every symbol before kMaskBits ends inside the stream
+    const bool masks = b + kMaskBits <= nbits;
+    auto put = [&](uint32_t o, uint64_t x, uint32_t n) {
+        const uint64_t e = x > nxt ? x - nxt : 0;
+        xmap[(uint64_t) c * K + o] = (uint8_t) min<uint64_t>(e, K - 1);
+        nmap[(uint64_t) c * K + o] = (uint16_t) n;
+    };
+    uint64_t mA[2] = {0, 0}, mB[2] = {0, 0};
+    uint64_t xA, xB = 0;
+    uint32_t nA, nB = 0;
+    bool hasB = false;
+    {
+        Cursor A;
+        A.init(proto, b);
+        while (A.pos < end) {
+            const uint64_t q = A.pos - b;
+            if (q < kMaskBits) mA[q >> 6] |= 1ull << (q & 63);
+            A.step(tab, T, nbits);
+        }
+        xA = A.pos, nA = A.k;
+    }
+    put(0, xA, nA);
+    for (uint32_t o = 1; o < K; o++) {
+        if (masks && mask_has(mA, o)) {
+            put(o, xA, nA - mask_below(mA, o));
+            continue;
+        }
+        if (masks && hasB && mask_has(mB, o)) {
+            put(o, xB, nB - mask_below(mB, o));
+            continue;
+        }
+        Cursor O;
+        O.init(proto, b + o);
+        uint64_t mO[2] = {0, 0};
+        bool merged = false;
+        while (O.pos < end) {
+            const uint64_t q = O.pos - b;
+            if (masks && q < kMaskBits) {
+                if (mask_has(mA, (uint32_t) q)) {
+                    put(o, xA, O.k + nA - mask_below(mA, (uint32_t) q));
+                    merged = true;
+                    break;
+                }
+                if (hasB && mask_has(mB, (uint32_t) q)) {
+                    put(o, xB, O.k + nB - mask_below(mB, (uint32_t) q));
+                    merged = true;
+                    break;
+                }
+                mO[q >> 6] |= 1ull << (q & 63);
+            }
+            O.step(tab, T, nbits);
+        }
+        if (merged) continue;
+        put(o, O.pos, O.k);
+        if (!hasB) {
+            hasB = true;
+            xB = O.pos, nB = O.k;
+            mB[0] = mO[0], mB[1] = mO[1];
+        }
+    }
+}
+
+// Pass 2, one Hillis-Steele step: out_c = in_c o in_{c-d} for c >= d.
+__global__ void huff_compose_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, uint64_t total,
+                                    uint32_t K, uint32_t d) {
+    const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint64_t c = i / K, o = i % K;
+    uint8_t v = in[i];
+    if (c >= d) v = in[c * K + in[(c - d) * K + o]];
+    out[i] = v;
+}
+
+// Entry of every chunk (chunk c's is the composed map of chunk c - 1 at 0)
+// and its symbol count at that entry.
+__global__ void huff_entry_kernel(const uint8_t *__restrict__ comp, const uint16_t *__restrict__ nmap,
+                                  uint32_t nchunks, uint32_t K, uint64_t *__restrict__ start,
+                                  uint32_t *__restrict__ count) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchunks) return;
+    const uint32_t e = c ? comp[(uint64_t) (c - 1) * K] : 0u;
+    start[c] = (uint64_t) c * kChunkBits + e;
+    count[c] = nmap[(uint64_t) c * K + e];
+}
+
+// Pass 3: chunk c writes its symbols from offset[c] on (those below num),
+// each sym + sym_offset, which must lie in [0, limit) unless limit is 0;
+// err[0] = 1 otherwise or when the bits leave the tree.
+template<class OUT>
+__global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
+                                                                 uint64_t nbits, uint32_t nchunks,
+                                                                 const uint32_t *__restrict__ gtab, Tree T,
+                                                                 const uint64_t *__restrict__ start,
+                                                                 const uint64_t *__restrict__ offset,
+                                                                 int32_t sym_offset, uint32_t limit, uint64_t num,
+                                                                 OUT *__restrict__ out, uint32_t *__restrict__ err) {
+    __shared__ uint32_t tab[1 << kTabBits];
+    __shared__ uint32_t lw[kStageSlots];
+    const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
+    stage(tab, lw, gtab, words, nwords, base);
+    const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
     if (c >= nchunks) return;
     const uint64_t end = min((uint64_t) (c + 1) * kChunkBits, nbits);
     uint64_t pos = start[c], o = offset[c];
     BitReader r;
-    r.words = words, r.nwords = nwords;
+    r.lw = lw, r.base = base, r.words = words, r.nwords = nwords;
     r.start(pos);
+    // symbols leave in 8-byte stores once out + o is 8-byte aligned
+    constexpr int P = 8 / sizeof(OUT);
+    constexpr int W = 8 * sizeof(OUT);
+    uint64_t acc = 0;
+    int na = 0;
+    bool bad = false;
     while (pos < end && o < num) {
         const int32_t k = decode_one(r, tab, T, pos);
         if (k < 0) {
-            atomicOr(err, 1u);
-            return;
+            bad = true;
+            break;
         }
         const uint32_t v = (uint32_t) (T.sym[k] + sym_offset);
         if (limit && v >= limit) {
-            atomicOr(err, 1u);
-            return;
+            bad = true;
+            break;
         }
-        out[o++] = (OUT) v;
+        if (na == 0 && (reinterpret_cast<uintptr_t>(out + o) & 7u) != 0) {
+            out[o++] = (OUT) v;
+            continue;
+        }
+        acc |= (uint64_t) (std::make_unsigned_t<OUT>) (OUT) v << (W * na);
+        na++;
+        o++;
+        if (na == P) {
+            *reinterpret_cast<uint64_t *>(out + (o - P)) = acc;
+            acc = 0;
+            na = 0;
+        }
     }
+    for (int j = 0; j < na; j++) out[o - na + j] = (OUT) (acc >> (W * j));
+    if (bad) atomicOr(err, 1u);
 }
 
 template<class OUT>
@@ -259,35 +419,71 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
         HIPCHECK(spin_wait(st));
         return FAZ_OK;
     }
-    // primary table: every kTabBits-bit prefix walked down from the root
+    // primary table: every kTabBits-bit prefix walked down from the root;
+    // second-level tables for the nodes below it
     const size_t nc = sym.size();
+    std::vector<uint32_t> sub;
+    // entry for the next `bits` bits (value v, MSB first) from node `from`
+    auto entry = [&](int32_t from, uint32_t v, int bits, int32_t &stop) -> uint32_t {
+        int32_t nd = from;
+        for (int d = 0; d < bits; d++) {
+            nd = ((v >> (bits - 1 - d)) & 1u) ? right[nd] : left[nd];
+            if (nd < 0) return kOffTree;
+            if (leaf[nd]) return (uint32_t) nd | (uint32_t) (d + 1) << 24;
+        }
+        stop = nd;  // an inner node after all `bits` bits
+        return kLongFlag;
+    };
+    // builds the table of node nd (depth `depth`) and those below it
+    std::function<uint32_t(int32_t, int)> build_sub = [&](int32_t nd, int depth) -> uint32_t {
+        if (depth >= 64 || sub.size() + ((size_t) 1 << kSubBits) > kMaxSubEntries)
+            return kLongFlag | kWalkFlag | (uint32_t) nd;
+        const size_t at = sub.size();
+        sub.resize(at + ((size_t) 1 << kSubBits));
+        for (uint32_t v = 0; v < (1u << kSubBits); v++) {
+            int32_t stop = -1;
+            uint32_t e = entry(nd, v, kSubBits, stop);
+            if (e == kLongFlag) e = build_sub(stop, depth + kSubBits);
+            sub[at + v] = e;
+        }
+        return kLongFlag | (uint32_t) at;
+    };
     std::vector<uint32_t> tab((size_t) 1 << kTabBits);
     for (uint32_t v = 0; v < tab.size(); v++) {
-        int32_t nd = 0;
-        uint32_t e = 0;
-        bool done = false;
-        for (int d = 0; d < kTabBits && !done; d++) {
-            const uint32_t bit = (v >> (kTabBits - 1 - d)) & 1u;
-            nd = bit ? right[nd] : left[nd];
-            if (nd < 0) {
-                e = kOffTree;
-                done = true;
-            } else if (leaf[nd]) {
-                e = (uint32_t) nd | (uint32_t) (d + 1) << 24;
-                done = true;
-            }
-        }
-        tab[v] = done ? e : (kLongFlag | (uint32_t) nd);
+        int32_t stop = -1;
+        uint32_t e = entry(0, v, kTabBits, stop);
+        if (e == kLongFlag) e = build_sub(stop, kTabBits);
+        tab[v] = e;
     }
+    // longest codeword: the entry offsets of a chunk lie below it
+    uint32_t maxlen = 0;
+    {
+        std::vector<std::pair<int32_t, uint32_t>> stack{{0, 0u}};
+        while (!stack.empty()) {
+            const auto [nd, d] = stack.back();
+            stack.pop_back();
+            if (d > 64) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword longer than 64 bits");
+            if (leaf[nd]) {
+                maxlen = std::max(maxlen, d);
+                continue;
+            }
+            if (left[nd] >= 0) stack.push_back({left[nd], d + 1});
+            if (right[nd] >= 0) stack.push_back({right[nd], d + 1});
+        }
+    }
+    const uint32_t K = std::max<uint32_t>(maxlen, 1);
     if (enc > (uint64_t) UINT32_MAX * (kChunkBits / 8)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman stream too long");
     const uint64_t nbits = enc * 8;
     const uint64_t nwords = (enc + 3) / 4;
     const uint32_t nchunks = (uint32_t) std::max<uint64_t>(1, (nbits + kChunkBits - 1) / kChunkBits);
+    const uint64_t nmaps = (uint64_t) nchunks * K;
     HIPCHECK(ctx->dec_words.ensure(nwords + 2));
-    HIPCHECK(ctx->dec_tab.ensure(tab.size() + 3 * nc));
+    HIPCHECK(ctx->dec_tab.ensure(tab.size() + 3 * nc + sub.size()));
     HIPCHECK(ctx->dec_leaf.ensure(nc));
-    HIPCHECK(ctx->dec_pos.ensure(3 * (size_t) nchunks));
+    HIPCHECK(ctx->dec_pos.ensure(2 * (size_t) nchunks));
     HIPCHECK(ctx->dec_cnt.ensure((size_t) nchunks + 2));
+    HIPCHECK(ctx->dec_x.ensure(2 * nmaps));
+    HIPCHECK(ctx->dec_n.ensure(nmaps));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->scalars.ensure(4));
     // the stream, its last word zero-padded
@@ -300,37 +496,44 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     HIPCHECK(hipMemcpyAsync(d_tree + nc, right.data(), nc * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(d_tree + 2 * nc, sym.data(), nc * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->dec_leaf.p, leaf.data(), nc, hipMemcpyHostToDevice, st));
-    const Tree T{d_tree, d_tree + nc, d_tree + 2 * nc, ctx->dec_leaf.p};
-    uint64_t *d_start = ctx->dec_pos.p, *d_exit = d_start + nchunks, *d_off = d_start + 2 * (size_t) nchunks;
-    uint32_t *d_cnt = ctx->dec_cnt.p, *d_flags = ctx->dec_cnt.p + nchunks;  // [0] changed, [1] error
-    const dim3 grid((nchunks + 255) / 256), block(256);
-    uint32_t *hf = ctx->hp_hist.data() + 2 * 65536;  // pinned scalars
-    uint32_t round = 0;
-    for (;; round++) {
-        launch_fill_u32(d_flags, 2, 0u, st);
-        hipLaunchKernelGGL(huff_count_kernel, grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab, T,
-                           d_start, d_exit, d_cnt, d_flags, (int) round);
-        HIPCHECK(hipGetLastError());
-        if (round == 0) continue;
-        HIPCHECK(hipMemcpyAsync(hf, d_flags, 4, hipMemcpyDeviceToHost, st));
-        HIPCHECK(spin_wait(st));
-        if (!hf[0]) break;
-        if (round > nchunks) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman chunk starts did not settle");
-    }
-    ctx->dec_rounds = round;
-    // symbol offsets of the chunks (exclusive scan), the total, then the write
-    launch_zero_scan(d_cnt, d_off, nchunks, ctx->tiles.p, ctx->scalars.p, st);
-    hipLaunchKernelGGL((huff_write_kernel<OUT>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab,
-                       T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_flags + 1);
+    uint32_t *d_sub = ctx->dec_tab.p + tab.size() + 3 * nc;
+    if (!sub.empty()) HIPCHECK(hipMemcpyAsync(d_sub, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, st));
+    const Tree T{d_tree, d_tree + nc, d_tree + 2 * nc, ctx->dec_leaf.p, d_sub};
+    uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nchunks;
+    uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nchunks;
+    const dim3 grid((nchunks + kDecThreads - 1) / kDecThreads), block(kDecThreads);
+    // pass 1: entry -> exit maps and counts
+    hipLaunchKernelGGL(huff_map_kernel, grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab, T, K,
+                       ctx->dec_x.p, ctx->dec_n.p);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(hf + 2, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hf + 4, d_flags + 1, 4, hipMemcpyDeviceToHost, st));
+    // pass 2: prefix composition, then every chunk's entry and count
+    uint8_t *cur = ctx->dec_x.p, *alt = ctx->dec_x.p + nmaps;
+    const dim3 cgrid((unsigned) ((nmaps + 255) / 256));
+    uint32_t steps = 0;
+    for (uint32_t d = 1; d < nchunks; d *= 2, steps++) {
+        hipLaunchKernelGGL(huff_compose_kernel, cgrid, dim3(256), 0, st, cur, alt, nmaps, K, d);
+        std::swap(cur, alt);
+        if (d > nchunks / 2) break;
+    }
+    ctx->dec_rounds = steps;
+    hipLaunchKernelGGL(huff_entry_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, cur, ctx->dec_n.p, nchunks,
+                       K, d_start, d_cnt);
+    HIPCHECK(hipGetLastError());
+    // output offsets of the chunks (exclusive scan) and the total, then pass 3
+    launch_zero_scan(d_cnt, d_off, nchunks, ctx->tiles.p, ctx->scalars.p, st);
+    launch_fill_u32(d_err, 1, 0u, st);
+    hipLaunchKernelGGL((huff_write_kernel<OUT>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab,
+                       T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err);
+    HIPCHECK(hipGetLastError());
+    uint32_t *hf = ctx->hp_hist.data() + 2 * 65536;  // pinned scalars
+    HIPCHECK(hipMemcpyAsync(hf, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hf + 2, d_err, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(spin_wait(st));
     uint64_t total;
-    memcpy(&total, hf + 2, 8);
+    memcpy(&total, hf, 8);
     if (total < num)
         CTX_FAIL(FAZ_ERR_FORMAT, "Huffman bitstream holds %llu symbols, %zu expected", (unsigned long long) total, num);
-    if (hf[4]) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman bitstream leaves the tree or a symbol is out of range");
+    if (hf[2]) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman bitstream leaves the tree or a symbol is out of range");
     return FAZ_OK;
 }
 

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -348,6 +349,19 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Adds the recorded launches of the last compression to the profile buckets
+// (their events must have completed).
+void flush_profile(faz_gpu_ctx *ctx) {
+    for (const auto &r: ctx->prof_pending) {
+        auto &b = ctx->prof[r.bucket];
+        b.ms += elapsed(ctx->prof_ev[2 * r.ev], ctx->prof_ev[2 * r.ev + 1]);
+        b.launches++;
+        b.points += r.points;
+        b.bytes += r.bytes;
+    }
+    ctx->prof_pending.clear();
+}
+
 }  // namespace
 
 extern "C" {
@@ -668,7 +682,6 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             return FAZ_OK;
         }
         HIPCHECK(hipEventRecord(ctx->ev[2], st));
-        if (profile) HIPCHECK(hipStreamSynchronize(st));
     } else {
         uint32_t errflag = 0;
         HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
@@ -687,15 +700,14 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         ctx->timings[6] = ctx->timings[1];
     }
     if (profile) {
-        // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block level launches
+        // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block
+        // level launches; the event times are read once the stream has drained
+        ctx->prof_pending.clear();
         size_t j = 0;
         for (size_t i = 0; i < nlv; i++) {
             const LevelPlan &L = pl.lv[i];
             if (kind[i]) {
-                auto &b = ctx->prof[kind[i] == 2 ? 0 : 2];
-                b.ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
-                b.launches++;
-                b.points += L.codes;
+                faz_gpu_ctx::ProfRec rec{kind[i] == 2 ? 0 : 2, (uint32_t) j, L.codes, 0.0};
                 // read each original and write its code; the reconstruction
                 // traffic is the coarser lattice read and this level's
                 // lattice written, where that happens
@@ -705,20 +717,18 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
                     coarse *= (pl.dims[a] - 1) / (2 * g3[i].s) + 1;
                 }
                 const bool writes = kind[i] == 1 || g3[i].dst != nullptr;
-                b.bytes += 6.0 * L.codes + 4.0 * coarse + (writes ? 4.0 * lat : 0.0);
+                rec.bytes = 6.0 * L.codes + 4.0 * coarse + (writes ? 4.0 * lat : 0.0);
+                ctx->prof_pending.push_back(rec);
                 j++;
                 continue;
             }
             for (auto &d: L.passes) {
-                if (d.total) {
-                    ctx->prof[1].ms += elapsed(ctx->prof_ev[2 * j], ctx->prof_ev[2 * j + 1]);
-                    ctx->prof[1].launches++;
-                    ctx->prof[1].points += d.total;
-                    ctx->prof[1].bytes += 10.0 * d.total;  // original in, reconstruction and code out
-                }
+                // original in, reconstruction and code out
+                if (d.total) ctx->prof_pending.push_back({1, (uint32_t) j, d.total, 10.0 * d.total});
                 j++;
             }
         }
+        if (!defer) flush_profile(ctx);  // the stream was drained above
     }
     ctx->hist_codes = ctx->codes.p;
     ctx->hist_n = pl.total_codes;
@@ -942,6 +952,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
     HIPCHECK(spin_wait(st));
+    flush_profile(ctx);
     uint64_t bits;
     memcpy(&bits, hs + 4, 8);
     const size_t nbytes = (size_t) ((bits + 7) / 8);
@@ -1034,8 +1045,6 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         int rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx);
         if (rc) return rc;
         HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
-        HIPCHECK(hipEventSynchronize(ctx->ev[7]));
-        ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);
         q.eb = eb * (double) (float) (mx - mn);
         absmax = std::max(std::fabs((double) mn), std::fabs((double) mx));
     } else {
@@ -1043,6 +1052,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
     }
     int rc = compress_device(ctx, d_in, d_recon, &q, absmax);
     if (rc) return rc;
+    if (rel_mode) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
     *raw_size = pending_size(ctx);
     return FAZ_OK;
 }
@@ -1089,6 +1099,10 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
                                          size_t num) {
     if (!ctx || !h_raw || !d_out || N < 1 || N > 4) return FAZ_ERR_INVALID;
     HIPCHECK(hipSetDevice(ctx->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     hipStream_t st = ctx->stream;
     EngineHeader h;
     const uint8_t *pos = h_raw;
@@ -1132,8 +1146,10 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->scalars.ensure(4));
     // codes decoded on the device (HUF:236-268), then the outliers' ranks
+    const auto td = std::chrono::steady_clock::now();
     int drc = gpu_huffman_decode<uint16_t>(ctx, pos, n - (pos - h_raw), num, 65536u, ctx->codes.p, nullptr);
     if (drc) return drc;
+    ctx->timings[6] = ms_since(td);
     if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
     ctx->prefix_key = 0;  // the compression path's table is gone
@@ -1170,6 +1186,7 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
         }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(st));
+    ctx->timings[7] = ms_since(t0);
     return FAZ_OK;
 }
 

@@ -61,6 +61,19 @@ def test_random_streams(engine, i):
     assert np.array_equal(out.cpu().numpy(), codes[:k])
 
 
+def test_unaligned_output(engine):
+    """Output that starts 4 bytes past an 8-byte boundary: the first symbol
+    is stored alone, the rest in 8-byte stores; nothing outside is touched."""
+    codes = _codes("peaked", 100003, 11)
+    sec = oracle.huffman_encode(codes)
+    buf = torch.full((codes.size + 3,), -7, dtype=torch.int32, device="cuda")
+    print("[gpu] Huffman decode into an unaligned slice", flush=True)
+    out, _ = engine.huffman_decode(sec, codes.size, out=buf[1:1 + codes.size])
+    h = buf.cpu().numpy()
+    assert h[0] == -7 and h[-2] == -7 and h[-1] == -7
+    assert np.array_equal(h[1:1 + codes.size], codes)
+
+
 def test_errors(engine):
     import faz_amd
     codes = _codes("peaked", 100000, 9)
