@@ -241,6 +241,7 @@ int make_plan(const faz_interp_params *p, Plan &pl, std::string &why) {
             d.P = P;
             d.axis = perm[P];
             for (int a = 0; a < kMaxN; a++) d.perm[a] = a < pl.N ? perm[a] : a;
+            for (int Q = 0; Q < kMaxN; Q++) d.invp[d.perm[Q]] = Q;
             uint64_t total = 1;
             for (int a = 0; a < pl.N; a++) {
                 d.dims[a] = pl.dims[a];
@@ -262,7 +263,11 @@ int make_plan(const faz_interp_params *p, Plan &pl, std::string &why) {
                 total *= d.lat[a];
             }
             d.s = (uint32_t) s;
+            d.ls = 0;
+            while (((uint64_t) 1 << d.ls) < s) d.ls++;
             d.bs = (uint32_t) bs;
+            for (int a = 0; a < pl.N; a++) d.dlat[a] = FastDiv(d.lat[a]);
+            d.dbs = FastDiv(d.bs);
             d.total = total;
             d.level_base = base;
             d.eb = L.eb;

@@ -20,19 +20,48 @@ constexpr int kMaxLevels = 64;
 // multiples of twice the stride (G2).
 enum AxisKind : uint8_t { kPassAxis = 0, kG1 = 1, kG2 = 2 };
 
+// Unsigned 32-bit division by a divisor fixed at launch: one high multiply,
+// a subtract and two shifts (the round-up multiplier method of Granlund and
+// Montgomery, "Division by Invariant Integers using Multiplication", 1994,
+// figure 4.1), exact for every x < 2^32.
+struct FastDiv {
+    uint32_t m = 1, sh1 = 0, sh2 = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t d) {
+        if (d == 0) d = 1;  // an empty lattice: never divided by
+        uint32_t l = 0;
+        while (l < 32 && ((uint64_t) 1 << l) < d) l++;
+        m = (uint32_t) ((((uint64_t) 1 << 32) * (((uint64_t) 1 << l) - d)) / d + 1);
+        sh1 = l ? 1 : 0;
+        sh2 = l ? l - 1 : 0;
+    }
+    __host__ __device__ uint32_t div(uint32_t x) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t t = __umulhi(m, x);
+#else
+        const uint32_t t = (uint32_t) (((uint64_t) m * x) >> 32);
+#endif
+        return (t + ((x - t) >> sh1)) >> sh2;
+    }
+};
+
 // One (level, pass) launch of the interpolation predictor.
 struct PassDesc {
     int N;
     int P;                    // pass number within the block (0 = first)
     int axis;                 // natural axis interpolated along
     int perm[kMaxN];          // dimension_sequences[direction]
+    int invp[kMaxN];          // invp[perm[Q]] = Q
     uint32_t dims[kMaxN];
     uint64_t offs[kMaxN];     // row-major element strides
     uint32_t lat[kMaxN];      // lattice extent per natural axis
     uint8_t kind[kMaxN];      // AxisKind per natural axis
     uint32_t nb[kMaxN];       // blocks per natural axis at this level
     uint32_t s;               // stride 2^(level-1)
+    uint32_t ls;              // log2(s)
     uint32_t bs;              // block edge interpBlockSize * stride
+    FastDiv dlat[kMaxN];      // division by lat[a]
+    FastDiv dbs;              // division by bs
     uint64_t total;           // lattice points (= codes of this pass)
     uint64_t level_base;      // codes pushed before this level
     const uint64_t *block_prefix;  // codes of earlier blocks in raster order

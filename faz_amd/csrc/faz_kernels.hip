@@ -25,11 +25,18 @@ struct PointSlot {
 template<int N>
 __device__ __forceinline__ bool locate(const PassDesc &d, uint32_t t, uint32_t *x, PointSlot &ps) {
     // lattice coordinates, last natural axis fastest
+    // (divisions by the stride are shifts, by lattice extents and the block
+    // edge multiplications: FastDiv)
+    const uint32_t ls = d.ls, ls2 = d.ls + 1;
     uint32_t r = t;
 #pragma unroll
     for (int a = N - 1; a >= 0; --a) {
-        uint32_t l = (a == 0) ? r : r % d.lat[a];
-        if (a) r /= d.lat[a];
+        uint32_t l = r;
+        if (a) {
+            const uint32_t q = d.dlat[a].div(r);
+            l = r - q * d.lat[a];
+            r = q;
+        }
         uint32_t k = d.kind[a];
         x[a] = (k == kPassAxis) ? d.s * (2 * l + 1) : (k == kG1 ? d.s * l : 2 * d.s * l);
     }
@@ -40,45 +47,62 @@ __device__ __forceinline__ bool locate(const PassDesc &d, uint32_t t, uint32_t *
 #pragma unroll
     for (int a = 0; a < N; ++a) {
         uint32_t xa = x[a];
-        uint32_t k = (d.kind[a] == kPassAxis) ? xa / d.bs : (xa ? (xa - 1) / d.bs : 0);
+        uint32_t k = (d.kind[a] == kPassAxis) ? d.dbs.div(xa) : (xa ? d.dbs.div(xa - 1) : 0);
         kb[a] = k;
         b[a] = k * d.bs;
         uint32_t e = min(b[a] + d.bs, d.dims[a] - 1);
-        nn[a] = (e - b[a]) / d.s + 1;
-        g1[a] = b[a] ? (e - b[a]) / d.s : e / d.s + 1;
-        g2[a] = b[a] ? (e - b[a]) / (2 * d.s) : e / (2 * d.s) + 1;
+        nn[a] = ((e - b[a]) >> ls) + 1;
+        g1[a] = b[a] ? (e - b[a]) >> ls : (e >> ls) + 1;
+        g2[a] = b[a] ? (e - b[a]) >> ls2 : (e >> ls2) + 1;
         kl = kl * d.nb[a] + k;
     }
-    // codes of the earlier passes of this block
+    // codes of the earlier passes of this block and the line index; every
+    // per-axis array is indexed by an unrolled natural axis (the permutation
+    // enters through comparisons of d.invp), so the arrays stay in registers.
+    // Pass Pp interpolates axis perm[Pp]: its count is n/2 along that axis
+    // times g1 (axes earlier in the permutation) or g2 (later) elsewhere.
     uint64_t off = 0;
-    for (int Pp = 0; Pp < d.P; ++Pp) {
-        uint64_t c = nn[d.perm[Pp]] / 2;
 #pragma unroll
-        for (int Q = 0; Q < N; ++Q) {
-            if (Q == Pp) continue;
-            c *= (Q < Pp) ? g1[d.perm[Q]] : g2[d.perm[Q]];
+    for (int a = 0; a < N; ++a) {
+        if (d.invp[a] >= d.P) continue;
+        uint64_t c = nn[a] / 2;
+#pragma unroll
+        for (int a2 = 0; a2 < N; ++a2) {
+            if (a2 == a) continue;
+            c *= d.invp[a2] < d.invp[a] ? g1[a2] : g2[a2];
         }
         off += c;
     }
-    // line index: outer axes in permutation order, earlier ones outer
+    // line index: the axes other than the pass axis in permutation order,
+    // earlier ones outer
+    uint32_t lidx[N], lsize[N];
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        if (d.invp[a] < d.P) {
+            lidx[a] = b[a] ? ((x[a] - b[a]) >> ls) - 1 : x[a] >> ls;
+            lsize[a] = g1[a];
+        } else {
+            lidx[a] = b[a] ? ((x[a] - b[a]) >> ls2) - 1 : x[a] >> ls2;
+            lsize[a] = g2[a];
+        }
+    }
     uint64_t line = 0;
 #pragma unroll
-    for (int Q = 0; Q < N; ++Q) {
-        if (Q == d.P) continue;
-        int ax = d.perm[Q];
-        uint32_t idx, size;
-        if (Q < d.P) {
-            idx = b[ax] ? (x[ax] - b[ax]) / d.s - 1 : x[ax] / d.s;
-            size = g1[ax];
-        } else {
-            idx = b[ax] ? (x[ax] - b[ax]) / (2 * d.s) - 1 : x[ax] / (2 * d.s);
-            size = g2[ax];
-        }
-        line = line * size + idx;
+    for (int a = 0; a < N; ++a) {
+        if (a == d.axis) continue;
+        uint64_t mult = 1;
+#pragma unroll
+        for (int a2 = 0; a2 < N; ++a2)
+            if (a2 != d.axis && d.invp[a2] > d.invp[a]) mult *= lsize[a2];
+        line += lidx[a] * mult;
     }
-    const int ax = d.axis;
-    uint32_t i = (x[ax] - b[ax]) / d.s;
-    uint32_t n = nn[ax];
+    uint32_t i = 0, n = 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+        if (a == d.axis) {
+            i = (x[a] - b[a]) >> ls;
+            n = nn[a];
+        }
     uint32_t rank, exit = 0;
     if (d.linear || n < 5) {
         rank = (i - 1) / 2;
@@ -159,7 +183,18 @@ __global__ void __launch_bounds__(256) recover_pass_kernel(PassDesc d, float *__
     const uint64_t S = (uint64_t) d.s * d.offs[d.axis];
     float pred = predict(data, f, S, ps, d.linear);
     int code = codes[ps.seq];
-    data[f] = code ? recover_point(pred, code, d.radius, d.eb) : unpred_at[ps.seq];
+    const float v = code ? recover_point(pred, code, d.radius, d.eb) : unpred_at[ps.seq];
+    if (N > 1 && d.P == N - 1 && S == 1) {
+        // A last pass along the contiguous axis at stride 1 covers whole rows:
+        // each point also stores its left neighbour (the value it read for
+        // its stencil, so concurrent readers see the same bits) and every line
+        // of the row is written in full instead of every other word.
+        const float left = data[f - 1];
+        __builtin_nontemporal_store(left, data + f - 1);
+        __builtin_nontemporal_store(v, data + f);
+    } else {
+        data[f] = v;
+    }
 }
 
 void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,

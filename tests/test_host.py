@@ -69,3 +69,47 @@ def test_host_huffman_rejects_empty_histogram():
 def test_cli_usage_runs_without_gpu():
     r = subprocess.run([faz_amd.CLI_PATH, "-h"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "Usage: faz" in r.stdout
+
+
+FASTDIV_CHECK = r'''
+#include <cstdio>
+#include <cstdint>
+#include <random>
+#include "faz_internal.h"
+int main() {
+    std::mt19937_64 g(7);
+    long bad = 0;
+    auto check = [&](uint32_t d, uint32_t x) {
+        if (faz::FastDiv(d).div(x) != x / d) bad++;
+    };
+    for (uint32_t d = 1; d <= 20000; d++)
+        for (uint32_t x : {0u, 1u, d - 1, d, d + 1, 2 * d - 1, 2 * d, 0xFFFFFFFFu, 0x80000000u, 0x7FFFFFFFu,
+                           (uint32_t) g(), (uint32_t) g()})
+            check(d, x);
+    for (int k = 0; k < 200000; k++) {
+        uint32_t d = (uint32_t) g();
+        check(d ? d : 1, (uint32_t) g());
+    }
+    for (uint32_t s = 0; s < 32; s++) {
+        check(1u << s, 0xFFFFFFFFu);
+        check((1u << s) + 1, 0xFFFFFFFFu);
+        check(s ? (1u << s) - 1 : 1, 0xFFFFFFFFu);
+    }
+    printf("%ld\n", bad);
+    return bad != 0;
+}
+'''
+
+
+def test_fastdiv_is_exact(tmp_path):
+    """The multiply-shift division the pass kernels use for lattice extents and
+    block edges equals integer division (divisors 1..20000 at their edge
+    values, random divisors, powers of two and their neighbours)."""
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "fastdiv.cpp"
+    src.write_text(FASTDIV_CHECK)
+    exe = tmp_path / "fastdiv"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-I" + os.path.join(here, "faz_amd", "csrc"), str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip() == "0"
