@@ -151,10 +151,11 @@ struct faz_gpu_ctx {
     // per-chunk start and symbol offset, symbol counts + error flag, the
     // per-chunk entry maps (two buffers) and counts per entry
     faz::DevBuf<uint32_t> dec_words, dec_tab, dec_cnt;
-    faz::DevBuf<uint8_t> dec_leaf, dec_x;
+    faz::DevBuf<uint8_t> dec_leaf, dec_x, dec_g;
     faz::DevBuf<uint16_t> dec_n;
+    faz::DevBuf<uint64_t> dec_wtab;  // multi-codeword output table
     faz::DevBuf<uint64_t> dec_pos;
-    uint32_t dec_rounds = 0;  // composition steps of the last decode
+    uint32_t dec_rounds = 0;  // chunk groups of the last decode
 };
 
 #define CTX_FAIL(code, ...)                                          \

@@ -65,6 +65,13 @@ struct Tree {
 
 __device__ __forceinline__ uint32_t slot(uint32_t i) { return i + (i >> 5); }
 
+// Multi-codeword entries for counting (ctab, one per kTabBits-bit window):
+// bits 0..11 the window offsets where a codeword starts, 12..15 how many
+// whole codewords the window holds (0: the first one is longer than the
+// window or leaves the tree), 16..20 the bits they take.
+__device__ __forceinline__ uint32_t cnt_of(uint32_t e) { return (e >> 12) & 15u; }
+__device__ __forceinline__ uint32_t used_of(uint32_t e) { return (e >> 16) & 31u; }
+
 struct BitReader {
     const uint32_t *lw;     // staged words (byte order swapped), see slot()
     uint64_t base;          // stream word index of lw's first word
@@ -169,7 +176,42 @@ struct Cursor {
         const int32_t nd = decode_one(r, tab, T, pos);
         if (nd >= 0 && pos <= nbits) k++;
     }
+    // The ctab entry of the next window when every codeword it holds starts
+    // before lim (0 otherwise); consume() then takes them.
+    __device__ uint32_t peek_multi(const uint32_t *ctab, uint64_t lim) {
+        if (pos + kTabBits > lim) return 0;
+        r.refill();
+        const uint32_t e = ctab[r.peek(kTabBits)];
+        return cnt_of(e) ? e : 0u;
+    }
+    __device__ void consume(uint32_t e) {
+        const int used = (int) used_of(e);
+        r.drop(used);
+        pos += (uint64_t) used;
+        k += cnt_of(e);
+    }
 };
+
+// ORs the 12-bit start mask S, placed at offset q, into the 128-bit mask m.
+__device__ __forceinline__ void or_mask(uint64_t *m, uint32_t S, uint64_t q) {
+    if (q < 64) {
+        m[0] |= (uint64_t) S << q;
+        if (q > 52) m[1] |= (uint64_t) S >> (64 - q);
+    } else if (q < 128) {
+        m[1] |= (uint64_t) S << (q - 64);
+    }
+}
+// The 12 bits of m from offset q on (zero past bit 127).
+__device__ __forceinline__ uint32_t mask_window(const uint64_t *m, uint64_t q) {
+    uint64_t w = 0;
+    if (q < 64) {
+        w = m[0] >> q;
+        if (q > 52) w |= m[1] << (64 - q);
+    } else if (q < 128) {
+        w = m[1] >> (q - 64);
+    }
+    return (uint32_t) w & 0xFFFu;
+}
 
 __device__ __forceinline__ bool mask_has(const uint64_t *m, uint32_t q) { return (m[q >> 6] >> (q & 63)) & 1u; }
 __device__ __forceinline__ uint32_t mask_below(const uint64_t *m, uint32_t q) {
@@ -185,12 +227,15 @@ __device__ __forceinline__ uint32_t mask_below(const uint64_t *m, uint32_t q) {
 // masks over the chunk's first kMaskBits bits.
 __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
                                                                uint64_t nbits, uint32_t nchunks,
-                                                               const uint32_t *__restrict__ gtab, Tree T, uint32_t K,
+                                                               const uint32_t *__restrict__ gtab,
+                                                               const uint32_t *__restrict__ gctab, Tree T, uint32_t K,
                                                                uint8_t *__restrict__ xmap,
                                                                uint16_t *__restrict__ nmap) {
     __shared__ uint32_t tab[1 << kTabBits];
+    __shared__ uint32_t ctab[1 << kTabBits];
     __shared__ uint32_t lw[kStageSlots];
     const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
+    for (uint32_t i = threadIdx.x; i < (1u << kTabBits); i += blockDim.x) ctab[i] = gctab[i];
     stage(tab, lw, gtab, words, nwords, base);
     const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
     if (c >= nchunks) return;
@@ -215,6 +260,12 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
         A.init(proto, b);
         while (A.pos < end) {
             const uint64_t q = A.pos - b;
+            const uint32_t e = A.peek_multi(ctab, end);
+            if (e) {
+                or_mask(mA, e & 0xFFFu, q);
+                A.consume(e);
+                continue;
+            }
             if (q < kMaskBits) mA[q >> 6] |= 1ull << (q & 63);
             A.step(tab, T, nbits);
         }
@@ -236,7 +287,32 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
         bool merged = false;
         while (O.pos < end) {
             const uint64_t q = O.pos - b;
-            if (masks && q < kMaskBits) {
+            const bool watch = masks && q < kMaskBits;
+            const uint32_t e = O.peek_multi(ctab, end);
+            if (e) {
+                if (watch) {
+                    // the first codeword start of the window on a start of A or B
+                    const uint32_t S = e & 0xFFFu;
+                    const uint32_t hitA = mask_window(mA, q) & S;
+                    const uint32_t hitB = hasB ? mask_window(mB, q) & S : 0u;
+                    const uint32_t hit = hitA | hitB;
+                    if (hit) {
+                        const uint32_t j = (uint32_t) __builtin_ctz(hit);
+                        const uint32_t at = O.k + (uint32_t) __popc(S & ((1u << j) - 1u));
+                        const uint32_t p = (uint32_t) q + j;
+                        if ((hitA >> j) & 1u)
+                            put(o, xA, at + nA - mask_below(mA, p));
+                        else
+                            put(o, xB, at + nB - mask_below(mB, p));
+                        merged = true;
+                        break;
+                    }
+                    or_mask(mO, S, q);
+                }
+                O.consume(e);
+                continue;
+            }
+            if (watch) {
                 if (mask_has(mA, (uint32_t) q)) {
                     put(o, xA, O.k + nA - mask_below(mA, (uint32_t) q));
                     merged = true;
@@ -261,44 +337,102 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
     }
 }
 
-// Pass 2, one Hillis-Steele step: out_c = in_c o in_{c-d} for c >= d.
-__global__ void huff_compose_kernel(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, uint64_t total,
-                                    uint32_t K, uint32_t d) {
-    const uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint64_t c = i / K, o = i % K;
-    uint8_t v = in[i];
-    if (c >= d) v = in[c * K + in[(c - d) * K + o]];
-    out[i] = v;
+// Pass 2a: inside each group of kDecThreads chunks, the inclusive
+// compositions M_i = x_i o ... o x_0 of the group's maps (Hillis-Steele in
+// LDS, one thread per chunk); gmap receives them and grp the group's whole
+// composition.
+__global__ void __launch_bounds__(kDecThreads) huff_group_compose_kernel(const uint8_t *__restrict__ xmap,
+                                                                         uint32_t nchunks, uint32_t K,
+                                                                         uint8_t *__restrict__ gmap,
+                                                                         uint8_t *__restrict__ grp) {
+    __shared__ uint8_t buf[2][kDecThreads * 64];
+    const uint32_t i = threadIdx.x;
+    const uint64_t c0 = (uint64_t) blockIdx.x * kDecThreads;
+    const uint32_t n = (uint32_t) min<uint64_t>(kDecThreads, nchunks - c0);
+    for (uint32_t j = i; j < n * K; j += kDecThreads) buf[0][j] = xmap[c0 * K + j];
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t d = 1; d < n; d *= 2) {
+        if (i < n) {
+            for (uint32_t o = 0; o < K; o++) {
+                uint8_t v = buf[cur][i * K + o];
+                if (i >= d) v = buf[cur][i * K + buf[cur][(i - d) * K + o]];
+                buf[cur ^ 1][i * K + o] = v;
+            }
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (uint32_t j = i; j < n * K; j += kDecThreads) gmap[c0 * K + j] = buf[cur][j];
+    for (uint32_t o = i; o < K; o += kDecThreads) grp[(uint64_t) blockIdx.x * K + o] = buf[cur][(n - 1) * K + o];
 }
 
-// Entry of every chunk (chunk c's is the composed map of chunk c - 1 at 0)
-// and its symbol count at that entry.
-__global__ void huff_entry_kernel(const uint8_t *__restrict__ comp, const uint16_t *__restrict__ nmap,
-                                  uint32_t nchunks, uint32_t K, uint64_t *__restrict__ start,
-                                  uint32_t *__restrict__ count) {
+// Pass 2b (one workgroup): the entry offset of every group, walking the
+// group compositions from entry 0, staged through LDS in batches.
+__global__ void __launch_bounds__(1024) huff_group_entry_kernel(const uint8_t *__restrict__ grp, uint32_t ngroups,
+                                                                uint32_t K, uint8_t *__restrict__ gentry) {
+    constexpr uint32_t kBuf = 48 * 1024;
+    __shared__ uint8_t buf[kBuf];
+    const uint32_t per = kBuf / K;
+    uint32_t e = 0;
+    for (uint32_t g0 = 0; g0 < ngroups; g0 += per) {
+        const uint32_t n = min(per, ngroups - g0);
+        for (uint32_t j = threadIdx.x; j < n * K; j += blockDim.x) buf[j] = grp[(uint64_t) g0 * K + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t j = 0; j < n; j++) {
+                gentry[g0 + j] = (uint8_t) e;
+                e = buf[j * K + e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Entry of every chunk (the group's entry, carried through the group's
+// earlier chunks) and its symbol count at that entry.
+__global__ void huff_entry_kernel(const uint8_t *__restrict__ gmap, const uint8_t *__restrict__ gentry,
+                                  const uint16_t *__restrict__ nmap, uint32_t nchunks, uint32_t K,
+                                  uint64_t *__restrict__ start, uint32_t *__restrict__ count) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nchunks) return;
-    const uint32_t e = c ? comp[(uint64_t) (c - 1) * K] : 0u;
+    const uint32_t ge = gentry[c / kDecThreads];
+    const uint32_t e = (c % kDecThreads) ? gmap[(uint64_t) (c - 1) * K + ge] : ge;
     start[c] = (uint64_t) c * kChunkBits + e;
     count[c] = nmap[(uint64_t) c * K + e];
 }
 
 // Pass 3: chunk c writes its symbols from offset[c] on (those below num),
 // each sym + sym_offset, which must lie in [0, limit) unless limit is 0;
-// err[0] = 1 otherwise or when the bits leave the tree.
-template<class OUT>
+// err[0] = 1 otherwise or when the bits leave the tree.  MULTI: wtab (LDS)
+// gives up to three output values per kTabBits-bit window (bits 0..47, in
+// order), their count (48..49, 0: decode one codeword through the primary
+// table, read from global memory) and their bits (52..56).
+template<class OUT, bool MULTI>
 __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t *__restrict__ words, uint64_t nwords,
                                                                  uint64_t nbits, uint32_t nchunks,
-                                                                 const uint32_t *__restrict__ gtab, Tree T,
+                                                                 const uint32_t *__restrict__ gtab,
+                                                                 const uint64_t *__restrict__ gwtab, Tree T,
                                                                  const uint64_t *__restrict__ start,
                                                                  const uint64_t *__restrict__ offset,
                                                                  int32_t sym_offset, uint32_t limit, uint64_t num,
                                                                  OUT *__restrict__ out, uint32_t *__restrict__ err) {
-    __shared__ uint32_t tab[1 << kTabBits];
+    __shared__ uint32_t tab[MULTI ? 1 : (1 << kTabBits)];
+    __shared__ uint64_t wtab[MULTI ? (1 << kTabBits) : 1];
     __shared__ uint32_t lw[kStageSlots];
     const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
-    stage(tab, lw, gtab, words, nwords, base);
+    for (uint32_t i = threadIdx.x; i < (1u << kTabBits); i += blockDim.x) {
+        if (MULTI)
+            wtab[i] = gwtab[i];
+        else
+            tab[i] = gtab[i];
+    }
+    for (uint32_t i = threadIdx.x; i < kStageWords; i += blockDim.x) {
+        const uint64_t g = base + i;
+        lw[slot(i)] = g < nwords ? __builtin_bswap32(words[g]) : 0u;
+    }
+    __syncthreads();
+    const uint32_t *dtab = MULTI ? gtab : tab;
     const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
     if (c >= nchunks) return;
     const uint64_t end = min((uint64_t) (c + 1) * kChunkBits, nbits);
@@ -311,21 +445,10 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
     constexpr int W = 8 * sizeof(OUT);
     uint64_t acc = 0;
     int na = 0;
-    bool bad = false;
-    while (pos < end && o < num) {
-        const int32_t k = decode_one(r, tab, T, pos);
-        if (k < 0) {
-            bad = true;
-            break;
-        }
-        const uint32_t v = (uint32_t) (T.sym[k] + sym_offset);
-        if (limit && v >= limit) {
-            bad = true;
-            break;
-        }
+    auto push = [&](uint32_t v) {
         if (na == 0 && (reinterpret_cast<uintptr_t>(out + o) & 7u) != 0) {
             out[o++] = (OUT) v;
-            continue;
+            return;
         }
         acc |= (uint64_t) (std::make_unsigned_t<OUT>) (OUT) v << (W * na);
         na++;
@@ -335,6 +458,34 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
             acc = 0;
             na = 0;
         }
+    };
+    bool bad = false;
+    while (pos < end && o < num) {
+        if (MULTI && pos + kTabBits <= end) {
+            r.refill();
+            const uint64_t e = wtab[r.peek(kTabBits)];
+            const uint32_t cnt = (uint32_t) (e >> 48) & 3u;
+            if (cnt && o + cnt <= num) {
+                const int used = (int) ((e >> 52) & 31u);
+                r.drop(used);
+                pos += (uint64_t) used;
+                push((uint32_t) e & 0xFFFFu);
+                if (cnt > 1) push((uint32_t) (e >> 16) & 0xFFFFu);
+                if (cnt > 2) push((uint32_t) (e >> 32) & 0xFFFFu);
+                continue;
+            }
+        }
+        const int32_t k = decode_one(r, dtab, T, pos);
+        if (k < 0) {
+            bad = true;
+            break;
+        }
+        const uint32_t v = (uint32_t) (T.sym[k] + sym_offset);
+        if (limit && v >= limit) {
+            bad = true;
+            break;
+        }
+        push(v);
     }
     for (int j = 0; j < na; j++) out[o - na + j] = (OUT) (acc >> (W * j));
     if (bad) atomicOr(err, 1u);
@@ -455,6 +606,45 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
         if (e == kLongFlag) e = build_sub(stop, kTabBits);
         tab[v] = e;
     }
+    // multi-codeword windows: whole codewords from the window start on
+    // (ctab for counting; wtab: up to three output values in [0, limit),
+    // for 16-bit outputs)
+    constexpr bool kMulti = sizeof(OUT) == 2;
+    std::vector<uint32_t> ctab(tab.size());
+    std::vector<uint64_t> wtab(kMulti ? tab.size() : 0);
+    for (uint32_t v = 0; v < tab.size(); v++) {
+        uint32_t at = 0, cnt = 0, S = 0, wcnt = 0, wused = 0;
+        uint64_t wv = 0;
+        bool wopen = kMulti && limit && limit <= 65536u;
+        while (at < (uint32_t) kTabBits) {
+            int32_t nd = 0;
+            uint32_t len = 0;
+            for (uint32_t p = at; p < (uint32_t) kTabBits; p++) {
+                nd = ((v >> (kTabBits - 1 - p)) & 1u) ? right[nd] : left[nd];
+                if (nd < 0) break;
+                if (leaf[nd]) {
+                    len = p - at + 1;
+                    break;
+                }
+            }
+            if (!len) break;
+            S |= 1u << at;
+            cnt++;
+            if (wopen) {
+                const uint32_t val = (uint32_t) (sym[nd] + offset);
+                if (val < limit && wcnt < 3) {
+                    wv |= (uint64_t) val << (16 * wcnt);
+                    wcnt++;
+                    wused += len;
+                } else {
+                    wopen = false;
+                }
+            }
+            at += len;
+        }
+        ctab[v] = S | cnt << 12 | at << 16;
+        if (kMulti) wtab[v] = wv | (uint64_t) wcnt << 48 | (uint64_t) wused << 52;
+    }
     // longest codeword: the entry offsets of a chunk lie below it
     uint32_t maxlen = 0;
     {
@@ -478,11 +668,13 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     const uint32_t nchunks = (uint32_t) std::max<uint64_t>(1, (nbits + kChunkBits - 1) / kChunkBits);
     const uint64_t nmaps = (uint64_t) nchunks * K;
     HIPCHECK(ctx->dec_words.ensure(nwords + 2));
-    HIPCHECK(ctx->dec_tab.ensure(tab.size() + 3 * nc + sub.size()));
+    HIPCHECK(ctx->dec_tab.ensure(2 * tab.size() + 3 * nc + sub.size()));
+    HIPCHECK(ctx->dec_wtab.ensure(tab.size()));
     HIPCHECK(ctx->dec_leaf.ensure(nc));
     HIPCHECK(ctx->dec_pos.ensure(2 * (size_t) nchunks));
     HIPCHECK(ctx->dec_cnt.ensure((size_t) nchunks + 2));
     HIPCHECK(ctx->dec_x.ensure(2 * nmaps));
+    HIPCHECK(ctx->dec_g.ensure((size_t) ((nchunks + kDecThreads - 1) / kDecThreads) * (K + 1)));
     HIPCHECK(ctx->dec_n.ensure(nmaps));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->scalars.ensure(4));
@@ -498,32 +690,33 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     HIPCHECK(hipMemcpyAsync(ctx->dec_leaf.p, leaf.data(), nc, hipMemcpyHostToDevice, st));
     uint32_t *d_sub = ctx->dec_tab.p + tab.size() + 3 * nc;
     if (!sub.empty()) HIPCHECK(hipMemcpyAsync(d_sub, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, st));
+    uint32_t *d_ctab = d_sub + sub.size();
+    HIPCHECK(hipMemcpyAsync(d_ctab, ctab.data(), ctab.size() * 4, hipMemcpyHostToDevice, st));
+    if (kMulti) HIPCHECK(hipMemcpyAsync(ctx->dec_wtab.p, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice, st));
     const Tree T{d_tree, d_tree + nc, d_tree + 2 * nc, ctx->dec_leaf.p, d_sub};
     uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nchunks;
     uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nchunks;
     const dim3 grid((nchunks + kDecThreads - 1) / kDecThreads), block(kDecThreads);
     // pass 1: entry -> exit maps and counts
-    hipLaunchKernelGGL(huff_map_kernel, grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab, T, K,
-                       ctx->dec_x.p, ctx->dec_n.p);
+    hipLaunchKernelGGL(huff_map_kernel, grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab, d_ctab,
+                       T, K, ctx->dec_x.p, ctx->dec_n.p);
     HIPCHECK(hipGetLastError());
-    // pass 2: prefix composition, then every chunk's entry and count
-    uint8_t *cur = ctx->dec_x.p, *alt = ctx->dec_x.p + nmaps;
-    const dim3 cgrid((unsigned) ((nmaps + 255) / 256));
-    uint32_t steps = 0;
-    for (uint32_t d = 1; d < nchunks; d *= 2, steps++) {
-        hipLaunchKernelGGL(huff_compose_kernel, cgrid, dim3(256), 0, st, cur, alt, nmaps, K, d);
-        std::swap(cur, alt);
-        if (d > nchunks / 2) break;
-    }
-    ctx->dec_rounds = steps;
-    hipLaunchKernelGGL(huff_entry_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, cur, ctx->dec_n.p, nchunks,
-                       K, d_start, d_cnt);
+    // pass 2: compositions inside groups of chunks, the groups' entries,
+    // then every chunk's entry and count
+    const uint32_t ngroups = (nchunks + kDecThreads - 1) / kDecThreads;
+    uint8_t *d_gmap = ctx->dec_x.p + nmaps, *d_grp = ctx->dec_g.p, *d_gentry = ctx->dec_g.p + (size_t) ngroups * K;
+    hipLaunchKernelGGL(huff_group_compose_kernel, dim3(ngroups), dim3(kDecThreads), 0, st, ctx->dec_x.p, nchunks, K,
+                       d_gmap, d_grp);
+    hipLaunchKernelGGL(huff_group_entry_kernel, dim3(1), dim3(1024), 0, st, d_grp, ngroups, K, d_gentry);
+    ctx->dec_rounds = ngroups;
+    hipLaunchKernelGGL(huff_entry_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, d_gmap, d_gentry,
+                       ctx->dec_n.p, nchunks, K, d_start, d_cnt);
     HIPCHECK(hipGetLastError());
     // output offsets of the chunks (exclusive scan) and the total, then pass 3
     launch_zero_scan(d_cnt, d_off, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     launch_fill_u32(d_err, 1, 0u, st);
-    hipLaunchKernelGGL((huff_write_kernel<OUT>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab,
-                       T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err);
+    hipLaunchKernelGGL((huff_write_kernel<OUT, kMulti>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks,
+                       d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err);
     HIPCHECK(hipGetLastError());
     uint32_t *hf = ctx->hp_hist.data() + 2 * 65536;  // pinned scalars
     HIPCHECK(hipMemcpyAsync(hf, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
