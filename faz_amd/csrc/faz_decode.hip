@@ -734,6 +734,102 @@ template int gpu_huffman_decode<uint16_t>(faz_gpu_ctx *, const uint8_t *, size_t
                                           size_t *);
 template int gpu_huffman_decode<int32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, int32_t *,
                                          size_t *);
+template int gpu_huffman_decode<uint32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint32_t *,
+                                          size_t *);
+
+// ---------------------------------------------------------------------------
+// Unpredictable values of 32-bit code streams (the general frontend): the
+// k-th zero code takes the k-th value of the list.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint32_t kZeroThreads = 256;
+constexpr uint32_t kZeroPer = 16;
+constexpr uint32_t kZeroChunk = kZeroThreads * kZeroPer;
+
+__device__ __forceinline__ uint32_t zeros16(const uint32_t *__restrict__ codes, uint64_t i0, uint64_t n,
+                                            uint32_t &mask) {
+    mask = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kZeroPer; j++) {
+        const uint64_t i = i0 + j;
+        if (i < n && codes[i] == 0) mask |= 1u << j;
+    }
+    return (uint32_t) __popc(mask);
+}
+
+__global__ void __launch_bounds__(kZeroThreads) zero_count_u32_kernel(const uint32_t *__restrict__ codes, uint64_t n,
+                                                                      uint32_t *__restrict__ chunk_zeros) {
+    __shared__ uint32_t s[kZeroThreads / 64];
+    uint32_t mask;
+    uint32_t z = zeros16(codes, (uint64_t) blockIdx.x * kZeroChunk + threadIdx.x * kZeroPer, n, mask);
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o);
+    if (threadIdx.x % 64 == 0) s[threadIdx.x / 64] = z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (uint32_t w = 0; w < kZeroThreads / 64; w++) t += s[w];
+        chunk_zeros[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(kZeroThreads) zero_scatter_u32_kernel(const uint32_t *__restrict__ codes, uint64_t n,
+                                                                        const uint64_t *__restrict__ chunk_base,
+                                                                        const float *__restrict__ unpred, uint64_t U,
+                                                                        float *__restrict__ outval) {
+    __shared__ uint32_t s[kZeroThreads / 64];
+    const uint64_t i0 = (uint64_t) blockIdx.x * kZeroChunk + threadIdx.x * kZeroPer;
+    uint32_t mask;
+    const uint32_t z = zeros16(codes, i0, n, mask);
+    // exclusive prefix of the per-thread counts inside the workgroup
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    uint32_t incl = z;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= (uint32_t) o) incl += t;
+    }
+    if (lane == 63) s[wave] = incl;
+    __syncthreads();
+    uint64_t rank = chunk_base[blockIdx.x] + incl - z;
+    for (uint32_t w = 0; w < wave; w++) rank += s[w];
+    while (mask) {
+        const uint32_t j = (uint32_t) __builtin_ctz(mask);
+        mask &= mask - 1;
+        if (rank < U) outval[i0 + j] = unpred[rank];
+        rank++;
+    }
+}
+
+}  // namespace
+
+// Device zero ranks of n 32-bit codes: outval[i] = unpred[rank of zero i]
+// for every zero code; fails when there are more zero codes than values.
+int gpu_scatter_unpred_u32(faz_gpu_ctx *ctx, const uint32_t *d_codes, uint64_t n, const float *h_unpred, uint64_t U,
+                           float *d_outval) {
+    hipStream_t st = ctx->stream;
+    const uint64_t nz64 = (n + kZeroChunk - 1) / kZeroChunk;
+    if (nz64 >= (1ull << 31)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "too many codes");
+    const uint32_t nz = (uint32_t) std::max<uint64_t>(nz64, 1);
+    HIPCHECK(ctx->unpred.ensure(U));
+    HIPCHECK(ctx->chunk_zeros.ensure(nz));
+    HIPCHECK(ctx->chunk_base.ensure(nz));
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(nz)));
+    HIPCHECK(ctx->scalars.ensure(4));
+    if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, h_unpred, U * 4, hipMemcpyHostToDevice, st));
+    if (!n) return FAZ_OK;
+    hipLaunchKernelGGL(zero_count_u32_kernel, dim3(nz), dim3(kZeroThreads), 0, st, d_codes, n, ctx->chunk_zeros.p);
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nz, ctx->tiles.p, ctx->scalars.p, st);
+    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned
+    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(spin_wait(st));
+    uint64_t zeros;
+    memcpy(&zeros, hs, 8);
+    if (zeros > U) CTX_FAIL(FAZ_ERR_FORMAT, "more zero codes than unpredictable values");
+    hipLaunchKernelGGL(zero_scatter_u32_kernel, dim3(nz), dim3(kZeroThreads), 0, st, d_codes, n, ctx->chunk_base.p,
+                       ctx->unpred.p, U, d_outval);
+    HIPCHECK(hipGetLastError());
+    return FAZ_OK;
+}
 
 }  // namespace faz
 

@@ -764,19 +764,6 @@ int gf_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, int N, int lo,
     if (!need(U * 4)) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
     const float *unpred_h = reinterpret_cast<const float *>(p);
     p += U * 4;
-    std::vector<int32_t> codes;
-    if (!decode_huffman(p, end - p, codes, num)) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman section malformed");
-    std::vector<uint32_t> c32(num);
-    std::vector<float> ov(num, 0.0f);
-    uint64_t k = 0;
-    for (size_t i = 0; i < num; i++) {
-        if (codes[i] < 0 || codes[i] >= 2 * radius) CTX_FAIL(FAZ_ERR_FORMAT, "code out of range");
-        c32[i] = (uint32_t) codes[i];
-        if (!c32[i]) {
-            if (k >= U) CTX_FAIL(FAZ_ERR_FORMAT, "more zero codes than unpredictable values");
-            ov[i] = unpred_h[k++];
-        }
-    }
     prm.eb = eb;
     prm.radius = radius;
     const GFQuant Q = gf_quant(eb, radius);
@@ -786,8 +773,11 @@ int gf_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, int N, int lo,
     HIPCHECK(ctx->gf_codes.ensure(num));
     HIPCHECK(ctx->outval.ensure(num));
     HIPCHECK(ctx->gf_sel.ensure(G.nblocks));
-    HIPCHECK(hipMemcpyAsync(ctx->gf_codes.p, c32.data(), num * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(ctx->outval.p, ov.data(), num * 4, hipMemcpyHostToDevice, st));
+    // codes decoded on the device (HUF:236-268), each below 2 * radius, then
+    // the unpredictable values at the zero codes
+    if ((rc = gpu_huffman_decode<uint32_t>(ctx, p, end - p, num, 2u * (uint32_t) radius, ctx->gf_codes.p, nullptr)))
+        return rc;
+    if ((rc = gpu_scatter_unpred_u32(ctx, ctx->gf_codes.p, num, unpred_h, U, ctx->outval.p))) return rc;
     if (!sel.empty()) HIPCHECK(hipMemcpyAsync(ctx->gf_sel.p, sel.data(), sel.size(), hipMemcpyHostToDevice, st));
     ctx->hist_codes = nullptr;
     gf_launch<1>(G, P, Q, nullptr, d_out, ctx->gf_codes.p, ctx->outval.p, ctx->gf_sel.p, ctx->gf_tab.p,

@@ -167,6 +167,11 @@ template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
                        size_t *used);
 
+// outval[i] = the unpredictable value of zero code i, for n 32-bit codes on
+// the device (faz_decode.hip); uploads the U host values first.
+int gpu_scatter_unpred_u32(faz_gpu_ctx *ctx, const uint32_t *d_codes, uint64_t n, const float *h_unpred, uint64_t U,
+                           float *d_outval);
+
 // Chunk geometry of the histogram / outlier / encode kernels.
 constexpr uint32_t kHistThreads = 256;
 constexpr uint32_t kHistPerThread = 64;
