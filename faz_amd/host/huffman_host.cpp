@@ -87,28 +87,30 @@ struct Node {
     bool leaf = false;
 };
 
-// HuffmanEncoder's 1-based binary heap (qinsert/qremove, HUF:455-485)
+// HuffmanEncoder's 1-based binary heap (qinsert/qremove, HUF:455-485).  The
+// entries carry their node's frequency, so the sift loops compare without
+// reaching into the node pool; the operations are the reference's.
 class Heap {
 public:
-    explicit Heap(const std::vector<Node> &pool) : pool_(pool) { q_.assign(2, -1); }
-    void insert(int n) {
+    explicit Heap(size_t cap) { q_.resize(cap + 2); }
+    void insert(int n, uint64_t f) {
         int i = end_++, j;
-        if ((int) q_.size() <= i) q_.resize(2 * i + 2, -1);
+        if ((int) q_.size() <= i) q_.resize(2 * (size_t) i + 2);
         while ((j = i >> 1)) {
-            if (pool_[q_[j]].freq <= pool_[n].freq) break;
+            if (q_[j].f <= f) break;
             q_[i] = q_[j];
             i = j;
         }
-        q_[i] = n;
+        q_[i] = {f, n};
     }
     int remove() {
         int i = 1, l;
-        int n = q_[1];
+        const int n = q_[1].n;
         end_--;
         q_[1] = q_[end_];
         while ((l = i << 1) < end_) {
-            if (l + 1 < end_ && pool_[q_[l + 1]].freq < pool_[q_[l]].freq) l++;
-            if (pool_[q_[i]].freq > pool_[q_[l]].freq) {
+            if (l + 1 < end_ && q_[l + 1].f < q_[l].f) l++;
+            if (q_[i].f > q_[l].f) {
                 std::swap(q_[i], q_[l]);
                 i = l;
             } else {
@@ -118,11 +120,14 @@ public:
         return n;
     }
     int size() const { return end_ - 1; }
-    int top() const { return q_[1]; }
+    int top() const { return q_[1].n; }
 
 private:
-    const std::vector<Node> &pool_;
-    std::vector<int> q_;
+    struct E {
+        uint64_t f;
+        int n;
+    };
+    std::vector<E> q_;
     int end_ = 1;
 };
 
@@ -183,7 +188,17 @@ void put_be32(std::vector<uint8_t> &out, uint32_t v) {
 
 int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
     std::vector<std::pair<uint32_t, int>> present;
-    for (int s = 0; s < nsym; s++)
+    present.reserve(1024);
+    int s0 = 0;
+    for (; s0 + 4 <= nsym; s0 += 4) {  // four counts at a time: most are zero
+        uint64_t a, b;
+        memcpy(&a, freq + s0, 8);
+        memcpy(&b, freq + s0 + 2, 8);
+        if (!(a | b)) continue;
+        for (int s = s0; s < s0 + 4; s++)
+            if (freq[s]) present.push_back({first[s], s});
+    }
+    for (int s = s0; s < nsym; s++)
         if (freq[s]) present.push_back({first[s], s});
     if (present.empty()) return -1;
     std::sort(present.begin(), present.end());
@@ -202,14 +217,14 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
 
     std::vector<Node> pool;
     pool.reserve(2 * present.size());
-    Heap heap(pool);
+    Heap heap(2 * present.size());
     for (int key: order) {
         Node n;
         n.leaf = true;
         n.freq = freq[key];
         n.c = key - mn;
         pool.push_back(n);
-        heap.insert((int) pool.size() - 1);
+        heap.insert((int) pool.size() - 1, n.freq);
     }
     // qinsert(new_node(0, 0, qremove(), qremove())) in the reference's g++
     // build evaluates the arguments right to left: the first node removed is
@@ -222,7 +237,7 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
         n.right = right;
         n.freq = pool[left].freq + pool[right].freq;
         pool.push_back(n);
-        heap.insert((int) pool.size() - 1);
+        heap.insert((int) pool.size() - 1, n.freq);
     }
     const int root = heap.top();
 
