@@ -797,23 +797,59 @@ faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, in
 
 // Packed code table of t in ctx->h_tab (one entry per symbol: codeword
 // right-aligned above an 8-bit length) and the LDS window of the encoder.
-static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow &ew) {
-    ctx->h_tab.resize((size_t) t->n_states);
+// The packed code table (ctx->h_tab) and its LDS window.  Only the window
+// (the present symbols other than 0) and symbol 0's entry are filled when the
+// window fits: the kernels then read nothing else, and *upload_lo/_n name the
+// entries to copy.  *maxbits = the longest codeword.
+static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow &ew, uint64_t *maxbits,
+                        size_t *upload_lo, size_t *upload_n) {
+    const int32_t ns = t->n_states;
+    const int64_t zi = (t->offset <= 0 && -t->offset < ns) ? -t->offset : -1;  // symbol 0's index
     int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
-    for (int32_t i = 0; i < t->n_states; i++) {
+    uint32_t maxlen = 0;
+    int32_t i = 0;
+    for (; i + 8 <= ns; i += 8) {  // most lengths are zero: eight at a time
+        uint64_t w;
+        memcpy(&w, t->len + i, 8);
+        if (!w) continue;
+        for (int32_t j = i; j < i + 8; j++) {
+            const uint32_t L = t->len[j];
+            if (!L) continue;
+            maxlen = std::max(maxlen, L);
+            if (j != zi) {
+                if (lo < 0) lo = j;
+                hi = j;
+            }
+        }
+    }
+    for (; i < ns; i++) {
         const uint32_t L = t->len[i];
-        if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
-        ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
-        if (L && i + t->offset != 0) {
+        if (!L) continue;
+        maxlen = std::max(maxlen, L);
+        if (i != zi) {
             if (lo < 0) lo = i;
             hi = i;
         }
     }
+    if (maxlen > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", maxlen);
+    *maxbits = maxlen;
+    auto pack = [&](int64_t k) {
+        const uint32_t L = t->len[k];
+        return L ? ((t->code[k] >> (64 - L)) << 8) | L : (uint64_t) 0;
+    };
+    ctx->h_tab.resize((size_t) ns);
     ew = EncodeWindow{0, 0, 0};
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
+        for (int64_t k = lo; k <= hi; k++) ctx->h_tab[k] = pack(k);
         ew.win = (uint32_t) lo;
         ew.nwin = (uint32_t) (hi - lo + 1);
-        ew.e0 = (t->offset <= 0 && -t->offset < t->n_states) ? ctx->h_tab[-t->offset] : 0;
+        ew.e0 = zi >= 0 ? pack(zi) : 0;
+        *upload_lo = (size_t) lo;
+        *upload_n = (size_t) (hi - lo + 1);
+    } else {
+        for (int64_t k = 0; k < ns; k++) ctx->h_tab[k] = pack(k);
+        *upload_lo = 0;
+        *upload_n = (size_t) ns;
     }
     return FAZ_OK;
 }
@@ -832,10 +868,10 @@ static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, c
     hipStream_t st = ctx->stream;
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     EncodeWindow ew;
-    int rc = encode_table(ctx, t, ew);
-    if (rc) return rc;
     uint64_t maxbits = 0;
-    for (int32_t i = 0; i < t->n_states; i++) maxbits = std::max<uint64_t>(maxbits, t->len[i]);
+    size_t up_lo = 0, up_n = 0;
+    int rc = encode_table(ctx, t, ew, &maxbits, &up_lo, &up_n);
+    if (rc) return rc;
     const size_t cap = (size_t) ((n * maxbits + 31) / 32 + 2);
     HIPCHECK(ctx->words.ensure(cap));
     HIPCHECK(ctx->enc_code.ensure((size_t) t->n_states));
@@ -843,7 +879,7 @@ static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, c
     HIPCHECK(ctx->chunk_off.ensure(nchunks));
     HIPCHECK(ctx->scalars.ensure(4));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
-    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p, ctx->h_tab.data(), (size_t) t->n_states * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ctx->enc_code.p + up_lo, ctx->h_tab.data() + up_lo, up_n * 8, hipMemcpyHostToDevice, st));
     launch_encode_sizes(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_bits.p, st);
     launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
     hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned) std::min<size_t>((cap + 255) / 256, 8192)), dim3(256), 0, st,
