@@ -462,7 +462,7 @@ faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n) {
 faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, float *h_min, float *h_max) {
     if (!ctx || !d_in || !n || !h_min || !h_max) return FAZ_ERR_INVALID;
     HIPCHECK(hipSetDevice(ctx->device));
-    HIPCHECK(ctx->minmax.ensure(2));
+    HIPCHECK(ctx->minmax.ensure(kRangeScratch));
     launch_range(d_in, n, ctx->minmax.p, ctx->stream);
     HIPCHECK(hipGetLastError());
     uint32_t *k = ctx->hp_hist.data() + 2 * 65536 + 6;  // pinned
@@ -1424,7 +1424,7 @@ faz_status faz_gpu_batch_create(faz_gpu_ctx *ctx, const faz_interp_params *field
         k->tile3d = ctx->tile3d, k->fused3d = ctx->fused3d, k->tile_grid = ctx->tile_grid;
         k->fused_min_blocks = ctx->fused_min_blocks;
         b->kid.push_back(k);
-        if (k->minmax.ensure(2) != hipSuccess) return fail(FAZ_ERR_HIP);
+        if (k->minmax.ensure(kRangeScratch) != hipSuccess) return fail(FAZ_ERR_HIP);
     }
     for (auto &e: b->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(FAZ_ERR_HIP);

@@ -121,6 +121,10 @@ void launch_interp_tile3d(const Level3D &L, const float *d_in, uint16_t *d_codes
 // Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
 void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
                            uint32_t *d_err, hipStream_t st);
+// Value range of n floats into d_minmax[0..1] (order-preserving keys); the
+// buffer holds kRangeScratch words: the pair, then per-workgroup partials.
+constexpr unsigned kRangeGrid = 512;
+constexpr size_t kRangeScratch = 2 + 2 * (size_t) kRangeGrid;
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st);
 void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
                         const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st);

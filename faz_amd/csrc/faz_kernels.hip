@@ -300,18 +300,25 @@ __device__ __forceinline__ uint32_t f2key(float f) {
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in, size_t n, uint32_t *minmax) {
+// Value range: each workgroup writes its (min, max) keys to the partial slots
+// after the result pair, and one small workgroup folds them.  (Same-address
+// atomics from every workgroup serialised in the L2 and cost a quarter of
+// the pass; kRangeGrid workgroups of four float4 loads per lane in flight
+// read 512 MiB at about 6 TB/s.)
+__global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in, size_t n,
+                                                   uint32_t *__restrict__ part) {
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-    size_t stride = (size_t) gridDim.x * blockDim.x;
-    size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
+    const size_t stride = (size_t) gridDim.x * blockDim.x;
+    const size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
     const float4 *in4 = reinterpret_cast<const float4 *>(in);
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 7 * stride < n4; i += 8 * stride) {
-        float4 v[8];
+    constexpr int U = 4;
+    for (; i + (U - 1) * stride < n4; i += U * stride) {
+        float4 v[U];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = in4[i + u * stride];
+        for (int u = 0; u < U; u++) v[u] = in4[i + u * stride];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < U; u++) {
             uint32_t a = f2key(v[u].x), b = f2key(v[u].y), c = f2key(v[u].z), e = f2key(v[u].w);
             lo = min(lo, min(min(a, b), min(c, e)));
             hi = max(hi, max(max(a, b), max(c, e)));
@@ -323,8 +330,8 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
         lo = min(lo, min(min(a, b), min(c, e)));
         hi = max(hi, max(max(a, b), max(c, e)));
     }
-    for (size_t i = n4 * 4 + (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t a = f2key(in[i]);
+    for (size_t j = n4 * 4 + (size_t) blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        uint32_t a = f2key(in[j]);
         lo = min(lo, a);
         hi = max(hi, a);
     }
@@ -333,7 +340,7 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
         hi = max(hi, (uint32_t) __shfl_xor((int) hi, o));
     }
     __shared__ uint32_t slo[4], shi[4];
-    int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     if (l == 0) {
         slo[w] = lo;
         shi[w] = hi;
@@ -344,21 +351,44 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
             lo = min(lo, slo[k]);
             hi = max(hi, shi[k]);
         }
-        atomicMin(&minmax[0], lo);
-        atomicMax(&minmax[1], hi);
+        part[2 * blockIdx.x] = lo;
+        part[2 * blockIdx.x + 1] = hi;
     }
 }
 
-__global__ void init_minmax_kernel(uint32_t *minmax) {
-    minmax[0] = 0xFFFFFFFFu;
-    minmax[1] = 0u;
+__global__ void __launch_bounds__(256) range_fold_kernel(const uint32_t *__restrict__ part, uint32_t np,
+                                                        uint32_t *__restrict__ minmax) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
+        lo = min(lo, part[2 * i]);
+        hi = max(hi, part[2 * i + 1]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t) __shfl_xor((int) lo, o));
+        hi = max(hi, (uint32_t) __shfl_xor((int) hi, o));
+    }
+    __shared__ uint32_t slo[4], shi[4];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        slo[w] = lo;
+        shi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; k++) {
+            lo = min(lo, slo[k]);
+            hi = max(hi, shi[k]);
+        }
+        minmax[0] = lo;
+        minmax[1] = hi;
+    }
 }
 
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st) {
-    hipLaunchKernelGGL(init_minmax_kernel, dim3(1), dim3(1), 0, st, d_minmax);
-    size_t want = (n / 4 + 255) / 256;
-    unsigned grid = (unsigned) (want < 2048 ? (want ? want : 1) : 2048);
-    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax);
+    const size_t want = (n / 4 + 255) / 256;
+    const unsigned grid = (unsigned) std::min<size_t>(std::max<size_t>(want, 1), kRangeGrid);
+    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax + 2);
+    hipLaunchKernelGGL(range_fold_kernel, dim3(1), dim3(256), 0, st, d_minmax + 2, grid, d_minmax);
 }
 
 // ---------------------------------------------------------------------------
