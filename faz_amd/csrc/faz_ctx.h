@@ -82,6 +82,15 @@ inline hipError_t spin_wait(hipStream_t st) {
     return hipStreamSynchronize(st);
 }
 
+// The same polling wait on one event (work enqueued after it keeps running).
+inline hipError_t spin_wait_event(hipEvent_t ev) {
+    for (int i = 0; i < 2000000; i++) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e != hipErrorNotReady) return e;
+    }
+    return hipEventSynchronize(ev);
+}
+
 }  // namespace faz
 
 struct faz_gpu_ctx {

@@ -672,9 +672,12 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     uint64_t n_unpred = 0;
     if (enqueue || defer) {
         // outliers gathered into one slot per code: no wait for their count
+        // (deferred: the caller enqueues the gather after its readback, so
+        // it runs while the host builds the tree)
         if (defer) HIPCHECK(ctx->unpred.ensure(pl.total_codes));
-        launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
-                             ctx->unpred.p, st);
+        if (enqueue)
+            launch_unpred_gather(ctx->codes.p, pl.total_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p,
+                                 ctx->unpred.p, st);
         HIPCHECK(hipGetLastError());
         if (enqueue) {
             out->d_codes = ctx->codes.p;
@@ -977,7 +980,12 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hs + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(spin_wait(st));
+    HIPCHECK(hipEventRecord(ctx->ev[3], st));
+    // the outlier gather runs while the host waits for the counts and builds the tree
+    launch_unpred_gather(io.d_codes, io.n_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p,
+                         st);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(spin_wait_event(ctx->ev[3]));
     const uint32_t errflag = hs[2];
     if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
