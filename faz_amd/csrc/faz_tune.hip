@@ -112,13 +112,14 @@ __global__ void __launch_bounds__(kTuneThreads) tune_kernel(TuneGeom G, const Tu
     const uint64_t org = origins[it.block];
     // stage the block
     for (uint32_t q = tid; q < npts; q += blockDim.x) {
-        uint32_t r = q, x[3] = {0, 0, 0};
-        for (int a = G.N - 1; a >= 0; a--) {
-            x[a] = r % G.bd[a];
+        uint32_t r = q;
+        uint64_t f = org;
+#pragma unroll
+        for (int a = 2; a >= 0; a--) {
+            if (a >= G.N) continue;
+            f += (uint64_t) (r % G.bd[a]) * G.fstr[a];
             r /= G.bd[a];
         }
-        uint64_t f = org;
-        for (int a = 0; a < G.N; a++) f += (uint64_t) x[a] * G.fstr[a];
         s_v[q] = field[f];
     }
     __syncthreads();
@@ -168,17 +169,26 @@ __global__ void __launch_bounds__(kTuneThreads) tune_kernel(TuneGeom G, const Tu
         const int *perm = G.perm[C.dir[li]];
         const double eb = C.eb[li], recip = C.recip[li];
         const float ebf = C.ebf[li];
+        // per-axis arrays are indexed by unrolled natural axes only (pick3
+        // selects by a run-time axis), so they stay in registers
+        auto pick3 = [](const uint32_t *arr, int ax) { return ax == 0 ? arr[0] : (ax == 1 ? arr[1] : arr[2]); };
         uint32_t nb[3] = {1, 1, 1};
-        for (int a = 0; a < G.N; a++) nb[a] = (G.bd[a] - 1) / B + 1;
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            if (a < G.N) nb[a] = (G.bd[a] - 1) / B + 1;
         const uint32_t nblk = nb[0] * nb[1] * nb[2];
         for (uint32_t blk = 0; blk < nblk; blk++) {
-            uint32_t kb[3], r = blk;
-            for (int a = G.N - 1; a >= 0; a--) {
+            uint32_t kb[3] = {0, 0, 0}, r = blk;
+#pragma unroll
+            for (int a = 2; a >= 0; a--) {
+                if (a >= G.N) continue;
                 kb[a] = r % nb[a];
                 r /= nb[a];
             }
-            uint32_t b[3], nn[3], g1[3], g2[3];
-            for (int a = 0; a < G.N; a++) {
+            uint32_t b[3] = {0, 0, 0}, nn[3] = {1, 1, 1}, g1[3] = {1, 1, 1}, g2[3] = {1, 1, 1};
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if (a >= G.N) continue;
                 b[a] = kb[a] * B;
                 const uint32_t e = min(b[a] + B, G.bd[a] - 1);
                 nn[a] = (e - b[a]) / s + 1;
@@ -188,31 +198,31 @@ __global__ void __launch_bounds__(kTuneThreads) tune_kernel(TuneGeom G, const Tu
             double blk_err = 0.0;  // thread 0's block accumulator (SIC:3143-3185)
             for (int P = 0; P < G.N; P++) {
                 const int ax = perm[P];
-                const uint32_t n = nn[ax], per_line = n / 2;
+                const uint32_t n = pick3(nn, ax), per_line = n / 2;
                 // outer axes in permutation order: G1 (stride s) before the
                 // pass axis, G2 (stride 2s) after it (SIC:3153-3176)
-                uint32_t size[3], axis_of[3], step[3], nq = 0;
                 uint32_t nlines = 1;
-                for (int Q = 0; Q < G.N; Q++) {
-                    if (Q == P) continue;
-                    const int oa = perm[Q];
-                    axis_of[nq] = oa;
-                    size[nq] = Q < P ? g1[oa] : g2[oa];
-                    step[nq] = Q < P ? s : 2 * s;
-                    nlines *= size[nq];
-                    nq++;
+#pragma unroll
+                for (int Q = 0; Q < 3; Q++) {
+                    if (Q >= G.N || Q == P) continue;
+                    nlines *= Q < P ? pick3(g1, perm[Q]) : pick3(g2, perm[Q]);
                 }
                 for (uint32_t line = tid; line < nlines; line += blockDim.x) {
                     uint32_t rr = line;
                     int f = 0;
-                    for (int q = (int) nq - 1; q >= 0; q--) {  // earlier permutation positions outer
-                        const uint32_t idx = rr % size[q];
-                        rr /= size[q];
-                        const int oa = axis_of[q];
-                        const uint32_t xc = b[oa] ? b[oa] + (idx + 1) * step[q] : idx * step[q];
+#pragma unroll
+                    for (int Q = 2; Q >= 0; Q--) {  // earlier permutation positions outer
+                        if (Q >= G.N || Q == P) continue;
+                        const int oa = perm[Q];
+                        const uint32_t size = Q < P ? pick3(g1, oa) : pick3(g2, oa);
+                        const uint32_t step = Q < P ? s : 2 * s;
+                        const uint32_t idx = rr % size;
+                        rr /= size;
+                        const uint32_t bo = pick3(b, oa);
+                        const uint32_t xc = bo ? bo + (idx + 1) * step : idx * step;
                         f += (int) (xc * G.bstr[oa]);
                     }
-                    f += (int) (b[ax] * G.bstr[ax]);
+                    f += (int) (pick3(b, ax) * G.bstr[ax]);
                     const int S = (int) (s * G.bstr[ax]);
                     double err = 0.0;
                     const uint64_t lbase = cbase + seq + (uint64_t) line * per_line;
