@@ -141,7 +141,7 @@ struct faz_gpu_ctx {
     bool tile3d = true;
     uint32_t tile_grid = 256;
     // levels with fewer blocks than this run as per-pass launches
-    uint64_t fused_min_blocks = 256;
+    uint64_t fused_min_blocks = 64;
     // Lorenzo/regression engine: per-block selector, coefficients, regression
     // ranks, the in-block hyperplane order table and its offsets
     faz::DevBuf<uint8_t> lr_ind;
