@@ -642,6 +642,12 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             }
         }
     }
+    // with level launches in the plan the profile times those only: the pass
+    // launches of the coarse levels are a few microseconds each, and event
+    // records around them would slow their enqueue below the device's pace
+    bool any_level = false;
+    for (size_t i = 0; i < nlv; i++) any_level |= kind[i] != 0;
+    const bool prof_pass = profile && !any_level;
     size_t k = 0;
     for (size_t i = 0; i < nlv; i++) {
         LevelPlan &L = pl.lv[i];
@@ -657,9 +663,9 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         }
         for (auto &d: L.passes) {
             d.block_prefix = ctx->prefix.p + L.prefix_at;
-            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
             launch_interp_pass(d, d_in, d_recon, ctx->codes.p, ctx->outval.p, nullptr, ctx->err_flag.p, st);
-            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+            if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
             k++;
         }
     }
@@ -732,7 +738,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             }
             for (auto &d: L.passes) {
                 // original in, reconstruction and code out
-                if (d.total) ctx->prof_pending.push_back({1, (uint32_t) j, d.total, 10.0 * d.total});
+                if (d.total && prof_pass) ctx->prof_pending.push_back({1, (uint32_t) j, d.total, 10.0 * d.total});
                 j++;
             }
         }
