@@ -425,12 +425,16 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
     uint32_t zeros_total = 0;
     const int lane = threadIdx.x % 64;
     const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
+    // 8-bit counters of hot0..hot0+7, hot0+8..hot0+15, folded every kFoldChunks
+    // chunks (a thread adds at most kHistPerThread codes per chunk)
+    constexpr uint32_t kFoldChunks = 255 / kHistPerThread;
+    uint64_t hot_a = 0, hot_b = 0;
+    uint32_t unfolded = 0;
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         if (threadIdx.x == 0) s_zero = 0;
         __syncthreads();
         const uint32_t base = c * kHistChunk;
         const uint32_t end = min(base + kHistChunk, n);
-        uint64_t hot_a = 0, hot_b = 0;  // 8-bit counters of hot0..hot0+7, hot0+8..hot0+15 (this chunk)
         // the thread's 8 groups of 8 consecutive codes (16-byte loads), all
         // issued before any is used
         constexpr int kGroups = kHistPerThread / 8;
@@ -482,12 +486,18 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
                 }
             }
         }
-        // fold this chunk's register counters: wave reduction, one LDS add per wave
+        // fold the register counters every kFoldChunks chunks and after the
+        // workgroup's last chunk (the same for all its threads): wave
+        // reduction, one LDS add per wave
+        if (++unfolded == kFoldChunks || c + gridDim.x >= nchunks) {
 #pragma unroll
-        for (int d = 0; d < kHot; d++) {
-            uint32_t cnt = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
-            for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t) __shfl_xor((int) cnt, o);
-            if (lane == 0 && cnt) atomicAdd(&s_cnt[hot0 + d - lo], cnt);
+            for (int d = 0; d < kHot; d++) {
+                uint32_t cnt = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
+                for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t) __shfl_xor((int) cnt, o);
+                if (lane == 0 && cnt) atomicAdd(&s_cnt[hot0 + d - lo], cnt);
+            }
+            hot_a = hot_b = 0;
+            unfolded = 0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
