@@ -165,6 +165,8 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint64_t> dec_wtab;  // multi-codeword output table
     faz::DevBuf<uint64_t> dec_pos;
     uint32_t dec_rounds = 0;  // chunk groups of the last decode
+    hipStream_t ustream = nullptr;  // uploads of the decoder (default priority)
+    hipEvent_t dec_ev[4] = {};
 };
 
 #define CTX_FAIL(code, ...)                                          \

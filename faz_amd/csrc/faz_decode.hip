@@ -33,6 +33,8 @@ namespace {
 
 constexpr int kTabBits = 12;
 constexpr uint32_t kChunkBits = 1024;
+// parts of the stream uploaded one after another, each followed by its map launch
+constexpr int kUploadParts = 4;
 constexpr uint32_t kDecThreads = 256;
 // stream words a workgroup stages: its chunks, then a margin for the last
 // codeword (at most 64 bits) and the 64-bit window read past it
@@ -230,14 +232,15 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
                                                                const uint32_t *__restrict__ gtab,
                                                                const uint32_t *__restrict__ gctab, Tree T, uint32_t K,
                                                                uint8_t *__restrict__ xmap,
-                                                               uint16_t *__restrict__ nmap) {
+                                                               uint16_t *__restrict__ nmap, uint32_t wg0) {
     __shared__ uint32_t tab[1 << kTabBits];
     __shared__ uint32_t ctab[1 << kTabBits];
     __shared__ uint32_t lw[kStageSlots];
-    const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
+    const uint32_t wg = blockIdx.x + wg0;  // launches cover consecutive ranges of workgroups
+    const uint64_t base = (uint64_t) wg * kGroupWords;
     for (uint32_t i = threadIdx.x; i < (1u << kTabBits); i += blockDim.x) ctab[i] = gctab[i];
     stage(tab, lw, gtab, words, nwords, base);
-    const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
+    const uint32_t c = wg * kDecThreads + threadIdx.x;
     if (c >= nchunks) return;
     BitReader proto;
     proto.lw = lw, proto.base = base, proto.words = words, proto.nwords = nwords;
@@ -678,9 +681,10 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     HIPCHECK(ctx->dec_n.ensure(nmaps));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->scalars.ensure(4));
-    // the stream, its last word zero-padded
-    launch_fill_u32(ctx->dec_words.p + (nwords ? nwords - 1 : 0), 2, 0u, st);
-    if (enc) HIPCHECK(hipMemcpyAsync(ctx->dec_words.p, buf + at, enc, hipMemcpyHostToDevice, st));
+    // the stream (its last word zero-padded) goes up on the upload stream in
+    // parts, below, so that the map launches overlap the later parts' copies
+    hipStream_t us = ctx->ustream;
+    launch_fill_u32(ctx->dec_words.p + (nwords ? nwords - 1 : 0), 2, 0u, us);
     uint32_t *d_tab = ctx->dec_tab.p;
     int32_t *d_tree = reinterpret_cast<int32_t *>(ctx->dec_tab.p + tab.size());
     HIPCHECK(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
@@ -697,10 +701,30 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nchunks;
     uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nchunks;
     const dim3 grid((nchunks + kDecThreads - 1) / kDecThreads), block(kDecThreads);
-    // pass 1: entry -> exit maps and counts
-    hipLaunchKernelGGL(huff_map_kernel, grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks, d_tab, d_ctab,
-                       T, K, ctx->dec_x.p, ctx->dec_n.p);
-    HIPCHECK(hipGetLastError());
+    // pass 1: entry -> exit maps and counts, part by part as the stream
+    // arrives (the last workgroup of a part stages a few words past its
+    // slice, which that part's copy includes)
+    {
+        const uint32_t nwg = grid.x;
+        const int parts = nwg >= 64 ? kUploadParts : 1;
+        uint64_t sent = 0;
+        uint8_t *dst = reinterpret_cast<uint8_t *>(ctx->dec_words.p);
+        for (int k = 0; k < parts; k++) {
+            const uint32_t w0 = (uint32_t) ((uint64_t) nwg * k / parts);
+            const uint32_t w1 = (uint32_t) ((uint64_t) nwg * (k + 1) / parts);
+            uint64_t need = k + 1 == parts ? enc : std::min<uint64_t>(enc, ((uint64_t) w1 * kGroupWords + kStageWords) * 4);
+            if (need > sent) {
+                HIPCHECK(hipMemcpyAsync(dst + sent, buf + at + sent, need - sent, hipMemcpyHostToDevice, us));
+                sent = need;
+            }
+            HIPCHECK(hipEventRecord(ctx->dec_ev[k], us));
+            HIPCHECK(hipStreamWaitEvent(st, ctx->dec_ev[k], 0));
+            if (w1 > w0)
+                hipLaunchKernelGGL(huff_map_kernel, dim3(w1 - w0), block, 0, st, ctx->dec_words.p, nwords, nbits,
+                                   nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0);
+            HIPCHECK(hipGetLastError());
+        }
+    }
     // pass 2: compositions inside groups of chunks, the groups' entries,
     // then every chunk's entry and count
     const uint32_t ngroups = (nchunks + kDecThreads - 1) / kDecThreads;

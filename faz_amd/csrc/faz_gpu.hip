@@ -395,6 +395,19 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
         delete ctx;
         return FAZ_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->ustream, hipStreamNonBlocking) != hipSuccess) {
+        (void) hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return FAZ_ERR_HIP;
+    }
+    for (auto &e: ctx->dec_ev) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            (void) hipStreamDestroy(ctx->ustream);
+            (void) hipStreamDestroy(ctx->stream);
+            delete ctx;
+            return FAZ_ERR_HIP;
+        }
+    }
     for (auto &e: ctx->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             delete ctx;
@@ -445,7 +458,10 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     for (auto &e: ctx->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e: ctx->prof_ev) (void) hipEventDestroy(e);
+    for (auto &e: ctx->dec_ev)
+        if (e) (void) hipEventDestroy(e);
     (void) hipStreamDestroy(ctx->stream);
+    if (ctx->ustream) (void) hipStreamDestroy(ctx->ustream);
     delete ctx;
 }
 
