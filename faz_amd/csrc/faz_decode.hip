@@ -10,13 +10,15 @@
 // lands on a codeword start of an earlier trajectory (a Huffman decoder falls
 // into step within a few codewords, except inside long runs of one codeword,
 // where two or three phases persist and are decoded in full).  Pass 2 composes
-// the maps with a parallel prefix (Hillis-Steele, log2(chunks) steps) to get
-// every e_c from e_0 = 0; the counts at the true entries are scanned into
-// output offsets, and pass 3 decodes each chunk again, writing its symbols.
-// Every pass stages its workgroup's slice of the stream in LDS.
+// the maps (a parallel prefix inside each group of kDecThreads chunks, then
+// one walk over the groups) to get every e_c from e_0 = 0; the counts at the
+// true entries are scanned into output offsets, and pass 3 decodes each chunk
+// again, writing its symbols.  Passes 1 and 3 stage their workgroup's slice
+// of the stream in LDS.
 //
-// Codewords of up to kTabBits bits come from one table lookup (LDS); longer
-// ones continue bit by bit down the tree from the node the table names.
+// Codewords of up to kTabBits bits come from one table lookup (LDS), longer
+// ones through second-level tables of kSubBits bits; multi-codeword tables
+// count (pass 1) and write (pass 3) every whole codeword of a window at once.
 #include "faz_gpu.h"
 #include "faz_ctx.h"
 #include "faz_internal.h"
