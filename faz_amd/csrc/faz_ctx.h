@@ -163,6 +163,7 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint8_t> dec_leaf, dec_x, dec_g;
     faz::DevBuf<uint16_t> dec_n;
     faz::DevBuf<uint64_t> dec_wtab;  // multi-codeword output table
+    bool hist_cleared = false;       // freq/first already cleared on the stream for this compression
     faz::DevBuf<uint64_t> dec_pos;
     uint32_t dec_rounds = 0;  // chunk groups of the last decode
     hipStream_t ustream = nullptr;  // uploads of the decoder (default priority)

@@ -354,6 +354,23 @@ double elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// The (min, max) pair of the last range pass on ctx->stream, through pinned
+// memory.
+int range_readback(faz_gpu_ctx *ctx, float *h_min, float *h_max) {
+    uint32_t *k = ctx->hp_hist.data() + 2 * 65536 + 6;  // pinned
+    HIPCHECK(hipMemcpyAsync(k, ctx->minmax.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(spin_wait(ctx->stream));
+    auto key2f = [](uint32_t key) {
+        uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    };
+    *h_min = key2f(k[0]);
+    *h_max = key2f(k[1]);
+    return FAZ_OK;
+}
+
 // Adds the recorded launches of the last compression to the profile buckets
 // (their events must have completed).
 void flush_profile(faz_gpu_ctx *ctx) {
@@ -481,18 +498,7 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
     HIPCHECK(ctx->minmax.ensure(kRangeScratch));
     launch_range(d_in, n, ctx->minmax.p, ctx->stream);
     HIPCHECK(hipGetLastError());
-    uint32_t *k = ctx->hp_hist.data() + 2 * 65536 + 6;  // pinned
-    HIPCHECK(hipMemcpyAsync(k, ctx->minmax.p, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(spin_wait(ctx->stream));
-    auto key2f = [](uint32_t key) {
-        uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
-        float f;
-        memcpy(&f, &u, 4);
-        return f;
-    };
-    *h_min = key2f(k[0]);
-    *h_max = key2f(k[1]);
-    return FAZ_OK;
+    return range_readback(ctx, h_min, h_max);
 }
 
 }  // extern "C"
@@ -688,7 +694,9 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(hipGetLastError());
     if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[1], st));
     // histogram, first positions and outlier compaction
-    launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st);
+    const bool cleared = ctx->hist_cleared && ctx->freq.cap >= 65536 && ctx->first.cap >= 65536;
+    ctx->hist_cleared = false;
+    launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st, cleared);
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     HIPCHECK(hipGetLastError());
     uint64_t n_unpred = 0;
@@ -1113,8 +1121,16 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         HIPCHECK(hipSetDevice(ctx->device));
         HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
         float mn, mx;
-        int rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx);
+        // the histogram's arrays are cleared while the host waits for the range
+        HIPCHECK(ctx->minmax.ensure(kRangeScratch));
+        HIPCHECK(ctx->freq.ensure(65536));
+        HIPCHECK(ctx->first.ensure(65536));
+        launch_range(d_in, num, ctx->minmax.p, ctx->stream);
+        launch_hist_clear(ctx->freq.p, ctx->first.p, ctx->stream);
+        HIPCHECK(hipGetLastError());
+        int rc = range_readback(ctx, &mn, &mx);
         if (rc) return rc;
+        ctx->hist_cleared = true;
         HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
         q.eb = eb * (double) (float) (mx - mn);
         absmax = std::max(std::fabs((double) mn), std::fabs((double) mx));
@@ -1122,6 +1138,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         q.eb = eb;
     }
     int rc = compress_device(ctx, d_in, d_recon, &q, absmax);
+    ctx->hist_cleared = false;
     if (rc) return rc;
     if (rel_mode) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
     *raw_size = pending_size(ctx);

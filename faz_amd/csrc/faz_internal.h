@@ -136,8 +136,10 @@ void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, ui
                         double *d_loss, uint32_t *d_err, hipStream_t st);
 void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_codes, const float *d_unpred_at,
                          hipStream_t st);
+// cleared: launch_hist_clear already ran on st for these arrays
 void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
-                 int radius, hipStream_t st);
+                 int radius, hipStream_t st, bool cleared = false);
+void launch_hist_clear(uint32_t *d_freq, uint32_t *d_first, hipStream_t st);
 void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
                       uint64_t *d_total, hipStream_t st);
 void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,

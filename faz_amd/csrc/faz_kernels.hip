@@ -531,10 +531,14 @@ void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st) 
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(256), 0, st, d, count, value);
 }
 
-void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
-                 int radius, hipStream_t st) {
+void launch_hist_clear(uint32_t *d_freq, uint32_t *d_first, hipStream_t st) {
     launch_fill_u32(d_freq, 65536, 0u, st);
     launch_fill_u32(d_first, 65536, 0xFFFFFFFFu, st);
+}
+
+void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
+                 int radius, hipStream_t st, bool cleared) {
+    if (!cleared) launch_hist_clear(d_freq, d_first, st);
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
     // window [lo, lo + kWin) inside 1..65535, hot symbols inside the window
     int lo = radius - kWin / 2;
