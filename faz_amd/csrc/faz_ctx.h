@@ -102,6 +102,7 @@ struct faz_gpu_ctx {
     // compact lattices handed from one whole-tile level to the next
     faz::DevBuf<float> lat[2];
     faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
+    faz::DevBuf<uint32_t> hist_pack;  // one readback block after the histogram
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to

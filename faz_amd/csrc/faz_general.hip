@@ -647,7 +647,7 @@ int gf_raw(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_genera
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     hipLaunchKernelGGL(gf_narrow_kernel, dim3((uint32_t) std::min<uint64_t>((n + 255) / 256, 8192)), dim3(256), 0,
                        st, ctx->gf_codes.p, ctx->codes.p, n);

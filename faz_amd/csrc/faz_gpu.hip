@@ -447,6 +447,7 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->lat[0].release();
     ctx->lat[1].release();
     ctx->freq.release();
+    ctx->hist_pack.release();
     ctx->first.release();
     ctx->chunk_zeros.release();
     ctx->err_flag.release();
@@ -539,7 +540,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     const uint32_t nchunks = (uint32_t) ((pl.total_codes + kHistChunk - 1) / kHistChunk);
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     if (!enqueue && (ctx->prefix_key != prefix_key || ctx->prefix_at != ctx->prefix.p)) {
@@ -694,7 +695,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(hipGetLastError());
     if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[1], st));
     // histogram, first positions and outlier compaction
-    const bool cleared = ctx->hist_cleared && ctx->freq.cap >= 65536 && ctx->first.cap >= 65536;
+    const bool cleared = ctx->hist_cleared && ctx->freq.cap >= kHistSlots && ctx->first.cap >= 65536;
     ctx->hist_cleared = false;
     launch_hist(ctx->codes.p, pl.total_codes, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, p->radius, st, cleared);
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
@@ -798,7 +799,7 @@ faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_
     if (!(ctx->hist_codes == d_codes && ctx->hist_n == n)) {
         const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
         HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
-        HIPCHECK(ctx->freq.ensure(65536));
+        HIPCHECK(ctx->freq.ensure(kHistSlots));
         HIPCHECK(ctx->first.ensure(65536));
         launch_hist(d_codes, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
         HIPCHECK(hipGetLastError());
@@ -1005,25 +1006,35 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     faz_interp_out io;
     int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax, false, true);
     if (rc) return rc;
+    // one readback: the histogram's window around the radius, symbol 0, the
+    // outside-window flag, the error flag and the outlier count
     uint32_t *hf = ctx->hp_hist.data(), *hs = ctx->hp_hist.data() + 2 * 65536;
-    HIPCHECK(hipMemcpyAsync(hf, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(ctx->hist_pack.ensure(kHistPackWords));
+    launch_hist_pack(ctx->freq.p, ctx->first.p, p->radius, ctx->scalars.p, ctx->err_flag.p, ctx->hist_pack.p, st);
+    HIPCHECK(hipMemcpyAsync(hf, ctx->hist_pack.p, kHistPackWords * 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
     // the outlier gather runs while the host waits for the counts and builds the tree
     launch_unpred_gather(io.d_codes, io.n_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p,
                          st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(spin_wait_event(ctx->ev[3]));
-    const uint32_t errflag = hs[2];
+    const uint32_t errflag = hf[kHistPackFreq0 + 3];
     if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
     uint64_t n_unpred;
-    memcpy(&n_unpred, hs, 8);
+    memcpy(&n_unpred, hf + kHistPackFreq0 + 4, 8);
     PendingRaw &pr = ctx->pending;
-    if (build_huffman(hf, hf + 65536, 65536, pr.tab))
-        CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
+    int brc;
+    if (!hf[kHistPackFreq0 + 2]) {
+        brc = build_huffman_window(hf, hf + kHistWin, hist_window_lo(p->radius), kHistWin, hf[kHistPackFreq0],
+                                   hf[kHistPackFreq0 + 1], pr.tab);
+    } else {  // a symbol outside the window: the whole arrays
+        HIPCHECK(hipMemcpyAsync(hf, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
+        HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
+        HIPCHECK(hipStreamSynchronize(ctx->ustream));
+        brc = build_huffman(hf, hf + 65536, 65536, pr.tab);
+    }
+    if (brc) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
     rc = encode_enqueue(ctx, io.d_codes, io.n_codes, &ht);
@@ -1123,7 +1134,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         float mn, mx;
         // the histogram's arrays are cleared while the host waits for the range
         HIPCHECK(ctx->minmax.ensure(kRangeScratch));
-        HIPCHECK(ctx->freq.ensure(65536));
+        HIPCHECK(ctx->freq.ensure(kHistSlots));
         HIPCHECK(ctx->first.ensure(65536));
         launch_range(d_in, num, ctx->minmax.p, ctx->stream);
         launch_hist_clear(ctx->freq.p, ctx->first.p, ctx->stream);
@@ -1230,7 +1241,7 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     HIPCHECK(ctx->prefix.ensure(pl.prefix.size()));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->scalars.ensure(4));
     // codes decoded on the device (HUF:236-268), then the outliers' ranks

@@ -136,9 +136,21 @@ void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, ui
                         double *d_loss, uint32_t *d_err, hipStream_t st);
 void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_codes, const float *d_unpred_at,
                          hipStream_t st);
-// cleared: launch_hist_clear already ran on st for these arrays
+// d_freq holds kHistSlots words: the 65536 counts, then a flag set when a
+// symbol other than 0 falls outside the window [hist_window_lo(radius),
+// + kHistWin).  cleared: launch_hist_clear already ran on st for the arrays.
+constexpr int kHistWin = 4096;
+constexpr int kHistSlots = 65537;
+int hist_window_lo(int radius);
 void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *d_first, uint32_t *d_chunk_zeros,
                  int radius, hipStream_t st, bool cleared = false);
+// One readback block after the histogram: counts and first positions of the
+// window, then freq[0], first[0], the outside flag, *d_err and the u64 at
+// d_scalar0 (two words).
+constexpr int kHistPackFreq0 = 2 * kHistWin;
+constexpr int kHistPackWords = kHistPackFreq0 + 6;
+void launch_hist_pack(const uint32_t *d_freq, const uint32_t *d_first, int radius, const uint64_t *d_scalar0,
+                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st);
 void launch_hist_clear(uint32_t *d_freq, uint32_t *d_first, hipStream_t st);
 void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
                       uint64_t *d_total, hipStream_t st);

@@ -401,7 +401,7 @@ void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t s
 // global atomics.  First positions: LDS atomicMin guarded by a read, so each
 // symbol costs at most a few atomics per workgroup.
 // ---------------------------------------------------------------------------
-constexpr int kWin = 4096;  // LDS bins around the quantizer radius
+constexpr int kWin = kHistWin;  // LDS bins around the quantizer radius
 constexpr int kHot = 16;    // register-counted symbols [hot0, hot0 + 16)
 static_assert(kHistPerThread <= 255, "8-bit register counters");
 
@@ -411,7 +411,7 @@ static_assert(kHistPerThread <= 255, "8-bit register counters");
 __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                            uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
                                                            uint32_t *__restrict__ chunk_zeros, int lo_,
-                                                           int hot0_, int aligned) {
+                                                           int hot0_, int aligned, uint32_t *__restrict__ outside) {
     __shared__ uint32_t s_cnt[kWin];
     __shared__ uint32_t s_first[kWin];  // first global position per window symbol
     __shared__ uint32_t s_zero, s_zero_first;
@@ -482,6 +482,7 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
                     } else {
                         atomicAdd(&freq[v[k]], 1u);
                         atomicMin(&first[v[k]], gi);
+                        *outside = 1u;
                     }
                 }
             }
@@ -532,7 +533,7 @@ void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st) 
 }
 
 void launch_hist_clear(uint32_t *d_freq, uint32_t *d_first, hipStream_t st) {
-    launch_fill_u32(d_freq, 65536, 0u, st);
+    launch_fill_u32(d_freq, kHistSlots, 0u, st);
     launch_fill_u32(d_first, 65536, 0xFFFFFFFFu, st);
 }
 
@@ -541,9 +542,7 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     if (!cleared) launch_hist_clear(d_freq, d_first, st);
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
     // window [lo, lo + kWin) inside 1..65535, hot symbols inside the window
-    int lo = radius - kWin / 2;
-    if (lo < 1) lo = 1;
-    if (lo + kWin > 65536) lo = 65536 - kWin;
+    const int lo = hist_window_lo(radius);
     int hot0 = radius - kHot / 2;
     if (hot0 < lo) hot0 = lo;
     if (hot0 + kHot > lo + kWin) hot0 = lo + kWin - kHot;
@@ -551,7 +550,37 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     const uint32_t grid = nchunks < 1024 ? nchunks : 1024;
     if (nchunks)
         hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
-                           d_first, d_chunk_zeros, lo, hot0, (int) (((uintptr_t) d_codes & 15) == 0));
+                           d_first, d_chunk_zeros, lo, hot0, (int) (((uintptr_t) d_codes & 15) == 0), d_freq + 65536);
+}
+
+int hist_window_lo(int radius) {
+    int lo = radius - kWin / 2;
+    if (lo < 1) lo = 1;
+    if (lo + kWin > 65536) lo = 65536 - kWin;
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) hist_pack_kernel(const uint32_t *__restrict__ freq,
+                                                        const uint32_t *__restrict__ first, int lo,
+                                                        const uint32_t *__restrict__ scalar0,
+                                                        const uint32_t *__restrict__ err, uint32_t *__restrict__ pack) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t v;
+    if (i < (uint32_t) kWin) v = freq[lo + i];
+    else if (i < 2u * kWin) v = first[lo + i - kWin];
+    else if (i == kHistPackFreq0) v = freq[0];
+    else if (i == kHistPackFreq0 + 1) v = first[0];
+    else if (i == kHistPackFreq0 + 2) v = freq[65536];
+    else if (i == kHistPackFreq0 + 3) v = err[0];
+    else if (i < (uint32_t) kHistPackWords) v = scalar0[i - (kHistPackFreq0 + 4)];
+    else return;
+    pack[i] = v;
+}
+
+void launch_hist_pack(const uint32_t *d_freq, const uint32_t *d_first, int radius, const uint64_t *d_scalar0,
+                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st) {
+    hipLaunchKernelGGL(hist_pack_kernel, dim3((kHistPackWords + 255) / 256), dim3(256), 0, st, d_freq, d_first,
+                       hist_window_lo(radius), reinterpret_cast<const uint32_t *>(d_scalar0), d_err, d_pack);
 }
 
 // ---------------------------------------------------------------------------

@@ -601,7 +601,7 @@ int lr_compress(faz_gpu_ctx *ctx, const float *d_in, float *recon, const faz_lor
     HIPCHECK(ctx->tiles.ensure(std::max(scan_tiles((uint32_t) G.nblocks), scan_tiles(nchunks))));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(hipEventRecord(ctx->ev[0], st));
     LRSel S{p->pred_dim, p->lorenzo != 0, p->lorenzo2 != 0, p->regression != 0, (float) p->eb};
@@ -815,7 +815,7 @@ int lr_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, const size_t *
     HIPCHECK(ctx->outval.ensure(num));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->scalars.ensure(4));
     HIPCHECK(ctx->lr_ind.ensure(G.nblocks));

@@ -368,7 +368,7 @@ int block_interp_test(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     HIPCHECK(ctx->first.ensure(65536));
     launch_hist(ctx->codes.p, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
@@ -456,7 +456,7 @@ int quant_intervals(faz_gpu_ctx *ctx, const float *d_in, const size_t *dims, dou
     if (samples) *samples = n;
     hipStream_t st = ctx->stream;
     HIPCHECK(ctx->chunk_base.ensure(std::max<uint64_t>(n, 1)));
-    HIPCHECK(ctx->freq.ensure(65536));
+    HIPCHECK(ctx->freq.ensure(kHistSlots));
     launch_fill_u32(ctx->freq.p, 32768, 0u, st);
     if (n) {
         HIPCHECK(hipMemcpyAsync(ctx->chunk_base.p, pos.data(), n * 8, hipMemcpyHostToDevice, st));

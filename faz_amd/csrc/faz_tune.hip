@@ -486,7 +486,7 @@ struct Tuner {
         const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
         HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
         HIPCHECK(ctx->chunk_base.ensure(nchunks));
-        HIPCHECK(ctx->freq.ensure(65536));
+        HIPCHECK(ctx->freq.ensure(kHistSlots));
         HIPCHECK(ctx->first.ensure(65536));
         HIPCHECK(ctx->scalars.ensure(4));
         HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));

@@ -186,6 +186,15 @@ void put_be32(std::vector<uint8_t> &out, uint32_t v) {
 
 }  // namespace
 
+namespace {
+
+// The tree of the present symbols: (first position, symbol) pairs and a
+// count lookup by symbol.
+template<class FreqOf>
+int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of, HuffTable &tab);
+
+}  // namespace
+
 int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
     std::vector<std::pair<uint32_t, int>> present;
     present.reserve(1024);
@@ -200,6 +209,23 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
     }
     for (int s = s0; s < nsym; s++)
         if (freq[s]) present.push_back({first[s], s});
+    return build_present(present, [&](int s) { return freq[s]; }, tab);
+}
+
+int build_huffman_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
+                         uint32_t first0, HuffTable &tab) {
+    std::vector<std::pair<uint32_t, int>> present;
+    present.reserve(1024);
+    if (freq0) present.push_back({first0, 0});
+    for (int i = 0; i < nwin; i++)
+        if (wfreq[i]) present.push_back({wfirst[i], lo + i});
+    return build_present(present, [&](int s) { return s == 0 ? freq0 : wfreq[s - lo]; }, tab);
+}
+
+namespace {
+
+template<class FreqOf>
+int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of, HuffTable &tab) {
     if (present.empty()) return -1;
     std::sort(present.begin(), present.end());
     int mn = present[0].second, mx = present[0].second;
@@ -221,7 +247,7 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
     for (int key: order) {
         Node n;
         n.leaf = true;
-        n.freq = freq[key];
+        n.freq = freq_of(key);
         n.c = key - mn;
         pool.push_back(n);
         heap.insert((int) pool.size() - 1, n.freq);
@@ -278,6 +304,8 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
         serialise<uint32_t>(pool, root, nodeCount, tab.tree);
     return 0;
 }
+
+}  // namespace
 
 // HuffmanEncoder::load + decode (HUF:236-296, 310-454)
 void encode_huffman(const HuffTable &tab, const int32_t *s, size_t n, std::vector<uint8_t> &out) {

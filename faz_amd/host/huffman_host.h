@@ -18,6 +18,10 @@ struct HuffTable {
 // symbols 0..nsym-1.  Returns 0, or < 0 for an empty histogram / codes longer
 // than 64 bits.
 int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab);
+// The same when every present symbol but 0 lies in [lo, lo + nwin): counts
+// and first positions of that window, and of symbol 0.
+int build_huffman_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
+                         uint32_t first0, HuffTable &tab);
 
 // HuffmanEncoder::encode (HuffmanEncoder.hpp:150-227) on the host: appends
 // the size_t byte count and the MSB-first bitstream of s[0..n) to out.  Every
