@@ -247,7 +247,8 @@ EXPORTS = [
     "faz_gpu_batch_create", "faz_gpu_batch_destroy", "faz_gpu_batch_compress", "faz_gpu_batch_fetch_raw",
     "faz_gpu_batch_stats", "faz_sz_compress_batch_f32", "faz_gpu_batch_field",
     "faz_gpu_sampling_f32", "faz_gpu_block_interp_test_f32", "faz_gpu_quant_intervals_f32", "faz_gpu_sz3_tune_f32",
-    "faz_sz_compress_sz3_f32", "faz_gpu_huffman_decode",
+    "faz_sz_compress_sz3_f32", "faz_gpu_huffman_decode", "faz_gpu_huffman_encode_noalloc",
+    "faz_gpu_noalloc_scratch_bytes",
 ]
 
 
@@ -315,6 +316,9 @@ def lib():
         L.faz_gpu_interp_compress_f32.argtypes = [vp, vp, vp, P(InterpParams), P(InterpOut)]
         L.faz_gpu_huffman_hist.argtypes = [vp, vp, sz, vp, vp]
         L.faz_gpu_huffman_encode.argtypes = [vp, vp, sz, P(HuffTable), vp, sz, P(sz)]
+        L.faz_gpu_huffman_encode_noalloc.argtypes = [vp, vp, sz, P(HuffTable), vp, sz, P(sz)]
+        L.faz_gpu_noalloc_scratch_bytes.argtypes = []
+        L.faz_gpu_noalloc_scratch_bytes.restype = sz
         L.faz_huffman_build.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_int32), vp, vp, vp, sz, P(sz)]
         L.faz_gpu_interp_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
         L.faz_gpu_huffman_decode.argtypes = [vp, vp, sz, sz, ctypes.c_uint32, vp, P(sz)]
@@ -510,6 +514,21 @@ class Engine:
                                                    self._dev(out_tensor), out_tensor.numel() *
                                                    out_tensor.element_size(), ctypes.byref(nb)),
                     "huffman_encode")
+        return nb.value
+
+    def huffman_encode_noalloc(self, d_codes, n, offset, code, ln, out_tensor):
+        """faz_gpu_huffman_encode_noalloc: the encode with the library's static
+        scratch array (no device allocation); returns the byte count."""
+        t = HuffTable()
+        t.offset = offset
+        t.n_states = code.size
+        t.code = code.ctypes.data
+        t.len = ln.ctypes.data
+        nb = ctypes.c_size_t()
+        self._check(self._L.faz_gpu_huffman_encode_noalloc(self.h, ctypes.c_void_p(d_codes), n, ctypes.byref(t),
+                                                           self._dev(out_tensor), out_tensor.numel() *
+                                                           out_tensor.element_size(), ctypes.byref(nb)),
+                    "huffman_encode_noalloc")
         return nb.value
 
     def interp_raw(self, x, params, recon=None):

@@ -405,6 +405,9 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (kern && strcmp(kern, "pass") == 0) ctx->fused3d = false;
     if (kern && (strcmp(kern, "pass") == 0 || strcmp(kern, "level") == 0)) ctx->tile3d = false;
     if (kern && strcmp(kern, "tile") == 0) ctx->fused_min_blocks = 1;
+    // the allocation-free encoder's scratch address on this device (its
+    // entry point reports a failure here)
+    (void) noalloc_init(device);
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu >= 8)
         ctx->tile_grid = (uint32_t) (ncu / 8 * 8);

@@ -190,6 +190,12 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
 int gpu_scatter_unpred_u32(faz_gpu_ctx *ctx, const uint32_t *d_codes, uint64_t n, const float *h_unpred, uint64_t U,
                            float *d_outval);
 
+// Device address of the allocation-free encoder's scratch array on `device`
+// (hipGetSymbolAddress, once per device; faz_noalloc.hip).  Call with the
+// device current.
+hipError_t noalloc_init(int device);
+size_t noalloc_scratch_bytes();
+
 // Chunk geometry of the histogram / outlier / encode kernels.
 constexpr uint32_t kHistThreads = 256;
 constexpr uint32_t kHistPerThread = 64;

@@ -107,6 +107,21 @@ faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_
 faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, const faz_huff_table *t,
                                   uint8_t *d_bits, size_t cap_bytes, size_t *nbytes);
 
+/* --- faz_gpu_huffman_encode without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, output and errors as faz_gpu_huffman_encode;
+ * its chunk totals, offsets, table and bit total live in one static device
+ * array per GPU (faz_gpu_noalloc_scratch_bytes(), about 5 MB, held until the
+ * process ends) instead of context buffers, and it calls no hipMalloc /
+ * hipFree.  At most 2^30 codes and 65537 table states (FAZ_ERR_INVALID
+ * beyond).  Every call on one GPU shares that GPU's array: calls that overlap
+ * on the same GPU (from any context or thread) race. */
+faz_status faz_gpu_huffman_encode_noalloc(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n,
+                                          const faz_huff_table *t, uint8_t *d_bits, size_t cap_bytes,
+                                          size_t *nbytes);
+size_t faz_gpu_noalloc_scratch_bytes(void);
+
 /* --- A12/A13 host side: HuffmanEncoder::init + save (HuffmanEncoder.hpp:
  * 103-137, 455-633).  From per-symbol counts and first indices (65536 entries
  * each) builds the tree exactly as the reference does and writes the code
