@@ -163,6 +163,12 @@ class TuneIn(ctypes.Structure):
         ("max_step", ctypes.c_size_t),
         ("levelwise", ctypes.c_int),
         ("sample_block_size", ctypes.c_int),
+        ("ab_list", ctypes.c_int),
+        ("test_lorenzo", ctypes.c_int),
+        ("learned_margin", ctypes.c_double),
+        ("learned_verify", ctypes.c_int),
+        ("learned_lo", ctypes.c_void_p),
+        ("learned_hi", ctypes.c_void_p),
     ]
 
 
@@ -190,6 +196,10 @@ class TuneOut(ctypes.Structure):
         ("test_psnr", ctypes.c_double * 128),
         ("test_bytes", ctypes.c_size_t * 128),
         ("ms", ctypes.c_double),
+        ("use_interp", ctypes.c_int),
+        ("learned_path", ctypes.c_int),
+        ("n_verified", ctypes.c_int),
+        ("level_res", ctypes.c_double * 32),
     ]
 
 
@@ -202,7 +212,8 @@ def tune_out_dict(o):
                 pred_loss=np.array([list(o.pred_loss[i]) for i in range(32)]),
                 tests=[dict(alpha=o.test_alpha[i], beta=o.test_beta[i], eb=o.test_eb[i], bitrate=o.test_bitrate[i],
                             psnr=o.test_psnr[i], bytes=o.test_bytes[i]) for i in range(nt)],
-                ms=getattr(o, "ms", 0.0))
+                ms=getattr(o, "ms", 0.0), use_interp=o.use_interp, learned_path=o.learned_path,
+                n_verified=o.n_verified, level_res=np.array(o.level_res[:lw]))
 
 
 def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0, block_size=32,
@@ -248,7 +259,7 @@ EXPORTS = [
     "faz_gpu_batch_stats", "faz_sz_compress_batch_f32", "faz_gpu_batch_field",
     "faz_gpu_sampling_f32", "faz_gpu_block_interp_test_f32", "faz_gpu_quant_intervals_f32", "faz_gpu_sz3_tune_f32",
     "faz_sz_compress_sz3_f32", "faz_gpu_huffman_decode", "faz_gpu_huffman_encode_noalloc",
-    "faz_gpu_noalloc_scratch_bytes",
+    "faz_gpu_noalloc_scratch_bytes", "faz_huffman_check_section", "faz_stream_info",
 ]
 
 
@@ -291,7 +302,7 @@ class BatchStats(ctypes.Structure):
     ]
 
 LEARNED_FEATURES = 40
-LEARNED_INPUTS = 49
+LEARNED_INPUTS = 54
 
 _lib = None
 _hip = None
@@ -319,6 +330,8 @@ def lib():
         L.faz_gpu_huffman_encode_noalloc.argtypes = [vp, vp, sz, P(HuffTable), vp, sz, P(sz)]
         L.faz_gpu_noalloc_scratch_bytes.argtypes = []
         L.faz_gpu_noalloc_scratch_bytes.restype = sz
+        L.faz_huffman_check_section.argtypes = [vp, sz, P(sz), P(sz)]
+        L.faz_stream_info.argtypes = [vp, sz, P(ctypes.c_int), vp, P(ctypes.c_int)]
         L.faz_huffman_build.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_int32), vp, vp, vp, sz, P(sz)]
         L.faz_gpu_interp_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
         L.faz_gpu_huffman_decode.argtypes = [vp, vp, sz, sz, ctypes.c_uint32, vp, P(sz)]
@@ -378,7 +391,7 @@ def lib():
         L.faz_gpu_block_features_f32.argtypes = [vp, vp, ctypes.c_int, vp, vp, sz, sz, ctypes.c_float,
                                                  ctypes.c_float, ctypes.c_double, vp]
         L.faz_learned_input.argtypes = [vp, sz, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                        ctypes.c_double, ctypes.c_int, vp, vp, vp]
+                                        ctypes.c_double, ctypes.c_int, vp, vp, vp, ctypes.c_int, vp]
         L.faz_learned_target.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp]
         L.faz_gpu_tune_learned_f32.argtypes = [vp, vp, P(TuneIn), vp, vp, vp, P(TuneOut)]
         L.faz_sz_compress_qoz_f32.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
@@ -429,6 +442,30 @@ def huffman_build(freq, first):
     if rc:
         raise FazError(rc, "faz_huffman_build")
     return off.value, code[:ns.value].copy(), ln[:ns.value].copy(), tree[:tsz.value].tobytes()
+
+
+def huffman_check_section(section):
+    """faz_huffman_check_section: (node count, section bytes); raises
+    FazError(-5) for a malformed section."""
+    b = np.frombuffer(bytes(section), np.uint8)
+    nodes, used = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = lib().faz_huffman_check_section(_np_ptr(b) if b.size else None, b.size, ctypes.byref(nodes),
+                                         ctypes.byref(used))
+    if rc:
+        raise FazError(rc, "huffman_check_section")
+    return nodes.value, used.value
+
+
+def stream_info(stream):
+    """faz_stream_info: (N, dims, cmprAlgo) of a stream's Config trailer."""
+    b = np.frombuffer(bytes(stream), np.uint8)
+    N, algo = ctypes.c_int(), ctypes.c_int()
+    dims = np.zeros(4, np.uint64)
+    rc = lib().faz_stream_info(_np_ptr(b) if b.size else None, b.size, ctypes.byref(N), _np_ptr(dims),
+                               ctypes.byref(algo))
+    if rc:
+        raise FazError(rc, "stream_info")
+    return N.value, tuple(int(d) for d in dims[:N.value]), algo.value
 
 
 class Engine:
@@ -739,19 +776,28 @@ class Engine:
         finally:
             self._L.faz_free(p)
 
-    def compress_test_lorenzo(self, x, shape, starts, sbs, eb):
-        """CompressTest's Lorenzo variant: (pre-zstd buffer, compressed size, squared error)."""
+    def compress_test_lorenzo(self, x, shape, starts, sbs, eb, want_raw=True):
+        """CompressTest's Lorenzo variant: (pre-zstd buffer or None, compressed size, squared error)."""
         import torch
         torch.cuda.synchronize()
         fd = np.array(list(shape) + [1] * (3 - len(shape)), np.uint64)
         st = np.ascontiguousarray(np.asarray(starts, np.uint64).reshape(-1, 3))
+        rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+
+        def call(buf, cap):
+            return self._L.faz_gpu_compress_test_lorenzo_f32(self.h, self._dev(x), len(shape), _np_ptr(fd),
+                                                             _np_ptr(st), st.shape[0], sbs, float(eb), buf, cap,
+                                                             ctypes.byref(rs), ctypes.byref(by), ctypes.byref(sq))
+        if not want_raw:
+            self._check(call(None, 0), "compress_test_lorenzo")
+            return None, by.value, sq.value
         cap = 64 * 1024 + st.shape[0] * (sbs + 1) ** len(shape) * 16
         raw = np.zeros(cap, np.uint8)
-        rs, by, sq = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
-        self._check(self._L.faz_gpu_compress_test_lorenzo_f32(self.h, self._dev(x), len(shape), _np_ptr(fd),
-                                                              _np_ptr(st), st.shape[0], sbs, float(eb), _np_ptr(raw),
-                                                              cap, ctypes.byref(rs), ctypes.byref(by),
-                                                              ctypes.byref(sq)), "compress_test_lorenzo")
+        rc = call(_np_ptr(raw), cap)
+        if rc == -4:  # the buffer's size is known now
+            raw = np.zeros(rs.value, np.uint8)
+            rc = call(_np_ptr(raw), raw.size)
+        self._check(rc, "compress_test_lorenzo")
         return raw[:rs.value].tobytes(), by.value, sq.value
 
     # --- default (SZ3) tuning mode --------------------------------------------
@@ -826,7 +872,7 @@ class Engine:
 
     # --- exhaustive QoZ tuner -----------------------------------------------
     def tune_qoz(self, x, shape, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0,
-                 sample_block_size=0):
+                 sample_block_size=0, ab_list=0, test_lorenzo=0):
         """faz -q tuner on the device field x (flat tensor of `shape`)."""
         import torch
         torch.cuda.synchronize()
@@ -837,6 +883,7 @@ class Engine:
         t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
         t.interp_block_size, t.max_step, t.levelwise, t.sample_block_size = interp_block_size, max_step, \
             levelwise, sample_block_size
+        t.ab_list, t.test_lorenzo = int(ab_list), int(test_lorenzo)
         o = TuneOut()
         self._check(self._L.faz_gpu_tune_qoz_f32(self.h, self._dev(x), ctypes.byref(t), ctypes.byref(o)), "tune_qoz")
         return tune_out_dict(o)
@@ -903,7 +950,13 @@ class Engine:
                                                        _np_ptr(out)), "block_features")
         return out
 
-    def tune_learned(self, x, shape, abs_eb, rel_eb, rng, mlp, mu, sigma, interp_block_size=32):
+    def tune_learned(self, x, shape, abs_eb, rel_eb, rng, mlp, mu, sigma, interp_block_size=32, test_lorenzo=0,
+                     margin=-1.0, verify=0, lo=None, hi=None, ab_list=0):
+        """The learned tuner: the network's choice among the alpha/beta (and,
+        with test_lorenzo, Lorenzo) candidates; candidates predicted within
+        `margin` bits/value of the best are verified by real CompressTests (at
+        most `verify`), rows outside the standardised range [lo, hi] fall back
+        to the exhaustive search."""
         import torch
         torch.cuda.synchronize()
         t = TuneIn()
@@ -912,6 +965,14 @@ class Engine:
             t.dims[i] = int(dd)
         t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
         t.interp_block_size = interp_block_size
+        t.ab_list, t.test_lorenzo = int(ab_list), int(test_lorenzo)
+        t.learned_margin, t.learned_verify = float(margin), int(verify)
+        keep = []
+        if lo is not None and hi is not None:
+            lo = np.ascontiguousarray(lo, np.float32)
+            hi = np.ascontiguousarray(hi, np.float32)
+            keep = [lo, hi]
+            t.learned_lo, t.learned_hi = lo.ctypes.data, hi.ctypes.data
         mu = np.ascontiguousarray(mu, np.float32)
         sigma = np.ascontiguousarray(sigma, np.float32)
         o = TuneOut()
@@ -987,14 +1048,18 @@ class MLP:
         self.e._check(self.L.faz_mlp_set_params(self.e.h, self.h, _np_ptr(p), p.size), "mlp_set_params")
 
 
-def learned_input(block_feats, eb_norm, rel_eb, alpha, beta, eb_scale, ops, dirs):
-    """MLP input row of one candidate (faz_learned_input)."""
+def learned_input(block_feats, eb_norm, rel_eb, alpha, beta, eb_scale, ops, dirs, level_res, lorenzo=False):
+    """MLP input row of one candidate (faz_learned_input); level_res: the
+    tuner's per-level mean prediction errors in units of the bound."""
     f = np.ascontiguousarray(block_feats, np.float32)
     o = np.ascontiguousarray(ops, np.uint8)
     d = np.ascontiguousarray(dirs, np.uint8)
+    lr = np.ascontiguousarray(level_res, np.float64)
+    assert lr.size >= len(o)
     row = np.zeros(LEARNED_INPUTS, np.float32)
     rc = lib().faz_learned_input(_np_ptr(f), f.shape[0], float(eb_norm), float(rel_eb), float(alpha), float(beta),
-                                 float(eb_scale), len(o), _np_ptr(o), _np_ptr(d), _np_ptr(row))
+                                 float(eb_scale), len(o), _np_ptr(o), _np_ptr(d), _np_ptr(lr), 1 if lorenzo else 0,
+                                 _np_ptr(row))
     if rc:
         raise FazError(rc, "learned_input")
     return row

@@ -542,12 +542,47 @@ bool parse_tree(const uint8_t *buf, size_t n, int32_t &offset, std::vector<int32
         left[i] = l ? (int32_t) l : -1;
         right[i] = r ? (int32_t) r : -1;
     }
+    // a serialised Huffman tree is a tree: every slot reachable from the
+    // root is reached once (no shared children, no cycles) and no path is
+    // deeper than 64 bits
+    if (!leaf[0]) {
+        std::vector<uint8_t> seen(nc, 0);
+        std::vector<std::pair<int32_t, uint32_t>> stack{{0, 0u}};
+        seen[0] = 1;
+        while (!stack.empty()) {
+            const auto [nd, d] = stack.back();
+            stack.pop_back();
+            if (leaf[nd]) continue;
+            if (d >= 64) return false;
+            for (const int32_t ch: {left[nd], right[nd]}) {
+                if (ch < 0) continue;
+                if (seen[ch]) return false;
+                seen[ch] = 1;
+                stack.push_back({ch, d + 1});
+            }
+        }
+    }
     memcpy(&enc_bytes, tb + treeBytes, 8);
     bits_at = 12 + treeBytes + 8;
     return bits_at + enc_bytes <= n;
 }
 
 }  // namespace
+
+// Host check of a saved Huffman section (no GPU needed): the tree parses,
+// is a tree, and its bitstream fits in n bytes.
+extern "C" faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, size_t *nodes, size_t *section_bytes) {
+    if (!buf) return FAZ_ERR_INVALID;
+    int32_t offset;
+    std::vector<int32_t> left, right, sym;
+    std::vector<uint8_t> leaf;
+    uint64_t enc = 0;
+    size_t at = 0;
+    if (!parse_tree(buf, n, offset, left, right, sym, leaf, enc, at)) return FAZ_ERR_FORMAT;
+    if (nodes) *nodes = sym.size();
+    if (section_bytes) *section_bytes = at + enc;
+    return FAZ_OK;
+}
 
 // HuffmanEncoder::decode on the device: num symbols (each plus the tree's
 // offset, which must land in [0, limit) unless limit is 0) into d_out on

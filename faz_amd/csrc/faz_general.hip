@@ -787,6 +787,8 @@ int gf_decompress(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, int N, int lo,
     return FAZ_OK;
 }
 
+}  // namespace
+
 // CompressTest's Lorenzo variant (SZI:1136-1170, 1254-1256, 1283-1294,
 // 1353-1392): one general compressor with lorenzo + lorenzo2, blockSize 5,
 // radius 65536 over the sample blocks, the squared error of the overwritten
@@ -862,7 +864,6 @@ int gf_compress_test(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *f
     return FAZ_OK;
 }
 
-}  // namespace
 }  // namespace faz
 
 int faz_general_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_general_params *p,

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <numeric>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -588,7 +589,11 @@ struct LearnedModel {
 int learned_predict(faz_gpu_ctx *ctx, const LearnedModel &lm, const std::vector<float> &rows, size_t nrows,
                     std::vector<float> &y);
 void learned_row(const float *block_feats, size_t nblocks, double eb_norm, double rel, double alpha, double beta,
-                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, float *row);
+                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, const double *level_res,
+                 int lorenzo, float *row);
+int gf_compress_test(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *fdims, const size_t *starts,
+                     size_t nblocks, size_t sbs, double eb, std::vector<uint8_t> &raw, size_t *bytes,
+                     double *square_error);
 int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origins, size_t nblocks,
                    const float *d_field, float vmin, float rng, float eb_norm, std::vector<float> &feats);
 
@@ -710,57 +715,47 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
     }
     out->interp_algo = 1;  // the last candidate tried stays in the Config
     out->direction = dir_cands[1];
-    // alpha / beta (SZI:2340-2420)
-    const double alphas[5] = {1, 1.25, 1.5, 1.75, 2};
+    // mean |mode-2 prediction error| per point of each tuned level for its
+    // chosen interpolator, in units of the bound: the top tuned level's run
+    // also covered every level above it
+    {
+        const int top = T.pl.G.levels;
+        auto lattice = [&](int sh) {
+            uint64_t c = 1;
+            for (int a = 0; a < N; a++) c *= sh >= 63 ? 1 : (uint64_t) ((T.pl.G.bd[a] - 1) >> sh) + 1;
+            return c;
+        };
+        for (int level = 1; level <= levelwise; level++) {
+            const int k = ops[level - 1] * 2 + (dirs[level - 1] != 0);
+            const uint64_t pts = lattice(level - 1) - lattice(level == levelwise ? top : level);
+            const double cnt = (double) pts * nb;
+            out->level_res[level - 1] = cnt > 0 ? out->pred_loss[level - 1][k] / cnt / in->abs_eb : 0.0;
+        }
+    }
+    // alpha / beta candidates (init_alphalist / init_betalist, SZI:869-933)
+    std::vector<double> alphas = {1, 1.25, 1.5, 1.75, 2};
+    if (in->ab_list == 1) alphas.insert(alphas.end(), {2.25, 2.5});
+    if (in->ab_list >= 2) alphas.insert(alphas.end(), {2.25, 2.5, 2.75, 3});
     const double betas[4] = {1.5, 2, 3, 4};
     double best_a = 1, best_b = 1, bestb = 9999, bestm = 0;
-    std::vector<float> feats;
+    out->use_interp = 1;
+    out->learned_path = lm ? 0 : -1;
     const double eb_norm = in->abs_eb / in->rng;
-    if (lm) {
-        float mn, mx;
-        if ((rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx))) return rc;
-        if ((rc = block_features(ctx, T.pl.G, T.d_origins.p, nb, d_in, mn, (float) in->rng, (float) eb_norm, feats)))
-            return rc;
-    }
-    // every alpha/beta pair at the base bound in one batch; re-tests at a
-    // scaled bound are predicted as they come
-    std::vector<float> y_base;
-    if (lm) {
-        std::vector<float> rows;
-        for (int i = 0; i < 5; i++)
-            for (int j = 0; j < 4; j++) {
-                std::vector<float> row(FAZ_LEARNED_INPUTS);
-                learned_row(feats.data(), nb, eb_norm, rel, alphas[i], betas[j], 1.0, levelwise, ops.data(),
-                            dirs.data(), row.data());
-                rows.insert(rows.end(), row.begin(), row.end());
-            }
-        if ((rc = learned_predict(ctx, *lm, rows, 20, y_base))) return rc;
-    }
-    auto test = [&](double a, double b, double eb, double &br, double &ps) -> int {
-        if (lm) {
-            std::vector<float> y;
-            if (eb == in->abs_eb) {
-                int i = 0, j = 0;
-                while (i < 4 && alphas[i] != a) i++;
-                while (j < 3 && betas[j] != b) j++;
-                y.assign(y_base.begin() + 2 * (4 * i + j), y_base.begin() + 2 * (4 * i + j) + 2);
-            } else {
-                std::vector<float> row(FAZ_LEARNED_INPUTS);
-                learned_row(feats.data(), nb, eb_norm, rel, a, b, eb / in->abs_eb, levelwise, ops.data(),
-                            dirs.data(), row.data());
-                int r = learned_predict(ctx, *lm, row, 1, y);
-                if (r) return r;
-            }
-            br = 4.0 * y[0];
-            ps = 5.0 * y[1] - 20 * std::log10(eb_norm * (eb / in->abs_eb));
-            const int t = out->n_tests;
-            if (t < 128) {
-                out->test_alpha[t] = a, out->test_beta[t] = b, out->test_eb[t] = eb;
-                out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = 0;
-            }
-            out->n_tests++;
-            return FAZ_OK;
+    size_t per = 1;
+    for (int a = 0; a < N; a++) per *= (size_t) sbs + 1;
+    std::vector<size_t> starts3;
+    for (size_t b = 0; b < out->n_blocks && b < 4096; b++)
+        for (int a = 0; a < 3; a++) starts3.push_back(out->block_starts[b][a]);
+    auto record = [&](double a, double b, double eb, double br, double ps, size_t bytes) {
+        const int t = out->n_tests;
+        if (t < 128) {
+            out->test_alpha[t] = a, out->test_beta[t] = b, out->test_eb[t] = eb;
+            out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = bytes;
         }
+        out->n_tests++;
+    };
+    // CompressTest with ALGO_INTERP (SZI:1131-1392) of one candidate
+    auto test = [&](double a, double b, double eb, double &br, double &ps) -> int {
         TuneCand c;
         memset(&c, 0, sizeof(c));
         fill_levels(c, T.pl.G.levels, eb, a, b, 1, dir_cands[1], ops, dirs);
@@ -770,37 +765,169 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
         size_t bytes = 0;
         r = T.compress_test(eb, br, ps, bytes);
         if (r) return r;
-        const int t = out->n_tests;
-        if (t < 128) {
-            out->test_alpha[t] = a, out->test_beta[t] = b, out->test_eb[t] = eb;
-            out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = bytes;
-        }
-        out->n_tests++;
+        record(a, b, eb, br, ps, bytes);
         return FAZ_OK;
     };
-    for (int i = 0; i < 5; i++) {
-        for (int j = 0; j < 4; j++) {
-            const double alpha = alphas[i], beta = betas[j];
-            if (alpha >= 1 && alpha > beta) continue;
-            double bitrate, metric;
-            if ((rc = test(alpha, beta, in->abs_eb, bitrate, metric))) return rc;
-            const bool stop = (alpha >= 1 && std::pow(alpha, max_interp_level - 1) <= beta) ||
-                              (alpha < 1 && alpha * (max_interp_level - 1) <= beta);
-            if (metric >= bestm && bitrate <= bestb) {
-                best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
-            } else if (metric <= bestm && bitrate >= bestb) {
+    // CompressTest with ALGO_LORENZO_REG, useFast = false (SZI:1136-1170,
+    // 1283-1294, 1353-1392; lorenzoBrFix = 1), recorded with alpha = beta = -1
+    auto lorenzo_test = [&](double eb, double &br, double &ps) -> int {
+        if (out->n_blocks > 4096) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "the Lorenzo test takes at most 4096 sample blocks");
+        std::vector<uint8_t> raw;
+        size_t bytes = 0;
+        double sq = 0;
+        int r = gf_compress_test(ctx, d_in, N, in->dims, starts3.data(), out->n_blocks, (size_t) sbs, eb, raw,
+                                 &bytes, &sq);
+        if (r) return r;
+        const double ele = (double) per * out->n_blocks;
+        br = 8 * double(bytes) / ele;
+        ps = 20 * std::log10(in->rng) - 10 * std::log10(sq / ele);
+        record(-1, -1, eb, br, ps, bytes);
+        return FAZ_OK;
+    };
+    bool exhaustive = lm == nullptr;
+    if (lm) {
+        // every candidate predicted in one batch; the lowest predicted cost
+        // (bits/value at equal PSNR along 6.02 dB per bit) wins unless the
+        // gates below ask for real CompressTests
+        std::vector<float> feats;
+        float mn, mx;
+        if ((rc = faz_gpu_range_f32(ctx, d_in, num, &mn, &mx))) return rc;
+        if ((rc = block_features(ctx, T.pl.G, T.d_origins.p, nb, d_in, mn, (float) in->rng, (float) eb_norm, feats)))
+            return rc;
+        struct LC {
+            double a, b;
+            int lorenzo;
+            double br, ps, cost;
+        };
+        std::vector<LC> lc;
+        for (double a: alphas)
+            for (double b: betas)
+                if (!(a >= 1 && a > b)) lc.push_back(LC{a, b, 0, 0, 0, 0});
+        if (in->test_lorenzo) lc.push_back(LC{-1, -1, 1, 0, 0, 0});
+        std::vector<float> rows(lc.size() * FAZ_LEARNED_INPUTS);
+        for (size_t c = 0; c < lc.size(); c++)
+            learned_row(feats.data(), nb, eb_norm, rel, lc[c].lorenzo ? 0.0 : lc[c].a, lc[c].lorenzo ? 0.0 : lc[c].b,
+                        1.0, levelwise, ops.data(), dirs.data(), out->level_res, lc[c].lorenzo,
+                        rows.data() + c * FAZ_LEARNED_INPUTS);
+        bool ood = false;
+        if (in->learned_lo && in->learned_hi)
+            for (size_t k = 0; k < rows.size(); k++) {
+                const int i = (int) (k % FAZ_LEARNED_INPUTS);
+                const float z = (rows[k] - lm->mu[i]) / lm->sigma[i];
+                if (!(z >= in->learned_lo[i] && z <= in->learned_hi[i])) ood = true;
+            }
+        if (ood) {
+            out->learned_path = 2;
+            exhaustive = true;
+        } else {
+            std::vector<float> y;
+            if ((rc = learned_predict(ctx, *lm, rows, lc.size(), y))) return rc;
+            for (size_t c = 0; c < lc.size(); c++) {
+                lc[c].br = std::exp2((double) y[2 * c]);
+                lc[c].ps = 5.0 * y[2 * c + 1] - 20 * std::log10(eb_norm);
+                lc[c].cost = lc[c].br - lc[c].ps / 6.02;
+                record(lc[c].a, lc[c].b, in->abs_eb, lc[c].br, lc[c].ps, 0);
+            }
+            std::vector<size_t> order(lc.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](size_t i, size_t j) { return lc[i].cost < lc[j].cost; });
+            const int kmax = in->learned_verify > 0 ? in->learned_verify : 4;
+            std::vector<size_t> ver;
+            if (in->learned_margin >= 0)
+                for (size_t c: order)
+                    if ((int) ver.size() < kmax && lc[c].cost <= lc[order[0]].cost + in->learned_margin) ver.push_back(c);
+            size_t pick = order[0];
+            if (ver.size() >= 2) {
+                // the closest candidates by real CompressTests (interpolation
+                // candidates in one launch) and the lowest real cost among them
+                out->learned_path = 1;
+                std::vector<TuneCand> cands;
+                std::vector<size_t> which;
+                for (size_t c: ver) {
+                    if (lc[c].lorenzo) continue;
+                    TuneCand tc;
+                    memset(&tc, 0, sizeof(tc));
+                    fill_levels(tc, T.pl.G.levels, in->abs_eb, lc[c].a, lc[c].b, 1, dir_cands[1], ops, dirs);
+                    set_range(tc, T.pl.G, 1, 0, 0);
+                    cands.push_back(tc);
+                    which.push_back(c);
+                }
+                double best_cost = std::numeric_limits<double>::max();
+                if (!cands.empty()) {
+                    if ((rc = T.run(cands, true))) return rc;
+                    for (size_t k = 0; k < cands.size(); k++) {
+                        double br, ps;
+                        size_t bytes = 0;
+                        if ((rc = T.compress_test(in->abs_eb, br, ps, bytes, nullptr, nullptr, (uint32_t) k))) return rc;
+                        record(lc[which[k]].a, lc[which[k]].b, in->abs_eb, br, ps, bytes);
+                        out->n_verified++;
+                        const double cost = br - ps / 6.02;
+                        if (cost < best_cost) best_cost = cost, pick = which[k];
+                    }
+                }
+                for (size_t c: ver) {
+                    if (!lc[c].lorenzo) continue;
+                    double br, ps;
+                    if ((rc = lorenzo_test(in->abs_eb, br, ps))) return rc;
+                    out->n_verified++;
+                    const double cost = br - ps / 6.02;
+                    if (cost < best_cost) best_cost = cost, pick = c;
+                }
+            }
+            if (lc[pick].lorenzo) {
+                out->use_interp = 0;
+                best_a = best_b = -1;
+            } else {
+                best_a = lc[pick].a;
+                best_b = lc[pick].b;
+            }
+        }
+    }
+    if (exhaustive) {
+        // alpha / beta (SZI:2340-2420)
+        for (size_t i = 0; i < alphas.size(); i++) {
+            for (int j = 0; j < 4; j++) {
+                const double alpha = alphas[i], beta = betas[j];
+                if (alpha >= 1 && alpha > beta) continue;
+                double bitrate, metric;
+                if ((rc = test(alpha, beta, in->abs_eb, bitrate, metric))) return rc;
+                const bool stop = (alpha >= 1 && std::pow(alpha, max_interp_level - 1) <= beta) ||
+                                  (alpha < 1 && alpha * (max_interp_level - 1) <= beta);
+                if (metric >= bestm && bitrate <= bestb) {
+                    best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
+                } else if (metric <= bestm && bitrate >= bestb) {
+                    if (stop) break;
+                    continue;
+                } else {
+                    const double fix = bitrate / bestb;
+                    double br_r, m_r;
+                    if ((rc = test(alpha, beta, in->abs_eb * fix, br_r, m_r))) return rc;
+                    const double a = (metric - m_r) / (bitrate - br_r);
+                    const double b = metric - a * bitrate;
+                    const double reg = a * bestb + b;
+                    if (reg > bestm) best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
+                }
                 if (stop) break;
-                continue;
+            }
+        }
+        // the Lorenzo test (SZI:2431-2493)
+        if (in->test_lorenzo) {
+            double bitrate, metric;
+            if ((rc = lorenzo_test(in->abs_eb, bitrate, metric))) return rc;
+            if (metric >= bestm && bitrate <= bestb) {
+                bestb = bitrate, bestm = metric, best_a = best_b = -1, out->use_interp = 0;
+            } else if (metric <= bestm && bitrate >= bestb) {
+                out->use_interp = 1;
             } else {
                 const double fix = bitrate / bestb;
                 double br_r, m_r;
-                if ((rc = test(alpha, beta, in->abs_eb * fix, br_r, m_r))) return rc;
+                if ((rc = lorenzo_test(in->abs_eb * fix, br_r, m_r))) return rc;
                 const double a = (metric - m_r) / (bitrate - br_r);
-                const double b = metric - a * bitrate;
-                const double reg = a * bestb + b;
-                if (reg > bestm) best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
+                if (a > 0) {
+                    const double b = metric - a * bitrate;
+                    if (a * bestb + b > bestm) bestb = bitrate, bestm = metric, best_a = best_b = -1, out->use_interp = 0;
+                }
             }
-            if (stop) break;
         }
     }
     out->alpha = best_a;
@@ -1143,9 +1270,12 @@ int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origin
 
 // Input row of the MLP (FAZ_LEARNED_INPUTS values): the block statistics
 // averaged over the sample blocks, magnitudes as log2 of their ratio to the
-// normalised bound, then the candidate.
+// normalised bound, then the candidate: its alpha/beta and interpolators, the
+// bound, the prediction error of levels 1..4 over that level's bound of the
+// candidate, the Lorenzo flag.
 void learned_row(const float *block_feats, size_t nblocks, double eb_norm, double rel, double alpha, double beta,
-                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, float *row) {
+                 double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs, const double *level_res,
+                 int lorenzo, float *row) {
     for (int i = 0; i < kFeat; i++) {
         double s = 0;
         for (size_t b = 0; b < nblocks; b++) {
@@ -1162,15 +1292,24 @@ void learned_row(const float *block_feats, size_t nblocks, double eb_norm, doubl
         mop += ops[l];
         mdir += dirs[l] != 0;
     }
-    row[40] = (float) (alpha - 1);
-    row[41] = (float) (beta / 4);
-    row[42] = (float) std::log2(beta);
+    row[40] = lorenzo ? 0.f : (float) (alpha - 1);
+    row[41] = lorenzo ? 0.f : (float) (beta / 4);
+    row[42] = lorenzo ? 0.f : (float) std::log2(std::max(beta, 1e-6));
     row[43] = (float) ops[0];
     row[44] = (float) (dirs[0] != 0);
     row[45] = (float) (mop / levelwise);
     row[46] = (float) (mdir / levelwise);
     row[47] = (float) std::log10(rel);
     row[48] = (float) std::log2(eb_scale);
+    // level l's bound is eb * scale / min(alpha^(l-1), beta) (SIC:392-398)
+    for (int l = 1; l <= 4; l++) {
+        const int k = std::min(l, levelwise) - 1;
+        double ratio = 1.0;
+        if (!lorenzo && alpha >= 1) ratio = std::min(std::pow(alpha, l - 1), beta);
+        const double r = level_res ? level_res[k] : 0.0;
+        row[48 + l] = (float) std::log2(std::max(r * ratio / eb_scale, 1e-6));
+    }
+    row[53] = lorenzo ? 1.f : 0.f;
 }
 
 int learned_predict(faz_gpu_ctx *ctx, const LearnedModel &lm, const std::vector<float> &rows, size_t nrows,
@@ -1229,16 +1368,18 @@ faz_status faz_gpu_block_features_f32(faz_gpu_ctx *ctx, const float *d_in, int N
 
 faz_status faz_learned_input(const float *block_feats, size_t nblocks, double eb_norm, double rel_eb, double alpha,
                              double beta, double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs,
-                             float *row) {
-    if (!block_feats || !nblocks || !ops || !dirs || !row || levelwise < 1 || !(eb_norm > 0) || !(rel_eb > 0))
+                             const double *level_res, int lorenzo, float *row) {
+    if (!block_feats || !nblocks || !ops || !dirs || !row || levelwise < 1 || !(eb_norm > 0) || !(rel_eb > 0) ||
+        !(eb_scale > 0))
         return FAZ_ERR_INVALID;
-    faz::learned_row(block_feats, nblocks, eb_norm, rel_eb, alpha, beta, eb_scale, levelwise, ops, dirs, row);
+    faz::learned_row(block_feats, nblocks, eb_norm, rel_eb, alpha, beta, eb_scale, levelwise, ops, dirs, level_res,
+                     lorenzo, row);
     return FAZ_OK;
 }
 
 faz_status faz_learned_target(double bitrate, double psnr, double eb_norm, double eb_scale, float *t) {
-    if (!t || !(eb_norm > 0) || !(eb_scale > 0)) return FAZ_ERR_INVALID;
-    t[0] = (float) (bitrate / 4);
+    if (!t || !(eb_norm > 0) || !(eb_scale > 0) || !(bitrate > 0)) return FAZ_ERR_INVALID;
+    t[0] = (float) std::log2(bitrate);
     t[1] = (float) ((psnr + 20 * std::log10(eb_norm * eb_scale)) / 5);
     return FAZ_OK;
 }

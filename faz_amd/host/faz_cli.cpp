@@ -43,7 +43,8 @@ void usage() {
            "\t-S <PSNR>\n"
            "\t-N <normErr>\n"
            "\t-q: QoZ mode\n"
-           "\t-F <1/0>: FAZ features\n"
+           "\t-F <1/0>: FAZ features (the reference's default, 1, needs wavelets, which this engine\n"
+           "\t          does not serve: without -F the SZ3 tuning mode runs, as with -F 0)\n"
            "\t-T <PSNR|CR|SSIM|AC>: tuning target\n"
            "\t-C <maxStep>  -B <sampleBlockSize>  -l: test Lorenzo\n"
            "* dimensions (fastest last): -1 nx | -2 nx ny | -3 nx ny nz | -4 nx ny nz nw\n");
@@ -107,7 +108,7 @@ void verify(const float *ori, const float *dec, size_t n) {
 int main(int argc, char *argv[]) {
     bool binaryOutput = true, compression = false, decompression = false, printCmpResults = false;
     bool doubleType = false, qoz = false, testLorenzo = false, delCmpPath = false;
-    int faz = 1, maxStep = 0, sampleBlockSize = 0;
+    int faz = -1, maxStep = 0, sampleBlockSize = 0;  // -1: -F not given
     char *inPath = nullptr, *cmpPath = nullptr, *conPath = nullptr, *decPath = nullptr;
     char *errBoundMode = nullptr, *errBound = nullptr, *absEB = nullptr, *relEB = nullptr, *psnrEB = nullptr,
          *normEB = nullptr, *tuningTarget = nullptr;
@@ -191,6 +192,16 @@ int main(int argc, char *argv[]) {
     else dims = {r4, r3, r2, r1};
     conf.setDims(dims.begin(), dims.end());
     if (qoz) conf.QoZ = 1;
+    // The reference turns FZ on by default (test/faz.cpp:110, 194); FZ tunes
+    // wavelet candidates, which this engine does not serve.  Without -F the
+    // SZ3 tuning mode runs instead (the reference's -F 0), with a notice;
+    // an explicit -F 1 stays FZ and fails with FAZ_ERR_UNSUPPORTED.
+    if (faz < 0) {
+        faz = 0;
+        if (compression && !qoz && !conPath)
+            fprintf(stderr, "faz: note: FZ mode (wavelets, the reference's default) is not served by the GPU engine; "
+                            "using the SZ3 tuning mode (-F 0); -q runs the QoZ tuner\n");
+    }
     if (faz) conf.FZ = 1;
     if (testLorenzo) conf.testLorenzo = 1;
     if (maxStep > 0) conf.maxStep = maxStep;

@@ -351,6 +351,7 @@ int compress_qoz_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_data
     tin.max_step = conf.maxStep;
     tin.levelwise = conf.levelwisePredictionSelection;
     tin.sample_block_size = conf.sampleBlockSize;
+    tin.ab_list = conf.abList;
     std::unique_ptr<faz_tune_out> tout(new faz_tune_out);
     if ((rc = faz_gpu_tune_qoz_f32(ctx, d_data, &tin, tout.get()))) return rc;
     conf.levelwisePredictionSelection = tout->levelwise;
@@ -374,7 +375,10 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
     memcpy(&confSize, stream + n - 4, 4);
     if (confSize <= 0 || (size_t) confSize + 4 > n) return FAZ_ERR_FORMAT;
     const QoZ::uchar *c = stream + (n - 4 - confSize);
-    conf.load(c);
+    if (!conf.load(c, stream + n - 4)) {
+        faz_ctx_set_error(ctx, "Config trailer truncated or corrupt");
+        return FAZ_ERR_FORMAT;
+    }
     if (conf.N < 1 || conf.N > 4 || conf.num != num) {
         faz_ctx_set_error(ctx, "stream shape does not match the output buffer");
         return FAZ_ERR_FORMAT;
@@ -459,7 +463,8 @@ void SZ_decompress<float>(QoZ::Config &config, char *cmpData, size_t cmpSize, fl
     memcpy(&confSize, cmpData + cmpSize - 4, 4);
     if (confSize <= 0 || (size_t) confSize + 4 > cmpSize) throw faz::Error(FAZ_ERR_FORMAT, "bad Config trailer");
     const QoZ::uchar *c = (const QoZ::uchar *) cmpData + (cmpSize - 4 - confSize);
-    conf.load(c);
+    if (!conf.load(c, (const QoZ::uchar *) cmpData + cmpSize - 4))
+        throw faz::Error(FAZ_ERR_FORMAT, "Config trailer truncated or corrupt");
     faz_gpu_ctx *ctx = api_ctx();
     float *dev = faz_ctx_work_in(ctx, conf.num);
     if (!dev) fail(ctx, FAZ_ERR_HIP, "device buffer");
@@ -646,10 +651,37 @@ faz_status faz_sz_decompress_f32(faz_gpu_ctx *ctx, const uint8_t *stream, size_t
         if (!d) return FAZ_ERR_HIP;
     }
     QoZ::Config conf;
-    int rc = decompress_stream(ctx, stream, n, conf, d, num);
+    int rc;
+    try {  // no C++ exception crosses the C boundary
+        rc = decompress_stream(ctx, stream, n, conf, d, num);
+    } catch (const std::exception &e) {
+        faz_ctx_set_error(ctx, e.what());
+        return FAZ_ERR_FORMAT;
+    }
     if (rc) return rc;
     if (!out_on_device && hipMemcpy(out, d, num * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         return FAZ_ERR_HIP;
+    return FAZ_OK;
+}
+
+// The Config of a stream's trailer without decoding it (host only): N, the
+// dimensions and cmprAlgo, for callers that size their output first.
+faz_status faz_stream_info(const uint8_t *stream, size_t n, int *N, size_t *dims, int *cmpr_algo) {
+    if (!stream || !N || !dims || !cmpr_algo) return FAZ_ERR_INVALID;
+    if (n < 4) return FAZ_ERR_FORMAT;
+    int confSize;
+    memcpy(&confSize, stream + n - 4, 4);
+    if (confSize <= 0 || (size_t) confSize + 4 > n) return FAZ_ERR_FORMAT;
+    const QoZ::uchar *c = stream + (n - 4 - confSize);
+    QoZ::Config conf;
+    try {
+        if (!conf.load(c, stream + n - 4)) return FAZ_ERR_FORMAT;
+    } catch (const std::exception &) {
+        return FAZ_ERR_FORMAT;
+    }
+    *N = conf.N;
+    for (int i = 0; i < conf.N; i++) dims[i] = conf.dims[i];
+    *cmpr_algo = conf.cmprAlgo;
     return FAZ_OK;
 }
 

@@ -28,7 +28,6 @@ import faz_amd
 ALPHAS = (1.0, 1.25, 1.5, 1.75, 2.0)
 BETAS = (1.5, 2.0, 3.0, 4.0)
 EB_SCALES = (0.8, 0.9, 1.0, 1.12, 1.25)
-TRAIN_GRID = (320, 320, 320)  # training fields are cut from GRFs synthesised on this grid
 MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "models", "learned_tuner.npz")
 
 
@@ -68,21 +67,28 @@ def grf_field(shape, seed, slope=11.0 / 3.0, fronts=0, noise=1e-4, width=4.0, am
     return out
 
 
+LORENZO_SCALES = (0.8, 1.0, 1.25)
+
+
 def field_tuples(engine, x, shape, rel):
-    """(rows, targets, info) of one field: the exhaustive tuner's tests and a
-    CompressTest grid over ALPHAS x BETAS x EB_SCALES."""
+    """(rows, targets, info) of one field: the exhaustive tuner's tests (with
+    the Lorenzo test), a CompressTest grid over ALPHAS x BETAS x EB_SCALES and
+    the Lorenzo CompressTest at LORENZO_SCALES."""
     mn, mx = engine.range(x)
     rng = float(np.float32(mx - mn))
     eb = rel * rng
-    ex = engine.tune_qoz(x, shape, eb, rel, rng)
+    ex = engine.tune_qoz(x, shape, eb, rel, rng, test_lorenzo=1)
     sbs, starts = ex["sample_block_size"], ex["block_starts"]
     eb_norm = eb / rng
     feats = engine.block_features(x, shape, starts, sbs, mn, rng, eb_norm)
-    rows, targets = [], []
+    rows, targets, kinds = [], [], []
 
     def add(alpha, beta, eb_t, bitrate, psnr):
-        rows.append(faz_amd.learned_input(feats, eb_norm, rel, alpha, beta, eb_t / eb, ex["ops"], ex["dirs"]))
+        lor = alpha < 0
+        rows.append(faz_amd.learned_input(feats, eb_norm, rel, 0.0 if lor else alpha, 0.0 if lor else beta,
+                                          eb_t / eb, ex["ops"], ex["dirs"], ex["level_res"], lor))
         targets.append(faz_amd.learned_target(bitrate, psnr, eb_norm, eb_t / eb))
+        kinds.append((alpha, beta, eb_t / eb))
 
     for t in ex["tests"]:
         add(t["alpha"], t["beta"], t["eb"], t["bitrate"], t["psnr"])
@@ -96,34 +102,59 @@ def field_tuples(engine, x, shape, rel):
         bitrate = 8.0 * float(nbt) / per
         psnr = 20 * np.log10(rng) - 10 * np.log10(sqe / per) if sqe > 0 else 200.0
         add(a, b, eb * s, bitrate, psnr)
-    return np.array(rows, np.float32), np.array(targets, np.float32), dict(ex=ex, rng=rng, eb=eb)
+    for s in LORENZO_SCALES:
+        _, nbt, sqe = engine.compress_test_lorenzo(x, shape, starts, sbs, eb * s, want_raw=False)
+        bitrate = 8.0 * float(nbt) / per
+        psnr = 20 * np.log10(rng) - 10 * np.log10(sqe / per) if sqe > 0 else 200.0
+        add(-1.0, -1.0, eb * s, bitrate, psnr)
+    return np.array(rows, np.float32), np.array(targets, np.float32), dict(ex=ex, rng=rng, eb=eb, kinds=kinds)
+
+
+TRAIN_GRIDS = (256, 320, 384, 512)  # cubic FFT grids the training fields are synthesised on
 
 
 def training_fields(n, seed=0):
-    """Seeded (shape, slope, fronts, rel) draws for the training set."""
+    """Seeded draws for the training set: a field synthesised on one of
+    TRAIN_GRIDS (so FFT plans are reused) and, for half of the draws, a
+    sub-box of edges 0.5..1 of the grid cut out of it; slope k^-1.5..k^-5.5,
+    0..16 fronts of width 2..8, noise 0..1e-3, REL 1e-6..1e-2 (log-uniform)."""
     r = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        shape = tuple(int(v) for v in r.integers(160, TRAIN_GRID[0] + 1, 3))
-        out.append(dict(shape=shape, seed=int(r.integers(1 << 40)), slope=float(r.uniform(2.0, 5.0)),
-                        fronts=int(r.integers(0, 17)), rel=float(10 ** r.uniform(-4.3, -2.7))))
+        g = int(r.choice(TRAIN_GRIDS))
+        if r.random() < 0.5:
+            shape = (g, g, g)
+        else:
+            shape = tuple(int(v) for v in r.integers(max(96, g // 2), g + 1, 3))
+        out.append(dict(shape=shape, grid=(g, g, g), seed=int(r.integers(1 << 40)), slope=float(r.uniform(1.5, 5.5)),
+                        fronts=int(r.integers(0, 17)), width=float(r.choice([2.0, 4.0, 8.0])),
+                        noise=float(r.choice([0.0, 1e-5, 1e-4, 1e-4, 1e-3])), rel=float(10 ** r.uniform(-6, -2))))
     return out
 
 
-def build_dataset(engine, specs, log=None):
+def build_dataset(engine, specs, log=None, deadline=None):
+    """Rows, targets and the field index of every tuple (fields after the
+    time.time() deadline, if any, are left out)."""
     import torch
-    X, T = [], []
+    X, T, G = [], [], []
     t0 = time.time()
     for i, sp in enumerate(specs):
-        x = grf_field(sp["shape"], sp["seed"], sp["slope"], sp["fronts"], grid=TRAIN_GRID)
+        if deadline is not None and time.time() > deadline:
+            if log:
+                log(f"deadline: stopping after {i} fields")
+            break
+        grid = sp.get("grid")
+        x = grf_field(sp["shape"], sp["seed"], sp["slope"], sp["fronts"], noise=sp.get("noise", 1e-4),
+                      width=sp.get("width", 4.0), grid=None if grid == sp["shape"] else grid)
         rows, targets, _ = field_tuples(engine, x, sp["shape"], sp["rel"])
         X.append(rows)
         T.append(targets)
+        G.append(np.full(len(rows), i, np.int32))
         del x
         if log and (i + 1) % 100 == 0:
             log(f"fields {i + 1}/{len(specs)}: {sum(len(a) for a in X)} tuples, {time.time() - t0:.0f} s")
             torch.cuda.empty_cache()
-    return np.concatenate(X), np.concatenate(T)
+    return np.concatenate(X), np.concatenate(T), np.concatenate(G)
 
 
 def train(engine, X, T, epochs=40, batch=1024, lr=1e-3, seed=0, hidden=256, layers=3, log=None):
@@ -152,10 +183,18 @@ def train(engine, X, T, epochs=40, batch=1024, lr=1e-3, seed=0, hidden=256, laye
     return net, mu.astype(np.float32), sigma.astype(np.float32), hist
 
 
-def save_model(path, net, mu, sigma, meta=None):
+def save_model(path, net, mu, sigma, meta=None, gates=None):
+    """Parameters, input standardisation and the confidence gates (lo / hi:
+    standardised input range accepted without the exhaustive fallback;
+    margin: predicted cost gap, in bits/value, below which the closest
+    candidates are verified by real CompressTests)."""
     os.makedirs(os.path.dirname(path), exist_ok=True)
+    g = gates or {}
     np.savez_compressed(path, params=net.get_params(), mu=mu, sigma=sigma, K=np.array(net.K), N=np.array(net.N),
                         in_dim=np.array([net.in_dim]), out_dim=np.array([net.out_dim]),
+                        lo=np.asarray(g.get("lo", np.full(net.in_dim, -np.inf)), np.float32),
+                        hi=np.asarray(g.get("hi", np.full(net.in_dim, np.inf)), np.float32),
+                        margin=np.array([g.get("margin", -1.0)]), verify=np.array([g.get("verify", 4)]),
                         meta=np.array(repr(meta or {})))
 
 
@@ -168,6 +207,49 @@ def load_model(engine, path=MODEL_PATH, hidden=256, layers=3):
     return net, z["mu"], z["sigma"]
 
 
+def load_gates(path=MODEL_PATH):
+    """Confidence gates stored with the model (see save_model)."""
+    z = np.load(path, allow_pickle=False)
+    return dict(lo=z["lo"], hi=z["hi"], margin=float(z["margin"][0]), verify=int(z["verify"][0]))
+
+
+def tune_learned(engine, x, shape, eb, rel, rng, net, mu, sigma, gates=None, test_lorenzo=1):
+    """The learned tuner with the model's gates (or none)."""
+    g = gates or {}
+    return engine.tune_learned(x, shape, eb, rel, rng, net, mu, sigma, test_lorenzo=test_lorenzo,
+                               margin=g.get("margin", -1.0), verify=g.get("verify", 0), lo=g.get("lo"),
+                               hi=g.get("hi"))
+
+
+def predict(net, mu, sigma, rows):
+    """(bitrate, PSNR relative to the uniform-quantisation PSNR) of rows."""
+    import torch
+    y = net.forward(torch.from_numpy(((rows - mu) / sigma).astype(np.float32)).cuda()).cpu().numpy()
+    return y
+
+
+def calibrate(net, mu, sigma, X, T, G, kinds_scale, quantile=0.9):
+    """Gates from held-out tuples: lo/hi = the standardised range of the
+    training rows widened by one unit, margin = the `quantile` of the
+    predicted cost gap between the predicted best and the truly best
+    base-bound candidate of each held-out field."""
+    Z = (X - mu) / sigma
+    lo = Z.min(0) - 1.0
+    hi = Z.max(0) + 1.0
+    y = predict(net, mu, sigma, X)
+    gaps = []
+    for g in np.unique(G):
+        m = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9)
+        if m.sum() < 2:
+            continue
+        # cost = bitrate - PSNR / 6.02 (the constant PSNR offset cancels)
+        true = np.exp2(T[m, 0]) - 5.0 * T[m, 1] / 6.02
+        pred = np.exp2(y[m, 0]) - 5.0 * y[m, 1] / 6.02
+        gaps.append(pred[np.argmin(true)] - pred.min())
+    margin = float(np.quantile(np.array(gaps), quantile)) if gaps else -1.0
+    return dict(lo=lo.astype(np.float32), hi=hi.astype(np.float32), margin=margin, verify=4), np.array(gaps)
+
+
 def final_stats(engine, x, shape, eb, params):
     """Compressed stream size and PSNR of the engine with `params` on the field."""
     import torch
@@ -177,11 +259,29 @@ def final_stats(engine, x, shape, eb, params):
     return len(raw), mse
 
 
-def evaluate(engine, net, mu, sigma, shape, seed, rel, slope=11.0 / 3.0, fronts=8):
+def lorenzo_stats(engine, x, shape, eb):
+    """Size and MSE of the 3-D Lorenzo engine the tuner's Lorenzo outcome
+    selects (lorenzo + lorenzo2, blockSize 5, SZI:2800-2806)."""
+    import torch
+    p = faz_amd.make_lorenzo_params(shape, eb, block_size=5, pred_dim=3, lorenzo=True, lorenzo2=True,
+                                    regression=False)
+    recon = torch.empty_like(x)
+    raw = engine.lorenzo_raw(x, p, recon=recon)
+    mse = torch.mean((recon.double() - x.double()) ** 2).item()
+    return len(raw), mse
+
+
+def choice_params(shape, eb, r):
+    return faz_amd.make_params(shape, eb, algo=r["interp_algo"], direction=r["direction"], alpha=r["alpha"],
+                               beta=r["beta"], max_step=r["max_step"], ops=tuple(r["ops"]), dirs=tuple(r["dirs"]))
+
+
+def evaluate(engine, net, mu, sigma, shape, seed, rel, slope=11.0 / 3.0, fronts=8, gates=None, test_lorenzo=1,
+             noise=1e-4):
     """Exhaustive vs learned vs default (setABwithRelBound, no alpha/beta
     search) on one field: tuning times, engine buffer sizes, PSNRs."""
     import torch
-    x = grf_field(shape, seed, slope, fronts)
+    x = grf_field(shape, seed, slope, fronts, noise=noise)
     mn, mx = engine.range(x)
     rng = float(np.float32(mx - mn))
     eb = rel * rng
@@ -190,20 +290,25 @@ def evaluate(engine, net, mu, sigma, shape, seed, rel, slope=11.0 / 3.0, fronts=
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         if name == "exhaustive":
-            r = engine.tune_qoz(x, shape, eb, rel, rng)
+            r = engine.tune_qoz(x, shape, eb, rel, rng, test_lorenzo=test_lorenzo)
         else:
-            r = engine.tune_learned(x, shape, eb, rel, rng, net, mu, sigma)
+            r = tune_learned(engine, x, shape, eb, rel, rng, net, mu, sigma, gates, test_lorenzo)
         torch.cuda.synchronize()
         out[name] = dict(tune_ms=(time.perf_counter() - t0) * 1e3, alpha=r["alpha"], beta=r["beta"], ops=r["ops"],
-                         dirs=r["dirs"], ex=r)
+                         dirs=r["dirs"], use_interp=r["use_interp"], learned_path=r["learned_path"],
+                         n_verified=r["n_verified"], ex=r)
     ex = out["exhaustive"]["ex"]
     a0, b0 = default_ab(rel)
-    out["default"] = dict(tune_ms=0.0, alpha=a0, beta=b0, ops=ex["ops"], dirs=ex["dirs"])
+    out["default"] = dict(tune_ms=0.0, alpha=a0, beta=b0, ops=ex["ops"], dirs=ex["dirs"], use_interp=1)
     n = x.numel()
     for name, o in out.items():
-        p = faz_amd.make_params(shape, eb, algo=ex["interp_algo"], direction=ex["direction"], alpha=o["alpha"],
-                                beta=o["beta"], max_step=ex["max_step"], ops=tuple(o["ops"]), dirs=tuple(o["dirs"]))
-        nbytes, mse = final_stats(engine, x, shape, eb, p)
+        if o["use_interp"]:
+            p = faz_amd.make_params(shape, eb, algo=ex["interp_algo"], direction=ex["direction"], alpha=o["alpha"],
+                                    beta=o["beta"], max_step=ex["max_step"], ops=tuple(o["ops"]),
+                                    dirs=tuple(o["dirs"]))
+            nbytes, mse = final_stats(engine, x, shape, eb, p)
+        else:
+            nbytes, mse = lorenzo_stats(engine, x, shape, eb)
         o["bitrate"] = 8.0 * nbytes / n
         o["psnr"] = 20 * np.log10(rng) - 10 * np.log10(mse)
         o.pop("ex", None)

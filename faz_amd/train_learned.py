@@ -1,10 +1,11 @@
-"""Trains the learned tuner's MLP on tuples from the exhaustive QoZ tuner and
-evaluates it on held-out 384^3 fields.
+"""Trains the learned tuner's MLP on tuples from the exhaustive QoZ tuner,
+calibrates its confidence gates on held-out fields and evaluates it on
+held-out 384^3 fields.
 
-    python faz_amd/train_learned.py --fields 2000 --out faz_amd/models/learned_tuner.npz
+    python faz_amd/train_learned.py --fields 3000 --out faz_amd/models/learned_tuner.npz
 
-Writes the model (.npz) and a JSON report of the held-out evaluation next to
-it.  Needs the GPU.
+Writes the model (.npz, with the gates) and a JSON report of the held-out
+evaluation next to it.  Needs the GPU.
 """
 import argparse
 import json
@@ -23,12 +24,14 @@ from faz_amd import learned  # noqa: E402
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--fields", type=int, default=2000)
+    ap.add_argument("--fields", type=int, default=3000)
+    ap.add_argument("--val-fields", type=int, default=300)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--out", default=learned.MODEL_PATH)
     ap.add_argument("--eval-fields", type=int, default=4)
     ap.add_argument("--eval-shape", type=int, default=384)
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--budget-s", type=float, default=0, help="stop generating tuples after this many seconds")
     a = ap.parse_args()
 
     def log(m):
@@ -36,32 +39,52 @@ def main():
 
     e = faz_amd.Engine(0)
     t0 = time.time()
-    X, T = learned.build_dataset(e, learned.training_fields(a.fields, a.seed), log=log)
-    log(f"dataset: {X.shape[0]} tuples from {a.fields} fields in {time.time() - t0:.0f} s")
+    specs = learned.training_fields(a.fields + a.val_fields, a.seed)
+    # validation fields first, so a time budget cuts training fields only
+    order = specs[a.fields:] + specs[:a.fields]
+    X, T, G = learned.build_dataset(e, order, log=log, deadline=t0 + a.budget_s if a.budget_s > 0 else None)
+    log(f"dataset: {X.shape[0]} tuples from {int(G.max()) + 1} fields in {time.time() - t0:.0f} s")
+    train_m = G >= a.val_fields
     t1 = time.time()
-    net, mu, sigma, hist = learned.train(e, X, T, epochs=a.epochs, seed=a.seed, log=log)
+    net, mu, sigma, hist = learned.train(e, X[train_m], T[train_m], epochs=a.epochs, seed=a.seed, log=log)
     log(f"training: {time.time() - t1:.1f} s, final loss {hist[-1]:.5f}")
-    learned.save_model(a.out, net, mu, sigma, dict(fields=a.fields, tuples=int(X.shape[0]), epochs=a.epochs,
-                                                   seed=a.seed, final_loss=float(hist[-1])))
-    report = dict(fields=a.fields, tuples=int(X.shape[0]), epochs=a.epochs, loss=hist, results=[])
+    # kinds: eb scale of every tuple (input 48 = log2 scale)
+    scale = np.exp2(X[:, 48].astype(np.float64))
+    gates, gaps = learned.calibrate(net, mu, sigma, X[~train_m], T[~train_m], G[~train_m], scale[~train_m])
+    Ztrain = (X[train_m] - mu) / sigma
+    gates["lo"] = (Ztrain.min(0) - 1.0).astype(np.float32)
+    gates["hi"] = (Ztrain.max(0) + 1.0).astype(np.float32)
+    y = learned.predict(net, mu, sigma, X[~train_m])
+    br_err = np.abs(np.exp2(y[:, 0]) / np.exp2(T[~train_m, 0]) - 1)
+    ps_err = 5 * np.abs(y[:, 1] - T[~train_m, 1])
+    log(f"validation: bitrate rel. error mean {br_err.mean():.4f} p90 {np.quantile(br_err, 0.9):.4f}; PSNR error "
+        f"mean {ps_err.mean():.3f} dB; margin {gates['margin']:.4f} bits/value (gap p50 {np.median(gaps):.4f})")
+    n_train = int(G.max()) + 1 - a.val_fields
+    learned.save_model(a.out, net, mu, sigma, dict(fields=n_train, val_fields=a.val_fields, tuples=int(X.shape[0]),
+                                                   epochs=a.epochs, seed=a.seed, final_loss=float(hist[-1])), gates)
+    report = dict(fields=n_train, val_fields=a.val_fields, tuples=int(X.shape[0]), epochs=a.epochs, loss=hist,
+                  validation=dict(bitrate_rel_err=float(br_err.mean()), psnr_err_db=float(ps_err.mean()),
+                                  margin=gates["margin"]), results=[])
     r = np.random.default_rng(a.seed + 1)
     shape = (a.eval_shape,) * 3
-    for rel in (1e-3, 1e-4):
+    for rel in (1e-2, 1e-3, 1e-4):
         for k in range(a.eval_fields):
             slope = float(r.uniform(2.0, 5.0))
+            fronts = int(r.integers(0, 17))
             seed = int(r.integers(1 << 40)) | (1 << 41)  # disjoint from the training draws
-            res = learned.evaluate(e, net, mu, sigma, shape, seed, rel, slope=slope, fronts=int(r.integers(0, 17)))
+            res = learned.evaluate(e, net, mu, sigma, shape, seed, rel, slope=slope, fronts=fronts, gates=gates)
             ex, le, de = res["exhaustive"], res["learned"], res["default"]
             cr_e = 32.0 / ex["bitrate"]
             cr_l = learned.ratio_at_equal_psnr(le["bitrate"], le["psnr"], ex["psnr"])
             cr_d = learned.ratio_at_equal_psnr(de["bitrate"], de["psnr"], ex["psnr"])
-            row = dict(rel=rel, slope=slope, seed=seed, exhaustive=ex, learned=le, default=de, ratio_exhaustive=cr_e,
-                       ratio_learned_at_equal_psnr=cr_l, ratio_default_at_equal_psnr=cr_d,
+            row = dict(rel=rel, slope=slope, fronts=fronts, seed=seed, exhaustive=ex, learned=le, default=de,
+                       ratio_exhaustive=cr_e, ratio_learned_at_equal_psnr=cr_l, ratio_default_at_equal_psnr=cr_d,
                        speedup=ex["tune_ms"] / max(le["tune_ms"], 1e-9))
             report["results"].append(row)
-            log(f"rel={rel} slope={slope:.2f}: exhaustive ({ex['alpha']},{ex['beta']}) {ex['tune_ms']:.1f} ms "
-                f"CR {cr_e:.3f} PSNR {ex['psnr']:.2f} | learned ({le['alpha']},{le['beta']}) {le['tune_ms']:.1f} ms "
-                f"CR@PSNR {cr_l:.3f} | default ({de['alpha']},{de['beta']}) CR@PSNR {cr_d:.3f}")
+            log(f"rel={rel} slope={slope:.2f} fronts={fronts}: exhaustive ({ex['alpha']},{ex['beta']}) "
+                f"{ex['tune_ms']:.1f} ms CR {cr_e:.3f} | learned ({le['alpha']},{le['beta']}) path "
+                f"{le['learned_path']} verified {le['n_verified']} {le['tune_ms']:.1f} ms CR@PSNR {cr_l:.3f} | "
+                f"default ({de['alpha']},{de['beta']}) CR@PSNR {cr_d:.3f}")
     with open(os.path.splitext(a.out)[0] + "_eval.json", "w") as f:
         json.dump(report, f, indent=1, default=float)
     log("done")

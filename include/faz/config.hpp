@@ -5,6 +5,7 @@
 #pragma once
 
 #include <array>
+#include <climits>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -130,64 +131,86 @@ public:
         if (conditioning > 0) write(meta.data(), 17, c);
     }
 
-    // Config.hpp:311-388
-    void load(const uchar *&c) {
-        read(N, c);
+    // Config.hpp:311-388.  The reference reads without bounds; load(c, end)
+    // checks every read against `end` and the field values a stream cannot
+    // hold (N outside 1..4, counts beyond the bytes left) and returns false on
+    // a truncated or corrupt trailer.
+    void load(const uchar *&c) { load_from(c, nullptr); }
+    bool load(const uchar *&c, const uchar *end) { return end && load_from(c, end); }
+
+private:
+    bool load_from(const uchar *&c, const uchar *end) {
+        bool ok = true;
+        auto left = [&]() { return end ? (size_t) (end - c) : SIZE_MAX; };
+        auto rd = [&](auto &v) {
+            if (ok && sizeof(v) <= left()) read(v, c);
+            else ok = false;
+        };
+        auto rda = [&](auto *a, size_t n) {
+            if (ok && n <= left() / sizeof(*a)) read(a, n, c);
+            else ok = false;
+        };
+        rd(N);
+        if (!ok || (end && (N < 1 || N > 4))) return false;
         dims.resize(N);
-        read(dims.data(), N, c);
-        read(num, c);
-        read(cmprAlgo, c);
-        read(errorBoundMode, c);
-        read(tuningTarget, c);
-        read(absErrorBound, c);
-        read(relErrorBound, c);
-        read(alpha, c);
-        read(beta, c);
-        read(lorenzo, c);
-        read(lorenzo2, c);
-        read(regression, c);
-        read(regression2, c);
-        read(interpAlgo, c);
-        read(interpDirection, c);
-        read(interpBlockSize, c);
-        read(lossless, c);
-        read(encoder, c);
-        read(quantbinCnt, c);
-        read(blockSize, c);
-        read(levelwisePredictionSelection, c);
-        read(blockwiseTuning, c);
-        read(stride, c);
-        read(maxStep, c);
-        read(pred_dim, c);
-        read(openmp, c);
-        read(fixBlockSize, c);
-        read(blockwiseSampleBlockSize, c);
-        read(crossBlock, c);
-        read(wavelet, c);
-        read(firstSize, c);
-        read(offsetPredictor, c);
-        read(coeffs_num, c);
-        if (coeffs_num > 0) {
-            size_t k;
-            read(k, c);
+        rda(dims.data(), (size_t) N);
+        rd(num);
+        rd(cmprAlgo);
+        rd(errorBoundMode);
+        rd(tuningTarget);
+        rd(absErrorBound);
+        rd(relErrorBound);
+        rd(alpha);
+        rd(beta);
+        rd(lorenzo);
+        rd(lorenzo2);
+        rd(regression);
+        rd(regression2);
+        rd(interpAlgo);
+        rd(interpDirection);
+        rd(interpBlockSize);
+        rd(lossless);
+        rd(encoder);
+        rd(quantbinCnt);
+        rd(blockSize);
+        rd(levelwisePredictionSelection);
+        rd(blockwiseTuning);
+        rd(stride);
+        rd(maxStep);
+        rd(pred_dim);
+        rd(openmp);
+        rd(fixBlockSize);
+        rd(blockwiseSampleBlockSize);
+        rd(crossBlock);
+        rd(wavelet);
+        rd(firstSize);
+        rd(offsetPredictor);
+        rd(coeffs_num);
+        if (ok && coeffs_num > 0) {
+            size_t k = 0;
+            rd(k);
+            if (!ok || k > left() / sizeof(size_t)) return false;
             coeffs_dims.resize(k);
-            read(coeffs_dims.data(), k, c);
+            rda(coeffs_dims.data(), k);
         }
-        read(pid, c);
-        read(blockOrder, c);
-        read(sperr, c);
-        read(sperrWithoutWave, c);
-        read(pyBind, c);
-        if (pyBind) {
-            size_t m;
-            read(m, c);
+        rd(pid);
+        rd(blockOrder);
+        rd(sperr);
+        rd(sperrWithoutWave);
+        rd(pyBind);
+        if (ok && pyBind) {
+            size_t m = 0;
+            rd(m);
+            if (!ok || m > left()) return false;
             metadata.resize(m);
-            read(&metadata[0], m, c);
+            if (m) rda(&metadata[0], m);
         }
-        read(conditioning, c);
-        if (conditioning > 0) read(meta.data(), 17, c);
+        rd(conditioning);
+        if (ok && conditioning > 0) rda(meta.data(), 17);
+        return ok;
     }
 
+public:
     char N = 0;
     std::vector<size_t> dims;
     size_t num = 0;

@@ -142,6 +142,21 @@ faz_status faz_huffman_build(const uint32_t *h_freq, const uint32_t *h_first, in
 faz_status faz_gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *h_buf, size_t n, size_t num, uint32_t limit,
                                   int32_t *d_out, size_t *used);
 
+/* Host-only validation of a saved Huffman section (HuffmanEncoder::load,
+ * HuffmanEncoder.hpp:236-262), the check faz_gpu_huffman_decode makes before
+ * any device work: the node arrays parse, every slot reachable from the root
+ * is reached once (no shared children or cycles), no path is deeper than 64
+ * bits and the bitstream fits in n bytes; FAZ_ERR_FORMAT otherwise.  *nodes
+ * and *section_bytes (may be NULL) get the node count and the section's
+ * size. */
+faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, size_t *nodes, size_t *section_bytes);
+
+/* Host-only read of a stream's Config trailer (Config::load, Config.hpp:
+ * 311-388) with every field bounded by the stream: N (1..4), dims[N] and
+ * cmprAlgo (QoZ::ALGO); FAZ_ERR_FORMAT for a truncated or corrupt trailer
+ * (the reference reads past its buffer). */
+faz_status faz_stream_info(const uint8_t *stream, size_t n, int *N, size_t *dims, int *cmpr_algo);
+
 /* --- A17: HuffmanEncoder::decode + SZInterpolationCompressor::decompress
  * (HuffmanEncoder.hpp:236-296; SIC:42-199) on a pre-zstd engine buffer. */
 faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
@@ -383,6 +398,20 @@ typedef struct {
     size_t max_step;        /* Config::maxStep, 0: the QoZ default */
     int levelwise;          /* Config::levelwisePredictionSelection, 0: the QoZ default */
     int sample_block_size;  /* Config::sampleBlockSize, 0: the QoZ default */
+    int ab_list;            /* Config::abList: alphas {1..2}, 1: up to 2.5, 2: up to 3 (SZI:883-891) */
+    int test_lorenzo;       /* Config::testLorenzo: the Lorenzo CompressTest after the alpha/beta
+                               search (SZI:2431-2493); the QoZ preset clears it (SZI:1616), the
+                               FZ preset sets it (SZI:1645) */
+    /* learned tuner only (faz_gpu_tune_learned_f32; no reference counterpart):
+     * candidates whose predicted cost (bits/value at equal PSNR, 6.02 dB per
+     * bit) lies within learned_margin of the best are verified by real
+     * CompressTests, at most learned_verify of them (0: 4; learned_margin < 0:
+     * never); an input row whose standardised value leaves [learned_lo[i],
+     * learned_hi[i]] (FAZ_LEARNED_INPUTS each, NULL: no check) falls back to
+     * the exhaustive search. */
+    double learned_margin;
+    int learned_verify;
+    const float *learned_lo, *learned_hi;
 } faz_tune_in;
 
 typedef struct {
@@ -401,6 +430,14 @@ typedef struct {
     double test_alpha[128], test_beta[128], test_eb[128], test_bitrate[128], test_psnr[128];
     size_t test_bytes[128]; /* compressed bytes of each CompressTest */
     double ms;              /* device+host time of the tuner */
+    int use_interp;         /* 1: interpolation (ALGO_INTERP), 0: the Lorenzo test won (ALGO_LORENZO_REG) */
+    int learned_path;       /* learned tuner: 0 the network's choice, 1 chosen among real CompressTests of
+                               the closest candidates, 2 exhaustive fallback (input out of the training
+                               range); -1 the exhaustive tuner */
+    int n_verified;         /* real CompressTests the learned tuner ran */
+    double level_res[32];   /* [level-1] mean |mode-2 prediction error| per point of the chosen
+                               interpolator and direction (levels above levelwise folded into the
+                               top tuned level), in units of abs_eb */
 } faz_tune_out;
 
 faz_status faz_gpu_tune_qoz_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tune_out *out);
@@ -459,22 +496,31 @@ faz_status faz_mlp_set_params(faz_gpu_ctx *ctx, faz_mlp *net, const float *h, si
  *   RMS Laplacian, max |Laplacian|; 31-33 lag-1 and 34-36 lag-2 correlation
  *   per axis; 37 kurtosis of the stride-1 Laplacian; 38 share of it above
  *   twice its RMS; 39 share of stride-1 differences below eb_norm.
- * The MLP input row (FAZ_LEARNED_INPUTS): those statistics averaged over the
- * blocks (magnitudes as log2 of their ratio to eb_norm = eb / rng), alpha - 1,
- * beta / 4, log2 beta, level-1 interpolator and direction flag, their means
- * over the levels, log10 of the relative bound, log2 of the bound's scale.
- * Targets: bitrate / 4 and (PSNR + 20 log10(eb_norm * scale)) / 5. */
+ * The MLP input row (FAZ_LEARNED_INPUTS) of one candidate: those statistics
+ * averaged over the blocks (magnitudes as log2 of their ratio to eb_norm =
+ * eb / rng), alpha - 1, beta / 4, log2 beta, level-1 interpolator and
+ * direction flag, their means over the levels, log10 of the relative bound,
+ * log2 of the bound's scale, for levels 1..4 log2 of the level's mean mode-2
+ * prediction error over the candidate's bound of that level (level_res in
+ * units of the base bound, times min(alpha^(level-1), beta) / scale), and
+ * 1 for the Lorenzo candidate (alpha, beta entries 0) or 0.
+ * Targets: log2 bitrate and (PSNR + 20 log10(eb_norm * scale)) / 5. */
 #define FAZ_LEARNED_FEATURES 40
-#define FAZ_LEARNED_INPUTS 49
+#define FAZ_LEARNED_INPUTS 54
 faz_status faz_gpu_block_features_f32(faz_gpu_ctx *ctx, const float *d_in, int N, const size_t *dims,
                                       const size_t *starts, size_t nblocks, size_t sbs, float vmin, float rng,
                                       double eb_norm, float *h_feats);
 faz_status faz_learned_input(const float *block_feats, size_t nblocks, double eb_norm, double rel_eb, double alpha,
                              double beta, double eb_scale, int levelwise, const uint8_t *ops, const uint8_t *dirs,
-                             float *row);
+                             const double *level_res, int lorenzo, float *row);
 faz_status faz_learned_target(double bitrate, double psnr, double eb_norm, double eb_scale, float *t);
-/* The tuner with the trained network (inputs standardised as (x - mu) / sigma);
- * out as faz_gpu_tune_qoz_f32, with predicted tests (test_bytes 0). */
+/* The tuner with the trained network (inputs standardised as (x - mu) / sigma):
+ * the network predicts every alpha/beta candidate (and the Lorenzo test when
+ * in->test_lorenzo) at once and the lowest predicted bits/value at equal PSNR
+ * wins, unless the confidence gates of faz_tune_in send the closest
+ * candidates to real CompressTests or the whole search to the exhaustive
+ * tuner.  out as faz_gpu_tune_qoz_f32: predicted tests (test_bytes 0), then
+ * the real ones. */
 faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_mlp *net,
                                     const float *mu, const float *sigma, faz_tune_out *out);
 

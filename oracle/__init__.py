@@ -301,6 +301,8 @@ class TuneIn(ctypes.Structure):
         ("max_step", ctypes.c_size_t),
         ("levelwise", ctypes.c_int),
         ("sample_block_size", ctypes.c_int),
+        ("ab_list", ctypes.c_int),
+        ("test_lorenzo", ctypes.c_int),
     ]
 
 
@@ -325,10 +327,12 @@ class TuneOut(ctypes.Structure):
         ("test_bitrate", ctypes.c_double * 128),
         ("test_psnr", ctypes.c_double * 128),
         ("test_bytes", ctypes.c_size_t * 128),
+        ("use_interp", ctypes.c_int),
     ]
 
 
-def qoz_tune(data, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0, sample_block_size=0):
+def qoz_tune(data, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelwise=0, sample_block_size=0,
+             ab_list=0, test_lorenzo=0):
     """QoZ exhaustive tuner (faz -q).  Returns a dict of the decisions and of
     every intermediate result."""
     data = np.ascontiguousarray(data, dtype=np.float32)
@@ -341,6 +345,7 @@ def qoz_tune(data, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelw
     t.abs_eb, t.rel_eb, t.rng = float(abs_eb), float(rel_eb), float(rng)
     t.interp_block_size, t.max_step, t.levelwise, t.sample_block_size = interp_block_size, max_step, levelwise, \
         sample_block_size
+    t.ab_list, t.test_lorenzo = int(ab_list), int(test_lorenzo)
     o = TuneOut()
     rc = L.oracle_qoz_tune(ctypes.byref(t), _ptr(data), ctypes.byref(o))
     assert rc == 0, rc
@@ -353,7 +358,8 @@ def qoz_tune(data, abs_eb, rel_eb, rng, interp_block_size=32, max_step=0, levelw
                 block_starts=np.array([list(o.block_starts[i]) for i in range(min(nb, 4096))], np.int64),
                 pred_loss=np.array([list(o.pred_loss[i]) for i in range(32)]),
                 tests=[dict(alpha=o.test_alpha[i], beta=o.test_beta[i], eb=o.test_eb[i], bitrate=o.test_bitrate[i],
-                            psnr=o.test_psnr[i], bytes=o.test_bytes[i]) for i in range(nt)])
+                            psnr=o.test_psnr[i], bytes=o.test_bytes[i]) for i in range(nt)],
+                use_interp=o.use_interp)
 
 
 def compress_test(field, starts, sbs, params, rng):

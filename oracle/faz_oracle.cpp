@@ -2074,6 +2074,14 @@ static void compress_test(const TuneState &ts, const oracle_interp_params &p, do
     psnr = 20 * std::log10(rng) - 10 * std::log10(mse);
 }
 
+}  // namespace
+
+extern "C" int oracle_compress_test_lorenzo(int N, double eb, const float *field, const size_t *fdims,
+                                            const size_t *starts, size_t nblocks, size_t sbs, uint8_t *raw, size_t cap,
+                                            size_t *raw_size, size_t *bytes, double *square_error);
+
+namespace {
+
 static int qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out *out) {
     const int N = in->N;
     if (N != 2 && N != 3) return -1;
@@ -2147,9 +2155,13 @@ static int qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out
     p.levelwise = levelwise;
     memcpy(p.ops, ops.data(), levelwise);
     memcpy(p.dirs, dirs.data(), levelwise);
-    const double alphas[5] = {1, 1.25, 1.5, 1.75, 2};
+    // init_alphalist / init_betalist, RD target, no wavelets (SZI:869-933)
+    std::vector<double> alphas = {1, 1.25, 1.5, 1.75, 2};
+    if (in->ab_list == 1) alphas.insert(alphas.end(), {2.25, 2.5});
+    if (in->ab_list >= 2) alphas.insert(alphas.end(), {2.25, 2.5, 2.75, 3});
     const double betas[4] = {1.5, 2, 3, 4};
     double best_a = 1, best_b = 1, bestb = 9999, bestm = 0;
+    out->use_interp = 1;
     auto test = [&](double a, double b, double eb, double &br, double &ps) {
         oracle_interp_params q = p;
         q.alpha = a, q.beta = b, q.eb = eb;
@@ -2162,7 +2174,7 @@ static int qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out
         }
         out->n_tests++;
     };
-    for (int i = 0; i < 5; i++) {
+    for (size_t i = 0; i < alphas.size(); i++) {
         for (int j = 0; j < 4; j++) {
             const double alpha = alphas[i], beta = betas[j];
             if (alpha >= 1 && alpha > beta) continue;
@@ -2185,6 +2197,48 @@ static int qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out
                 if (reg > bestm) best_a = alpha, best_b = beta, bestb = bitrate, bestm = metric;
             }
             if (stop) break;
+        }
+    }
+    // the Lorenzo test (SZI:2431-2493): CompressTest with ALGO_LORENZO_REG,
+    // useFast = false, lorenzoBrFix = 1, recorded with alpha = beta = -1
+    if (in->test_lorenzo) {
+        const size_t nb = std::min<size_t>(out->n_blocks, 4096);
+        std::vector<size_t> starts(3 * nb);
+        for (size_t b = 0; b < nb; b++)
+            for (int a = 0; a < 3; a++) starts[3 * b + a] = out->block_starts[b][a];
+        const double ele = std::pow((double) (sbs + 1), N) * (double) nb;
+        auto lor = [&](double eb, double &br, double &ps) -> bool {
+            size_t raw_size = 0, bytes = 0;
+            double sq = 0;
+            // no buffer asked for: -3 (capacity) after the size and error are set
+            const int rc = oracle_compress_test_lorenzo(N, eb, data, in->dims, starts.data(), nb, (size_t) sbs,
+                                                        nullptr, 0, &raw_size, &bytes, &sq);
+            if (rc != 0 && rc != -3) return false;
+            br = 8 * double(bytes) / ele;
+            ps = 20 * std::log10(in->rng) - 10 * std::log10(sq / ele);
+            int t = out->n_tests;
+            if (t < 128) {
+                out->test_alpha[t] = -1, out->test_beta[t] = -1, out->test_eb[t] = eb;
+                out->test_bitrate[t] = br, out->test_psnr[t] = ps, out->test_bytes[t] = bytes;
+            }
+            out->n_tests++;
+            return true;
+        };
+        double bitrate, metric;
+        if (!lor(in->abs_eb, bitrate, metric)) return -2;
+        if (metric >= bestm && bitrate <= bestb) {
+            bestb = bitrate, bestm = metric, best_a = best_b = -1, out->use_interp = 0;
+        } else if (metric <= bestm && bitrate >= bestb) {
+            out->use_interp = 1;
+        } else {
+            const double fix = bitrate / bestb;
+            double br_r, m_r;
+            if (!lor(in->abs_eb * fix, br_r, m_r)) return -2;
+            const double a = (metric - m_r) / (bitrate - br_r);
+            if (a > 0) {
+                const double b = metric - a * bitrate;
+                if (a * bestb + b > bestm) bestb = bitrate, bestm = metric, best_a = best_b = -1, out->use_interp = 0;
+            }
         }
     }
     out->alpha = best_a;

@@ -179,6 +179,8 @@ typedef struct {
     size_t max_step;          /* conf.maxStep (0: QoZ default) */
     int levelwise;            /* conf.levelwisePredictionSelection (0: QoZ default) */
     int sample_block_size;    /* conf.sampleBlockSize (0: QoZ default) */
+    int ab_list;              /* conf.abList (SZI:883-891) */
+    int test_lorenzo;         /* conf.testLorenzo: the Lorenzo test of SZI:2431-2493 */
 } oracle_tune_in;
 
 typedef struct {
@@ -194,6 +196,7 @@ typedef struct {
     int n_tests;              /* CompressTest calls in order */
     double test_alpha[128], test_beta[128], test_eb[128], test_bitrate[128], test_psnr[128];
     size_t test_bytes[128];   /* compressed bytes of encoding_lossless */
+    int use_interp;           /* 0: the Lorenzo test won (alpha = beta = -1) */
 } oracle_tune_out;
 
 int oracle_qoz_tune(const oracle_tune_in *in, const float *data, oracle_tune_out *out);

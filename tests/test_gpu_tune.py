@@ -52,7 +52,7 @@ def _compare(g, o):
         assert (a["alpha"], a["beta"], a["eb"]) == (b["alpha"], b["beta"], b["eb"])
         assert a["bytes"] == b["bytes"]
         assert a["bitrate"] == b["bitrate"] and a["psnr"] == b["psnr"]
-    for k in ("alpha", "beta", "levelwise", "ops", "dirs", "interp_algo", "direction", "max_step"):
+    for k in ("alpha", "beta", "levelwise", "ops", "dirs", "interp_algo", "direction", "max_step", "use_interp"):
         assert g[k] == o[k], k
 
 
@@ -66,6 +66,42 @@ def test_tuner_matches_oracle(engine, i):
     o = oracle.qoz_tune(x, eb, rel, rng)
     g = engine.tune_qoz(torch.from_numpy(x.ravel()).cuda(), shape, eb, rel, rng)
     _compare(g, o)
+
+
+def _walk(shape, seed):
+    """Random walk along every axis: rough at the sample-block scale, where
+    the Lorenzo test can win."""
+    x = np.random.default_rng(seed).standard_normal(shape)
+    for ax in range(len(shape)):
+        x = np.cumsum(x, axis=ax)
+    return x.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,rel,kind,ab_list", [((96, 96, 96), 1e-3, "walk", 0), ((300, 400), 1e-2, "walk", 1),
+                                                    ((130, 140, 150), 1e-4, "grf", 2), ((200, 200, 200), 1e-2, "grf", 1),
+                                                    ((120, 130, 110), 1e-5, "walk", 2)])
+def test_tuner_lorenzo_test_and_alpha_lists_match_oracle(engine, shape, rel, kind, ab_list):
+    """conf.testLorenzo (the Lorenzo CompressTest after the alpha/beta search,
+    SZI:2431-2493) and the wider alpha lists of conf.abList (SZI:883-891)."""
+    x = _walk(shape, 5) if kind == "walk" else _grf(shape, seed=6, fronts=2)
+    rng = float(np.float32(x.max() - x.min()))
+    eb = rel * rng
+    print(f"[gpu] tuner {shape} {kind} rel={rel} abList={ab_list} testLorenzo=1", flush=True)
+    o = oracle.qoz_tune(x, eb, rel, rng, ab_list=ab_list, test_lorenzo=1)
+    g = engine.tune_qoz(torch.from_numpy(x.ravel()).cuda(), shape, eb, rel, rng, ab_list=ab_list, test_lorenzo=1)
+    _compare(g, o)
+    assert any(t["alpha"] < 0 for t in g["tests"])  # the Lorenzo test ran
+    if ab_list:
+        assert max(t["alpha"] for t in g["tests"]) > 2.0 or g["tests"][-1]["alpha"] < 0
+
+
+def test_tuner_lorenzo_wins_on_a_rough_field(engine):
+    x = _walk((96, 96, 96), 5)
+    rng = float(np.float32(x.max() - x.min()))
+    g = engine.tune_qoz(torch.from_numpy(x.ravel()).cuda(), x.shape, 1e-3 * rng, 1e-3, rng, test_lorenzo=1)
+    o = oracle.qoz_tune(x, 1e-3 * rng, 1e-3, rng, test_lorenzo=1)
+    print(f"[gpu] rough field: use_interp GPU {g['use_interp']} oracle {o['use_interp']}", flush=True)
+    assert g["use_interp"] == o["use_interp"]
 
 
 from golden_cases import CTEST, ctest_params, load  # noqa: E402

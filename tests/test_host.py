@@ -113,3 +113,108 @@ def test_fastdiv_is_exact(tmp_path):
                     "-I" + os.path.join(here, "faz_amd", "csrc"), str(src), "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and r.stdout.strip() == "0"
+
+
+# --- malformed input, host only ------------------------------------------------
+
+def _section(left, right, leafs, syms, enc=b"", nbits_bytes=None):
+    """A Huffman section in the saved layout (HuffmanEncoder.hpp:119-137,
+    569-633): int offset, big-endian node count and stateNum / 2, then the
+    endian byte, L[], R[] (u8 for <= 256 nodes), C[] (int32), t[] and the
+    size_t byte count of the bits."""
+    import struct
+    nc = len(left)
+    assert nc <= 256
+    b = struct.pack("<i", 0) + struct.pack(">I", nc) + struct.pack(">I", 1)
+    b += b"\0" + bytes(left) + bytes(right) + b"".join(struct.pack("<i", c) for c in syms) + bytes(leafs)
+    b += struct.pack("<Q", len(enc) if nbits_bytes is None else nbits_bytes) + enc
+    return b
+
+
+def test_huffman_check_accepts_reference_sections():
+    for name in HUFFMAN:
+        sec = load(name)["encoded"].tobytes()
+        nodes, used = faz_amd.huffman_check_section(sec)
+        assert used == len(sec) and nodes >= 1, name
+
+
+def test_huffman_check_rejects_dag_cycle_deep_and_truncated_trees():
+    # a valid three-node tree: root with two leaves
+    ok = _section([1, 0, 0], [2, 0, 0], [0, 1, 1], [0, 5, 7], enc=b"\x80")
+    assert faz_amd.huffman_check_section(ok) == (3, len(ok))
+    # DAG: inner nodes i share one child (left = right = i + 1) for 60 levels;
+    # a walk over all paths would take 2^60 steps
+    n = 62
+    L = [i + 1 for i in range(n - 1)] + [0]
+    leafs = [0] * (n - 1) + [1]
+    dag = _section(L, L, leafs, [0] * n)
+    with pytest.raises(faz_amd.FazError) as e:
+        faz_amd.huffman_check_section(dag)
+    assert e.value.status == -5
+    # cycle: 1 -> 2 -> 1
+    cyc = _section([1, 2, 1, 0], [3, 0, 0, 0], [0, 0, 0, 1], [0, 0, 0, 4])
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.huffman_check_section(cyc)
+    # a proper tree 70 levels deep (each inner node: left leaf, right inner)
+    depth = 70
+    nc = 2 * depth + 1
+    Ld, Rd, lf = [0] * nc, [0] * nc, [0] * nc
+    for i in range(depth):
+        inner, leaf_i, nxt = 2 * i, 2 * i + 1, 2 * i + 2
+        Ld[inner], Rd[inner] = leaf_i, nxt
+        lf[leaf_i] = 1
+    lf[nc - 1] = 1
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.huffman_check_section(_section(Ld, Rd, lf, list(range(nc))))
+    # bitstream longer than the buffer; child index out of range
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.huffman_check_section(_section([1, 0, 0], [2, 0, 0], [0, 1, 1], [0, 5, 7], nbits_bytes=1000))
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.huffman_check_section(_section([1, 0, 0], [9, 0, 0], [0, 1, 1], [0, 5, 7]))
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.huffman_check_section(ok[:10])
+
+
+def _trailer_offset(stream):
+    n = int(np.frombuffer(stream[-4:], np.int32)[0])
+    return len(stream) - 4 - n
+
+
+def test_stream_info_reads_reference_trailers():
+    from golden_cases import INTERP
+    for name in INTERP:
+        c = load(name)
+        s = c["stream"].tobytes()
+        N, dims, algo = faz_amd.stream_info(s)
+        assert dims == tuple(c["params"]["dims"]) and N == len(dims) and algo == 2, name
+
+
+def test_stream_info_rejects_truncated_and_corrupt_trailers():
+    import struct
+    from golden_cases import INTERP
+    s = bytearray(load(INTERP[0])["stream"].tobytes())
+    at = _trailer_offset(s)
+    conf_size = len(s) - 4 - at
+
+    def bad(b):
+        with pytest.raises(faz_amd.FazError) as e:
+            faz_amd.stream_info(bytes(b))
+        assert e.value.status == -5
+
+    with pytest.raises(faz_amd.FazError):
+        faz_amd.stream_info(b"")
+    bad(s[:3])
+    # trailer size beyond the stream, or a trailer cut short
+    bad(s[:-4] + struct.pack("<i", len(s)))
+    for keep in (1, 8, conf_size // 2):
+        t = s[at:at + keep]
+        bad(s[:at] + t + struct.pack("<i", len(t)))
+    # N outside 1..4 (a signed char in the reference)
+    for v in (0, 5, 0x80, 0xFF):
+        b = bytearray(s)
+        b[at] = v
+        bad(b)
+    # a huge dimension count would ask for an impossible allocation in the reference
+    b = bytearray(s)
+    b[at] = 0x7F
+    bad(b)
