@@ -105,6 +105,7 @@ class GeneralParams(ctypes.Structure):
         ("lorenzo", ctypes.c_int),
         ("lorenzo2", ctypes.c_int),
         ("radius", ctypes.c_int),
+        ("regression", ctypes.c_int),
     ]
 
 
@@ -115,10 +116,12 @@ class GeneralOut(ctypes.Structure):
         ("d_outval", ctypes.c_void_p),
         ("d_selection", ctypes.c_void_p),
         ("n_selection", ctypes.c_size_t),
+        ("n_regression", ctypes.c_size_t),
+        ("passes", ctypes.c_int),
     ]
 
 
-def make_general_params(dims, eb, block_size=None, lorenzo=True, lorenzo2=True, radius=32768):
+def make_general_params(dims, eb, block_size=None, lorenzo=True, lorenzo2=True, radius=32768, regression=False):
     """General-frontend parameters; block_size defaults to the Config's for N
     (128, 16, 6, 6)."""
     p = GeneralParams()
@@ -132,6 +135,7 @@ def make_general_params(dims, eb, block_size=None, lorenzo=True, lorenzo2=True, 
     p.lorenzo = int(bool(lorenzo))
     p.lorenzo2 = int(bool(lorenzo2))
     p.radius = int(radius)
+    p.regression = int(bool(regression))
     return p
 
 
@@ -169,6 +173,7 @@ class TuneIn(ctypes.Structure):
         ("learned_verify", ctypes.c_int),
         ("learned_lo", ctypes.c_void_p),
         ("learned_hi", ctypes.c_void_p),
+        ("learned_margin_lorenzo", ctypes.c_double),
     ]
 
 
@@ -357,7 +362,8 @@ def lib():
         L.faz_gpu_tune_qoz_f32.argtypes = [vp, vp, P(TuneIn), P(TuneOut)]
         L.faz_gpu_general_compress_f32.argtypes = [vp, vp, vp, P(GeneralParams), P(GeneralOut)]
         L.faz_gpu_general_raw_f32.argtypes = [vp, vp, vp, P(GeneralParams), vp, sz, P(sz)]
-        L.faz_gpu_general_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, sz]
+        L.faz_gpu_general_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, vp, sz]
         L.faz_gpu_compress_test_lorenzo_f32.argtypes = [vp, vp, ctypes.c_int, vp, vp, sz, sz, ctypes.c_double, vp,
                                                         sz, P(sz), P(sz), P(ctypes.c_double)]
         L.faz_sz_compress_general_f32.argtypes = [vp, P(GeneralParams), ctypes.c_int, ctypes.c_double, vp,
@@ -721,7 +727,8 @@ class Engine:
     # --- general frontend (Lorenzo predictors, any N) ------------------------
     def general_compress(self, x, params, recon=None):
         """Returns dict(codes uint32 block-major, outval at zero codes,
-        selection uint8, recon tensor)."""
+        selection uint8 (255: none recorded), recon tensor, n_regression,
+        passes)."""
         import torch
         if recon is None:
             recon = torch.empty_like(x)
@@ -733,7 +740,8 @@ class Engine:
         codes = d2h(out.d_codes, out.n_codes * 4, np.uint32)
         outval = d2h(out.d_outval, out.n_codes * 4, np.float32)
         return dict(codes=codes, unpred=outval[codes == 0].copy(),
-                    selection=d2h(out.d_selection, out.n_selection, np.uint8), recon=recon)
+                    selection=d2h(out.d_selection, out.n_selection, np.uint8), recon=recon,
+                    n_regression=out.n_regression, passes=out.passes)
 
     def general_raw(self, x, params, recon=None):
         import torch
@@ -749,12 +757,13 @@ class Engine:
                                                     buf.size, ctypes.byref(need)), "general_raw")
         return buf[:need.value].tobytes()
 
-    def general_decompress_raw(self, raw, dims, out, lorenzo=True, lorenzo2=True):
+    def general_decompress_raw(self, raw, dims, out, lorenzo=True, lorenzo2=True, regression=False):
         import torch
         b = np.frombuffer(raw, np.uint8)  # read only by the library
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_general_decompress_raw(self.h, _np_ptr(b), b.size, len(dims), int(bool(lorenzo)),
-                                                           int(bool(lorenzo2)), self._dev(out), int(np.prod(dims))),
+                                                           int(bool(lorenzo2)), int(bool(regression)),
+                                                           self._dev(out), int(np.prod(dims))),
                     "general_decompress_raw")
         return out
 
@@ -951,7 +960,7 @@ class Engine:
         return out
 
     def tune_learned(self, x, shape, abs_eb, rel_eb, rng, mlp, mu, sigma, interp_block_size=32, test_lorenzo=0,
-                     margin=-1.0, verify=0, lo=None, hi=None, ab_list=0):
+                     margin=-1.0, verify=0, lo=None, hi=None, ab_list=0, margin_lorenzo=-1.0):
         """The learned tuner: the network's choice among the alpha/beta (and,
         with test_lorenzo, Lorenzo) candidates; candidates predicted within
         `margin` bits/value of the best are verified by real CompressTests (at
@@ -967,6 +976,7 @@ class Engine:
         t.interp_block_size = interp_block_size
         t.ab_list, t.test_lorenzo = int(ab_list), int(test_lorenzo)
         t.learned_margin, t.learned_verify = float(margin), int(verify)
+        t.learned_margin_lorenzo = float(margin_lorenzo)
         keep = []
         if lo is not None and hi is not None:
             lo = np.ascontiguousarray(lo, np.float32)

@@ -103,6 +103,8 @@ struct faz_gpu_ctx {
     faz::DevBuf<float> lat[2];
     faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     faz::DevBuf<uint32_t> hist_pack;  // one readback block after the histogram
+    faz::DevBuf<uint32_t> enc_head;   // one-pass encode: bits of each chunk's first, shared word
+    bool enc_onepass = true;          // FAZ_ENCODE=twopass selects the sizes + scan + pack encoder
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to
@@ -156,6 +158,8 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint32_t> gf_codes, gf_tab, gf_diag;
     faz::DevBuf<uint8_t> gf_sel;
     faz::DevBuf<float> gf_fields, gf_recon;
+    faz::DevBuf<float> gf_reg_raw, gf_reg_deq;  // RegressionPredictor planes, 5 floats per block
+    faz::DevBuf<uint32_t> gf_reg_code;          // 1-D: coefficient codes per block
     uint32_t gf_tab_key = 0;  // N << 16 | bs of the table
     // Huffman decoder: the bitstream as words, the lookup table and tree,
     // per-chunk start and symbol offset, symbol counts + error flag, the

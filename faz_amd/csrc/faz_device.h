@@ -21,14 +21,15 @@ __device__ __forceinline__ int32_t x86_trunc_i32(double v) {
 
 // The same for v = |diff| * recip with recip > 0 (v is >= 0 or NaN): one
 // comparison sends NaN and everything from 2^31 up to INT_MIN.
-__device__ __forceinline__ int32_t x86_trunc_i32_nonneg(double v) {
+__host__ __device__ __forceinline__ int32_t x86_trunc_i32_nonneg(double v) {
     return v < 2147483648.0 ? (int32_t) v : INT32_MIN;
 }
 
 // LinearQuantizer::quantize_and_overwrite with trimToZero == 0
-// (IntegerQuantizer.hpp:69-108); recip = 1 / eb > 0.  Returns the code; *rec gets the value the
+// (IntegerQuantizer.hpp:69-108); recip = 1 / eb > 0.  Also built for the host
+// (the RegressionPredictor's coefficient chain).  Returns the code; *rec gets the value the
 // reference leaves in its data buffer (the original when unpredictable).
-__device__ __forceinline__ int quantize_point(float orig, float pred, double eb, double recip, int radius,
+__host__ __device__ __forceinline__ int quantize_point(float orig, float pred, double eb, double recip, int radius,
                                               float *rec) {
     float diff = orig - pred;
     uint32_t qi = (uint32_t) x86_trunc_i32_nonneg((double) fabsf(diff) * recip) + 1u;
@@ -93,7 +94,7 @@ __device__ __forceinline__ int quantize_bounded(float orig, float pred, double e
 }
 
 // LinearQuantizer::recover (IntegerQuantizer.hpp:160-175)
-__device__ __forceinline__ float recover_point(float pred, int code, int radius, double eb) {
+__host__ __device__ __forceinline__ float recover_point(float pred, int code, int radius, double eb) {
     return (float) ((double) pred + (double) (2 * (code - radius)) * eb);
 }
 

@@ -256,6 +256,135 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
                                    chunk_off[c], s_tab, tab, tw, offset, s_words, s_wave, words);
 }
 
+// One-pass encode: chunks taken in ticket order; each publishes its bit
+// total at once and finds its bit offset by a decoupled look-back over the
+// published totals and inclusive prefixes (Merrill and Garland, "Single-pass
+// Parallel Prefix Scan with Decoupled Look-back", 2016), 64 predecessors per
+// step across wave 0.  Every output word is stored by exactly one chunk: the
+// one whose range holds the word's first bit.  A chunk starting inside a
+// word hands its bits of that word to head_bits[c] instead, and a fix-up
+// kernel ORs them in after the pack, so the output needs no zero fill and no
+// chunk waits for another's stores.  Needs codewords of at most 32 bits (the
+// LDS bit buffer holds 32 bits per code).
+constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2;
+constexpr uint32_t kLookbackSpins = 1u << 24;
+
+__device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template<bool LDS_TAB>
+__global__ void __launch_bounds__(kEncThreads) encode_onepass_kernel(
+        const uint16_t *__restrict__ codes, uint32_t n, const uint64_t *__restrict__ tab, TabWindow tw, int32_t offset,
+        uint32_t *__restrict__ words, uint64_t *__restrict__ status, uint64_t *__restrict__ chunk_off,
+        uint32_t *__restrict__ head_bits, uint32_t *__restrict__ ticket, uint64_t *__restrict__ total_bits,
+        uint32_t *__restrict__ err, int aligned) {
+    extern __shared__ uint64_t s_tab[];
+    __shared__ uint32_t s_words[kPackWords];
+    __shared__ uint32_t s_wave[kEncThreads / 64];
+    __shared__ uint32_t s_c;
+    __shared__ uint64_t s_prefix;
+    stage_table<LDS_TAB>(s_tab, tab, tw);
+    const uint32_t nchunks = (n + kEncChunk - 1) / kEncChunk;
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (;;) {
+        if (threadIdx.x == 0) s_c = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t c = s_c;
+        if (c >= nchunks) break;
+        const uint32_t end = min(c * kEncChunk + kEncChunk, n);
+        const bool full = aligned && end - c * kEncChunk == kEncChunk;
+        const Codes16 v = full ? fetch_full(codes, c) : fetch_chunk(codes, c, n, aligned != 0);
+        const uint32_t my0 = c * kEncChunk + threadIdx.x * kEncPerThread;
+        uint64_t e[16];
+        uint32_t mybits = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            e[k] = my0 + k < end ? entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) : 0;
+            mybits += (uint32_t) e[k] & 0xFF;
+        }
+        uint32_t total;
+        const uint32_t before = wg_exclusive(mybits, s_wave, &total);
+        // the chunk's offset: its total goes out at once, then the look-back
+        if (wave == 0) {
+            uint64_t prefix = 0;
+            if (c == 0) {
+                if (lane == 0) status_store(&status[0], ((uint64_t) total << 2) | kFlagPrefix);
+            } else {
+                if (lane == 0) status_store(&status[c], ((uint64_t) total << 2) | kFlagAgg);
+                int64_t j = (int64_t) c - 1;
+                uint32_t spins = 0;
+                for (;;) {
+                    const int64_t idx = j - (int64_t) lane;
+                    // before chunk 0: an inclusive prefix of 0
+                    const uint64_t sv = idx >= 0 ? status_load(&status[idx]) : kFlagPrefix;
+                    const uint32_t flag = (uint32_t) (sv & 3);
+                    const uint64_t pmask = __ballot(flag == kFlagPrefix);
+                    const uint64_t zmask = __ballot(flag == 0);
+                    const uint32_t firstp = pmask ? (uint32_t) __ffsll((unsigned long long) pmask) - 1 : 64u;
+                    // every predecessor nearer than the first prefix must have its total out
+                    const uint64_t nearer = firstp < 64 ? ((1ull << firstp) - 1) : ~0ull;
+                    if (zmask & nearer) {
+                        // a predecessor has not published its total yet
+                        if (++spins > kLookbackSpins) {
+                            if (lane == 0) atomicOr(err, 4u);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    uint64_t add = lane <= firstp ? (sv >> 2) : 0;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) add += (uint64_t) __shfl_xor((long long) add, o);
+                    prefix += add;
+                    if (firstp < 64) break;
+                    j -= 64;
+                }
+                if (lane == 0) status_store(&status[c], ((prefix + total) << 2) | kFlagPrefix);
+            }
+            if (lane == 0) {
+                s_prefix = prefix;
+                chunk_off[c] = prefix;
+                if (c == nchunks - 1) *total_bits = prefix + total;
+            }
+        }
+        __syncthreads();
+        const uint64_t g0 = s_prefix;
+        const uint32_t head = (uint32_t) (g0 & 31);
+        const uint32_t nw = (head + total + 31) / 32;
+        const uint64_t gw = g0 >> 5;
+        for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
+        __syncthreads();
+        uint32_t pos = head + before;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t L = (uint32_t) e[k] & 0xFF;
+            if (L) put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&s_words[w], bits); });
+            pos += L;
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
+            const uint32_t val = bswap32(s_words[i]);
+            if (i == 0 && head) head_bits[c] = val;  // the word's first bit belongs to an earlier chunk
+            else words[gw + i] = val;
+        }
+        if (threadIdx.x == 0 && (nw == 0 || !head)) head_bits[c] = 0;
+        __syncthreads();
+    }
+}
+
+// ORs every chunk's bits of the word it starts inside into the output.
+__global__ void encode_head_fix_kernel(const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ head_bits,
+                                       uint32_t nchunks, uint32_t *__restrict__ words) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nchunks; c += gridDim.x * blockDim.x) {
+        const uint32_t hb = head_bits[c];
+        if (hb) atomicOr(&words[chunk_off[c] >> 5], hb);
+    }
+}
+
 uint32_t encode_grid(uint32_t nchunks) {
     static uint32_t ncu = 0;
     if (!ncu) {
@@ -300,5 +429,26 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab
 }
 
 uint32_t encode_lds_entries() { return kTabLds; }
+
+void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
+                           int32_t offset, uint32_t *d_words, uint64_t *d_status, uint64_t *d_chunk_off,
+                           uint32_t *d_head_bits, uint32_t *d_ticket, uint64_t *d_total_bits, uint32_t *d_err,
+                           hipStream_t st) {
+    const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
+    if (!nchunks) return;
+    const int aligned = ((uintptr_t) d_codes & 15) == 0;
+    const TabWindow tw{w.win, w.nwin, w.e0};
+    const dim3 grid(encode_grid(nchunks));
+    if (w.nwin && w.nwin <= kTabLds)
+        hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), w.nwin * 8, st, d_codes, (uint32_t) n,
+                           d_tab, tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_ticket, d_total_bits,
+                           d_err, aligned);
+    else
+        hipLaunchKernelGGL(encode_onepass_kernel<false>, grid, dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_tab,
+                           tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_ticket, d_total_bits, d_err,
+                           aligned);
+    hipLaunchKernelGGL(encode_head_fix_kernel, dim3(std::min<uint32_t>((nchunks + 255) / 256, 1024)), dim3(256), 0,
+                       st, d_chunk_off, d_head_bits, nchunks, d_words);
+}
 
 }  // namespace faz

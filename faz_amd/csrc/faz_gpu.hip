@@ -405,6 +405,8 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (kern && strcmp(kern, "pass") == 0) ctx->fused3d = false;
     if (kern && (strcmp(kern, "pass") == 0 || strcmp(kern, "level") == 0)) ctx->tile3d = false;
     if (kern && strcmp(kern, "tile") == 0) ctx->fused_min_blocks = 1;
+    const char *enc = getenv("FAZ_ENCODE");
+    if (enc && strcmp(enc, "twopass") == 0) ctx->enc_onepass = false;
     // the allocation-free encoder's scratch address on this device (its
     // entry point reports a failure here)
     (void) noalloc_init(device);
@@ -917,6 +919,18 @@ static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, c
     HIPCHECK(ctx->scalars.ensure(4));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(hipMemcpyAsync(ctx->enc_code.p + up_lo, ctx->h_tab.data() + up_lo, up_n * 8, hipMemcpyHostToDevice, st));
+    if (ctx->enc_onepass && maxbits <= 32) {
+        // chunk status words and the ticket (scalars[2], low half) start at zero
+        HIPCHECK(ctx->enc_head.ensure(nchunks));
+        HIPCHECK(ctx->err_flag.ensure(1));
+        launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->chunk_bits.p), 2 * (size_t) nchunks, 0u, st);
+        launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->scalars.p + 2), 2, 0u, st);
+        launch_encode_onepass(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->words.p, ctx->chunk_bits.p,
+                              ctx->chunk_off.p, ctx->enc_head.p, reinterpret_cast<uint32_t *>(ctx->scalars.p + 2),
+                              ctx->scalars.p + 1, ctx->err_flag.p, st);
+        HIPCHECK(hipGetLastError());
+        return FAZ_OK;
+    }
     launch_encode_sizes(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_bits.p, st);
     launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
     hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned) std::min<size_t>((cap + 255) / 256, 8192)), dim3(256), 0, st,
@@ -1028,7 +1042,10 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     memcpy(&n_unpred, hf + kHistPackFreq0 + 4, 8);
     PendingRaw &pr = ctx->pending;
     int brc;
-    if (!hf[kHistPackFreq0 + 2]) {
+    // a symbol outside the window: the full arrays replace the window in hf
+    const bool outside = hf[kHistPackFreq0 + 2] != 0;
+    const uint32_t freq0 = hf[kHistPackFreq0];
+    if (!outside) {
         brc = build_huffman_window(hf, hf + kHistWin, hist_window_lo(p->radius), kHistWin, hf[kHistPackFreq0],
                                    hf[kHistPackFreq0 + 1], pr.tab);
     } else {  // a symbol outside the window: the whole arrays
@@ -1038,16 +1055,37 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         brc = build_huffman(hf, hf + 65536, 65536, pr.tab);
     }
     if (brc) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
+    // the stream's bit total from the histogram and the code lengths: the
+    // encoder's own total must agree
+    uint64_t want_bits = 0;
+    {
+        auto add = [&](uint32_t sym, uint64_t cnt) {
+            const int64_t k = (int64_t) sym - pr.tab.offset;
+            if (cnt && k >= 0 && k < pr.tab.n_states) want_bits += cnt * pr.tab.len[(size_t) k];
+        };
+        if (!outside) {
+            const uint32_t lo = (uint32_t) hist_window_lo(p->radius);
+            for (uint32_t i = 0; i < (uint32_t) kHistWin; i++) add(lo + i, hf[i]);
+            add(0, freq0);
+        } else {
+            for (uint32_t i = 0; i < 65536; i++) add(i, hf[i]);
+        }
+    }
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
     rc = encode_enqueue(ctx, io.d_codes, io.n_codes, &ht);
     if (rc) return rc;
     HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(hs + 6, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
     HIPCHECK(spin_wait(st));
     flush_profile(ctx);
     uint64_t bits;
     memcpy(&bits, hs + 4, 8);
+    if (hs[6] & 4u) CTX_FAIL(FAZ_ERR_HIP, "internal: an encode look-back wait ran out");
+    if (bits != want_bits)
+        CTX_FAIL(FAZ_ERR_HIP, "internal: encoded %llu bits, the histogram gives %llu", (unsigned long long) bits,
+                 (unsigned long long) want_bits);
     const size_t nbytes = (size_t) ((bits + 7) / 8);
     io.n_unpred = n_unpred;
     ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);

@@ -799,10 +799,16 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
             int lorenzo;
             double br, ps, cost;
         };
+        // the candidates the exhaustive search can reach: per alpha, the betas
+        // up to the first one its stop rule ends on (larger betas give the
+        // same bounds on every level)
         std::vector<LC> lc;
         for (double a: alphas)
-            for (double b: betas)
-                if (!(a >= 1 && a > b)) lc.push_back(LC{a, b, 0, 0, 0, 0});
+            for (double b: betas) {
+                if (a >= 1 && a > b) continue;
+                lc.push_back(LC{a, b, 0, 0, 0, 0});
+                if (a >= 1 && std::pow(a, max_interp_level - 1) <= b) break;
+            }
         if (in->test_lorenzo) lc.push_back(LC{-1, -1, 1, 0, 0, 0});
         std::vector<float> rows(lc.size() * FAZ_LEARNED_INPUTS);
         for (size_t c = 0; c < lc.size(); c++)
@@ -828,19 +834,30 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
                 lc[c].cost = lc[c].br - lc[c].ps / 6.02;
                 record(lc[c].a, lc[c].b, in->abs_eb, lc[c].br, lc[c].ps, 0);
             }
-            std::vector<size_t> order(lc.size());
-            std::iota(order.begin(), order.end(), 0);
+            // interpolation candidates by predicted cost; the Lorenzo candidate apart
+            std::vector<size_t> order;
+            int64_t lor = -1;
+            for (size_t c = 0; c < lc.size(); c++) {
+                if (lc[c].lorenzo) lor = (int64_t) c;
+                else order.push_back(c);
+            }
             std::stable_sort(order.begin(), order.end(), [&](size_t i, size_t j) { return lc[i].cost < lc[j].cost; });
             const int kmax = in->learned_verify > 0 ? in->learned_verify : 4;
             std::vector<size_t> ver;
             if (in->learned_margin >= 0)
                 for (size_t c: order)
                     if ((int) ver.size() < kmax && lc[c].cost <= lc[order[0]].cost + in->learned_margin) ver.push_back(c);
+            // the predictor choice is verified when the two predicted costs are close
+            const bool lor_close = lor >= 0 && in->learned_margin_lorenzo >= 0 &&
+                                   std::fabs(lc[lor].cost - lc[order[0]].cost) <= in->learned_margin_lorenzo;
             size_t pick = order[0];
-            if (ver.size() >= 2) {
+            if (lor >= 0 && lc[lor].cost < lc[order[0]].cost) pick = (size_t) lor;
+            if (ver.size() >= 2 || lor_close) {
                 // the closest candidates by real CompressTests (interpolation
                 // candidates in one launch) and the lowest real cost among them
                 out->learned_path = 1;
+                if (ver.empty()) ver.push_back(order[0]);
+                if (lor_close) ver.push_back((size_t) lor);
                 std::vector<TuneCand> cands;
                 std::vector<size_t> which;
                 for (size_t c: ver) {
@@ -865,13 +882,12 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
                         if (cost < best_cost) best_cost = cost, pick = which[k];
                     }
                 }
-                for (size_t c: ver) {
-                    if (!lc[c].lorenzo) continue;
+                if (lor_close) {
                     double br, ps;
                     if ((rc = lorenzo_test(in->abs_eb, br, ps))) return rc;
                     out->n_verified++;
                     const double cost = br - ps / 6.02;
-                    if (cost < best_cost) best_cost = cost, pick = c;
+                    if (cost < best_cost) best_cost = cost, pick = (size_t) lor;
                 }
             }
             if (lc[pick].lorenzo) {

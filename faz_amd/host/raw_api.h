@@ -15,10 +15,11 @@ int faz_compress_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, co
 // SZFastFrontend).
 int faz_lorenzo_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_lorenzo_params *p,
                         std::vector<uint8_t> &raw);
-// The same for the general-frontend Lorenzo engine (any N).
+// The same for the general-frontend engine (any N).
 int faz_general_raw_vec(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_general_params *p,
                         std::vector<uint8_t> &raw);
 int faz_general_decompress_raw_vec(faz_gpu_ctx *ctx, const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2,
+                                   int regression,
                                    float *d_out, size_t num);
 // Device scratch of the context, grown on demand (n floats).
 float *faz_ctx_work_in(faz_gpu_ctx *ctx, size_t n);

@@ -217,6 +217,7 @@ faz_general_params general_params_from_config(const QoZ::Config &c) {
     p.block_size = c.blockSize;
     p.lorenzo = c.lorenzo;
     p.lorenzo2 = c.lorenzo2;
+    p.regression = c.regression;
     p.radius = c.quantbinCnt / 2;
     return p;
 }
@@ -225,9 +226,8 @@ int compress_lorenzo_stream(faz_gpu_ctx *ctx, QoZ::Config &conf, const float *d_
     // SZ_compress_LorenzoReg (LR:95-114): the fast frontend for 3-D fields
     // without regression2, else the general frontend
     const bool fast = conf.N == 3 && !conf.regression2;
-    if (!fast && (conf.regression || conf.regression2)) {
-        faz_ctx_set_error(ctx, "the general frontend runs on the GPU with the Lorenzo predictors only "
-                               "(regression / regression2 off)");
+    if (conf.regression2) {
+        faz_ctx_set_error(ctx, "the PolyRegressionPredictor (regression2) is outside the GPU path");
         return FAZ_ERR_UNSUPPORTED;
     }
     if (conf.wavelet) {
@@ -386,10 +386,10 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
     // SZ_decompress_LorenzoReg (LR:117-135): fast frontend for 3-D without
     // regression2, else the general frontend
     const bool lorenzo = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && conf.N == 3 && !conf.regression2;
-    const bool general = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && !lorenzo && !conf.regression && !conf.regression2;
+    const bool general = conf.cmprAlgo == QoZ::ALGO_LORENZO_REG && !lorenzo && !conf.regression2;
     if (!(conf.cmprAlgo == QoZ::ALGO_INTERP || lorenzo || general) || conf.wavelet != 0) {
-        faz_ctx_set_error(ctx, "only ALGO_INTERP and ALGO_LORENZO_REG streams (3-D, or any N with the Lorenzo "
-                               "predictors only) without wavelets decode on the GPU path");
+        faz_ctx_set_error(ctx, "only ALGO_INTERP and ALGO_LORENZO_REG streams (without regression2) without "
+                               "wavelets decode on the GPU path");
         return FAZ_ERR_UNSUPPORTED;
     }
     std::vector<uint8_t> raw;
@@ -402,7 +402,8 @@ int decompress_stream(faz_gpu_ctx *ctx, const uint8_t *stream, size_t n, QoZ::Co
         return faz_gpu_lorenzo_decompress_raw(ctx, raw.data(), raw.size(), dims, d_out);
     }
     if (general)
-        return faz_general_decompress_raw_vec(ctx, raw.data(), raw.size(), conf.N, conf.lorenzo, conf.lorenzo2, d_out,
+        return faz_general_decompress_raw_vec(ctx, raw.data(), raw.size(), conf.N, conf.lorenzo, conf.lorenzo2,
+                                              conf.regression, d_out,
                                               num);
     return faz_gpu_interp_decompress_raw(ctx, raw.data(), raw.size(), conf.N, d_out, num);
 }
@@ -786,7 +787,7 @@ faz_status faz_sz_compress_general_f32(faz_gpu_ctx *ctx, const faz_general_param
     conf.blockSize = p->block_size;
     conf.lorenzo = p->lorenzo != 0;
     conf.lorenzo2 = p->lorenzo2 != 0;
-    conf.regression = false;
+    conf.regression = p->regression != 0;
     conf.regression2 = false;
     conf.quantbinCnt = p->radius * 2;
     if (rel_mode) {

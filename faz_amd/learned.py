@@ -67,7 +67,7 @@ def grf_field(shape, seed, slope=11.0 / 3.0, fronts=0, noise=1e-4, width=4.0, am
     return out
 
 
-LORENZO_SCALES = (0.8, 1.0, 1.25)
+LORENZO_SCALES = (0.6, 0.8, 0.9, 1.0, 1.12, 1.25, 1.6)
 
 
 def field_tuples(engine, x, shape, rel):
@@ -113,11 +113,30 @@ def field_tuples(engine, x, shape, rel):
 TRAIN_GRIDS = (256, 320, 384, 512)  # cubic FFT grids the training fields are synthesised on
 
 
+def walk_field(shape, seed, noise=0.0):
+    """Random walk along every axis (cumulative sums of white noise), min-max
+    normalised to [0, 1]: rough at the sample-block scale, where the Lorenzo
+    predictor can beat interpolation (synthetic input data)."""
+    import torch
+    g = torch.Generator(device="cuda")
+    g.manual_seed(int(seed))
+    f = torch.randn(shape, generator=g, device="cuda")
+    for a in range(len(shape)):
+        f = torch.cumsum(f, dim=a)
+    f = (f - f.min()) / (f.max() - f.min())
+    if noise:
+        f += noise * torch.randn(shape, generator=g, device="cuda")
+    out = f.to(torch.float32).contiguous().view(-1)
+    torch.cuda.synchronize()
+    return out
+
+
 def training_fields(n, seed=0):
     """Seeded draws for the training set: a field synthesised on one of
     TRAIN_GRIDS (so FFT plans are reused) and, for half of the draws, a
-    sub-box of edges 0.5..1 of the grid cut out of it; slope k^-1.5..k^-5.5,
-    0..16 fronts of width 2..8, noise 0..1e-3, REL 1e-6..1e-2 (log-uniform)."""
+    sub-box of edges 0.5..1 of the grid cut out of it; slope k^-1..k^-5.5,
+    0..16 fronts of width 2..8, noise 0..1e-2, REL 1e-6..1e-2 (log-uniform);
+    15% are random walks instead (walk_field)."""
     r = np.random.default_rng(seed)
     out = []
     for i in range(n):
@@ -126,9 +145,10 @@ def training_fields(n, seed=0):
             shape = (g, g, g)
         else:
             shape = tuple(int(v) for v in r.integers(max(96, g // 2), g + 1, 3))
-        out.append(dict(shape=shape, grid=(g, g, g), seed=int(r.integers(1 << 40)), slope=float(r.uniform(1.5, 5.5)),
+        out.append(dict(shape=shape, grid=(g, g, g), seed=int(r.integers(1 << 40)), slope=float(r.uniform(1.0, 5.5)),
                         fronts=int(r.integers(0, 17)), width=float(r.choice([2.0, 4.0, 8.0])),
-                        noise=float(r.choice([0.0, 1e-5, 1e-4, 1e-4, 1e-3])), rel=float(10 ** r.uniform(-6, -2))))
+                        noise=float(r.choice([0.0, 1e-5, 1e-4, 1e-4, 1e-3, 1e-2])), rel=float(10 ** r.uniform(-6, -2)),
+                        kind="walk" if r.random() < 0.15 else "grf"))
     return out
 
 
@@ -144,8 +164,11 @@ def build_dataset(engine, specs, log=None, deadline=None):
                 log(f"deadline: stopping after {i} fields")
             break
         grid = sp.get("grid")
-        x = grf_field(sp["shape"], sp["seed"], sp["slope"], sp["fronts"], noise=sp.get("noise", 1e-4),
-                      width=sp.get("width", 4.0), grid=None if grid == sp["shape"] else grid)
+        if sp.get("kind") == "walk":
+            x = walk_field(sp["shape"], sp["seed"], noise=sp.get("noise", 0.0))
+        else:
+            x = grf_field(sp["shape"], sp["seed"], sp["slope"], sp["fronts"], noise=sp.get("noise", 1e-4),
+                          width=sp.get("width", 4.0), grid=None if grid == sp["shape"] else grid)
         rows, targets, _ = field_tuples(engine, x, sp["shape"], sp["rel"])
         X.append(rows)
         T.append(targets)
@@ -158,8 +181,13 @@ def build_dataset(engine, specs, log=None, deadline=None):
 
 
 def train(engine, X, T, epochs=40, batch=1024, lr=1e-3, seed=0, hidden=256, layers=3, log=None):
-    """Trains a fresh MLP on standardised inputs; returns (mlp, mu, sigma, history)."""
+    """Trains a fresh MLP on standardised inputs; returns (mlp, mu, sigma,
+    history).  The Lorenzo candidate's rows (a few per field) are repeated
+    four times so the predictor choice weighs as much as the alpha/beta grid."""
     import torch
+    lor = X[:, 53] > 0.5
+    X = np.concatenate([X] + [X[lor]] * 3)
+    T = np.concatenate([T] + [T[lor]] * 3)
     mu = X.mean(0)
     sigma = X.std(0)
     sigma[sigma < 1e-6] = 1.0
@@ -195,6 +223,7 @@ def save_model(path, net, mu, sigma, meta=None, gates=None):
                         lo=np.asarray(g.get("lo", np.full(net.in_dim, -np.inf)), np.float32),
                         hi=np.asarray(g.get("hi", np.full(net.in_dim, np.inf)), np.float32),
                         margin=np.array([g.get("margin", -1.0)]), verify=np.array([g.get("verify", 4)]),
+                        margin_lorenzo=np.array([g.get("margin_lorenzo", -1.0)]),
                         meta=np.array(repr(meta or {})))
 
 
@@ -210,7 +239,8 @@ def load_model(engine, path=MODEL_PATH, hidden=256, layers=3):
 def load_gates(path=MODEL_PATH):
     """Confidence gates stored with the model (see save_model)."""
     z = np.load(path, allow_pickle=False)
-    return dict(lo=z["lo"], hi=z["hi"], margin=float(z["margin"][0]), verify=int(z["verify"][0]))
+    ml = float(z["margin_lorenzo"][0]) if "margin_lorenzo" in z.files else -1.0
+    return dict(lo=z["lo"], hi=z["hi"], margin=float(z["margin"][0]), verify=int(z["verify"][0]), margin_lorenzo=ml)
 
 
 def tune_learned(engine, x, shape, eb, rel, rng, net, mu, sigma, gates=None, test_lorenzo=1):
@@ -218,7 +248,7 @@ def tune_learned(engine, x, shape, eb, rel, rng, net, mu, sigma, gates=None, tes
     g = gates or {}
     return engine.tune_learned(x, shape, eb, rel, rng, net, mu, sigma, test_lorenzo=test_lorenzo,
                                margin=g.get("margin", -1.0), verify=g.get("verify", 0), lo=g.get("lo"),
-                               hi=g.get("hi"))
+                               hi=g.get("hi"), margin_lorenzo=g.get("margin_lorenzo", -1.0))
 
 
 def predict(net, mu, sigma, rows):
@@ -228,18 +258,42 @@ def predict(net, mu, sigma, rows):
     return y
 
 
+def reachable(X, max_interp_level=5):
+    """Rows whose candidate the exhaustive search can reach (the Lorenzo
+    test, and per alpha the betas up to the first one its stop rule ends on;
+    3-D fields with maxStep 32 have max_interp_level 5)."""
+    lor = X[:, 53] > 0.5
+    a = X[:, 40].astype(np.float64) + 1.0
+    b = X[:, 41].astype(np.float64) * 4.0
+    ok = np.zeros(len(X), bool)
+    for i in range(len(X)):
+        if lor[i]:
+            ok[i] = True
+            continue
+        first = None
+        for bb in BETAS:
+            if a[i] > bb:
+                continue
+            if a[i] ** (max_interp_level - 1) <= bb:
+                first = bb
+                break
+        ok[i] = first is None or b[i] <= first + 1e-9
+    return ok
+
+
 def calibrate(net, mu, sigma, X, T, G, kinds_scale, quantile=0.9):
     """Gates from held-out tuples: lo/hi = the standardised range of the
-    training rows widened by one unit, margin = the `quantile` of the
-    predicted cost gap between the predicted best and the truly best
-    base-bound candidate of each held-out field."""
+    rows widened by one unit, margin = the `quantile` of the predicted cost
+    gap between the predicted best and the truly best reachable base-bound
+    candidate of each held-out field."""
     Z = (X - mu) / sigma
     lo = Z.min(0) - 1.0
     hi = Z.max(0) + 1.0
     y = predict(net, mu, sigma, X)
+    reach = reachable(X)
     gaps = []
     for g in np.unique(G):
-        m = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9)
+        m = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9) & reach
         if m.sum() < 2:
             continue
         # cost = bitrate - PSNR / 6.02 (the constant PSNR offset cancels)
@@ -247,7 +301,27 @@ def calibrate(net, mu, sigma, X, T, G, kinds_scale, quantile=0.9):
         pred = np.exp2(y[m, 0]) - 5.0 * y[m, 1] / 6.02
         gaps.append(pred[np.argmin(true)] - pred.min())
     margin = float(np.quantile(np.array(gaps), quantile)) if gaps else -1.0
-    return dict(lo=lo.astype(np.float32), hi=hi.astype(np.float32), margin=margin, verify=4), np.array(gaps)
+    # the predictor choice: the predicted cost difference between the Lorenzo
+    # candidate and the best interpolation candidate of every held-out field;
+    # the margin covers every field where its sign was wrong
+    wrong, diffs = [], []
+    lorm = X[:, 53] > 0.5
+    for g in np.unique(G):
+        base = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9)
+        mi, ml = base & reach & ~lorm, base & lorm
+        if not mi.any() or not ml.any():
+            continue
+        t_i = np.exp2(T[mi, 0]) - 5.0 * T[mi, 1] / 6.02
+        p_i = np.exp2(y[mi, 0]) - 5.0 * y[mi, 1] / 6.02
+        t_l = float((np.exp2(T[ml, 0]) - 5.0 * T[ml, 1] / 6.02).min())
+        p_l = float((np.exp2(y[ml, 0]) - 5.0 * y[ml, 1] / 6.02).min())
+        d_true, d_pred = t_l - t_i.min(), p_l - p_i.min()
+        diffs.append(abs(d_pred - d_true))
+        if np.sign(d_true) != np.sign(d_pred):
+            wrong.append(abs(d_pred))
+    margin_lor = 1.1 * max(wrong) if wrong else (float(np.quantile(diffs, quantile)) if diffs else -1.0)
+    return dict(lo=lo.astype(np.float32), hi=hi.astype(np.float32), margin=margin, verify=4,
+                margin_lorenzo=float(margin_lor)), np.array(gaps)
 
 
 def final_stats(engine, x, shape, eb, params):

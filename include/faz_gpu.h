@@ -326,14 +326,16 @@ faz_status faz_sz_compress_batch_f32(faz_batch *b, const int *rel_mode, const do
 /* Parameters field i was bound with. */
 faz_status faz_gpu_batch_field(const faz_batch *b, int i, faz_interp_params *p);
 
-/* --- A20: the general frontend with the Lorenzo predictors, N = 1..4:
- * SZGeneralFrontend (include/QoZ/frontend/SZGeneralFrontend.hpp:36-116)
- * over ComposedPredictor{LorenzoPredictor<1>, LorenzoPredictor<2>}
- * (ComposedPredictor.hpp:45-93, 203-249; LorenzoPredictor.hpp:25-247) or one
- * of them, as make_lorenzo_regression_compressor composes them with
- * regression and regression2 off (SZLorenzoReg.hpp:27-91).  This is what
- * SZ_compress_LorenzoReg runs for N != 3 (LR:103-110) and what the tuner's
- * Lorenzo CompressTest runs for every N (SZI:1136-1170). */
+/* --- A20: the general frontend, N = 1..4: SZGeneralFrontend
+ * (include/QoZ/frontend/SZGeneralFrontend.hpp:36-116) over
+ * ComposedPredictor (ComposedPredictor.hpp:45-93, 203-249) of the enabled
+ * predictors in the order LorenzoPredictor<1>, LorenzoPredictor<2>
+ * (LorenzoPredictor.hpp:25-247), RegressionPredictor
+ * (RegressionPredictor.hpp:17-246), or the one enabled alone, as
+ * make_lorenzo_regression_compressor composes them with regression2 off
+ * (SZLorenzoReg.hpp:27-91).  This is what SZ_compress_LorenzoReg runs for
+ * N != 3 (LR:103-110) and what the tuner's Lorenzo CompressTest runs for
+ * every N (SZI:1136-1170). */
 typedef struct {
     int N;                  /* 1..4 */
     size_t dims[4];
@@ -342,6 +344,7 @@ typedef struct {
     int lorenzo;            /* Config::lorenzo */
     int lorenzo2;           /* Config::lorenzo2 (needs block_size >= 2) */
     int radius;             /* Config::quantbinCnt / 2, <= 65536 */
+    int regression;         /* Config::regression (one field per call) */
 } faz_general_params;
 
 /* Results of faz_gpu_general_compress_f32 (device pointers owned by ctx). */
@@ -349,8 +352,12 @@ typedef struct {
     uint32_t *d_codes;          /* quantization codes, block-major (GF:47-55) */
     size_t n_codes;
     float *d_outval;            /* at every zero code, the unpredictable value */
-    const uint8_t *d_selection; /* per block, index into the predictor list */
+    const uint8_t *d_selection; /* per block, index into the predictor list; 255 where the
+                                   chosen predictor declined the block and none is recorded */
     size_t n_selection;         /* 0 when a single predictor is on */
+    size_t n_regression;        /* blocks predicted by the RegressionPredictor */
+    int passes;                 /* wavefront passes until the predictor choices settled (1
+                                   without the RegressionPredictor) */
 } faz_general_out;
 
 /* SZGeneralFrontend::compress (GF:36-71); d_recon (n floats) receives the
@@ -362,10 +369,11 @@ faz_status faz_gpu_general_compress_f32(faz_gpu_ctx *ctx, const float *d_in, flo
  * tree and bits.  Needs radius <= 32768.  d_recon may be NULL. */
 faz_status faz_gpu_general_raw_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_general_params *p,
                                    uint8_t *h_raw, size_t cap, size_t *raw_size);
-/* SZGeneralCompressor::decompress after zstd (SGC:115-140, GF:72-96, 105-116)
- * into d_out (num floats); lorenzo / lorenzo2 come from the stream's Config. */
+/* SZGeneralCompressor::decompress after zstd (SGC:115-140, GF:72-96, 105-116,
+ * RP:134-156) into d_out (num floats); lorenzo / lorenzo2 / regression come
+ * from the stream's Config. */
 faz_status faz_gpu_general_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, int lorenzo,
-                                          int lorenzo2, float *d_out, size_t num);
+                                          int lorenzo2, int regression, float *d_out, size_t num);
 /* CompressTest with algo = ALGO_LORENZO_REG, useFast = false (SZI:1136-1170,
  * 1254-1256, 1283-1294, 1353-1392): one general compressor (lorenzo +
  * lorenzo2, blockSize 5, radius 65536) over the (sbs+1)^N sample blocks at
@@ -412,6 +420,10 @@ typedef struct {
     double learned_margin;
     int learned_verify;
     const float *learned_lo, *learned_hi;
+    /* with test_lorenzo: the Lorenzo candidate and the best interpolation
+     * candidate are both verified when their predicted costs lie within this
+     * margin of each other (< 0: never) */
+    double learned_margin_lorenzo;
 } faz_tune_in;
 
 typedef struct {

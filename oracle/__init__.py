@@ -392,10 +392,11 @@ class GFParams(ctypes.Structure):
         ("lorenzo", ctypes.c_int),
         ("lorenzo2", ctypes.c_int),
         ("radius", ctypes.c_int),
+        ("regression", ctypes.c_int),
     ]
 
 
-def make_gf_params(dims, eb, block_size=5, lorenzo=True, lorenzo2=True, radius=32768):
+def make_gf_params(dims, eb, block_size=5, lorenzo=True, lorenzo2=True, radius=32768, regression=False):
     p = GFParams()
     p.N = len(dims)
     for i, d in enumerate(dims):
@@ -405,6 +406,7 @@ def make_gf_params(dims, eb, block_size=5, lorenzo=True, lorenzo2=True, radius=3
     p.lorenzo = int(bool(lorenzo))
     p.lorenzo2 = int(bool(lorenzo2))
     p.radius = int(radius)
+    p.regression = int(bool(regression))
     return p
 
 
@@ -414,8 +416,8 @@ def _gf_lib():
     P = ctypes.POINTER(GFParams)
     L.oracle_gf_compress.argtypes = [P, vp, vp, vp, vp, sz, vp, sz]
     L.oracle_gf_raw.argtypes = [P, vp, vp, ctypes.c_size_t, sz]
-    L.oracle_gf_decompress_raw.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
-                                           ctypes.c_size_t]
+    L.oracle_gf_decompress_raw.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, vp, ctypes.c_size_t]
     L.oracle_gf_compress_stream.argtypes = [P, ctypes.c_int, ctypes.c_double, vp, vp, ctypes.c_size_t, sz]
     L.oracle_compress_test_lorenzo.argtypes = [ctypes.c_int, ctypes.c_double, vp, vp, vp, ctypes.c_size_t,
                                                ctypes.c_size_t, vp, ctypes.c_size_t, sz, sz,
@@ -452,12 +454,12 @@ def gf_raw(data, params):
     return out[:n.value].tobytes()
 
 
-def gf_decompress_raw(raw, dims, lorenzo=True, lorenzo2=True):
+def gf_decompress_raw(raw, dims, lorenzo=True, lorenzo2=True, regression=False):
     buf = np.frombuffer(raw, np.uint8).copy()
     num = int(np.prod(dims))
     out = np.zeros(num, np.float32)
     rc = _gf_lib().oracle_gf_decompress_raw(_ptr(buf), buf.size, len(dims), int(bool(lorenzo)),
-                                            int(bool(lorenzo2)), _ptr(out), num)
+                                            int(bool(lorenzo2)), int(bool(regression)), _ptr(out), num)
     assert rc == 0, rc
     return out
 

@@ -1291,20 +1291,32 @@ struct GFEngine {
     size_t dims[4] = {1, 1, 1, 1}, st[4] = {0, 0, 0, 0};
     int bs = 5;
     int npred = 2;
-    int layer[2] = {1, 2};   // predictor list order: 1-layer first (LR:39-67)
+    // predictor list in the order of LR:36-91: 1 = 1-layer Lorenzo, 2 =
+    // 2-layer Lorenzo, 3 = RegressionPredictor
+    int layer[3] = {1, 2, 3};
     float noise[3] = {0, 0, 0};
     double eb = 0;
     Quantizer quantizer;
     std::vector<int32_t> codes, selection;
+    // RegressionPredictor (RP:17-246): coefficient quantizers (independent
+    // term eb / (N+1), linear terms eb / (N+1) / blockSize), the block's
+    // coefficients and the last committed ones, the coefficient codes
+    bool reg = false;
+    Quantizer q_ind, q_lin;
+    float cur[5] = {0, 0, 0, 0, 0}, prev_c[5] = {0, 0, 0, 0, 0};
+    std::vector<int32_t> reg_codes;
+    size_t reg_index = 0;
 
     bool set(const oracle_gf_params *p) {
         N = p->N;
         if (N < 1 || N > 4 || p->block_size < 1) return false;
-        if (!p->lorenzo && !p->lorenzo2) return false;
+        if (!p->lorenzo && !p->lorenzo2 && !p->regression) return false;
         if (p->lorenzo2 && p->block_size < 2) return false;  // a 2-layer read would leave the array
         npred = 0;
         if (p->lorenzo) layer[npred++] = 1;
         if (p->lorenzo2) layer[npred++] = 2;
+        if (p->regression) layer[npred++] = 3;
+        reg = p->regression != 0;
         bs = p->block_size;
         size_t s = 1;
         for (int d = N - 1; d >= 0; d--) {
@@ -1321,6 +1333,70 @@ struct GFEngine {
         noise[2] = (float) (n2[N - 1] * eb);
         quantizer.set_eb(eb);
         quantizer.radius = p->radius;
+        q_ind.set_eb(eb / (N + 1));        // RP:22-25
+        q_lin.set_eb(eb / (N + 1) / bs);
+        return true;
+    }
+
+    // RegressionPredictor::precompress_block (RP:48-85): least-squares plane
+    // of the block's values, row sums in double along the last axis; false
+    // (coefficients left as they were) for a block of extent 1 on any axis
+    bool reg_fit(const float *data, const size_t *o, const size_t *sz) {
+        size_t n = 1;
+        for (int d = 0; d < N; d++) {
+            if (sz[d] <= 1) return false;
+            n *= sz[d];
+        }
+        const float recip = (float) (1.0 / (double) n);
+        double sum[5] = {0, 0, 0, 0, 0};
+        const size_t rows = n / sz[N - 1];
+        size_t l[4] = {0, 0, 0, 0};
+        for (size_t r = 0; r < rows; r++) {
+            size_t idx = 0;
+            for (int d = 0; d < N - 1; d++) idx += (o[d] + l[d]) * st[d];
+            double cum = 0;
+            for (size_t t = 0; t < sz[N - 1]; t++) {
+                const float v = data[idx + (o[N - 1] + t) * st[N - 1]];
+                cum += v;
+                sum[N - 1] += (double) t * v;
+            }
+            for (int i = 0; i < N - 1; i++) sum[i] += cum * (double) l[i];
+            sum[N] += cum;
+            for (int d = N - 2; d >= 0; d--) {
+                if (++l[d] < sz[d]) break;
+                l[d] = 0;
+            }
+        }
+        for (int i = 0; i <= N; i++) cur[i] = 0;
+        cur[N] = (float) (sum[N] * recip);
+        for (int i = 0; i < N; i++) {
+            cur[i] = (float) ((2 * sum[i] / (double) (sz[i] - 1) - sum[N]) * 6 * recip / (double) (sz[i] + 1));
+            cur[N] -= (float) (sz[i] - 1) * cur[i] / 2;
+        }
+        return true;
+    }
+
+    // RegressionPredictor::predict (RP:91-98) at block-local index l, in T
+    float reg_predict(const size_t *l) const {
+        float pred = 0;
+        for (int i = 0; i < N; i++) pred += (float) l[i] * cur[i];
+        pred += cur[N];
+        return pred;
+    }
+
+    // precompress_block_commit: coefficients predicted from the last
+    // committed ones and overwritten with their reconstruction (RP:87-90, 224-230)
+    void reg_commit() {
+        for (int i = 0; i < N; i++) reg_codes.push_back(q_lin.quantize_and_overwrite(cur[i], prev_c[i]));
+        reg_codes.push_back(q_ind.quantize_and_overwrite(cur[N], prev_c[N]));
+        for (int i = 0; i <= N; i++) prev_c[i] = cur[i];
+    }
+
+    // pred_and_recover_coefficients (RP:233-240)
+    bool reg_recover() {
+        if (reg_index + N + 1 > reg_codes.size()) return false;
+        for (int i = 0; i < N; i++) cur[i] = q_lin.recover(cur[i], reg_codes[reg_index++]);
+        cur[N] = q_ind.recover(cur[N], reg_codes[reg_index++]);
         return true;
     }
 
@@ -1373,27 +1449,34 @@ struct GFEngine {
         }
     }
 
-    // LorenzoPredictor::estimate_error (LP:102-104): |x - pred| + noise in T
-    float estimate(const float *data, const size_t *c, int L) const {
+    // LorenzoPredictor::estimate_error (LP:102-104): |x - pred| + noise in T;
+    // RegressionPredictor::estimate_error (RP:44-46): |x - pred| with the
+    // block's unquantized coefficients
+    float estimate(const float *data, const size_t *c, int L, const size_t *o) const {
         size_t idx = 0;
         for (int d = 0; d < N; d++) idx += c[d] * st[d];
         float x = data[idx];
+        if (L == 3) {
+            size_t l[4];
+            for (int d = 0; d < N; d++) l[d] = c[d] - o[d];
+            return std::fabs(x - reg_predict(l));
+        }
         return std::fabs(x - predict(data + idx, c, L)) + noise[L];
     }
 
     // ComposedPredictor::do_estimate_error (CP:203-249): predictor with the
     // smallest summed estimate over the block diagonals, first on ties.
     int select(const float *data, const size_t *o, const size_t *sz) const {
-        double pe[2] = {0, 0};
+        double pe[3] = {0, 0, 0};
         size_t m = sz[0];
         for (int d = 1; d < N; d++) m = std::min(m, sz[d]);
         size_t c[4];
         if (N == 1) {
             for (int p = 0; p < npred; p++) {
                 c[0] = o[0];
-                pe[p] += estimate(data, c, layer[p]);
+                pe[p] += estimate(data, c, layer[p], o);
                 c[0] = o[0] + m - 1;
-                pe[p] += estimate(data, c, layer[p]);
+                pe[p] += estimate(data, c, layer[p], o);
             }
         } else {
             const int ndiag = 1 << (N - 1);
@@ -1403,7 +1486,7 @@ struct GFEngine {
                         c[0] = o[0] + k;
                         for (int d = 1; d < N; d++)
                             c[d] = o[d] + (((t >> (N - 1 - d)) & 1) ? m - 1 - k : k);
-                        pe[p] += estimate(data, c, layer[p]);
+                        pe[p] += estimate(data, c, layer[p], o);
                     }
         }
         int sid = 0;
@@ -1428,11 +1511,20 @@ struct GFEngine {
                 o[d] = b[d] * bs;
                 sz[d] = b[d] == nb[d] - 1 ? dims[d] - o[d] : (size_t) bs;
             }
+            // every predictor's precompress_block, then the choice (CP:45-63);
+            // a block the chosen predictor declines runs on the 1-layer
+            // Lorenzo fallback with no selection recorded (GF:47-52)
+            const bool reg_ok = reg ? reg_fit(data, o, sz) : true;
             int L = layer[0];
-            if (npred > 1) {
-                int sid = select(data, o, sz);
-                selection.push_back(sid);
-                L = layer[sid];
+            int sid = 0;
+            if (npred > 1) sid = select(data, o, sz);
+            L = layer[sid];
+            const bool ok = L != 3 || reg_ok;
+            if (ok) {
+                if (npred > 1) selection.push_back(sid);
+                if (L == 3) reg_commit();
+            } else {
+                L = 1;
             }
             size_t bn = 1;
             for (int d = 0; d < N; d++) bn *= sz[d];
@@ -1443,7 +1535,7 @@ struct GFEngine {
                     c[d] = o[d] + l[d];
                     idx += c[d] * st[d];
                 }
-                float pred = predict(data + idx, c, L);
+                float pred = L == 3 ? reg_predict(l) : predict(data + idx, c, L);
                 codes.push_back(quantizer.quantize_and_overwrite(data[idx], pred));
                 for (int d = N - 1; d >= 0; d--) {
                     if (++l[d] < sz[d]) break;
@@ -1458,7 +1550,9 @@ struct GFEngine {
     }
 
     // SZGeneralFrontend::decompress (GF:72-96) given the codes and selections
-    void decompress(const int32_t *cds, float *out) {
+    // (one selection per block, as ComposedPredictor::predecompress_block
+    // reads them, CP:64-68); false when the lists run out
+    bool decompress(const int32_t *cds, float *out) {
         size_t nb[4], b[4] = {0, 0, 0, 0};
         size_t total = 1;
         for (int d = 0; d < N; d++) {
@@ -1473,7 +1567,14 @@ struct GFEngine {
                 o[d] = b[d] * bs;
                 sz[d] = b[d] == nb[d] - 1 ? dims[d] - o[d] : (size_t) bs;
             }
+            if (npred > 1 && t >= selection.size()) return false;
             int L = npred > 1 ? layer[selection[t]] : layer[0];
+            if (L == 3) {
+                bool ok = true;
+                for (int d = 0; d < N; d++) ok = ok && sz[d] > 1;
+                if (!ok) L = 1;
+                else if (!reg_recover()) return false;
+            }
             size_t bn = 1;
             for (int d = 0; d < N; d++) bn *= sz[d];
             size_t l[4] = {0, 0, 0, 0};
@@ -1483,7 +1584,8 @@ struct GFEngine {
                     c[d] = o[d] + l[d];
                     idx += c[d] * st[d];
                 }
-                out[idx] = quantizer.recover(predict(out + idx, c, L), cds[ci++]);
+                const float pred = L == 3 ? reg_predict(l) : predict(out + idx, c, L);
+                out[idx] = quantizer.recover(pred, cds[ci++]);
                 for (int d = N - 1; d >= 0; d--) {
                     if (++l[d] < sz[d]) break;
                     l[d] = 0;
@@ -1494,6 +1596,17 @@ struct GFEngine {
                 b[d] = 0;
             }
         }
+        return true;
+    }
+
+    // LinearQuantizer::save (Q:185-201)
+    static void save_quantizer(const Quantizer &q, std::vector<uint8_t> &out) {
+        out.push_back(2);
+        put(out, q.eb);
+        put(out, (int) q.radius);
+        put(out, (size_t) q.unpred.size());
+        const uint8_t *u = (const uint8_t *) q.unpred.data();
+        out.insert(out.end(), u, u + q.unpred.size() * sizeof(float));
     }
 
     // SZGeneralFrontend::save (GF:98-103) + quantizer, then the codes'
@@ -1501,7 +1614,23 @@ struct GFEngine {
     bool save(std::vector<uint8_t> &out) const {
         for (int d = 0; d < N; d++) put(out, (uint64_t) dims[d]);
         put(out, (uint32_t) bs);
-        for (int p = 0; p < npred; p++) out.push_back(1);  // LorenzoPredictor::save (LP:57-59)
+        for (int p = 0; p < npred; p++) {
+            if (layer[p] != 3) {
+                out.push_back(1);  // LorenzoPredictor::save (LP:57-59)
+                continue;
+            }
+            // RegressionPredictor::save (RP:100-117)
+            out.push_back(2);
+            put(out, (size_t) reg_codes.size());
+            if (!reg_codes.empty()) {
+                save_quantizer(q_ind, out);
+                save_quantizer(q_lin, out);
+                Huffman hr;
+                if (!hr.init(reg_codes.data(), reg_codes.size())) return false;
+                hr.save(out);
+                hr.encode(reg_codes.data(), reg_codes.size(), out);
+            }
+        }
         if (npred > 1) {                                    // ComposedPredictor::save (CP:70-81)
             put(out, (uint64_t) selection.size());
             if (!selection.empty()) {
@@ -1511,12 +1640,7 @@ struct GFEngine {
                 hs.encode(selection.data(), selection.size(), out);
             }
         }
-        out.push_back(2);  // LinearQuantizer::save (Q:185-201)
-        put(out, eb);
-        put(out, (int) quantizer.radius);
-        put(out, (size_t) quantizer.unpred.size());
-        const uint8_t *u = (const uint8_t *) quantizer.unpred.data();
-        out.insert(out.end(), u, u + quantizer.unpred.size() * sizeof(float));
+        save_quantizer(quantizer, out);
         Huffman h;
         if (!h.init(codes.data(), codes.size())) return false;
         h.save(out);
@@ -1525,14 +1649,29 @@ struct GFEngine {
     }
 };
 
+// LinearQuantizer::load (Q:203-226)
+static bool load_quantizer(const uint8_t *&p, const uint8_t *end, Quantizer &q) {
+    if (end - p < 1 + 8 + 4 + 8) return false;
+    p += 1;  // uid
+    q.set_eb(take<double>(p));
+    q.radius = take<int>(p);
+    const size_t U = take<size_t>(p);
+    if (U > (size_t) (end - p) / sizeof(float)) return false;
+    q.unpred.assign((const float *) p, (const float *) p + U);
+    q.index = 0;
+    p += U * sizeof(float);
+    return true;
+}
+
 // SGC:115-140 + GF:105-116 + CP:82-93 on a pre-zstd buffer; N and the
-// predictor list come from the Config (lorenzo, lorenzo2).
-static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int lo2, float *out, size_t num) {
+// predictor list come from the Config (lorenzo, lorenzo2, regression).
+static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int lo2, int regr, float *out,
+                              size_t num) {
     const uint8_t *p = raw, *end = raw + n;
     oracle_gf_params prm;
     memset(&prm, 0, sizeof(prm));
     prm.N = N;
-    prm.lorenzo = lo, prm.lorenzo2 = lo2;
+    prm.lorenzo = lo, prm.lorenzo2 = lo2, prm.regression = regr;
     if (n < (size_t) N * 8 + 4) return false;
     for (int d = 0; d < N; d++) prm.dims[d] = (size_t) take<uint64_t>(p);
     prm.block_size = (int) take<uint32_t>(p);
@@ -1540,7 +1679,18 @@ static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int l
     prm.radius = 32768;
     GFEngine e;
     if (!e.set(&prm) || e.num() != num) return false;
-    p += e.npred;  // predictor ids
+    for (int k = 0; k < e.npred; k++) {  // predictor ids; the regression predictor's coefficients (RP:134-156)
+        p += 1;
+        if (e.layer[k] != 3) continue;
+        const size_t nc = take<size_t>(p);
+        if (!nc) continue;
+        if (!load_quantizer(p, end, e.q_ind) || !load_quantizer(p, end, e.q_lin)) return false;
+        e.reg_codes.resize(nc);
+        HuffDecoder hr;
+        const size_t used = hr.decode(p, end - p, e.reg_codes.data(), nc);
+        if (!used) return false;
+        p += used;
+    }
     size_t nblk = 1;
     for (int d = 0; d < N; d++) nblk *= (e.dims[d] - 1) / e.bs + 1;
     if (e.npred > 1) {
@@ -1561,8 +1711,7 @@ static bool gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lo, int l
     std::vector<int32_t> codes(num);
     HuffDecoder hd;
     if (!hd.decode(p, end - p, codes.data(), num)) return false;
-    e.decompress(codes.data(), out);
-    return true;
+    return e.decompress(codes.data(), out);
 }
 
 // ---------------------------------------------------------------------------
@@ -2363,8 +2512,8 @@ int oracle_decompress_stream(const uint8_t *stream, size_t n, float *out, size_t
             if (dims[0] * dims[1] * dims[2] != num) return -1;
             return lr_decompress_raw(raw.data(), raw.size(), dims, out) ? 0 : -2;
         }
-        if (f[2] || f[3]) return -1;  // regression predictors: not restated
-        return gf_decompress_raw(raw.data(), raw.size(), N, f[0], f[1], out, num) ? 0 : -2;
+        if (f[3]) return -1;  // the second-order regression predictor: not restated
+        return gf_decompress_raw(raw.data(), raw.size(), N, f[0], f[1], f[2], out, num) ? 0 : -2;
     }
     return interp_decompress_raw(raw.data(), raw.size(), N, out, num) ? 0 : -2;
 }
@@ -2512,9 +2661,9 @@ int oracle_gf_raw(const oracle_gf_params *p, const float *data, uint8_t *out, si
     return 0;
 }
 
-int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, float *out,
-                             size_t num) {
-    return gf_decompress_raw(raw, n, N, lorenzo, lorenzo2, out, num) ? 0 : -2;
+int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, int regression,
+                             float *out, size_t num) {
+    return gf_decompress_raw(raw, n, N, lorenzo, lorenzo2, regression, out, num) ? 0 : -2;
 }
 
 int oracle_gf_compress_stream(const oracle_gf_params *p, int rel_mode, double eb_in, const float *data,
@@ -2550,7 +2699,7 @@ int oracle_gf_compress_stream(const oracle_gf_params *p, int rel_mode, double eb
     ip.radius = q.radius;
     TrailerFields tf;
     tf.cmprAlgo = 0;  // ALGO_LORENZO_REG
-    tf.lorenzo = q.lorenzo != 0, tf.lorenzo2 = q.lorenzo2 != 0, tf.regression = false;
+    tf.lorenzo = q.lorenzo != 0, tf.lorenzo2 = q.lorenzo2 != 0, tf.regression = q.regression != 0;
     tf.blockSize = q.block_size;
     size_t before = s.size();
     config_save(s, &ip, q.eb, rel, tf);

@@ -108,6 +108,7 @@ typedef struct {
     int block_size;       /* conf.blockSize */
     int lorenzo, lorenzo2;
     int radius;           /* quantbinCnt / 2 */
+    int regression;       /* conf.regression: RegressionPredictor in the list */
 } oracle_gf_params;
 
 /* SZGeneralFrontend::compress (GF:36-71). Outputs optional: codes (n,
@@ -117,9 +118,9 @@ int oracle_gf_compress(const oracle_gf_params *p, const float *data, int32_t *co
                        int32_t *selection, size_t *n_sel, float *unpred, size_t *n_unpred);
 /* Pre-zstd buffer of SZGeneralCompressor<SZGeneralFrontend> (SGC:30-71, GF:98-103). */
 int oracle_gf_raw(const oracle_gf_params *p, const float *data, uint8_t *out, size_t cap, size_t *n_out);
-/* SGC:115-140 on a pre-zstd buffer; lorenzo/lorenzo2 as in the Config. */
-int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, float *out,
-                             size_t num);
+/* SGC:115-140 on a pre-zstd buffer; lorenzo/lorenzo2/regression as in the Config. */
+int oracle_gf_decompress_raw(const uint8_t *raw, size_t n, int N, int lorenzo, int lorenzo2, int regression,
+                             float *out, size_t num);
 /* SZ_compress stream for ALGO_LORENZO_REG through the general frontend. */
 int oracle_gf_compress_stream(const oracle_gf_params *p, int rel_mode, double eb_in, const float *data,
                               uint8_t *out, size_t cap, size_t *n_out);

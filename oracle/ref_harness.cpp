@@ -413,22 +413,31 @@ static int run_ctest(const Args &a) {
     return 0;
 }
 
-// make_lorenzo_regression_compressor (SZLorenzoReg.hpp:27-91) with the
-// Lorenzo predictors only: one LorenzoPredictor when a single one is enabled,
-// else ComposedPredictor{1-layer, 2-layer}, in an SZGeneralFrontend.
+// make_lorenzo_regression_compressor (SZLorenzoReg.hpp:27-91) without the
+// second-order regression: the one enabled predictor alone, else a
+// ComposedPredictor over the enabled ones in the order 1-layer Lorenzo,
+// 2-layer Lorenzo, RegressionPredictor, in an SZGeneralFrontend.
 template<QoZ::uint N, class Lossless>
 static QoZ::concepts::CompressorInterface<float> *general_compressor(const QoZ::Config &conf, int radius,
                                                                       Lossless lossless) {
     typedef float T;
     auto q = QoZ::LinearQuantizer<T>(conf.absErrorBound, radius);
-    if (conf.lorenzo && conf.lorenzo2) {
+    const int count = conf.lorenzo + conf.lorenzo2 + conf.regression;
+    if (count > 1) {
         std::vector<std::shared_ptr<QoZ::concepts::PredictorInterface<T, N>>> ps;
-        ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 1>>(conf.absErrorBound));
-        ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 2>>(conf.absErrorBound));
+        if (conf.lorenzo) ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 1>>(conf.absErrorBound));
+        if (conf.lorenzo2) ps.push_back(std::make_shared<QoZ::LorenzoPredictor<T, N, 2>>(conf.absErrorBound));
+        if (conf.regression)
+            ps.push_back(std::make_shared<QoZ::RegressionPredictor<T, N>>(conf.blockSize, conf.absErrorBound));
         return QoZ::make_sz_general_compressor<T, N>(
                 QoZ::make_sz_general_frontend<T, N>(conf, QoZ::ComposedPredictor<T, N>(ps), q),
                 QoZ::HuffmanEncoder<int>(), lossless);
     }
+    if (conf.regression)
+        return QoZ::make_sz_general_compressor<T, N>(
+                QoZ::make_sz_general_frontend<T, N>(
+                        conf, QoZ::RegressionPredictor<T, N>(conf.blockSize, conf.absErrorBound), q),
+                QoZ::HuffmanEncoder<int>(), lossless);
     if (conf.lorenzo2)
         return QoZ::make_sz_general_compressor<T, N>(
                 QoZ::make_sz_general_frontend<T, N>(conf, QoZ::LorenzoPredictor<T, N, 2>(conf.absErrorBound), q),
@@ -451,7 +460,7 @@ static int run_general(const Args &a) {
     if (a.count("block")) conf.blockSize = atoi(get(a, "block").c_str());
     conf.lorenzo = atoi(get(a, "lo", "1").c_str()) != 0;
     conf.lorenzo2 = atoi(get(a, "lo2", "1").c_str()) != 0;
-    conf.regression = false;
+    conf.regression = atoi(get(a, "reg", "0").c_str()) != 0;
     conf.regression2 = false;
     conf.verbose = 0;
     QoZ::calAbsErrorBound(conf, input.data());

@@ -184,6 +184,30 @@ def general_cases():
     add("g4d_9x10x11x12_default", smooth(r, (9, 10, 11, 12), 2), mode="rel", eb=1e-3)
     add("g4d_8x7x9x10_bs3_lo2only", smooth(r, (8, 7, 9, 10), 2), mode="rel", eb=1e-3, block=3, lo=0, lo2=1)
     add("g4d_6x7x5x8_bs2", smooth(r, (6, 7, 5, 8), 2), mode="rel", eb=1e-4, block=2)
+    # RegressionPredictor inside the ComposedPredictor: the Config default
+    # (lorenzo, regression; CFG:410-413) and the other compositions, blocks
+    # where regression wins (planes plus noise) and where Lorenzo does
+    plane = lambda shape: (sum(np.indices(shape, dtype=np.float64)[a] * (0.013 * (a + 1))
+                               for a in range(len(shape))) + 1e-4 * r.standard_normal(shape)).astype(np.float32)
+    mixed = lambda shape: (np.where(np.indices(shape)[0] < shape[0] // 2, plane(shape), smooth(r, shape, 2))
+                           ).astype(np.float32)
+    add("r1d_1000_default", smooth(r, (1000,), 2), mode="rel", eb=1e-3, lo=1, lo2=0, reg=1)
+    add("r1d_900_planes", plane((900,)), mode="rel", eb=1e-4, block=30, lo=1, lo2=0, reg=1)
+    add("r2d_100x130_default", smooth(r, (100, 130), 2), mode="rel", eb=1e-3, lo=1, lo2=0, reg=1)
+    add("r2d_96x112_mixed", mixed((96, 112)), mode="rel", eb=1e-4, lo=1, lo2=0, reg=1)
+    add("r2d_61x47_bs5_all3", mixed((61, 47)), mode="rel", eb=1e-3, block=5, lo=1, lo2=1, reg=1)
+    add("r2d_64x80_regonly", plane((64, 80)), mode="rel", eb=1e-3, lo=0, lo2=0, reg=1)
+    add("r2d_40x56_lo2reg", mixed((40, 56)), mode="rel", eb=1e-3, block=8, lo=0, lo2=1, reg=1)
+    add("r2d_50x60_qbins1024", mixed((50, 60)), mode="rel", eb=1e-6, qbins=1024, lo=1, lo2=0, reg=1)
+    spikes = plane((48, 40))
+    spikes.flat[r.choice(spikes.size, 12, replace=False)] = np.float32(3e30)
+    add("r2d_spikes_overflow", spikes, mode="abs", eb=1e-3, block=8, lo=1, lo2=0, reg=1)
+    add("r3d_20x24x18_bs6", mixed((20, 24, 18)), mode="rel", eb=1e-3, block=6, lo=1, lo2=0, reg=1)
+    add("r4d_9x10x11x12_default", smooth(r, (9, 10, 11, 12), 2), mode="rel", eb=1e-3, lo=1, lo2=0, reg=1)
+    add("r4d_8x9x10x12_mixed", mixed((8, 9, 10, 12)), mode="rel", eb=1e-4, block=3, lo=1, lo2=0, reg=1)
+    # a last block of extent 1 along an axis (33 = 2 * 16 + 1): the
+    # regression predictor declines it and the 1-layer Lorenzo fallback runs
+    add("r2d_33x40_edge1", mixed((33, 40)), mode="rel", eb=1e-3, lo=1, lo2=0, reg=1)
     return c
 
 
@@ -278,7 +302,7 @@ def main():
         x.astype(np.float32).tofile(inp)
         a = [HARNESS, "general", f"in={inp}", f"out={out}", "dims=" + ",".join(map(str, x.shape)),
              f"mode={kw.get('mode', 'abs')}", f"eb={kw['eb']!r}", f"qbins={kw.get('qbins', 65536)}",
-             f"lo={kw.get('lo', 1)}", f"lo2={kw.get('lo2', 1)}"]
+             f"lo={kw.get('lo', 1)}", f"lo2={kw.get('lo2', 1)}", f"reg={kw.get('reg', 0)}"]
         if "block" in kw:
             a.append(f"block={kw['block']}")
         subprocess.check_call(a)

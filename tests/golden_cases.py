@@ -20,6 +20,7 @@ CTEST = names("q")
 GENERAL = names("g")
 SZ3 = names("z")
 CTEST_LORENZO = names("k")
+REGRESSION = names("r")
 
 
 def load(name):
@@ -61,7 +62,8 @@ def gf_kwargs(case):
     n = len(p["dims"])
     return dict(dims=p["dims"], eb=float(np.asarray(case["eb"]).ravel()[0]),
                 block_size=p.get("block", 128 if n == 1 else (16 if n == 2 else 6)),
-                lorenzo=p.get("lo", 1), lorenzo2=p.get("lo2", 1), radius=p.get("qbins", 65536) // 2)
+                lorenzo=p.get("lo", 1), lorenzo2=p.get("lo2", 1), radius=p.get("qbins", 65536) // 2,
+                regression=p.get("reg", 0))
 
 
 def trailer(stream):

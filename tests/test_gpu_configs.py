@@ -173,17 +173,17 @@ def test_c3_48_field_batch(engine):
 def model(engine):
     from faz_amd import learned
     net, mu, sigma = learned.load_model(engine)
-    yield net, mu, sigma
+    yield net, mu, sigma, learned.load_gates()
     net.close()
 
 
 @pytest.mark.parametrize("rel,slope,fronts", [(1e-3, 2.6, 4), (1e-3, 4.4, 12), (1e-4, 3.3, 8), (1e-4, 4.8, 0)])
 def test_c4_learned_tuner_384(engine, model, rel, slope, fronts):
     from faz_amd import learned
-    net, mu, sigma = model
+    net, mu, sigma, gates = model
     print(f"[gpu] C4 384^3 rel={rel} slope={slope} fronts={fronts}: learned vs exhaustive", flush=True)
     r = learned.evaluate(engine, net, mu, sigma, (384, 384, 384), (1 << 43) + int(slope * 1000) + fronts, rel,
-                         slope=slope, fronts=fronts)
+                         slope=slope, fronts=fronts, gates=gates)
     ex, le = r["exhaustive"], r["learned"]
     cr_e = 32.0 / ex["bitrate"]
     cr_l = learned.ratio_at_equal_psnr(le["bitrate"], le["psnr"], ex["psnr"])
@@ -210,24 +210,34 @@ def field_c5():
 
 @pytest.mark.parametrize("rel", [1e-2, 1e-3, 1e-4, 1e-6])
 def test_c5_1024x1024x512_learned_choice(engine, model, field_c5, rel):
+    """The learned tuner chooses among the interpolation candidates and the
+    Lorenzo test (the exhaustive search as fallback when its gates say so);
+    the chosen engine keeps every point within the bound in FP64 and the GPU
+    decode equals the compression's reconstruction."""
     import faz_amd
-    net, mu, sigma = model
+    from faz_amd import learned
+    net, mu, sigma, gates = model
     x = field_c5
     eb, rng, _, _ = _abs_eb(engine, x, rel)
     print(f"[gpu] C5 1024x1024x512 rel={rel}: learned choice, bound, decode", flush=True)
-    r = engine.tune_learned(x, C5_SHAPE, eb, rel, rng, net, mu, sigma)
-    p = faz_amd.make_params(C5_SHAPE, eb, algo=r["interp_algo"], direction=r["direction"], alpha=r["alpha"],
-                            beta=r["beta"], max_step=r["max_step"], ops=tuple(r["ops"]), dirs=tuple(r["dirs"]))
+    r = learned.tune_learned(engine, x, C5_SHAPE, eb, rel, rng, net, mu, sigma, gates)
     recon = torch.empty_like(x)
-    raw = engine.interp_raw(x, p, recon=recon)
+    out = torch.empty_like(x)
+    if r["use_interp"]:
+        raw = engine.interp_raw(x, learned.choice_params(C5_SHAPE, eb, r), recon=recon)
+        engine.decompress_raw(raw, 3, out)
+    else:
+        p = faz_amd.make_lorenzo_params(C5_SHAPE, eb, block_size=5, pred_dim=3, lorenzo=True, lorenzo2=True,
+                                        regression=False)
+        raw = engine.lorenzo_raw(x, p, recon=recon)
+        engine.lorenzo_decompress_raw(raw, C5_SHAPE, out)
     err = (recon.double() - x.double()).abs().max().item()
     assert err <= eb
     mse = ((recon.double() - x.double()) ** 2).mean().item()
-    out = torch.empty_like(x)
-    engine.decompress_raw(raw, 3, out)
     assert torch.equal(out.view(torch.int32), recon.view(torch.int32))
-    print(f"  alpha {r['alpha']} beta {r['beta']} ops {r['ops']}: raw ratio {4.0 * x.numel() / len(raw):.2f}, "
-          f"PSNR {20 * math.log10(rng) - 10 * math.log10(mse):.2f} dB, max err {err:.3e} <= {eb:.3e}", flush=True)
+    print(f"  use_interp {r['use_interp']} alpha {r['alpha']} beta {r['beta']} ops {r['ops']} path "
+          f"{r['learned_path']}: raw ratio {4.0 * x.numel() / len(raw):.2f}, PSNR "
+          f"{20 * math.log10(rng) - 10 * math.log10(mse):.2f} dB, max err {err:.3e} <= {eb:.3e}", flush=True)
     del recon, out
     _free()
 

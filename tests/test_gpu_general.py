@@ -1,5 +1,6 @@
-"""GPU parity of the general-frontend Lorenzo engine (SZGeneralFrontend with
-ComposedPredictor{1-layer, 2-layer Lorenzo}, N = 1..4; needs an MI355X).
+"""GPU parity of the general-frontend engine (SZGeneralFrontend with
+ComposedPredictor over the 1-layer and 2-layer Lorenzo predictors and the
+RegressionPredictor, or one of them alone, N = 1..4; needs an MI355X).
 
 Bar: bit-exact block-major codes, selections, unpredictable values,
 reconstructions and pre-zstd buffers against the reference's golden vectors
@@ -11,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import CTEST_LORENZO, GENERAL, gf_kwargs, load, trailer
+from golden_cases import CTEST_LORENZO, GENERAL, REGRESSION, gf_kwargs, load, trailer
 
 pytestmark = pytest.mark.gpu
 
@@ -30,14 +31,17 @@ def _params(kw):
 def _check_against_oracle(engine, x, kw):
     o = oracle.gf_compress(x, oracle.make_gf_params(**kw))
     r = engine.general_compress(_dev(x), _params(kw))
-    assert np.array_equal(r["selection"].astype(np.int32), o["selection"])
+    sel = r["selection"][r["selection"] != 255]  # 255: the block recorded no selection
+    assert np.array_equal(sel.astype(np.int32), o["selection"])
     assert np.array_equal(r["codes"].astype(np.int32), o["codes"])
     assert np.array_equal(r["unpred"].view(np.uint32), o["unpred"].view(np.uint32))
     assert np.array_equal(r["recon"].cpu().numpy().view(np.uint32), o["recon"].view(np.uint32))
+    if kw.get("regression"):
+        print(f"  {r['n_regression']} regression blocks, {r['passes']} wavefront passes", flush=True)
     return o
 
 
-@pytest.mark.parametrize("name", GENERAL)
+@pytest.mark.parametrize("name", GENERAL + REGRESSION)
 def test_golden_raw_recon_decode(engine, name):
     c = load(name)
     kw = gf_kwargs(c)
@@ -48,11 +52,12 @@ def test_golden_raw_recon_decode(engine, name):
     assert raw == c["raw"].tobytes()
     assert np.array_equal(recon.cpu().numpy().view(np.uint32), c["recon"].view(np.uint32))
     out = torch.empty_like(recon)
-    engine.general_decompress_raw(c["raw"].tobytes(), kw["dims"], out, kw["lorenzo"], kw["lorenzo2"])
+    engine.general_decompress_raw(c["raw"].tobytes(), kw["dims"], out, kw["lorenzo"], kw["lorenzo2"],
+                                  kw["regression"])
     assert np.array_equal(out.cpu().numpy().view(np.uint32), c["recon"].view(np.uint32))
 
 
-@pytest.mark.parametrize("name", [n for n in GENERAL if not n.startswith("g3d")])
+@pytest.mark.parametrize("name", [n for n in GENERAL + REGRESSION if not n[1:].startswith("3d")])
 def test_golden_stream(engine, name):
     """SZ_compress stream (N != 3 takes the general frontend): the reference's
     Config trailer; the reference's stream decodes on the GPU and ours on the
@@ -108,6 +113,19 @@ CASES = [
     ((17, 19, 21, 23), dict(), 1e-3),
     ((20, 9, 10, 31), dict(block_size=8, lorenzo=False), 1e-3),
     ((150, 160), dict(block_size=16, radius=512), 1e-6),
+    # the RegressionPredictor: composed with each Lorenzo predictor, alone,
+    # all three, blocks of extent 1 at the high faces, N = 1..4
+    ((20001,), dict(regression=True), 1e-4),
+    ((3001,), dict(block_size=10, lorenzo=False, lorenzo2=False, regression=True), 1e-3),
+    ((2500,), dict(block_size=7, lorenzo2=True, regression=True), 1e-3),
+    ((300, 410), dict(lorenzo2=False, regression=True), 1e-3),
+    ((257, 193), dict(block_size=8, lorenzo2=False, regression=True), 1e-4),
+    ((129, 97), dict(block_size=16, lorenzo=False, lorenzo2=False, regression=True), 1e-3),
+    ((201, 150), dict(block_size=5, lorenzo2=True, regression=True), 1e-3),
+    ((61, 50, 73), dict(block_size=6, lorenzo2=False, regression=True), 1e-3),
+    ((31, 34, 25), dict(block_size=6, lorenzo=False, regression=True), 1e-5),
+    ((17, 19, 21, 25), dict(block_size=6, lorenzo2=False, regression=True), 1e-3),
+    ((13, 9, 10, 31), dict(block_size=4, lorenzo=False, lorenzo2=False, regression=True), 1e-3),
 ]
 
 
@@ -120,12 +138,22 @@ def test_random_fields_against_oracle(engine, i):
     if "block_size" not in kw:
         kw["block_size"] = 128 if len(shape) == 1 else (16 if len(shape) == 2 else 6)
     print(f"[gpu] general oracle parity {shape} {extra} eb={eb}", flush=True)
-    _check_against_oracle(engine, x, kw)
+    o = _check_against_oracle(engine, x, kw)
     raw = engine.general_raw(_dev(x), _params(kw))
     assert raw == oracle.gf_raw(x, oracle.make_gf_params(**kw))
     out = torch.empty(x.size, dtype=torch.float32, device="cuda")
-    engine.general_decompress_raw(raw, shape, out, kw.get("lorenzo", True), kw.get("lorenzo2", True))
-    o = oracle.gf_decompress_raw(raw, shape, kw.get("lorenzo", True), kw.get("lorenzo2", True))
+    lo, lo2, reg = kw.get("lorenzo", True), kw.get("lorenzo2", True), kw.get("regression", False)
+    nblocks = int(np.prod([(d - 1) // kw["block_size"] + 1 for d in shape]))
+    if int(lo) + int(lo2) + int(reg) > 1 and o["selection"].size < nblocks:
+        # a 1-D block of one point chose the regression, which declined it:
+        # no selection was recorded and the reference's decoder would read
+        # past its list, so both decoders reject the buffer
+        import faz_amd
+        with pytest.raises(faz_amd.FazError):
+            engine.general_decompress_raw(raw, shape, out, lo, lo2, reg)
+        return
+    engine.general_decompress_raw(raw, shape, out, lo, lo2, reg)
+    o = oracle.gf_decompress_raw(raw, shape, lo, lo2, reg)
     assert np.array_equal(out.cpu().numpy().view(np.uint32), o.view(np.uint32))
 
 
@@ -145,6 +173,40 @@ def test_1800x3600_full_parity(engine):
     out = torch.empty_like(recon)
     engine.general_decompress_raw(raw, (1800, 3600), out)
     assert torch.equal(out.view(torch.int32), recon.view(torch.int32))
+
+
+def test_1800x3600_regression_full_parity(engine):
+    """ALGO_LORENZO_REG's default predictors for a 2-D field (1-layer Lorenzo
+    and the RegressionPredictor, blockSize 16) on the climate-grid shape: the
+    pre-zstd buffer equals the oracle's, the stream decodes on the GPU and on
+    the oracle to the same reconstruction, within the bound in FP64.  The
+    field is smooth on its left half (planes win) and carries a ripple of
+    period close to the block size on its right half (Lorenzo wins), so the
+    choices mix and the passes over them matter."""
+    import faz_amd
+    x = _field((1800, 3600), 12)
+    i, j = np.meshgrid(np.arange(1800), np.arange(1800, 3600), indexing="ij")
+    x[:, 1800:] += (0.02 * np.sin(2 * np.pi * i / 23.0) * np.sin(2 * np.pi * j / 17.0)).astype(np.float32)
+    eb = 1e-3 * float(np.float32(x.max() - x.min()))
+    kw = dict(dims=(1800, 3600), eb=eb, block_size=16, lorenzo=True, lorenzo2=False, regression=True)
+    print("[gpu] general 1800x3600 with the RegressionPredictor", flush=True)
+    d = _dev(x)
+    r = engine.general_compress(d, _params(kw))
+    print(f"  {r['n_regression']} of {113 * 225} blocks regression, {r['passes']} passes", flush=True)
+    assert 0 < r["n_regression"] < 113 * 225
+    raw = engine.general_raw(d, _params(kw))
+    assert raw == oracle.gf_raw(x, oracle.make_gf_params(**kw))
+    out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+    engine.general_decompress_raw(raw, (1800, 3600), out, True, False, True)
+    assert (out.double() - d.double()).abs().max().item() <= eb
+    o = oracle.gf_decompress_raw(raw, (1800, 3600), True, False, True)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), o.view(np.uint32))
+    # the stream path with the same Config
+    pp = faz_amd.make_general_params(**dict(kw, eb=0.0))
+    s = engine.sz_compress_general(x, pp, eb, rel=False)
+    dec = engine.sz_decompress(s, x.size)
+    assert np.array_equal(dec.view(np.uint32), o.view(np.uint32))
+    assert np.array_equal(oracle.decompress_stream(s, x.size).view(np.uint32), o.view(np.uint32))
 
 
 def test_tuner_blocks_against_oracle(engine):

@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
-from golden_cases import (CTEST, CTEST_LORENZO, GENERAL, HUFFMAN, INTERP, LORENZO, SZ3, TUNING, ctest_params, gf_kwargs,
+from golden_cases import (CTEST, CTEST_LORENZO, GENERAL, HUFFMAN, INTERP, LORENZO, REGRESSION, SZ3, TUNING, ctest_params,
+                          gf_kwargs,
                           load, lr_kwargs, lr_raw_equal, param_kwargs, trailer)
 
 
@@ -156,19 +157,19 @@ def test_compress_test_buffer(name):
     assert abs(nbytes - int(c["ref_zstd_bytes"][0])) <= 0.005 * nbytes + 8
 
 
-@pytest.mark.parametrize("name", GENERAL)
+@pytest.mark.parametrize("name", GENERAL + REGRESSION)
 def test_general_raw_recon_decode(name):
-    """General frontend with the Lorenzo predictors (SZGeneralFrontend +
-    ComposedPredictor, N = 1..4): the pre-zstd buffer bit for bit, the
-    overwritten array equals the reference's decoded reconstruction, and the
-    oracle decodes the reference's buffer to it."""
+    """General frontend (SZGeneralFrontend + ComposedPredictor, N = 1..4)
+    with the Lorenzo predictors and the RegressionPredictor: the pre-zstd
+    buffer bit for bit, the overwritten array equals the reference's decoded
+    reconstruction, and the oracle decodes the reference's buffer to it."""
     c = load(name)
     kw = gf_kwargs(c)
     prm = oracle.make_gf_params(**kw)
     assert oracle.gf_raw(c["input"], prm) == c["raw"].tobytes()
     r = oracle.gf_compress(c["input"], prm)
     assert np.array_equal(r["recon"].view(np.uint32), c["recon"].view(np.uint32))
-    dec = oracle.gf_decompress_raw(c["raw"].tobytes(), kw["dims"], kw["lorenzo"], kw["lorenzo2"])
+    dec = oracle.gf_decompress_raw(c["raw"].tobytes(), kw["dims"], kw["lorenzo"], kw["lorenzo2"], kw["regression"])
     assert np.array_equal(dec.view(np.uint32), c["recon"].view(np.uint32))
     x = c["input"].astype(np.float64).ravel()
     bad = np.abs(r["recon"].astype(np.float64) - x) > kw["eb"]
@@ -176,7 +177,7 @@ def test_general_raw_recon_decode(name):
     assert r["unpred"].size == int((r["codes"] == 0).sum())
 
 
-@pytest.mark.parametrize("name", GENERAL)
+@pytest.mark.parametrize("name", GENERAL + REGRESSION)
 def test_general_stream(name):
     """SZ_compress stream: the Config trailer bit for bit; for N != 3 (where
     SZ_decompress takes the general frontend, LR:117-135) the oracle decodes
@@ -208,6 +209,25 @@ def test_general_fixtures_cover_every_branch():
         thin += any(d % kw["block_size"] == 1 for d in kw["dims"])
     assert (sel > 50).all() and ns == {1, 2, 3, 4} and unpred > 100 and thin >= 3
     assert {gf_kwargs(load(n))["block_size"] for n in GENERAL} >= {1, 2}
+
+
+def test_regression_fixtures_cover_every_branch():
+    """The RegressionPredictor chosen in many blocks next to Lorenzo blocks,
+    alone, with both Lorenzo predictors, N = 1..4, a last block of extent 1,
+    coefficients beyond the coefficient quantizers' range."""
+    chosen, mixed, ns = 0, 0, set()
+    for name in REGRESSION:
+        c = load(name)
+        kw = gf_kwargs(c)
+        r = oracle.gf_compress(c["input"], oracle.make_gf_params(**kw))
+        ns.add(len(kw["dims"]))
+        reg_sid = int(kw["lorenzo"]) + int(kw["lorenzo2"])  # the regression predictor's index
+        if r["selection"].size:
+            k = int((r["selection"] == reg_sid).sum())
+            chosen += k
+            mixed += 0 < k < r["selection"].size
+    assert chosen > 100 and mixed >= 4 and ns == {1, 2, 3, 4}
+    assert any(d % gf_kwargs(load(n))["block_size"] == 1 for n in REGRESSION for d in gf_kwargs(load(n))["dims"])
 
 
 @pytest.mark.parametrize("name", CTEST_LORENZO)
