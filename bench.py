@@ -87,6 +87,17 @@ def measured_traffic(kernel, dims):
     return int(m.group(1)) if m else None
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo, as lscpu prints it)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(x, dims, rel):
     """Reference CPU compressor (oracle/_ref, built from the reference sources)
     on this host, one thread; falls back to the oracle port when absent."""
@@ -105,7 +116,7 @@ def cpu_baseline(x, dims, rel):
             if out.returncode == 0:
                 r = json.loads(out.stdout.strip().splitlines()[-1])
                 return {"value": round(4.0 * n / r["seconds"] / 1e9, 4), "unit": "GB/s", "cores": 1,
-                        "kind": "reference", "sample": sample,
+                        "kind": "reference", "sample": sample, "cpu_model": cpu_model(),
                         "ratio": round(4.0 * n / r["bytes"], 3)}
     # oracle port: time one interpolation pass over a 256^3 crop (single thread)
     crop = host.reshape(dims)[:256, :256, :256].copy()
@@ -115,7 +126,7 @@ def cpu_baseline(x, dims, rel):
     oracle.compress_stream(crop, p, rel, rel=True)
     dt = time.perf_counter() - t0
     return {"value": round(4.0 * crop.size / dt / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": "256^3 crop of the field, REL %g, oracle port incl. zstd" % rel}
+            "sample": "256^3 crop of the field, REL %g, oracle port incl. zstd" % rel, "cpu_model": cpu_model()}
 
 
 def tuner_report(eng, x, dims, rel, mn, mx):
@@ -128,6 +139,7 @@ def tuner_report(eng, x, dims, rel, mn, mx):
     import faz_amd
     from faz_amd import learned
     net, mu, sigma = learned.load_model(eng)
+    gates = learned.load_gates()
     rng = float(np.float32(mx - mn))
     eb = rel * rng
     n = x.numel()
@@ -137,8 +149,9 @@ def tuner_report(eng, x, dims, rel, mn, mx):
         for _ in range(3):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
+            # faz -q: the QoZ preset clears testLorenzo (SZI:1616) for both
             r = eng.tune_qoz(x, dims, eb, rel, rng) if name == "exhaustive" else \
-                eng.tune_learned(x, dims, eb, rel, rng, net, mu, sigma)
+                learned.tune_learned(eng, x, dims, eb, rel, rng, net, mu, sigma, gates, test_lorenzo=0)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t0) * 1e3
             best = ms if best is None else min(best, ms)
@@ -152,6 +165,7 @@ def tuner_report(eng, x, dims, rel, mn, mx):
         nbytes, mse = learned.final_stats(eng, x, dims, eb, p)
         res[name] = {"tune_ms": round(best, 3), "compress_ms": round(cms, 3), "alpha": r["alpha"], "beta": r["beta"],
                      "ops": r["ops"], "dirs": r["dirs"], "compress_tests": len(r["tests"]), "raw_bytes": size,
+                     "learned_path": r["learned_path"], "n_verified": r["n_verified"],
                      "bitrate": round(8.0 * nbytes / n, 5), "psnr_db": round(20 * math.log10(rng) -
                                                                              10 * math.log10(mse), 4)}
     ex, le = res["exhaustive"], res["learned"]
@@ -173,6 +187,125 @@ def tuner_report(eng, x, dims, rel, mn, mx):
     except Exception as e:
         out["cpu_oracle_exhaustive_ms"] = None
         out["cpu_oracle_error"] = str(e)[:200]
+    return out
+
+
+def rel_report(eng, x, dims, rel, steps=5):
+    """The bench step at another error bound on the same field: time per
+    step, GB/s, engine buffer ratio, PSNR and the bound checked in FP64."""
+    import torch
+    import faz_amd
+    p = faz_amd.make_params(dims, 0.0, algo=1, max_step=32)
+    eng.compress_device(x, p, True, rel)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        size = eng.compress_device(x, p, True, rel)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    recon = torch.empty_like(x)
+    eng.compress_device(x, p, True, rel, recon=recon)
+    mn, mx = eng.range(x)
+    rng = float(np.float32(mx - mn))
+    err = recon.double() - x.double()
+    mse = (err * err).mean().item()
+    out = {"rel_eb": rel, "ms_per_step": round(dt * 1e3, 4), "GB/s": round(4.0 * x.numel() / dt / 1e9, 2),
+           "raw_ratio": round(4.0 * x.numel() / size, 3), "psnr_db": round(20 * math.log10(rng) - 10 * math.log10(mse), 3),
+           "max_abs_err": err.abs().max().item(), "eb": rel * rng}
+    out["bound_holds"] = out["max_abs_err"] <= out["eb"]
+    del recon, err
+    return out
+
+
+def c5_report(eng, fields, rels=(1e-2, 1e-3, 1e-4, 1e-6), shape=(1024, 1024, 512), seed=500):
+    """The combined path at scale: fields of 1024x1024x512 f32 (2 GiB) with
+    spectral slopes k^-2..k^-5 plus tanh fronts, generated one at a time; per
+    field and REL bound the learned tuner (Lorenzo test on, gates on:
+    out-of-range inputs fall back to the exhaustive search) picks the
+    interpolation configuration or the Lorenzo engine, and the choice is
+    compressed on the device.  Reported per bound: mean engine-buffer ratio
+    (pre-zstd) and PSNR, the compression GB/s, the learned tuning time, the
+    predictor and gate outcomes, and the same field through the plain GPU
+    path (cubic, untuned alpha/beta) for comparison; the first field also
+    runs the exhaustive tuner."""
+    import torch
+    import faz_amd
+    from faz_amd import learned
+    net, mu, sigma = learned.load_model(eng)
+    gates = learned.load_gates()
+    n = int(np.prod(shape))
+    per = {rel: {"ratio": [], "psnr": [], "gbs": [], "tune_ms": [], "plain_ratio": [], "plain_psnr": [],
+                 "lorenzo": 0, "path": [0, 0, 0], "bound_ok": True} for rel in rels}
+    exhaustive = {}
+    t_start = time.perf_counter()
+    for k in range(fields):
+        slope = 2.0 + 3.0 * k / max(fields - 1, 1)
+        x = make_field(shape, seed=seed + k, fronts=8, slope=slope)
+        mn, mx = eng.range(x)
+        rng = float(np.float32(mx - mn))
+        for rel in rels:
+            eb = rel * rng
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            t = learned.tune_learned(eng, x, shape, eb, rel, rng, net, mu, sigma, gates, test_lorenzo=1)
+            torch.cuda.synchronize()
+            r = per[rel]
+            r["tune_ms"].append((time.perf_counter() - t0) * 1e3)
+            r["path"][t["learned_path"]] += 1
+            recon = torch.empty_like(x)
+            if t["use_interp"]:
+                p = learned.choice_params(shape, eb, t)
+                eng.compress_device(x, p, False, eb)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                size = eng.compress_device(x, p, False, eb)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                eng.compress_device(x, p, False, eb, recon=recon)
+            else:
+                r["lorenzo"] += 1
+                lp = faz_amd.make_lorenzo_params(shape, eb, block_size=5, pred_dim=3, lorenzo=True, lorenzo2=True,
+                                                 regression=False)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                size = len(eng.lorenzo_raw(x, lp, recon=recon))
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+            err = recon.double() - x.double()
+            r["bound_ok"] &= err.abs().max().item() <= eb
+            mse = (err * err).mean().item()
+            r["ratio"].append(4.0 * n / size)
+            r["psnr"].append(20 * math.log10(rng) - 10 * math.log10(mse))
+            r["gbs"].append(4.0 * n / dt / 1e9)
+            del err
+            plain = faz_amd.make_params(shape, eb, algo=1, max_step=32)
+            psize = eng.compress_device(x, plain, False, eb, recon=recon)
+            err = recon.double() - x.double()
+            r["plain_ratio"].append(4.0 * n / psize)
+            r["plain_psnr"].append(20 * math.log10(rng) - 10 * math.log10((err * err).mean().item()))
+            del err, recon
+            if k == 0:
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                ex = eng.tune_qoz(x, shape, eb, rel, rng, test_lorenzo=1)
+                torch.cuda.synchronize()
+                exhaustive[str(rel)] = {"tune_ms": round((time.perf_counter() - t0) * 1e3, 2),
+                                        "same_choice": (ex["use_interp"], ex["alpha"], ex["beta"]) ==
+                                                       (t["use_interp"], t["alpha"], t["beta"]),
+                                        "learned_tune_ms": round(r["tune_ms"][-1], 2)}
+        del x
+        torch.cuda.empty_cache()
+    net.close()
+    out = {"fields": fields, "shape": list(shape), "slopes": "k^-2..k^-5", "ratio": "engine buffer (pre-zstd)",
+           "seconds": round(time.perf_counter() - t_start, 1), "exhaustive_first_field": exhaustive, "by_rel": {}}
+    for rel, r in per.items():
+        out["by_rel"][str(rel)] = {
+            "ratio_mean": round(float(np.mean(r["ratio"])), 3), "psnr_mean": round(float(np.mean(r["psnr"])), 3),
+            "GB/s_mean": round(float(np.mean(r["gbs"])), 1), "learned_tune_ms_mean": round(float(np.mean(r["tune_ms"])), 2),
+            "plain_ratio_mean": round(float(np.mean(r["plain_ratio"])), 3),
+            "plain_psnr_mean": round(float(np.mean(r["plain_psnr"])), 3),
+            "lorenzo_chosen": r["lorenzo"], "learned_paths_direct_verified_fallback": r["path"],
+            "bound_holds": bool(r["bound_ok"])}
     return out
 
 
@@ -303,6 +436,9 @@ def main():
     ap.add_argument("--no-tuner", action="store_true", help="skip the exhaustive / learned tuner comparison")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven-fields report")
     ap.add_argument("--batch-fields", type=int, default=48)
+    ap.add_argument("--no-rel-sweep", action="store_true", help="skip the REL 1e-5 step on the same field")
+    ap.add_argument("--c5-fields", type=int, default=4,
+                    help="1024x1024x512 fields of the combined-path leg (0 skips it; the full workload is 24)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one device copy of the field (4n bytes read, 4n written) before the steps, the known-size "
                          "kernel that calibrates FETCH_SIZE / WRITE_SIZE in rocprofv3 --pmc runs")
@@ -392,6 +528,11 @@ def main():
         line["decompress"] = decompress_report(eng, raw, recon)
     except Exception as e:  # reported beside the measurement
         line["decompress"] = {"error": str(e)[:200]}
+    if not args.no_rel_sweep:
+        try:
+            line["rel_1e-5"] = rel_report(eng, x, dims, 1e-5)
+        except Exception as e:  # reported beside the measurement
+            line["rel_1e-5"] = {"error": str(e)[:200]}
     if not args.no_tuner:
         try:
             line["tuner"] = tuner_report(eng, x, dims, args.rel, mn, mx)
@@ -402,6 +543,11 @@ def main():
             line["batch"] = batch_report(eng, args.batch_fields, args.seed + 17, args.rel)
         except Exception as e:  # reported beside the measurement
             line["batch"] = {"error": str(e)[:200]}
+    if args.c5_fields > 0:
+        try:
+            line["c5"] = c5_report(eng, args.c5_fields)
+        except Exception as e:  # reported beside the measurement
+            line["c5"] = {"error": str(e)[:200]}
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline(x, dims, args.rel)

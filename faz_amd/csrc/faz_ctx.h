@@ -104,7 +104,9 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint32_t> freq, first, chunk_zeros, err_flag, minmax, words;
     faz::DevBuf<uint32_t> hist_pack;  // one readback block after the histogram
     faz::DevBuf<uint32_t> enc_head;   // one-pass encode: bits of each chunk's first, shared word
-    bool enc_onepass = true;          // FAZ_ENCODE=twopass selects the sizes + scan + pack encoder
+    // FAZ_ENCODE=onepass selects the decoupled look-back encoder over the
+    // sizes + scan + pack one (slower at 512^3: 0.41 ms against 0.20 ms)
+    bool enc_onepass = false;
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to
@@ -160,6 +162,9 @@ struct faz_gpu_ctx {
     faz::DevBuf<float> gf_fields, gf_recon;
     faz::DevBuf<float> gf_reg_raw, gf_reg_deq;  // RegressionPredictor planes, 5 floats per block
     faz::DevBuf<uint32_t> gf_reg_code;          // 1-D: coefficient codes per block
+    // the tuner's device buffers (owned by the tuner's source file)
+    void *tune_cache = nullptr;
+    void (*tune_cache_free)(void *) = nullptr;
     uint32_t gf_tab_key = 0;  // N << 16 | bs of the table
     // Huffman decoder: the bitstream as words, the lookup table and tree,
     // per-chunk start and symbol offset, symbol counts + error flag, the

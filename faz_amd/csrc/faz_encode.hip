@@ -269,11 +269,17 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
 constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2;
 constexpr uint32_t kLookbackSpins = 1u << 24;
 
+// Status words are read and written with relaxed device-scope atomics: they
+// stay coherent across the XCDs (each has its own L2) without ordering any
+// other access, which is all the look-back needs, since a word carries its
+// flag and its value together and the bit words and offsets are read only by
+// later launches.  Acquire / release would add an L2 invalidate / write-back
+// to every access.
 __device__ __forceinline__ uint64_t status_load(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template<bool LDS_TAB>

@@ -963,14 +963,30 @@ int gf_compress(faz_gpu_ctx *ctx, const float *d_in, float *recon, const faz_gen
 // Huffman of a host-side int stream over nsym symbols: HuffmanEncoder<int>
 // preprocess_encode + save + encode (HUF:103-227).
 bool gf_host_huffman(const int32_t *s, size_t n, int nsym, std::vector<uint8_t> &out) {
-    std::vector<uint32_t> freq(nsym, 0), first(nsym, UINT32_MAX);
+    // counts over the window of the nonzero symbols, symbol 0 apart
+    uint32_t lo = UINT32_MAX, hi = 0;
     for (size_t i = 0; i < n; i++) {
         const uint32_t v = (uint32_t) s[i];
         if (v >= (uint32_t) nsym) return false;
-        if (!freq[v]++) first[v] = (uint32_t) i;
+        if (v) {
+            lo = std::min(lo, v);
+            hi = std::max(hi, v);
+        }
+    }
+    if (lo > hi) lo = hi = 1;
+    const uint32_t nwin = hi - lo + 1;
+    std::vector<uint32_t> freq(nwin, 0), first(nwin, UINT32_MAX);
+    uint32_t freq0 = 0, first0 = UINT32_MAX;
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t v = (uint32_t) s[i];
+        if (!v) {
+            if (!freq0++) first0 = (uint32_t) i;
+        } else if (!freq[v - lo]++) {
+            first[v - lo] = (uint32_t) i;
+        }
     }
     HuffTable t;
-    if (build_huffman(freq.data(), first.data(), nsym, t)) return false;
+    if (build_huffman_window(freq.data(), first.data(), (int) lo, (int) nwin, freq0, first0, t)) return false;
     out.insert(out.end(), t.tree.begin(), t.tree.end());
     encode_huffman(t, s, n, out);
     return true;

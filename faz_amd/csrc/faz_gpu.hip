@@ -406,7 +406,7 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (kern && (strcmp(kern, "pass") == 0 || strcmp(kern, "level") == 0)) ctx->tile3d = false;
     if (kern && strcmp(kern, "tile") == 0) ctx->fused_min_blocks = 1;
     const char *enc = getenv("FAZ_ENCODE");
-    if (enc && strcmp(enc, "twopass") == 0) ctx->enc_onepass = false;
+    if (enc && strcmp(enc, "onepass") == 0) ctx->enc_onepass = true;
     // the allocation-free encoder's scratch address on this device (its
     // entry point reports a failure here)
     (void) noalloc_init(device);
@@ -478,6 +478,12 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->gf_sel.release();
     ctx->gf_fields.release();
     ctx->gf_recon.release();
+    ctx->gf_reg_raw.release();
+    ctx->gf_reg_deq.release();
+    ctx->gf_reg_code.release();
+    ctx->enc_head.release();
+    if (ctx->tune_cache_free) ctx->tune_cache_free(ctx->tune_cache);
+    ctx->tune_cache = nullptr;
     for (auto &e: ctx->ev)
         if (e) (void) hipEventDestroy(e);
     for (auto &e: ctx->prof_ev) (void) hipEventDestroy(e);

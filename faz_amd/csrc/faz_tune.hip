@@ -415,12 +415,9 @@ void set_range(TuneCand &c, const TuneGeom &G, int mode, int start, int end) {
     c.end = end;
 }
 
-struct Tuner {
-    faz_gpu_ctx *ctx;
-    TunePlan pl;
-    std::vector<uint64_t> origins;  // field offsets of the sample blocks
-    const float *d_field;
-    double rng;
+// Device buffers of the tuner, kept by the context across calls (they only
+// grow), so a tuning call allocates nothing once the sizes have been seen.
+struct TuneBufs {
     DevBuf<TuneCand> d_cands;
     DevBuf<TuneItem> d_items;
     DevBuf<uint64_t> d_origins;
@@ -428,9 +425,8 @@ struct Tuner {
     DevBuf<uint16_t> d_codes, d_q;
     DevBuf<float> d_outval;
     DevBuf<QSeg> d_segs;
-    std::vector<double> h_loss;
-
-    ~Tuner() {
+    DevBuf<float> d_feat;
+    void release() {
         d_cands.release();
         d_items.release();
         d_origins.release();
@@ -439,7 +435,41 @@ struct Tuner {
         d_q.release();
         d_outval.release();
         d_segs.release();
+        d_feat.release();
     }
+};
+
+TuneBufs &tune_bufs(faz_gpu_ctx *ctx) {
+    if (!ctx->tune_cache) {
+        ctx->tune_cache = new TuneBufs();
+        ctx->tune_cache_free = [](void *p) {
+            TuneBufs *b = static_cast<TuneBufs *>(p);
+            b->release();
+            delete b;
+        };
+    }
+    return *static_cast<TuneBufs *>(ctx->tune_cache);
+}
+
+// One tuning run over the sample blocks at `origins` (one at a time per context).
+struct Tuner {
+    faz_gpu_ctx *ctx;
+    TunePlan pl;
+    std::vector<uint64_t> origins;  // field offsets of the sample blocks
+    const float *d_field = nullptr;
+    double rng = 0;
+    DevBuf<TuneCand> &d_cands;
+    DevBuf<TuneItem> &d_items;
+    DevBuf<uint64_t> &d_origins;
+    DevBuf<double> &d_loss;
+    DevBuf<uint16_t> &d_codes, &d_q;
+    DevBuf<float> &d_outval;
+    DevBuf<QSeg> &d_segs;
+    std::vector<double> h_loss;
+
+    Tuner(faz_gpu_ctx *c, TuneBufs *b)
+        : ctx(c), d_cands(b->d_cands), d_items(b->d_items), d_origins(b->d_origins), d_loss(b->d_loss),
+          d_codes(b->d_codes), d_q(b->d_q), d_outval(b->d_outval), d_segs(b->d_segs) {}
 
     // Runs every candidate on every block; h_loss[c * nblocks + b].
     int run(const std::vector<TuneCand> &cands, bool mode1) {
@@ -637,8 +667,7 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
     uint64_t fstr[3] = {0, 0, 0};
     fstr[N - 1] = 1;
     for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * in->dims[a + 1];
-    Tuner T;
-    T.ctx = ctx;
+    Tuner T(ctx, &tune_bufs(ctx));
     T.d_field = d_in;
     T.rng = in->rng;
     make_tune_plan(N, (uint32_t) sbs + 1, fstr, max_step, in->interp_block_size, T.pl);
@@ -847,12 +876,13 @@ int tune_qoz(faz_gpu_ctx *ctx, const float *d_in, const faz_tune_in *in, faz_tun
             if (in->learned_margin >= 0)
                 for (size_t c: order)
                     if ((int) ver.size() < kmax && lc[c].cost <= lc[order[0]].cost + in->learned_margin) ver.push_back(c);
-            // the predictor choice is verified when the two predicted costs are close
+            // the predictor choice is verified when the two predicted costs
+            // are close; a clear Lorenzo win needs no interpolation test
             const bool lor_close = lor >= 0 && in->learned_margin_lorenzo >= 0 &&
                                    std::fabs(lc[lor].cost - lc[order[0]].cost) <= in->learned_margin_lorenzo;
-            size_t pick = order[0];
-            if (lor >= 0 && lc[lor].cost < lc[order[0]].cost) pick = (size_t) lor;
-            if (ver.size() >= 2 || lor_close) {
+            const bool lor_better = lor >= 0 && lc[lor].cost < lc[order[0]].cost;
+            size_t pick = lor_better ? (size_t) lor : order[0];
+            if (!(lor_better && !lor_close) && (ver.size() >= 2 || lor_close)) {
                 // the closest candidates by real CompressTests (interpolation
                 // candidates in one launch) and the lowest real cost among them
                 out->learned_path = 1;
@@ -976,8 +1006,7 @@ int compress_test_at(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims, c
     uint64_t fstr[3] = {0, 0, 0};
     fstr[N - 1] = 1;
     for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * fdims[a + 1];
-    Tuner T;
-    T.ctx = ctx;
+    Tuner T(ctx, &tune_bufs(ctx));
     T.d_field = d_in;
     T.rng = rng;
     make_tune_plan(N, (uint32_t) sbs + 1, fstr, p->max_step, p->block_size, T.pl);
@@ -1024,8 +1053,7 @@ int compress_test_batch(faz_gpu_ctx *ctx, const float *d_in, const size_t *fdims
     uint64_t fstr[3] = {0, 0, 0};
     fstr[N - 1] = 1;
     for (int a = N - 2; a >= 0; a--) fstr[a] = fstr[a + 1] * fdims[a + 1];
-    Tuner T;
-    T.ctx = ctx;
+    Tuner T(ctx, &tune_bufs(ctx));
     T.d_field = d_in;
     T.rng = rng;
     make_tune_plan(N, (uint32_t) sbs + 1, fstr, ps[0].max_step, ps[0].block_size, T.pl);
@@ -1098,20 +1126,35 @@ namespace faz {
 namespace {
 
 constexpr int kFeat = 40;
-constexpr int kFeatThreads = 256;
+constexpr int kFeatThreads = 1024;
+constexpr int kFeatWaves = kFeatThreads / 64;
 constexpr int kFeatAcc = 36;  // double accumulators per thread
 
+// Sum over the wave in a fixed butterfly pattern (the same for every run).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
 // Statistics of one sample block u = (v - vmin) / rng (see faz_gpu.h for the
-// list).  Per-thread partial sums over a fixed point assignment, added in
-// thread order: the result does not depend on scheduling.
+// list).  Per-thread partial sums over a fixed point assignment, reduced in a
+// fixed order (butterflies within each wave, then the waves in order): the
+// result does not depend on scheduling.
 __global__ void __launch_bounds__(kFeatThreads) block_features_kernel(TuneGeom G, const uint64_t *__restrict__ origins,
                                                                       const float *__restrict__ field, float vmin,
                                                                       float rng, float eb_norm,
                                                                       float *__restrict__ out) {
     __shared__ float s_u[kTuneMaxPoints];
-    __shared__ double s_part[kFeatThreads];
-    __shared__ float s_max[kFeatThreads];
+    __shared__ double s_wpart[kFeatAcc + 1][kFeatWaves];
+    __shared__ float s_wmax[5][kFeatWaves];
     __shared__ double s_res[kFeatAcc + 8];
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint32_t tid = threadIdx.x;
     const uint64_t org = origins[blockIdx.x];
     const int N = G.N;
@@ -1173,29 +1216,32 @@ __global__ void __launch_bounds__(kFeatThreads) block_features_kernel(TuneGeom G
             if (c[a] + 2 < dd[a]) acc[29 + a] += (double) u * s_u[q + 2 * st[a]];
         }
     }
-    // deterministic reductions: partials in LDS, summed by thread 0 in order
+    // deterministic reductions: each wave's butterfly sum, then the waves in order
 #pragma unroll
     for (int i = 0; i < kFeatAcc; i++) {
-        s_part[tid] = acc[i];
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0;
-            for (int k = 0; k < kFeatThreads; k++) t += s_part[k];
-            s_res[i] = t;
-        }
-        __syncthreads();
+        const double w = wave_sum(acc[i]);
+        if (lane == 0) s_wpart[i][wave] = w;
     }
-    const float mxs[6] = {mn, mx, lmax[0], lmax[1], lmax[2], 0.f};
-    for (int i = 0; i < 5; i++) {
-        s_max[tid] = i == 0 ? -mxs[0] : mxs[i];
-        __syncthreads();
-        if (tid == 0) {
-            float t = s_max[0];
-            for (int k = 1; k < kFeatThreads; k++) t = fmaxf(t, s_max[k]);
-            s_res[kFeatAcc + i] = i == 0 ? -t : t;
+    {
+        const float mxs[5] = {-mn, mx, lmax[0], lmax[1], lmax[2]};
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const float w = wave_max(mxs[i]);
+            if (lane == 0) s_wmax[i][wave] = w;
         }
-        __syncthreads();
     }
+    __syncthreads();
+    if (tid < kFeatAcc) {
+        double t = 0;
+        for (int k = 0; k < kFeatWaves; k++) t += s_wpart[tid][k];
+        s_res[tid] = t;
+    } else if (tid >= 64 && tid < 69) {
+        const int i = tid - 64;
+        float t = s_wmax[i][0];
+        for (int k = 1; k < kFeatWaves; k++) t = fmaxf(t, s_wmax[i][k]);
+        s_res[kFeatAcc + i] = i == 0 ? -t : t;
+    }
+    __syncthreads();
     // second pass: share of |L1| above twice its RMS
     uint64_t n_int1 = 1;
     for (int a = 0; a < N; a++) n_int1 *= dd[a] - 2;
@@ -1213,11 +1259,14 @@ __global__ void __launch_bounds__(kFeatThreads) block_features_kernel(TuneGeom G
         }
         if (interior && fabsf(L) > 2.0 * rms1) big += 1.0;
     }
-    s_part[tid] = big;
+    {
+        const double w = wave_sum(big);
+        if (lane == 0) s_wpart[kFeatAcc][wave] = w;
+    }
     __syncthreads();
     if (tid == 0) {
         double t = 0;
-        for (int k = 0; k < kFeatThreads; k++) t += s_part[k];
+        for (int k = 0; k < kFeatWaves; k++) t += s_wpart[kFeatAcc][k];
         const double n = (double) npts;
         const double mean = s_res[0] / n, var = fmax(s_res[1] / n - mean * mean, 0.0);
         float *o = out + (uint64_t) blockIdx.x * kFeat;
@@ -1272,7 +1321,7 @@ namespace faz {
 
 int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origins, size_t nblocks,
                    const float *d_field, float vmin, float rng, float eb_norm, std::vector<float> &feats) {
-    DevBuf<float> d_out;
+    DevBuf<float> &d_out = tune_bufs(ctx).d_feat;
     HIPCHECK(d_out.ensure(nblocks * kFeat));
     hipLaunchKernelGGL(block_features_kernel, dim3((uint32_t) nblocks), dim3(kFeatThreads), 0, ctx->stream, G,
                        d_origins, d_field, vmin, rng, eb_norm, d_out.p);
@@ -1280,7 +1329,6 @@ int block_features(faz_gpu_ctx *ctx, const TuneGeom &G, const uint64_t *d_origin
     feats.resize(nblocks * kFeat);
     HIPCHECK(hipMemcpyAsync(feats.data(), d_out.p, feats.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-    d_out.release();
     return FAZ_OK;
 }
 

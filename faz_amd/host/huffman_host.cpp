@@ -316,26 +316,42 @@ void encode_huffman(const HuffTable &tab, const int32_t *s, size_t n, std::vecto
     out.resize(at + 8 + nbytes, 0);
     memcpy(out.data() + at, &nbytes, 8);
     uint8_t *w = out.data() + at + 8;
-    uint64_t acc = 0;  // pending bits, MSB-aligned
-    int nacc = 0;
+    // pending bits MSB-aligned in acc, fewer than 32 of them between codes;
+    // codewords (MSB-aligned, zero below their length) go in at most 32
+    // bits at a time and whole 32-bit groups leave big-endian
+    uint64_t acc = 0;
+    uint32_t nacc = 0;
+    auto flush32 = [&]() {
+        const uint32_t hi = (uint32_t) (acc >> 32);
+        w[0] = (uint8_t) (hi >> 24);
+        w[1] = (uint8_t) (hi >> 16);
+        w[2] = (uint8_t) (hi >> 8);
+        w[3] = (uint8_t) hi;
+        w += 4;
+        acc <<= 32;
+        nacc -= 32;
+    };
     for (size_t i = 0; i < n; i++) {
         const int st = s[i] - tab.offset;
-        int l = tab.len[st];
-        uint64_t c = tab.code[st];
-        while (l > 0) {
-            const int take = std::min(l, 64 - nacc);
-            acc |= take == 64 ? c : (c >> nacc) & (~0ull << (64 - nacc - take));
-            if (take < 64) c <<= take;
-            l -= take;
-            nacc += take;
-            while (nacc >= 8) {
-                *w++ = (uint8_t) (acc >> 56);
-                acc <<= 8;
-                nacc -= 8;
-            }
+        const uint32_t l = tab.len[st];
+        const uint64_t c = tab.code[st];
+        if (l <= 32) {
+            acc |= c >> nacc;
+            nacc += l;
+        } else {
+            acc |= (c & 0xFFFFFFFF00000000ull) >> nacc;
+            nacc += 32;
+            flush32();
+            acc |= (c << 32) >> nacc;
+            nacc += l - 32;
         }
+        if (nacc >= 32) flush32();
     }
-    if (nacc) *w++ = (uint8_t) (acc >> 56);
+    while (nacc > 0) {
+        *w++ = (uint8_t) (acc >> 56);
+        acc <<= 8;
+        nacc = nacc > 8 ? nacc - 8 : 0;
+    }
 }
 
 size_t decode_huffman(const uint8_t *buf, size_t n, std::vector<int32_t> &out, size_t num) {

@@ -281,46 +281,57 @@ def reachable(X, max_interp_level=5):
     return ok
 
 
-def calibrate(net, mu, sigma, X, T, G, kinds_scale, quantile=0.9):
-    """Gates from held-out tuples: lo/hi = the standardised range of the
-    rows widened by one unit, margin = the `quantile` of the predicted cost
-    gap between the predicted best and the truly best reachable base-bound
-    candidate of each held-out field."""
+def calibrate(net, mu, sigma, X, T, G, kinds_scale, quantile=0.9, tol=0.01, log=None, quantile_lorenzo=0.8):
+    """Gates from held-out tuples.
+
+    lo/hi: the standardised range of the rows widened by one unit.
+    margin: per held-out field, the predicted cost gap between the predicted
+    best and the truly best reachable base-bound candidate, counted only
+    where taking the predicted best would cost more than `tol` of the best
+    bitrate (elsewhere the prediction alone is good enough, gap 0); the
+    margin is the `quantile` of these gaps.
+    margin_lorenzo: the predicted cost difference between the Lorenzo
+    candidate and the best interpolation candidate, the `quantile_lorenzo`
+    (x 1.1) of its size over the held-out fields where its sign was wrong by
+    more than `tol` of the best bitrate.  Cost = bitrate - PSNR / 6.02, bits
+    per value at equal PSNR."""
     Z = (X - mu) / sigma
     lo = Z.min(0) - 1.0
     hi = Z.max(0) + 1.0
     y = predict(net, mu, sigma, X)
     reach = reachable(X)
-    gaps = []
+    base = np.abs(kinds_scale - 1.0) < 1e-9
+    lorm = X[:, 53] > 0.5
+    gaps, regrets = [], []
+    wrong, n_lor, n_lor_wrong = [], 0, 0
     for g in np.unique(G):
-        m = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9) & reach
+        m = (G == g) & base & reach & ~lorm
         if m.sum() < 2:
             continue
-        # cost = bitrate - PSNR / 6.02 (the constant PSNR offset cancels)
+        # the PSNR offset is the same for every candidate of a field and cancels
         true = np.exp2(T[m, 0]) - 5.0 * T[m, 1] / 6.02
         pred = np.exp2(y[m, 0]) - 5.0 * y[m, 1] / 6.02
-        gaps.append(pred[np.argmin(true)] - pred.min())
+        br_best = float(np.exp2(T[m, 0])[np.argmin(true)])
+        regret = float(true[np.argmin(pred)] - true.min()) / br_best
+        regrets.append(regret)
+        gaps.append(float(pred[np.argmin(true)] - pred.min()) if regret > tol else 0.0)
+        ml = (G == g) & base & lorm
+        if ml.any():
+            n_lor += 1
+            t_l = float((np.exp2(T[ml, 0]) - 5.0 * T[ml, 1] / 6.02).min())
+            p_l = float((np.exp2(y[ml, 0]) - 5.0 * y[ml, 1] / 6.02).min())
+            d_true, d_pred = t_l - true.min(), p_l - pred.min()
+            if np.sign(d_true) != np.sign(d_pred) and abs(d_true) > tol * br_best:
+                n_lor_wrong += 1
+                wrong.append(abs(d_pred))
     margin = float(np.quantile(np.array(gaps), quantile)) if gaps else -1.0
-    # the predictor choice: the predicted cost difference between the Lorenzo
-    # candidate and the best interpolation candidate of every held-out field;
-    # the margin covers every field where its sign was wrong
-    wrong, diffs = [], []
-    lorm = X[:, 53] > 0.5
-    for g in np.unique(G):
-        base = (G == g) & (np.abs(kinds_scale - 1.0) < 1e-9)
-        mi, ml = base & reach & ~lorm, base & lorm
-        if not mi.any() or not ml.any():
-            continue
-        t_i = np.exp2(T[mi, 0]) - 5.0 * T[mi, 1] / 6.02
-        p_i = np.exp2(y[mi, 0]) - 5.0 * y[mi, 1] / 6.02
-        t_l = float((np.exp2(T[ml, 0]) - 5.0 * T[ml, 1] / 6.02).min())
-        p_l = float((np.exp2(y[ml, 0]) - 5.0 * y[ml, 1] / 6.02).min())
-        d_true, d_pred = t_l - t_i.min(), p_l - p_i.min()
-        diffs.append(abs(d_pred - d_true))
-        if np.sign(d_true) != np.sign(d_pred):
-            wrong.append(abs(d_pred))
-    margin_lor = 1.1 * max(wrong) if wrong else (float(np.quantile(diffs, quantile)) if diffs else -1.0)
-    return dict(lo=lo.astype(np.float32), hi=hi.astype(np.float32), margin=margin, verify=4,
+    margin_lor = 1.1 * float(np.quantile(np.array(wrong), quantile_lorenzo)) if wrong else 0.0
+    if log:
+        r = np.array(regrets)
+        log(f"calibration over {len(gaps)} fields: regret > {tol:.0%} in {(r > tol).mean():.1%} "
+            f"(p50 {np.median(r):.4f}, p90 {np.quantile(r, 0.9):.4f}); predictor choice wrong by more than "
+            f"{tol:.0%} in {n_lor_wrong} of {n_lor}")
+    return dict(lo=lo.astype(np.float32), hi=hi.astype(np.float32), margin=margin, verify=3,
                 margin_lorenzo=float(margin_lor)), np.array(gaps)
 
 

@@ -36,6 +36,11 @@ def main():
     ap.add_argument("--recalibrate", default="", help="keep this model's network; rebuild only the validation "
                                                       "fields and its gates")
     ap.add_argument("--quantile", type=float, default=0.9)
+    ap.add_argument("--tol", type=float, default=0.01,
+                    help="bitrate loss (relative) below which a misprediction needs no verification")
+    ap.add_argument("--quantile-lorenzo", type=float, default=0.8,
+                    help="share of the held-out predictor-choice errors the Lorenzo margin covers")
+    ap.add_argument("--dump-val", default="", help="also write the validation tuples (X, T, G) to this .npz")
     a = ap.parse_args()
 
     def log(m):
@@ -49,11 +54,14 @@ def main():
         net, mu, sigma = learned.load_model(e, a.recalibrate)
         old = learned.load_gates(a.recalibrate)
         scale = np.exp2(X[:, 48].astype(np.float64))
-        gates, gaps = learned.calibrate(net, mu, sigma, X, T, G, scale, a.quantile)
+        if a.dump_val:
+            np.savez_compressed(a.dump_val, X=X, T=T, G=G)
+        gates, gaps = learned.calibrate(net, mu, sigma, X, T, G, scale, a.quantile, a.tol, log, a.quantile_lorenzo)
         gates["lo"], gates["hi"] = old["lo"], old["hi"]
         z = np.load(a.recalibrate, allow_pickle=False)
         meta = ast.literal_eval(str(z["meta"])) if str(z["meta"]).startswith("{") else {}
-        meta["recalibrated"] = dict(val_fields=a.val_fields, quantile=a.quantile)
+        meta["recalibrated"] = dict(val_fields=a.val_fields, quantile=a.quantile, tol=a.tol,
+                                    quantile_lorenzo=a.quantile_lorenzo)
         learned.save_model(a.out, net, mu, sigma, meta, gates)
         log(f"recalibrated: margin {gates['margin']:.4f} bits/value (gap p50 {np.median(gaps):.4f}, "
             f"p{int(100 * a.quantile)} {gates['margin']:.4f}); Lorenzo margin {gates['margin_lorenzo']:.4f}")
@@ -72,7 +80,9 @@ def main():
     # kinds: eb scale of every tuple (input 48 = log2 scale)
     scale = np.exp2(X[:, 48].astype(np.float64))
     gates, gaps = learned.calibrate(net, mu, sigma, X[~train_m], T[~train_m], G[~train_m], scale[~train_m],
-                                    a.quantile)
+                                    a.quantile, a.tol, log, a.quantile_lorenzo)
+    if a.dump_val:
+        np.savez_compressed(a.dump_val, X=X[~train_m], T=T[~train_m], G=G[~train_m])
     Ztrain = (X[train_m] - mu) / sigma
     gates["lo"] = (Ztrain.min(0) - 1.0).astype(np.float32)
     gates["hi"] = (Ztrain.max(0) + 1.0).astype(np.float32)

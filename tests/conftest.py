@@ -21,18 +21,18 @@ def engine():
     e.close()
 
 
-def _engine_with(kernel):
+def _engine_with(kernel, var="FAZ_INTERP_KERNEL"):
     import faz_amd
 
-    old = os.environ.get("FAZ_INTERP_KERNEL")
-    os.environ["FAZ_INTERP_KERNEL"] = kernel
+    old = os.environ.get(var)
+    os.environ[var] = kernel
     try:
         return faz_amd.Engine(0)
     finally:
         if old is None:
-            del os.environ["FAZ_INTERP_KERNEL"]
+            del os.environ[var]
         else:
-            os.environ["FAZ_INTERP_KERNEL"] = old
+            os.environ[var] = old
 
 
 @pytest.fixture(scope="session")
@@ -47,6 +47,14 @@ def engine_pass():
 def engine_level():
     """A context on the per-block level kernel (no whole-tile fast path)."""
     e = _engine_with("level")
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_onepass():
+    """A context on the one-pass (decoupled look-back) Huffman encoder."""
+    e = _engine_with("onepass", "FAZ_ENCODE")
     yield e
     e.close()
 

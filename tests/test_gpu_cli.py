@@ -4,8 +4,11 @@ against include/faz/sz.hpp, run as processes on the GPU (needs an MI355X).
 Bar: `-z` in the default mode (the SZ3 tuning mode, which the CLI runs when
 -F is not given) writes the oracle's stream byte for byte; `-q` streams and
 ALGO_INTERP `-c` config streams decode (on the oracle and through `-x`) to
-the oracle's reconstruction with the same parameters; `-a` prints the error
-statistics; `-F 1` (FZ, wavelets) fails loudly.
+the oracle's reconstruction with the same parameters; `-F 0` writes the
+default mode's stream without the notice; ALGO_LORENZO_REG `-c` config
+streams (2-D: the general frontend with the Config's 1-layer Lorenzo +
+RegressionPredictor; 3-D: the fast frontend) equal the oracle's byte for
+byte; `-a` prints the error statistics; `-F 1` (FZ, wavelets) fails loudly.
 """
 import os
 import subprocess
@@ -98,6 +101,40 @@ def test_cli_config_file_algo_interp(tmp_path, mode, bound):
     _run(FAZ, ["-f", "-s", str(z), "-x", str(out), "-t"] + _dims(shape), tmp_path)
     txt = np.loadtxt(out, dtype=np.float64)
     assert txt.size == x.size and np.abs(txt - want.ravel().astype(np.float64)).max() <= 1e-9 * max(1.0, eb)
+
+
+def test_cli_sz3_mode_flag(tmp_path):
+    shape = (64, 80, 72)
+    x = _field(shape, 3)
+    f, z = tmp_path / "f.dat", tmp_path / "f.faz"
+    x.tofile(f)
+    r = _run(FAZ, ["-f", "-F", "0", "-i", str(f), "-z", str(z)] + _dims(shape) + ["-M", "REL", "1e-3"], tmp_path)
+    assert "FZ mode" not in r.stderr
+    ref, _ = oracle.sz3_compress_stream(x, 1e-3, rel=True)
+    assert z.read_bytes() == ref
+
+
+@pytest.mark.parametrize("shape", [(300, 410), (60, 70, 80)])
+def test_cli_config_file_algo_lorenzo_reg(tmp_path, shape):
+    x = _field(shape, 7)
+    i, j = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    ripple = 0.02 * np.sin(2 * np.pi * i / 9.0) * np.sin(2 * np.pi * j / 7.0)
+    x = (x + (ripple if len(shape) == 2 else ripple[..., None]) * (np.arange(shape[-1]) > shape[-1] // 2)).astype(np.float32)
+    f, z, out, cfg = tmp_path / "f.dat", tmp_path / "f.faz", tmp_path / "f.out", tmp_path / "faz.config"
+    x.tofile(f)
+    cfg.write_text("[GlobalSettings]\nCmprAlgo = ALGO_LORENZO_REG\n")
+    _run(FAZ, ["-f", "-c", str(cfg), "-i", str(f), "-z", str(z)] + _dims(shape) + ["-M", "REL", "1e-3"], tmp_path)
+    s = z.read_bytes()
+    if len(shape) == 2:  # Config defaults for N = 2: blockSize 16, lorenzo + regression
+        p = oracle.make_gf_params(shape, 0.0, block_size=16, lorenzo=True, lorenzo2=False, regression=True)
+        ref = oracle.gf_compress_stream(x, p, 1e-3, rel=True)
+    else:
+        p = oracle.make_lr_params(shape, 0.0, block_size=6, pred_dim=3, lorenzo=True, lorenzo2=False, regression=True)
+        ref = oracle.lr_compress_stream(x, p, 1e-3, rel=True)
+    assert s == ref
+    _run(FAZ, ["-f", "-s", str(z), "-x", str(out)] + _dims(shape), tmp_path)
+    assert np.array_equal(np.fromfile(out, np.float32).view(np.uint32),
+                          oracle.decompress_stream(s, x.size).view(np.uint32))
 
 
 def test_cli_fz_mode_fails_loudly(tmp_path):

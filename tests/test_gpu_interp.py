@@ -286,3 +286,20 @@ def test_full_size_kernels_agree(engine, engine_level, engine_pass):
             del b
     del x
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("rel", [1e-3, 1e-5])
+def test_one_pass_encoder_agrees(engine, engine_onepass, rel):
+    """The one-pass Huffman encoder (decoupled look-back over the chunks)
+    writes the same engine buffer as the sizes + scan + pack encoder, at
+    512^3 and on a small field of a few chunks."""
+    import faz_amd
+    for shape, seed in (((512, 512, 512), 2), ((40, 50, 60), 3)):
+        x = _grf_512(seed=seed) if shape[0] == 512 else _dev(np.cumsum(np.random.default_rng(seed).standard_normal(shape), 2))
+        mn, mx = engine.range(x)
+        eb = rel * float(np.float32(mx - mn))
+        p = faz_amd.make_params(shape, eb, algo=1, max_step=32 if shape[0] == 512 else 16)
+        print(f"[gpu] one-pass vs two-pass encoder {shape} rel={rel}", flush=True)
+        assert engine_onepass.interp_raw(x, p) == engine.interp_raw(x, p)
+        del x
+    torch.cuda.empty_cache()

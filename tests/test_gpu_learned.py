@@ -143,7 +143,7 @@ def test_gate_out_of_range_falls_back_to_the_exhaustive_search(engine, model):
     assert [(t["alpha"], t["beta"], t["eb"], t["bytes"]) for t in le["tests"]] == \
            [(t["alpha"], t["beta"], t["eb"], t["bytes"]) for t in ex["tests"]]
     print("[gpu] learned tuner verifying every reachable candidate", flush=True)
-    wide = dict(gates, margin=1e9, verify=64)
+    wide = dict(gates, margin=1e9, verify=64, margin_lorenzo=1e9)
     lv = learned.tune_learned(engine, x, shape, rel * rng, rel, rng, net, mu, sigma, wide)
     assert lv["learned_path"] == 1 and lv["n_verified"] >= 9
     real = [t for t in lv["tests"] if t["bytes"] > 0]
