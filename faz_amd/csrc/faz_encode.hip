@@ -7,12 +7,14 @@
 // Workgroups are persistent over chunks of kEncChunk codes, so the code table
 // is staged in LDS once per workgroup: the window of at most kTabLds entries
 // that holds every symbol but 0 (the outlier code, far from the others, kept
-// in a register).  Tables whose symbols spread wider read global memory.  Every code is placed on its own: its
-// bit position is the chunk offset plus a workgroup prefix of lengths, and its
-// bits are OR-ed into at most three words of an LDS bit buffer, with no
-// sequential accumulator per thread.  Interior words then go out with plain
-// stores; the first and last word of a chunk, shared with its neighbours, with
-// atomic ORs into the zeroed output.
+// in a register).  Tables whose symbols spread wider read global memory.  A
+// thread's 16 consecutive codes form one run of bits starting at the chunk
+// offset plus a workgroup prefix of lengths; the thread assembles the run in
+// a 64-bit register and writes it to an LDS bit buffer, words wholly inside
+// the run with plain stores and its first and last word, which it may share
+// with the neighbouring threads, with atomic ORs.  The chunk's words then go
+// out with plain stores; the first and last word of a chunk, shared with the
+// neighbouring chunks, with atomic ORs into the zeroed output.
 #include "faz_internal.h"
 
 namespace faz {
@@ -64,6 +66,7 @@ __device__ __forceinline__ Codes16 fetch_chunk(const uint16_t *__restrict__ code
 struct TabWindow {
     uint32_t win, nwin;
     uint64_t e0;
+    uint32_t max_len;  // longest codeword of the table
 };
 
 template<bool LDS_TAB>
@@ -137,6 +140,33 @@ __device__ __forceinline__ void put_code(uint64_t c, uint32_t L, uint32_t pos, W
     if (sh + L > 64) word_or(w + 2, (uint32_t) (hi << (32 - sh)));
 }
 
+// The thread's run of codes e[0..15] (code << 8 | length, lengths <= 32) at
+// bit pos of the LDS bit buffer: MSB-first, assembled in a register; words
+// inside the run are stored, the run's first and last word OR-ed.
+__device__ __forceinline__ void put_run(const uint64_t *e, uint32_t pos, uint32_t *s_words) {
+    uint32_t w = pos >> 5, nacc = pos & 31;
+    uint64_t acc = 0;  // pending bits MSB-aligned, nacc of them (the first pos & 31 are not the thread's)
+    bool first = true;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const uint32_t L = (uint32_t) e[k] & 0xFF;
+        if (L) {
+            acc |= ((e[k] >> 8) << (64 - L)) >> nacc;
+            nacc += L;
+        }
+        if (nacc >= 32) {
+            const uint32_t word = (uint32_t) (acc >> 32);
+            if (first) atomicOr(&s_words[w], word);
+            else s_words[w] = word;
+            first = false;
+            w++;
+            acc <<= 32;
+            nacc -= 32;
+        }
+    }
+    if (nacc > (first ? (pos & 31) : 0u)) atomicOr(&s_words[w], (uint32_t) (acc >> 32));
+}
+
 // Bits of chunk c, which starts at bit g0 of the stream.
 template<bool LDS_TAB, bool FULL>
 __device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_t end, uint64_t g0,
@@ -157,15 +187,10 @@ __device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_
     const uint32_t nw = (head + total + 31) / 32;
     const uint64_t gw = g0 >> 5;
     uint32_t pos = head + before;
-    if (nw <= kPackWords) {
+    if (nw <= kPackWords && tw.max_len <= 32) {
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
         __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const uint32_t L = (uint32_t) e[k] & 0xFF;
-            if (L) put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&s_words[w], bits); });
-            pos += L;
-        }
+        put_run(e, pos, s_words);
         __syncthreads();
         const bool open_tail = ((head + total) & 31) != 0;
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
@@ -364,13 +389,7 @@ __global__ void __launch_bounds__(kEncThreads) encode_onepass_kernel(
         const uint64_t gw = g0 >> 5;
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
         __syncthreads();
-        uint32_t pos = head + before;
-#pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const uint32_t L = (uint32_t) e[k] & 0xFF;
-            if (L) put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&s_words[w], bits); });
-            pos += L;
-        }
+        put_run(e, head + before, s_words);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
             const uint32_t val = bswap32(s_words[i]);
@@ -411,7 +430,7 @@ void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_ta
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
-    const TabWindow tw{w.win, w.nwin, w.e0};
+    const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     if (w.nwin && w.nwin <= kTabLds)
         hipLaunchKernelGGL(encode_sizes_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
                            d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_bits, aligned);
@@ -425,7 +444,7 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
-    const TabWindow tw{w.win, w.nwin, w.e0};
+    const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     if (w.nwin && w.nwin <= kTabLds)
         hipLaunchKernelGGL(encode_pack_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
                            d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_off, d_words, aligned);
@@ -443,7 +462,7 @@ void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
-    const TabWindow tw{w.win, w.nwin, w.e0};
+    const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     const dim3 grid(encode_grid(nchunks));
     if (w.nwin && w.nwin <= kTabLds)
         hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), w.nwin * 8, st, d_codes, (uint32_t) n,

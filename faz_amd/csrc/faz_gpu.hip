@@ -884,6 +884,7 @@ static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow 
     };
     ctx->h_tab.resize((size_t) ns);
     ew = EncodeWindow{0, 0, 0};
+    ew.max_len = maxlen;
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
         for (int64_t k = lo; k <= hi; k++) ctx->h_tab[k] = pack(k);
         ew.win = (uint32_t) lo;
@@ -958,16 +959,19 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     // one packed entry per symbol: codeword right-aligned above an 8-bit length
     ctx->h_tab.resize((size_t) t->n_states);
     int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
+    uint32_t maxlen = 0;
     for (int32_t i = 0; i < t->n_states; i++) {
         const uint32_t L = t->len[i];
         if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
         ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
+        maxlen = std::max(maxlen, L);
         if (L && i + t->offset != 0) {
             if (lo < 0) lo = i;
             hi = i;
         }
     }
     EncodeWindow ew{0, 0, 0};
+    ew.max_len = maxlen;
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
         ew.win = (uint32_t) lo;
         ew.nwin = (uint32_t) (hi - lo + 1);

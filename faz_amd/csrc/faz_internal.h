@@ -164,6 +164,7 @@ void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outv
 struct EncodeWindow {
     uint32_t win, nwin;
     uint64_t e0;
+    uint32_t max_len = 56;  // longest codeword (runs of codes assemble in registers up to 32)
 };
 uint32_t encode_lds_entries();
 void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,

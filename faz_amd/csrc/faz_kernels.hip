@@ -405,22 +405,51 @@ constexpr int kWin = kHistWin;  // LDS bins around the quantizer radius
 constexpr int kHot = 16;    // register-counted symbols [hot0, hot0 + 16)
 static_assert(kHistPerThread <= 255, "8-bit register counters");
 
+// The thread's 8 groups of 8 consecutive codes of chunk c (16-byte loads
+// for a full chunk of aligned codes, element loads past the end otherwise).
+constexpr int kHistGroups = kHistPerThread / 8;
+__device__ __forceinline__ void hist_load(const uint16_t *__restrict__ codes, uint32_t c, uint32_t n, int aligned,
+                                          uint4 *q) {
+    const uint32_t base = c * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    const bool full = aligned && end - base == kHistChunk;
+#pragma unroll
+    for (int g = 0; g < kHistGroups; g++) {
+        const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
+        if (full) {
+            q[g] = *reinterpret_cast<const uint4 *>(codes + j);
+        } else {
+            uint32_t t[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) t[k] = j + k < end ? codes[j + k] : 0;
+            q[g] = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
+        }
+    }
+}
+
 // Persistent: a workgroup keeps its LDS window and register counters across
 // the chunks it takes (in increasing order), so the window is cleared and
 // flushed to the global histogram once per workgroup, not once per chunk.
+// The next chunk's codes are in flight while the current one is counted, and
+// the per-chunk outlier counter alternates between two slots, so a chunk
+// costs one barrier.
 __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                            uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
                                                            uint32_t *__restrict__ chunk_zeros, int lo_,
                                                            int hot0_, int aligned, uint32_t *__restrict__ outside) {
     __shared__ uint32_t s_cnt[kWin];
     __shared__ uint32_t s_first[kWin];  // first global position per window symbol
-    __shared__ uint32_t s_zero, s_zero_first;
+    __shared__ uint32_t s_zero[2], s_zero_first;
     const uint32_t lo = (uint32_t) lo_, hot0 = (uint32_t) hot0_;
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
         s_cnt[i] = 0;
         s_first[i] = 0xFFFFFFFFu;
     }
-    if (threadIdx.x == 0) s_zero_first = 0xFFFFFFFFu;
+    if (threadIdx.x == 0) {
+        s_zero_first = 0xFFFFFFFFu;
+        s_zero[0] = s_zero[1] = 0;
+    }
+    __syncthreads();
     uint32_t seen = 0;  // hot symbols whose first position this thread already offered
     uint32_t zeros_total = 0;
     const int lane = threadIdx.x % 64;
@@ -429,29 +458,15 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
     // chunks (a thread adds at most kHistPerThread codes per chunk)
     constexpr uint32_t kFoldChunks = 255 / kHistPerThread;
     uint64_t hot_a = 0, hot_b = 0;
-    uint32_t unfolded = 0;
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        if (threadIdx.x == 0) s_zero = 0;
-        __syncthreads();
+    uint32_t unfolded = 0, slot = 0;
+    constexpr int kGroups = kHistGroups;
+    uint4 q[kGroups], qn[kGroups];
+    if (blockIdx.x < nchunks) hist_load(codes, blockIdx.x, n, aligned, q);
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x, slot ^= 1u) {
         const uint32_t base = c * kHistChunk;
         const uint32_t end = min(base + kHistChunk, n);
-        // the thread's 8 groups of 8 consecutive codes (16-byte loads), all
-        // issued before any is used
-        constexpr int kGroups = kHistPerThread / 8;
-        uint4 q[kGroups];
-        const bool full = aligned && end - base == kHistChunk;
-#pragma unroll
-        for (int g = 0; g < kGroups; g++) {
-            const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
-            if (full) {
-                q[g] = *reinterpret_cast<const uint4 *>(codes + j);
-            } else {
-                uint32_t t[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) t[k] = j + k < end ? codes[j + k] : 0;
-                q[g] = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
-            }
-        }
+        uint32_t *sz = &s_zero[slot];
+        if (c + gridDim.x < nchunks) hist_load(codes, c + gridDim.x, n, aligned, qn);
 #pragma unroll
         for (int g = 0; g < kGroups; g++) {
             const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
@@ -472,7 +487,7 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
                         if (s_first[w] > gi) atomicMin(&s_first[w], gi);
                     }
                 } else if (v[k] == 0) {
-                    atomicAdd(&s_zero, 1u);
+                    atomicAdd(sz, 1u);
                     if (s_zero_first > gi) atomicMin(&s_zero_first, gi);
                 } else {
                     const uint32_t w = (uint32_t) v[k] - lo;
@@ -502,9 +517,13 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            chunk_zeros[c] = s_zero;
-            zeros_total += s_zero;
+            // this slot is next used two chunks on, after one more barrier
+            chunk_zeros[c] = *sz;
+            zeros_total += *sz;
+            *sz = 0;
         }
+#pragma unroll
+        for (int g = 0; g < kGroups; g++) q[g] = qn[g];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {

@@ -103,15 +103,18 @@ faz_status faz_gpu_huffman_encode_noalloc(faz_gpu_ctx *ctx, const uint16_t *d_co
     const uint32_t ns = (uint32_t) t->n_states;
     // window of the present symbols other than 0 (read from the caller's lengths)
     int64_t lo = -1, hi = -1;
+    uint32_t maxlen = 0;
     for (uint32_t i = 0; i < ns; i++) {
         const uint32_t L = t->len[i];
         if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
+        maxlen = std::max(maxlen, L);
         if (L && (int64_t) i + t->offset != 0) {
             if (lo < 0) lo = i;
             hi = i;
         }
     }
     EncodeWindow ew{0, 0, 0};
+    ew.max_len = maxlen;
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
         ew.win = (uint32_t) lo;
         ew.nwin = (uint32_t) (hi - lo + 1);

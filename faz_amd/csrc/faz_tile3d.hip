@@ -41,6 +41,12 @@ constexpr int kEdgeSlots = (kEdge + kThreads - 1) / kThreads;
 #define FAZ_TILE_LANES 4
 #endif
 constexpr int kLanesPerLine = FAZ_TILE_LANES;  // lanes sharing one line of 16 odd positions
+// rounds of a pass unrolled together (more loads in flight, more registers)
+#ifndef FAZ_TILE_UNROLL
+#define FAZ_TILE_UNROLL 1
+#endif
+#define FAZ_PRAGMA(x) _Pragma(#x)
+#define FAZ_UNROLL(n) FAZ_PRAGMA(unroll n)
 constexpr int kPerLane = 16 / kLanesPerLine;  // odd positions per lane
 constexpr int kLinesPerRound = kThreads / kLanesPerLine;
 
@@ -296,7 +302,7 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
     // which read a position inside it instead and never use the value
     const int olo = first_lane ? 0 : -2 * ESA;
     const int ohi = last_lane ? 2 * kPerLane * ESA : (2 * kPerLane + 2) * ESA;
-#pragma unroll 1
+    FAZ_UNROLL(FAZ_TILE_UNROLL)
     for (int r = 0; r < ROUNDS; r++) {
         const uint32_t line = line0 + (uint32_t) kLinesPerRound * r;
         const uint32_t u = line / NW, w = line - u * NW;

@@ -89,13 +89,17 @@ struct Node {
 
 // HuffmanEncoder's 1-based binary heap (qinsert/qremove, HUF:455-485).  The
 // entries carry their node's frequency, so the sift loops compare without
-// reaching into the node pool; the operations are the reference's.
+// reaching into the node pool; the operations are the reference's.  The
+// sift-down moves a hole instead of swapping (the same final array) and
+// picks the smaller child arithmetically: that choice is data-dependent and
+// mispredicts as a branch (about half the cost of a build of a few thousand
+// symbols).
 class Heap {
 public:
-    explicit Heap(size_t cap) { q_.resize(cap + 2); }
+    explicit Heap(size_t cap) { q_.resize(cap + 3); }
     void insert(int n, uint64_t f) {
         int i = end_++, j;
-        if ((int) q_.size() <= i) q_.resize(2 * (size_t) i + 2);
+        if ((int) q_.size() <= i + 1) q_.resize(2 * (size_t) i + 3);
         while ((j = i >> 1)) {
             if (q_[j].f <= f) break;
             q_[i] = q_[j];
@@ -104,19 +108,18 @@ public:
         q_[i] = {f, n};
     }
     int remove() {
-        int i = 1, l;
         const int n = q_[1].n;
         end_--;
-        q_[1] = q_[end_];
+        const E x = q_[end_];
+        int i = 1, l;
         while ((l = i << 1) < end_) {
-            if (l + 1 < end_ && q_[l + 1].f < q_[l].f) l++;
-            if (q_[i].f > q_[l].f) {
-                std::swap(q_[i], q_[l]);
-                i = l;
-            } else {
-                break;
-            }
+            // q_[end_] (the moved entry's old slot) stays readable: no bounds branch
+            l += (int) ((l + 1 < end_) & (q_[l + 1].f < q_[l].f));
+            if (!(x.f > q_[l].f)) break;
+            q_[i] = q_[l];
+            i = l;
         }
+        q_[i] = x;
         return n;
     }
     int size() const { return end_ - 1; }
@@ -233,6 +236,11 @@ int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of
         mn = std::min(mn, p.second);
         mx = std::max(mx, p.second);
     }
+    // a table of the same span keeps its arrays: only last build's codewords
+    // are cleared (the outlier code 0 makes the span tens of thousands of
+    // states wide for a few hundred present symbols)
+    const bool reuse = tab.offset == mn && tab.n_states == mx - mn + 1 && (int) tab.code.size() == tab.n_states &&
+                       (int) tab.len.size() == tab.n_states;
     tab.offset = mn;
     tab.n_states = mx - mn + 1;
     const int stateNum = mx - mn + 2;  // HUF:551
@@ -268,8 +276,16 @@ int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of
     const int root = heap.top();
 
     // codes: left = 0, right = 1, MSB-aligned (build_code, HUF:493-521)
-    tab.code.assign(tab.n_states, 0);
-    tab.len.assign(tab.n_states, 0);
+    if (reuse) {
+        for (int32_t k: tab.used) {
+            tab.code[k] = 0;
+            tab.len[k] = 0;
+        }
+    } else {
+        tab.code.assign(tab.n_states, 0);
+        tab.len.assign(tab.n_states, 0);
+    }
+    tab.used.clear();
     struct Item { int node; int depth; uint64_t bits; };
     std::vector<Item> st;
     st.push_back({root, 0, 0});
@@ -281,6 +297,7 @@ int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of
             if (it.depth > 64) return -2;  // codes beyond 64 bits need > 2.7e13 symbols
             tab.code[nd.c] = it.depth ? it.bits << (64 - it.depth) : 0;
             tab.len[nd.c] = (uint8_t) it.depth;
+            tab.used.push_back(nd.c);
             continue;
         }
         if (it.depth >= 64) return -2;

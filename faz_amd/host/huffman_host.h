@@ -12,6 +12,7 @@ struct HuffTable {
     std::vector<uint64_t> code;    // MSB-aligned codewords by symbol - offset
     std::vector<uint8_t> len;      // codeword lengths
     std::vector<uint8_t> tree;     // HuffmanEncoder::save bytes
+    std::vector<int32_t> used;     // indices with a codeword (what a rebuild of the same span clears)
 };
 
 // Builds the reference Huffman tree from counts and first positions of
