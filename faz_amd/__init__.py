@@ -246,7 +246,7 @@ def make_params(dims, eb, algo=1, direction=0, alpha=-1.0, beta=-1.0, max_step=0
 EXPORTS = [
     "faz_gpu_ctx_create", "faz_gpu_ctx_destroy", "faz_gpu_last_error", "faz_gpu_ctx_device",
     "faz_gpu_range_f32", "faz_gpu_interp_compress_f32", "faz_gpu_huffman_hist", "faz_gpu_huffman_encode",
-    "faz_huffman_build", "faz_gpu_interp_decompress_raw", "faz_gpu_interp_raw_f32",
+    "faz_huffman_build", "faz_gpu_interp_decompress_raw", "faz_gpu_interp_decompress_dev", "faz_gpu_interp_raw_f32",
     "faz_sz_compress_f32", "faz_sz_decompress_f32", "faz_free", "faz_gpu_last_timings",
     "faz_gpu_compress_f32", "faz_gpu_fetch_raw", "faz_gpu_set_profiling", "faz_gpu_profile_read",
     "faz_gpu_profile_kernel",
@@ -339,6 +339,7 @@ def lib():
         L.faz_stream_info.argtypes = [vp, sz, P(ctypes.c_int), vp, P(ctypes.c_int)]
         L.faz_huffman_build.argtypes = [vp, vp, P(ctypes.c_int32), P(ctypes.c_int32), vp, vp, vp, sz, P(sz)]
         L.faz_gpu_interp_decompress_raw.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
+        L.faz_gpu_interp_decompress_dev.argtypes = [vp, vp, sz, ctypes.c_int, vp, sz]
         L.faz_gpu_huffman_decode.argtypes = [vp, vp, sz, sz, ctypes.c_uint32, vp, P(sz)]
         L.faz_gpu_interp_raw_f32.argtypes = [vp, vp, vp, P(InterpParams), vp, sz, P(sz)]
         L.faz_sz_compress_f32.argtypes = [vp, P(InterpParams), ctypes.c_int, ctypes.c_double, vp, ctypes.c_int,
@@ -611,6 +612,18 @@ class Engine:
         torch.cuda.synchronize()
         self._check(self._L.faz_gpu_interp_decompress_raw(self.h, _np_ptr(b), b.size, int(N), self._dev(out),
                                                           out.numel()), "decompress_raw")
+        return out
+
+    def decompress_dev(self, d_raw, N, out, nbytes=None):
+        """Decompression of an engine buffer held in device memory (a uint8
+        tensor on this context's GPU)."""
+        import torch
+        assert d_raw.is_cuda and d_raw.dtype == torch.uint8 and d_raw.is_contiguous()
+        n = d_raw.numel() if nbytes is None else int(nbytes)
+        assert 0 <= n <= d_raw.numel()
+        torch.cuda.synchronize()
+        self._check(self._L.faz_gpu_interp_decompress_dev(self.h, ctypes.c_void_p(d_raw.data_ptr()), n, int(N),
+                                                          self._dev(out), out.numel()), "decompress_dev")
         return out
 
     def sz_compress(self, data, params, eb, rel=True):

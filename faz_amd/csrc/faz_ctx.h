@@ -113,6 +113,8 @@ struct faz_gpu_ctx {
     // engine plan of the last parameter set (see cached_plan in faz_gpu.hip)
     std::shared_ptr<void> plan_cache;
     std::vector<uint8_t> plan_key;
+    // decoder of a device-resident buffer: its head and its Huffman tree on the host
+    std::vector<uint8_t> h_stage, h_stage2;
     hipEvent_t ev[8] = {};
     double timings[8] = {};
     // histogram of the codes last produced by interp_compress

@@ -587,10 +587,12 @@ extern "C" faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, si
 // HuffmanEncoder::decode on the device: num symbols (each plus the tree's
 // offset, which must land in [0, limit) unless limit is 0) into d_out on
 // ctx->stream.  Returns with the stream drained; *used = bytes of the Huffman
-// section.
+// section.  d_buf, when given, is the same section in device memory: the
+// bitstream is then copied on the device (buf needs only the tree and the
+// byte count), otherwise it is uploaded from buf.
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
-                       size_t *used) {
+                       size_t *used, const uint8_t *d_buf) {
     int32_t offset;
     std::vector<int32_t> left, right, sym;
     std::vector<uint8_t> leaf;
@@ -721,7 +723,7 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     // the stream (its last word zero-padded) goes up on the upload stream in
     // parts, below, so that the map launches overlap the later parts' copies
     hipStream_t us = ctx->ustream;
-    launch_fill_u32(ctx->dec_words.p + (nwords ? nwords - 1 : 0), 2, 0u, us);
+    launch_fill_u32(ctx->dec_words.p + (nwords ? nwords - 1 : 0), 2, 0u, d_buf ? st : us);
     uint32_t *d_tab = ctx->dec_tab.p;
     int32_t *d_tree = reinterpret_cast<int32_t *>(ctx->dec_tab.p + tab.size());
     HIPCHECK(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
@@ -743,9 +745,13 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     // slice, which that part's copy includes)
     {
         const uint32_t nwg = grid.x;
-        const int parts = nwg >= 64 ? kUploadParts : 1;
+        const int parts = nwg >= 64 && !d_buf ? kUploadParts : 1;
         uint64_t sent = 0;
         uint8_t *dst = reinterpret_cast<uint8_t *>(ctx->dec_words.p);
+        if (d_buf) {  // resident: one copy on the stream, one map launch
+            if (enc) HIPCHECK(hipMemcpyAsync(dst, d_buf + at, enc, hipMemcpyDeviceToDevice, st));
+            sent = enc;
+        }
         for (int k = 0; k < parts; k++) {
             const uint32_t w0 = (uint32_t) ((uint64_t) nwg * k / parts);
             const uint32_t w1 = (uint32_t) ((uint64_t) nwg * (k + 1) / parts);
@@ -754,8 +760,10 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
                 HIPCHECK(hipMemcpyAsync(dst + sent, buf + at + sent, need - sent, hipMemcpyHostToDevice, us));
                 sent = need;
             }
-            HIPCHECK(hipEventRecord(ctx->dec_ev[k], us));
-            HIPCHECK(hipStreamWaitEvent(st, ctx->dec_ev[k], 0));
+            if (!d_buf) {
+                HIPCHECK(hipEventRecord(ctx->dec_ev[k], us));
+                HIPCHECK(hipStreamWaitEvent(st, ctx->dec_ev[k], 0));
+            }
             if (w1 > w0)
                 hipLaunchKernelGGL(huff_map_kernel, dim3(w1 - w0), block, 0, st, ctx->dec_words.p, nwords, nbits,
                                    nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0);
@@ -792,11 +800,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
 }
 
 template int gpu_huffman_decode<uint16_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint16_t *,
-                                          size_t *);
+                                          size_t *, const uint8_t *);
 template int gpu_huffman_decode<int32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, int32_t *,
-                                         size_t *);
+                                          size_t *, const uint8_t *);
 template int gpu_huffman_decode<uint32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint32_t *,
-                                          size_t *);
+                                          size_t *, const uint8_t *);
 
 // ---------------------------------------------------------------------------
 // Unpredictable values of 32-bit code streams (the general frontend): the

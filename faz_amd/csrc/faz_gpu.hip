@@ -515,6 +515,101 @@ faz_status faz_gpu_range_f32(faz_gpu_ctx *ctx, const float *d_in, size_t n, floa
 
 }  // extern "C"
 
+// Kernel of each level, coarse to fine: 0 per-pass launches, 1 the per-block
+// level kernel, 2 the whole-tile kernel (the decoder has no per-block level
+// kernel: its levels are 0 or 2).  g3 holds the level launches, with the
+// whole-tile levels' lattice plumbing: each reads the coarser lattice and
+// writes its own either at full resolution in d_recon or, between two
+// whole-tile levels, as a compact array in ctx->lat.
+static hipError_t level_kernels(faz_gpu_ctx *ctx, const Plan &pl, const faz_interp_params *p, float *d_recon,
+                                bool final_recon, bool decode, std::vector<Level3D> &g3, std::vector<int> &kind) {
+    const bool fused = ctx->fused3d && pl.N == 3 && (uint32_t) p->block_size <= kFusedMaxBlock;
+    // levels with few blocks run as per-pass launches, which spread a coarse
+    // level's points over the whole device
+    auto level_fused = [&](const LevelPlan &L) {
+        if (!fused) return false;
+        uint64_t nbw = 1;
+        for (int a = 0; a < 3; a++) nbw *= pl.dims[a] == 1 ? 1 : (pl.dims[a] - 1 + L.bs - 1) / L.bs;
+        return nbw >= ctx->fused_min_blocks;
+    };
+    // kernel per level (coarse to fine): 0 per-pass launches, 1 per-block
+    // level kernel, 2 whole-tile kernel
+    const size_t nlv = pl.lv.size();
+    g3.assign(nlv, Level3D{});
+    kind.assign(nlv, 0);
+    for (size_t i = 0; i < nlv; i++) {
+        const LevelPlan &L = pl.lv[i];
+        if (!level_fused(L)) continue;
+        Level3D &g = g3[i];
+        memset(&g, 0, sizeof(g));
+        for (int a = 0; a < 3; a++) {
+            g.dims[a] = pl.dims[a];
+            g.offs[a] = pl.offs[a];
+            g.nb[a] = L.nb[a];
+            g.nbw[a] = pl.dims[a] == 1 ? 1 : (uint32_t) ((pl.dims[a] - 1 + L.bs - 1) / L.bs);
+            g.p[a] = pl.perms[L.dir][a];
+        }
+        g.s = (uint32_t) ((uint64_t) 1 << (L.level - 1));
+        g.bs = (uint32_t) L.bs;
+        g.linear = L.linear;
+        g.eb = L.eb;
+        g.recip = L.recip;
+        g.ebf = float_floor(L.eb);
+        g.radius = p->radius;
+        g.level_base = L.base;
+        g.block_prefix = ctx->prefix.p + L.prefix_at;
+        g.code_limit = pl.total_codes;
+        kind[i] = ctx->tile3d && tile3d_eligible(g) ? 2 : (decode ? 0 : 1);
+    }
+    // Whole-tile levels read the coarser lattice and write their own either
+    // at full resolution in d_recon or, between two whole-tile levels, as a
+    // compact array (lattice index c at c0 m1 m2 + c1 m2 + c2).
+    {
+        int cur = -1;  // lat[] buffer with the previous level's compact lattice
+        for (size_t i = 0; i < nlv; i++) {
+            if (kind[i] != 2) {
+                cur = -1;
+                continue;
+            }
+            Level3D &g = g3[i];
+            const uint64_t s = g.s;
+            auto compact = [&](uint64_t step, uint64_t *str, uint64_t &count) {
+                uint64_t m[3];
+                for (int a = 0; a < 3; a++) m[a] = (pl.dims[a] - 1) / step + 1;
+                str[0] = m[1] * m[2];
+                str[1] = m[2];
+                str[2] = 1;
+                count = m[0] * m[1] * m[2];
+            };
+            uint64_t count = 0;
+            if (cur < 0) {
+                g.csrc = d_recon;
+                for (int a = 0; a < 3; a++) g.cstr[a] = 2 * s * pl.offs[a];
+            } else {
+                g.csrc = ctx->lat[cur].p;
+                compact(2 * s, g.cstr, count);
+            }
+            const bool last = i + 1 == nlv;
+            const bool next_tile = !last && kind[i + 1] == 2;
+            if (next_tile) {
+                const int nb = cur < 0 ? 0 : 1 - cur;
+                compact(s, g.dstr, count);
+                {
+                    const hipError_t e = ctx->lat[nb].ensure(count);
+                    if (e != hipSuccess) return e;
+                }
+                g.dst = ctx->lat[nb].p;
+                cur = nb;
+            } else {
+                g.dst = last && !final_recon ? nullptr : d_recon;
+                for (int a = 0; a < 3; a++) g.dstr[a] = s * pl.offs[a];
+                cur = -1;
+            }
+        }
+    }
+    return hipSuccess;
+}
+
 // Interpolation + quantization of one field.  d_recon is a full-size working
 // array that receives every reconstructed point when final_recon is set;
 // otherwise it only carries the coarse levels (the compression itself needs
@@ -577,44 +672,10 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         ctx->prof_ev.resize(2 * npass);
         for (size_t i = have; i < ctx->prof_ev.size(); i++) HIPCHECK(hipEventCreate(&ctx->prof_ev[i]));
     }
-    const bool fused = ctx->fused3d && pl.N == 3 && (uint32_t) p->block_size <= kFusedMaxBlock;
-    // levels with few blocks run as per-pass launches, which spread a coarse
-    // level's points over the whole device
-    auto level_fused = [&](const LevelPlan &L) {
-        if (!fused) return false;
-        uint64_t nbw = 1;
-        for (int a = 0; a < 3; a++) nbw *= pl.dims[a] == 1 ? 1 : (pl.dims[a] - 1 + L.bs - 1) / L.bs;
-        return nbw >= ctx->fused_min_blocks;
-    };
-    // kernel per level (coarse to fine): 0 per-pass launches, 1 per-block
-    // level kernel, 2 whole-tile kernel
     const size_t nlv = pl.lv.size();
-    std::vector<Level3D> g3(nlv);
-    std::vector<int> kind(nlv, 0);
-    for (size_t i = 0; i < nlv; i++) {
-        const LevelPlan &L = pl.lv[i];
-        if (!level_fused(L)) continue;
-        Level3D &g = g3[i];
-        memset(&g, 0, sizeof(g));
-        for (int a = 0; a < 3; a++) {
-            g.dims[a] = pl.dims[a];
-            g.offs[a] = pl.offs[a];
-            g.nb[a] = L.nb[a];
-            g.nbw[a] = pl.dims[a] == 1 ? 1 : (uint32_t) ((pl.dims[a] - 1 + L.bs - 1) / L.bs);
-            g.p[a] = pl.perms[L.dir][a];
-        }
-        g.s = (uint32_t) ((uint64_t) 1 << (L.level - 1));
-        g.bs = (uint32_t) L.bs;
-        g.linear = L.linear;
-        g.eb = L.eb;
-        g.recip = L.recip;
-        g.ebf = float_floor(L.eb);
-        g.radius = p->radius;
-        g.level_base = L.base;
-        g.block_prefix = ctx->prefix.p + L.prefix_at;
-        g.code_limit = pl.total_codes;
-        kind[i] = ctx->tile3d && tile3d_eligible(g) ? 2 : 1;
-    }
+    std::vector<Level3D> g3;
+    std::vector<int> kind;
+    HIPCHECK(level_kernels(ctx, pl, p, d_recon, final_recon, false, g3, kind));
     // The bounded quantizer needs |x - pred| / eb < 2^30 on the level.  Every
     // prediction is a stencil of reconstructed values of magnitude at most
     // max|x| + eb with coefficient sum at most 28/8 (quad_3), so
@@ -631,49 +692,6 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             if (kind[i] != 2) continue;
             const double v = 4.5 * absmax * g3[i].recip + 8.0;
             g3[i].bounded = std::isfinite(absmax) && absmax < 1e37 && v < 1073741824.0;
-        }
-    }
-    // Whole-tile levels read the coarser lattice and write their own either
-    // at full resolution in d_recon or, between two whole-tile levels, as a
-    // compact array (lattice index c at c0 m1 m2 + c1 m2 + c2).
-    {
-        int cur = -1;  // lat[] buffer with the previous level's compact lattice
-        for (size_t i = 0; i < nlv; i++) {
-            if (kind[i] != 2) {
-                cur = -1;
-                continue;
-            }
-            Level3D &g = g3[i];
-            const uint64_t s = g.s;
-            auto compact = [&](uint64_t step, uint64_t *str, uint64_t &count) {
-                uint64_t m[3];
-                for (int a = 0; a < 3; a++) m[a] = (pl.dims[a] - 1) / step + 1;
-                str[0] = m[1] * m[2];
-                str[1] = m[2];
-                str[2] = 1;
-                count = m[0] * m[1] * m[2];
-            };
-            uint64_t count = 0;
-            if (cur < 0) {
-                g.csrc = d_recon;
-                for (int a = 0; a < 3; a++) g.cstr[a] = 2 * s * pl.offs[a];
-            } else {
-                g.csrc = ctx->lat[cur].p;
-                compact(2 * s, g.cstr, count);
-            }
-            const bool last = i + 1 == nlv;
-            const bool next_tile = !last && kind[i + 1] == 2;
-            if (next_tile) {
-                const int nb = cur < 0 ? 0 : 1 - cur;
-                compact(s, g.dstr, count);
-                HIPCHECK(ctx->lat[nb].ensure(count));
-                g.dst = ctx->lat[nb].p;
-                cur = nb;
-            } else {
-                g.dst = last && !final_recon ? nullptr : d_recon;
-                for (int a = 0; a < 3; a++) g.dstr[a] = s * pl.offs[a];
-                cur = -1;
-            }
         }
     }
     // with level launches in the plan the profile times those only: the pass
@@ -1245,28 +1263,26 @@ int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx) {
     return best;
 }
 
-faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
-                                         size_t num) {
-    if (!ctx || !h_raw || !d_out || N < 1 || N > 4) return FAZ_ERR_INVALID;
-    HIPCHECK(hipSetDevice(ctx->device));
+}  // extern "C"
+
+// SZInterpolationCompressor::decompress (SIC:42-199) of a parsed engine
+// buffer: the unpredictable list either in host memory (h_unpred) or in
+// device memory (d_unpred, any alignment), the Huffman section's tree and byte
+// count in host memory (huff, huff_n bytes up to the end of the buffer) and,
+// when d_huff is given, the same section in device memory (the bitstream is
+// then read there).  Whole-tile levels run the tile decoder, the others one
+// recovery launch per pass.
+static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb, int radius, uint64_t U,
+                             const float *h_unpred, const uint8_t *d_unpred, const uint8_t *huff, size_t huff_n,
+                             const uint8_t *d_huff, int N, float *d_out, size_t num) {
     const auto t0 = std::chrono::steady_clock::now();
     auto ms_since = [](std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     hipStream_t st = ctx->stream;
-    EngineHeader h;
-    const uint8_t *pos = h_raw;
-    if (!read_engine_header(pos, n, N, h)) CTX_FAIL(FAZ_ERR_FORMAT, "engine header truncated or unsupported");
     size_t total = 1;
     for (int i = 0; i < N; i++) total *= h.dims[i];
     if (total != num) CTX_FAIL(FAZ_ERR_INVALID, "stream holds %zu points, caller expects %zu", total, num);
-    double eb;
-    int radius;
-    uint64_t U;
-    if (!read_quantizer_head(pos, n - (pos - h_raw), eb, radius, U)) CTX_FAIL(FAZ_ERR_FORMAT, "quantizer truncated");
-    if ((size_t) (n - (pos - h_raw)) < U * sizeof(float)) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
-    const float *unpred_h = reinterpret_cast<const float *>(pos);
-    pos += U * sizeof(float);
     faz_interp_params p;
     memset(&p, 0, sizeof(p));
     p.N = N;
@@ -1292,28 +1308,29 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
     HIPCHECK(ctx->prefix.ensure(pl.prefix.size()));
     HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
     HIPCHECK(ctx->chunk_base.ensure(nchunks));
-    HIPCHECK(ctx->freq.ensure(kHistSlots));
-    HIPCHECK(ctx->first.ensure(65536));
     HIPCHECK(ctx->scalars.ensure(4));
+    HIPCHECK(ctx->err_flag.ensure(1));
     // codes decoded on the device (HUF:236-268), then the outliers' ranks
     const auto td = std::chrono::steady_clock::now();
-    int drc = gpu_huffman_decode<uint16_t>(ctx, pos, n - (pos - h_raw), num, 65536u, ctx->codes.p, nullptr);
+    int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff);
     if (drc) return drc;
     ctx->timings[6] = ms_since(td);
-    if (U) HIPCHECK(hipMemcpyAsync(ctx->unpred.p, unpred_h, U * 4, hipMemcpyHostToDevice, st));
+    if (U) {
+        if (d_unpred)
+            HIPCHECK(hipMemcpyAsync(ctx->unpred.p, d_unpred, U * 4, hipMemcpyDeviceToDevice, st));
+        else
+            HIPCHECK(hipMemcpyAsync(ctx->unpred.p, h_unpred, U * 4, hipMemcpyHostToDevice, st));
+    }
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
     ctx->prefix_key = 0;  // the compression path's table is gone
-    launch_hist(ctx->codes.p, num, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, radius, st);
+    launch_zero_count_u16(ctx->codes.p, num, ctx->chunk_zeros.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
-    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned: zero count, first code
+    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned: zero count
     HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->codes.p, 2, hipMemcpyDeviceToHost, st));
     HIPCHECK(spin_wait(st));
     uint64_t zeros;
     memcpy(&zeros, hs, 8);
-    uint16_t c0;
-    memcpy(&c0, hs + 2, 2);
     if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
                              (unsigned long long) U, (unsigned long long) zeros);
     launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
@@ -1325,19 +1342,90 @@ faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw,
         if (U < pl.anchor_count) CTX_FAIL(FAZ_ERR_FORMAT, "fewer unpredictable values than anchors");
         launch_anchor_scatter(ctx->unpred.p, d_out, N, pl.dims, pl.offs, (uint32_t) h.max_step, pl.anchor_count, st);
     } else {
-        float v0 = c0 ? (float) ((double) 0.0f + (double) (2 * ((int) c0 - radius)) * eb) : unpred_h[0];
-        HIPCHECK(hipMemcpyAsync(d_out, &v0, 4, hipMemcpyHostToDevice, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        launch_first_value(ctx->codes.p, ctx->unpred.p, U, radius, eb, d_out, st);
     }
-    for (auto &L: pl.lv)
+    std::vector<Level3D> g3;
+    std::vector<int> kind;
+    HIPCHECK(level_kernels(ctx, pl, &p, d_out, true, true, g3, kind));
+    launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
+    for (size_t i = 0; i < pl.lv.size(); i++) {
+        LevelPlan &L = pl.lv[i];
+        if (kind[i] == 2) {
+            launch_recover_tile3d(g3[i], ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid, st);
+            continue;
+        }
         for (auto &d: L.passes) {
             d.block_prefix = ctx->prefix.p + L.prefix_at;
             launch_recover_pass(d, d_out, ctx->codes.p, ctx->outval.p, st);
         }
+    }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(st));
+    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(spin_wait(st));
+    if (hs[2]) CTX_FAIL(FAZ_ERR_FORMAT, "a block's codes lie outside the stream");
     ctx->timings[7] = ms_since(t0);
     return FAZ_OK;
+}
+
+extern "C" {
+
+faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
+                                         size_t num) {
+    if (!ctx || !h_raw || !d_out || N < 1 || N > 4) return FAZ_ERR_INVALID;
+    HIPCHECK(hipSetDevice(ctx->device));
+    EngineHeader h;
+    const uint8_t *pos = h_raw;
+    if (!read_engine_header(pos, n, N, h)) CTX_FAIL(FAZ_ERR_FORMAT, "engine header truncated or unsupported");
+    double eb;
+    int radius;
+    uint64_t U;
+    if (!read_quantizer_head(pos, n - (pos - h_raw), eb, radius, U)) CTX_FAIL(FAZ_ERR_FORMAT, "quantizer truncated");
+    if ((size_t) (n - (pos - h_raw)) / sizeof(float) < U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
+    const float *unpred_h = reinterpret_cast<const float *>(pos);
+    pos += U * sizeof(float);
+    return (faz_status) interp_decompress(ctx, h, eb, radius, U, unpred_h, nullptr, pos, n - (pos - h_raw), nullptr,
+                                          N, d_out, num);
+}
+
+faz_status faz_gpu_interp_decompress_dev(faz_gpu_ctx *ctx, const uint8_t *d_raw, size_t n, int N, float *d_out,
+                                         size_t num) {
+    if (!ctx || !d_raw || !d_out || N < 1 || N > 4) return FAZ_ERR_INVALID;
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // the header and the quantizer head lie in the first few hundred bytes
+    std::vector<uint8_t> &hb = ctx->h_stage;
+    const size_t head = std::min<size_t>(n, 4096);
+    hb.resize(head);
+    HIPCHECK(hipMemcpyAsync(hb.data(), d_raw, head, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    EngineHeader h;
+    const uint8_t *pos = hb.data();
+    if (!read_engine_header(pos, head, N, h)) CTX_FAIL(FAZ_ERR_FORMAT, "engine header truncated or unsupported");
+    double eb;
+    int radius;
+    uint64_t U;
+    if (!read_quantizer_head(pos, head - (pos - hb.data()), eb, radius, U))
+        CTX_FAIL(FAZ_ERR_FORMAT, "quantizer truncated");
+    const size_t at_unpred = (size_t) (pos - hb.data());
+    if ((n - at_unpred) / sizeof(float) < U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable list truncated");
+    const size_t at_huff = at_unpred + U * sizeof(float);
+    // the Huffman section's tree (node count in its first 12 bytes) and
+    // byte count, read in one or two copies
+    size_t want = std::min<size_t>(n - at_huff, 1 << 16);
+    std::vector<uint8_t> &tb = ctx->h_stage2;
+    for (int round = 0; round < 2; round++) {
+        tb.resize(want);
+        if (want) HIPCHECK(hipMemcpyAsync(tb.data(), d_raw + at_huff, want, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (want < 12) break;
+        const unsigned nc = ((unsigned) tb[4] << 24) | ((unsigned) tb[5] << 16) | ((unsigned) tb[6] << 8) | tb[7];
+        const size_t w = nc <= 256 ? 1 : (nc <= 65536 ? 2 : 4);
+        const size_t need = std::min<size_t>(n - at_huff, 12 + (1 + 2 * (size_t) nc * w + (size_t) nc * 5) + 8);
+        if (need <= want) break;
+        want = need;
+    }
+    return (faz_status) interp_decompress(ctx, h, eb, radius, U, nullptr, d_raw + at_unpred, tb.data(),
+                                          n - at_huff, d_raw + at_huff, N, d_out, num);
 }
 
 }  // extern "C"

@@ -117,6 +117,14 @@ constexpr int kPerm3[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0,
 bool tile3d_eligible(const Level3D &L);
 void launch_interp_tile3d(const Level3D &L, const float *d_in, uint16_t *d_codes, float *d_outval, uint32_t *d_err,
                           uint32_t grid, hipStream_t st);
+// The decoder of a whole-tile level: codes and the unpredictable value of
+// every code-0 position (indexed by code position) in, L.csrc / L.dst as in
+// the compressor; bit 1 of *d_err when a block segment leaves the stream.
+void launch_recover_tile3d(const Level3D &L, const uint16_t *d_codes, const float *d_outval, uint32_t *d_err,
+                           uint32_t grid, hipStream_t st);
+// Zero codes per kHistChunk chunk of n 16-bit codes (the outlier ranks of
+// the decoder, without the histogram).
+void launch_zero_count_u16(const uint16_t *d_codes, size_t n, uint32_t *d_chunk_zeros, hipStream_t st);
 
 // Kernel launchers (faz_kernels.hip).  All are asynchronous on `st`.
 void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
@@ -132,6 +140,10 @@ void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint3
                            uint32_t step, uint64_t count, hipStream_t st);
 void launch_first_point(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, double eb,
                         double recip, int radius, uint32_t *d_err, hipStream_t st);
+// out[0] of a decoder without anchors: code 0 recovered from prediction 0, or
+// the first unpredictable value.
+void launch_first_value(const uint16_t *d_codes, const float *d_unpred, uint64_t n_unpred, int radius, double eb,
+                        float *d_out, hipStream_t st);
 void launch_interp_pass(const PassDesc &d, const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval,
                         double *d_loss, uint32_t *d_err, hipStream_t st);
 void launch_recover_pass(const PassDesc &d, float *d_data, const uint16_t *d_codes, const float *d_unpred_at,
@@ -189,10 +201,11 @@ void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unp
 
 // HuffmanEncoder::decode on the device (faz_decode.hip): num symbols into
 // d_out on ctx->stream, each symbol + the tree's offset checked to lie in
-// [0, limit); returns a faz_status with the stream drained.
+// [0, limit); returns a faz_status with the stream drained.  With d_buf (the
+// section in device memory) the host buf needs only the tree and byte count.
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
-                       size_t *used);
+                       size_t *used, const uint8_t *d_buf = nullptr);
 
 // outval[i] = the unpredictable value of zero code i, for n 32-bit codes on
 // the device (faz_decode.hip); uploads the U host values first.

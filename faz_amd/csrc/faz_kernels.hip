@@ -276,6 +276,19 @@ void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint3
                        N, dims[1], N > 2 ? dims[2] : 1, offs[0], N > 2 ? offs[1] : 1, step, (uint32_t) count);
 }
 
+// The decoder's first point without anchors (SIC:372-374 in recover mode):
+// predicted as 0, or the first unpredictable value.
+__global__ void first_value_kernel(const uint16_t *codes, const float *unpred, uint64_t n_unpred, int radius,
+                                   double eb, float *out) {
+    const int code = codes[0];
+    out[0] = code ? recover_point(0.0f, code, radius, eb) : (n_unpred ? unpred[0] : 0.0f);
+}
+
+void launch_first_value(const uint16_t *d_codes, const float *d_unpred, uint64_t n_unpred, int radius, double eb,
+                        float *d_out, hipStream_t st) {
+    hipLaunchKernelGGL(first_value_kernel, dim3(1), dim3(1), 0, st, d_codes, d_unpred, n_unpred, radius, eb, d_out);
+}
+
 __global__ void first_point_kernel(const float *in, float *recon, uint16_t *codes, float *outval, double eb,
                                    double recip, int radius, uint32_t *err) {
     float rec;
@@ -424,6 +437,38 @@ __device__ __forceinline__ void hist_load(const uint16_t *__restrict__ codes, ui
             for (int k = 0; k < 8; k++) t[k] = j + k < end ? codes[j + k] : 0;
             q[g] = make_uint4(t[0] | t[1] << 16, t[2] | t[3] << 16, t[4] | t[5] << 16, t[6] | t[7] << 16);
         }
+    }
+}
+
+// Zero codes per chunk only (the decoder's outlier ranks need no histogram):
+// one workgroup per chunk, the histogram's coalesced loads.
+__global__ void __launch_bounds__(kHistThreads) zero_count_u16_kernel(const uint16_t *__restrict__ codes, uint32_t n,
+                                                                     uint32_t *__restrict__ chunk_zeros, int aligned) {
+    __shared__ uint32_t s_wave[kHistThreads / 64];
+    const uint32_t c = blockIdx.x;
+    const uint32_t base = c * kHistChunk;
+    const uint32_t end = min(base + kHistChunk, n);
+    uint4 q[kHistGroups];
+    hist_load(codes, c, n, aligned, q);
+    uint32_t z = 0;
+#pragma unroll
+    for (int g = 0; g < kHistGroups; g++) {
+        const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
+        const uint32_t w[4] = {q[g].x, q[g].y, q[g].z, q[g].w};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t v = k & 1 ? w[k >> 1] >> 16 : w[k >> 1] & 0xFFFFu;
+            z += (v == 0 && j + k < end) ? 1u : 0u;  // past the end: padding, not a code
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) z += __shfl_xor(z, o);
+    if (threadIdx.x % 64 == 0) s_wave[threadIdx.x / 64] = z;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kHistThreads / 64; w++) t += s_wave[w];
+        chunk_zeros[c] = t;
     }
 }
 
@@ -594,6 +639,13 @@ __global__ void __launch_bounds__(256) hist_pack_kernel(const uint32_t *__restri
     else if (i < (uint32_t) kHistPackWords) v = scalar0[i - (kHistPackFreq0 + 4)];
     else return;
     pack[i] = v;
+}
+
+void launch_zero_count_u16(const uint16_t *d_codes, size_t n, uint32_t *d_chunk_zeros, hipStream_t st) {
+    const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
+    if (nchunks)
+        hipLaunchKernelGGL(zero_count_u16_kernel, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
+                           d_chunk_zeros, (int) (((uintptr_t) d_codes & 15) == 0));
 }
 
 void launch_hist_pack(const uint32_t *d_freq, const uint32_t *d_first, int radius, const uint64_t *d_scalar0,

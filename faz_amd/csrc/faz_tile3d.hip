@@ -57,6 +57,7 @@ __host__ __device__ constexpr int lds_stride(int axis) { return axis == 0 ? kPla
 enum : uint32_t { kHas = 1, kAt32_0 = 2, kAt32_1 = 4, kAt32_2 = 8, kAllEven = 16 };
 
 struct Tile {
+    uint32_t k[3];      // block coordinates in the launched raster
     uint32_t n[3];
     uint32_t first;     // bit a set when the block starts at 0 on axis a
     uint32_t miss;      // kAt32_a set when axis a has no position 32
@@ -97,16 +98,15 @@ __device__ __forceinline__ void tile_origin(const Level3D &L, uint32_t blk, uint
     cbase = c;
 }
 
+// The block at raster coordinates (k0, k1, k2).
 template<int DIR>
-__device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
+__device__ __forceinline__ Tile tile_at(const Level3D &L, uint32_t k0, uint32_t k1, uint32_t k2) {
     constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
     Tile B;
-    uint32_t r = blk;
-    const uint32_t k2 = r % L.nbw[2];
-    r /= L.nbw[2];
-    const uint32_t k1 = r % L.nbw[1];
-    const uint32_t k0 = r / L.nbw[1];
     const uint32_t k[3] = {k0, k1, k2};
+    B.k[0] = k0;
+    B.k[1] = k1;
+    B.k[2] = k2;
     uint32_t b[3];
     B.miss = 0;
     B.nown = 0;
@@ -134,6 +134,22 @@ __device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
     B.total = B.cnt0 + B.cnt1 + g1_of(B, p0) * g1_of(B, p1) * 16;
     B.emit = B.seg + B.total <= L.code_limit ? 1u : 0u;
     return B;
+}
+
+template<int DIR>
+__device__ __forceinline__ Tile tile_of(const Level3D &L, uint32_t blk) {
+    uint32_t r = blk;
+    const uint32_t k2 = r % L.nbw[2];
+    r /= L.nbw[2];
+    const uint32_t k1 = r % L.nbw[1];
+    const uint32_t k0 = r / L.nbw[1];
+    return tile_at<DIR>(L, k0, k1, k2);
+}
+
+// First code of pass PASS in block B's segment.
+template<int DIR, int PASS>
+__device__ __forceinline__ uint64_t pass_base(const Tile &B) {
+    return B.seg + (PASS == 0 ? 0u : (PASS == 1 ? B.cnt0 : B.cnt0 + B.cnt1));
 }
 
 // Per-thread placement of the tile loads, fixed for the whole launch.
@@ -263,6 +279,46 @@ __device__ __forceinline__ void store_tile(const float *T, const Place &P, const
         if (slot_on(P.ef[j], B.nown)) st(db, T[P.ef[j] >> 8], P.dg[j], 0);
 }
 
+// Predictions of a lane's kPerLane odd positions of one line (stride ESA in
+// LDS, line start at base + lane offset already applied): lane t reads the
+// even positions 2 kPerLane t - 2 + 2k, k < kPerLane + 3 (olo / ohi keep the
+// first and last reads inside the line for the first and last lane, which
+// never use them), and the end-of-line stencils are selected for the first
+// and last lane (SIC:1360-1414).
+template<bool LINEAR, int ESA>
+__device__ __forceinline__ void line_pred(const float *T, int base, int olo, int ohi, bool first_lane, bool last_lane,
+                                          uint32_t na, float *pr) {
+    constexpr int NE = kPerLane + 3;
+    float e[NE];
+    e[0] = LINEAR ? 0.f : T[base + olo];
+#pragma unroll
+    for (int k = 1; k < NE - 1; k++) e[k] = T[base + (2 * k - 2) * ESA];
+    e[NE - 1] = LINEAR ? 0.f : T[base + ohi];
+    constexpr int h = kPerLane;  // e[h - 2 .. h + 1] hold positions 26 .. 32 for the last lane
+    if (LINEAR) {
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) pr[j] = i_linear(e[j + 1], e[j + 2]);
+        if (na == 32) {
+            const float l1 = i_linear1(e[h - 1], e[h]);
+            pr[h - 1] = last_lane ? l1 : pr[h - 1];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) pr[j] = i_cubic(e[j], e[j + 1], e[j + 2], e[j + 3]);
+        const float q1 = i_quad1(e[1], e[2], e[3]);
+        pr[0] = first_lane ? q1 : pr[0];
+        if (na == 33) {
+            const float q2 = i_quad2(e[h - 1], e[h], e[h + 1]);
+            pr[h - 1] = last_lane ? q2 : pr[h - 1];
+        } else {
+            const float q2 = i_quad2(e[h - 2], e[h - 1], e[h]);
+            const float q3 = i_quad3(e[h - 2], e[h - 1], e[h]);
+            pr[h - 2] = last_lane ? q2 : pr[h - 2];
+            pr[h - 1] = last_lane ? q3 : pr[h - 1];
+        }
+    }
+}
+
 // One pass of the block over the tile in LDS.
 template<int DIR, bool LINEAR, bool BOUNDED, int PASS>
 __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D &L, Rsrc cb, Rsrc ob,
@@ -277,7 +333,6 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
     constexpr uint32_t LINES = NU * NW;
     constexpr int ROUNDS = (int) ((LINES + kLinesPerRound - 1) / kLinesPerRound);
 
-    constexpr int NE = kPerLane + 3;                           // even neighbours per lane
     const uint32_t t = tid % kLanesPerLine, line0 = tid / kLanesPerLine;
     const uint32_t na = B.n[A];
     const uint32_t umax = UE ? (B.n[U] - 1) / 2 : B.n[U] - 1;
@@ -308,36 +363,10 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
         const uint32_t u = line / NW, w = line - u * NW;
         if ((LINES % kLinesPerRound == 0 || line < LINES) && u <= umax && w <= wmax) {
             const int base = (int) (u * ESU + w * ESW) + (int) t * 2 * kPerLane * ESA;
-            float e[NE], o[kPerLane], pr[kPerLane];
-            e[0] = LINEAR ? 0.f : T[base + olo];
-#pragma unroll
-            for (int k = 1; k < NE - 1; k++) e[k] = T[base + (2 * k - 2) * ESA];
-            e[NE - 1] = LINEAR ? 0.f : T[base + ohi];
+            float o[kPerLane], pr[kPerLane];
 #pragma unroll
             for (int j = 0; j < kPerLane; j++) o[j] = T[base + (2 * j + 1) * ESA];
-            constexpr int h = kPerLane;  // e[h - 2 .. h + 1] hold positions 26 .. 32 for the last lane
-            if (LINEAR) {
-#pragma unroll
-                for (int j = 0; j < kPerLane; j++) pr[j] = i_linear(e[j + 1], e[j + 2]);
-                if (na == 32) {
-                    const float l1 = i_linear1(e[h - 1], e[h]);
-                    pr[h - 1] = last_lane ? l1 : pr[h - 1];
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < kPerLane; j++) pr[j] = i_cubic(e[j], e[j + 1], e[j + 2], e[j + 3]);
-                const float q1 = i_quad1(e[1], e[2], e[3]);
-                pr[0] = first_lane ? q1 : pr[0];
-                if (na == 33) {
-                    const float q2 = i_quad2(e[h - 1], e[h], e[h + 1]);
-                    pr[h - 1] = last_lane ? q2 : pr[h - 1];
-                } else {
-                    const float q2 = i_quad2(e[h - 2], e[h - 1], e[h]);
-                    const float q3 = i_quad3(e[h - 2], e[h - 1], e[h]);
-                    pr[h - 2] = last_lane ? q2 : pr[h - 2];
-                    pr[h - 1] = last_lane ? q3 : pr[h - 1];
-                }
-            }
+            line_pred<LINEAR, ESA>(T, base, olo, ohi, first_lane, last_lane, na, pr);
             const bool own = B.emit && (fu | u) && (fw | w);
             const uint32_t cl = pbase + ((u + fu - 1) * gw + (w + fw - 1)) * 16;
             const uint32_t coff = own ? 2 * cl : kDropped;  // codes of lines owned elsewhere are dropped
@@ -353,6 +382,157 @@ __device__ __forceinline__ void tile_pass(float *T, const Tile &B, const Level3D
             }
         }
     }
+}
+
+// Code lines of one pass as the decoder finds them.  A line of the block at
+// position u = 0 (w = 0) of a block that does not start the axis lies on the
+// face that the previous block along U (W) owns, at that block's last
+// position; its codes sit in that block's segment.  owner[c] for c = mu | 2 mw
+// (mu, mw: the line moves to the previous block along U, W) gives the first
+// code of the pass in the owning block and its line count along W.
+struct LineOwners {
+    uint64_t base[4];
+    uint32_t gw[4];
+    uint32_t ulast, wlast;  // own index of the last U / W position in the previous block
+    uint32_t ok;            // every owning block's segment lies inside the stream
+};
+
+template<int DIR, int PASS>
+__device__ __forceinline__ LineOwners line_owners(const Level3D &L, const Tile &B) {
+    constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
+    constexpr int U = PASS == 0 ? p1 : p0;
+    constexpr int W = PASS == 2 ? p1 : p2;
+    constexpr bool UE = PASS == 0, WE = PASS != 2;
+    LineOwners O;
+    const uint32_t gw = WE ? g2_of(B, W) : g1_of(B, W);
+    O.base[0] = pass_base<DIR, PASS>(B);
+    O.gw[0] = gw;
+    O.base[1] = O.base[2] = O.base[3] = O.base[0];
+    O.gw[1] = O.gw[2] = O.gw[3] = gw;
+    O.ulast = O.wlast = 0;
+    O.ok = B.emit;
+    const uint32_t fu = first_of(B, U), fw = first_of(B, W);
+    // a previous block along an axis is never the last one: 33 positions,
+    // the last of them (16 of the even ones, or 32) owned by it
+    if (!fu) {
+        uint32_t k[3] = {B.k[0], B.k[1], B.k[2]};
+        k[U]--;
+        const Tile Q = tile_at<DIR>(L, k[0], k[1], k[2]);
+        O.base[1] = pass_base<DIR, PASS>(Q);
+        O.ulast = (UE ? 16u : 32u) + first_of(Q, U) - 1;
+        O.ok &= Q.emit;
+    }
+    if (!fw) {
+        uint32_t k[3] = {B.k[0], B.k[1], B.k[2]};
+        k[W]--;
+        const Tile Q = tile_at<DIR>(L, k[0], k[1], k[2]);
+        O.base[2] = pass_base<DIR, PASS>(Q);
+        O.gw[2] = WE ? g2_of(Q, W) : g1_of(Q, W);
+        O.wlast = (WE ? 16u : 32u) + first_of(Q, W) - 1;
+        O.ok &= Q.emit;
+        if (!fu) {
+            k[U]--;
+            const Tile R = tile_at<DIR>(L, k[0], k[1], k[2]);
+            O.base[3] = pass_base<DIR, PASS>(R);
+            O.gw[3] = O.gw[2];
+            O.ok &= R.emit;
+        }
+    }
+    return O;
+}
+
+// One pass of the decoder over the tile in LDS (PB_recover, SIC:110-196 with
+// Q:160-180): the stencils of the compressor, each odd position recovered
+// from its code, or the unpredictable value at the code's position.
+template<int DIR, bool LINEAR, int PASS>
+__device__ __forceinline__ void tile_recover_pass(float *T, const Tile &B, const Level3D &L,
+                                                  const uint16_t *__restrict__ codes,
+                                                  const float *__restrict__ outval, uint32_t tid, uint32_t *err) {
+    constexpr int p0 = kPerm3[DIR][0], p1 = kPerm3[DIR][1], p2 = kPerm3[DIR][2];
+    constexpr int A = PASS == 0 ? p0 : (PASS == 1 ? p1 : p2);
+    constexpr int U = PASS == 0 ? p1 : p0;
+    constexpr int W = PASS == 2 ? p1 : p2;
+    constexpr bool UE = PASS == 0, WE = PASS != 2;
+    constexpr uint32_t NU = UE ? 17 : 33, NW = WE ? 17 : 33;
+    constexpr int ESA = lds_stride(A), ESU = lds_stride(U) * (UE ? 2 : 1), ESW = lds_stride(W) * (WE ? 2 : 1);
+    constexpr uint32_t LINES = NU * NW;
+    constexpr int ROUNDS = (int) ((LINES + kLinesPerRound - 1) / kLinesPerRound);
+
+    const LineOwners O = line_owners<DIR, PASS>(L, B);
+    if (!O.ok) {  // uniform: a neighbour's segment falls outside the stream
+        if (tid == 0) atomicOr(err, 2u);
+        return;
+    }
+    const uint32_t t = tid % kLanesPerLine, line0 = tid / kLanesPerLine;
+    const uint32_t na = B.n[A];
+    const uint32_t umax = UE ? (B.n[U] - 1) / 2 : B.n[U] - 1;
+    const uint32_t wmax = WE ? (B.n[W] - 1) / 2 : B.n[W] - 1;
+    const uint32_t fu = first_of(B, U), fw = first_of(B, W);
+    const bool first_lane = t == 0, last_lane = t == kLanesPerLine - 1;
+    uint32_t rk[kPerLane];
+    {
+        const uint32_t ncub = na == 33 ? 14u : 13u;
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) {
+            const uint32_t q = kPerLane * t + j;
+            rk[j] = LINEAR ? q : (q == 0 ? ncub : (q <= ncub ? q - 1 : q));
+        }
+    }
+    const int olo = first_lane ? 0 : -2 * ESA;
+    const int ohi = last_lane ? 2 * kPerLane * ESA : (2 * kPerLane + 2) * ESA;
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint32_t line = line0 + (uint32_t) kLinesPerRound * r;
+        const uint32_t u = line / NW, w = line - u * NW;
+        if ((LINES % kLinesPerRound == 0 || line < LINES) && u <= umax && w <= wmax) {
+            const int base = (int) (u * ESU + w * ESW) + (int) t * 2 * kPerLane * ESA;
+            float pr[kPerLane];
+            line_pred<LINEAR, ESA>(T, base, olo, ohi, first_lane, last_lane, na, pr);
+            const uint32_t mu = (u | fu) == 0, mw = (w | fw) == 0;
+            const uint32_t c = mu | mw << 1;
+            const uint64_t lb = c == 0 ? O.base[0] : (c == 1 ? O.base[1] : (c == 2 ? O.base[2] : O.base[3]));
+            const uint32_t gw = c < 2 ? O.gw[0] : O.gw[2];
+            const uint32_t uu = mu ? O.ulast : u + fu - 1;
+            const uint32_t ww = mw ? O.wlast : w + fw - 1;
+            const uint64_t cl = lb + (uint64_t) (uu * gw + ww) * 16;
+            const uint16_t *cp = codes + cl;
+            int code[kPerLane];
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) code[j] = cp[rk[j]];
+#pragma unroll
+            for (int j = 0; j < kPerLane; j++) {
+                const float rec = code[j] ? recover_point(pr[j], code[j], L.radius, L.eb) : outval[cl + rk[j]];
+                T[base + (2 * j + 1) * ESA] = rec;
+            }
+        }
+    }
+}
+
+// Loads of the coarser lattice only (the decoder predicts every other
+// point of the tile before reading it).
+__device__ __forceinline__ void issue_lattice_loads(Pre &R, const Place &P, const Strides &X, uint64_t cbase,
+                                                    uint32_t miss, const float *__restrict__ csrc) {
+    const Rsrc rb = rsrc_at(csrc + cbase);
+    if (P.lane_even) {
+#pragma unroll
+        for (int k = 0; k < 33; k += 2) {
+            if (k == 32 && (miss & kAt32_1)) break;
+            R.m[k] = ld_coarse(rb, P.cgl, 4 * (k >> 1) * X.C[1]);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++)
+        if (slot_on(P.ef[j], miss) && (P.ef[j] & kAllEven)) R.ed[j] = ld_coarse(rb, P.eg[j], 0);
+}
+
+__device__ __forceinline__ void fill_lattice(float *T, const Pre &R, const Place &P, uint32_t miss) {
+    if (P.lane_even) {
+#pragma unroll
+        for (int k = 0; k < 32; k += 2) T[P.ll + k * kRow] = R.m[k];
+        if (!(miss & kAt32_1)) T[P.ll + 32 * kRow] = R.m[32];
+    }
+#pragma unroll
+    for (int j = 0; j < kEdgeSlots; j++)
+        if (slot_on(P.ef[j], miss) && (P.ef[j] & kAllEven)) T[P.ef[j] >> 8] = R.ed[j];
 }
 
 }  // namespace
@@ -427,6 +607,98 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
         blk = nxt;
     }
     if (!BOUNDED && wide) atomicOr(err, 1u);
+}
+
+// The decoder of one whole-tile level (SZInterpolationCompressor::decompress
+// with block_interpolation in recover mode, SIC:42-199): the same persistent
+// tiles, the coarser lattice the only HBM read besides the codes, L.dst the
+// level's lattice (at full resolution for the finest level).
+template<int DIR, bool LINEAR>
+__global__ void __launch_bounds__(kThreads) recover_tile3d_kernel(Level3D L, const uint16_t *__restrict__ codes,
+                                                                  const float *__restrict__ outval,
+                                                                  uint32_t *__restrict__ err) {
+    __shared__ float T[kTileFloats];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
+    uint32_t blk, end, step;
+    if (gridDim.x % 8 == 0) {
+        const uint32_t per = gridDim.x / 8, xcd = blockIdx.x % 8, chunk = (nblk + 7) / 8;
+        blk = xcd * chunk + blockIdx.x / 8;
+        end = min(nblk, (xcd + 1) * chunk);
+        step = per;
+    } else {
+        blk = blockIdx.x;
+        end = nblk;
+        step = gridDim.x;
+    }
+    if (blk >= end) return;
+    Strides X;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        X.S[a] = L.s * (uint32_t) (a == 2 ? 1 : L.offs[a]);
+        X.C[a] = (uint32_t) L.cstr[a];
+        X.D[a] = (uint32_t) L.dstr[a];
+    }
+    const Place P = place_of(tid, X);
+    Pre R;
+    {
+        uint64_t g, c;
+        uint32_t miss;
+        tile_origin(L, blk, g, c, miss);
+        issue_lattice_loads(R, P, X, c, miss, L.csrc);
+    }
+    for (;;) {
+        const Tile B = tile_of<DIR>(L, blk);
+        fill_lattice(T, R, P, B.miss);
+        __syncthreads();
+        const uint32_t nxt = blk + step;
+        if (nxt < end) {
+            uint64_t g, c;
+            uint32_t miss;
+            tile_origin(L, nxt, g, c, miss);
+            issue_lattice_loads(R, P, X, c, miss, L.csrc);
+        }
+        if (B.emit) {
+            tile_recover_pass<DIR, LINEAR, 0>(T, B, L, codes, outval, tid, err);
+            __syncthreads();
+            tile_recover_pass<DIR, LINEAR, 1>(T, B, L, codes, outval, tid, err);
+            __syncthreads();
+            tile_recover_pass<DIR, LINEAR, 2>(T, B, L, codes, outval, tid, err);
+            __syncthreads();
+            store_tile(T, P, X, B, L.dst);
+        } else if (tid == 0) {
+            atomicOr(err, 2u);
+        }
+        if (nxt >= end) break;
+        __syncthreads();
+        blk = nxt;
+    }
+}
+
+void launch_recover_tile3d(const Level3D &L, const uint16_t *d_codes, const float *d_outval, uint32_t *d_err,
+                           uint32_t grid, hipStream_t st) {
+    const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
+    if (!nblk || !grid) return;
+    dim3 g(std::min(grid, nblk)), block(kThreads);
+    int dir = 0;
+    for (int d = 0; d < 6; d++)
+        if (kPerm3[d][0] == L.p[0] && kPerm3[d][1] == L.p[1] && kPerm3[d][2] == L.p[2]) dir = d;
+#define FAZ_R3D(D)                                                                                                 \
+    do {                                                                                                           \
+        if (L.linear)                                                                                              \
+            hipLaunchKernelGGL((recover_tile3d_kernel<D, true>), g, block, 0, st, L, d_codes, d_outval, d_err);  \
+        else                                                                                                       \
+            hipLaunchKernelGGL((recover_tile3d_kernel<D, false>), g, block, 0, st, L, d_codes, d_outval, d_err); \
+    } while (0)
+    switch (dir) {
+        case 0: FAZ_R3D(0); break;
+        case 1: FAZ_R3D(1); break;
+        case 2: FAZ_R3D(2); break;
+        case 3: FAZ_R3D(3); break;
+        case 4: FAZ_R3D(4); break;
+        default: FAZ_R3D(5); break;
+    }
+#undef FAZ_R3D
 }
 
 bool tile3d_eligible(const Level3D &L) {

@@ -161,6 +161,13 @@ faz_status faz_stream_info(const uint8_t *stream, size_t n, int *N, size_t *dims
  * (HuffmanEncoder.hpp:236-296; SIC:42-199) on a pre-zstd engine buffer. */
 faz_status faz_gpu_interp_decompress_raw(faz_gpu_ctx *ctx, const uint8_t *h_raw, size_t n, int N, float *d_out,
                                          size_t num);
+/* The same decompression of an engine buffer already in device memory
+ * (d_raw, n bytes, any alignment): the header and the Huffman tree are read
+ * back to the host, the unpredictable list and the bitstream stay on the
+ * device.  Same SIC:42-199 / HUF:236-296 replacement for callers that keep
+ * streams resident in HBM. */
+faz_status faz_gpu_interp_decompress_dev(faz_gpu_ctx *ctx, const uint8_t *d_raw, size_t n, int N, float *d_out,
+                                         size_t num);
 
 /* --- A11..A16 on one GPU: the whole engine buffer that
  * SZInterpolationCompressor::compress hands to zstd (SIC:485-518): header,
@@ -540,9 +547,10 @@ faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const f
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
  * [3] host tree build, [4] encode, [5] device total, [6] dominant kernel
- * (interpolation pass launches) summed.  faz_gpu_interp_decompress_raw sets
- * [6] to its Huffman decode (tree parse, upload, device decode) and [7] to
- * the whole call, host clock. */
+ * (interpolation pass launches) summed.  faz_gpu_interp_decompress_raw and
+ * faz_gpu_interp_decompress_dev set [6] to their Huffman decode (tree parse,
+ * upload or device copy, device decode) and [7] to the whole call, host
+ * clock. */
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
 /* Per-launch HIP-event timing of the interpolation kernels, accumulated while
