@@ -107,6 +107,7 @@ struct faz_gpu_ctx {
     // FAZ_ENCODE=onepass selects the decoupled look-back encoder over the
     // sizes + scan + pack one (slower at 512^3: 0.41 ms against 0.20 ms)
     bool enc_onepass = false;
+    bool dec_run = true;  // the decoder skips the search inside the anchors' run of zero codes
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to

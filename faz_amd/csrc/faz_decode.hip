@@ -224,6 +224,51 @@ __device__ __forceinline__ uint32_t mask_below(const uint64_t *m, uint32_t q) {
     return (uint32_t) (__popcll(m[0]) + __popcll(m[1] & ((1ull << (q - 64)) - 1)));
 }
 
+// A stream that opens with `count` copies of one codeword (the anchors of an
+// interpolation engine buffer are its first codes, all 0): bits [0, bits)
+// repeat the len-bit codeword.  *ok is set by huff_run_check_kernel when the
+// stream really holds that run; chunks wholly inside it then need no
+// trajectory search (the codeword starts are the multiples of len) and no
+// decoding.
+struct RunPrefix {
+    uint64_t bits;       // 0: no run
+    uint32_t len;        // codeword length
+    uint32_t value;      // output value of the codeword
+    uint64_t code;       // codeword, right-aligned
+    const uint32_t *ok;  // device flag: 1 when the bits match
+};
+
+__device__ __forceinline__ bool run_chunk(const RunPrefix &run, uint64_t b) {
+    return run.bits && b + kChunkBits <= run.bits && *run.ok;
+}
+// codeword starts of the run below position p (p <= run.bits)
+__device__ __forceinline__ uint64_t run_starts_below(const RunPrefix &run, uint64_t p) {
+    return (p + run.len - 1) / run.len;
+}
+// first codeword start at or after p (p <= run.bits)
+__device__ __forceinline__ uint64_t run_next_start(const RunPrefix &run, uint64_t p) {
+    return run_starts_below(run, p) * run.len;
+}
+
+// Clears *ok when some bit of [0, run.bits) differs from the repeated
+// codeword; one thread per 32-bit stream word.
+__global__ void huff_run_check_kernel(const uint32_t *__restrict__ words, RunPrefix run, uint32_t *__restrict__ ok) {
+    const uint64_t w = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t p0 = w * 32;
+    if (p0 >= run.bits) return;
+    const uint32_t got = __builtin_bswap32(words[w]);
+    uint32_t want = 0, care = 0;
+    uint32_t ph = (uint32_t) (p0 % run.len);
+    for (uint32_t j = 0; j < 32; j++) {
+        if (p0 + j < run.bits) {
+            want |= (uint32_t) ((run.code >> (run.len - 1 - ph)) & 1u) << (31 - j);
+            care |= 1u << (31 - j);
+        }
+        ph = ph + 1 == run.len ? 0 : ph + 1;
+    }
+    if ((got ^ want) & care) atomicAnd(ok, 0u);
+}
+
 // Pass 1: for chunk c and every entry offset o < K, the next chunk's entry
 // offset (xmap, clamped to K - 1) and the symbols that start in the chunk
 // (nmap).  A trajectory merges with an earlier one when it reaches one of its
@@ -234,7 +279,8 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
                                                                const uint32_t *__restrict__ gtab,
                                                                const uint32_t *__restrict__ gctab, Tree T, uint32_t K,
                                                                uint8_t *__restrict__ xmap,
-                                                               uint16_t *__restrict__ nmap, uint32_t wg0) {
+                                                               uint16_t *__restrict__ nmap, uint32_t wg0,
+                                                               RunPrefix run) {
     __shared__ uint32_t tab[1 << kTabBits];
     __shared__ uint32_t ctab[1 << kTabBits];
     __shared__ uint32_t lw[kStageSlots];
@@ -256,6 +302,14 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
         xmap[(uint64_t) c * K + o] = (uint8_t) min<uint64_t>(e, K - 1);
         nmap[(uint64_t) c * K + o] = (uint16_t) n;
     };
+    if (run_chunk(run, b)) {
+        // inside the verified run every entry is mapped to the run's own
+        // exit: only the true entry is ever composed
+        const uint64_t ex = run_next_start(run, b + kChunkBits);
+        const uint32_t n = (uint32_t) (run_starts_below(run, b + kChunkBits) - run_starts_below(run, b));
+        for (uint32_t o = 0; o < K; o++) put(o, ex, n);
+        return;
+    }
     uint64_t mA[2] = {0, 0}, mB[2] = {0, 0};
     uint64_t xA, xB = 0;
     uint32_t nA, nB = 0;
@@ -421,7 +475,8 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
                                                                  const uint64_t *__restrict__ start,
                                                                  const uint64_t *__restrict__ offset,
                                                                  int32_t sym_offset, uint32_t limit, uint64_t num,
-                                                                 OUT *__restrict__ out, uint32_t *__restrict__ err) {
+                                                                 OUT *__restrict__ out, uint32_t *__restrict__ err,
+                                                                 RunPrefix run) {
     __shared__ uint32_t tab[MULTI ? 1 : (1 << kTabBits)];
     __shared__ uint64_t wtab[MULTI ? (1 << kTabBits) : 1];
     __shared__ uint32_t lw[kStageSlots];
@@ -465,6 +520,12 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
         }
     };
     bool bad = false;
+    if (run_chunk(run, (uint64_t) c * kChunkBits)) {
+        // the run's codewords: one value, no decoding
+        for (uint64_t k = run_starts_below(run, end) - run_starts_below(run, pos); k > 0 && o < num; k--)
+            push(run.value);
+        pos = end;
+    }
     while (pos < end && o < num) {
         if (MULTI && pos + kTabBits <= end) {
             r.refill();
@@ -589,10 +650,12 @@ extern "C" faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, si
 // ctx->stream.  Returns with the stream drained; *used = bytes of the Huffman
 // section.  d_buf, when given, is the same section in device memory: the
 // bitstream is then copied on the device (buf needs only the tree and the
-// byte count), otherwise it is uploaded from buf.
+// byte count), otherwise it is uploaded from buf.  run_count > 0 announces
+// that the symbols open with run_count copies of run_value (checked on the
+// device; a stream without the run decodes the general way).
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
-                       size_t *used, const uint8_t *d_buf) {
+                       size_t *used, const uint8_t *d_buf, uint32_t run_value, uint64_t run_count) {
     int32_t offset;
     std::vector<int32_t> left, right, sym;
     std::vector<uint8_t> leaf;
@@ -714,12 +777,47 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     HIPCHECK(ctx->dec_wtab.ensure(tab.size()));
     HIPCHECK(ctx->dec_leaf.ensure(nc));
     HIPCHECK(ctx->dec_pos.ensure(2 * (size_t) nchunks));
-    HIPCHECK(ctx->dec_cnt.ensure((size_t) nchunks + 2));
+    HIPCHECK(ctx->dec_cnt.ensure((size_t) nchunks + 3));
     HIPCHECK(ctx->dec_x.ensure(2 * nmaps));
     HIPCHECK(ctx->dec_g.ensure((size_t) ((nchunks + kDecThreads - 1) / kDecThreads) * (K + 1)));
     HIPCHECK(ctx->dec_n.ensure(nmaps));
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(ctx->scalars.ensure(4));
+    // the opening run: its codeword from the tree (a leaf whose value is
+    // run_value), checked against the stream before the map launch
+    RunPrefix run{0, 0, 0, 0, ctx->dec_cnt.p + nchunks + 2};
+    if (run_count && K <= 64) {
+        std::vector<int32_t> parent(nc, -1);
+        for (size_t i = 0; i < nc; i++) {
+            if (leaf[i]) continue;
+            if (left[i] >= 0) parent[left[i]] = (int32_t) i;
+            if (right[i] >= 0) parent[right[i]] = (int32_t) i;
+        }
+        for (size_t i = 0; i < nc; i++) {
+            if (!leaf[i] || (uint32_t) (sym[i] + offset) != run_value) continue;
+            // the codeword read from the leaf up to the root (its last bit first)
+            uint64_t code = 0;
+            uint32_t len = 0;
+            bool path = true;
+            for (int32_t nd = (int32_t) i; nd != 0;) {
+                const int32_t pa = parent[nd];
+                if (pa < 0 || len >= 64) {
+                    path = false;
+                    break;
+                }
+                code |= (uint64_t) (right[pa] == nd) << len;
+                len++;
+                nd = pa;
+            }
+            if (path && len >= 1 && run_count <= nbits / len && (!limit || run_value < limit)) {
+                run.bits = run_count * len;
+                run.len = len;
+                run.value = run_value;
+                run.code = code;
+            }
+            break;
+        }
+    }
     // the stream (its last word zero-padded) goes up on the upload stream in
     // parts, below, so that the map launches overlap the later parts' copies
     hipStream_t us = ctx->ustream;
@@ -764,9 +862,15 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
                 HIPCHECK(hipEventRecord(ctx->dec_ev[k], us));
                 HIPCHECK(hipStreamWaitEvent(st, ctx->dec_ev[k], 0));
             }
+            if (k == 0 && run.bits) {
+                launch_fill_u32(const_cast<uint32_t *>(run.ok), 1, 1u, st);
+                const uint64_t rw = (run.bits + 31) / 32;
+                hipLaunchKernelGGL(huff_run_check_kernel, dim3((unsigned) ((rw + 255) / 256)), dim3(256), 0, st,
+                                   ctx->dec_words.p, run, const_cast<uint32_t *>(run.ok));
+            }
             if (w1 > w0)
                 hipLaunchKernelGGL(huff_map_kernel, dim3(w1 - w0), block, 0, st, ctx->dec_words.p, nwords, nbits,
-                                   nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0);
+                                   nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0, run);
             HIPCHECK(hipGetLastError());
         }
     }
@@ -785,7 +889,8 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     launch_zero_scan(d_cnt, d_off, nchunks, ctx->tiles.p, ctx->scalars.p, st);
     launch_fill_u32(d_err, 1, 0u, st);
     hipLaunchKernelGGL((huff_write_kernel<OUT, kMulti>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks,
-                       d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err);
+                       d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err,
+                       run);
     HIPCHECK(hipGetLastError());
     uint32_t *hf = ctx->hp_hist.data() + 2 * 65536;  // pinned scalars
     HIPCHECK(hipMemcpyAsync(hf, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
@@ -800,11 +905,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
 }
 
 template int gpu_huffman_decode<uint16_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint16_t *,
-                                          size_t *, const uint8_t *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t);
 template int gpu_huffman_decode<int32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, int32_t *,
-                                          size_t *, const uint8_t *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t);
 template int gpu_huffman_decode<uint32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint32_t *,
-                                          size_t *, const uint8_t *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t);
 
 // ---------------------------------------------------------------------------
 // Unpredictable values of 32-bit code streams (the general frontend): the

@@ -407,6 +407,10 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     if (kern && strcmp(kern, "tile") == 0) ctx->fused_min_blocks = 1;
     const char *enc = getenv("FAZ_ENCODE");
     if (enc && strcmp(enc, "onepass") == 0) ctx->enc_onepass = true;
+    // FAZ_DECODE_RUN=off: the decoder searches the anchors' run of zero codes
+    // like any other part of the bitstream
+    const char *drun = getenv("FAZ_DECODE_RUN");
+    if (drun && strcmp(drun, "off") == 0) ctx->dec_run = false;
     // the allocation-free encoder's scratch address on this device (its
     // entry point reports a failure here)
     (void) noalloc_init(device);
@@ -1312,7 +1316,9 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     HIPCHECK(ctx->err_flag.ensure(1));
     // codes decoded on the device (HUF:236-268), then the outliers' ranks
     const auto td = std::chrono::steady_clock::now();
-    int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff);
+    // with anchors the codes open with anchor_count zeros (SIC:854-900)
+    int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff, 0u,
+                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0);
     if (drc) return drc;
     ctx->timings[6] = ms_since(td);
     if (U) {

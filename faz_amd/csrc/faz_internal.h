@@ -203,9 +203,12 @@ void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unp
 // d_out on ctx->stream, each symbol + the tree's offset checked to lie in
 // [0, limit); returns a faz_status with the stream drained.  With d_buf (the
 // section in device memory) the host buf needs only the tree and byte count.
+// run_count > 0: the symbols are expected to open with run_count copies of
+// run_value (checked on the device, used to skip their decoding).
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
-                       size_t *used, const uint8_t *d_buf = nullptr);
+                       size_t *used, const uint8_t *d_buf = nullptr, uint32_t run_value = 0,
+                       uint64_t run_count = 0);
 
 // outval[i] = the unpredictable value of zero code i, for n 32-bit codes on
 // the device (faz_decode.hip); uploads the U host values first.

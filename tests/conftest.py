@@ -66,3 +66,12 @@ def engine_tile():
     e = _engine_with("tile")
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_norun():
+    """A context whose decoder treats the anchors' opening run of zero codes
+    like the rest of the bitstream (no run check, no shortcut)."""
+    e = _engine_with("off", "FAZ_DECODE_RUN")
+    yield e
+    e.close()

@@ -98,8 +98,53 @@ def test_truncated_device_buffer_is_an_error(engine_tile):
     assert np.array_equal(out.cpu().numpy().view(np.uint32), want.view(np.uint32))
 
 
+def _bitstream_at(raw, N=3, levelwise=0):
+    """Offset of the Huffman bitstream in an engine buffer (header SIC:489-512,
+    LinearQuantizer::save Q:185-201, HuffmanEncoder::save HUF:119-137)."""
+    import struct
+    h = N * 8 + 61 + 2 * levelwise
+    U = struct.unpack_from("<Q", raw, h + 13)[0]
+    t = h + 21 + 4 * U
+    nc = int.from_bytes(raw[t + 4:t + 8], "big")
+    w = 1 if nc <= 256 else (2 if nc <= 65536 else 4)
+    at = t + 12 + 1 + 2 * nc * w + 5 * nc + 8
+    assert struct.unpack_from("<Q", raw, at - 8)[0] == len(raw) - at
+    return at
+
+
+def test_anchor_run_shortcut_agrees_with_full_search(engine_tile, engine_norun):
+    """The decoder skips the trajectory search inside the anchors' opening
+    run of zero codes once a device check has matched the run's bits; with a
+    bit of that run flipped the check fails and the decoder must behave as
+    the full search does (same output or the same error)."""
+    import faz_amd
+    shape = (129, 130, 131)  # 5^3 anchors at maxStep 32: about two chunks of run
+    x = _field(shape, 952)
+    eb = 1e-4 * float(np.float32(x.max() - x.min()))
+    op = oracle.make_params(shape, eb, algo=1, max_step=32)
+    raw = bytearray(oracle.interp_raw(x, op))
+    want = oracle.interp_compress(x, op)["recon"].view(np.uint32)
+    for eng in (engine_tile, engine_norun):
+        out = torch.empty(x.size, dtype=torch.float32, device="cuda")
+        eng.decompress_dev(torch.frombuffer(bytearray(raw), dtype=torch.uint8).cuda(), 3, out)
+        assert np.array_equal(out.cpu().numpy().view(np.uint32), want)
+    bits_at = _bitstream_at(raw)
+    raw[bits_at + 40] ^= 0x10  # inside the anchors' run
+    results = []
+    for eng in (engine_tile, engine_norun):
+        out = torch.zeros(x.size, dtype=torch.float32, device="cuda")
+        try:
+            eng.decompress_dev(torch.frombuffer(bytearray(raw), dtype=torch.uint8).cuda(), 3, out)
+            results.append(("ok", out.cpu().numpy().view(np.uint32).copy()))
+        except faz_amd.FazError as e:
+            results.append(("error", str(e).split(":")[0]))
+    assert results[0][0] == results[1][0]
+    if results[0][0] == "ok":
+        assert np.array_equal(results[0][1], results[1][1])
+
+
 @pytest.mark.parametrize("rel", [1e-3, 1e-5])
-def test_full_size_512_device_decode(engine, rel):
+def test_full_size_512_device_decode(engine, engine_norun, rel):
     """512^3: the device-resident decoder reproduces the compression's
     reconstruction bit for bit (whole-tile decoder on levels 3-1)."""
     import bench
@@ -114,6 +159,9 @@ def test_full_size_512_device_decode(engine, rel):
     d_raw = torch.frombuffer(bytearray(raw), dtype=torch.uint8).cuda()
     out = torch.empty_like(x)
     engine.decompress_dev(d_raw, 3, out)
+    assert torch.equal(out.view(torch.int32), recon.view(torch.int32))
+    out.fill_(float("nan"))
+    engine_norun.decompress_raw(raw, 3, out)
     assert torch.equal(out.view(torch.int32), recon.view(torch.int32))
     del x, recon, out, d_raw
     torch.cuda.empty_cache()
