@@ -676,8 +676,8 @@ class Engine:
         return ms.value, la.value, pts.value, name, self._L.faz_gpu_profile_bytes(self.h)
 
     def timings(self):
-        t = (ctypes.c_double * 8)()
-        self._L.faz_gpu_last_timings(self.h, t, 8)
+        t = (ctypes.c_double * 10)()
+        self._L.faz_gpu_last_timings(self.h, t, 10)
         return list(t)
 
     # --- 3-D Lorenzo/regression engine -------------------------------------

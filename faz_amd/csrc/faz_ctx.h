@@ -13,6 +13,7 @@
 
 #include "faz_gpu.h"
 #include "../host/engine_format.h"
+#include "faz_internal.h"
 #include "../host/huffman_host.h"
 
 namespace faz {
@@ -36,6 +37,40 @@ struct HostPinned {
     }
     HostPinned(const HostPinned &) = delete;
     HostPinned &operator=(const HostPinned &) = delete;
+    T *data() { return p; }
+};
+
+// Page-locked host array that grows on demand (contents not kept across
+// growth); a pageable source would make hipMemcpyAsync wait for its copy.
+template<class T>
+struct PinnedVec {
+    T *p = nullptr;
+    size_t cap = 0;
+    bool pinned = false;
+    T *ensure(size_t n) {
+        if (n <= cap && p) return p;
+        release();
+        const size_t want = std::max<size_t>(n, 1);
+        if (hipHostMalloc((void **) &p, want * sizeof(T), 0) == hipSuccess) {
+            pinned = true;
+        } else {
+            (void) hipGetLastError();
+            p = new T[want];
+            pinned = false;
+        }
+        cap = want;
+        return p;
+    }
+    void release() {
+        if (p) {
+            if (pinned) (void) hipHostFree(p);
+            else delete[] p;
+        }
+        p = nullptr;
+        cap = 0;
+    }
+    ~PinnedVec() { release(); }
+    T &operator[](size_t i) { return p[i]; }
     T *data() { return p; }
 };
 
@@ -108,6 +143,8 @@ struct faz_gpu_ctx {
     // sizes + scan + pack one (slower at 512^3: 0.41 ms against 0.20 ms)
     bool enc_onepass = false;
     bool dec_run = true;  // the decoder skips the search inside the anchors' run of zero codes
+    bool dev_bounds = true;  // REL compressions compute the level bounds on the device
+    faz::DevBuf<faz::LevelEB> lvl;  // level bounds of the last REL compression (entry 0: the base bound)
     faz::DevBuf<uint64_t> chunk_base, scalars, prefix, chunk_bits, chunk_off, enc_code, tiles;
     uint64_t prefix_key = 0;  // identity of the block prefix table in `prefix`
     const uint64_t *prefix_at = nullptr;  // the allocation that table was uploaded to
@@ -117,14 +154,14 @@ struct faz_gpu_ctx {
     // decoder of a device-resident buffer: its head and its Huffman tree on the host
     std::vector<uint8_t> h_stage, h_stage2;
     hipEvent_t ev[8] = {};
-    double timings[8] = {};
+    double timings[10] = {};
     // histogram of the codes last produced by interp_compress
     const uint16_t *hist_codes = nullptr;
     size_t hist_n = 0;
     std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
     // pinned: histogram and first positions, then 8 words of scalars
     faz::HostPinned<uint32_t> hp_hist{2 * 65536 + 8};
-    std::vector<uint64_t> h_tab;  // packed code table of the last encode
+    faz::PinnedVec<uint64_t> h_tab;  // packed code table of the last encode (pinned: asynchronous upload)
     faz::PendingRaw pending;
     // per-launch timing of the interpolation pass kernels
     bool profile = false;

@@ -411,6 +411,10 @@ faz_status faz_gpu_ctx_create(int device, faz_gpu_ctx **out) {
     // like any other part of the bitstream
     const char *drun = getenv("FAZ_DECODE_RUN");
     if (drun && strcmp(drun, "off") == 0) ctx->dec_run = false;
+    // FAZ_DEVICE_BOUNDS=off: the REL compression reads the value range back
+    // and sets the level bounds on the host
+    const char *dbnd = getenv("FAZ_DEVICE_BOUNDS");
+    if (dbnd && strcmp(dbnd, "off") == 0) ctx->dev_bounds = false;
     // the allocation-free encoder's scratch address on this device (its
     // entry point reports a failure here)
     (void) noalloc_init(device);
@@ -486,6 +490,7 @@ void faz_gpu_ctx_destroy(faz_gpu_ctx *ctx) {
     ctx->gf_reg_deq.release();
     ctx->gf_reg_code.release();
     ctx->enc_head.release();
+    ctx->lvl.release();
     if (ctx->tune_cache_free) ctx->tune_cache_free(ctx->tune_cache);
     ctx->tune_cache = nullptr;
     for (auto &e: ctx->ev)
@@ -504,7 +509,7 @@ int faz_gpu_ctx_device(const faz_gpu_ctx *ctx) { return ctx ? ctx->device : -1; 
 
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n) {
     if (!ctx || !ms) return FAZ_ERR_INVALID;
-    for (int i = 0; i < n && i < 8; i++) ms[i] = ctx->timings[i];
+    for (int i = 0; i < n && i < 10; i++) ms[i] = ctx->timings[i];
     return FAZ_OK;
 }
 
@@ -614,6 +619,32 @@ static hipError_t level_kernels(faz_gpu_ctx *ctx, const Plan &pl, const faz_inte
     return hipSuccess;
 }
 
+// The level schedule of a plan as factors of the base bound (the arithmetic
+// of make_plan, SIC:384-422), for launch_level_bounds.
+static LevelScale level_scale(const Plan &pl, const faz_interp_params *p) {
+    LevelScale sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.n = (int) std::min<size_t>(pl.lv.size(), kMaxLevels);
+    for (int i = 0; i < sc.n; i++) {
+        const unsigned level = pl.lv[i].level;
+        if (p->alpha < 0) {
+            sc.div[i] = 0;
+            sc.f[i] = level >= 3 ? 0.5 : 1.0;
+        } else if (p->alpha >= 1) {
+            double r = std::pow(p->alpha, level - 1);
+            if (r > p->beta) r = p->beta;
+            sc.div[i] = 1;
+            sc.f[i] = r;
+        } else {
+            double r = 1 - (level - 1) * p->alpha;
+            if (r < p->beta) r = p->beta;
+            sc.div[i] = 0;
+            sc.f[i] = r;
+        }
+    }
+    return sc;
+}
+
 // Interpolation + quantization of one field.  d_recon is a full-size working
 // array that receives every reconstructed point when final_recon is set;
 // otherwise it only carries the coarse levels (the compression itself needs
@@ -627,9 +658,11 @@ static hipError_t level_kernels(faz_gpu_ctx *ctx, const Plan &pl, const faz_inte
 // defer: no host synchronisation either, but allocations and the prefix
 // upload are allowed; the caller reads the outlier count and the error flag
 // after its own wait and computes timings[1..2] from ev[0..2].
+// d_lvl: the bounds of the first point (entry 0) and of every level (1 + i)
+// on the device (launch_level_bounds); p->eb is then not used.
 static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
                            faz_interp_out *out, bool final_recon, double absmax, bool enqueue = false,
-                           bool defer = false) {
+                           bool defer = false, const LevelEB *d_lvl = nullptr) {
     int rc = FAZ_OK;
     Plan &pl = cached_plan(ctx, p, rc);
     if (rc) return rc;
@@ -666,7 +699,8 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
                            (uint32_t) p->max_step, pl.anchor_count, st);
     } else {
         double eb0 = p->eb;
-        launch_first_point(d_in, d_recon, ctx->codes.p, ctx->outval.p, eb0, 1.0 / eb0, p->radius, ctx->err_flag.p, st);
+        launch_first_point(d_in, d_recon, ctx->codes.p, ctx->outval.p, eb0, 1.0 / eb0, p->radius, ctx->err_flag.p, st,
+                           d_lvl);
     }
     size_t npass = 0;
     for (auto &L: pl.lv) npass += L.passes.size();
@@ -685,7 +719,9 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     // max|x| + eb with coefficient sum at most 28/8 (quad_3), so
     // |x - pred| <= 4.5 max|x| + 3.5 eb; half of the range is kept as margin
     // for rounding.  A non-finite or huge max |x| keeps the exact quantizer.
-    if (std::find(kind.begin(), kind.end(), 2) != kind.end()) {
+    if (d_lvl) {
+        for (size_t i = 0; i < nlv; i++) g3[i].ebp = d_lvl + 1 + i;
+    } else if (std::find(kind.begin(), kind.end(), 2) != kind.end()) {
         if (!(absmax >= 0)) {  // never in enqueue mode (checked above)
             float mn = 0, mx = 0;
             int rr = faz_gpu_range_f32(ctx, d_in, pl.num, &mn, &mx);
@@ -719,6 +755,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         }
         for (auto &d: L.passes) {
             d.block_prefix = ctx->prefix.p + L.prefix_at;
+            d.ebp = d_lvl ? d_lvl + 1 + i : nullptr;
             if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
             launch_interp_pass(d, d_in, d_recon, ctx->codes.p, ctx->outval.p, nullptr, ctx->err_flag.p, st);
             if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
@@ -904,7 +941,7 @@ static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow 
         const uint32_t L = t->len[k];
         return L ? ((t->code[k] >> (64 - L)) << 8) | L : (uint64_t) 0;
     };
-    ctx->h_tab.resize((size_t) ns);
+    ctx->h_tab.ensure((size_t) ns);
     ew = EncodeWindow{0, 0, 0};
     ew.max_len = maxlen;
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
@@ -979,7 +1016,7 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     hipStream_t st = ctx->stream;
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     // one packed entry per symbol: codeword right-aligned above an 8-bit length
-    ctx->h_tab.resize((size_t) t->n_states);
+    ctx->h_tab.ensure((size_t) t->n_states);
     int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
     uint32_t maxlen = 0;
     for (int32_t i = 0; i < t->n_states; i++) {
@@ -1038,7 +1075,7 @@ void faz_ctx_set_error(faz_gpu_ctx *ctx, const char *msg) { ctx->err = msg; }
 // Device part of the engine buffer: interpolation + quantization, histogram
 // and outliers, host tree, encode.  Results stay in the context until fetched.
 static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
-                           double absmax = -1.0) {
+                           double absmax = -1.0, const LevelEB *d_lvl = nullptr) {
     if (!ctx || !d_in || !p) return FAZ_ERR_INVALID;
     ctx->pending.valid = false;
     HIPCHECK(hipSetDevice(ctx->device));
@@ -1053,7 +1090,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     // one wait after the histogram (counts, first positions, outlier count
     // and error flag through pinned memory), one after the encode
     faz_interp_out io;
-    int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax, false, true);
+    int rc = interp_compress(ctx, d_in, recon, p, &io, d_recon != nullptr, absmax, false, true, d_lvl);
     if (rc) return rc;
     // one readback: the histogram's window around the radius, symbol 0, the
     // outside-window flag, the error flag and the outlier count
@@ -1061,12 +1098,20 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     HIPCHECK(ctx->hist_pack.ensure(kHistPackWords));
     launch_hist_pack(ctx->freq.p, ctx->first.p, p->radius, ctx->scalars.p, ctx->err_flag.p, ctx->hist_pack.p, st);
     HIPCHECK(hipMemcpyAsync(hf, ctx->hist_pack.p, kHistPackWords * 4, hipMemcpyDeviceToHost, st));
+    // the base bound computed on the device comes back with it
+    if (d_lvl) HIPCHECK(hipMemcpyAsync(hf + kHistPackWords, &d_lvl->eb, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
     // the outlier gather runs while the host waits for the counts and builds the tree
     launch_unpred_gather(io.d_codes, io.n_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p,
                          st);
     HIPCHECK(hipGetLastError());
     HIPCHECK(spin_wait_event(ctx->ev[3]));
+    const auto th0 = std::chrono::steady_clock::now();
+    double eb = p->eb;
+    if (d_lvl) {
+        memcpy(&eb, hf + kHistPackWords, 8);
+        if (!(eb > 0)) CTX_FAIL(FAZ_ERR_INVALID, "error bound must be positive");
+    }
     const uint32_t errflag = hf[kHistPackFreq0 + 3];
     if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
@@ -1103,10 +1148,14 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             for (uint32_t i = 0; i < 65536; i++) add(i, hf[i]);
         }
     }
+    const auto th1 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
     rc = encode_enqueue(ctx, io.d_codes, io.n_codes, &ht);
     if (rc) return rc;
+    const auto th2 = std::chrono::steady_clock::now();
+    ctx->timings[8] = std::chrono::duration<double, std::milli>(th1 - th0).count();
+    ctx->timings[9] = std::chrono::duration<double, std::milli>(th2 - th1).count();
     HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hs + 6, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
@@ -1139,7 +1188,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     h.levelwise = p->levelwise;
     h.ops.assign(p->ops, p->ops + p->levelwise);
     h.dirs.assign(p->dirs, p->dirs + p->levelwise);
-    pr.eb = p->eb;
+    pr.eb = eb;
     pr.radius = p->radius;
     pr.n_unpred = io.n_unpred;
     pr.nbytes = nbytes;
@@ -1212,6 +1261,26 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         launch_range(d_in, num, ctx->minmax.p, ctx->stream);
         launch_hist_clear(ctx->freq.p, ctx->first.p, ctx->stream);
         HIPCHECK(hipGetLastError());
+        if (ctx->dev_bounds) {
+            // the level bounds follow from the range on the device: no host
+            // round trip before the level launches (the plan does not depend
+            // on the bound; 1.0 stands in for it)
+            q.eb = 1.0;
+            int rc = FAZ_OK;
+            const Plan &pl = cached_plan(ctx, &q, rc);
+            if (rc) return rc;
+            HIPCHECK(ctx->lvl.ensure(kMaxLevels + 1));
+            launch_level_bounds(ctx->minmax.p, eb, level_scale(pl, &q), ctx->lvl.p, ctx->stream);
+            HIPCHECK(hipGetLastError());
+            ctx->hist_cleared = true;
+            HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
+            rc = compress_device(ctx, d_in, d_recon, &q, -1.0, ctx->lvl.p);
+            ctx->hist_cleared = false;
+            if (rc) return rc;
+            ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
+            *raw_size = pending_size(ctx);
+            return FAZ_OK;
+        }
         int rc = range_readback(ctx, &mn, &mx);
         if (rc) return rc;
         ctx->hist_cleared = true;

@@ -45,6 +45,22 @@ struct FastDiv {
     }
 };
 
+// Error bound of one level computed on the device from the value range (the
+// REL path without a host round trip after the range pass).
+struct LevelEB {
+    double eb, recip;  // level error bound and 1.0 / eb
+    float ebf;         // largest float <= eb
+    int bounded;       // |x - pred| / eb provably below 2^30 (see interp_compress)
+};
+// How a plan's levels scale the base bound: level i (coarse to fine) has
+// base * f[i] (div[i] = 0) or base / f[i] (div[i] = 1), the arithmetic of
+// the level schedule (SIC:384-422) with its factors computed on the host.
+struct LevelScale {
+    int n;
+    uint8_t div[kMaxLevels];
+    double f[kMaxLevels];
+};
+
 // One (level, pass) launch of the interpolation predictor.
 struct PassDesc {
     int N;
@@ -69,6 +85,7 @@ struct PassDesc {
     int radius;
     int linear;               // 1: linear interpolator, 0: cubic
     int tuning;               // 0 / 1 / 2
+    const LevelEB *ebp;       // when set, eb and recip come from here (device)
 };
 
 // One level of the 3-D engine as a single launch: one workgroup per block.
@@ -95,6 +112,7 @@ struct Level3D {
     uint64_t cstr[3];
     float *dst;
     uint64_t dstr[3];
+    const LevelEB *ebp;       // when set, eb, recip, ebf and bounded come from here (device)
 };
 
 // Largest float not above a positive double.
@@ -134,12 +152,17 @@ void launch_interp_level3d(const Level3D &L, const float *d_in, float *d_recon, 
 constexpr unsigned kRangeGrid = 512;
 constexpr size_t kRangeScratch = 2 + 2 * (size_t) kRangeGrid;
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st);
+// After launch_range on the same stream: d_out[0] = the base bound
+// rel * (float) (max - min), d_out[1 + i] = level i of the plan (sc), with
+// the bounded-quantizer test of each level.
+void launch_level_bounds(const uint32_t *d_minmax, double rel, const LevelScale &sc, LevelEB *d_out,
+                         hipStream_t st);
 void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
                         const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st);
 void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint32_t *dims, const uint64_t *offs,
                            uint32_t step, uint64_t count, hipStream_t st);
 void launch_first_point(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, double eb,
-                        double recip, int radius, uint32_t *d_err, hipStream_t st);
+                        double recip, int radius, uint32_t *d_err, hipStream_t st, const LevelEB *d_eb = nullptr);
 // out[0] of a decoder without anchors: code 0 recovered from prediction 0, or
 // the first unpredictable value.
 void launch_first_value(const uint16_t *d_codes, const float *d_unpred, uint64_t n_unpred, int radius, double eb,

@@ -160,7 +160,8 @@ __global__ void __launch_bounds__(256) interp_pass_kernel(PassDesc d, const floa
     float pred = predict(recon, f, S, ps, d.linear);
     float orig = in[f];
     float rec;
-    int code = quantize_point(orig, pred, d.eb, d.recip, d.radius, &rec);
+    const double eb = d.ebp ? d.ebp->eb : d.eb, recip = d.ebp ? d.ebp->recip : d.recip;
+    int code = quantize_point(orig, pred, eb, recip, d.radius, &rec);
     recon[f] = rec;
     if ((unsigned) code > 65535u) atomicOr(err, 1u);
     codes[ps.seq] = (uint16_t) code;
@@ -290,7 +291,11 @@ void launch_first_value(const uint16_t *d_codes, const float *d_unpred, uint64_t
 }
 
 __global__ void first_point_kernel(const float *in, float *recon, uint16_t *codes, float *outval, double eb,
-                                   double recip, int radius, uint32_t *err) {
+                                   double recip, int radius, uint32_t *err, const LevelEB *ebp) {
+    if (ebp) {
+        eb = ebp->eb;
+        recip = ebp->recip;
+    }
     float rec;
     int code = quantize_point(in[0], 0.0f, eb, recip, radius, &rec);
     recon[0] = rec;
@@ -300,9 +305,9 @@ __global__ void first_point_kernel(const float *in, float *recon, uint16_t *code
 }
 
 void launch_first_point(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, double eb, double recip,
-                        int radius, uint32_t *d_err, hipStream_t st) {
+                        int radius, uint32_t *d_err, hipStream_t st, const LevelEB *d_eb) {
     hipLaunchKernelGGL(first_point_kernel, dim3(1), dim3(1), 0, st, d_in, d_recon, d_codes, d_outval, eb, recip, radius,
-                       d_err);
+                       d_err, d_eb);
 }
 
 // ---------------------------------------------------------------------------
@@ -395,6 +400,42 @@ __global__ void __launch_bounds__(256) range_fold_kernel(const uint32_t *__restr
         minmax[0] = lo;
         minmax[1] = hi;
     }
+}
+
+// The range's keys back to floats (the inverse of f2key).
+__device__ __forceinline__ float key2f(uint32_t key) {
+    return __uint_as_float((key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key);
+}
+
+// Level bounds from the folded range, in the host's arithmetic: the base
+// bound rel * (double) (float) (max - min) (QoZ::data_range and the REL
+// conversion of SZ_compress), each level's bound from it (SIC:384-422), the
+// reciprocal, the largest float not above the bound, and the bounded
+// quantizer's test 4.5 max|x| / eb + 8 < 2^30.
+__global__ void level_bounds_kernel(const uint32_t *__restrict__ minmax, double rel, LevelScale sc,
+                                    LevelEB *__restrict__ out) {
+    const int i = (int) threadIdx.x - 1;  // -1: the base bound
+    if (i >= sc.n) return;
+    const float mn = key2f(minmax[0]), mx = key2f(minmax[1]);
+    const double base = rel * (double) (float) (mx - mn);
+    const double amn = fabs((double) mn), amx = fabs((double) mx);
+    const double absmax = amn < amx ? amx : amn;  // std::max, NaN included
+    const double eb = i < 0 ? base : (sc.div[i] ? base / sc.f[i] : base * sc.f[i]);
+    const double recip = 1.0 / eb;
+    float ebf = (float) eb;
+    if ((double) ebf > eb) ebf = __uint_as_float(__float_as_uint(ebf) - 1);  // next float toward 0 (ebf > 0)
+    const double v = 4.5 * absmax * recip + 8.0;
+    LevelEB r;
+    r.eb = eb;
+    r.recip = recip;
+    r.ebf = ebf;
+    r.bounded = isfinite(absmax) && absmax < 1e37 && v < 1073741824.0 ? 1 : 0;
+    out[i + 1] = r;
+}
+
+void launch_level_bounds(const uint32_t *d_minmax, double rel, const LevelScale &sc, LevelEB *d_out,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(level_bounds_kernel, dim3(1), dim3(kMaxLevels + 1), 0, st, d_minmax, rel, sc, d_out);
 }
 
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st) {

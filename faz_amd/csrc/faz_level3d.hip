@@ -153,6 +153,10 @@ __global__ void __launch_bounds__(kThreads) interp_level3d_kernel(Level3D L, con
                                                                   uint16_t *__restrict__ codes,
                                                                   float *__restrict__ outval,
                                                                   uint32_t *__restrict__ err) {
+    if (L.ebp) {
+        L.eb = L.ebp->eb;
+        L.recip = L.ebp->recip;
+    }
     __shared__ float E[kEMax];
     __shared__ uint32_t s_codes32[(kCodeMax + 1) / 2];
     uint16_t *s_codes = reinterpret_cast<uint16_t *>(s_codes32);

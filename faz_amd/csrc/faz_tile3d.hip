@@ -538,11 +538,9 @@ __device__ __forceinline__ void fill_lattice(float *T, const Pre &R, const Place
 }  // namespace
 
 template<int DIR, bool LINEAR, bool BOUNDED>
-__global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, const float *__restrict__ in,
-                                                                 uint16_t *__restrict__ codes,
-                                                                 float *__restrict__ outval,
-                                                                 uint32_t *__restrict__ err) {
-    __shared__ float T[kTileFloats];
+__device__ __forceinline__ void tile3d_compress(float *T, const Level3D &L, const float *__restrict__ in,
+                                                uint16_t *__restrict__ codes, float *__restrict__ outval,
+                                                uint32_t *__restrict__ err) {
     const uint32_t tid = threadIdx.x;
     const uint32_t nblk = L.nbw[0] * L.nbw[1] * L.nbw[2];
     // blocks of one XCD (workgroup id mod 8) form a contiguous raster range,
@@ -607,6 +605,28 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
         blk = nxt;
     }
     if (!BOUNDED && wide) atomicOr(err, 1u);
+}
+
+// MODE 0: the exact quantizer, 1: the bounded one, 2: the level's bounds and
+// the choice between the two read from L.ebp (computed on the device).
+template<int DIR, bool LINEAR, int MODE>
+__global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, const float *__restrict__ in,
+                                                                 uint16_t *__restrict__ codes,
+                                                                 float *__restrict__ outval,
+                                                                 uint32_t *__restrict__ err) {
+    __shared__ float T[kTileFloats];
+    if (MODE == 2) {
+        const LevelEB e = *L.ebp;
+        L.eb = e.eb;
+        L.recip = e.recip;
+        L.ebf = e.ebf;
+        if (e.bounded)
+            tile3d_compress<DIR, LINEAR, true>(T, L, in, codes, outval, err);
+        else
+            tile3d_compress<DIR, LINEAR, false>(T, L, in, codes, outval, err);
+    } else {
+        tile3d_compress<DIR, LINEAR, MODE == 1>(T, L, in, codes, outval, err);
+    }
 }
 
 // The decoder of one whole-tile level (SZInterpolationCompressor::decompress
@@ -725,12 +745,15 @@ void launch_interp_tile3d(const Level3D &L, const float *d_in, uint16_t *d_codes
         if (kPerm3[d][0] == L.p[0] && kPerm3[d][1] == L.p[1] && kPerm3[d][2] == L.p[2]) dir = d;
 #define FAZ_T3D_B(D, LIN)                                                                                        \
     do {                                                                                                         \
-        if (L.bounded)                                                                                           \
-            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, true>), g, block, 0, st, L, d_in, d_codes, d_outval, \
+        if (L.ebp)                                                                                               \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, 2>), g, block, 0, st, L, d_in, d_codes, d_outval,    \
+                               d_err);                                                                           \
+        else if (L.bounded)                                                                                      \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, 1>), g, block, 0, st, L, d_in, d_codes, d_outval,    \
                                d_err);                                                                           \
         else                                                                                                     \
-            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, false>), g, block, 0, st, L, d_in, d_codes,         \
-                               d_outval, d_err);                                                                 \
+            hipLaunchKernelGGL((interp_tile3d_kernel<D, LIN, 0>), g, block, 0, st, L, d_in, d_codes, d_outval,    \
+                               d_err);                                                                           \
     } while (0)
 #define FAZ_T3D(D)                 \
     do {                           \

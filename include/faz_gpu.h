@@ -547,10 +547,11 @@ faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const f
  * milliseconds, measured with HIP events on the context stream:
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
  * [3] host tree build, [4] encode, [5] device total, [6] dominant kernel
- * (interpolation pass launches) summed.  faz_gpu_interp_decompress_raw and
- * faz_gpu_interp_decompress_dev set [6] to their Huffman decode (tree parse,
- * upload or device copy, device decode) and [7] to the whole call, host
- * clock. */
+ * (interpolation pass launches) summed, [8] host time from the histogram's
+ * arrival to the Huffman tree, [9] host time enqueueing the encode.
+ * faz_gpu_interp_decompress_raw and faz_gpu_interp_decompress_dev set [6] to
+ * their Huffman decode (tree parse, upload or device copy, device decode) and
+ * [7] to the whole call, host clock. */
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
 /* Per-launch HIP-event timing of the interpolation kernels, accumulated while

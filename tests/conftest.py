@@ -75,3 +75,12 @@ def engine_norun():
     e = _engine_with("off", "FAZ_DECODE_RUN")
     yield e
     e.close()
+
+
+@pytest.fixture(scope="session")
+def engine_hostbounds():
+    """A context whose REL compressions read the value range back and set the
+    level bounds on the host."""
+    e = _engine_with("off", "FAZ_DEVICE_BOUNDS")
+    yield e
+    e.close()

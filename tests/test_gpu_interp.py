@@ -219,6 +219,50 @@ def test_codes_outside_histogram_window(engine_tile, rel):
     assert engine_tile.fetch_raw(size) == oracle.interp_raw(y, oracle.make_params(shape, eb, **params))
 
 
+REL_CASES = [
+    ((64, 65, 96), dict(algo=1, direction=0, max_step=32), 1e-3),
+    ((97, 64, 33), dict(algo=0, direction=3, max_step=32), 1e-4),
+    ((128, 96, 65), dict(algo=1, direction=5, max_step=32, alpha=1.5, beta=2.0), 1e-3),
+    ((50, 45, 40), dict(algo=1, direction=2, alpha=0.5, beta=0.75, ops=(1, 0, 1), dirs=(0, 5, 3)), 1e-2),
+    ((96, 64, 64), dict(algo=1, max_step=32, alpha=1.25, beta=1.5), 1e-5),
+    ((300, 421), dict(algo=1, direction=1, max_step=64), 1e-3),
+    ((100000,), dict(algo=1), 1e-4),
+    ((64, 65, 64), dict(algo=1, direction=2, max_step=32), 1e-9),  # exact quantizer on the tile levels
+]
+
+
+@pytest.mark.parametrize("i", range(len(REL_CASES)))
+def test_rel_bounds_on_device(engine, engine_tile, engine_hostbounds, i):
+    """REL compression computes the level bounds on the device from the value
+    range (no host round trip): the engine buffer equals the oracle's at
+    eb = rel * (max - min) and the host-bound path's."""
+    import faz_amd
+    shape, kw, rel = REL_CASES[i]
+    x = _field(shape, 1100 + i) + np.float32(1e4 if rel < 1e-8 else 0.0)
+    mn, mx = oracle.data_range(x)
+    eb = rel * float(np.float32(mx - mn))
+    print(f"[gpu] device level bounds {shape} {kw} rel={rel}", flush=True)
+    want = oracle.interp_raw(x, oracle.make_params(shape, eb, **kw))
+    d = _dev(x)
+    for eng in (engine, engine_tile, engine_hostbounds):
+        size = eng.compress_device(d, faz_amd.make_params(shape, 1.0, **kw), True, rel)
+        assert eng.fetch_raw(size) == want
+
+
+def test_rel_bounds_constant_field_is_an_error(engine):
+    import faz_amd
+    d = _dev(np.full((40, 40, 40), 3.0, np.float32))
+    with pytest.raises(faz_amd.FazError):
+        engine.compress_device(d, faz_amd.make_params((40, 40, 40), 1.0, algo=1, max_step=32), True, 1e-3)
+    # the context compresses again afterwards
+    x = _field((40, 40, 40), 1200)
+    mn, mx = oracle.data_range(x)
+    eb = 1e-3 * float(np.float32(mx - mn))
+    p = dict(algo=1, max_step=32)
+    size = engine.compress_device(_dev(x), faz_amd.make_params((40, 40, 40), 1.0, **p), True, 1e-3)
+    assert engine.fetch_raw(size) == oracle.interp_raw(x, oracle.make_params((40, 40, 40), eb, **p))
+
+
 def test_range_matches_oracle(engine):
     x = _field((33, 45, 57), 5) - 3.0
     mn, mx = engine.range(_dev(x))
