@@ -1122,14 +1122,15 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     // a symbol outside the window: the full arrays replace the window in hf
     const bool outside = hf[kHistPackFreq0 + 2] != 0;
     const uint32_t freq0 = hf[kHistPackFreq0];
+    // the code table now, the tree bytes once the encode is enqueued
     if (!outside) {
-        brc = build_huffman_window(hf, hf + kHistWin, hist_window_lo(p->radius), kHistWin, hf[kHistPackFreq0],
-                                   hf[kHistPackFreq0 + 1], pr.tab);
+        brc = build_huffman_window_codes(hf, hf + kHistWin, hist_window_lo(p->radius), kHistWin,
+                                         hf[kHistPackFreq0], hf[kHistPackFreq0 + 1], pr.tab);
     } else {  // a symbol outside the window: the whole arrays
         HIPCHECK(hipMemcpyAsync(hf, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
         HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
         HIPCHECK(hipStreamSynchronize(ctx->ustream));
-        brc = build_huffman(hf, hf + 65536, 65536, pr.tab);
+        brc = build_huffman_codes(hf, hf + 65536, 65536, pr.tab);
     }
     if (brc) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman tree could not be built (codes longer than 64 bits)");
     // the stream's bit total from the histogram and the code lengths: the
@@ -1156,6 +1157,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     const auto th2 = std::chrono::steady_clock::now();
     ctx->timings[8] = std::chrono::duration<double, std::milli>(th1 - th0).count();
     ctx->timings[9] = std::chrono::duration<double, std::milli>(th2 - th1).count();
+    finish_huffman_tree(pr.tab);  // while the encode runs
     HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hs + 6, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[5], st));

@@ -80,26 +80,24 @@ private:
     }
 };
 
-struct Node {
-    int left = -1, right = -1;
-    uint64_t freq = 0;
-    int c = 0;
-    bool leaf = false;
-};
-
 // HuffmanEncoder's 1-based binary heap (qinsert/qremove, HUF:455-485).  The
 // entries carry their node's frequency, so the sift loops compare without
-// reaching into the node pool; the operations are the reference's.  The
-// sift-down moves a hole instead of swapping (the same final array) and
-// picks the smaller child arithmetically: that choice is data-dependent and
-// mispredicts as a branch (about half the cost of a build of a few thousand
-// symbols).
+// reaching into the node arrays.  qremove's sift-down is done bottom-up: the
+// hole descends along the smaller child (the left one on ties) to the
+// bottom, then the moved element climbs back while its parent's frequency is
+// not below its own.  Along that path frequencies never decrease, so this
+// moves up exactly the elements smaller than the moved one, as the
+// reference's top-down loop does: the same array after every operation,
+// with one branch-free compare per level on the way down.
 class Heap {
 public:
-    explicit Heap(size_t cap) { q_.resize(cap + 3); }
+    void reset(size_t cap) {
+        if (q_.size() < 4 * cap + 8) q_.resize(4 * cap + 8);  // prefetches reach 4 i
+        end_ = 1;
+    }
     void insert(int n, uint64_t f) {
         int i = end_++, j;
-        if ((int) q_.size() <= i + 1) q_.resize(2 * (size_t) i + 3);
+        if (q_.size() <= 4 * (size_t) i + 8) q_.resize(8 * (size_t) i + 8);
         while ((j = i >> 1)) {
             if (q_[j].f <= f) break;
             q_[i] = q_[j];
@@ -113,11 +111,19 @@ public:
         const E x = q_[end_];
         int i = 1, l;
         while ((l = i << 1) < end_) {
+            // the four grandchildren of i share a cache line: fetch it while
+            // the children are compared
+            __builtin_prefetch(&q_[(size_t) 4 * i]);
             // q_[end_] (the moved entry's old slot) stays readable: no bounds branch
             l += (int) ((l + 1 < end_) & (q_[l + 1].f < q_[l].f));
-            if (!(x.f > q_[l].f)) break;
             q_[i] = q_[l];
             i = l;
+        }
+        while (i > 1) {
+            const int p = i >> 1;
+            if (q_[p].f < x.f) break;
+            q_[i] = q_[p];
+            i = p;
         }
         q_[i] = x;
         return n;
@@ -134,107 +140,107 @@ private:
     int end_ = 1;
 };
 
+// Pre-order numbering of the tree (pad_tree, HUF:569-584; internal nodes
+// carry C = 0) written as HuffmanEncoder::save's arrays at w: the
+// endianness byte, L and R (T1 each), C (int32), t (uint8).
 template<class T1>
-void serialise(const std::vector<Node> &pool, int root, unsigned nodeCount, std::vector<uint8_t> &out) {
-    // pre-order numbering (pad_tree, HUF:569-584); internal nodes carry C = 0
-    std::vector<T1> L(nodeCount, 0), R(nodeCount, 0);
-    std::vector<int32_t> C(nodeCount, 0);
-    std::vector<uint8_t> t(nodeCount, 0);
-    unsigned counter = 0;
-    struct Frame { int node; unsigned slot; int stage; };
-    std::vector<Frame> st;
-    st.push_back({root, 0, 0});
-    while (!st.empty()) {
-        Frame &f = st.back();
-        const Node &nd = pool[f.node];
-        if (f.stage == 0) {
-            C[f.slot] = nd.leaf ? nd.c : 0;
-            t[f.slot] = nd.leaf ? 1 : 0;
-            f.stage = 1;
-            if (nd.left >= 0) {
-                counter++;
-                L[f.slot] = (T1) counter;
-                st.push_back({nd.left, counter, 0});
-                continue;
-            }
-        }
-        if (f.stage == 1) {
-            f.stage = 2;
-            if (nd.right >= 0) {
-                counter++;
-                R[f.slot] = (T1) counter;
-                st.push_back({nd.right, counter, 0});
-                continue;
-            }
-        }
-        st.pop_back();
-    }
-    out.push_back(0);  // endianness marker: little-endian
-    auto app = [&out](const void *p, size_t n) {
-        const uint8_t *b = (const uint8_t *) p;
-        out.insert(out.end(), b, b + n);
+void serialise(const HuffTable &tab, unsigned nodeCount, uint8_t *w) {
+    *w++ = 0;  // endianness marker: little-endian
+    uint8_t *L = w, *R = w + (size_t) nodeCount * sizeof(T1), *C = w + 2 * (size_t) nodeCount * sizeof(T1);
+    uint8_t *t = C + (size_t) nodeCount * 4;
+    memset(w, 0, (size_t) nodeCount * (2 * sizeof(T1) + 5));
+    auto put1 = [](uint8_t *a, unsigned i, unsigned v) {
+        const T1 x = (T1) v;
+        memcpy(a + (size_t) i * sizeof(T1), &x, sizeof(T1));
     };
-    app(L.data(), nodeCount * sizeof(T1));
-    app(R.data(), nodeCount * sizeof(T1));
-    app(C.data(), nodeCount * sizeof(int32_t));
-    app(t.data(), nodeCount);
-}
-
-void put_be32(std::vector<uint8_t> &out, uint32_t v) {
-    out.push_back((uint8_t) (v >> 24));
-    out.push_back((uint8_t) (v >> 16));
-    out.push_back((uint8_t) (v >> 8));
-    out.push_back((uint8_t) v);
-}
-
-}  // namespace
-
-namespace {
-
-// The tree of the present symbols: (first position, symbol) pairs and a
-// count lookup by symbol.
-template<class FreqOf>
-int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of, HuffTable &tab);
-
-}  // namespace
-
-int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
-    std::vector<std::pair<uint32_t, int>> present;
-    present.reserve(1024);
-    int s0 = 0;
-    for (; s0 + 4 <= nsym; s0 += 4) {  // four counts at a time: most are zero
-        uint64_t a, b;
-        memcpy(&a, freq + s0, 8);
-        memcpy(&b, freq + s0 + 2, 8);
-        if (!(a | b)) continue;
-        for (int s = s0; s < s0 + 4; s++)
-            if (freq[s]) present.push_back({first[s], s});
+    struct Item {
+        int32_t node;
+        uint32_t at;    // slot of the node, or of the parent for a pending right child
+        uint32_t right; // 1: a right child whose slot is the next number
+    };
+    std::vector<Item> st;
+    st.reserve(160);
+    st.push_back({tab.root, 0, 0});
+    unsigned counter = 0;
+    while (!st.empty()) {
+        const Item it = st.back();
+        st.pop_back();
+        unsigned slot = it.at;
+        if (it.right) {
+            slot = ++counter;
+            put1(R, it.at, slot);
+        }
+        const int32_t nd = it.node;
+        const bool leaf = tab.left[nd] < 0 && tab.right[nd] < 0;
+        if (leaf) {
+            memcpy(C + (size_t) slot * 4, &tab.sym[nd], 4);
+            t[slot] = 1;
+            continue;
+        }
+        // the right subtree is numbered after the whole left one
+        if (tab.right[nd] >= 0) st.push_back({tab.right[nd], slot, 1});
+        if (tab.left[nd] >= 0) {
+            const unsigned ls = ++counter;
+            put1(L, slot, ls);
+            st.push_back({tab.left[nd], ls, 0});
+        }
     }
-    for (int s = s0; s < nsym; s++)
-        if (freq[s]) present.push_back({first[s], s});
-    return build_present(present, [&](int s) { return freq[s]; }, tab);
 }
 
-int build_huffman_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
-                         uint32_t first0, HuffTable &tab) {
-    std::vector<std::pair<uint32_t, int>> present;
-    present.reserve(1024);
-    if (freq0) present.push_back({first0, 0});
-    for (int i = 0; i < nwin; i++)
-        if (wfreq[i]) present.push_back({wfirst[i], lo + i});
-    return build_present(present, [&](int s) { return s == 0 ? freq0 : wfreq[s - lo]; }, tab);
-}
+
+}  // namespace
 
 namespace {
 
+// Ascending sort of the keys (first position << 32 | symbol): LSD radix over
+// 11-bit digits, skipping digits that every key shares; std::sort for short
+// lists.
+void sort_keys(std::vector<uint64_t> &k, std::vector<uint64_t> &tmp) {
+    const size_t n = k.size();
+    if (n < 256) {
+        std::sort(k.begin(), k.end());
+        return;
+    }
+    uint64_t all_or = 0, all_and = ~0ull;
+    for (uint64_t v: k) {
+        all_or |= v;
+        all_and &= v;
+    }
+    const uint64_t varying = all_or ^ all_and;
+    tmp.resize(n);
+    uint64_t *src = k.data(), *dst = tmp.data();
+    constexpr int kBits = 11;
+    constexpr uint32_t kBuckets = 1u << kBits;
+    std::vector<uint32_t> cnt(kBuckets);
+    for (int sh = 0; sh < 64; sh += kBits) {
+        if (!((varying >> sh) & (kBuckets - 1))) continue;
+        std::fill(cnt.begin(), cnt.end(), 0u);
+        for (size_t i = 0; i < n; i++) cnt[(src[i] >> sh) & (kBuckets - 1)]++;
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < kBuckets; b++) {
+            const uint32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+        for (size_t i = 0; i < n; i++) dst[cnt[(src[i] >> sh) & (kBuckets - 1)]++] = src[i];
+        std::swap(src, dst);
+    }
+    if (src != k.data()) std::copy(src, src + n, k.data());
+}
+
+// The tree of the present symbols (tab.keys: first position << 32 | symbol)
+// and its code table; the tree bytes too unless defer.
 template<class FreqOf>
-int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of, HuffTable &tab) {
-    if (present.empty()) return -1;
-    std::sort(present.begin(), present.end());
-    int mn = present[0].second, mx = present[0].second;
-    for (auto &p: present) {
-        mn = std::min(mn, p.second);
-        mx = std::max(mx, p.second);
+int build_present(FreqOf freq_of, HuffTable &tab, bool defer) {
+    std::vector<uint64_t> &keys = tab.keys;
+    if (keys.empty()) return -1;
+    sort_keys(keys, tab.scratch);
+    const size_t np = keys.size();
+    int mn = (int) (uint32_t) keys[0], mx = mn;
+    for (uint64_t k: keys) {
+        const int sy = (int) (uint32_t) k;
+        mn = std::min(mn, sy);
+        mx = std::max(mx, sy);
     }
     // a table of the same span keeps its arrays: only last build's codewords
     // are cleared (the outlier code 0 makes the span tens of thousands of
@@ -243,37 +249,40 @@ int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of
                        (int) tab.len.size() == tab.n_states;
     tab.offset = mn;
     tab.n_states = mx - mn + 1;
-    const int stateNum = mx - mn + 2;  // HUF:551
 
-    SkaOrder ska(present.size());
-    for (auto &p: present) ska.insert(p.second);
-    std::vector<int> order = ska.iteration_order();
+    SkaOrder ska(np);
+    for (uint64_t k: keys) ska.insert((int) (uint32_t) k);
+    const std::vector<int> order = ska.iteration_order();
 
-    std::vector<Node> pool;
-    pool.reserve(2 * present.size());
-    Heap heap(2 * present.size());
-    for (int key: order) {
-        Node n;
-        n.leaf = true;
-        n.freq = freq_of(key);
-        n.c = key - mn;
-        pool.push_back(n);
-        heap.insert((int) pool.size() - 1, n.freq);
+    // nodes: leaves 0..np-1 in the map's iteration order, then the merges
+    const size_t nn = 2 * np - 1;
+    tab.left.assign(nn, -1);
+    tab.right.assign(nn, -1);
+    tab.sym.resize(nn);
+    std::vector<uint64_t> freq(nn);
+    static thread_local Heap heap;
+    heap.reset(2 * np);
+    for (size_t i = 0; i < np; i++) {
+        const int key = order[i];
+        freq[i] = freq_of(key);
+        tab.sym[i] = key - mn;
+        heap.insert((int) i, freq[i]);
     }
     // qinsert(new_node(0, 0, qremove(), qremove())) in the reference's g++
     // build evaluates the arguments right to left: the first node removed is
     // the right child.
+    int32_t next = (int32_t) np;
     while (heap.size() > 1) {
-        int right = heap.remove();
-        int left = heap.remove();
-        Node n;
-        n.left = left;
-        n.right = right;
-        n.freq = pool[left].freq + pool[right].freq;
-        pool.push_back(n);
-        heap.insert((int) pool.size() - 1, n.freq);
+        const int right = heap.remove();
+        const int left = heap.remove();
+        tab.left[next] = left;
+        tab.right[next] = right;
+        tab.sym[next] = 0;
+        freq[next] = freq[left] + freq[right];
+        heap.insert(next, freq[next]);
+        next++;
     }
-    const int root = heap.top();
+    tab.root = heap.top();
 
     // codes: left = 0, right = 1, MSB-aligned (build_code, HUF:493-521)
     if (reuse) {
@@ -286,43 +295,105 @@ int build_present(std::vector<std::pair<uint32_t, int>> &present, FreqOf freq_of
         tab.len.assign(tab.n_states, 0);
     }
     tab.used.clear();
-    struct Item { int node; int depth; uint64_t bits; };
-    std::vector<Item> st;
-    st.push_back({root, 0, 0});
-    while (!st.empty()) {
-        Item it = st.back();
-        st.pop_back();
-        const Node &nd = pool[it.node];
-        if (nd.leaf) {
+    struct Item {
+        int32_t node;
+        int32_t depth;
+        uint64_t bits;
+    };
+    Item st[136];
+    int sp = 0;
+    st[sp++] = {tab.root, 0, 0};
+    while (sp) {
+        const Item it = st[--sp];
+        const int32_t l = tab.left[it.node], r = tab.right[it.node];
+        if (l < 0 && r < 0) {
             if (it.depth > 64) return -2;  // codes beyond 64 bits need > 2.7e13 symbols
-            tab.code[nd.c] = it.depth ? it.bits << (64 - it.depth) : 0;
-            tab.len[nd.c] = (uint8_t) it.depth;
-            tab.used.push_back(nd.c);
+            const int32_t c = tab.sym[it.node];
+            tab.code[c] = it.depth ? it.bits << (64 - it.depth) : 0;
+            tab.len[c] = (uint8_t) it.depth;
+            tab.used.push_back(c);
             continue;
         }
         if (it.depth >= 64) return -2;
-        st.push_back({nd.right, it.depth + 1, (it.bits << 1) | 1});
-        st.push_back({nd.left, it.depth + 1, it.bits << 1});
+        st[sp++] = {r, it.depth + 1, (it.bits << 1) | 1};
+        st[sp++] = {l, it.depth + 1, it.bits << 1};
     }
-
-    // save (HUF:119-137)
-    const unsigned nodeCount = (unsigned) (2 * present.size() - 1);
     tab.tree.clear();
-    uint8_t b4[4];
-    memcpy(b4, &mn, 4);
-    tab.tree.insert(tab.tree.end(), b4, b4 + 4);
-    put_be32(tab.tree, nodeCount);
-    put_be32(tab.tree, (uint32_t) (stateNum / 2));
-    if (nodeCount <= 256)
-        serialise<uint8_t>(pool, root, nodeCount, tab.tree);
-    else if (nodeCount <= 65536)
-        serialise<uint16_t>(pool, root, nodeCount, tab.tree);
-    else
-        serialise<uint32_t>(pool, root, nodeCount, tab.tree);
+    tab.tree_pending = true;
+    if (!defer) finish_huffman_tree(tab);
     return 0;
 }
 
+int build_full(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab, bool defer) {
+    tab.keys.clear();
+    int s0 = 0;
+    for (; s0 + 4 <= nsym; s0 += 4) {  // four counts at a time: most are zero
+        uint64_t a, b;
+        memcpy(&a, freq + s0, 8);
+        memcpy(&b, freq + s0 + 2, 8);
+        if (!(a | b)) continue;
+        for (int s = s0; s < s0 + 4; s++)
+            if (freq[s]) tab.keys.push_back((uint64_t) first[s] << 32 | (uint32_t) s);
+    }
+    for (int s = s0; s < nsym; s++)
+        if (freq[s]) tab.keys.push_back((uint64_t) first[s] << 32 | (uint32_t) s);
+    return build_present([&](int s) { return freq[s]; }, tab, defer);
+}
+
+int build_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0, uint32_t first0,
+                 HuffTable &tab, bool defer) {
+    tab.keys.clear();
+    if (freq0) tab.keys.push_back((uint64_t) first0 << 32);
+    for (int i = 0; i < nwin; i++)
+        if (wfreq[i]) tab.keys.push_back((uint64_t) wfirst[i] << 32 | (uint32_t) (lo + i));
+    return build_present([&](int s) { return s == 0 ? freq0 : wfreq[s - lo]; }, tab, defer);
+}
+
 }  // namespace
+
+int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
+    return build_full(freq, first, nsym, tab, false);
+}
+
+int build_huffman_codes(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab) {
+    return build_full(freq, first, nsym, tab, true);
+}
+
+int build_huffman_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
+                         uint32_t first0, HuffTable &tab) {
+    return build_window(wfreq, wfirst, lo, nwin, freq0, first0, tab, false);
+}
+
+int build_huffman_window_codes(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
+                               uint32_t first0, HuffTable &tab) {
+    return build_window(wfreq, wfirst, lo, nwin, freq0, first0, tab, true);
+}
+
+// save (HUF:119-137)
+void finish_huffman_tree(HuffTable &tab) {
+    if (!tab.tree_pending) return;
+    const unsigned nodeCount = (unsigned) tab.left.size();
+    const int stateNum = tab.n_states + 1;  // HUF:551
+    const size_t w = nodeCount <= 256 ? 1 : (nodeCount <= 65536 ? 2 : 4);
+    tab.tree.resize(12 + 1 + (size_t) nodeCount * (2 * w + 5));
+    uint8_t *o = tab.tree.data();
+    memcpy(o, &tab.offset, 4);
+    auto be32 = [](uint8_t *d, uint32_t v) {
+        d[0] = (uint8_t) (v >> 24);
+        d[1] = (uint8_t) (v >> 16);
+        d[2] = (uint8_t) (v >> 8);
+        d[3] = (uint8_t) v;
+    };
+    be32(o + 4, nodeCount);
+    be32(o + 8, (uint32_t) (stateNum / 2));
+    if (w == 1)
+        serialise<uint8_t>(tab, nodeCount, o + 12);
+    else if (w == 2)
+        serialise<uint16_t>(tab, nodeCount, o + 12);
+    else
+        serialise<uint32_t>(tab, nodeCount, o + 12);
+    tab.tree_pending = false;
+}
 
 // HuffmanEncoder::load + decode (HUF:236-296, 310-454)
 void encode_huffman(const HuffTable &tab, const int32_t *s, size_t n, std::vector<uint8_t> &out) {

@@ -13,6 +13,12 @@ struct HuffTable {
     std::vector<uint8_t> len;      // codeword lengths
     std::vector<uint8_t> tree;     // HuffmanEncoder::save bytes
     std::vector<int32_t> used;     // indices with a codeword (what a rebuild of the same span clears)
+    // nodes of the last build (children -1 for leaves, symbol - offset of a
+    // leaf), kept for a deferred serialisation and reused across builds
+    std::vector<int32_t> left, right, sym;
+    std::vector<uint64_t> keys, scratch;  // sort buffers of the present symbols
+    int32_t root = -1;
+    bool tree_pending = false;     // nodes built, tree bytes not yet written
 };
 
 // Builds the reference Huffman tree from counts and first positions of
@@ -23,6 +29,14 @@ int build_huffman(const uint32_t *freq, const uint32_t *first, int nsym, HuffTab
 // and first positions of that window, and of symbol 0.
 int build_huffman_window(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
                          uint32_t first0, HuffTable &tab);
+// The same with the tree bytes left for finish_huffman_tree (the code table
+// is complete): a caller can enqueue the encode first.
+int build_huffman_window_codes(const uint32_t *wfreq, const uint32_t *wfirst, int lo, int nwin, uint32_t freq0,
+                               uint32_t first0, HuffTable &tab);
+int build_huffman_codes(const uint32_t *freq, const uint32_t *first, int nsym, HuffTable &tab);
+// Writes tab.tree (HuffmanEncoder::save) after a *_codes build; no-op when
+// the bytes are already there.
+void finish_huffman_tree(HuffTable &tab);
 
 // HuffmanEncoder::encode (HuffmanEncoder.hpp:150-227) on the host: appends
 // the size_t byte count and the MSB-first bitstream of s[0..n) to out.  Every
