@@ -162,6 +162,7 @@ struct faz_gpu_ctx {
     // pinned: histogram and first positions, then 8 words of scalars
     faz::HostPinned<uint32_t> hp_hist{2 * 65536 + 8};
     faz::PinnedVec<uint64_t> h_tab;  // packed code table of the last encode (pinned: asynchronous upload)
+    faz::PinnedVec<uint32_t> h_dec;  // the decoder's lookup tables and tree, staged for upload
     faz::PendingRaw pending;
     // per-launch timing of the interpolation pass kernels
     bool profile = false;

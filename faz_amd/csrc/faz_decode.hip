@@ -704,16 +704,41 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
         }
         return kLongFlag | (uint32_t) at;
     };
-    std::vector<uint32_t> tab((size_t) 1 << kTabBits);
-    for (uint32_t v = 0; v < tab.size(); v++) {
-        int32_t stop = -1;
-        uint32_t e = entry(0, v, kTabBits, stop);
-        if (e == kLongFlag) e = build_sub(stop, kTabBits);
-        tab[v] = e;
+    // the primary table by a walk over the tree's first kTabBits levels in
+    // code order (the order the second-level tables are built in): a leaf at
+    // depth d fills its 2^(kTabBits - d) windows, an inner node at depth
+    // kTabBits gets its second-level table, missing children leave kOffTree
+    std::vector<uint32_t> tab((size_t) 1 << kTabBits, kOffTree);
+    {
+        struct Visit {
+            int32_t nd;
+            uint32_t code;
+            int depth;
+        };
+        std::vector<Visit> st;
+        st.reserve(2 * kTabBits + 2);
+        st.push_back({0, 0u, 0});
+        while (!st.empty()) {
+            const Visit f = st.back();
+            st.pop_back();
+            if (f.depth > 0 && leaf[f.nd]) {
+                const uint32_t lo = f.code << (kTabBits - f.depth), cnt = 1u << (kTabBits - f.depth);
+                for (uint32_t v = lo; v < lo + cnt; v++) tab[v] = (uint32_t) f.nd | (uint32_t) f.depth << 24;
+                continue;
+            }
+            if (f.depth == kTabBits) {
+                tab[f.code] = build_sub(f.nd, kTabBits);
+                continue;
+            }
+            if (right[f.nd] >= 0) st.push_back({right[f.nd], f.code << 1 | 1u, f.depth + 1});
+            if (left[f.nd] >= 0) st.push_back({left[f.nd], f.code << 1, f.depth + 1});
+        }
     }
     // multi-codeword windows: whole codewords from the window start on
     // (ctab for counting; wtab: up to three output values in [0, limit),
-    // for 16-bit outputs)
+    // for 16-bit outputs), each codeword read through the primary table with
+    // the window's remaining bits (a codeword that needs bits past the window
+    // ends the run)
     constexpr bool kMulti = sizeof(OUT) == 2;
     std::vector<uint32_t> ctab(tab.size());
     std::vector<uint64_t> wtab(kMulti ? tab.size() : 0);
@@ -722,17 +747,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
         uint64_t wv = 0;
         bool wopen = kMulti && limit && limit <= 65536u;
         while (at < (uint32_t) kTabBits) {
-            int32_t nd = 0;
-            uint32_t len = 0;
-            for (uint32_t p = at; p < (uint32_t) kTabBits; p++) {
-                nd = ((v >> (kTabBits - 1 - p)) & 1u) ? right[nd] : left[nd];
-                if (nd < 0) break;
-                if (leaf[nd]) {
-                    len = p - at + 1;
-                    break;
-                }
-            }
-            if (!len) break;
+            const uint32_t e = tab[(v << at) & ((1u << kTabBits) - 1)];
+            if (e & kLongFlag) break;  // longer than the window, or off the tree
+            const uint32_t len = e >> 24;
+            if (len > (uint32_t) kTabBits - at) break;
+            const int32_t nd = (int32_t) (e & 0xFFFFFFu);
             S |= 1u << at;
             cnt++;
             if (wopen) {
@@ -824,16 +843,31 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     launch_fill_u32(ctx->dec_words.p + (nwords ? nwords - 1 : 0), 2, 0u, d_buf ? st : us);
     uint32_t *d_tab = ctx->dec_tab.p;
     int32_t *d_tree = reinterpret_cast<int32_t *>(ctx->dec_tab.p + tab.size());
-    HIPCHECK(hipMemcpyAsync(d_tab, tab.data(), tab.size() * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(d_tree, left.data(), nc * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(d_tree + nc, right.data(), nc * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(d_tree + 2 * nc, sym.data(), nc * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipMemcpyAsync(ctx->dec_leaf.p, leaf.data(), nc, hipMemcpyHostToDevice, st));
+    // tables, tree arrays, leaf flags and the multi-codeword table staged in
+    // pinned memory in their device layout: three asynchronous copies
     uint32_t *d_sub = ctx->dec_tab.p + tab.size() + 3 * nc;
-    if (!sub.empty()) HIPCHECK(hipMemcpyAsync(d_sub, sub.data(), sub.size() * 4, hipMemcpyHostToDevice, st));
     uint32_t *d_ctab = d_sub + sub.size();
-    HIPCHECK(hipMemcpyAsync(d_ctab, ctab.data(), ctab.size() * 4, hipMemcpyHostToDevice, st));
-    if (kMulti) HIPCHECK(hipMemcpyAsync(ctx->dec_wtab.p, wtab.data(), wtab.size() * 8, hipMemcpyHostToDevice, st));
+    {
+        const size_t nt = tab.size() + 3 * nc + sub.size() + ctab.size();
+        const size_t nl = (nc + 3) / 4, nw = kMulti ? 2 * wtab.size() : 0;
+        uint32_t *h = ctx->h_dec.ensure(nt + nl + nw);
+        uint32_t *w = h;
+        auto put = [&w](const void *src, size_t words) {
+            memcpy(w, src, words * 4);
+            w += words;
+        };
+        put(tab.data(), tab.size());
+        put(left.data(), nc);
+        put(right.data(), nc);
+        put(sym.data(), nc);
+        if (!sub.empty()) put(sub.data(), sub.size());
+        put(ctab.data(), ctab.size());
+        memcpy(h + nt, leaf.data(), nc);
+        if (kMulti) memcpy(h + nt + nl, wtab.data(), wtab.size() * 8);
+        HIPCHECK(hipMemcpyAsync(d_tab, h, nt * 4, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(ctx->dec_leaf.p, h + nt, nc, hipMemcpyHostToDevice, st));
+        if (kMulti) HIPCHECK(hipMemcpyAsync(ctx->dec_wtab.p, h + nt + nl, wtab.size() * 8, hipMemcpyHostToDevice, st));
+    }
     const Tree T{d_tree, d_tree + nc, d_tree + 2 * nc, ctx->dec_leaf.p, d_sub};
     uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nchunks;
     uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nchunks;
