@@ -213,6 +213,9 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint32_t> dec_words, dec_tab, dec_cnt;
     faz::DevBuf<uint8_t> dec_leaf, dec_x, dec_g;
     faz::DevBuf<uint16_t> dec_n;
+    // per chunk and entry offset: first codeword start past each inner quarter boundary, codewords before it
+    faz::DevBuf<uint32_t> dec_qx;
+    faz::DevBuf<uint64_t> dec_qn;
     faz::DevBuf<uint64_t> dec_wtab;  // multi-codeword output table
     bool hist_cleared = false;       // freq/first already cleared on the stream for this compression
     faz::DevBuf<uint64_t> dec_pos;

@@ -47,6 +47,16 @@ constexpr uint32_t kStageWords = kGroupWords + 5;
 constexpr uint32_t kStageSlots = kStageWords + kStageWords / 32 + 1;
 // codeword starts of a trajectory are kept for its first kMaskBits bits
 constexpr uint32_t kMaskBits = 128;
+// The write pass decodes quarter chunks: the map pass records, for every
+// entry offset, where the trajectory's first codeword at or after each of
+// the chunk's three inner quarter boundaries starts and how many codewords
+// lie before it.
+constexpr uint32_t kQuarters = 4;
+constexpr uint32_t kQuarterBits = kChunkBits / kQuarters;
+// the write pass stages a quarter of the map pass's words per workgroup
+constexpr uint32_t kWGroupWords = kGroupWords / kQuarters;
+constexpr uint32_t kWStageWords = kWGroupWords + 5;
+constexpr uint32_t kWStageSlots = kWStageWords + kWStageWords / 32 + 1;
 // Lookup entries.  Primary table (kTabBits bits, LDS) and second-level
 // tables (kSubBits bits each, global memory, one per tree node reached after
 // kTabBits + j * kSubBits bits):
@@ -79,6 +89,7 @@ __device__ __forceinline__ uint32_t used_of(uint32_t e) { return (e >> 16) & 31u
 struct BitReader {
     const uint32_t *lw;     // staged words (byte order swapped), see slot()
     uint64_t base;          // stream word index of lw's first word
+    uint32_t staged = kStageWords;  // words in lw
     const uint32_t *words;  // the stream as 32-bit words (bytes in stream order)
     uint64_t nwords;
     uint64_t buf;           // next bits, MSB first
@@ -87,7 +98,7 @@ struct BitReader {
 
     __device__ uint32_t word(uint64_t i) const {
         const uint64_t li = i - base;
-        if (li < kStageWords) return lw[slot((uint32_t) li)];
+        if (li < staged) return lw[slot((uint32_t) li)];
         return i < nwords ? __builtin_bswap32(words[i]) : 0u;
     }
     __device__ void start(uint64_t pos) {
@@ -170,15 +181,36 @@ struct Cursor {
     BitReader r;
     uint64_t pos;
     uint32_t k;
-    __device__ void init(const BitReader &proto, uint64_t p) {
+    // the first codeword start at or after each inner quarter boundary of
+    // the chunk: byte q of qx its distance from boundary q, 16 bits q of qn
+    // the codewords before it (boundaries past the trajectory's end: 0 and
+    // the final count)
+    uint64_t qnext;
+    uint32_t qi, qx;
+    uint64_t qn;
+    __device__ void init(const BitReader &proto, uint64_t p, uint64_t chunk) {
         r = proto;
         r.start(p);
         pos = p;
         k = 0;
+        qnext = chunk + kQuarterBits;
+        qi = 0;
+        qx = 0;
+        qn = 0;
+    }
+    __device__ void qrecord(uint64_t start, uint32_t count) {
+        qx |= (uint32_t) (start - qnext) << (8 * qi);
+        qn |= (uint64_t) count << (16 * qi);
+        qi++;
+        qnext = qi < kQuarters - 1 ? qnext + kQuarterBits : ~0ull;
+    }
+    __device__ void qfinish() {
+        for (; qi < kQuarters - 1; qi++) qn |= (uint64_t) k << (16 * qi);
     }
     __device__ void step(const uint32_t *tab, const Tree &T, uint64_t nbits) {
         const int32_t nd = decode_one(r, tab, T, pos);
         if (nd >= 0 && pos <= nbits) k++;
+        if (pos >= qnext) qrecord(pos, k);
     }
     // The ctab entry of the next window when every codeword it holds starts
     // before lim (0 otherwise); consume() then takes them.
@@ -190,11 +222,35 @@ struct Cursor {
     }
     __device__ void consume(uint32_t e) {
         const int used = (int) used_of(e);
+        const uint64_t before = pos;
+        const uint32_t kb = k;
         r.drop(used);
         pos += (uint64_t) used;
         k += cnt_of(e);
+        if (pos >= qnext) {
+            // the boundary falls inside the window: its first start there, or
+            // the start after the window
+            const uint32_t S = e & 0xFFFu, j0 = (uint32_t) (qnext - before);
+            const uint32_t hi = S & ~((1u << j0) - 1u);
+            if (hi) {
+                const uint32_t j = (uint32_t) __builtin_ctz(hi);
+                qrecord(before + j, kb + (uint32_t) __popc(S & ((1u << j) - 1u)));
+            } else {
+                qrecord(pos, k);
+            }
+        }
     }
 };
+
+// Quarter data of a trajectory that joins another one: the same starts,
+// counts shifted by the codewords the two have counted differently.
+__device__ __forceinline__ uint64_t qshift(uint64_t qn, int32_t delta) {
+    uint64_t out = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kQuarters - 1; q++)
+        out |= (uint64_t) (uint16_t) ((int32_t) ((qn >> (16 * q)) & 0xFFFFu) + delta) << (16 * q);
+    return out;
+}
 
 // ORs the 12-bit start mask S, placed at offset q, into the 128-bit mask m.
 __device__ __forceinline__ void or_mask(uint64_t *m, uint32_t S, uint64_t q) {
@@ -280,7 +336,8 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
                                                                const uint32_t *__restrict__ gctab, Tree T, uint32_t K,
                                                                uint8_t *__restrict__ xmap,
                                                                uint16_t *__restrict__ nmap, uint32_t wg0,
-                                                               RunPrefix run) {
+                                                               RunPrefix run, uint32_t *__restrict__ qxmap,
+                                                               uint64_t *__restrict__ qnmap) {
     __shared__ uint32_t tab[1 << kTabBits];
     __shared__ uint32_t ctab[1 << kTabBits];
     __shared__ uint32_t lw[kStageSlots];
@@ -297,26 +354,39 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
     const uint64_t nxt = b + kChunkBits;
     // masks are exact only where every symbol before kMaskBits ends inside the stream
     const bool masks = b + kMaskBits <= nbits;
-    auto put = [&](uint32_t o, uint64_t x, uint32_t n) {
+    // the maps are offset-major (entry o of chunk c at o * nchunks + c), so
+    // a wave's stores for one offset are contiguous
+    auto put = [&](uint32_t o, uint64_t x, uint32_t n, uint32_t qx, uint64_t qn) {
         const uint64_t e = x > nxt ? x - nxt : 0;
-        xmap[(uint64_t) c * K + o] = (uint8_t) min<uint64_t>(e, K - 1);
-        nmap[(uint64_t) c * K + o] = (uint16_t) n;
+        const uint64_t at = (uint64_t) o * nchunks + c;
+        xmap[at] = (uint8_t) min<uint64_t>(e, K - 1);
+        nmap[at] = (uint16_t) n;
+        qxmap[at] = qx;
+        qnmap[at] = qn;
     };
     if (run_chunk(run, b)) {
         // inside the verified run every entry is mapped to the run's own
         // exit: only the true entry is ever composed
         const uint64_t ex = run_next_start(run, b + kChunkBits);
-        const uint32_t n = (uint32_t) (run_starts_below(run, b + kChunkBits) - run_starts_below(run, b));
-        for (uint32_t o = 0; o < K; o++) put(o, ex, n);
+        const uint64_t k0 = run_starts_below(run, b);
+        const uint32_t n = (uint32_t) (run_starts_below(run, b + kChunkBits) - k0);
+        uint32_t qx = 0;
+        uint64_t qn = 0;
+        for (uint32_t q = 0; q < kQuarters - 1; q++) {
+            const uint64_t bq = b + (uint64_t) (q + 1) * kQuarterBits;
+            qx |= (uint32_t) (run_next_start(run, bq) - bq) << (8 * q);
+            qn |= (uint64_t) (run_starts_below(run, bq) - k0) << (16 * q);
+        }
+        for (uint32_t o = 0; o < K; o++) put(o, ex, n, qx, qn);
         return;
     }
     uint64_t mA[2] = {0, 0}, mB[2] = {0, 0};
-    uint64_t xA, xB = 0;
-    uint32_t nA, nB = 0;
+    uint64_t xA, xB = 0, qnA, qnB = 0;
+    uint32_t nA, nB = 0, qxA, qxB = 0;
     bool hasB = false;
     {
         Cursor A;
-        A.init(proto, b);
+        A.init(proto, b, b);
         while (A.pos < end) {
             const uint64_t q = A.pos - b;
             const uint32_t e = A.peek_multi(ctab, end);
@@ -328,20 +398,30 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
             if (q < kMaskBits) mA[q >> 6] |= 1ull << (q & 63);
             A.step(tab, T, nbits);
         }
-        xA = A.pos, nA = A.k;
+        A.qfinish();
+        xA = A.pos, nA = A.k, qxA = A.qx, qnA = A.qn;
     }
-    put(0, xA, nA);
+    put(0, xA, nA, qxA, qnA);
+    // a trajectory that joins A (or B) at p after `at` codewords of its own
+    auto join_A = [&](uint32_t o, uint32_t at, uint32_t p) {
+        const uint32_t below = mask_below(mA, p);
+        put(o, xA, at + nA - below, qxA, qshift(qnA, (int32_t) at - (int32_t) below));
+    };
+    auto join_B = [&](uint32_t o, uint32_t at, uint32_t p) {
+        const uint32_t below = mask_below(mB, p);
+        put(o, xB, at + nB - below, qxB, qshift(qnB, (int32_t) at - (int32_t) below));
+    };
     for (uint32_t o = 1; o < K; o++) {
         if (masks && mask_has(mA, o)) {
-            put(o, xA, nA - mask_below(mA, o));
+            join_A(o, 0, o);
             continue;
         }
         if (masks && hasB && mask_has(mB, o)) {
-            put(o, xB, nB - mask_below(mB, o));
+            join_B(o, 0, o);
             continue;
         }
         Cursor O;
-        O.init(proto, b + o);
+        O.init(proto, b + o, b);
         uint64_t mO[2] = {0, 0};
         bool merged = false;
         while (O.pos < end) {
@@ -360,9 +440,9 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
                         const uint32_t at = O.k + (uint32_t) __popc(S & ((1u << j) - 1u));
                         const uint32_t p = (uint32_t) q + j;
                         if ((hitA >> j) & 1u)
-                            put(o, xA, at + nA - mask_below(mA, p));
+                            join_A(o, at, p);
                         else
-                            put(o, xB, at + nB - mask_below(mB, p));
+                            join_B(o, at, p);
                         merged = true;
                         break;
                     }
@@ -373,12 +453,12 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
             }
             if (watch) {
                 if (mask_has(mA, (uint32_t) q)) {
-                    put(o, xA, O.k + nA - mask_below(mA, (uint32_t) q));
+                    join_A(o, O.k, (uint32_t) q);
                     merged = true;
                     break;
                 }
                 if (hasB && mask_has(mB, (uint32_t) q)) {
-                    put(o, xB, O.k + nB - mask_below(mB, (uint32_t) q));
+                    join_B(o, O.k, (uint32_t) q);
                     merged = true;
                     break;
                 }
@@ -387,10 +467,11 @@ __global__ void __launch_bounds__(kDecThreads) huff_map_kernel(const uint32_t *_
             O.step(tab, T, nbits);
         }
         if (merged) continue;
-        put(o, O.pos, O.k);
+        O.qfinish();
+        put(o, O.pos, O.k, O.qx, O.qn);
         if (!hasB) {
             hasB = true;
-            xB = O.pos, nB = O.k;
+            xB = O.pos, nB = O.k, qxB = O.qx, qnB = O.qn;
             mB[0] = mO[0], mB[1] = mO[1];
         }
     }
@@ -408,7 +489,10 @@ __global__ void __launch_bounds__(kDecThreads) huff_group_compose_kernel(const u
     const uint32_t i = threadIdx.x;
     const uint64_t c0 = (uint64_t) blockIdx.x * kDecThreads;
     const uint32_t n = (uint32_t) min<uint64_t>(kDecThreads, nchunks - c0);
-    for (uint32_t j = i; j < n * K; j += kDecThreads) buf[0][j] = xmap[c0 * K + j];
+    for (uint32_t j = i; j < n * K; j += kDecThreads) {  // offset-major source, chunk-major in LDS
+        const uint32_t o = j / n, ci = j - o * n;
+        buf[0][ci * K + o] = xmap[(uint64_t) o * nchunks + c0 + ci];
+    }
     __syncthreads();
     int cur = 0;
     for (uint32_t d = 1; d < n; d *= 2) {
@@ -449,20 +533,35 @@ __global__ void __launch_bounds__(1024) huff_group_entry_kernel(const uint8_t *_
 }
 
 // Entry of every chunk (the group's entry, carried through the group's
-// earlier chunks) and its symbol count at that entry.
+// earlier chunks), and at that entry the first codeword start of each
+// quarter of the chunk and the symbols of each quarter.
 __global__ void huff_entry_kernel(const uint8_t *__restrict__ gmap, const uint8_t *__restrict__ gentry,
-                                  const uint16_t *__restrict__ nmap, uint32_t nchunks, uint32_t K,
+                                  const uint16_t *__restrict__ nmap, const uint32_t *__restrict__ qxmap,
+                                  const uint64_t *__restrict__ qnmap, uint32_t nchunks, uint32_t K,
                                   uint64_t *__restrict__ start, uint32_t *__restrict__ count) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nchunks) return;
     const uint32_t ge = gentry[c / kDecThreads];
     const uint32_t e = (c % kDecThreads) ? gmap[(uint64_t) (c - 1) * K + ge] : ge;
-    start[c] = (uint64_t) c * kChunkBits + e;
-    count[c] = nmap[(uint64_t) c * K + e];
+    const uint64_t at = (uint64_t) e * nchunks + c;
+    const uint64_t b = (uint64_t) c * kChunkBits;
+    const uint32_t qx = qxmap[at];
+    const uint64_t qn = qnmap[at];
+    uint32_t before = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < kQuarters; q++) {
+        const uint32_t upto = q + 1 < kQuarters ? (uint32_t) ((qn >> (16 * q)) & 0xFFFFu) : (uint32_t) nmap[at];
+        start[(uint64_t) kQuarters * c + q] = q ? b + (uint64_t) q * kQuarterBits + ((qx >> (8 * (q - 1))) & 0xFFu)
+                                                : b + e;
+        count[(uint64_t) kQuarters * c + q] = upto - before;
+        before = upto;
+    }
 }
 
-// Pass 3: chunk c writes its symbols from offset[c] on (those below num),
-// each sym + sym_offset, which must lie in [0, limit) unless limit is 0;
+// Pass 3: one thread per quarter chunk (chunk t / 4, quarter t % 4) writes
+// the symbols of the codewords that start in its quarter from start[t] on,
+// at offset[t] (those below num), each sym + sym_offset, which must lie in
+// [0, limit) unless limit is 0;
 // err[0] = 1 otherwise or when the bits leave the tree.  MULTI: wtab (LDS)
 // gives up to three output values per kTabBits-bit window (bits 0..47, in
 // order), their count (48..49, 0: decode one codeword through the primary
@@ -479,26 +578,28 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
                                                                  RunPrefix run) {
     __shared__ uint32_t tab[MULTI ? 1 : (1 << kTabBits)];
     __shared__ uint64_t wtab[MULTI ? (1 << kTabBits) : 1];
-    __shared__ uint32_t lw[kStageSlots];
-    const uint64_t base = (uint64_t) blockIdx.x * kGroupWords;
+    __shared__ uint32_t lw[kWStageSlots];
+    const uint64_t base = (uint64_t) blockIdx.x * kWGroupWords;
     for (uint32_t i = threadIdx.x; i < (1u << kTabBits); i += blockDim.x) {
         if (MULTI)
             wtab[i] = gwtab[i];
         else
             tab[i] = gtab[i];
     }
-    for (uint32_t i = threadIdx.x; i < kStageWords; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < kWStageWords; i += blockDim.x) {
         const uint64_t g = base + i;
         lw[slot(i)] = g < nwords ? __builtin_bswap32(words[g]) : 0u;
     }
     __syncthreads();
     const uint32_t *dtab = MULTI ? gtab : tab;
-    const uint32_t c = blockIdx.x * kDecThreads + threadIdx.x;
-    if (c >= nchunks) return;
-    const uint64_t end = min((uint64_t) (c + 1) * kChunkBits, nbits);
-    uint64_t pos = start[c], o = offset[c];
+    const uint64_t t = (uint64_t) blockIdx.x * kDecThreads + threadIdx.x;
+    if (t >= (uint64_t) kQuarters * nchunks) return;
+    const uint32_t c = (uint32_t) (t / kQuarters), q = (uint32_t) (t % kQuarters);
+    const uint64_t end = min((uint64_t) c * kChunkBits + (uint64_t) (q + 1) * kQuarterBits, nbits);
+    uint64_t pos = start[t], o = offset[t];
     BitReader r;
     r.lw = lw, r.base = base, r.words = words, r.nwords = nwords;
+    r.staged = kWStageWords;
     r.start(pos);
     // symbols leave in 8-byte stores once out + o is 8-byte aligned
     constexpr int P = 8 / sizeof(OUT);
@@ -795,16 +896,18 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     HIPCHECK(ctx->dec_tab.ensure(2 * tab.size() + 3 * nc + sub.size()));
     HIPCHECK(ctx->dec_wtab.ensure(tab.size()));
     HIPCHECK(ctx->dec_leaf.ensure(nc));
-    HIPCHECK(ctx->dec_pos.ensure(2 * (size_t) nchunks));
-    HIPCHECK(ctx->dec_cnt.ensure((size_t) nchunks + 3));
+    HIPCHECK(ctx->dec_pos.ensure(2 * (size_t) kQuarters * nchunks));
+    HIPCHECK(ctx->dec_cnt.ensure((size_t) kQuarters * nchunks + 3));
+    HIPCHECK(ctx->dec_qx.ensure(nmaps));
+    HIPCHECK(ctx->dec_qn.ensure(nmaps));
     HIPCHECK(ctx->dec_x.ensure(2 * nmaps));
     HIPCHECK(ctx->dec_g.ensure((size_t) ((nchunks + kDecThreads - 1) / kDecThreads) * (K + 1)));
     HIPCHECK(ctx->dec_n.ensure(nmaps));
-    HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
+    HIPCHECK(ctx->tiles.ensure(scan_tiles(kQuarters * nchunks)));
     HIPCHECK(ctx->scalars.ensure(4));
     // the opening run: its codeword from the tree (a leaf whose value is
     // run_value), checked against the stream before the map launch
-    RunPrefix run{0, 0, 0, 0, ctx->dec_cnt.p + nchunks + 2};
+    RunPrefix run{0, 0, 0, 0, ctx->dec_cnt.p + (size_t) kQuarters * nchunks + 2};
     if (run_count && K <= 64) {
         std::vector<int32_t> parent(nc, -1);
         for (size_t i = 0; i < nc; i++) {
@@ -869,8 +972,10 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
         if (kMulti) HIPCHECK(hipMemcpyAsync(ctx->dec_wtab.p, h + nt + nl, wtab.size() * 8, hipMemcpyHostToDevice, st));
     }
     const Tree T{d_tree, d_tree + nc, d_tree + 2 * nc, ctx->dec_leaf.p, d_sub};
-    uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nchunks;
-    uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nchunks;
+    // per quarter chunk: first codeword start, output offset, symbol count
+    const uint32_t nquart = kQuarters * nchunks;
+    uint64_t *d_start = ctx->dec_pos.p, *d_off = d_start + nquart;
+    uint32_t *d_cnt = ctx->dec_cnt.p, *d_err = ctx->dec_cnt.p + nquart;
     const dim3 grid((nchunks + kDecThreads - 1) / kDecThreads), block(kDecThreads);
     // pass 1: entry -> exit maps and counts, part by part as the stream
     // arrives (the last workgroup of a part stages a few words past its
@@ -904,7 +1009,8 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
             }
             if (w1 > w0)
                 hipLaunchKernelGGL(huff_map_kernel, dim3(w1 - w0), block, 0, st, ctx->dec_words.p, nwords, nbits,
-                                   nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0, run);
+                                   nchunks, d_tab, d_ctab, T, K, ctx->dec_x.p, ctx->dec_n.p, w0, run, ctx->dec_qx.p,
+                                   ctx->dec_qn.p);
             HIPCHECK(hipGetLastError());
         }
     }
@@ -917,12 +1023,14 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     hipLaunchKernelGGL(huff_group_entry_kernel, dim3(1), dim3(1024), 0, st, d_grp, ngroups, K, d_gentry);
     ctx->dec_rounds = ngroups;
     hipLaunchKernelGGL(huff_entry_kernel, dim3((nchunks + 255) / 256), dim3(256), 0, st, d_gmap, d_gentry,
-                       ctx->dec_n.p, nchunks, K, d_start, d_cnt);
+                       ctx->dec_n.p, ctx->dec_qx.p, ctx->dec_qn.p, nchunks, K, d_start, d_cnt);
     HIPCHECK(hipGetLastError());
-    // output offsets of the chunks (exclusive scan) and the total, then pass 3
-    launch_zero_scan(d_cnt, d_off, nchunks, ctx->tiles.p, ctx->scalars.p, st);
+    // output offsets of the quarter chunks (exclusive scan) and the total,
+    // then pass 3
+    launch_zero_scan(d_cnt, d_off, nquart, ctx->tiles.p, ctx->scalars.p, st);
     launch_fill_u32(d_err, 1, 0u, st);
-    hipLaunchKernelGGL((huff_write_kernel<OUT, kMulti>), grid, block, 0, st, ctx->dec_words.p, nwords, nbits, nchunks,
+    hipLaunchKernelGGL((huff_write_kernel<OUT, kMulti>), dim3((nquart + kDecThreads - 1) / kDecThreads), block, 0, st,
+                       ctx->dec_words.p, nwords, nbits, nchunks,
                        d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err,
                        run);
     HIPCHECK(hipGetLastError());
