@@ -601,22 +601,29 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
     r.lw = lw, r.base = base, r.words = words, r.nwords = nwords;
     r.staged = kWStageWords;
     r.start(pos);
-    // symbols leave in 8-byte stores once out + o is 8-byte aligned
-    constexpr int P = 8 / sizeof(OUT);
+    // symbols leave in 16-byte stores once out + o is 16-byte aligned (each
+    // lane writes its own range: wide stores keep the partial-line writes
+    // that reach the L2 few)
+    constexpr int P = 16 / sizeof(OUT), H = P / 2;
     constexpr int W = 8 * sizeof(OUT);
-    uint64_t acc = 0;
+    uint64_t acc0 = 0, acc1 = 0;
     int na = 0;
     auto push = [&](uint32_t v) {
-        if (na == 0 && (reinterpret_cast<uintptr_t>(out + o) & 7u) != 0) {
+        if (na == 0 && (reinterpret_cast<uintptr_t>(out + o) & 15u) != 0) {
             out[o++] = (OUT) v;
             return;
         }
-        acc |= (uint64_t) (std::make_unsigned_t<OUT>) (OUT) v << (W * na);
+        const uint64_t x = (uint64_t) (std::make_unsigned_t<OUT>) (OUT) v;
+        if (na < H)
+            acc0 |= x << (W * na);
+        else
+            acc1 |= x << (W * (na - H));
         na++;
         o++;
         if (na == P) {
-            *reinterpret_cast<uint64_t *>(out + (o - P)) = acc;
-            acc = 0;
+            *reinterpret_cast<uint4 *>(out + (o - P)) =
+                make_uint4((uint32_t) acc0, (uint32_t) (acc0 >> 32), (uint32_t) acc1, (uint32_t) (acc1 >> 32));
+            acc0 = acc1 = 0;
             na = 0;
         }
     };
@@ -654,7 +661,7 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
         }
         push(v);
     }
-    for (int j = 0; j < na; j++) out[o - na + j] = (OUT) (acc >> (W * j));
+    for (int j = 0; j < na; j++) out[o - na + j] = (OUT) (j < H ? acc0 >> (W * j) : acc1 >> (W * (j - H)));
     if (bad) atomicOr(err, 1u);
 }
 
