@@ -1625,6 +1625,8 @@ int batch_capture_pred(faz_batch *b, hipGraph_t &g) {
             b->ctx->err = k->err;
             break;
         }
+        // the histogram's window, symbol 0, flags and outlier count in one block
+        launch_hist_pack(k->freq.p, k->first.p, q.radius, k->scalars.p, k->err_flag.p, k->hist_pack.p, k->stream);
         if (hipEventRecord(k->ev[7], k->stream) != hipSuccess || hipStreamWaitEvent(cs, k->ev[7], 0) != hipSuccess)
             rc = FAZ_ERR_HIP;
     }
@@ -1649,7 +1651,8 @@ int batch_prepare_field(faz_batch *b, int i, double &ms) {
     q.eb = b->eb[i];
     int rc = FAZ_OK;
     if (hipSetDevice(k->device) != hipSuccess) BFAIL(FAZ_ERR_HIP, "hipSetDevice failed");
-    if (k->work_out.ensure(b->num[i]) != hipSuccess || k->unpred.ensure(b->num[i]) != hipSuccess)
+    if (k->work_out.ensure(b->num[i]) != hipSuccess || k->unpred.ensure(b->num[i]) != hipSuccess ||
+        k->hist_pack.ensure(kHistPackWords) != hipSuccess)
         BFAIL(FAZ_ERR_HIP, "batch: device allocation failed");
     faz_interp_out io;
     rc = interp_compress(k, b->d_in[i], k->work_out.p, &q, &io, false, b->absmax[i]);
@@ -1821,55 +1824,62 @@ faz_status faz_gpu_batch_compress(faz_batch *b, const int *rel_mode, const doubl
     HIPCHECK(hipEventRecord(b->ev[2], cs));
     HIPCHECK(hipGraphLaunch(b->x_pred, cs));
     HIPCHECK(hipEventRecord(b->ev[3], cs));
-    // 3. histograms to pinned host memory, trees built by a pool of host
-    //    threads, each field's encode enqueued as soon as its tree is ready,
-    //    one wait at the end
+    // 3. each field's packed histogram block (window counts and first
+    //    positions, symbol 0, flags, outlier count) to pinned host memory;
+    //    a pool of host threads builds the trees and enqueues each field's
+    //    encode on its own stream as soon as its tree exists; one wait at the
+    //    end
     for (int i = 0; i < n; i++) {
         faz_gpu_ctx *k = b->kid[i];
         uint32_t *hh = b->h_hist + (size_t) i * 2 * 65536;
-        HIPCHECK(hipMemcpyAsync(hh, k->freq.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
-        HIPCHECK(hipMemcpyAsync(hh + 65536, k->first.p, 65536 * 4, hipMemcpyDeviceToHost, cs));
+        HIPCHECK(hipMemcpyAsync(hh, k->hist_pack.p, kHistPackWords * 4, hipMemcpyDeviceToHost, cs));
     }
     HIPCHECK(hipStreamSynchronize(cs));
-    std::vector<int> tree_rc(n, 1);  // 1 pending, 0 built, < 0 failed
-    std::mutex mu;
-    std::condition_variable cv;
+    std::vector<int> field_rc(n, FAZ_OK);
+    std::vector<std::string> field_err(n);
     std::atomic<int> next{0};
     auto builder = [&]() {
         for (int i; (i = next.fetch_add(1)) < n;) {
-            PendingRaw &pr = b->kid[i]->pending;
+            faz_gpu_ctx *k = b->kid[i];
+            PendingRaw &pr = k->pending;
             pr.valid = false;
-            const uint32_t *hh = b->h_hist + (size_t) i * 2 * 65536;
-            const int r = build_huffman(hh, hh + 65536, 65536, pr.tab) ? -1 : 0;
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                tree_rc[i] = r;
+            uint32_t *hf = b->h_hist + (size_t) i * 2 * 65536;
+            int r;
+            if (hf[kHistPackFreq0 + 2] == 0) {
+                r = build_huffman_window_codes(hf, hf + kHistWin, hist_window_lo(b->prm[i].radius), kHistWin,
+                                               hf[kHistPackFreq0], hf[kHistPackFreq0 + 1], pr.tab);
+            } else {  // a symbol outside the window: the whole arrays
+                if (hipSetDevice(k->device) != hipSuccess ||
+                    hipMemcpy(hf, k->freq.p, 65536 * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                    hipMemcpy(hf + 65536, k->first.p, 65536 * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+                    field_rc[i] = FAZ_ERR_HIP;
+                    field_err[i] = "histogram readback failed";
+                    continue;
+                }
+                r = build_huffman_codes(hf, hf + 65536, 65536, pr.tab);
             }
-            cv.notify_all();
+            if (r) {
+                field_rc[i] = FAZ_ERR_UNSUPPORTED;
+                field_err[i] = "Huffman tree could not be built";
+                continue;
+            }
+            faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
+            if (hipSetDevice(k->device) != hipSuccess) {
+                field_rc[i] = FAZ_ERR_HIP;
+                field_err[i] = "hipSetDevice failed";
+                continue;
+            }
+            field_rc[i] = encode_enqueue(k, k->codes.p, b->ncodes[i], &ht);
+            if (field_rc[i]) field_err[i] = k->err;
+            finish_huffman_tree(pr.tab);  // while the encode runs
         }
     };
     const int nthreads = std::max(1, std::min(n, 8));
     std::vector<std::thread> pool;
     for (int t = 0; t < nthreads; t++) pool.emplace_back(builder);
-    int rc = FAZ_OK;
-    for (int i = 0; i < n && !rc; i++) {
-        {
-            std::unique_lock<std::mutex> lk(mu);
-            cv.wait(lk, [&] { return tree_rc[i] != 1; });
-        }
-        faz_gpu_ctx *k = b->kid[i];
-        PendingRaw &pr = k->pending;
-        if (tree_rc[i]) {
-            snprintf(errbuf, sizeof(errbuf), "field %d: Huffman tree could not be built", i);
-            rc = FAZ_ERR_UNSUPPORTED;
-            break;
-        }
-        faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
-        rc = encode_enqueue(k, k->codes.p, b->ncodes[i], &ht);
-        if (rc) snprintf(errbuf, sizeof(errbuf), "%s", k->err.c_str());
-    }
     for (auto &th: pool) th.join();
-    if (rc) CTX_FAIL(rc, "%s", errbuf);
+    for (int i = 0; i < n; i++)
+        if (field_rc[i]) CTX_FAIL(field_rc[i], "field %d: %s", i, field_err[i].c_str());
     for (int i = 0; i < n; i++) {
         faz_gpu_ctx *k = b->kid[i];
         HIPCHECK(hipEventRecord(k->ev[6], k->stream));
