@@ -866,14 +866,16 @@ faz_status faz_gpu_huffman_hist(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_
     if (n >= (1ull << 32)) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "more than 2^32-1 codes");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    if (!(ctx->hist_codes == d_codes && ctx->hist_n == n)) {
+    // the histogram the last compression left is reused only for the
+    // context's own code buffer, which nothing else writes
+    if (!(d_codes == ctx->codes.p && ctx->hist_codes == d_codes && ctx->hist_n == n)) {
         const uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
         HIPCHECK(ctx->chunk_zeros.ensure(nchunks));
         HIPCHECK(ctx->freq.ensure(kHistSlots));
         HIPCHECK(ctx->first.ensure(65536));
         launch_hist(d_codes, n, ctx->freq.p, ctx->first.p, ctx->chunk_zeros.p, 32768, st);
         HIPCHECK(hipGetLastError());
-        ctx->hist_codes = d_codes;
+        ctx->hist_codes = d_codes == ctx->codes.p ? d_codes : nullptr;
         ctx->hist_n = n;
     }
     HIPCHECK(hipMemcpyAsync(h_freq, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, st));

@@ -448,16 +448,15 @@ void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t s
 // ---------------------------------------------------------------------------
 // Histogram + first occurrence + per-chunk outlier counts
 //
-// Codes cluster tightly around the quantizer radius, so the 16 symbols
-// nearest it are counted in per-thread packed 8-bit registers (a thread sees
-// at most kHistPerThread <= 255 codes per chunk) and folded once per chunk;
-// the rest of a 4096-symbol window goes to LDS atomics, anything beyond it to
-// global atomics.  First positions: LDS atomicMin guarded by a read, so each
-// symbol costs at most a few atomics per workgroup.
+// Codes cluster tightly around the quantizer radius, so the 64 symbols
+// nearest it are counted in lane-private LDS counters (no two lanes of a
+// wave add to one word); the rest of a 4096-symbol window goes to LDS
+// atomics, anything beyond it to global atomics.  First positions: LDS
+// atomicMin guarded by a read, so each symbol costs at most a few atomics per
+// workgroup.
 // ---------------------------------------------------------------------------
 constexpr int kWin = kHistWin;  // LDS bins around the quantizer radius
-constexpr int kHot = 16;    // register-counted symbols [hot0, hot0 + 16)
-static_assert(kHistPerThread <= 255, "8-bit register counters");
+constexpr int kLaneWin = 64;  // symbols [hot0, hot0 + 64) counted per lane
 
 // The thread's 8 groups of 8 consecutive codes of chunk c (16-byte loads
 // for a full chunk of aligned codes, element loads past the end otherwise).
@@ -513,40 +512,68 @@ __global__ void __launch_bounds__(kHistThreads) zero_count_u16_kernel(const uint
     }
 }
 
-// Persistent: a workgroup keeps its LDS window and register counters across
-// the chunks it takes (in increasing order), so the window is cleared and
-// flushed to the global histogram once per workgroup, not once per chunk.
-// The next chunk's codes are in flight while the current one is counted, and
-// the per-chunk outlier counter alternates between two slots, so a chunk
-// costs one barrier.
+// Persistent: a workgroup keeps its LDS window and lane counters across the
+// chunks it takes (in increasing order), so they are cleared and flushed to
+// the global histogram once per workgroup, not once per chunk.  The next
+// chunk's codes are in flight while the current one is counted, and the
+// per-chunk outlier counter alternates between two slots, so a chunk costs
+// one barrier.
+//
+// A group of eight codes that all lie in [hot0, hot0 + kLaneWin) takes the
+// fast path: one LDS add per code into the lane's own counter of the symbol
+// (counters laid out [symbol][lane], so a wave's adds hit 64 distinct banks
+// whatever the symbols) and a 64-bit mask of the symbols seen; only symbols
+// new to the thread go on to the first-position update.  Any other group
+// goes code by code.
 __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__restrict__ codes, uint32_t n,
                                                            uint32_t *__restrict__ freq, uint32_t *__restrict__ first,
                                                            uint32_t *__restrict__ chunk_zeros, int lo_,
                                                            int hot0_, int aligned, uint32_t *__restrict__ outside) {
     __shared__ uint32_t s_cnt[kWin];
     __shared__ uint32_t s_first[kWin];  // first global position per window symbol
+    __shared__ uint32_t s_lane[kLaneWin * 64];
     __shared__ uint32_t s_zero[2], s_zero_first;
     const uint32_t lo = (uint32_t) lo_, hot0 = (uint32_t) hot0_;
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
         s_cnt[i] = 0;
         s_first[i] = 0xFFFFFFFFu;
     }
+    for (int i = threadIdx.x; i < kLaneWin * 64; i += kHistThreads) s_lane[i] = 0;
     if (threadIdx.x == 0) {
         s_zero_first = 0xFFFFFFFFu;
         s_zero[0] = s_zero[1] = 0;
     }
     __syncthreads();
-    uint32_t seen = 0;  // hot symbols whose first position this thread already offered
-    uint32_t zeros_total = 0;
-    const int lane = threadIdx.x % 64;
+    uint64_t seen = 0;  // lane-window symbols whose first position this thread already offered
+    uint32_t zeros_total = 0, slot = 0;
+    const uint32_t lane = threadIdx.x % 64;
     const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
-    // 8-bit counters of hot0..hot0+7, hot0+8..hot0+15, folded every kFoldChunks
-    // chunks (a thread adds at most kHistPerThread codes per chunk)
-    constexpr uint32_t kFoldChunks = 255 / kHistPerThread;
-    uint64_t hot_a = 0, hot_b = 0;
-    uint32_t unfolded = 0, slot = 0;
     constexpr int kGroups = kHistGroups;
     uint4 q[kGroups], qn[kGroups];
+    // one code outside the fast path
+    auto count_one = [&](uint32_t v, uint32_t gi, uint32_t *sz) {
+        const uint32_t d = v - hot0;
+        if (d < (uint32_t) kLaneWin) {
+            atomicAdd(&s_lane[d * 64 + lane], 1u);
+            if (!((seen >> d) & 1u)) {
+                seen |= 1ull << d;
+                if (s_first[v - lo] > gi) atomicMin(&s_first[v - lo], gi);
+            }
+        } else if (v == 0) {
+            atomicAdd(sz, 1u);
+            if (s_zero_first > gi) atomicMin(&s_zero_first, gi);
+        } else {
+            const uint32_t w = v - lo;
+            if (w < (uint32_t) kWin) {
+                atomicAdd(&s_cnt[w], 1u);
+                if (s_first[w] > gi) atomicMin(&s_first[w], gi);
+            } else {
+                atomicAdd(&freq[v], 1u);
+                atomicMin(&first[v], gi);
+                *outside = 1u;
+            }
+        }
+    };
     if (blockIdx.x < nchunks) hist_load(codes, blockIdx.x, n, aligned, q);
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x, slot ^= 1u) {
         const uint32_t base = c * kHistChunk;
@@ -556,50 +583,36 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
 #pragma unroll
         for (int g = 0; g < kGroups; g++) {
             const uint32_t j = base + (threadIdx.x + g * kHistThreads) * 8;
-            const uint16_t v[8] = {(uint16_t) q[g].x, (uint16_t) (q[g].x >> 16), (uint16_t) q[g].y,
-                                   (uint16_t) (q[g].y >> 16), (uint16_t) q[g].z, (uint16_t) (q[g].z >> 16),
-                                   (uint16_t) q[g].w, (uint16_t) (q[g].w >> 16)};
+            const uint32_t v[8] = {q[g].x & 0xFFFFu, q[g].x >> 16, q[g].y & 0xFFFFu, q[g].y >> 16,
+                                   q[g].z & 0xFFFFu, q[g].z >> 16, q[g].w & 0xFFFFu, q[g].w >> 16};
+            uint32_t d[8], any = 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                if (j + k >= end) break;
-                const uint32_t gi = j + k;
-                const uint32_t d = (uint32_t) v[k] - hot0;
-                if (d < (uint32_t) kHot) {
-                    const uint64_t one = 1ull << (8 * (d & 7));
-                    if (d < 8) hot_a += one; else hot_b += one;
-                    if (!((seen >> d) & 1u)) {
-                        seen |= 1u << d;
-                        const uint32_t w = v[k] - lo;
-                        if (s_first[w] > gi) atomicMin(&s_first[w], gi);
-                    }
-                } else if (v[k] == 0) {
-                    atomicAdd(sz, 1u);
-                    if (s_zero_first > gi) atomicMin(&s_zero_first, gi);
-                } else {
-                    const uint32_t w = (uint32_t) v[k] - lo;
-                    if (w < (uint32_t) kWin) {
-                        atomicAdd(&s_cnt[w], 1u);
-                        if (s_first[w] > gi) atomicMin(&s_first[w], gi);
-                    } else {
-                        atomicAdd(&freq[v[k]], 1u);
-                        atomicMin(&first[v[k]], gi);
-                        *outside = 1u;
+                d[k] = v[k] - hot0;
+                any |= d[k];
+            }
+            if (j + 8 <= end && any < (uint32_t) kLaneWin) {
+                uint64_t m = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    atomicAdd(&s_lane[d[k] * 64 + lane], 1u);
+                    m |= 1ull << d[k];
+                }
+                if (m & ~seen) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        if (!((seen >> d[k]) & 1u)) {
+                            seen |= 1ull << d[k];
+                            const uint32_t gi = j + k;
+                            if (s_first[v[k] - lo] > gi) atomicMin(&s_first[v[k] - lo], gi);
+                        }
                     }
                 }
-            }
-        }
-        // fold the register counters every kFoldChunks chunks and after the
-        // workgroup's last chunk (the same for all its threads): wave
-        // reduction, one LDS add per wave
-        if (++unfolded == kFoldChunks || c + gridDim.x >= nchunks) {
+            } else {
 #pragma unroll
-            for (int d = 0; d < kHot; d++) {
-                uint32_t cnt = (uint32_t) (((d < 8 ? hot_a : hot_b) >> (8 * (d & 7))) & 0xFF);
-                for (int o = 32; o > 0; o >>= 1) cnt += (uint32_t) __shfl_xor((int) cnt, o);
-                if (lane == 0 && cnt) atomicAdd(&s_cnt[hot0 + d - lo], cnt);
+                for (int k = 0; k < 8; k++)
+                    if (j + k < end) count_one(v[k], j + k, sz);
             }
-            hot_a = hot_b = 0;
-            unfolded = 0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -610,6 +623,16 @@ __global__ void __launch_bounds__(kHistThreads) hist_kernel(const uint16_t *__re
         }
 #pragma unroll
         for (int g = 0; g < kGroups; g++) q[g] = qn[g];
+    }
+    __syncthreads();
+    // lane counters into the window: four threads per symbol, 16 lanes each
+    {
+        const uint32_t d = threadIdx.x / 4, part = threadIdx.x % 4;
+        static_assert(kHistThreads == 4 * kLaneWin, "four threads per lane-window symbol");
+        uint32_t t = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) t += s_lane[d * 64 + part * 16 + i];
+        if (t) atomicAdd(&s_cnt[hot0 + d - lo], t);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < kWin; i += kHistThreads) {
@@ -648,11 +671,22 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
     // window [lo, lo + kWin) inside 1..65535, hot symbols inside the window
     const int lo = hist_window_lo(radius);
-    int hot0 = radius - kHot / 2;
+    int hot0 = radius - kLaneWin / 2;
     if (hot0 < lo) hot0 = lo;
-    if (hot0 + kHot > lo + kWin) hot0 = lo + kWin - kHot;
-    // persistent workgroups: a few per compute unit
-    const uint32_t grid = nchunks < 1024 ? nchunks : 1024;
+    if (hot0 + kLaneWin > lo + kWin) hot0 = lo + kWin - kLaneWin;
+    // persistent workgroups: as many as are resident at once (one round)
+    static uint32_t resident = 0;
+    if (!resident) {
+        int dev = 0, cus = 0, per = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hist_kernel, kHistThreads, 0) == hipSuccess &&
+            cus > 0 && per > 0)
+            resident = (uint32_t) (cus * per);
+        else
+            resident = 768;
+    }
+    const uint32_t grid = nchunks < resident ? nchunks : resident;
     if (nchunks)
         hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
                            d_first, d_chunk_zeros, lo, hot0, (int) (((uintptr_t) d_codes & 15) == 0), d_freq + 65536);

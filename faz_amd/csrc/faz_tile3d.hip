@@ -128,7 +128,10 @@ __device__ __forceinline__ Tile tile_at(const Level3D &L, uint32_t k0, uint32_t 
         B.cbase += (uint64_t) (b[a] / (2 * L.s)) * L.cstr[a];
         B.dbase += (uint64_t) (b[a] / L.s) * L.dstr[a];
     }
-    B.seg = L.level_base + L.block_prefix[((uint64_t) k0 * L.nb[1] + k1) * L.nb[2] + k2];
+    // read through the constant address space: a scalar load whose wait does
+    // not include the vector loads in flight (the next tile's values)
+    const auto *prefix = (const __attribute__((address_space(4))) uint64_t *) L.block_prefix;
+    B.seg = L.level_base + prefix[((uint64_t) k0 * L.nb[1] + k1) * L.nb[2] + k2];
     B.cnt0 = 16 * g2_of(B, p1) * g2_of(B, p2);
     B.cnt1 = g1_of(B, p0) * 16 * g2_of(B, p2);
     B.total = B.cnt0 + B.cnt1 + g1_of(B, p0) * g1_of(B, p1) * 16;
@@ -480,24 +483,45 @@ __device__ __forceinline__ void tile_recover_pass(float *T, const Tile &B, const
     }
     const int olo = first_lane ? 0 : -2 * ESA;
     const int ohi = last_lane ? 2 * kPerLane * ESA : (2 * kPerLane + 2) * ESA;
-    for (int r = 0; r < ROUNDS; r++) {
+    // first code of the lane's line in round r (through its owner), or ~0
+    // when the lane has no line in that round
+    auto line_code = [&](int r) -> uint64_t {
         const uint32_t line = line0 + (uint32_t) kLinesPerRound * r;
         const uint32_t u = line / NW, w = line - u * NW;
-        if ((LINES % kLinesPerRound == 0 || line < LINES) && u <= umax && w <= wmax) {
+        if (!((LINES % kLinesPerRound == 0 || line < LINES) && u <= umax && w <= wmax)) return ~0ull;
+        const uint32_t mu = (u | fu) == 0, mw = (w | fw) == 0;
+        const uint32_t c = mu | mw << 1;
+        const uint64_t lb = c == 0 ? O.base[0] : (c == 1 ? O.base[1] : (c == 2 ? O.base[2] : O.base[3]));
+        const uint32_t gw = c < 2 ? O.gw[0] : O.gw[2];
+        const uint32_t uu = mu ? O.ulast : u + fu - 1;
+        const uint32_t ww = mw ? O.wlast : w + fw - 1;
+        return lb + (uint64_t) (uu * gw + ww) * 16;
+    };
+    // the codes of round r + 1 are read while round r is decoded
+    uint64_t cl_next = line_code(0);
+    int code_next[kPerLane];
+    if (cl_next != ~0ull) {
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) code_next[j] = codes[cl_next + rk[j]];
+    }
+    for (int r = 0; r < ROUNDS; r++) {
+        const uint64_t cl = cl_next;
+        int code[kPerLane];
+#pragma unroll
+        for (int j = 0; j < kPerLane; j++) code[j] = code_next[j];
+        if (r + 1 < ROUNDS) {
+            cl_next = line_code(r + 1);
+            if (cl_next != ~0ull) {
+#pragma unroll
+                for (int j = 0; j < kPerLane; j++) code_next[j] = codes[cl_next + rk[j]];
+            }
+        }
+        if (cl != ~0ull) {
+            const uint32_t line = line0 + (uint32_t) kLinesPerRound * r;
+            const uint32_t u = line / NW, w = line - u * NW;
             const int base = (int) (u * ESU + w * ESW) + (int) t * 2 * kPerLane * ESA;
             float pr[kPerLane];
             line_pred<LINEAR, ESA>(T, base, olo, ohi, first_lane, last_lane, na, pr);
-            const uint32_t mu = (u | fu) == 0, mw = (w | fw) == 0;
-            const uint32_t c = mu | mw << 1;
-            const uint64_t lb = c == 0 ? O.base[0] : (c == 1 ? O.base[1] : (c == 2 ? O.base[2] : O.base[3]));
-            const uint32_t gw = c < 2 ? O.gw[0] : O.gw[2];
-            const uint32_t uu = mu ? O.ulast : u + fu - 1;
-            const uint32_t ww = mw ? O.wlast : w + fw - 1;
-            const uint64_t cl = lb + (uint64_t) (uu * gw + ww) * 16;
-            const uint16_t *cp = codes + cl;
-            int code[kPerLane];
-#pragma unroll
-            for (int j = 0; j < kPerLane; j++) code[j] = cp[rk[j]];
 #pragma unroll
             for (int j = 0; j < kPerLane; j++) {
                 const float rec = code[j] ? recover_point(pr[j], code[j], L.radius, L.eb) : outval[cl + rk[j]];
