@@ -218,6 +218,7 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint64_t> dec_qn;
     faz::DevBuf<uint64_t> dec_wtab;  // multi-codeword output table
     bool hist_cleared = false;       // freq/first already cleared on the stream for this compression
+    bool err_cleared = false;        // err_flag already cleared on the stream for this compression
     faz::DevBuf<uint64_t> dec_pos;
     uint32_t dec_rounds = 0;  // chunk groups of the last decode
     hipStream_t ustream = nullptr;  // uploads of the decoder (default priority)

@@ -6,13 +6,15 @@
 //
 // Workgroups are persistent over chunks of kEncChunk codes, so the code table
 // is staged in LDS once per workgroup: the window of at most kTabLds entries
-// that holds every symbol but 0 (the outlier code, far from the others, kept
-// in a register).  Tables whose symbols spread wider read global memory.  A
-// thread's 16 consecutive codes form one run of bits starting at the chunk
-// offset plus a workgroup prefix of lengths; the thread assembles the run in
-// a 64-bit register and writes it to an LDS bit buffer, words wholly inside
-// the run with plain stores and its first and last word, which it may share
-// with the neighbouring threads, with atomic ORs.  The chunk's words then go
+// that holds every symbol but 0, followed by symbol 0's entry (the outlier
+// code, far from the others), which every index outside the window clamps
+// to.  Tables whose symbols spread wider read global memory.  A thread's 16
+// consecutive codes form one run of bits starting at the chunk offset plus a
+// workgroup prefix of lengths; the thread assembles the run in a 64-bit
+// register (codewords of at most 32 bits are stored left-aligned, so each is
+// one shift and an OR) and writes it to an LDS bit buffer, words wholly
+// inside the run with plain stores and its first and last word, which it may
+// share with the neighbouring threads, with atomic ORs.  The chunk's words then go
 // out with plain stores; the first and last word of a chunk, shared with the
 // neighbouring chunks, with atomic ORs into the zeroed output.
 #include "faz_internal.h"
@@ -62,25 +64,36 @@ __device__ __forceinline__ Codes16 fetch_chunk(const uint16_t *__restrict__ code
     return fetch16(codes, base + threadIdx.x * kEncPerThread, min(base + kEncChunk, n), aligned);
 }
 
-// Table entries [win, win + nwin) live in LDS; e0 is the entry of symbol 0.
+// Table entries [win, win + nwin) live in LDS, then e0, the entry of symbol
+// 0.  narrow: every codeword has at most 32 bits (encode_entry's layouts).
 struct TabWindow {
     uint32_t win, nwin;
     uint64_t e0;
     uint32_t max_len;  // longest codeword of the table
+    __device__ __forceinline__ bool narrow() const { return max_len <= 32; }
 };
 
+// Symbol 0 lies below the window (every other symbol is positive), so its
+// index wraps above nwin and clamps onto e0's slot.
 template<bool LDS_TAB>
 __device__ __forceinline__ uint64_t entry(const uint64_t *s_tab, const uint64_t *__restrict__ tab, const TabWindow &w,
                                           uint16_t sym, int32_t offset) {
     if (!LDS_TAB) return tab[(int32_t) sym - offset];
-    const uint64_t e = s_tab[min((uint32_t) ((int32_t) sym - offset) - w.win, w.nwin - 1)];
-    return sym == 0 ? w.e0 : e;
+    return s_tab[min((uint32_t) ((int32_t) sym - offset) - w.win, w.nwin)];
 }
+
+__device__ __forceinline__ uint32_t entry_len(uint64_t e, bool narrow) {
+    return narrow ? (uint32_t) (e >> 32) : (uint32_t) e & 0xFF;
+}
+
+// LDS bytes of a staged window
+__host__ __device__ constexpr size_t tab_lds_bytes(uint32_t nwin) { return 8 * ((size_t) nwin + 1); }
 
 template<bool LDS_TAB>
 __device__ __forceinline__ void stage_table(uint64_t *s_tab, const uint64_t *__restrict__ tab, const TabWindow &w) {
     if (!LDS_TAB) return;
     for (uint32_t i = threadIdx.x; i < w.nwin; i += kEncThreads) s_tab[i] = tab[w.win + i];
+    if (threadIdx.x == 0) s_tab[w.nwin] = w.e0;
     __syncthreads();
 }
 
@@ -115,7 +128,7 @@ __device__ __forceinline__ void sizes_chunk(uint32_t c, const Codes16 &v, uint32
     uint32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < 16; k++)
-        if (FULL || my0 + k < end) sum += (uint32_t) entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) & 0xFF;
+        if (FULL || my0 + k < end) sum += entry_len(entry<LDS_TAB>(s_tab, tab, tw, v[k], offset), tw.narrow());
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) sum += (uint32_t) __shfl_xor((int) sum, o);
     if (threadIdx.x % 64 == 0) s_red[threadIdx.x / 64] = sum;
@@ -140,31 +153,22 @@ __device__ __forceinline__ void put_code(uint64_t c, uint32_t L, uint32_t pos, W
     if (sh + L > 64) word_or(w + 2, (uint32_t) (hi << (32 - sh)));
 }
 
-// The thread's run of codes e[0..15] (code << 8 | length, lengths <= 32) at
-// bit pos of the LDS bit buffer: MSB-first, assembled in a register; words
-// inside the run are stored, the run's first and last word OR-ed.
+// The thread's run of codes e[0..15] (narrow entries: length << 32 | the
+// codeword left-aligned in 32 bits) at bit pos of the zeroed LDS bit buffer,
+// MSB-first: each codeword shifted into a 64-bit pair of words at its bit
+// offset and OR-ed into the two words it can touch (the second is OR-ed with
+// zero when the codeword ends inside the first).  No branch per code, and
+// words shared with the neighbouring runs need no special case.
 __device__ __forceinline__ void put_run(const uint64_t *e, uint32_t pos, uint32_t *s_words) {
-    uint32_t w = pos >> 5, nacc = pos & 31;
-    uint64_t acc = 0;  // pending bits MSB-aligned, nacc of them (the first pos & 31 are not the thread's)
-    bool first = true;
+    uint32_t p = pos;
 #pragma unroll
     for (int k = 0; k < 16; k++) {
-        const uint32_t L = (uint32_t) e[k] & 0xFF;
-        if (L) {
-            acc |= ((e[k] >> 8) << (64 - L)) >> nacc;
-            nacc += L;
-        }
-        if (nacc >= 32) {
-            const uint32_t word = (uint32_t) (acc >> 32);
-            if (first) atomicOr(&s_words[w], word);
-            else s_words[w] = word;
-            first = false;
-            w++;
-            acc <<= 32;
-            nacc -= 32;
-        }
+        const uint64_t v = (uint64_t) (uint32_t) e[k] << (32 - (p & 31));
+        uint32_t *w = s_words + (p >> 5);
+        atomicOr(w, (uint32_t) (v >> 32));
+        atomicOr(w + 1, (uint32_t) v);
+        p += (uint32_t) (e[k] >> 32);
     }
-    if (nacc > (first ? (pos & 31) : 0u)) atomicOr(&s_words[w], (uint32_t) (acc >> 32));
 }
 
 // Bits of chunk c, which starts at bit g0 of the stream.
@@ -176,10 +180,11 @@ __device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_
     const uint32_t my0 = c * kEncChunk + threadIdx.x * kEncPerThread;
     uint64_t e[16];
     uint32_t mybits = 0;
+    const bool narrow = tw.narrow();
 #pragma unroll
     for (int k = 0; k < 16; k++) {
         e[k] = FULL || my0 + k < end ? entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) : 0;
-        mybits += (uint32_t) e[k] & 0xFF;
+        mybits += entry_len(e[k], narrow);
     }
     uint32_t total;
     const uint32_t before = wg_exclusive(mybits, s_wave, &total);
@@ -187,7 +192,7 @@ __device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_
     const uint32_t nw = (head + total + 31) / 32;
     const uint64_t gw = g0 >> 5;
     uint32_t pos = head + before;
-    if (nw <= kPackWords && tw.max_len <= 32) {
+    if (nw < kPackWords && narrow) {  // a run's last OR may touch word nw
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
         __syncthreads();
         put_run(e, pos, s_words);
@@ -204,9 +209,9 @@ __device__ __forceinline__ void pack_chunk(uint32_t c, const Codes16 &v, uint32_
         // long codewords: straight into the zeroed output
 #pragma unroll
         for (int k = 0; k < 16; k++) {
-            const uint32_t L = (uint32_t) e[k] & 0xFF;
-            if (L)
-                put_code(e[k] >> 8, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&words[gw + w], bswap32(bits)); });
+            const uint32_t L = entry_len(e[k], narrow);
+            const uint64_t c = narrow ? (uint64_t) ((uint32_t) e[k] >> (32 - L)) : e[k] >> 8;
+            if (L) put_code(c, L, pos, [&](uint32_t w, uint32_t bits) { atomicOr(&words[gw + w], bswap32(bits)); });
             pos += L;
         }
     }
@@ -335,7 +340,7 @@ __global__ void __launch_bounds__(kEncThreads) encode_onepass_kernel(
 #pragma unroll
         for (int k = 0; k < 16; k++) {
             e[k] = my0 + k < end ? entry<LDS_TAB>(s_tab, tab, tw, v[k], offset) : 0;
-            mybits += (uint32_t) e[k] & 0xFF;
+            mybits += (uint32_t) (e[k] >> 32);  // narrow entries only
         }
         uint32_t total;
         const uint32_t before = wg_exclusive(mybits, s_wave, &total);
@@ -432,7 +437,7 @@ void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_ta
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
     const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     if (w.nwin && w.nwin <= kTabLds)
-        hipLaunchKernelGGL(encode_sizes_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
+        hipLaunchKernelGGL(encode_sizes_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), tab_lds_bytes(w.nwin), st,
                            d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_bits, aligned);
     else
         hipLaunchKernelGGL(encode_sizes_kernel<false>, dim3(encode_grid(nchunks)), dim3(kEncThreads), 0, st, d_codes,
@@ -446,7 +451,7 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
     const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     if (w.nwin && w.nwin <= kTabLds)
-        hipLaunchKernelGGL(encode_pack_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), w.nwin * 8, st,
+        hipLaunchKernelGGL(encode_pack_kernel<true>, dim3(encode_grid(nchunks)), dim3(kEncThreads), tab_lds_bytes(w.nwin), st,
                            d_codes, (uint32_t) n, d_tab, tw, offset, d_chunk_off, d_words, aligned);
     else
         hipLaunchKernelGGL(encode_pack_kernel<false>, dim3(encode_grid(nchunks)), dim3(kEncThreads), 0, st, d_codes,
@@ -465,7 +470,7 @@ void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_
     const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
     const dim3 grid(encode_grid(nchunks));
     if (w.nwin && w.nwin <= kTabLds)
-        hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), w.nwin * 8, st, d_codes, (uint32_t) n,
+        hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), tab_lds_bytes(w.nwin), st, d_codes, (uint32_t) n,
                            d_tab, tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_ticket, d_total_bits,
                            d_err, aligned);
     else

@@ -691,7 +691,8 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         ctx->prefix_key = prefix_key;
         ctx->prefix_at = ctx->prefix.p;
     }
-    launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
+    if (!ctx->err_cleared) launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
+    ctx->err_cleared = false;
 
     if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[0], st));
     if (pl.anchor) {
@@ -939,10 +940,8 @@ static int encode_table(faz_gpu_ctx *ctx, const faz_huff_table *t, EncodeWindow 
     }
     if (maxlen > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", maxlen);
     *maxbits = maxlen;
-    auto pack = [&](int64_t k) {
-        const uint32_t L = t->len[k];
-        return L ? ((t->code[k] >> (64 - L)) << 8) | L : (uint64_t) 0;
-    };
+    const bool narrow = maxlen <= 32;
+    auto pack = [&](int64_t k) { return encode_entry(t->code[k], t->len[k], narrow); };
     ctx->h_tab.ensure((size_t) ns);
     ew = EncodeWindow{0, 0, 0};
     ew.max_len = maxlen;
@@ -1017,20 +1016,20 @@ faz_status faz_gpu_huffman_encode(faz_gpu_ctx *ctx, const uint16_t *d_codes, siz
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
-    // one packed entry per symbol: codeword right-aligned above an 8-bit length
+    // one packed entry per symbol (encode_entry)
     ctx->h_tab.ensure((size_t) t->n_states);
     int64_t lo = -1, hi = -1;  // table indices of the present symbols other than 0
     uint32_t maxlen = 0;
     for (int32_t i = 0; i < t->n_states; i++) {
         const uint32_t L = t->len[i];
         if (L > 56) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "Huffman codeword of %u bits (at most 56 supported)", L);
-        ctx->h_tab[i] = L ? ((t->code[i] >> (64 - L)) << 8) | L : 0;
         maxlen = std::max(maxlen, L);
         if (L && i + t->offset != 0) {
             if (lo < 0) lo = i;
             hi = i;
         }
     }
+    for (int32_t i = 0; i < t->n_states; i++) ctx->h_tab[i] = encode_entry(t->code[i], t->len[i], maxlen <= 32);
     EncodeWindow ew{0, 0, 0};
     ew.max_len = maxlen;
     if (lo >= 0 && hi - lo + 1 <= (int64_t) encode_lds_entries()) {
@@ -1258,33 +1257,38 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         HIPCHECK(hipSetDevice(ctx->device));
         HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
         float mn, mx;
-        // the histogram's arrays are cleared while the host waits for the range
         HIPCHECK(ctx->minmax.ensure(kRangeScratch));
         HIPCHECK(ctx->freq.ensure(kHistSlots));
         HIPCHECK(ctx->first.ensure(65536));
-        launch_range(d_in, num, ctx->minmax.p, ctx->stream);
-        launch_hist_clear(ctx->freq.p, ctx->first.p, ctx->stream);
-        HIPCHECK(hipGetLastError());
         if (ctx->dev_bounds) {
             // the level bounds follow from the range on the device: no host
             // round trip before the level launches (the plan does not depend
-            // on the bound; 1.0 stands in for it)
+            // on the bound; 1.0 stands in for it).  The range pass also clears
+            // the histogram and the error flag, its fold computes the bounds.
             q.eb = 1.0;
             int rc = FAZ_OK;
             const Plan &pl = cached_plan(ctx, &q, rc);
             if (rc) return rc;
             HIPCHECK(ctx->lvl.ensure(kMaxLevels + 1));
-            launch_level_bounds(ctx->minmax.p, eb, level_scale(pl, &q), ctx->lvl.p, ctx->stream);
+            HIPCHECK(ctx->err_flag.ensure(1));
+            launch_range_rel(d_in, num, ctx->minmax.p, ctx->freq.p, ctx->first.p, ctx->err_flag.p, eb,
+                             level_scale(pl, &q), ctx->lvl.p, ctx->stream);
             HIPCHECK(hipGetLastError());
             ctx->hist_cleared = true;
+            ctx->err_cleared = true;
             HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
             rc = compress_device(ctx, d_in, d_recon, &q, -1.0, ctx->lvl.p);
             ctx->hist_cleared = false;
+            ctx->err_cleared = false;
             if (rc) return rc;
             ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
             *raw_size = pending_size(ctx);
             return FAZ_OK;
         }
+        // the histogram's arrays are cleared while the host waits for the range
+        launch_range(d_in, num, ctx->minmax.p, ctx->stream);
+        launch_hist_clear(ctx->freq.p, ctx->first.p, ctx->stream);
+        HIPCHECK(hipGetLastError());
         int rc = range_readback(ctx, &mn, &mx);
         if (rc) return rc;
         ctx->hist_cleared = true;

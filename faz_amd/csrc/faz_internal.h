@@ -157,6 +157,11 @@ void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t s
 // the bounded-quantizer test of each level.
 void launch_level_bounds(const uint32_t *d_minmax, double rel, const LevelScale &sc, LevelEB *d_out,
                          hipStream_t st);
+// The value range with the REL compression's set-up folded in: the histogram
+// arrays and the error flag cleared by the range pass, the level bounds
+// computed by the fold (as launch_level_bounds).
+void launch_range_rel(const float *d_in, size_t n, uint32_t *d_minmax, uint32_t *d_freq, uint32_t *d_first,
+                      uint32_t *d_err, double rel, const LevelScale &sc, LevelEB *d_lvl, hipStream_t st);
 void launch_anchor_grid(const float *d_in, float *d_recon, uint16_t *d_codes, float *d_outval, int N,
                         const uint32_t *dims, const uint64_t *offs, uint32_t step, uint64_t count, hipStream_t st);
 void launch_anchor_scatter(const float *d_vals, float *d_out, int N, const uint32_t *dims, const uint64_t *offs,
@@ -191,11 +196,20 @@ void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uin
                       uint64_t *d_total, hipStream_t st);
 void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outval, const uint32_t *d_chunk_zeros,
                           const uint64_t *d_chunk_base, float *d_unpred, hipStream_t st);
-// Huffman encode (faz_encode.hip).  tab holds one entry per symbol - offset:
-// the codeword right-aligned in bits 8..63 and its length (<= 56) in bits 0..7.
-// When every present symbol but 0 lies in table indices [win, win + nwin)
-// with nwin <= encode_lds_entries(), that window is staged in LDS and e0 is
-// symbol 0's entry; nwin = 0 reads the whole table from global memory.
+// Huffman encode (faz_encode.hip).  tab holds one entry per symbol - offset
+// (encode_entry).  When every present symbol but 0 lies in table indices
+// [win, win + nwin) with nwin <= encode_lds_entries(), that window is staged
+// in LDS and e0 is symbol 0's entry; nwin = 0 reads the whole table from
+// global memory.
+// Entry of a codeword of L bits (MSB-first in the top L bits of code): when
+// every codeword of the table has at most 32 bits (narrow), the codeword
+// left-aligned in the low word and L in the high word, so the packer ORs it
+// in with one shift; otherwise the codeword right-aligned in bits 8..63 and L
+// (<= 56) in bits 0..7.
+__host__ __device__ inline uint64_t encode_entry(uint64_t code, uint32_t L, bool narrow) {
+    if (!L) return 0;
+    return narrow ? ((uint64_t) L << 32) | (uint32_t) (code >> 32) : ((code >> (64 - L)) << 8) | L;
+}
 struct EncodeWindow {
     uint32_t win, nwin;
     uint64_t e0;

@@ -323,8 +323,23 @@ __device__ __forceinline__ uint32_t f2key(float f) {
 // atomics from every workgroup serialised in the L2 and cost a quarter of
 // the pass; kRangeGrid workgroups of four float4 loads per lane in flight
 // read 512 MiB at about 6 TB/s.)
+// When clr.freq is set, the launch also clears the histogram arrays and
+// the error flag for the compression that follows (a launch of its own cost
+// more than the stores).
+struct RangeClear {
+    uint32_t *freq, *first, *err;
+};
+
 __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in, size_t n,
-                                                   uint32_t *__restrict__ part) {
+                                                   uint32_t *__restrict__ part, RangeClear clr) {
+    if (clr.freq) {
+        const uint32_t nt = gridDim.x * blockDim.x;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (uint32_t) kHistSlots; i += nt) {
+            clr.freq[i] = 0u;
+            if (i < 65536u) clr.first[i] = 0xFFFFFFFFu;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) *clr.err = 0u;
+    }
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
     const size_t stride = (size_t) gridDim.x * blockDim.x;
     const size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
@@ -374,8 +389,13 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
     }
 }
 
+__device__ void level_bounds(const uint32_t *minmax, double rel, const LevelScale &sc, LevelEB *out, int i);
+
+// With lvl set, the same workgroup goes on to the level bounds
+// (level_bounds_kernel's arithmetic, one thread per bound).
 __global__ void __launch_bounds__(256) range_fold_kernel(const uint32_t *__restrict__ part, uint32_t np,
-                                                        uint32_t *__restrict__ minmax) {
+                                                        uint32_t *__restrict__ minmax, double rel, LevelScale sc,
+                                                        LevelEB *__restrict__ lvl) {
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
     for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
         lo = min(lo, part[2 * i]);
@@ -399,7 +419,13 @@ __global__ void __launch_bounds__(256) range_fold_kernel(const uint32_t *__restr
         }
         minmax[0] = lo;
         minmax[1] = hi;
+        slo[0] = lo;
+        shi[0] = hi;
     }
+    if (!lvl) return;
+    __syncthreads();
+    const uint32_t folded[2] = {slo[0], shi[0]};
+    if ((int) threadIdx.x <= sc.n) level_bounds(folded, rel, sc, lvl, (int) threadIdx.x - 1);
 }
 
 // The range's keys back to floats (the inverse of f2key).
@@ -412,10 +438,8 @@ __device__ __forceinline__ float key2f(uint32_t key) {
 // conversion of SZ_compress), each level's bound from it (SIC:384-422), the
 // reciprocal, the largest float not above the bound, and the bounded
 // quantizer's test 4.5 max|x| / eb + 8 < 2^30.
-__global__ void level_bounds_kernel(const uint32_t *__restrict__ minmax, double rel, LevelScale sc,
-                                    LevelEB *__restrict__ out) {
-    const int i = (int) threadIdx.x - 1;  // -1: the base bound
-    if (i >= sc.n) return;
+// i = -1: the base bound.
+__device__ void level_bounds(const uint32_t *minmax, double rel, const LevelScale &sc, LevelEB *out, int i) {
     const float mn = key2f(minmax[0]), mx = key2f(minmax[1]);
     const double base = rel * (double) (float) (mx - mn);
     const double amn = fabs((double) mn), amx = fabs((double) mx);
@@ -433,6 +457,12 @@ __global__ void level_bounds_kernel(const uint32_t *__restrict__ minmax, double 
     out[i + 1] = r;
 }
 
+__global__ void level_bounds_kernel(const uint32_t *__restrict__ minmax, double rel, LevelScale sc,
+                                    LevelEB *__restrict__ out) {
+    const int i = (int) threadIdx.x - 1;
+    if (i < sc.n) level_bounds(minmax, rel, sc, out, i);
+}
+
 void launch_level_bounds(const uint32_t *d_minmax, double rel, const LevelScale &sc, LevelEB *d_out,
                          hipStream_t st) {
     hipLaunchKernelGGL(level_bounds_kernel, dim3(1), dim3(kMaxLevels + 1), 0, st, d_minmax, rel, sc, d_out);
@@ -441,8 +471,18 @@ void launch_level_bounds(const uint32_t *d_minmax, double rel, const LevelScale 
 void launch_range(const float *d_in, size_t n, uint32_t *d_minmax, hipStream_t st) {
     const size_t want = (n / 4 + 255) / 256;
     const unsigned grid = (unsigned) std::min<size_t>(std::max<size_t>(want, 1), kRangeGrid);
-    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax + 2);
-    hipLaunchKernelGGL(range_fold_kernel, dim3(1), dim3(256), 0, st, d_minmax + 2, grid, d_minmax);
+    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax + 2, RangeClear{});
+    hipLaunchKernelGGL(range_fold_kernel, dim3(1), dim3(256), 0, st, d_minmax + 2, grid, d_minmax, 0.0, LevelScale{},
+                       (LevelEB *) nullptr);
+}
+
+void launch_range_rel(const float *d_in, size_t n, uint32_t *d_minmax, uint32_t *d_freq, uint32_t *d_first,
+                      uint32_t *d_err, double rel, const LevelScale &sc, LevelEB *d_lvl, hipStream_t st) {
+    const size_t want = (n / 4 + 255) / 256;
+    const unsigned grid = (unsigned) std::min<size_t>(std::max<size_t>(want, 1), kRangeGrid);
+    hipLaunchKernelGGL(range_kernel, dim3(grid), dim3(256), 0, st, d_in, n, d_minmax + 2,
+                       RangeClear{d_freq, d_first, d_err});
+    hipLaunchKernelGGL(range_fold_kernel, dim3(1), dim3(256), 0, st, d_minmax + 2, grid, d_minmax, rel, sc, d_lvl);
 }
 
 // ---------------------------------------------------------------------------

@@ -43,13 +43,11 @@ unsigned char *g_ptr[kMaxDevices] = {};
 hipError_t g_err[kMaxDevices] = {};
 bool g_done[kMaxDevices] = {};
 
-// Packed entry per table state: codeword right-aligned above an 8-bit length.
+// Packed entry per table state (encode_entry).
 __global__ void pack_table_kernel(const uint64_t *__restrict__ code, const uint8_t *__restrict__ len, uint32_t ns,
-                                  uint64_t *__restrict__ tab) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x) {
-        const uint32_t L = len[i];
-        tab[i] = L ? ((code[i] >> (64 - L)) << 8) | L : 0ull;
-    }
+                                  int narrow, uint64_t *__restrict__ tab) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
+        tab[i] = encode_entry(code[i], len[i], narrow != 0);
 }
 
 }  // namespace
@@ -120,7 +118,7 @@ faz_status faz_gpu_huffman_encode_noalloc(faz_gpu_ctx *ctx, const uint16_t *d_co
         ew.nwin = (uint32_t) (hi - lo + 1);
         const int64_t zi = -(int64_t) t->offset;
         if (zi >= 0 && zi < (int64_t) ns && t->len[zi])
-            ew.e0 = ((t->code[zi] >> (64 - t->len[zi])) << 8) | t->len[zi];
+            ew.e0 = encode_entry(t->code[zi], t->len[zi], maxlen <= 32);
     }
     uint64_t *tab = reinterpret_cast<uint64_t *>(base + kTabOff);
     uint64_t *code = reinterpret_cast<uint64_t *>(base + kCodeOff);
@@ -134,7 +132,7 @@ faz_status faz_gpu_huffman_encode_noalloc(faz_gpu_ctx *ctx, const uint16_t *d_co
     HIPCHECK(hipMemcpyAsync(code, t->code, (size_t) ns * 8, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(len, t->len, ns, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(pack_table_kernel, dim3(std::min<uint32_t>((ns + 255) / 256, 1024)), dim3(256), 0, st, code,
-                       len, ns, tab);
+                       len, ns, (int) (maxlen <= 32), tab);
     launch_encode_sizes(d_codes, n, tab, ew, t->offset, chunk_bits, st);
     launch_scan_u64(chunk_bits, chunk_off, nchunks, tiles, total, st);
     HIPCHECK(hipGetLastError());
