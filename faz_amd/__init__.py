@@ -664,8 +664,11 @@ class Engine:
         self._check(self._L.faz_gpu_fetch_raw(self.h, _np_ptr(buf), size), "fetch_raw")
         return buf.tobytes()
 
-    def set_profiling(self, on):
-        self._check(self._L.faz_gpu_set_profiling(self.h, 1 if on else 0), "set_profiling")
+    def set_profiling(self, on, phases=False):
+        """on: HIP-event timing of the interpolation kernels (profile());
+        phases: the compression's phase timings (timings())."""
+        flags = (1 if on else 0) | (2 if phases else 0)
+        self._check(self._L.faz_gpu_set_profiling(self.h, flags), "set_profiling")
 
     def profile(self):
         ms, la, pts = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()

@@ -164,8 +164,11 @@ struct faz_gpu_ctx {
     faz::PinnedVec<uint64_t> h_tab;  // packed code table of the last encode (pinned: asynchronous upload)
     faz::PinnedVec<uint32_t> h_dec;  // the decoder's lookup tables and tree, staged for upload
     faz::PendingRaw pending;
-    // per-launch timing of the interpolation pass kernels
+    // timing of the interpolation kernels (whole-tile launches timed as one
+    // span per run of consecutive launches); phase_events: the compression's
+    // phase timings (each event record costs the stream a few microseconds)
     bool profile = false;
+    bool phase_events = false;
     struct ProfBucket {
         double ms = 0, bytes = 0;  // bytes: algorithmic HBM bytes of the launches
         uint64_t launches = 0, points = 0;
@@ -178,6 +181,7 @@ struct faz_gpu_ctx {
         uint32_t ev;
         uint64_t points;
         double bytes;
+        uint32_t launches;
     };
     std::vector<ProfRec> prof_pending;
     // 3-D fields: one fused launch per level (0 = per-pass launches)

@@ -377,7 +377,7 @@ void flush_profile(faz_gpu_ctx *ctx) {
     for (const auto &r: ctx->prof_pending) {
         auto &b = ctx->prof[r.bucket];
         b.ms += elapsed(ctx->prof_ev[2 * r.ev], ctx->prof_ev[2 * r.ev + 1]);
-        b.launches++;
+        b.launches += r.launches;
         b.points += r.points;
         b.bytes += r.bytes;
     }
@@ -694,7 +694,9 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     if (!ctx->err_cleared) launch_fill_u32(ctx->err_flag.p, 1, 0u, st);
     ctx->err_cleared = false;
 
-    if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    // phase events: always on the synchronous path, on request when deferred
+    const bool phase_ev = !enqueue && (!defer || ctx->phase_events);
+    if (phase_ev) HIPCHECK(hipEventRecord(ctx->ev[0], st));
     if (pl.anchor) {
         launch_anchor_grid(d_in, d_recon, ctx->codes.p, ctx->outval.p, pl.N, pl.dims, pl.offs,
                            (uint32_t) p->max_step, pl.anchor_count, st);
@@ -737,21 +739,46 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     }
     // with level launches in the plan the profile times those only: the pass
     // launches of the coarse levels are a few microseconds each, and event
-    // records around them would slow their enqueue below the device's pace
+    // records around them would slow their enqueue below the device's pace.
+    // A run of consecutive whole-tile launches is timed as one span (an event
+    // record between two launches idles the stream for a few microseconds).
     bool any_level = false;
     for (size_t i = 0; i < nlv; i++) any_level |= kind[i] != 0;
     const bool prof_pass = profile && !any_level;
+    if (profile) ctx->prof_pending.clear();
     size_t k = 0;
     for (size_t i = 0; i < nlv; i++) {
         LevelPlan &L = pl.lv[i];
         if (kind[i]) {
-            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+            const bool opens = kind[i] != 2 || i == 0 || kind[i - 1] != 2;
+            const bool closes = kind[i] != 2 || i + 1 == nlv || kind[i + 1] != 2;
+            if (profile && opens) {
+                HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
+                ctx->prof_pending.push_back({kind[i] == 2 ? 0 : 2, (uint32_t) k, 0, 0.0, 0});
+            }
             if (kind[i] == 2)
                 launch_interp_tile3d(g3[i], d_in, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, ctx->tile_grid, st);
             else
                 launch_interp_level3d(g3[i], d_in, d_recon, ctx->codes.p, ctx->outval.p, ctx->err_flag.p, st);
-            if (profile) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
-            k++;
+            if (profile) {
+                // read each original and write its code; the reconstruction
+                // traffic is the coarser lattice read and this level's
+                // lattice written, where that happens
+                uint64_t lat = 1, coarse = 1;
+                for (int a = 0; a < 3; a++) {
+                    lat *= (pl.dims[a] - 1) / g3[i].s + 1;
+                    coarse *= (pl.dims[a] - 1) / (2 * g3[i].s) + 1;
+                }
+                const bool writes = kind[i] == 1 || g3[i].dst != nullptr;
+                faz_gpu_ctx::ProfRec &rec = ctx->prof_pending.back();
+                rec.points += L.codes;
+                rec.bytes += 6.0 * L.codes + 4.0 * coarse + (writes ? 4.0 * lat : 0.0);
+                rec.launches++;
+                if (closes) {
+                    HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+                    k++;
+                }
+            }
             continue;
         }
         for (auto &d: L.passes) {
@@ -759,12 +786,16 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             d.ebp = d_lvl ? d_lvl + 1 + i : nullptr;
             if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k], st));
             launch_interp_pass(d, d_in, d_recon, ctx->codes.p, ctx->outval.p, nullptr, ctx->err_flag.p, st);
-            if (prof_pass) HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
-            k++;
+            if (prof_pass) {
+                HIPCHECK(hipEventRecord(ctx->prof_ev[2 * k + 1], st));
+                // original in, reconstruction and code out
+                if (d.total) ctx->prof_pending.push_back({1, (uint32_t) k, d.total, 10.0 * d.total, 1});
+                k++;
+            }
         }
     }
     HIPCHECK(hipGetLastError());
-    if (!enqueue) HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    if (phase_ev) HIPCHECK(hipEventRecord(ctx->ev[1], st));
     // histogram, first positions and outlier compaction
     const bool cleared = ctx->hist_cleared && ctx->freq.cap >= kHistSlots && ctx->first.cap >= 65536;
     ctx->hist_cleared = false;
@@ -791,7 +822,7 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
             out->levels = pl.levels;
             return FAZ_OK;
         }
-        HIPCHECK(hipEventRecord(ctx->ev[2], st));
+        if (phase_ev) HIPCHECK(hipEventRecord(ctx->ev[2], st));
     } else {
         uint32_t errflag = 0;
         HIPCHECK(hipMemcpyAsync(&n_unpred, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
@@ -809,37 +840,9 @@ static int interp_compress(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
         ctx->timings[6] = ctx->timings[1];
     }
-    if (profile) {
-        // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block
-        // level launches; the event times are read once the stream has drained
-        ctx->prof_pending.clear();
-        size_t j = 0;
-        for (size_t i = 0; i < nlv; i++) {
-            const LevelPlan &L = pl.lv[i];
-            if (kind[i]) {
-                faz_gpu_ctx::ProfRec rec{kind[i] == 2 ? 0 : 2, (uint32_t) j, L.codes, 0.0};
-                // read each original and write its code; the reconstruction
-                // traffic is the coarser lattice read and this level's
-                // lattice written, where that happens
-                uint64_t lat = 1, coarse = 1;
-                for (int a = 0; a < 3; a++) {
-                    lat *= (pl.dims[a] - 1) / g3[i].s + 1;
-                    coarse *= (pl.dims[a] - 1) / (2 * g3[i].s) + 1;
-                }
-                const bool writes = kind[i] == 1 || g3[i].dst != nullptr;
-                rec.bytes = 6.0 * L.codes + 4.0 * coarse + (writes ? 4.0 * lat : 0.0);
-                ctx->prof_pending.push_back(rec);
-                j++;
-                continue;
-            }
-            for (auto &d: L.passes) {
-                // original in, reconstruction and code out
-                if (d.total && prof_pass) ctx->prof_pending.push_back({1, (uint32_t) j, d.total, 10.0 * d.total});
-                j++;
-            }
-        }
-        if (!defer) flush_profile(ctx);  // the stream was drained above
-    }
+    // bucket 0: whole-tile level launches, 1: per-pass launches, 2: per-block
+    // level launches; the event times are read once the stream has drained
+    if (profile && !defer) flush_profile(ctx);  // the stream was drained above
     ctx->hist_codes = ctx->codes.p;
     ctx->hist_n = pl.total_codes;
 
@@ -1151,7 +1154,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         }
     }
     const auto th1 = std::chrono::steady_clock::now();
-    HIPCHECK(hipEventRecord(ctx->ev[4], st));
+    if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[4], st));
     faz_huff_table ht{pr.tab.offset, pr.tab.n_states, pr.tab.code.data(), pr.tab.len.data()};
     rc = encode_enqueue(ctx, io.d_codes, io.n_codes, &ht);
     if (rc) return rc;
@@ -1161,7 +1164,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     finish_huffman_tree(pr.tab);  // while the encode runs
     HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hs + 6, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipEventRecord(ctx->ev[5], st));
+    if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[5], st));
     HIPCHECK(spin_wait(st));
     flush_profile(ctx);
     uint64_t bits;
@@ -1172,12 +1175,16 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
                  (unsigned long long) want_bits);
     const size_t nbytes = (size_t) ((bits + 7) / 8);
     io.n_unpred = n_unpred;
-    ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
-    ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
-    ctx->timings[6] = ctx->timings[1];
-    ctx->timings[3] = elapsed(ctx->ev[2], ctx->ev[4]);
-    ctx->timings[4] = elapsed(ctx->ev[4], ctx->ev[5]);
-    ctx->timings[5] = elapsed(ctx->ev[0], ctx->ev[5]);
+    if (ctx->phase_events) {
+        ctx->timings[1] = elapsed(ctx->ev[0], ctx->ev[1]);
+        ctx->timings[2] = elapsed(ctx->ev[1], ctx->ev[2]);
+        ctx->timings[6] = ctx->timings[1];
+        ctx->timings[3] = elapsed(ctx->ev[2], ctx->ev[4]);
+        ctx->timings[4] = elapsed(ctx->ev[4], ctx->ev[5]);
+        ctx->timings[5] = elapsed(ctx->ev[0], ctx->ev[5]);
+    } else {
+        for (int i = 0; i < 7; i++) ctx->timings[i] = 0;
+    }
     // engine buffer header (SIC:489-512)
     EngineHeader &h = pr.h;
     h.N = p->N;
@@ -1255,7 +1262,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         size_t num = 1;
         for (int i = 0; i < p->N; i++) num *= p->dims[i];
         HIPCHECK(hipSetDevice(ctx->device));
-        HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
+        if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
         float mn, mx;
         HIPCHECK(ctx->minmax.ensure(kRangeScratch));
         HIPCHECK(ctx->freq.ensure(kHistSlots));
@@ -1276,12 +1283,12 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
             HIPCHECK(hipGetLastError());
             ctx->hist_cleared = true;
             ctx->err_cleared = true;
-            HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
+            if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
             rc = compress_device(ctx, d_in, d_recon, &q, -1.0, ctx->lvl.p);
             ctx->hist_cleared = false;
             ctx->err_cleared = false;
             if (rc) return rc;
-            ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
+            if (ctx->phase_events) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
             *raw_size = pending_size(ctx);
             return FAZ_OK;
         }
@@ -1292,7 +1299,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
         int rc = range_readback(ctx, &mn, &mx);
         if (rc) return rc;
         ctx->hist_cleared = true;
-        HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
+        if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[7], ctx->stream));
         q.eb = eb * (double) (float) (mx - mn);
         absmax = std::max(std::fabs((double) mn), std::fabs((double) mx));
     } else {
@@ -1301,7 +1308,7 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
     int rc = compress_device(ctx, d_in, d_recon, &q, absmax);
     ctx->hist_cleared = false;
     if (rc) return rc;
-    if (rel_mode) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
+    if (rel_mode && ctx->phase_events) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
     *raw_size = pending_size(ctx);
     return FAZ_OK;
 }
@@ -1316,7 +1323,8 @@ faz_status faz_gpu_fetch_raw(faz_gpu_ctx *ctx, uint8_t *h_raw, size_t cap) {
 
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on) {
     if (!ctx) return FAZ_ERR_INVALID;
-    ctx->profile = on != 0;
+    ctx->profile = (on & 1) != 0;
+    ctx->phase_events = (on & 2) != 0;
     for (auto &b: ctx->prof) b = faz_gpu_ctx::ProfBucket();
     return FAZ_OK;
 }

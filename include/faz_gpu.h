@@ -544,7 +544,9 @@ faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const f
                                     const float *mu, const float *sigma, faz_tune_out *out);
 
 /* Timing of the last faz_gpu_interp_raw_f32 / faz_sz_compress_f32 call, in
- * milliseconds, measured with HIP events on the context stream:
+ * milliseconds, measured with HIP events on the context stream (entries 0-6
+ * of the device-resident compressions only while faz_gpu_set_profiling has
+ * bit 1 set, else 0):
  * [0] range, [1] interpolation+quantization, [2] histogram+outlier compaction,
  * [3] host tree build, [4] encode, [5] device total, [6] dominant kernel
  * (interpolation pass launches) summed, [8] host time from the histogram's
@@ -554,12 +556,16 @@ faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const f
  * [7] to the whole call, host clock. */
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
-/* Per-launch HIP-event timing of the interpolation kernels, accumulated while
- * enabled for each kernel family separately; faz_gpu_profile_read reports the
+/* on: bit 0 enables HIP-event timing of the interpolation kernels,
+ * accumulated for each kernel family separately (consecutive whole-tile level
+ * launches are timed as one span, so the average launch time includes the
+ * few idle microseconds between them); faz_gpu_profile_read reports the
  * family that took longest (the dominant kernel): total milliseconds,
  * launches and predicted points, and faz_gpu_profile_kernel which family that
  * is (0 whole-tile 3-D level kernel, 1 per-pass kernel, 2 per-block 3-D level
- * kernel).  Enabling resets the counters. */
+ * kernel).  Bit 1 enables the phase timings of faz_gpu_last_timings.  Each
+ * event record idles the stream for a few microseconds.  Enabling resets the
+ * counters. */
 faz_status faz_gpu_set_profiling(faz_gpu_ctx *ctx, int on);
 faz_status faz_gpu_profile_read(const faz_gpu_ctx *ctx, double *ms, uint64_t *launches, uint64_t *points);
 int faz_gpu_profile_kernel(const faz_gpu_ctx *ctx);
