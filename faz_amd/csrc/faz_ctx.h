@@ -23,8 +23,8 @@ template<class T>
 struct HostPinned {
     T *p = nullptr;
     bool pinned = false;
-    explicit HostPinned(size_t n) {
-        if (hipHostMalloc((void **) &p, n * sizeof(T), 0) == hipSuccess) {
+    explicit HostPinned(size_t n, unsigned flags = 0) {
+        if (hipHostMalloc((void **) &p, n * sizeof(T), flags) == hipSuccess) {
             pinned = true;
         } else {
             (void) hipGetLastError();
@@ -161,6 +161,10 @@ struct faz_gpu_ctx {
     std::vector<uint32_t> h_freq = std::vector<uint32_t>(65536), h_first = std::vector<uint32_t>(65536);
     // pinned: histogram and first positions, then 8 words of scalars
     faz::HostPinned<uint32_t> hp_hist{2 * 65536 + 8};
+    // coherent pinned words that kernels write directly: the histogram block
+    // and base bound, then (at kMailReport) the encode's bit total and error flag
+    static constexpr size_t kMailReport = faz::kHistPackWords + 4;
+    faz::HostPinned<uint32_t> mailbox{kMailReport + 4, hipHostMallocCoherent};
     faz::PinnedVec<uint64_t> h_tab;  // packed code table of the last encode (pinned: asynchronous upload)
     faz::PinnedVec<uint32_t> h_dec;  // the decoder's lookup tables and tree, staged for upload
     faz::PendingRaw pending;

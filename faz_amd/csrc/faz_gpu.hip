@@ -1098,12 +1098,21 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     if (rc) return rc;
     // one readback: the histogram's window around the radius, symbol 0, the
     // outside-window flag, the error flag and the outlier count
+    // (the pack kernel writes it into the coherent pinned mailbox directly,
+    // or into device memory and a copy brings it back), and the base bound
+    // computed on the device with it
     uint32_t *hf = ctx->hp_hist.data(), *hs = ctx->hp_hist.data() + 2 * 65536;
-    HIPCHECK(ctx->hist_pack.ensure(kHistPackWords));
-    launch_hist_pack(ctx->freq.p, ctx->first.p, p->radius, ctx->scalars.p, ctx->err_flag.p, ctx->hist_pack.p, st);
-    HIPCHECK(hipMemcpyAsync(hf, ctx->hist_pack.p, kHistPackWords * 4, hipMemcpyDeviceToHost, st));
-    // the base bound computed on the device comes back with it
-    if (d_lvl) HIPCHECK(hipMemcpyAsync(hf + kHistPackWords, &d_lvl->eb, 8, hipMemcpyDeviceToHost, st));
+    const bool mail = ctx->mailbox.pinned;
+    const uint32_t *pk = mail ? ctx->mailbox.data() : hf;
+    if (mail) {
+        launch_hist_pack(ctx->freq.p, ctx->first.p, p->radius, ctx->scalars.p, ctx->err_flag.p, ctx->mailbox.data(),
+                         st, d_lvl ? &d_lvl->eb : nullptr);
+    } else {
+        HIPCHECK(ctx->hist_pack.ensure(kHistPackWords));
+        launch_hist_pack(ctx->freq.p, ctx->first.p, p->radius, ctx->scalars.p, ctx->err_flag.p, ctx->hist_pack.p, st);
+        HIPCHECK(hipMemcpyAsync(hf, ctx->hist_pack.p, kHistPackWords * 4, hipMemcpyDeviceToHost, st));
+        if (d_lvl) HIPCHECK(hipMemcpyAsync(hf + kHistPackWords, &d_lvl->eb, 8, hipMemcpyDeviceToHost, st));
+    }
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
     // the outlier gather runs while the host waits for the counts and builds the tree
     launch_unpred_gather(io.d_codes, io.n_codes, ctx->outval.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->unpred.p,
@@ -1113,23 +1122,23 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     const auto th0 = std::chrono::steady_clock::now();
     double eb = p->eb;
     if (d_lvl) {
-        memcpy(&eb, hf + kHistPackWords, 8);
+        memcpy(&eb, pk + kHistPackWords, 8);
         if (!(eb > 0)) CTX_FAIL(FAZ_ERR_INVALID, "error bound must be positive");
     }
-    const uint32_t errflag = hf[kHistPackFreq0 + 3];
+    const uint32_t errflag = pk[kHistPackFreq0 + 3];
     if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
     uint64_t n_unpred;
-    memcpy(&n_unpred, hf + kHistPackFreq0 + 4, 8);
+    memcpy(&n_unpred, pk + kHistPackFreq0 + 4, 8);
     PendingRaw &pr = ctx->pending;
     int brc;
-    // a symbol outside the window: the full arrays replace the window in hf
-    const bool outside = hf[kHistPackFreq0 + 2] != 0;
-    const uint32_t freq0 = hf[kHistPackFreq0];
+    // a symbol outside the window: the full arrays instead of the window
+    const bool outside = pk[kHistPackFreq0 + 2] != 0;
+    const uint32_t freq0 = pk[kHistPackFreq0];
     // the code table now, the tree bytes once the encode is enqueued
     if (!outside) {
-        brc = build_huffman_window_codes(hf, hf + kHistWin, hist_window_lo(p->radius), kHistWin,
-                                         hf[kHistPackFreq0], hf[kHistPackFreq0 + 1], pr.tab);
+        brc = build_huffman_window_codes(pk, pk + kHistWin, hist_window_lo(p->radius), kHistWin,
+                                         pk[kHistPackFreq0], pk[kHistPackFreq0 + 1], pr.tab);
     } else {  // a symbol outside the window: the whole arrays
         HIPCHECK(hipMemcpyAsync(hf, ctx->freq.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
         HIPCHECK(hipMemcpyAsync(hf + 65536, ctx->first.p, 65536 * 4, hipMemcpyDeviceToHost, ctx->ustream));
@@ -1147,7 +1156,7 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         };
         if (!outside) {
             const uint32_t lo = (uint32_t) hist_window_lo(p->radius);
-            for (uint32_t i = 0; i < (uint32_t) kHistWin; i++) add(lo + i, hf[i]);
+            for (uint32_t i = 0; i < (uint32_t) kHistWin; i++) add(lo + i, pk[i]);
             add(0, freq0);
         } else {
             for (uint32_t i = 0; i < 65536; i++) add(i, hf[i]);
@@ -1162,14 +1171,19 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
     ctx->timings[8] = std::chrono::duration<double, std::milli>(th1 - th0).count();
     ctx->timings[9] = std::chrono::duration<double, std::milli>(th2 - th1).count();
     finish_huffman_tree(pr.tab);  // while the encode runs
-    HIPCHECK(hipMemcpyAsync(hs + 4, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(hs + 6, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    uint32_t *rep = mail ? ctx->mailbox.data() + faz_gpu_ctx::kMailReport : hs + 4;
+    if (mail) {
+        launch_report(ctx->scalars.p + 1, ctx->err_flag.p, rep, st);
+    } else {
+        HIPCHECK(hipMemcpyAsync(rep, ctx->scalars.p + 1, 8, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(rep + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    }
     if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[5], st));
     HIPCHECK(spin_wait(st));
     flush_profile(ctx);
     uint64_t bits;
-    memcpy(&bits, hs + 4, 8);
-    if (hs[6] & 4u) CTX_FAIL(FAZ_ERR_HIP, "internal: an encode look-back wait ran out");
+    memcpy(&bits, rep, 8);
+    if (rep[2] & 4u) CTX_FAIL(FAZ_ERR_HIP, "internal: an encode look-back wait ran out");
     if (bits != want_bits)
         CTX_FAIL(FAZ_ERR_HIP, "internal: encoded %llu bits, the histogram gives %llu", (unsigned long long) bits,
                  (unsigned long long) want_bits);

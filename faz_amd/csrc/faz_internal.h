@@ -190,7 +190,9 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
 constexpr int kHistPackFreq0 = 2 * kHistWin;
 constexpr int kHistPackWords = kHistPackFreq0 + 6;
 void launch_hist_pack(const uint32_t *d_freq, const uint32_t *d_first, int radius, const uint64_t *d_scalar0,
-                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st);
+                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st, const double *d_eb = nullptr);
+// words 0-1: *d_bits, word 2: *d_err (out may be coherent pinned host memory)
+void launch_report(const uint64_t *d_bits, const uint32_t *d_err, uint32_t *out, hipStream_t st);
 void launch_hist_clear(uint32_t *d_freq, uint32_t *d_first, hipStream_t st);
 void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
                       uint64_t *d_total, hipStream_t st);

@@ -739,11 +739,18 @@ int hist_window_lo(int radius) {
     return lo;
 }
 
+// pack may be host memory (coherent pinned): the block goes straight to the
+// host, and with eb set the base bound follows it at word kHistPackWords.
 __global__ void __launch_bounds__(256) hist_pack_kernel(const uint32_t *__restrict__ freq,
                                                         const uint32_t *__restrict__ first, int lo,
                                                         const uint32_t *__restrict__ scalar0,
-                                                        const uint32_t *__restrict__ err, uint32_t *__restrict__ pack) {
+                                                        const uint32_t *__restrict__ err, uint32_t *__restrict__ pack,
+                                                        const uint32_t *__restrict__ eb) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (eb && i - (uint32_t) kHistPackWords < 2u) {
+        pack[i] = eb[i - kHistPackWords];
+        return;
+    }
     uint32_t v;
     if (i < (uint32_t) kWin) v = freq[lo + i];
     else if (i < 2u * kWin) v = first[lo + i - kWin];
@@ -764,9 +771,25 @@ void launch_zero_count_u16(const uint16_t *d_codes, size_t n, uint32_t *d_chunk_
 }
 
 void launch_hist_pack(const uint32_t *d_freq, const uint32_t *d_first, int radius, const uint64_t *d_scalar0,
-                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st) {
-    hipLaunchKernelGGL(hist_pack_kernel, dim3((kHistPackWords + 255) / 256), dim3(256), 0, st, d_freq, d_first,
-                       hist_window_lo(radius), reinterpret_cast<const uint32_t *>(d_scalar0), d_err, d_pack);
+                      const uint32_t *d_err, uint32_t *d_pack, hipStream_t st, const double *d_eb) {
+    hipLaunchKernelGGL(hist_pack_kernel, dim3((kHistPackWords + 2 + 255) / 256), dim3(256), 0, st, d_freq, d_first,
+                       hist_window_lo(radius), reinterpret_cast<const uint32_t *>(d_scalar0), d_err, d_pack,
+                       reinterpret_cast<const uint32_t *>(d_eb));
+}
+
+// The encode's bit total and the error flag into (host) memory.
+__global__ void report_kernel(const uint64_t *__restrict__ bits, const uint32_t *__restrict__ err,
+                              uint32_t *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        const uint64_t b = *bits;
+        out[0] = (uint32_t) b;
+        out[1] = (uint32_t) (b >> 32);
+        out[2] = *err;
+    }
+}
+
+void launch_report(const uint64_t *d_bits, const uint32_t *d_err, uint32_t *out, hipStream_t st) {
+    hipLaunchKernelGGL(report_kernel, dim3(1), dim3(64), 0, st, d_bits, d_err, out);
 }
 
 // ---------------------------------------------------------------------------
