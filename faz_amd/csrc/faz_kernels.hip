@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) range_kernel(const float *__restrict__ in
     const size_t n4 = ((uintptr_t) in & 15) ? 0 : n / 4;
     const float4 *in4 = reinterpret_cast<const float4 *>(in);
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    constexpr int U = 4;
+    constexpr int U = 4;  // 8 loads per lane, or 1024-2048 workgroups: slower on 512 MiB
     for (; i + (U - 1) * stride < n4; i += U * stride) {
         float4 v[U];
 #pragma unroll
