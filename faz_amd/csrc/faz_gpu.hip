@@ -1002,9 +1002,16 @@ static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, c
         return FAZ_OK;
     }
     launch_encode_sizes(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_bits.p, st);
-    launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
-    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned) std::min<size_t>((cap + 255) / 256, 8192)), dim3(256), 0, st,
-                       ctx->words.p, ctx->scalars.p + 1, ctx->words.cap);
+    // codewords of at most 32 bits: the packer ORs only into the words the
+    // offsets fall in, which the scan zeroes; longer ones OR every word
+    if (maxbits <= 32) {
+        launch_encode_offsets(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1,
+                              ctx->words.p, st);
+    } else {
+        launch_scan_u64(ctx->chunk_bits.p, ctx->chunk_off.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
+        hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned) std::min<size_t>((cap + 255) / 256, 8192)), dim3(256), 0,
+                           st, ctx->words.p, ctx->scalars.p + 1, ctx->words.cap);
+    }
     launch_encode_pack(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->chunk_off.p, ctx->words.p, st);
     HIPCHECK(hipGetLastError());
     return FAZ_OK;

@@ -220,6 +220,10 @@ struct EncodeWindow {
 uint32_t encode_lds_entries();
 void launch_encode_sizes(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
                          int32_t offset, uint64_t *d_chunk_bits, hipStream_t st);
+// The encoder's chunk bit offsets and total (launch_scan_u64), zeroing the
+// output words its packer ORs into when every codeword has at most 32 bits.
+void launch_encode_offsets(const uint64_t *d_bits, uint64_t *d_off, uint32_t nchunks, uint64_t *d_tiles,
+                           uint64_t *d_total, uint32_t *d_words, hipStream_t st);
 void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
                      hipStream_t st);
 // tiles of the two-level scans (scratch of this many uint64 per scan)

@@ -795,6 +795,9 @@ void launch_report(const uint64_t *d_bits, const uint32_t *d_err, uint32_t *out,
 // ---------------------------------------------------------------------------
 // Exclusive scans over chunk totals: tiles of kScanTile entries are reduced,
 // then every tile scans itself and adds the sum of the tiles before it.
+// With words set (the encoder's bit offsets) the apply pass also zeroes the
+// output word each offset falls in and the word of the last bit: the only
+// words the packer ORs into (every other word is stored whole by one chunk).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kScanThreads = 256;
 constexpr uint32_t kScanPer = 16;
@@ -823,7 +826,7 @@ template<typename TIn>
 __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const TIn *__restrict__ in, uint32_t count,
                                                                  const uint64_t *__restrict__ tile_sum,
                                                                  uint32_t ntiles, uint64_t *__restrict__ out,
-                                                                 uint64_t *total) {
+                                                                 uint64_t *total, uint32_t *__restrict__ words) {
     __shared__ uint64_t s_wave[kScanThreads / 64];
     __shared__ uint64_t s_tile_base;
     // sum of the tiles before this one
@@ -858,25 +861,36 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const TIn *__r
     for (int w = 0; w < wave; w++) run += s_wave[w];
 #pragma unroll
     for (uint32_t k = 0; k < kScanPer; k++) {
-        if (i0 + k < count) out[i0 + k] = run;
+        if (i0 + k < count) {
+            out[i0 + k] = run;
+            if (words) words[run >> 5] = 0u;
+        }
         run += v[k];
     }
-    if (total && blockIdx.x == ntiles - 1 && threadIdx.x == kScanThreads - 1) *total = run;
+    if (blockIdx.x == ntiles - 1 && threadIdx.x == kScanThreads - 1) {
+        if (total) *total = run;
+        if (words && run) words[(run - 1) >> 5] = 0u;
+    }
 }
 
 template<typename TIn>
 static void scan_two_level(const TIn *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
-                           hipStream_t st) {
+                           hipStream_t st, uint32_t *d_words = nullptr) {
     uint32_t ntiles = (count + kScanTile - 1) / kScanTile;
     if (ntiles == 0) return;
     hipLaunchKernelGGL(tile_reduce_kernel<TIn>, dim3(ntiles), dim3(kScanThreads), 0, st, d_in, count, d_tiles);
     hipLaunchKernelGGL(scan_apply_kernel<TIn>, dim3(ntiles), dim3(kScanThreads), 0, st, d_in, count, d_tiles, ntiles,
-                       d_out, d_total);
+                       d_out, d_total, d_words);
 }
 
 void launch_zero_scan(const uint32_t *d_chunk_zeros, uint64_t *d_chunk_base, uint32_t nchunks, uint64_t *d_tiles,
                       uint64_t *d_total, hipStream_t st) {
     scan_two_level<uint32_t>(d_chunk_zeros, d_chunk_base, nchunks, d_tiles, d_total, st);
+}
+
+void launch_encode_offsets(const uint64_t *d_bits, uint64_t *d_off, uint32_t nchunks, uint64_t *d_tiles,
+                           uint64_t *d_total, uint32_t *d_words, hipStream_t st) {
+    scan_two_level<uint64_t>(d_bits, d_off, nchunks, d_tiles, d_total, st, d_words);
 }
 
 void launch_scan_u64(const uint64_t *d_in, uint64_t *d_out, uint32_t count, uint64_t *d_tiles, uint64_t *d_total,
