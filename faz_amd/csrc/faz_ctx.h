@@ -162,9 +162,11 @@ struct faz_gpu_ctx {
     // pinned: histogram and first positions, then 8 words of scalars
     faz::HostPinned<uint32_t> hp_hist{2 * 65536 + 8};
     // coherent pinned words that kernels write directly: the histogram block
-    // and base bound, then (at kMailReport) the encode's bit total and error flag
+    // and base bound, then (at kMailReport) the encode's bit total and error
+    // flag, then (at kMailDecode) the decompression's checks
     static constexpr size_t kMailReport = faz::kHistPackWords + 4;
-    faz::HostPinned<uint32_t> mailbox{kMailReport + 4, hipHostMallocCoherent};
+    static constexpr size_t kMailDecode = kMailReport + 4;
+    faz::HostPinned<uint32_t> mailbox{kMailDecode + 8, hipHostMallocCoherent};
     faz::PinnedVec<uint64_t> h_tab;  // packed code table of the last encode (pinned: asynchronous upload)
     faz::PinnedVec<uint32_t> h_dec;  // the decoder's lookup tables and tree, staged for upload
     faz::PendingRaw pending;

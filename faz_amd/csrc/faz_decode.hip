@@ -763,7 +763,9 @@ extern "C" faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, si
 // device; a stream without the run decodes the general way).
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
-                       size_t *used, const uint8_t *d_buf, uint32_t run_value, uint64_t run_count) {
+                       size_t *used, const uint8_t *d_buf, uint32_t run_value, uint64_t run_count,
+                       DecodeCheck *defer) {
+    if (defer) *defer = DecodeCheck{};
     int32_t offset;
     std::vector<int32_t> left, right, sym;
     std::vector<uint8_t> leaf;
@@ -1041,6 +1043,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
                        d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err,
                        run);
     HIPCHECK(hipGetLastError());
+    if (defer) {
+        defer->total = ctx->scalars.p;
+        defer->err = d_err;
+        return FAZ_OK;
+    }
     uint32_t *hf = ctx->hp_hist.data() + 2 * 65536;  // pinned scalars
     HIPCHECK(hipMemcpyAsync(hf, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipMemcpyAsync(hf + 2, d_err, 4, hipMemcpyDeviceToHost, st));
@@ -1054,11 +1061,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
 }
 
 template int gpu_huffman_decode<uint16_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint16_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
 template int gpu_huffman_decode<int32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, int32_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
 template int gpu_huffman_decode<uint32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint32_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
 
 // ---------------------------------------------------------------------------
 // Unpredictable values of 32-bit code streams (the general frontend): the

@@ -1422,9 +1422,13 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     HIPCHECK(ctx->err_flag.ensure(1));
     // codes decoded on the device (HUF:236-268), then the outliers' ranks
     const auto td = std::chrono::steady_clock::now();
-    // with anchors the codes open with anchor_count zeros (SIC:854-900)
+    // with anchors the codes open with anchor_count zeros (SIC:854-900).
+    // Every check of the stream (symbol count, tree, zero count, segments)
+    // is read back once at the end: the device work is bounded whatever the
+    // stream holds, so the whole decompression is enqueued without a wait.
+    DecodeCheck chk;
     int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff, 0u,
-                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0);
+                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0, &chk);
     if (drc) return drc;
     ctx->timings[6] = ms_since(td);
     if (U) {
@@ -1437,15 +1441,10 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     ctx->prefix_key = 0;  // the compression path's table is gone
     launch_zero_count_u16(ctx->codes.p, num, ctx->chunk_zeros.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
-    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p, st);
-    uint32_t *hs = ctx->hp_hist.data() + 2 * 65536;  // pinned: zero count
-    HIPCHECK(hipMemcpyAsync(hs, ctx->scalars.p, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(spin_wait(st));
-    uint64_t zeros;
-    memcpy(&zeros, hs, 8);
-    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
-                             (unsigned long long) U, (unsigned long long) zeros);
-    launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st);
+    // the zero count in scalars[1] (scalars[0] holds the decoded symbol count)
+    launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);
+    launch_unpred_scatter(ctx->codes.p, num, ctx->unpred.p, ctx->chunk_zeros.p, ctx->chunk_base.p, ctx->outval.p, st,
+                          U);
     ctx->hist_codes = nullptr;
     // head: anchors / first point come straight from the unpredictable list
     if (pl.anchor) {
@@ -1472,9 +1471,27 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
         }
     }
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(hs + 2, ctx->err_flag.p, 4, hipMemcpyDeviceToHost, st));
+    uint32_t *rep;
+    if (ctx->mailbox.pinned) {
+        rep = ctx->mailbox.data() + faz_gpu_ctx::kMailDecode;
+        launch_decode_report(chk.total, chk.err, ctx->scalars.p + 1, ctx->err_flag.p, rep, st);
+    } else {
+        HIPCHECK(ctx->hist_pack.ensure(kHistPackWords));
+        rep = ctx->hp_hist.data() + 2 * 65536;
+        launch_decode_report(chk.total, chk.err, ctx->scalars.p + 1, ctx->err_flag.p, ctx->hist_pack.p, st);
+        HIPCHECK(hipMemcpyAsync(rep, ctx->hist_pack.p, 24, hipMemcpyDeviceToHost, st));
+    }
     HIPCHECK(spin_wait(st));
-    if (hs[2]) CTX_FAIL(FAZ_ERR_FORMAT, "a block's codes lie outside the stream");
+    uint64_t symbols, zeros;
+    memcpy(&symbols, rep, 8);
+    memcpy(&zeros, rep + 4, 8);
+    if (symbols < num)
+        CTX_FAIL(FAZ_ERR_FORMAT, "Huffman bitstream holds %llu symbols, %zu expected", (unsigned long long) symbols,
+                 num);
+    if (rep[2]) CTX_FAIL(FAZ_ERR_FORMAT, "Huffman bitstream leaves the tree or a symbol is out of range");
+    if (zeros != U) CTX_FAIL(FAZ_ERR_FORMAT, "unpredictable count %llu does not match %llu zero codes",
+                             (unsigned long long) U, (unsigned long long) zeros);
+    if (rep[3]) CTX_FAIL(FAZ_ERR_FORMAT, "a block's codes lie outside the stream");
     ctx->timings[7] = ms_since(t0);
     return FAZ_OK;
 }

@@ -240,7 +240,11 @@ void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_
                            uint32_t *d_head_bits, uint32_t *d_ticket, uint64_t *d_total_bits, uint32_t *d_err,
                            hipStream_t st);
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
-                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st);
+                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st,
+                           uint64_t count = ~0ull);
+// words 0-1: *d_symbols, 2: *d_dec_err, 3: *d_rec_err, 4-5: *d_zeros
+void launch_decode_report(const uint64_t *d_symbols, const uint32_t *d_dec_err, const uint64_t *d_zeros,
+                          const uint32_t *d_rec_err, uint32_t *out, hipStream_t st);
 
 // HuffmanEncoder::decode on the device (faz_decode.hip): num symbols into
 // d_out on ctx->stream, each symbol + the tree's offset checked to lie in
@@ -248,10 +252,18 @@ void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unp
 // section in device memory) the host buf needs only the tree and byte count.
 // run_count > 0: the symbols are expected to open with run_count copies of
 // run_value (checked on the device, used to skip their decoding).
+// With defer set the checks are left to the caller (no final wait): the
+// decoded symbol count (>= num expected) at defer->total and the error flag
+// (0 expected) at defer->err, both on the device, or null when already
+// checked.  The output is written within [0, num) whatever the stream holds.
+struct DecodeCheck {
+    const uint64_t *total = nullptr;
+    const uint32_t *err = nullptr;
+};
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
                        size_t *used, const uint8_t *d_buf = nullptr, uint32_t run_value = 0,
-                       uint64_t run_count = 0);
+                       uint64_t run_count = 0, DecodeCheck *defer = nullptr);
 
 // outval[i] = the unpredictable value of zero code i, for n 32-bit codes on
 // the device (faz_decode.hip); uploads the U host values first.

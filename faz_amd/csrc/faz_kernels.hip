@@ -958,7 +958,8 @@ __global__ void __launch_bounds__(kHistThreads) unpred_move_kernel(const uint16_
                                                                   const float *__restrict__ src,
                                                                   const uint32_t *__restrict__ chunk_zeros,
                                                                   const uint64_t *__restrict__ chunk_base,
-                                                                  float *__restrict__ dst, int aligned) {
+                                                                  float *__restrict__ dst, int aligned,
+                                                                  uint64_t limit) {
     static_assert(kHistPerThread == 64, "one load64 per thread and chunk");
     __shared__ uint32_t s_wave[kHistThreads / 64];
     const uint32_t nchunks = (n + kHistChunk - 1) / kHistChunk;
@@ -980,7 +981,7 @@ __global__ void __launch_bounds__(kHistThreads) unpred_move_kernel(const uint16_
                     // gather: the list in code order; scatter: back to code positions
                     if (GATHER)
                         dst[run] = src[my0 + k];
-                    else
+                    else if (run < limit)  // a count mismatch is reported by the caller
                         dst[my0 + k] = src[run];
                     run++;
                 }
@@ -996,19 +997,43 @@ void launch_unpred_gather(const uint16_t *d_codes, size_t n, const float *d_outv
     if (nchunks > 2048) nchunks = 2048;
     if (nchunks)
         hipLaunchKernelGGL(unpred_move_kernel<true>, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n,
-                           d_outval, d_chunk_zeros, d_chunk_base, d_unpred, (int) (((uintptr_t) d_codes & 15) == 0));
+                           d_outval, d_chunk_zeros, d_chunk_base, d_unpred, (int) (((uintptr_t) d_codes & 15) == 0),
+                           ~0ull);
 }
 
 // The inverse for decompression: unpredictable value of every code-0
-// position, indexed by code position (unpred_move_kernel<false>).
+// position, indexed by code position (unpred_move_kernel<false>); zeros
+// ranked at or beyond count (a stream whose list is too short) take nothing.
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
-                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st) {
+                           const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st, uint64_t count) {
     uint32_t nchunks = (uint32_t) ((n + kHistChunk - 1) / kHistChunk);
     if (nchunks > 2048) nchunks = 2048;
     if (nchunks)
         hipLaunchKernelGGL(unpred_move_kernel<false>, dim3(nchunks), dim3(kHistThreads), 0, st, d_codes,
                            (uint32_t) n, d_unpred, d_chunk_zeros, d_chunk_base, d_unpred_at,
-                           (int) (((uintptr_t) d_codes & 15) == 0));
+                           (int) (((uintptr_t) d_codes & 15) == 0), (uint64_t) count);
+}
+
+// The decompression's checks in one block of (host) memory: the decoded
+// symbol count, the decoder's error flag, the zero-code count and the
+// recovery's error flag.
+__global__ void decode_report_kernel(const uint64_t *__restrict__ symbols, const uint32_t *__restrict__ dec_err,
+                                     const uint64_t *__restrict__ zeros, const uint32_t *__restrict__ rec_err,
+                                     uint32_t *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        const uint64_t a = symbols ? *symbols : ~0ull, b = *zeros;  // null: checked by the decoder
+        out[0] = (uint32_t) a;
+        out[1] = (uint32_t) (a >> 32);
+        out[2] = dec_err ? *dec_err : 0u;
+        out[3] = *rec_err;
+        out[4] = (uint32_t) b;
+        out[5] = (uint32_t) (b >> 32);
+    }
+}
+
+void launch_decode_report(const uint64_t *d_symbols, const uint32_t *d_dec_err, const uint64_t *d_zeros,
+                          const uint32_t *d_rec_err, uint32_t *out, hipStream_t st) {
+    hipLaunchKernelGGL(decode_report_kernel, dim3(1), dim3(64), 0, st, d_symbols, d_dec_err, d_zeros, d_rec_err, out);
 }
 
 

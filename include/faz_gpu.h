@@ -552,8 +552,9 @@ faz_status faz_gpu_tune_learned_f32(faz_gpu_ctx *ctx, const float *d_in, const f
  * (interpolation pass launches) summed, [8] host time from the histogram's
  * arrival to the Huffman tree, [9] host time enqueueing the encode.
  * faz_gpu_interp_decompress_raw and faz_gpu_interp_decompress_dev set [6] to
- * their Huffman decode (tree parse, upload or device copy, device decode) and
- * [7] to the whole call, host clock. */
+ * the host time of their Huffman stage (tree parse, tables, upload or device
+ * copy enqueued; the device decode runs on behind it) and [7] to the whole
+ * call, host clock. */
 faz_status faz_gpu_last_timings(const faz_gpu_ctx *ctx, double *ms, int n);
 
 /* on: bit 0 enables HIP-event timing of the interpolation kernels,
