@@ -1426,11 +1426,8 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     // Every check of the stream (symbol count, tree, zero count, segments)
     // is read back once at the end: the device work is bounded whatever the
     // stream holds, so the whole decompression is enqueued without a wait.
-    DecodeCheck chk;
-    int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff, 0u,
-                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0, &chk);
-    if (drc) return drc;
-    ctx->timings[6] = ms_since(td);
+    // the unpredictable list and the block prefixes go first: the device is
+    // idle while the host builds the decoder's tables
     if (U) {
         if (d_unpred)
             HIPCHECK(hipMemcpyAsync(ctx->unpred.p, d_unpred, U * 4, hipMemcpyDeviceToDevice, st));
@@ -1439,6 +1436,11 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     }
     HIPCHECK(hipMemcpyAsync(ctx->prefix.p, pl.prefix.data(), pl.prefix.size() * 8, hipMemcpyHostToDevice, st));
     ctx->prefix_key = 0;  // the compression path's table is gone
+    DecodeCheck chk;
+    int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff, 0u,
+                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0, &chk);
+    if (drc) return drc;
+    ctx->timings[6] = ms_since(td);
     launch_zero_count_u16(ctx->codes.p, num, ctx->chunk_zeros.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     // the zero count in scalars[1] (scalars[0] holds the decoded symbol count)
