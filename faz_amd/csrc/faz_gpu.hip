@@ -1085,6 +1085,10 @@ void faz_ctx_set_error(faz_gpu_ctx *ctx, const char *msg) { ctx->err = msg; }
 
 // Device part of the engine buffer: interpolation + quantization, histogram
 // and outliers, host tree, encode.  Results stay in the context until fetched.
+// compress_device's answer when a whole-tile level flagged that its
+// device-computed bound needs the exact quantizer (not a faz_status).
+constexpr int kRetryHostBounds = 1000;
+
 static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, const faz_interp_params *p,
                            double absmax = -1.0, const LevelEB *d_lvl = nullptr) {
     if (!ctx || !d_in || !p) return FAZ_ERR_INVALID;
@@ -1133,6 +1137,9 @@ static int compress_device(faz_gpu_ctx *ctx, const float *d_in, float *d_recon, 
         if (!(eb > 0)) CTX_FAIL(FAZ_ERR_INVALID, "error bound must be positive");
     }
     const uint32_t errflag = pk[kHistPackFreq0 + 3];
+    // a whole-tile level whose device bound needs the exact quantizer: the
+    // caller compresses again with host bounds
+    if (errflag & 8u) return kRetryHostBounds;
     if (errflag & 2u) CTX_FAIL(FAZ_ERR_INVALID, "internal: a block segment fell outside the code stream");
     if (errflag) CTX_FAIL(FAZ_ERR_UNSUPPORTED, "a quantization code exceeded 16 bits");
     uint64_t n_unpred;
@@ -1308,10 +1315,14 @@ faz_status faz_gpu_compress_f32(faz_gpu_ctx *ctx, const float *d_in, float *d_re
             rc = compress_device(ctx, d_in, d_recon, &q, -1.0, ctx->lvl.p);
             ctx->hist_cleared = false;
             ctx->err_cleared = false;
-            if (rc) return rc;
-            if (ctx->phase_events) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
-            *raw_size = pending_size(ctx);
-            return FAZ_OK;
+            if (rc != kRetryHostBounds) {
+                if (rc) return rc;
+                if (ctx->phase_events) ctx->timings[0] = elapsed(ctx->ev[6], ctx->ev[7]);  // drained by compress_device
+                *raw_size = pending_size(ctx);
+                return FAZ_OK;
+            }
+            // rare: bounds too fine for the bounded quantizer on a tile level
+            if (ctx->phase_events) HIPCHECK(hipEventRecord(ctx->ev[6], ctx->stream));
         }
         // the histogram's arrays are cleared while the host waits for the range
         launch_range(d_in, num, ctx->minmax.p, ctx->stream);

@@ -631,8 +631,12 @@ __device__ __forceinline__ void tile3d_compress(float *T, const Level3D &L, cons
     if (!BOUNDED && wide) atomicOr(err, 1u);
 }
 
-// MODE 0: the exact quantizer, 1: the bounded one, 2: the level's bounds and
-// the choice between the two read from L.ebp (computed on the device).
+// MODE 0: the exact quantizer, 1: the bounded one, 2: the bounded one with
+// the level's bounds read from L.ebp (computed on the device); a level whose
+// bound does not admit it flags bit 3 of err and writes nothing, and the
+// caller compresses again with bounds from the host (which then picks the
+// exact quantizer): a kernel that holds both forms keeps more uniform values
+// spilled than either.
 template<int DIR, bool LINEAR, int MODE>
 __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, const float *__restrict__ in,
                                                                  uint16_t *__restrict__ codes,
@@ -641,13 +645,14 @@ __global__ void __launch_bounds__(kThreads) interp_tile3d_kernel(Level3D L, cons
     __shared__ float T[kTileFloats];
     if (MODE == 2) {
         const LevelEB e = *L.ebp;
+        if (!e.bounded) {
+            if (threadIdx.x == 0) atomicOr(err, 8u);
+            return;
+        }
         L.eb = e.eb;
         L.recip = e.recip;
         L.ebf = e.ebf;
-        if (e.bounded)
-            tile3d_compress<DIR, LINEAR, true>(T, L, in, codes, outval, err);
-        else
-            tile3d_compress<DIR, LINEAR, false>(T, L, in, codes, outval, err);
+        tile3d_compress<DIR, LINEAR, true>(T, L, in, codes, outval, err);
     } else {
         tile3d_compress<DIR, LINEAR, MODE == 1>(T, L, in, codes, outval, err);
     }
