@@ -575,7 +575,7 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
                                                                  const uint64_t *__restrict__ offset,
                                                                  int32_t sym_offset, uint32_t limit, uint64_t num,
                                                                  OUT *__restrict__ out, uint32_t *__restrict__ err,
-                                                                 RunPrefix run) {
+                                                                 RunPrefix run, uint32_t *__restrict__ zc) {
     __shared__ uint32_t tab[MULTI ? 1 : (1 << kTabBits)];
     __shared__ uint64_t wtab[MULTI ? (1 << kTabBits) : 1];
     __shared__ uint32_t lw[kWStageSlots];
@@ -608,7 +608,19 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
     constexpr int W = 8 * sizeof(OUT);
     uint64_t acc0 = 0, acc1 = 0;
     int na = 0;
+    // zc: zero symbols per kHistChunk outputs (the outlier ranks), counted in
+    // a register and added once per chunk the thread's range touches
+    uint32_t zn = 0, zh = 0;
     auto push = [&](uint32_t v) {
+        if (zc && v == 0) {
+            const uint32_t h = (uint32_t) (o / kHistChunk);
+            if (h != zh && zn) {
+                atomicAdd(&zc[zh], zn);
+                zn = 0;
+            }
+            zh = h;
+            zn++;
+        }
         if (na == 0 && (reinterpret_cast<uintptr_t>(out + o) & 15u) != 0) {
             out[o++] = (OUT) v;
             return;
@@ -662,6 +674,7 @@ __global__ void __launch_bounds__(kDecThreads) huff_write_kernel(const uint32_t 
         push(v);
     }
     for (int j = 0; j < na; j++) out[o - na + j] = (OUT) (j < H ? acc0 >> (W * j) : acc1 >> (W * (j - H)));
+    if (zn) atomicAdd(&zc[zh], zn);
     if (bad) atomicOr(err, 1u);
 }
 
@@ -764,7 +777,7 @@ extern "C" faz_status faz_huffman_check_section(const uint8_t *buf, size_t n, si
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
                        size_t *used, const uint8_t *d_buf, uint32_t run_value, uint64_t run_count,
-                       DecodeCheck *defer) {
+                       DecodeCheck *defer, uint32_t *zero_counts) {
     if (defer) *defer = DecodeCheck{};
     int32_t offset;
     std::vector<int32_t> left, right, sym;
@@ -1038,10 +1051,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
     // then pass 3
     launch_zero_scan(d_cnt, d_off, nquart, ctx->tiles.p, ctx->scalars.p, st);
     launch_fill_u32(d_err, 1, 0u, st);
+    if (zero_counts) launch_fill_u32(zero_counts, (num + kHistChunk - 1) / kHistChunk, 0u, st);
     hipLaunchKernelGGL((huff_write_kernel<OUT, kMulti>), dim3((nquart + kDecThreads - 1) / kDecThreads), block, 0, st,
                        ctx->dec_words.p, nwords, nbits, nchunks,
                        d_tab, ctx->dec_wtab.p, T, d_start, d_off, offset, limit, (uint64_t) num, d_out, d_err,
-                       run);
+                       run, zero_counts);
     HIPCHECK(hipGetLastError());
     if (defer) {
         defer->total = ctx->scalars.p;
@@ -1061,11 +1075,11 @@ int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t nu
 }
 
 template int gpu_huffman_decode<uint16_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint16_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *, uint32_t *);
 template int gpu_huffman_decode<int32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, int32_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *, uint32_t *);
 template int gpu_huffman_decode<uint32_t>(faz_gpu_ctx *, const uint8_t *, size_t, size_t, uint32_t, uint32_t *,
-                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *);
+                                          size_t *, const uint8_t *, uint32_t, uint64_t, DecodeCheck *, uint32_t *);
 
 // ---------------------------------------------------------------------------
 // Unpredictable values of 32-bit code streams (the general frontend): the

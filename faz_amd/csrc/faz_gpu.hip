@@ -1449,10 +1449,13 @@ static int interp_decompress(faz_gpu_ctx *ctx, const EngineHeader &h, double eb,
     ctx->prefix_key = 0;  // the compression path's table is gone
     DecodeCheck chk;
     int drc = gpu_huffman_decode<uint16_t>(ctx, huff, huff_n, num, 65536u, ctx->codes.p, nullptr, d_huff, 0u,
-                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0, &chk);
+                                           pl.anchor && ctx->dec_run ? pl.anchor_count : 0, &chk,
+                                           ctx->chunk_zeros.p);
     if (drc) return drc;
     ctx->timings[6] = ms_since(td);
-    launch_zero_count_u16(ctx->codes.p, num, ctx->chunk_zeros.p, st);
+    // the write pass counted the zero codes per chunk; the single-symbol
+    // stream (no write pass) is counted here
+    if (chk.total == nullptr) launch_zero_count_u16(ctx->codes.p, num, ctx->chunk_zeros.p, st);
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     // the zero count in scalars[1] (scalars[0] holds the decoded symbol count)
     launch_zero_scan(ctx->chunk_zeros.p, ctx->chunk_base.p, nchunks, ctx->tiles.p, ctx->scalars.p + 1, st);

@@ -260,10 +260,12 @@ struct DecodeCheck {
     const uint64_t *total = nullptr;
     const uint32_t *err = nullptr;
 };
+// zero_counts (ceil(num / kHistChunk) words): the zero symbols of every
+// kHistChunk outputs, counted by the write pass (the outlier ranks).
 template<class OUT>
 int gpu_huffman_decode(faz_gpu_ctx *ctx, const uint8_t *buf, size_t n, size_t num, uint32_t limit, OUT *d_out,
                        size_t *used, const uint8_t *d_buf = nullptr, uint32_t run_value = 0,
-                       uint64_t run_count = 0, DecodeCheck *defer = nullptr);
+                       uint64_t run_count = 0, DecodeCheck *defer = nullptr, uint32_t *zero_counts = nullptr);
 
 // outval[i] = the unpredictable value of zero code i, for n 32-bit codes on
 // the device (faz_decode.hip); uploads the U host values first.
