@@ -140,7 +140,9 @@ struct faz_gpu_ctx {
     faz::DevBuf<uint32_t> hist_pack;  // one readback block after the histogram
     faz::DevBuf<uint32_t> enc_head;   // one-pass encode: bits of each chunk's first, shared word
     // FAZ_ENCODE=onepass selects the decoupled look-back encoder over the
-    // sizes + scan + pack one (slower at 512^3: 0.41 ms against 0.20 ms)
+    // sizes + scan + pack one (slower at 512^3: the 512^3 step takes about
+    // 40 us longer, 43 rounds of look-back latency against one more read of
+    // the codes)
     bool enc_onepass = false;
     bool dec_run = true;  // the decoder skips the search inside the anchors' run of zero codes
     bool dev_bounds = true;  // REL compressions compute the level bounds on the device

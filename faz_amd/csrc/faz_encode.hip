@@ -286,16 +286,20 @@ __global__ void __launch_bounds__(kEncThreads) encode_pack_kernel(const uint16_t
                                    chunk_off[c], s_tab, tab, tw, offset, s_words, s_wave, words);
 }
 
-// One-pass encode: chunks taken in ticket order; each publishes its bit
-// total at once and finds its bit offset by a decoupled look-back over the
-// published totals and inclusive prefixes (Merrill and Garland, "Single-pass
-// Parallel Prefix Scan with Decoupled Look-back", 2016), 64 predecessors per
-// step across wave 0.  Every output word is stored by exactly one chunk: the
-// one whose range holds the word's first bit.  A chunk starting inside a
-// word hands its bits of that word to head_bits[c] instead, and a fix-up
-// kernel ORs them in after the pack, so the output needs no zero fill and no
-// chunk waits for another's stores.  Needs codewords of at most 32 bits (the
-// LDS bit buffer holds 32 bits per code).
+// One-pass encode: workgroups all resident at once (the grid is one round of
+// the device's occupancy), workgroup b taking chunks b, b + G, b + 2G, ... with
+// the next chunk's codes in flight.  Each chunk publishes its bit total as
+// soon as it is known and finds its bit offset by a decoupled look-back over
+// the published totals and inclusive prefixes (Merrill and Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016), 64
+// predecessors per step across wave 0; every predecessor of a chunk belongs
+// to an earlier round or to a resident workgroup of the same round, so every
+// wait ends.  The chunk's bits are assembled in LDS from bit 0 while the
+// look-back runs and shifted into place as they are stored.  Every output
+// word is stored by exactly one chunk: the one whose range holds the word's
+// first bit.  A chunk starting inside a word hands its bits of that word to
+// head_bits[c] instead, and a fix-up kernel ORs them in after the pack, so
+// the output needs no zero fill.  Needs codewords of at most 32 bits.
 constexpr uint64_t kFlagAgg = 1, kFlagPrefix = 2;
 constexpr uint32_t kLookbackSpins = 1u << 24;
 
@@ -312,28 +316,60 @@ __device__ __forceinline__ void status_store(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Exclusive bit offset of chunk c > 0 (wave 0; every lane returns it).
+__device__ __forceinline__ uint64_t look_back(const uint64_t *status, uint32_t c, uint32_t lane, uint32_t *err) {
+    uint64_t prefix = 0;
+    int64_t j = (int64_t) c - 1;
+    uint32_t spins = 0;
+    for (;;) {
+        const int64_t idx = j - (int64_t) lane;
+        // before chunk 0: an inclusive prefix of 0
+        const uint64_t sv = idx >= 0 ? status_load(&status[idx]) : kFlagPrefix;
+        const uint32_t flag = (uint32_t) (sv & 3);
+        const uint64_t pmask = __ballot(flag == kFlagPrefix);
+        const uint64_t zmask = __ballot(flag == 0);
+        const uint32_t firstp = pmask ? (uint32_t) __ffsll((unsigned long long) pmask) - 1 : 64u;
+        // every predecessor nearer than the first prefix must have its total out
+        const uint64_t nearer = firstp < 64 ? ((1ull << firstp) - 1) : ~0ull;
+        if (zmask & nearer) {
+            if (++spins > kLookbackSpins) {
+                if (lane == 0) atomicOr(err, 4u);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t add = lane <= firstp ? (sv >> 2) : 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) add += (uint64_t) __shfl_xor((long long) add, o);
+        prefix += add;
+        if (firstp < 64) return prefix;
+        j -= 64;
+    }
+}
+
 template<bool LDS_TAB>
 __global__ void __launch_bounds__(kEncThreads) encode_onepass_kernel(
         const uint16_t *__restrict__ codes, uint32_t n, const uint64_t *__restrict__ tab, TabWindow tw, int32_t offset,
         uint32_t *__restrict__ words, uint64_t *__restrict__ status, uint64_t *__restrict__ chunk_off,
-        uint32_t *__restrict__ head_bits, uint32_t *__restrict__ ticket, uint64_t *__restrict__ total_bits,
-        uint32_t *__restrict__ err, int aligned) {
+        uint32_t *__restrict__ head_bits, uint64_t *__restrict__ total_bits, uint32_t *__restrict__ err,
+        int aligned) {
     extern __shared__ uint64_t s_tab[];
-    __shared__ uint32_t s_words[kPackWords];
+    __shared__ uint32_t s_words[kPackWords + 1];
     __shared__ uint32_t s_wave[kEncThreads / 64];
-    __shared__ uint32_t s_c;
     __shared__ uint64_t s_prefix;
     stage_table<LDS_TAB>(s_tab, tab, tw);
     const uint32_t nchunks = (n + kEncChunk - 1) / kEncChunk;
+    const uint32_t nfull = aligned ? n / kEncChunk : 0;
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    for (;;) {
-        if (threadIdx.x == 0) s_c = atomicAdd(ticket, 1u);
-        __syncthreads();
-        const uint32_t c = s_c;
-        if (c >= nchunks) break;
+    uint32_t c = blockIdx.x;
+    if (c >= nchunks) return;
+    Codes16 v = c < nfull ? fetch_full(codes, c) : fetch_chunk(codes, c, n, aligned != 0);
+    for (; c < nchunks; c += gridDim.x) {
+        const uint32_t cn = c + gridDim.x;
+        Codes16 next{};
+        if (cn < nchunks) next = cn < nfull ? fetch_full(codes, cn) : fetch_chunk(codes, cn, n, aligned != 0);
         const uint32_t end = min(c * kEncChunk + kEncChunk, n);
-        const bool full = aligned && end - c * kEncChunk == kEncChunk;
-        const Codes16 v = full ? fetch_full(codes, c) : fetch_chunk(codes, c, n, aligned != 0);
         const uint32_t my0 = c * kEncChunk + threadIdx.x * kEncPerThread;
         uint64_t e[16];
         uint32_t mybits = 0;
@@ -344,65 +380,38 @@ __global__ void __launch_bounds__(kEncThreads) encode_onepass_kernel(
         }
         uint32_t total;
         const uint32_t before = wg_exclusive(mybits, s_wave, &total);
-        // the chunk's offset: its total goes out at once, then the look-back
+        // the total goes out at once; the chunk's bits are assembled from bit 0
+        if (threadIdx.x == 0)
+            status_store(&status[c], ((uint64_t) total << 2) | (c == 0 ? kFlagPrefix : kFlagAgg));
+        const uint32_t nwz = (total + 31) / 32 + 1;
+        for (uint32_t i = threadIdx.x; i < nwz; i += kEncThreads) s_words[i] = 0;
+        __syncthreads();
+        put_run(e, before, s_words);
         if (wave == 0) {
-            uint64_t prefix = 0;
-            if (c == 0) {
-                if (lane == 0) status_store(&status[0], ((uint64_t) total << 2) | kFlagPrefix);
-            } else {
-                if (lane == 0) status_store(&status[c], ((uint64_t) total << 2) | kFlagAgg);
-                int64_t j = (int64_t) c - 1;
-                uint32_t spins = 0;
-                for (;;) {
-                    const int64_t idx = j - (int64_t) lane;
-                    // before chunk 0: an inclusive prefix of 0
-                    const uint64_t sv = idx >= 0 ? status_load(&status[idx]) : kFlagPrefix;
-                    const uint32_t flag = (uint32_t) (sv & 3);
-                    const uint64_t pmask = __ballot(flag == kFlagPrefix);
-                    const uint64_t zmask = __ballot(flag == 0);
-                    const uint32_t firstp = pmask ? (uint32_t) __ffsll((unsigned long long) pmask) - 1 : 64u;
-                    // every predecessor nearer than the first prefix must have its total out
-                    const uint64_t nearer = firstp < 64 ? ((1ull << firstp) - 1) : ~0ull;
-                    if (zmask & nearer) {
-                        // a predecessor has not published its total yet
-                        if (++spins > kLookbackSpins) {
-                            if (lane == 0) atomicOr(err, 4u);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    uint64_t add = lane <= firstp ? (sv >> 2) : 0;
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) add += (uint64_t) __shfl_xor((long long) add, o);
-                    prefix += add;
-                    if (firstp < 64) break;
-                    j -= 64;
-                }
-                if (lane == 0) status_store(&status[c], ((prefix + total) << 2) | kFlagPrefix);
-            }
+            const uint64_t prefix = c == 0 ? 0 : look_back(status, c, lane, err);
             if (lane == 0) {
+                if (c) status_store(&status[c], ((prefix + total) << 2) | kFlagPrefix);
                 s_prefix = prefix;
                 chunk_off[c] = prefix;
                 if (c == nchunks - 1) *total_bits = prefix + total;
             }
         }
         __syncthreads();
+        // out: word i of the output range takes the chunk's bits shifted right
+        // by head (the bits of the word before come from s_words[i - 1])
         const uint64_t g0 = s_prefix;
         const uint32_t head = (uint32_t) (g0 & 31);
         const uint32_t nw = (head + total + 31) / 32;
         const uint64_t gw = g0 >> 5;
-        for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) s_words[i] = 0;
-        __syncthreads();
-        put_run(e, head + before, s_words);
-        __syncthreads();
         for (uint32_t i = threadIdx.x; i < nw; i += kEncThreads) {
-            const uint32_t val = bswap32(s_words[i]);
+            const uint32_t hi = i ? s_words[i - 1] : 0u, lo = s_words[i];
+            const uint32_t val = bswap32(head ? (hi << (32 - head)) | (lo >> head) : lo);
             if (i == 0 && head) head_bits[c] = val;  // the word's first bit belongs to an earlier chunk
             else words[gw + i] = val;
         }
         if (threadIdx.x == 0 && (nw == 0 || !head)) head_bits[c] = 0;
         __syncthreads();
+        v = next;
     }
 }
 
@@ -462,21 +471,32 @@ uint32_t encode_lds_entries() { return kTabLds; }
 
 void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
                            int32_t offset, uint32_t *d_words, uint64_t *d_status, uint64_t *d_chunk_off,
-                           uint32_t *d_head_bits, uint32_t *d_ticket, uint64_t *d_total_bits, uint32_t *d_err,
-                           hipStream_t st) {
+                           uint32_t *d_head_bits, uint64_t *d_total_bits, uint32_t *d_err, hipStream_t st) {
     const uint32_t nchunks = (uint32_t) ((n + kEncChunk - 1) / kEncChunk);
     if (!nchunks) return;
     const int aligned = ((uintptr_t) d_codes & 15) == 0;
     const TabWindow tw{w.win, w.nwin, w.e0, w.max_len};
-    const dim3 grid(encode_grid(nchunks));
-    if (w.nwin && w.nwin <= kTabLds)
-        hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), tab_lds_bytes(w.nwin), st, d_codes, (uint32_t) n,
-                           d_tab, tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_ticket, d_total_bits,
-                           d_err, aligned);
+    const bool lds = w.nwin && w.nwin <= kTabLds;
+    const size_t shmem = lds ? tab_lds_bytes(w.nwin) : 0;
+    // one round of resident workgroups (the look-back's waits need every
+    // predecessor's workgroup running)
+    int dev = 0, cus = 0, per = 0;
+    uint32_t resident = 256;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, lds ? (const void *) encode_onepass_kernel<true> : (const void *) encode_onepass_kernel<false>,
+            kEncThreads, shmem) == hipSuccess &&
+        cus > 0 && per > 0)
+        resident = (uint32_t) (cus * per);
+    const dim3 grid(std::min<uint32_t>(nchunks, resident));
+    if (lds)
+        hipLaunchKernelGGL(encode_onepass_kernel<true>, grid, dim3(kEncThreads), shmem, st, d_codes, (uint32_t) n,
+                           d_tab, tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_total_bits, d_err,
+                           aligned);
     else
         hipLaunchKernelGGL(encode_onepass_kernel<false>, grid, dim3(kEncThreads), 0, st, d_codes, (uint32_t) n, d_tab,
-                           tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_ticket, d_total_bits, d_err,
-                           aligned);
+                           tw, offset, d_words, d_status, d_chunk_off, d_head_bits, d_total_bits, d_err, aligned);
     hipLaunchKernelGGL(encode_head_fix_kernel, dim3(std::min<uint32_t>((nchunks + 255) / 256, 1024)), dim3(256), 0,
                        st, d_chunk_off, d_head_bits, nchunks, d_words);
 }

@@ -990,14 +990,12 @@ static int encode_enqueue(faz_gpu_ctx *ctx, const uint16_t *d_codes, size_t n, c
     HIPCHECK(ctx->tiles.ensure(scan_tiles(nchunks)));
     HIPCHECK(hipMemcpyAsync(ctx->enc_code.p + up_lo, ctx->h_tab.data() + up_lo, up_n * 8, hipMemcpyHostToDevice, st));
     if (ctx->enc_onepass && maxbits <= 32) {
-        // chunk status words and the ticket (scalars[2], low half) start at zero
+        // chunk status words start at zero
         HIPCHECK(ctx->enc_head.ensure(nchunks));
         HIPCHECK(ctx->err_flag.ensure(1));
         launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->chunk_bits.p), 2 * (size_t) nchunks, 0u, st);
-        launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->scalars.p + 2), 2, 0u, st);
         launch_encode_onepass(d_codes, n, ctx->enc_code.p, ew, t->offset, ctx->words.p, ctx->chunk_bits.p,
-                              ctx->chunk_off.p, ctx->enc_head.p, reinterpret_cast<uint32_t *>(ctx->scalars.p + 2),
-                              ctx->scalars.p + 1, ctx->err_flag.p, st);
+                              ctx->chunk_off.p, ctx->enc_head.p, ctx->scalars.p + 1, ctx->err_flag.p, st);
         HIPCHECK(hipGetLastError());
         return FAZ_OK;
     }
@@ -1768,6 +1766,10 @@ faz_status faz_gpu_batch_create(faz_gpu_ctx *ctx, const faz_interp_params *field
         if (rc) return fail(rc);
         k->tile3d = ctx->tile3d, k->fused3d = ctx->fused3d, k->tile_grid = ctx->tile_grid;
         k->fused_min_blocks = ctx->fused_min_blocks;
+        // the fields' encodes run on concurrent streams: the one-pass encoder's
+        // look-back needs all of its workgroups resident, so the batch keeps
+        // the two-pass encoder
+        k->enc_onepass = false;
         b->kid.push_back(k);
         if (k->minmax.ensure(kRangeScratch) != hipSuccess) return fail(FAZ_ERR_HIP);
     }

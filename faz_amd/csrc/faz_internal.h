@@ -232,13 +232,12 @@ void launch_encode_pack(const uint16_t *d_codes, size_t n, const uint64_t *d_tab
                         int32_t offset, const uint64_t *d_chunk_off, uint32_t *d_words, hipStream_t st);
 void launch_fill_u32(uint32_t *d, size_t count, uint32_t value, hipStream_t st);
 // One-pass encode (codewords of at most 32 bits): d_status (one u64 per
-// chunk) and *d_ticket must be zero; writes the bitstream words (no zero fill
-// needed), per-chunk bit offsets, the bit total, and sets bit 2 of *d_err if
-// a look-back wait ran out.
+// chunk) must be zero; writes the bitstream words (no zero fill needed),
+// per-chunk bit offsets, the bit total, and sets bit 2 of *d_err if a
+// look-back wait ran out.
 void launch_encode_onepass(const uint16_t *d_codes, size_t n, const uint64_t *d_tab, const EncodeWindow &w,
                            int32_t offset, uint32_t *d_words, uint64_t *d_status, uint64_t *d_chunk_off,
-                           uint32_t *d_head_bits, uint32_t *d_ticket, uint64_t *d_total_bits, uint32_t *d_err,
-                           hipStream_t st);
+                           uint32_t *d_head_bits, uint64_t *d_total_bits, uint32_t *d_err, hipStream_t st);
 void launch_unpred_scatter(const uint16_t *d_codes, size_t n, const float *d_unpred, const uint32_t *d_chunk_zeros,
                            const uint64_t *d_chunk_base, float *d_unpred_at, hipStream_t st,
                            uint64_t count = ~0ull);
