@@ -714,18 +714,17 @@ void launch_hist(const uint16_t *d_codes, size_t n, uint32_t *d_freq, uint32_t *
     int hot0 = radius - kLaneWin / 2;
     if (hot0 < lo) hot0 = lo;
     if (hot0 + kLaneWin > lo + kWin) hot0 = lo + kWin - kLaneWin;
-    // persistent workgroups: as many as are resident at once (one round)
-    static uint32_t resident = 0;
-    if (!resident) {
+    // persistent workgroups: as many as are resident at once (one round);
+    // measured once per process (thread-safe static initialisation)
+    static const uint32_t resident = [] {
         int dev = 0, cus = 0, per = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, hist_kernel, kHistThreads, 0) == hipSuccess &&
             cus > 0 && per > 0)
-            resident = (uint32_t) (cus * per);
-        else
-            resident = 768;
-    }
+            return (uint32_t) (cus * per);
+        return 768u;
+    }();
     const uint32_t grid = nchunks < resident ? nchunks : resident;
     if (nchunks)
         hipLaunchKernelGGL(hist_kernel, dim3(grid), dim3(kHistThreads), 0, st, d_codes, (uint32_t) n, d_freq,
