@@ -425,15 +425,15 @@ __global__ void encode_head_fix_kernel(const uint64_t *__restrict__ chunk_off, c
 }
 
 uint32_t encode_grid(uint32_t nchunks) {
-    static uint32_t ncu = 0;
-    if (!ncu) {
+    // the batch's worker threads enqueue encodes concurrently: thread-safe
+    // static initialisation
+    static const uint32_t ncu = [] {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
-            ncu = (uint32_t) v;
-        else
-            ncu = 256;
-    }
+            return (uint32_t) v;
+        return 256u;
+    }();
     return std::min<uint32_t>(nchunks, ncu * kEncGridPerCu);
 }
 
